@@ -1,0 +1,202 @@
+"""Host-side description of one HydroPol2D run (flags, running_control, rasters).
+
+`RunSpec` is the Python mirror of the workspace structs that
+HydroPol2D_Main_While.m reads (flags, running_control, Elevation_Properties,
+LULC_Properties, Soil_Properties, Rainfall_Parameters, WQ/GW states).  It
+converts itself into the plain-C `hp2d_run_config` / `hp2d_run_inputs` pair of
+include/hp2d.h.  No compute happens here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+
+
+def _f64(a):
+    return None if a is None else np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _u8(a):
+    return None if a is None else np.asfortranarray(np.asarray(a, dtype=np.uint8))
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return C.POINTER(ctype)()
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class GreenAmpt:
+    """Class-indexed Green-Ampt soil parameters (HydroPol2D_preprocessing.m:2249-2253)."""
+
+    ksat_mmh: np.ndarray
+    psi_mm: np.ndarray
+    dtheta: np.ndarray
+    theta_sat_minus_r: np.ndarray
+    soil_class: Optional[np.ndarray] = None   # ny x nx uint8, None = class 0
+    lu_m: Optional[np.ndarray] = None         # per class, NaN = derived from ksat
+    I0_mm: Optional[np.ndarray] = None        # ny x nx initial infiltrated depth
+    I_min_mm: float = 1e-3                    # min_soil_moisture (preprocessing.m:2386)
+
+
+@dataclass
+class WaterQuality:
+    """Class-indexed build-up / wash-off parameters (preprocessing.m:2159-2174)."""
+
+    C1: np.ndarray
+    C2: np.ndarray
+    C3: np.ndarray
+    C4: np.ndarray
+    lulc_class: Optional[np.ndarray] = None
+    ADD_days: float = 7.0
+    flag_wq_model: int = 0
+    Bmin: float = 0.0
+    Bmax: float = 1e30
+    min_Bt: float = 0.0
+
+
+@dataclass
+class Groundwater:
+    """Boussinesq aquifer (Boussinesq_2D_explicit.m) run every `every_steps` steps."""
+
+    h0: np.ndarray
+    z0: np.ndarray
+    K: float = 1e-5
+    Sy: float = 0.2
+    every_steps: int = 10
+    courant: float = 0.5
+
+
+@dataclass
+class RunSpec:
+    elevation: np.ndarray
+    roughness: np.ndarray
+    resolution: float
+    outlet_mask: Optional[np.ndarray] = None
+    d0: Optional[np.ndarray] = None
+    rain_time_min: np.ndarray = field(default_factory=lambda: np.array([1e9]))
+    rain_intensity_mmh: np.ndarray = field(default_factory=lambda: np.array([0.0]))
+    rain_pattern: Optional[np.ndarray] = None
+    precision: str = "fp64"
+    routing: str = "inertial"
+    flag_numerical_scheme: int = 1
+    flag_critical: int = 0
+    outlet_type: int = 2
+    slope_outlet: float = 0.0
+    routing_time_min: float = 60.0
+    min_time_step_s: float = 0.01
+    max_time_step_s: float = 10.0
+    time_step_increments_s: float = 0.001
+    time_step_change_s: float = 1e-4
+    alfa_min: float = 0.7
+    alfa_max: float = 0.7
+    flag_timestep: int = 1
+    flag_adaptive_timestepping: int = 1
+    depth_tolerance_mm: float = 1.0
+    record_time_hydrographs_min: float = 1.0
+    report_every_percent: float = 5.0
+    factor_reduction: float = 2.0
+    infiltration: Optional[GreenAmpt] = None
+    water_quality: Optional[WaterQuality] = None
+    groundwater: Optional[Groundwater] = None
+
+    @property
+    def shape(self):
+        return np.asarray(self.elevation).shape
+
+    def n_hydro_records(self) -> int:
+        return int(np.floor(self.routing_time_min / self.record_time_hydrographs_min + 1e-12)) + 1
+
+    def to_c(self):
+        """Return (RunConfig, RunInputs, keepalive) for the C ABI."""
+        ny, nx = self.shape
+        keep = []
+
+        def f(a):
+            arr = _f64(a)
+            if arr is not None:
+                keep.append(arr)
+            return arr
+
+        def u(a):
+            arr = _u8(a)
+            if arr is not None:
+                keep.append(arr)
+            return arr
+
+        cfg = abi.RunConfig()
+        cfg.ny, cfg.nx = ny, nx
+        cfg.precision = abi.HP2D_FP64 if self.precision == "fp64" else abi.HP2D_FP32
+        cfg.routing = abi.HP2D_ROUTING_INERTIAL if self.routing == "inertial" else abi.HP2D_ROUTING_CA
+        cfg.flag_numerical_scheme = self.flag_numerical_scheme
+        cfg.flag_critical = self.flag_critical
+        cfg.outlet_type = self.outlet_type
+        cfg.slope_outlet = self.slope_outlet
+        cfg.resolution = self.resolution
+        cfg.routing_time_min = self.routing_time_min
+        cfg.min_time_step_s = self.min_time_step_s
+        cfg.max_time_step_s = self.max_time_step_s
+        cfg.time_step_increments_s = self.time_step_increments_s
+        cfg.time_step_change_s = self.time_step_change_s
+        cfg.alfa_min = self.alfa_min
+        cfg.alfa_max = self.alfa_max
+        cfg.flag_timestep = self.flag_timestep
+        cfg.flag_adaptive_timestepping = self.flag_adaptive_timestepping
+        cfg.depth_tolerance_mm = self.depth_tolerance_mm
+        cfg.record_time_hydrographs_min = self.record_time_hydrographs_min
+        cfg.report_every_percent = self.report_every_percent
+        cfg.factor_reduction = self.factor_reduction
+        cfg.flag_infiltration = 1 if self.infiltration is not None else 0
+        cfg.flag_waterquality = 1 if self.water_quality is not None else 0
+        cfg.flag_wq_model = self.water_quality.flag_wq_model if self.water_quality else 0
+        cfg.flag_groundwater = 1 if self.groundwater is not None else 0
+        cfg.gw_every_steps = self.groundwater.every_steps if self.groundwater else 0
+        cfg.gw_courant = self.groundwater.courant if self.groundwater else 0.5
+
+        ins = abi.RunInputs()
+        ins.elevation = _ptr(f(self.elevation), C.c_double)
+        ins.roughness = _ptr(f(self.roughness), C.c_double)
+        ins.d0 = _ptr(f(self.d0), C.c_double)
+        ins.outlet_mask = _ptr(u(self.outlet_mask), C.c_uint8)
+        rt = f(np.atleast_1d(self.rain_time_min))
+        ri = f(np.atleast_1d(self.rain_intensity_mmh))
+        ins.rain_time_min = _ptr(rt, C.c_double)
+        ins.rain_intensity_mmh = _ptr(ri, C.c_double)
+        ins.n_rain = int(min(rt.size, ri.size))
+        ins.rain_pattern = _ptr(f(self.rain_pattern), C.c_double)
+        ga = self.infiltration
+        if ga is not None:
+            cfg.n_soil_classes = int(np.atleast_1d(ga.ksat_mmh).size)
+            ins.soil_class = _ptr(u(ga.soil_class), C.c_uint8)
+            ins.ksat_mmh = _ptr(f(np.atleast_1d(ga.ksat_mmh)), C.c_double)
+            ins.psi_mm = _ptr(f(np.atleast_1d(ga.psi_mm)), C.c_double)
+            ins.dtheta = _ptr(f(np.atleast_1d(ga.dtheta)), C.c_double)
+            ins.theta_sat_minus_r = _ptr(f(np.atleast_1d(ga.theta_sat_minus_r)), C.c_double)
+            ins.lu_m = _ptr(f(None if ga.lu_m is None else np.atleast_1d(ga.lu_m)), C.c_double)
+            ins.I0_mm = _ptr(f(ga.I0_mm), C.c_double)
+            ins.I_min_mm = ga.I_min_mm
+        wq = self.water_quality
+        if wq is not None:
+            cfg.n_lulc_classes = int(np.atleast_1d(wq.C1).size)
+            ins.lulc_class = _ptr(u(wq.lulc_class), C.c_uint8)
+            ins.wq_C1 = _ptr(f(np.atleast_1d(wq.C1)), C.c_double)
+            ins.wq_C2 = _ptr(f(np.atleast_1d(wq.C2)), C.c_double)
+            ins.wq_C3 = _ptr(f(np.atleast_1d(wq.C3)), C.c_double)
+            ins.wq_C4 = _ptr(f(np.atleast_1d(wq.C4)), C.c_double)
+            ins.wq_ADD_days = wq.ADD_days
+            ins.wq_Bmin = wq.Bmin
+            ins.wq_Bmax = wq.Bmax
+            ins.wq_min_Bt = wq.min_Bt
+        gw = self.groundwater
+        if gw is not None:
+            ins.gw_h0 = _ptr(f(gw.h0), C.c_double)
+            ins.gw_z0 = _ptr(f(gw.z0), C.c_double)
+            ins.gw_K = gw.K
+            ins.gw_Sy = gw.Sy
+        return cfg, ins, keep

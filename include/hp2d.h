@@ -1,0 +1,321 @@
+/*
+ * hp2d.h - C ABI of the MI355X (gfx950) explicit 2D surface-water step.
+ *
+ * The library replaces the per-step physics of HydroPol2D
+ * (HydroPol2D_Functions/HydroPol2D_Main_While.m:403-994) behind the
+ * reference's own function boundaries:
+ *
+ *   Inertial_Solver            HydroPol2D_Functions/Inertial_Solver.m:1-26
+ *   Local_Inertial_Model_D4    HydroPol2D_Functions/Local_Inertial_Model_D4.m:1-516
+ *   CA_Routing                 HydroPol2D_Functions/CA_Routing.m:1-171
+ *   build_up_wash_off          HydroPol2D_Functions/build_up_wash_off.m:1-183
+ *   Boussinesq_2D_explicit     HydroPol2D_Functions/Boussinesq_2D_explicit.m:1-195
+ *   refreshing_timestep        HydroPol2D_Functions/refreshing_timestep.m:17-368
+ *   the Main_While step loop   HydroPol2D_Functions/HydroPol2D_Main_While.m:403-994
+ *
+ * Conventions (identical to the MATLAB state layout):
+ *   - every grid is column-major ny x nx: element (i,j) (1-based in MATLAB)
+ *     lives at offset (i-1) + (j-1)*ny; 3-page arrays (outflow, Hf) stack the
+ *     pages: page p at offset p*ny*nx;
+ *   - depths in mm, face fluxes (outflow pages) in mm/h of cell area, Hf in m,
+ *     time_step in minutes, every *_s field in seconds, model time in minutes;
+ *   - NaN elevation marks cells outside the catchment (idx_nan);
+ *   - outlet rows/cols are 1-based, as MATLAB's find() returns them.
+ *
+ * Every function returns hp2d_status (0 = OK). On failure, hp2d_last_error()
+ * of the handle (or hp2d_global_last_error() when no handle exists) gives the
+ * text. A non-OK status from a step means the step was rejected; the caller's
+ * state arrays are left untouched, so the MATLAB catch/rollback of
+ * HydroPol2D_Main_While.m:926-993 can run unchanged.
+ *
+ * Threading: one handle is bound to one HIP stream and is not re-entrant.
+ * Different handles may be used from different threads.
+ */
+#ifndef HP2D_H
+#define HP2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t hp2d_status;
+enum {
+    HP2D_OK = 0,
+    HP2D_E_ARG = 1,          /* bad argument / shape */
+    HP2D_E_HIP = 2,          /* HIP runtime error */
+    HP2D_E_NOMEM = 3,        /* device allocation failed */
+    HP2D_E_STATE = 4,        /* call out of order (e.g. run before terrain) */
+    HP2D_E_UNSUPPORTED = 5,  /* flag combination not implemented */
+    HP2D_E_INSTABILITY = 6,  /* step rejected (NaN state, WQ instability, injected failure) */
+    HP2D_E_NO_DEVICE = 7     /* no HIP device visible */
+};
+
+enum { HP2D_FP64 = 0, HP2D_FP32 = 1 };
+
+enum { HP2D_ROUTING_INERTIAL = 1, HP2D_ROUTING_CA = 2 };
+
+/* ------------------------------------------------------------------------ */
+/* Library / handle                                                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct hp2d_handle hp2d_handle;
+
+/* Version string of the library build (never NULL). */
+const char *hp2d_version(void);
+
+/* Number of visible HIP devices (0 when none). */
+int32_t hp2d_device_count(void);
+
+/* Text of the last error raised on a thread without a handle. */
+const char *hp2d_global_last_error(void);
+
+/* Create a handle for grids of ny x nx cells computing in `precision`
+ * (HP2D_FP64 = MATLAB double, HP2D_FP32 = MATLAB single / flag_single).
+ * The handle owns device buffers sized for the grid and one HIP stream.
+ * Replaces the per-call allocations of Local_Inertial_Model_D4.m:73-84,145-149. */
+hp2d_status hp2d_create(int32_t ny, int32_t nx, int32_t precision, hp2d_handle **out);
+void hp2d_destroy(hp2d_handle *h);
+const char *hp2d_last_error(const hp2d_handle *h);
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in per-call boundaries (host arrays in, host arrays out)             */
+/* ------------------------------------------------------------------------ */
+
+/* q = Inertial_Solver(flag_numerical_scheme, q_p, dt, Hf, S, n_sq, dx, idx_nan)
+ * Inertial_Solver.m:1-26 (scheme 1, Bates 2010).  q_p, Hf, S, q_out hold
+ * n_pages pages of ny*nx; n_sq is one page, broadcast over pages as MATLAB's
+ * implicit expansion does.  dt in seconds.  Non-finite results are returned
+ * as computed (the caller zeroes them, Local_Inertial_Model_D4.m:264). */
+hp2d_status hp2d_inertial_solver(hp2d_handle *h, int32_t flag_numerical_scheme,
+                                 const double *q_p, double dt, const double *Hf,
+                                 const double *S, const double *n_sq, int32_t n_pages,
+                                 double *q_out);
+
+/* Local_Inertial_Model_D4.m:1-5 with flag_subgrid = 0, flag_reservoir = 0
+ * (the arguments that are unused on that branch are omitted).
+ * Inputs:  z, d_tot [mm], roughness_cell, roughness_squared: ny*nx;
+ *          outflow_prev: ny*nx*3 [mm/h] (the `outflow` argument);
+ *          row_outlet/col_outlet: n_outlet 1-based indices.
+ * Outputs (caller-allocated; any may be NULL to skip it):
+ *          qout_left, qout_right, qout_up, qout_down, outlet_flow [mm/h],
+ *          d_t [mm], I_tot_end_cell [m3]: ny*nx;
+ *          outflow [mm/h], Hf [m]: ny*nx*3.
+ * d_p, d_tolerance, idx_nan and the subgrid/reservoir arguments do not enter
+ * the computation on this branch (Local_Inertial_Model_D4.m:9,13).
+ * Returns the scalar outputs Qc = Qf = Qci = Qfi = 0 and C_a = resolution^2
+ * implicitly (Local_Inertial_Model_D4.m:257-266). */
+hp2d_status hp2d_local_inertial_d4(hp2d_handle *h, int32_t flag_numerical_scheme,
+                                   const double *z, const double *d_tot,
+                                   const double *roughness_cell,
+                                   const double *roughness_squared, double cell_area,
+                                   double time_step_min, double resolution,
+                                   int32_t outlet_type, double slope_outlet,
+                                   const int32_t *row_outlet, const int32_t *col_outlet,
+                                   int32_t n_outlet, const double *outflow_prev,
+                                   int32_t flag_critical,
+                                   double *qout_left, double *qout_right, double *qout_up,
+                                   double *qout_down, double *outlet_flow, double *d_t,
+                                   double *I_tot_end_cell, double *outflow, double *Hf);
+
+/* Same boundary on single-precision arrays (MATLAB flag_single = 1,
+ * HydroPol2D_preprocessing.m:3287-3569).  Handle must be HP2D_FP32. */
+hp2d_status hp2d_local_inertial_d4_f32(hp2d_handle *h, int32_t flag_numerical_scheme,
+                                       const float *z, const float *d_tot,
+                                       const float *roughness_cell,
+                                       const float *roughness_squared, float cell_area,
+                                       float time_step_min, float resolution,
+                                       int32_t outlet_type, float slope_outlet,
+                                       const int32_t *row_outlet, const int32_t *col_outlet,
+                                       int32_t n_outlet, const float *outflow_prev,
+                                       int32_t flag_critical,
+                                       float *qout_left, float *qout_right, float *qout_up,
+                                       float *qout_down, float *outlet_flow, float *d_t,
+                                       float *I_tot_end_cell, float *outflow, float *Hf);
+
+/* [qout_left,qout_right,qout_up,qout_down,outlet_flow,d_t,I_tot_end_cell] =
+ *   CA_Routing(..., z, d_tot, roughness, cell_area, time_step, h_0, Resolution,
+ *              I_tot_end_cell, outlet_index, outlet_type, slope_outlet,
+ *              row_outlet, col_outlet, idx_nan, flag_critical)
+ * CA_Routing.m:1-171 (flag_reservoir = 0).  I_tot_end_cell_prev in, the new
+ * I_tot_end_cell out (may alias).  h_0 [mm] may be NULL (zeros). */
+hp2d_status hp2d_ca_routing(hp2d_handle *h, const double *z, const double *d_tot,
+                            const double *roughness, double cell_area, double time_step_min,
+                            const double *h_0, double resolution,
+                            const double *I_tot_end_cell_prev, int32_t outlet_type,
+                            double slope_outlet, const int32_t *row_outlet,
+                            const int32_t *col_outlet, int32_t n_outlet,
+                            int32_t flag_critical,
+                            double *qout_left, double *qout_right, double *qout_up,
+                            double *qout_down, double *outlet_flow, double *d_t,
+                            double *I_tot_end_cell);
+
+/* [B_t,P_conc,Out_Conc,tmin_wq,tot_W_out,mass_lost,Tot_Washed] =
+ *   build_up_wash_off(C_3,C_4,qout_left,qout_right,qout_up,qout_down,
+ *                     outlet_flow,B_t,time_step,nx,ny,cell_area,outlet_index,
+ *                     idx_nan_5,flag_wq_model,mass_lost,Tot_Washed,Bmin,Bmax,min_Bt)
+ * build_up_wash_off.m:1-183.  C_3, C_4, B_t, Tot_Washed, outlet_mask: ny*nx
+ * (outlet_mask 0/1); idx_nan given through NaN in B_t. Bmin/Bmax are used only
+ * for flag_wq_model = 1.  B_t_out, P_conc, Tot_Washed_out: ny*nx.
+ * scalars out: Out_Conc [mg/L], tmin_wq [s], tot_W_out [kg], mass_lost [kg]
+ * (mass_lost in = *mass_lost). */
+hp2d_status hp2d_build_up_wash_off(hp2d_handle *h, const double *C_3, const double *C_4,
+                                   const double *qout_left, const double *qout_right,
+                                   const double *qout_up, const double *qout_down,
+                                   const double *outlet_flow, const double *B_t,
+                                   double time_step_min, double cell_area,
+                                   const uint8_t *outlet_mask, int32_t flag_wq_model,
+                                   const double *Tot_Washed, double Bmin, double Bmax,
+                                   double min_Bt,
+                                   double *B_t_out, double *P_conc, double *Out_Conc,
+                                   double *tmin_wq, double *tot_W_out, double *mass_lost,
+                                   double *Tot_Washed_out);
+
+/* [h_new, q_exf, dt_used, n_sub] = Boussinesq_2D_explicit(...)
+ * Boussinesq_2D_explicit.m:1-195 on the coarse grid (no river coupling):
+ * h (head) [m], z0 (aquifer base) [m], K [m/s], Sy [-], R (recharge) [m/s],
+ * h_surf (surface elevation) [m]: ny*nx; dirichlet_mask 0/1 or NULL; NaN in
+ * z0 marks inactive cells.  T_total [s] is integrated with internal
+ * Courant-limited sub-steps.  Outputs h_new [m], q_exf [m/s] ny*nx. */
+hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, const double *h_in, const double *z0,
+                               const double *K, const double *Sy, const double *R,
+                               const double *h_surf, const uint8_t *dirichlet_mask,
+                               double dx, double dy, double T_total, double courant,
+                               double *h_out, double *q_exf, double *dt_used_last,
+                               int32_t *n_substeps);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident step loop (the performance path)                         */
+/* ------------------------------------------------------------------------ */
+
+/* Flags and running_control fields of one run (Config/input_data_bypass_script.m
+ * :160-170,314-381; input_data_script.m:84-160). */
+typedef struct hp2d_run_config {
+    int32_t ny, nx;
+    int32_t precision;                 /* HP2D_FP64 / HP2D_FP32 (state + fluxes) */
+    int32_t routing;                   /* HP2D_ROUTING_INERTIAL / HP2D_ROUTING_CA */
+    int32_t flag_numerical_scheme;     /* 1 = Bates */
+    int32_t flag_critical;
+    int32_t outlet_type;               /* 1 normal slope, 2 critical depth */
+    double slope_outlet;
+    double resolution;                 /* dx [m] */
+    double routing_time_min;           /* running_control.routing_time */
+    double min_time_step_s, max_time_step_s;
+    double time_step_increments_s, time_step_change_s;
+    double alfa_min, alfa_max;
+    int32_t flag_timestep;             /* 1: min(dt_adv, dt_wave) */
+    int32_t flag_adaptive_timestepping;/* 1: dt_wave from celerity only */
+    double depth_tolerance_mm;         /* CA_States.depth_tolerance */
+    double record_time_hydrographs_min;
+    double report_every_percent;       /* forced to 5 by Main_While.m:106 */
+    double factor_reduction;           /* rollback dt shrink, default 2 */
+    int32_t flag_infiltration;         /* 1 = Green-Ampt */
+    int32_t flag_waterquality;         /* 1 = build-up / wash-off */
+    int32_t flag_wq_model;             /* 0 mass model, 1 rating model */
+    int32_t flag_groundwater;          /* 1 = Boussinesq every gw_every_steps */
+    int32_t gw_every_steps;
+    double gw_courant;
+    int32_t n_soil_classes;            /* GA class tables below */
+    int32_t n_lulc_classes;            /* WQ class tables below */
+} hp2d_run_config;
+
+/* Host inputs of a run; all grids ny*nx column-major.  Pointers may be NULL
+ * where marked optional. */
+typedef struct hp2d_run_inputs {
+    const double *elevation;           /* [m], NaN outside the domain */
+    const double *roughness;           /* Manning n */
+    const double *d0;                  /* initial depth [mm] (optional: zeros) */
+    const uint8_t *outlet_mask;        /* 1 = outlet cell (required unless no outlet) */
+    /* lumped, end-stamped piecewise-constant rainfall (update_spatial_BC.m:1095-1185) */
+    const double *rain_time_min;       /* end times [min] */
+    const double *rain_intensity_mmh;  /* rates [mm/h] */
+    int32_t n_rain;
+    const double *rain_pattern;        /* optional per-cell multiplier of the lumped rate */
+    /* Green-Ampt, class-indexed (HydroPol2D_preprocessing.m:2249-2253) */
+    const uint8_t *soil_class;         /* optional (all class 0) */
+    const double *ksat_mmh;            /* [n_soil_classes] */
+    const double *psi_mm;
+    const double *dtheta;              /* theta_sat - theta_i */
+    const double *theta_sat_minus_r;   /* for k_out */
+    const double *lu_m;                /* optional top-layer depth override (NaN = derived) */
+    const double *I0_mm;               /* optional initial infiltrated depth per cell */
+    double I_min_mm;                   /* min_soil_moisture (HydroPol2D_preprocessing.m:2386-2389) */
+    /* Water quality, class-indexed by LULC (HydroPol2D_preprocessing.m:2159-2174) */
+    const uint8_t *lulc_class;
+    const double *wq_C1, *wq_C2, *wq_C3, *wq_C4; /* [n_lulc_classes] */
+    double wq_ADD_days;
+    double wq_Bmin, wq_Bmax, wq_min_Bt;
+    /* Groundwater (Boussinesq) */
+    const double *gw_h0;               /* initial head [m] */
+    const double *gw_z0;               /* aquifer base [m] */
+    double gw_K, gw_Sy;
+} hp2d_run_inputs;
+
+/* Results of hp2d_sim_run, accumulated on the device in FP64. */
+typedef struct hp2d_run_report {
+    int64_t steps;                     /* accepted steps (k - 1) */
+    int64_t rejected_steps;            /* rollbacks */
+    double t_min;                      /* model time at exit [min] */
+    double time_step_s;                /* dt of the next step [s] */
+    double max_velocity;               /* velocities.max_velocity */
+    double volume_error_m3;            /* last mass_balance_check volume_error */
+    int32_t n_reports;                 /* mass-balance reports so far */
+    double outflow_volume_m3;          /* BC_States.outflow_volume */
+    double inflow_volume_m3;           /* BC_States.inflow_volume */
+    double outlet_runoff_volume_mm;    /* save_output_maps.m:611-617 */
+    double cum_rain_m3, cum_infiltration_m3, cum_outflow_m3; /* every-step FP64 ledgers */
+    double storage_m3;                 /* surface storage at exit */
+    int32_t catch_index;
+} hp2d_run_report;
+
+typedef struct hp2d_sim hp2d_sim;
+
+/* Upload a run; allocates every device buffer once. */
+hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
+                            hp2d_sim **out);
+void hp2d_sim_destroy(hp2d_sim *s);
+const char *hp2d_sim_last_error(const hp2d_sim *s);
+
+/* Advance up to max_steps steps of the Main_While loop (or until t passes
+ * routing_time + min_dt, HydroPol2D_Main_While.m:403).  No host round-trip
+ * per step: dt, rainfall, hydrograph and mass ledgers live on the device.
+ * *done = 1 once the end of the simulation is reached. */
+hp2d_status hp2d_sim_run(hp2d_sim *s, int64_t max_steps, int32_t *done);
+
+/* Advance exactly n_steps LI steps with a fixed step pattern and no time
+ * control bookkeeping beyond the device dt (timing / throughput harness).
+ * Uses a captured hipGraph of `graph_steps` steps when graph_steps > 0. */
+hp2d_status hp2d_sim_run_fixed(hp2d_sim *s, int64_t n_steps, int32_t graph_steps);
+
+hp2d_status hp2d_sim_report(hp2d_sim *s, hp2d_run_report *rep);
+
+/* Copy state back to the host (arrays of ny*nx unless noted; NULL skips).
+ * outflow: ny*nx*3 [mm/h]; Hf: ny*nx*3 [m] (of the last step). */
+hp2d_status hp2d_sim_get_state(hp2d_sim *s, double *d_t, double *outflow, double *Hf,
+                               double *I_t, double *dmax, double *B_t, double *gw_h);
+
+/* Hydrograph records (save_output_maps.m:266-269,409,481-490):
+ * q_m3s[n], time_min[n], depth_outlet_mm[n]; n_max capacity, *n_out filled. */
+hp2d_status hp2d_sim_get_hydrograph(hp2d_sim *s, double *q_m3s, double *time_min,
+                                    double *depth_outlet_mm, int32_t n_max, int32_t *n_out);
+
+/* Per-step dt history [s] (last n_max accepted steps, oldest first). */
+hp2d_status hp2d_sim_get_dt_history(hp2d_sim *s, double *dt_s, int32_t n_max,
+                                    int32_t *n_out);
+
+/* Force the step with counter k == step_k to be rejected (kind 1), to test the
+ * host rollback path of HydroPol2D_Main_While.m:926-993 without a real fault. */
+hp2d_status hp2d_debug_inject_failure(hp2d_sim *s, int64_t step_k, int32_t kind);
+
+/* Average duration [ms] of the fused local-inertial kernel over the last
+ * hp2d_sim_run_fixed call, measured with HIP events on the sim's stream. */
+hp2d_status hp2d_sim_kernel_time(hp2d_sim *s, double *li_ms, double *total_ms,
+                                 int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HP2D_H */

@@ -1,0 +1,735 @@
+/*
+ * hp2d_oracle.c - FP64 CPU restatement of the HydroPol2D explicit step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity oracle of the HIP library.  Loaded by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg; never
+ * linked into, called by, or shipped with hydropol2d_amd/libhp2d.so.
+ *
+ * Each function names the reference file:line it restates.  MATLAB max/min
+ * omit NaN (fmax/fmin here), nansum skips NaN, and column-major arrays are
+ * indexed (i,j) -> i + j*ny with 0-based i, j.
+ */
+#include "hp2d_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define G 9.81
+#define IX(i, j) ((int64_t)(i) + (int64_t)(j) * ny)
+
+static double *zalloc(int64_t n) { return (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+
+/* MATLAB nansum over a full array, column-major order (nansum(nansum(X))). */
+static double nansum2(int32_t ny, int32_t nx, const double *x) {
+    double total = 0.0;
+    for (int32_t j = 0; j < nx; ++j) {
+        double col = 0.0;
+        for (int32_t i = 0; i < ny; ++i) {
+            double v = x[IX(i, j)];
+            if (!isnan(v)) col += v;
+        }
+        total += col;
+    }
+    return total;
+}
+
+/* ------------------------------------------------------------------------ */
+/* A1  Inertial_Solver.m:19-26 (flag_numerical_scheme == 1, Bates 2010)      */
+/* ------------------------------------------------------------------------ */
+void oracle_inertial_solver(int32_t scheme, int64_t n_cells, int32_t n_pages,
+                            const double *q_p, double dt, const double *Hf,
+                            const double *S, const double *n_sq, double *q) {
+    (void)scheme;
+    const double e73 = 7.0 / 3.0;
+    for (int32_t p = 0; p < n_pages; ++p) {
+        for (int64_t c = 0; c < n_cells; ++c) {
+            int64_t o = c + (int64_t)p * n_cells;
+            double num = q_p[o] - G * Hf[o] * dt * S[o];
+            double den = 1.0 + G * dt * n_sq[c] * fabs(q_p[o]) / pow(Hf[o], e73);
+            q[o] = num / den;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* A4  apply_coarse_draining_limiter, Local_Inertial_Model_D4.m:799-858      */
+/* Qx, Qy: discharge [m3/s] pages, modified in place.                        */
+/* ------------------------------------------------------------------------ */
+static void coarse_draining_limiter(int32_t ny, int32_t nx, double *Qx, double *Qy,
+                                    double *Q3, const double *d_tot, double dt,
+                                    double cell_area) {
+    int64_t n = (int64_t)ny * nx;
+    double *avail = zalloc(n), *out = zalloc(n), *scale = zalloc(n);
+    double dtm = fmax(dt, DBL_EPSILON);
+    for (int64_t c = 0; c < n; ++c) avail[c] = fmax(d_tot[c] / 1000.0, 0.0) * cell_area / dtm;
+    for (int64_t c = 0; c < n; ++c) out[c] = 0.0 + fmax(Qx[c], 0.0);
+    for (int32_t j = 1; j < nx; ++j)
+        for (int32_t i = 0; i < ny; ++i) out[IX(i, j)] += fmax(-Qx[IX(i, j - 1)], 0.0);
+    for (int64_t c = 0; c < n; ++c) out[c] += fmax(Qy[c], 0.0);
+    for (int32_t j = 0; j < nx; ++j)
+        for (int32_t i = 0; i < ny - 1; ++i) out[IX(i, j)] += fmax(-Qy[IX(i + 1, j)], 0.0);
+    for (int64_t c = 0; c < n; ++c) {
+        double s = 1.0;
+        if (out[c] > avail[c] && out[c] > 0.0) s = avail[c] / out[c];
+        if (!isfinite(s)) s = 0.0;
+        s = fmax(fmin(s, 1.0), 0.0);
+        scale[c] = s;
+    }
+    if (nx > 1) {
+        for (int32_t j = 0; j < nx - 1; ++j)
+            for (int32_t i = 0; i < ny; ++i) {
+                double q = Qx[IX(i, j)];
+                if (q > 0) q = q * scale[IX(i, j)];
+                else if (q < 0) q = q * scale[IX(i, j + 1)];
+                Qx[IX(i, j)] = q;
+            }
+        for (int32_t i = 0; i < ny; ++i) Qx[IX(i, nx - 1)] = 0.0;
+    }
+    if (ny > 1) {
+        for (int32_t j = 0; j < nx; ++j)
+            for (int32_t i = 1; i < ny; ++i) {
+                double q = Qy[IX(i, j)];
+                if (q > 0) q = q * scale[IX(i, j)];
+                else if (q < 0) q = q * scale[IX(i - 1, j)];
+                Qy[IX(i, j)] = q;
+            }
+        for (int32_t j = 0; j < nx; ++j) Qy[IX(0, j)] = 0.0;
+    }
+    for (int64_t c = 0; c < n; ++c) {
+        if (!isfinite(Qx[c])) Qx[c] = 0.0;
+        if (!isfinite(Qy[c])) Qy[c] = 0.0;
+        if (!isfinite(Q3[c])) Q3[c] = 0.0;
+    }
+    free(avail);
+    free(out);
+    free(scale);
+}
+
+/* ------------------------------------------------------------------------ */
+/* A2-A7  Local_Inertial_Model_D4.m:7-516, branch flag_subgrid = 0,          */
+/*        flag_reservoir = 0, h_min = 0, flag_HC = 0.                        */
+/* outlet_idx: 0-based linear indices (sub2ind - 1).                         */
+/* ------------------------------------------------------------------------ */
+int32_t oracle_li_d4(int32_t ny, int32_t nx, int32_t scheme, const double *z,
+                     const double *d_tot, const double *roughness_cell,
+                     const double *roughness_squared, double cell_area,
+                     double time_step_min, double resolution, int32_t outlet_type,
+                     double slope_outlet, const int64_t *outlet_idx, int32_t n_outlet,
+                     const double *outflow_prev, int32_t flag_critical,
+                     double *qout_left, double *qout_right, double *qout_up,
+                     double *qout_down, double *outlet_flow, double *d_t,
+                     double *I_tot_end_cell, double *outflow, double *Hf) {
+    const int64_t n = (int64_t)ny * nx;
+    const double R = resolution;
+    double *y = zalloc(n), *depth = zalloc(n);
+    double *S = zalloc(3 * n);     /* matrix_store */
+    double *qprev = zalloc(3 * n); /* outflow_prev in m2/s */
+    double *q = zalloc(3 * n);
+
+    /* :31 depth_cell = max(d_tot/1000,0); :43 y = z_dem + depth_cell */
+    for (int64_t c = 0; c < n; ++c) {
+        depth[c] = fmax(d_tot[c] / 1000.0, 0.0);
+        y[c] = z[c] + depth[c];
+    }
+    /* :145-154 matrix_store = 0*outflow; Hf = 0*outflow (page 3 keeps 0*outflow) */
+    for (int64_t c = 0; c < 3 * n; ++c) {
+        S[c] = 0.0 * outflow_prev[c];
+        Hf[c] = 0.0 * outflow_prev[c];
+    }
+    double *Sx = S, *Sy = S + n, *Hfx = Hf, *Hfy = Hf + n;
+    /* :91-92, :123-128, :133-142 */
+    for (int32_t j = 0; j < nx; ++j)
+        for (int32_t i = 0; i < ny; ++i) {
+            int64_t c = IX(i, j);
+            if (j < nx - 1) {
+                int64_t e = IX(i, j + 1);
+                Sx[c] = (y[e] - y[c]) / R;
+                double zf = fmax(z[c], z[e]);
+                Hfx[c] = fmax(fmax(y[c], y[e]) - zf, 0.0);
+            } else {
+                Sx[c] = 0.0;
+                Hfx[c] = 0.0;
+            }
+            if (i > 0) {
+                int64_t u = IX(i - 1, j);
+                Sy[c] = (y[u] - y[c]) / R;
+                double zf = fmax(z[u], z[c]);
+                Hfy[c] = fmax(fmax(y[u], y[c]) - zf, 0.0);
+            } else {
+                Sy[c] = 0.0;
+                Hfy[c] = 0.0;
+            }
+        }
+    /* :184 Hf(Hf<0)=0 */
+    for (int64_t c = 0; c < 3 * n; ++c)
+        if (Hf[c] < 0) Hf[c] = 0.0;
+    /* :198 outflow = outflow/1000/3600*Resolution^2/Resolution  [m2/s] */
+    const double R2 = R * R;
+    for (int64_t c = 0; c < 3 * n; ++c) qprev[c] = outflow_prev[c] / 1000.0 / 3600.0 * R2 / R;
+    const double dt = time_step_min * 60.0; /* :201 */
+    /* :263-264 */
+    oracle_inertial_solver(scheme, n, 3, qprev, dt, Hf, S, roughness_squared, q);
+    for (int64_t c = 0; c < 3 * n; ++c)
+        if (!isfinite(q[c])) q[c] = 0.0;
+    /* :272-275 critical velocity cap */
+    if (flag_critical == 1) {
+        for (int64_t c = 0; c < 3 * n; ++c) {
+            double cv = Hf[c] * sqrt(9.81 * Hf[c]);
+            q[c] = fmin(fmax(q[c], -cv), cv);
+        }
+    }
+    /* :292-294 velocity cap: q = min(max(q,-10 Hf),10 Hf) */
+    for (int64_t c = 0; c < 3 * n; ++c) {
+        double thr = Hf[c] * 10.0;
+        q[c] = fmin(fmax(q[c], -thr), thr);
+    }
+    /* :304-306 outflow_rate = q.*cell_width; limiter */
+    double *Q = q; /* reuse: Q = q*R in place */
+    for (int64_t c = 0; c < 3 * n; ++c) Q[c] = q[c] * R;
+    coarse_draining_limiter(ny, nx, Q, Q + n, Q + 2 * n, d_tot, dt, cell_area);
+    /* :309 outflow = outflow_rate./cell_area*1000*3600 */
+    for (int64_t c = 0; c < 3 * n; ++c) outflow[c] = Q[c] / cell_area * 1000.0 * 3600.0;
+    const double *Qx = Q, *Qy = Q + n;
+    /* :313-315 Vol_Flux; :390 d_t = d_tot + 1000*Vol_Flux/C_a; :415 */
+    const double C_a = R * R; /* :266 */
+    for (int32_t j = 0; j < nx; ++j)
+        for (int32_t i = 0; i < ny; ++i) {
+            int64_t c = IX(i, j);
+            double west = (j > 0) ? Qx[IX(i, j - 1)] : 0.0;
+            double south = (i < ny - 1) ? Qy[IX(i + 1, j)] : 0.0;
+            double vol = dt * (west - Qx[c] - Qy[c] + south);
+            double dn = d_tot[c] + 1000.0 * vol / C_a;
+            if (dn < 0) dn = 0.0;
+            d_t[c] = dn;
+        }
+    /* :347-350 output fluxes (from matrix_store = outflow) */
+    for (int32_t j = 0; j < nx; ++j)
+        for (int32_t i = 0; i < ny; ++i) {
+            int64_t c = IX(i, j);
+            if (qout_left) qout_left[c] = -((j > 0) ? outflow[IX(i, j - 1)] : 0.0);
+            if (qout_right) qout_right[c] = outflow[c];
+            if (qout_up) qout_up[c] = outflow[n + c];
+            if (qout_down) qout_down[c] = -((i < ny - 1) ? outflow[n + IX(i + 1, j)] : 0.0);
+        }
+    /* :418-498 outlet */
+    for (int64_t c = 0; c < n; ++c) {
+        if (outlet_flow) outlet_flow[c] = 0.0;
+        outflow[2 * n + c] = 0.0;
+        Hf[2 * n + c] = 0.0;
+    }
+    const double dt_h = time_step_min / 60.0;
+    for (int32_t k = 0; k < n_outlet; ++k) {
+        int64_t c = outlet_idx[k];
+        double h_out = fmax(d_t[c], 0.0) / 1000.0;
+        double sqrtS;
+        if (outlet_type == 1) {
+            sqrtS = sqrt(fabs(slope_outlet));
+        } else {
+            double h_safe = fmax(h_out, 1e-12);
+            sqrtS = sqrt(9.81 * roughness_squared[c] * pow(h_safe, -1.0 / 3.0));
+            if (!isfinite(sqrtS)) sqrtS = 0.0;
+        }
+        Hf[2 * n + c] = h_out;
+        double q_out = (1.0 / roughness_cell[c]) * R * pow(h_out, 5.0 / 3.0) * sqrtS / cell_area *
+                       1000.0 * 3600.0;
+        double qmax = fmax(d_t[c] - 1000.0 * 0.0, 0.0) / dt_h;
+        q_out = fmin(q_out, qmax);
+        if (!isfinite(q_out)) q_out = 0.0;
+        if (outlet_flow) outlet_flow[c] = q_out;
+        outflow[2 * n + c] = q_out;
+        d_t[c] = d_t[c] - q_out * dt_h;
+    }
+    /* :513-514 I_tot_end_cell = abs(sum(outflow,3))*dt/1000*1/3600*Resolution^2 */
+    if (I_tot_end_cell)
+        for (int64_t c = 0; c < n; ++c)
+            I_tot_end_cell[c] =
+                fabs(outflow[c] + outflow[n + c] + outflow[2 * n + c]) * dt / 1000.0 * 1.0 / 3600.0 * R2;
+    free(y);
+    free(depth);
+    free(S);
+    free(qprev);
+    free(q);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* A8  refreshing_timestep.m:17-355, inertial D4 branch (flag_CA = 0,        */
+/*     flag_D8 = 0, flag_full_momentum = 0).                                 */
+/* ------------------------------------------------------------------------ */
+double oracle_refresh_dt(int32_t ny, int32_t nx, const double *qout_right,
+                         const double *qout_up, const double *Hf,
+                         const oracle_dt_params *p, double *max_velocity,
+                         double *velocity_raster) {
+    const int64_t n = (int64_t)ny * nx;
+    const double R = p->resolution;
+    double *vr = zalloc(n), *vu = zalloc(n), *dts = zalloc(n);
+    /* :63-77 */
+    for (int64_t c = 0; c < n; ++c) {
+        double f1 = Hf[c] * 1000.0, f2 = Hf[n + c] * 1000.0, f3 = Hf[2 * n + c] * 1000.0;
+        dts[c] = fmax(fmax(f1 / 1000.0, f2 / 1000.0), f3 / 1000.0);
+        double s1 = (f1 < p->depth_tolerance_mm) ? 1e12 : f1;
+        double s2 = (f2 < p->depth_tolerance_mm) ? 1e12 : f2;
+        vr[c] = (qout_right[c] / 1000.0 / 3600.0) * R / (s1 / 1000.0);
+        vu[c] = (qout_up[c] / 1000.0 / 3600.0) * R / (s2 / 1000.0);
+    }
+    double vmax = NAN;
+    double dtw = NAN;
+    for (int32_t j = 0; j < nx; ++j)
+        for (int32_t i = 0; i < ny; ++i) {
+            int64_t c = IX(i, j);
+            double vl = -((j > 0) ? vr[c] : 0.0); /* :73 unshifted quirk */
+            double vd = -((i < ny - 1) ? vu[IX(i + 1, j)] : 0.0);
+            double raster = fmax(fmax(fmax(fabs(vl), fabs(vr[c])), fabs(vu[c])), fabs(vd));
+            if (velocity_raster) velocity_raster[c] = raster;
+            double rc = 0.5 * (vr[c] - vl), uc = 0.5 * (vu[c] - vd);
+            double tot = hypot(rc, uc);
+            int stagnant = tot < 0.01;
+            vmax = fmax(vmax, tot);
+            /* :227-238 */
+            double cw = sqrt(9.81 * fmax(dts[c], 0.0));
+            cw = fmax(cw, 1e-10);
+            if (isinf(cw)) cw = NAN;
+            if (stagnant) cw = NAN;
+            double cand = (p->flag_adaptive_timestepping == 1) ? p->alfa_min * R / cw
+                                                                : p->alfa_min * R / (raster + cw);
+            dtw = fmin(dtw, cand);
+        }
+    *max_velocity = vmax;
+    /* :205-221 */
+    double dt_adv;
+    if (vmax > 0) {
+        dt_adv = 1.0 * R / fmax(vmax, 1e-10);
+        double tsf = fmax(p->alfa_max - 2.0 * fmax(vmax - 5.0, 0.0), p->alfa_min);
+        dt_adv = dt_adv * tsf;
+        dt_adv = fmin(dt_adv, p->max_time_step);
+    } else {
+        dt_adv = p->max_time_step;
+    }
+    if (isnan(dtw)) dtw = p->max_time_step; /* :240-242 */
+    double nt = (p->flag_timestep == 1) ? fmin(dt_adv, dtw) : dtw;
+    if (p->flag_waterquality == 1) nt = fmin(1.0 * p->tmin_wq, nt);
+    if (isnan(nt) || isinf(nt)) nt = p->max_time_step;
+    /* :344-353 */
+    double tcr = fmax(p->time_step_increments * floor(nt / p->time_step_increments), p->min_time_step);
+    tcr = fmin(tcr, p->max_time_step);
+    free(vr);
+    free(vu);
+    free(dts);
+    return tcr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* A18  localAverageRateOverStep + localAdvanceEndStampedCursor +            */
+/*      localEstimateConstantDt, update_spatial_BC.m:1095-1185,1240-1250.    */
+/* cursor is 1-based as in MATLAB.                                           */
+/* ------------------------------------------------------------------------ */
+double oracle_rain_avg(const double *times, const double *values, int32_t n, double t0,
+                       double t1, int32_t *cursor) {
+    if (n <= 0 || !isfinite(t0) || !isfinite(t1) || t1 <= t0) return 0.0;
+    double dt_total = t1 - t0, accum = 0.0;
+    int32_t cur = *cursor;
+    if (cur < 1) cur = 1;
+    if (cur > n) cur = n;
+    while (cur < n && t0 > times[cur - 1]) cur++;
+    double dt_force = (n >= 2) ? times[1] - times[0] : 0.0;
+    for (int32_t i = cur; i <= n; ++i) {
+        double ts = (i == 1) ? times[0] - dt_force : times[i - 2];
+        double te = times[i - 1];
+        double overlap = fmax(0.0, fmin(t1, te) - fmax(t0, ts));
+        if (overlap > 0 && isfinite(values[i - 1])) accum = accum + values[i - 1] * overlap;
+        if (te >= t1) {
+            cur = i;
+            break;
+        }
+    }
+    *cursor = cur;
+    return accum / dt_total;
+}
+
+/* ------------------------------------------------------------------------ */
+/* A20  HydroPol2D_Main_While.m:106-127, 211-260, 403-994 (LI + GA + WQ).    */
+/* ------------------------------------------------------------------------ */
+typedef struct ga_tables {
+    double ksat, psi, dtheta, Lu_mm, k_out, Tr;
+} ga_tables;
+
+/* GA derived parameters, HydroPol2D_preprocessing.m:2249-2253 */
+static ga_tables ga_class(const hp2d_run_inputs *in, int32_t k) {
+    ga_tables g;
+    g.ksat = in->ksat_mmh[k];
+    g.psi = in->psi_mm[k];
+    g.dtheta = in->dtheta[k];
+    double kr = (1.0 / 75.0 * (sqrt(g.ksat / 25.4)));
+    g.Tr = 4.5 / sqrt(g.ksat / 25.4);
+    double Lu = 4.0 * sqrt(g.ksat / 25.4);
+    Lu = Lu * 2.54 / 100.0;
+    if (in->lu_m && !isnan(in->lu_m[k])) Lu = in->lu_m[k];
+    g.k_out = in->theta_sat_minus_r[k] * kr * Lu * 1000.0;
+    g.Lu_mm = Lu * 1000.0;
+    return g;
+}
+
+int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
+                   oracle_run_out *out) {
+    const int32_t ny = cfg->ny, nx = cfg->nx;
+    const int64_t n = (int64_t)ny * nx;
+    const double R = cfg->resolution, A = R * R;
+    if (cfg->routing != HP2D_ROUTING_INERTIAL) return HP2D_E_UNSUPPORTED;
+
+    double *z = zalloc(n), *nman = zalloc(n), *n2 = zalloc(n);
+    double *d_t = zalloc(n), *d_p = zalloc(n), *d_tot = zalloc(n);
+    double *outflow = zalloc(3 * n), *outflow_prev = zalloc(3 * n), *Hf = zalloc(3 * n);
+    double *qr = zalloc(n), *qu = zalloc(n), *qof = zalloc(n), *dmax = zalloc(n);
+    double *I_t = zalloc(n), *I_p = zalloc(n), *Tclk = zalloc(n), *vraster = zalloc(n);
+    double *area_nan = zalloc(n);
+    uint8_t *is_nan = (uint8_t *)calloc((size_t)n, 1);
+    int64_t *oidx = (int64_t *)calloc((size_t)n, sizeof(int64_t));
+    int32_t n_out = 0;
+    int32_t n_classes = cfg->n_soil_classes > 0 ? cfg->n_soil_classes : 1;
+    ga_tables *ga = (ga_tables *)calloc((size_t)n_classes, sizeof(ga_tables));
+    const int do_ga = cfg->flag_infiltration == 1;
+    if (do_ga)
+        for (int32_t k = 0; k < n_classes; ++k) ga[k] = ga_class(in, k);
+
+    for (int64_t c = 0; c < n; ++c) {
+        z[c] = in->elevation[c];
+        nman[c] = in->roughness[c];
+        n2[c] = nman[c] * nman[c]; /* Main_While.m:187 roughness.^2 */
+        is_nan[c] = isnan(z[c]);
+        d_t[c] = in->d0 ? in->d0[c] : 0.0;
+        if (is_nan[c]) d_t[c] = NAN;
+        area_nan[c] = is_nan[c] ? NAN : A; /* mass_balance_check.m:12-13 */
+        if (in->outlet_mask && in->outlet_mask[c]) oidx[n_out++] = c; /* find() order */
+        if (do_ga) {
+            I_t[c] = in->I0_mm ? in->I0_mm[c] : in->I_min_mm;
+            if (is_nan[c]) I_t[c] = NAN;
+        }
+    }
+    memcpy(d_p, d_t, (size_t)n * sizeof(double));
+    memcpy(I_p, I_t, (size_t)n * sizeof(double));
+
+    /* running_control */
+    double min_dt0 = cfg->min_time_step_s, max_dt = cfg->max_time_step_s;
+    double min_dt = min_dt0;
+    double rec = cfg->record_time_hydrographs_min;
+    double routing_time = cfg->routing_time_min;
+    oracle_dt_params dp = {R, cfg->alfa_min, cfg->alfa_max, 0, 0, cfg->time_step_increments_s,
+                           cfg->depth_tolerance_mm, cfg->flag_timestep,
+                           cfg->flag_adaptive_timestepping, 0, 0.0};
+
+    /* Main_While.m:109-127 + preprocessing.m:2469,2615,2618 */
+    int64_t k = 1;
+    double time_step = min_dt0 / 60.0;
+    double t = min_dt0 / 60.0;
+    double progress = (t / routing_time) * 100.0;
+    double t_previous = 0.0;
+    int32_t cursor = 1;
+    double delta_p_agg = 0.0;
+    {
+        double dt_step = fmax(t - t_previous, 0.0);
+        double avg = 0.0;
+        if (dt_step > 0) {
+            avg = oracle_rain_avg(in->rain_time_min, in->rain_intensity_mmh, in->n_rain, t_previous,
+                                  t, &cursor);
+            if (isnan(avg)) avg = 0.0;
+            delta_p_agg = avg * dt_step / 60.0;
+        }
+    }
+    t_previous = 0.0;
+    max_dt = fmin(rec * 60.0, max_dt);
+    min_dt = fmin(rec * 60.0, min_dt);
+    double time_save_previous = 0.0;
+    double tcr = 0.0;
+
+    /* Main_While.m:248-260 initial storage */
+    double S_prev;
+    {
+        double *tmp = zalloc(n);
+        for (int64_t c = 0; c < n; ++c) tmp[c] = A * d_t[c] / 1000.0;
+        double S_p = nansum2(ny, nx, tmp);
+        double S_UZ = 0.0;
+        if (do_ga) {
+            for (int64_t c = 0; c < n; ++c) tmp[c] = A * I_t[c] / 1000.0;
+            S_UZ = nansum2(ny, nx, tmp);
+        }
+        S_prev = 0.0 + S_p + S_UZ + 0.0;
+        free(tmp);
+    }
+    int32_t catch_index = 1;
+    double next_report = cfg->report_every_percent;
+    double outflow_volume = 0.0, inflow_volume = 0.0, volume_error = 0.0, runoff_mm = 0.0;
+    int32_t n_reports = 0;
+    int32_t last_record = 1;
+    double drainage_area = 0.0;
+    for (int64_t c = 0; c < n; ++c)
+        if (!is_nan[c]) drainage_area += A;
+    double cum_rain = 0.0, cum_inf = 0.0, cum_out = 0.0;
+    int64_t rejected = 0, accepted = 0;
+    double max_velocity = 0.0;
+    int64_t inject = out->inject_fail_step;
+    int32_t n_hydro = 0;
+    int32_t n_dt = 0;
+    double *pat = (double *)in->rain_pattern;
+
+    while (t <= (routing_time + min_dt / 60.0)) {
+        if (out->max_steps > 0 && accepted >= out->max_steps) break;
+        int failed = 0;
+        /* ---------------- Hydrological_Model.m:15-55 ---------------- */
+        /* interception_module.m:59-71, Snow_Module.m:97-99: d_t += Eff_Rainfall */
+        double rain_vol_step = 0.0, inf_vol_step = 0.0;
+        const double dt_used_min = time_step;
+        for (int64_t c = 0; c < n; ++c) {
+            double eff = pat ? pat[c] * delta_p_agg : delta_p_agg;
+            if (is_nan[c]) eff = NAN;
+            else rain_vol_step += eff / 1000.0 * A;
+            d_t[c] = d_t[c] + eff;
+        }
+        /* Green-Ampt in the Infiltration_Module slot (legacy closure,
+         * HydroPol2D_Main_While_Ensemble.m:243-298; eff_depth as in
+         * Infiltration_Module.m:128-172) */
+        if (do_ga) {
+            double dt_h = time_step / 60.0;
+            for (int64_t c = 0; c < n; ++c) {
+                int32_t cls = in->soil_class ? in->soil_class[c] : 0;
+                const ga_tables *g = &ga[cls];
+                double eff = d_t[c];
+                double i_a = eff / fmax(dt_h, 1e-12);
+                double f;
+                if (k == 1) {
+                    double C = g->ksat * (1.0 + ((eff + g->psi) * g->dtheta) / I_p[c]);
+                    f = fmin(C, i_a);
+                    I_t[c] = fmax(I_p[c] + f * dt_h, in->I_min_mm);
+                    Tclk[c] = g->Tr;
+                } else {
+                    double C = g->ksat * (1.0 + ((eff + g->psi) * g->dtheta) / I_p[c]);
+                    if (I_p[c] > g->Lu_mm) C = g->ksat;
+                    int idx_C = (i_a <= C);
+                    Tclk[c] = Tclk[c] - dt_h;
+                    if (i_a > C) Tclk[c] = g->Tr;
+                    f = fmin(C, i_a);
+                    I_t[c] = fmax(I_p[c] + f * dt_h - g->k_out * (double)idx_C * dt_h, in->I_min_mm);
+                    if (Tclk[c] < 0) I_t[c] = in->I_min_mm;
+                }
+                if (is_nan[c]) {
+                    I_t[c] = NAN;
+                    f = NAN;
+                }
+                d_t[c] = d_t[c] - f * dt_h;
+                if (!isnan(f)) inf_vol_step += f * dt_h / 1000.0 * A;
+            }
+        }
+        /* Hydrological_Model.m:30 (no inflow), :45-55 */
+        double dmin = NAN;
+        for (int64_t c = 0; c < n; ++c) dmin = fmin(dmin, d_t[c]);
+        if (dmin < -1e-8) {
+            catch_index = catch_index + 1;
+        } else {
+            for (int64_t c = 0; c < n; ++c)
+                if (d_t[c] < 1e-6) d_t[c] = 0.0;
+        }
+        for (int64_t c = 0; c < n; ++c) {
+            if (d_t[c] < 0) d_t[c] = 0.0;
+            d_tot[c] = d_t[c];
+        }
+        /* ---------------- Local_Inertial_Model_D4 (:596-615) ---------------- */
+        oracle_li_d4(ny, nx, cfg->flag_numerical_scheme, z, d_tot, nman, n2, A, time_step, R,
+                     cfg->outlet_type, cfg->slope_outlet, oidx, n_out, outflow_prev,
+                     cfg->flag_critical, NULL, qr, qu, NULL, qof, d_t, NULL, outflow, Hf);
+        if (inject > 0 && k == inject) {
+            failed = 1;
+            inject = 0;
+        }
+        if (!failed) {
+            /* ---------------- refreshing_timestep (:693-705) ---------------- */
+            int64_t pos_save = (int64_t)ceil((t * 60.0) / cfg->time_step_change_s);
+            double time_save = (double)(pos_save - 1) * cfg->time_step_change_s / 60.0;
+            double delta_time_save = time_save - time_save_previous;
+            time_save_previous = time_save;
+            if (delta_time_save > 0 || k == 1) {
+                dp.max_time_step = max_dt;
+                dp.min_time_step = min_dt;
+                tcr = oracle_refresh_dt(ny, nx, qr, qu, Hf, &dp, &max_velocity, vraster);
+                time_step = tcr / 60.0;
+            } else {
+                tcr = time_step * 60.0;
+            }
+            /* :711-715 first report check (consumes thresholds) */
+            if (progress >= next_report || progress >= 100.0) next_report += cfg->report_every_percent;
+            /* update_spatial_BC (:721) */
+            {
+                double dt_step = fmax(t - t_previous, 0.0);
+                delta_p_agg = 0.0;
+                if (dt_step > 0) {
+                    double avg = oracle_rain_avg(in->rain_time_min, in->rain_intensity_mmh,
+                                                 in->n_rain, t_previous, t, &cursor);
+                    if (isnan(avg)) avg = 0.0;
+                    delta_p_agg = avg * dt_step / 60.0;
+                }
+            }
+            /* :734-740 */
+            if (k == 1) t_previous = tcr / 60.0;
+            else t_previous = t;
+            double t_save = t + tcr / 60.0;
+            /* save_output_maps.m:266-269, 409-411, 481-490 */
+            int32_t actual = 0;
+            {
+                int32_t n_rec = (int32_t)floor(routing_time / rec + 1e-12) + 1;
+                for (int32_t r = 1; r <= n_rec; ++r)
+                    if ((double)(r - 1) * rec <= t_save) actual = r;
+            }
+            double qsum = 0.0;
+            for (int32_t kk = 0; kk < n_out; ++kk) qsum += qof[oidx[kk]] * A; /* nansum(outlet_flow.*C_a) */
+            double dsum = 0.0;
+            for (int32_t kk = 0; kk < n_out; ++kk) dsum += d_t[oidx[kk]];
+            double dmean = n_out > 0 ? dsum / n_out : NAN;
+            if (k == 1) {
+                if (n_hydro < out->n_hydro_max) {
+                    out->hydro_q[0] = qsum / 1000.0 / 3600.0;
+                    out->hydro_t[0] = tcr / 60.0;
+                    out->hydro_depth[0] = dmean;
+                    if (n_hydro < 1) n_hydro = 1;
+                }
+            } else if (actual - last_record > 0) {
+                int32_t ts = actual;
+                if (ts - 1 < out->n_hydro_max) {
+                    out->hydro_q[ts - 1] = qsum / 1000.0 / 3600.0;
+                    out->hydro_t[ts - 1] = t;
+                    out->hydro_depth[ts - 1] = dmean;
+                    if (n_hydro < ts) n_hydro = ts;
+                }
+            }
+            last_record = actual;
+            /* :588-600 dmax */
+            for (int64_t c = 0; c < n; ++c) dmax[c] = (k == 1) ? d_t[c] : fmax(d_t[c], dmax[c]);
+            /* :611-617 */
+            runoff_mm = qsum * time_step / 60.0 / drainage_area + runoff_mm;
+            /* FP64 running ledgers (every step) */
+            cum_rain += rain_vol_step;
+            cum_inf += inf_vol_step;
+            cum_out += qsum / 1000.0 / 3600.0 * (dt_used_min * 60.0);
+            /* :766-774 */
+            memcpy(d_p, d_t, (size_t)n * sizeof(double));
+            memcpy(I_p, I_t, (size_t)n * sizeof(double));
+            memcpy(outflow_prev, outflow, (size_t)3 * n * sizeof(double));
+            if (n_dt < out->n_dt_max) out->dt_hist[n_dt] = tcr;
+            n_dt++;
+            /* :779-787 */
+            t = tcr / 60.0 + t;
+            k = k + 1;
+            accepted++;
+            progress = (t / routing_time) * 100.0;
+            int do_report = 0;
+            if (progress >= next_report || progress >= 100.0) {
+                do_report = 1;
+                next_report += cfg->report_every_percent;
+            }
+            if (do_report) {
+                /* mass_balance_check.m:6-73 + system_mass_balance.m:27-36 */
+                double *tmp = zalloc(n);
+                for (int64_t c = 0; c < n; ++c) tmp[c] = qof[c] * area_nan[c];
+                outflow_volume = nansum2(ny, nx, tmp) / 1000.0 / 3600.0 * time_step * 60.0 + outflow_volume;
+                for (int64_t c = 0; c < n; ++c) tmp[c] = 0.0 / 1000.0 * area_nan[c];
+                double inflow_part = nansum2(ny, nx, tmp);
+                for (int64_t c = 0; c < n; ++c)
+                    tmp[c] = (pat ? pat[c] * delta_p_agg : delta_p_agg) / 1000.0 * area_nan[c];
+                double Pv = nansum2(ny, nx, tmp);
+                double inflow_vol = inflow_part + Pv + 0.0;
+                inflow_volume = inflow_vol + inflow_volume;
+                double Qin = inflow_vol - Pv;
+                for (int64_t c = 0; c < n; ++c) tmp[c] = qof[c] * area_nan[c];
+                double Qout = nansum2(ny, nx, tmp) / 1000.0 / 3600.0 * time_step * 60.0;
+                Qout = Qout + 0.0; /* recharge_rate = 0 without groundwater */
+                for (int64_t c = 0; c < n; ++c) tmp[c] = area_nan[c] * d_t[c] / 1000.0;
+                double S_p = nansum2(ny, nx, tmp);
+                double S_UZ = 0.0;
+                if (do_ga) {
+                    for (int64_t c = 0; c < n; ++c) tmp[c] = area_nan[c] * I_t[c] / 1000.0;
+                    S_UZ = nansum2(ny, nx, tmp);
+                }
+                double fluxes = Pv + Qin - 0.0 - 0.0 - 0.0 - Qout - 0.0;
+                double dS = (0.0 + S_p + S_UZ + 0.0 + 0.0) - S_prev;
+                S_prev = S_prev + dS;
+                volume_error = dS - fluxes;
+                if (out->volume_error_hist && n_reports < out->n_report_max)
+                    out->volume_error_hist[n_reports] = volume_error;
+                n_reports++;
+                free(tmp);
+            }
+        } else {
+            /* catch block, Main_While.m:926-993 */
+            rejected++;
+            t = t - time_step;
+            t_previous = t;
+            double dmx = NAN;
+            for (int64_t c = 0; c < n; ++c) {
+                dmx = fmax(dmx, d_tot[c] / 1000.0);
+            }
+            double dmx2 = NAN;
+            for (int64_t c = 0; c < n; ++c) dmx2 = fmax(dmx2, d_p[c] / 1000.0);
+            double wave_celerity = sqrt(9.81 * fmax(dmx, dmx2));
+            double max_vel = NAN;
+            for (int64_t c = 0; c < n; ++c) max_vel = fmax(max_vel, vraster[c]);
+            catch_index = catch_index + 1;
+            double factor = 1.0 / catch_index / cfg->factor_reduction;
+            double nt;
+            if (cfg->flag_adaptive_timestepping == 1)
+                nt = factor * (cfg->alfa_min * R / wave_celerity);
+            else
+                nt = factor * (cfg->alfa_min * R / (max_vel + wave_celerity));
+            double dt_wb = nt;
+            nt = fmin(nt, dt_wb);
+            if (catch_index > 1) max_dt = fmin(nt * 1.5, max_dt);
+            nt = fmin(nt, max_dt);
+            if (isinf(nt)) nt = min_dt;
+            time_step = nt / 60.0;
+            t = t + time_step;
+            memcpy(d_t, d_p, (size_t)n * sizeof(double));
+            memcpy(I_t, I_p, (size_t)n * sizeof(double));
+            {
+                double dt_step = fmax(t - t_previous, 0.0);
+                delta_p_agg = 0.0;
+                if (dt_step > 0) {
+                    double avg = oracle_rain_avg(in->rain_time_min, in->rain_intensity_mmh,
+                                                 in->n_rain, t_previous, t, &cursor);
+                    if (isnan(avg)) avg = 0.0;
+                    delta_p_agg = avg * dt_step / 60.0;
+                }
+            }
+        }
+    }
+
+    /* outputs */
+    if (out->d_t) memcpy(out->d_t, d_t, (size_t)n * sizeof(double));
+    if (out->outflow) memcpy(out->outflow, outflow, (size_t)3 * n * sizeof(double));
+    if (out->I_t && do_ga) memcpy(out->I_t, I_t, (size_t)n * sizeof(double));
+    if (out->dmax) memcpy(out->dmax, dmax, (size_t)n * sizeof(double));
+    out->n_hydro = n_hydro;
+    hp2d_run_report *rp = &out->rep;
+    rp->steps = accepted;
+    rp->rejected_steps = rejected;
+    rp->t_min = t;
+    rp->time_step_s = time_step * 60.0;
+    rp->max_velocity = max_velocity;
+    rp->volume_error_m3 = volume_error;
+    rp->n_reports = n_reports;
+    rp->outflow_volume_m3 = outflow_volume;
+    rp->inflow_volume_m3 = inflow_volume;
+    rp->outlet_runoff_volume_mm = runoff_mm;
+    rp->cum_rain_m3 = cum_rain;
+    rp->cum_infiltration_m3 = cum_inf;
+    rp->cum_outflow_m3 = cum_out;
+    {
+        double *tmp = zalloc(n);
+        for (int64_t c = 0; c < n; ++c) tmp[c] = area_nan[c] * d_t[c] / 1000.0;
+        rp->storage_m3 = nansum2(ny, nx, tmp);
+        free(tmp);
+    }
+    rp->catch_index = catch_index;
+    free(z); free(nman); free(n2); free(d_t); free(d_p); free(d_tot);
+    free(outflow); free(outflow_prev); free(Hf); free(qr); free(qu); free(qof); free(dmax);
+    free(I_t); free(I_p); free(Tclk); free(vraster); free(area_nan); free(is_nan); free(oidx);
+    free(ga);
+    return 0;
+}

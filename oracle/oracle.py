@@ -1,0 +1,209 @@
+"""ctypes wrapper of the FP64 CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker.  The product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from hydropol2d_amd import abi
+from hydropol2d_amd.runspec import RunSpec
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libhp2d_oracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_SO):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _SO
+
+
+class DtParams(C.Structure):
+    _fields_ = [
+        ("resolution", C.c_double), ("alfa_min", C.c_double), ("alfa_max", C.c_double),
+        ("max_time_step", C.c_double), ("min_time_step", C.c_double),
+        ("time_step_increments", C.c_double), ("depth_tolerance_mm", C.c_double),
+        ("flag_timestep", C.c_int32), ("flag_adaptive_timestepping", C.c_int32),
+        ("flag_waterquality", C.c_int32), ("tmin_wq", C.c_double),
+    ]
+
+
+class RunOut(C.Structure):
+    _fields_ = [
+        ("d_t", _dp), ("outflow", _dp), ("I_t", _dp), ("dmax", _dp), ("B_t", _dp),
+        ("gw_h", _dp), ("hydro_q", _dp), ("hydro_t", _dp), ("hydro_depth", _dp),
+        ("n_hydro_max", C.c_int32), ("n_hydro", C.c_int32), ("dt_hist", _dp),
+        ("n_dt_max", C.c_int32), ("max_steps", C.c_int64), ("inject_fail_step", C.c_int64),
+        ("rep", abi.RunReport), ("volume_error_hist", _dp), ("n_report_max", C.c_int32),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_SO)
+        L.oracle_inertial_solver.argtypes = [C.c_int32, C.c_int64, C.c_int32, _dp, C.c_double,
+                                             _dp, _dp, _dp, _dp]
+        L.oracle_li_d4.argtypes = [C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp,
+                                   C.c_double, C.c_double, C.c_double, C.c_int32, C.c_double,
+                                   _i64p, C.c_int32, _dp, C.c_int32] + [_dp] * 9
+        L.oracle_li_d4.restype = C.c_int32
+        L.oracle_refresh_dt.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp,
+                                        C.POINTER(DtParams), _dp, _dp]
+        L.oracle_refresh_dt.restype = C.c_double
+        L.oracle_rain_avg.argtypes = [_dp, _dp, C.c_int32, C.c_double, C.c_double,
+                                      C.POINTER(C.c_int32)]
+        L.oracle_rain_avg.restype = C.c_double
+        L.oracle_run.argtypes = [C.POINTER(abi.RunConfig), C.POINTER(abi.RunInputs),
+                                 C.POINTER(RunOut)]
+        L.oracle_run.restype = C.c_int32
+        if hasattr(L, "oracle_ca_routing"):
+            L.oracle_ca_routing.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp, C.c_double,
+                                            C.c_double, _dp, C.c_double, _dp, C.c_int32,
+                                            C.c_double, _i64p, C.c_int32, C.c_int32] + [_dp] * 7
+            L.oracle_ca_routing.restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else _dp()
+
+
+def outlet_linear(row_outlet, col_outlet, ny):
+    """sub2ind(...)-1 of 1-based MATLAB rows/cols (unique, find() order)."""
+    r = np.asarray(row_outlet, dtype=np.int64).ravel()
+    c = np.asarray(col_outlet, dtype=np.int64).ravel()
+    return np.ascontiguousarray((r - 1) + (c - 1) * ny, dtype=np.int64)
+
+
+def inertial_solver(q_p, dt, Hf, S, n_sq):
+    q_p, Hf, S, n_sq = _f(q_p), _f(Hf), _f(S), _f(n_sq)
+    n_cells = n_sq.size
+    n_pages = q_p.size // n_cells
+    q = np.zeros_like(q_p)
+    lib().oracle_inertial_solver(1, n_cells, n_pages, _p(q_p), dt, _p(Hf), _p(S), _p(n_sq), _p(q))
+    return q
+
+
+def li_d4(z, d_tot, roughness_cell, roughness_squared, cell_area, time_step_min, resolution,
+          outlet_type, slope_outlet, row_outlet, col_outlet, outflow_prev, flag_critical=0):
+    """Local_Inertial_Model_D4 (subgrid off) on the oracle; returns a dict of outputs."""
+    z = _f(z)
+    ny, nx = z.shape
+    d_tot = _f(d_tot)
+    rc, rs = _f(roughness_cell), _f(roughness_squared)
+    of_prev = _f(outflow_prev).reshape(ny, nx, 3, order="F")
+    oidx = outlet_linear(row_outlet, col_outlet, ny)
+    outs = {k: np.zeros((ny, nx), order="F") for k in
+            ("qout_left", "qout_right", "qout_up", "qout_down", "outlet_flow", "d_t",
+             "I_tot_end_cell")}
+    outs["outflow"] = np.zeros((ny, nx, 3), order="F")
+    outs["Hf"] = np.zeros((ny, nx, 3), order="F")
+    lib().oracle_li_d4(ny, nx, 1, _p(z), _p(d_tot), _p(rc), _p(rs), cell_area, time_step_min,
+                       resolution, outlet_type, slope_outlet, oidx.ctypes.data_as(_i64p),
+                       oidx.size, _p(of_prev), flag_critical,
+                       *[_p(outs[k]) for k in ("qout_left", "qout_right", "qout_up", "qout_down",
+                                               "outlet_flow", "d_t", "I_tot_end_cell",
+                                               "outflow", "Hf")])
+    return outs
+
+
+def ca_routing(z, d_tot, roughness, cell_area, time_step_min, h_0, resolution,
+               I_tot_end_cell_prev, outlet_type, slope_outlet, row_outlet, col_outlet,
+               flag_critical=0):
+    z = _f(z)
+    ny, nx = z.shape
+    oidx = outlet_linear(row_outlet, col_outlet, ny)
+    h0 = _f(h_0) if h_0 is not None else None
+    outs = {k: np.zeros((ny, nx), order="F") for k in
+            ("qout_left", "qout_right", "qout_up", "qout_down", "outlet_flow", "d_t",
+             "I_tot_end_cell")}
+    lib().oracle_ca_routing(ny, nx, _p(z), _p(_f(d_tot)), _p(_f(roughness)), cell_area,
+                            time_step_min, _p(h0), resolution, _p(_f(I_tot_end_cell_prev)),
+                            outlet_type, slope_outlet, oidx.ctypes.data_as(_i64p), oidx.size,
+                            flag_critical,
+                            *[_p(outs[k]) for k in ("qout_left", "qout_right", "qout_up",
+                                                    "qout_down", "outlet_flow", "d_t",
+                                                    "I_tot_end_cell")])
+    return outs
+
+
+def refresh_dt(qout_right, qout_up, Hf, resolution, alfa_min=0.7, alfa_max=0.7,
+               max_time_step=10.0, min_time_step=0.01, time_step_increments=0.001,
+               depth_tolerance_mm=1.0, flag_timestep=1, flag_adaptive_timestepping=1,
+               flag_waterquality=0, tmin_wq=np.inf):
+    qr, qu, Hf = _f(qout_right), _f(qout_up), _f(Hf)
+    ny, nx = qr.shape
+    p = DtParams(resolution, alfa_min, alfa_max, max_time_step, min_time_step,
+                 time_step_increments, depth_tolerance_mm, flag_timestep,
+                 flag_adaptive_timestepping, flag_waterquality, tmin_wq)
+    vmax = C.c_double()
+    raster = np.zeros((ny, nx), order="F")
+    tcr = lib().oracle_refresh_dt(ny, nx, _p(qr), _p(qu), _p(Hf), C.byref(p), C.byref(vmax),
+                                  _p(raster))
+    return tcr, vmax.value, raster
+
+
+def rain_avg(times, values, t0, t1, cursor=1):
+    times, values = _f(times).ravel(), _f(values).ravel()
+    cur = C.c_int32(cursor)
+    v = lib().oracle_rain_avg(_p(times), _p(values), min(times.size, values.size), t0, t1,
+                              C.byref(cur))
+    return v, cur.value
+
+
+def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: int = 200000,
+        n_report_max: int = 64):
+    """Full Main_While run on the oracle.  Returns a dict of final state + ledgers."""
+    ny, nx = spec.shape
+    cfg, ins, keep = spec.to_c()
+    cfg.precision = abi.HP2D_FP64
+    n_h = spec.n_hydro_records()
+    res = {
+        "d_t": np.zeros((ny, nx), order="F"),
+        "outflow": np.zeros((ny, nx, 3), order="F"),
+        "I_t": np.zeros((ny, nx), order="F"),
+        "dmax": np.zeros((ny, nx), order="F"),
+        "hydro_q": np.zeros(n_h),
+        "hydro_t": np.zeros(n_h),
+        "hydro_depth": np.zeros(n_h),
+        "dt_hist": np.zeros(n_dt_max),
+        "volume_error_hist": np.zeros(n_report_max),
+    }
+    out = RunOut()
+    out.d_t, out.outflow, out.I_t, out.dmax = (_p(res[k]) for k in ("d_t", "outflow", "I_t", "dmax"))
+    out.hydro_q, out.hydro_t, out.hydro_depth = (_p(res[k]) for k in
+                                                 ("hydro_q", "hydro_t", "hydro_depth"))
+    out.n_hydro_max = n_h
+    out.dt_hist = _p(res["dt_hist"])
+    out.n_dt_max = n_dt_max
+    out.max_steps = max_steps
+    out.inject_fail_step = inject_fail_step
+    out.volume_error_hist = _p(res["volume_error_hist"])
+    out.n_report_max = n_report_max
+    rc = lib().oracle_run(C.byref(cfg), C.byref(ins), C.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle_run failed with status {rc}")
+    rep = out.rep.as_dict()
+    res["report"] = rep
+    res["n_hydro"] = out.n_hydro
+    res["dt_hist"] = res["dt_hist"][: min(rep["steps"], n_dt_max)]
+    res["volume_error_hist"] = res["volume_error_hist"][: min(rep["n_reports"], n_report_max)]
+    del keep
+    return res
