@@ -1,0 +1,134 @@
+#!/usr/bin/env python3
+"""Throughput of the HydroPol2D explicit surface-water step on one MI355X.
+
+A "step" is one iteration of the device-resident Main_While loop on the
+4096 x 4096 full-physics workload (fractal DEM, 8 LULC classes, 5 Green-Ampt
+soil classes, Gaussian-random-field rainfall on a triangular hyetograph,
+critical-depth outlets on one edge): the fused rainfall + Green-Ampt +
+local-inertial kernel followed by the one-block dt/bookkeeping kernel.
+Inputs are resident in HBM before timing; steps run as captured hipGraphs.
+
+Prints one JSON line (value = cell-updates/s of the whole job).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=400)
+    p.add_argument("--warmup", type=int, default=40)
+    p.add_argument("--size", type=int, default=4096)
+    p.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    p.add_argument("--graph-steps", type=int, default=50)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-steps", type=int, default=3)
+    return p.parse_args()
+
+
+def algorithmic_bytes_per_cell(spec, elem):
+    """Distinct per-cell arrays read once and written once per step."""
+    reads = 5 * elem            # d, z, n^2 (outlet flag in the sign), q_x, q_y
+    writes = 3 * elem           # d, q_x, q_y
+    if spec.rain_pattern is not None:
+        reads += elem           # rainfall multiplier map
+    if spec.infiltration is not None:
+        reads += 2 * elem + (1 if spec.infiltration.soil_class is not None else 0)  # I, T, class
+        writes += 2 * elem      # I, T
+    reads += elem               # dmax (save_output_maps.m:588-600)
+    writes += elem
+    return reads, writes
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "decomposition is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    from hydropol2d_amd import synthetic
+    from hydropol2d_amd.simulation import Simulation
+
+    n = args.size
+    t0 = time.time()
+    spec = synthetic.full_physics_c2(n, n, 2.0, precision=args.precision, routing_time_min=1e6)
+    print(f"[bench] built {n}x{n} workload in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    elem = 4 if args.precision == "fp32" else 8
+    cells = n * n
+    sim = Simulation(spec)
+    # warm-up (also lets water start moving: rainfall begins at t = 0)
+    sim.run_fixed(args.warmup, graph_steps=min(args.graph_steps, max(args.warmup, 1)))
+    print("[bench] warm-up done", file=sys.stderr, flush=True)
+    # timed region: K whole steps through captured graphs
+    t1 = time.perf_counter()
+    sim.run_fixed(args.steps, graph_steps=args.graph_steps)
+    t2 = time.perf_counter()
+    _, total_ms, _ = sim.kernel_time()
+    wall_s = t2 - t1
+    elapsed_s = total_ms / 1000.0 if total_ms > 0 else wall_s
+    ms_per_step = elapsed_s * 1000.0 / args.steps
+    value = cells * args.steps / elapsed_s
+    # dominant kernel: average duration from HIP events around each launch on the sim stream
+    sim.run_fixed(min(args.steps, 50), graph_steps=0)
+    li_ms, _, launches = sim.kernel_time()
+    rd, wr = algorithmic_bytes_per_cell(spec, elem)
+    bytes_per_launch = (rd + wr) * cells
+    achieved = bytes_per_launch / (li_ms / 1000.0) / 1e9
+    rep = sim.report()
+    sim.close()
+    print(f"[bench] {args.steps} steps, {ms_per_step:.4f} ms/step, LI kernel {li_ms:.4f} ms "
+          f"({achieved:.0f} GB/s), t={rep['t_min']:.3f} min", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        cpu_spec = synthetic.full_physics_c2(n, n, 2.0, precision="fp64", routing_time_min=1e6)
+        tc = time.perf_counter()
+        oracle.run(cpu_spec, max_steps=args.cpu_steps, n_dt_max=16)
+        cpu_s = time.perf_counter() - tc
+        cpu = {"value": cells * args.cpu_steps / cpu_s, "unit": "cell-updates/s", "cores": 1,
+               "kind": "port",
+               "sample": f"first {args.cpu_steps} steps of the same {n}x{n} workload, FP64 C "
+                         f"oracle, single thread, {cpu_s:.1f} s"}
+        print(f"[bench] cpu oracle {cpu['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "cell-updates/s (% HBM roofline)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"full-physics single domain {n}x{n} dx=2 m: fractal DEM, "
+                               "8 LULC Manning classes, 5 Green-Ampt soil classes, GRF rainfall "
+                               "on a 2 h triangular hyetograph, critical-depth edge outlets; "
+                               "fused rain+GA+local-inertial step + dt control",
+                   "cells": cells, "precision": args.precision,
+                   "bytes_per_cell_update": rd + wr},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "li_step_kernel", "avg_kernel_ms": li_ms,
+                     "launches_timed": launches},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1047 @@
+// hp2d_capi.cpp - C ABI of include/hp2d.h: handles, drop-in per-call entry
+// points and the device-resident step loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/hp2d.h"
+#include "hp2d_forcing.hpp"
+#include "hp2d_internal.hpp"
+#include "hp2d_li.hpp"
+#include "hp2d_sim.hpp"
+
+using namespace hp2d;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct ErrSink {
+    std::string *dst;
+    void operator()(const std::string &m) const {
+        if (dst) *dst = m;
+        g_last_error = m;
+    }
+};
+
+int default_tj(int nx) { return nx >= 4096 ? 64 : (nx >= 512 ? 32 : 16); }
+
+template <typename T> struct DevArray {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void **)&p, count * sizeof(T));
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Sorted unique 0-based linear outlet indices from 1-based MATLAB rows/cols.
+hp2d_status outlet_list(int ny, int nx, const int32_t *rows, const int32_t *cols, int n,
+                        std::vector<long long> &out, std::string &err) {
+    out.clear();
+    for (int k = 0; k < n; ++k) {
+        if (rows[k] < 1 || rows[k] > ny || cols[k] < 1 || cols[k] > nx) {
+            err = "outlet index out of range";
+            return HP2D_E_ARG;
+        }
+        out.push_back((long long)(rows[k] - 1) + (long long)(cols[k] - 1) * ny);
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return HP2D_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// Handle (drop-in per-call boundaries)
+// ============================================================================
+struct hp2d_handle {
+    int ny = 0, nx = 0, precision = HP2D_FP64;
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t elem = 8;
+    void *dev[20] = {nullptr};   // z, d, n2s, qx, qy, d_t, ox, oy, q3, hfx, hfy, hf3, qof, ql, qr, qu, qd, itot
+    long long *out_idx = nullptr;
+    void *out_n = nullptr;
+    int out_cap = 0;
+    int tj = 16, wpb = 4;
+};
+
+static void set_err(hp2d_handle *h, const std::string &m) {
+    if (h) h->err = m;
+    g_last_error = m;
+}
+
+#define HTRY(h, expr)                                                                 \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess) {                                                       \
+            set_err(h, std::string(#expr) + ": " + hipGetErrorString(_e));            \
+            return HP2D_E_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+extern "C" {
+
+const char *hp2d_version(void) { return "hp2d-gfx950 0.1.0"; }
+
+int32_t hp2d_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char *hp2d_global_last_error(void) { return g_last_error.c_str(); }
+
+hp2d_status hp2d_create(int32_t ny, int32_t nx, int32_t precision, hp2d_handle **out) {
+    if (!out || ny < 1 || nx < 1 || (precision != HP2D_FP64 && precision != HP2D_FP32)) {
+        g_last_error = "hp2d_create: bad arguments";
+        return HP2D_E_ARG;
+    }
+    if (hp2d_device_count() < 1) {
+        g_last_error = "hp2d_create: no HIP device visible";
+        return HP2D_E_NO_DEVICE;
+    }
+    hp2d_handle *h = new hp2d_handle();
+    h->ny = ny;
+    h->nx = nx;
+    h->precision = precision;
+    h->elem = precision == HP2D_FP64 ? 8 : 4;
+    h->tj = default_tj(nx);
+    const size_t n = (size_t)ny * nx;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 18 && e == hipSuccess; ++i) e = hipMalloc(&h->dev[i], n * h->elem);
+    if (e != hipSuccess) {
+        g_last_error = std::string("hp2d_create: ") + hipGetErrorString(e);
+        hp2d_destroy(h);
+        return e == hipErrorOutOfMemory ? HP2D_E_NOMEM : HP2D_E_HIP;
+    }
+    *out = h;
+    return HP2D_OK;
+}
+
+void hp2d_destroy(hp2d_handle *h) {
+    if (!h) return;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (auto &p : h->dev)
+        if (p) hipFree(p);
+    if (h->out_idx) hipFree(h->out_idx);
+    if (h->out_n) hipFree(h->out_n);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char *hp2d_last_error(const hp2d_handle *h) { return h ? h->err.c_str() : g_last_error.c_str(); }
+
+}  // extern "C"
+
+template <typename T>
+static hp2d_status li_dropin(hp2d_handle *h, int scheme, const T *z, const T *d_tot,
+                             const T *rough, const T *rough2, T cell_area, T time_step_min,
+                             T resolution, int outlet_type, T slope_outlet, const int32_t *rows,
+                             const int32_t *cols, int n_out, const T *outflow_prev, int critical,
+                             T *ql, T *qr, T *qu, T *qd, T *qof, T *d_t, T *itot, T *outflow,
+                             T *Hf) {
+    if (!h) return HP2D_E_ARG;
+    if (scheme != 1) {
+        set_err(h, "only flag_numerical_scheme = 1 (Bates) is implemented");
+        return HP2D_E_UNSUPPORTED;
+    }
+    if (!z || !d_tot || !rough || !rough2 || !outflow_prev || (n_out > 0 && (!rows || !cols))) {
+        set_err(h, "hp2d_local_inertial_d4: NULL input");
+        return HP2D_E_ARG;
+    }
+    const int ny = h->ny, nx = h->nx;
+    const size_t n = (size_t)ny * nx;
+    std::vector<long long> oidx;
+    std::string err;
+    if (outlet_list(ny, nx, rows, cols, n_out, oidx, err) != HP2D_OK) {
+        set_err(h, err);
+        return HP2D_E_ARG;
+    }
+    // n^2 with the outlet flag in the sign bit; roughness of each outlet
+    std::vector<T> n2s(rough2, rough2 + n);
+    std::vector<T> on(oidx.size());
+    for (size_t k = 0; k < oidx.size(); ++k) {
+        n2s[oidx[k]] = std::copysign(std::fabs(n2s[oidx[k]]), T(-1));
+        on[k] = rough[oidx[k]];
+    }
+    if ((int)oidx.size() > h->out_cap) {
+        if (h->out_idx) hipFree(h->out_idx);
+        if (h->out_n) hipFree(h->out_n);
+        h->out_idx = nullptr;
+        h->out_n = nullptr;
+        h->out_cap = (int)oidx.size();
+        HTRY(h, hipMalloc((void **)&h->out_idx, oidx.size() * sizeof(long long)));
+        HTRY(h, hipMalloc(&h->out_n, oidx.size() * sizeof(T)));
+    }
+    hipStream_t s = h->stream;
+    T **D = (T **)h->dev;
+    HTRY(h, hipMemcpyAsync(D[0], z, n * sizeof(T), hipMemcpyHostToDevice, s));
+    HTRY(h, hipMemcpyAsync(D[1], d_tot, n * sizeof(T), hipMemcpyHostToDevice, s));
+    HTRY(h, hipMemcpyAsync(D[2], n2s.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+    HTRY(h, hipMemcpyAsync(D[3], outflow_prev, n * sizeof(T), hipMemcpyHostToDevice, s));
+    HTRY(h, hipMemcpyAsync(D[4], outflow_prev + n, n * sizeof(T), hipMemcpyHostToDevice, s));
+    if (!oidx.empty()) {
+        HTRY(h, hipMemcpyAsync(h->out_idx, oidx.data(), oidx.size() * sizeof(long long),
+                               hipMemcpyHostToDevice, s));
+        HTRY(h, hipMemcpyAsync(h->out_n, on.data(), on.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    LiArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.ny = ny;
+    a.nx = nx;
+    a.tj = h->tj;
+    a.n_slabs = (ny + kRowsOut - 1) / kRowsOut;
+    a.waves_per_block = h->wpb;
+    a.R = resolution;
+    a.R2 = resolution * resolution;
+    a.cell_area = cell_area;
+    a.C_a = resolution * resolution;
+    a.flag_critical = critical;
+    a.outlet_type = outlet_type;
+    a.sqrtS_type1 = std::sqrt(std::fabs(slope_outlet));
+    a.dt_s = time_step_min * T(60);
+    a.dt_h = time_step_min / T(60);
+    a.z = D[0];
+    a.d_src = D[1];
+    a.n2s = D[2];
+    a.qx_src = D[3];
+    a.qy_src = D[4];
+    a.d_dst = D[5];
+    a.qx_dst = D[6];
+    a.qy_dst = D[7];
+    a.q3 = D[8];
+    a.hfx = D[9];
+    a.hfy = D[10];
+    a.hf3 = D[11];
+    a.outlet_flow = D[12];
+    a.qleft = D[13];
+    a.qright = D[14];
+    a.qup = D[15];
+    a.qdown = D[16];
+    a.itot = D[17];
+    a.out_idx = h->out_idx;
+    a.out_n = (const T *)h->out_n;
+    a.n_outlet = (int)oidx.size();
+    HTRY(h, launch_li_step<T>(a, kModeDropIn, s));
+    auto get = [&](T *dst, T *src) -> hipError_t {
+        if (!dst) return hipSuccess;
+        return hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s);
+    };
+    HTRY(h, get(d_t, D[5]));
+    HTRY(h, get(ql, D[13]));
+    HTRY(h, get(qr, D[14]));
+    HTRY(h, get(qu, D[15]));
+    HTRY(h, get(qd, D[16]));
+    HTRY(h, get(qof, D[12]));
+    HTRY(h, get(itot, D[17]));
+    if (outflow) {
+        HTRY(h, get(outflow, D[6]));
+        HTRY(h, get(outflow + n, D[7]));
+        HTRY(h, get(outflow + 2 * n, D[8]));
+    }
+    if (Hf) {
+        HTRY(h, get(Hf, D[9]));
+        HTRY(h, get(Hf + n, D[10]));
+        HTRY(h, get(Hf + 2 * n, D[11]));
+    }
+    HTRY(h, hipStreamSynchronize(s));
+    return HP2D_OK;
+}
+
+extern "C" {
+
+hp2d_status hp2d_local_inertial_d4(hp2d_handle *h, int32_t flag_numerical_scheme,
+                                   const double *z, const double *d_tot,
+                                   const double *roughness_cell,
+                                   const double *roughness_squared, double cell_area,
+                                   double time_step_min, double resolution,
+                                   int32_t outlet_type, double slope_outlet,
+                                   const int32_t *row_outlet, const int32_t *col_outlet,
+                                   int32_t n_outlet, const double *outflow_prev,
+                                   int32_t flag_critical, double *qout_left, double *qout_right,
+                                   double *qout_up, double *qout_down, double *outlet_flow,
+                                   double *d_t, double *I_tot_end_cell, double *outflow,
+                                   double *Hf) {
+    if (!h || h->precision != HP2D_FP64) {
+        set_err(h, "hp2d_local_inertial_d4 needs an HP2D_FP64 handle");
+        return HP2D_E_ARG;
+    }
+    return li_dropin<double>(h, flag_numerical_scheme, z, d_tot, roughness_cell,
+                             roughness_squared, cell_area, time_step_min, resolution, outlet_type,
+                             slope_outlet, row_outlet, col_outlet, n_outlet, outflow_prev,
+                             flag_critical, qout_left, qout_right, qout_up, qout_down, outlet_flow,
+                             d_t, I_tot_end_cell, outflow, Hf);
+}
+
+hp2d_status hp2d_local_inertial_d4_f32(hp2d_handle *h, int32_t flag_numerical_scheme,
+                                       const float *z, const float *d_tot,
+                                       const float *roughness_cell,
+                                       const float *roughness_squared, float cell_area,
+                                       float time_step_min, float resolution,
+                                       int32_t outlet_type, float slope_outlet,
+                                       const int32_t *row_outlet, const int32_t *col_outlet,
+                                       int32_t n_outlet, const float *outflow_prev,
+                                       int32_t flag_critical, float *qout_left,
+                                       float *qout_right, float *qout_up, float *qout_down,
+                                       float *outlet_flow, float *d_t, float *I_tot_end_cell,
+                                       float *outflow, float *Hf) {
+    if (!h || h->precision != HP2D_FP32) {
+        set_err(h, "hp2d_local_inertial_d4_f32 needs an HP2D_FP32 handle");
+        return HP2D_E_ARG;
+    }
+    return li_dropin<float>(h, flag_numerical_scheme, z, d_tot, roughness_cell, roughness_squared,
+                            cell_area, time_step_min, resolution, outlet_type, slope_outlet,
+                            row_outlet, col_outlet, n_outlet, outflow_prev, flag_critical,
+                            qout_left, qout_right, qout_up, qout_down, outlet_flow, d_t,
+                            I_tot_end_cell, outflow, Hf);
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Device-resident simulation
+// ============================================================================
+struct hp2d_sim {
+    hp2d_run_config cfg{};
+    int ny = 0, nx = 0;
+    size_t n = 0, elem = 8;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // device memory
+    void *z = nullptr, *n2s = nullptr, *pattern = nullptr, *dmax = nullptr;
+    void *d[2] = {nullptr, nullptr}, *qx[2] = {nullptr, nullptr}, *qy[2] = {nullptr, nullptr};
+    void *I[2] = {nullptr, nullptr}, *Tc[2] = {nullptr, nullptr};
+    uint8_t *soil = nullptr;
+    GaClass *ga = nullptr;
+    long long *out_idx = nullptr;
+    void *out_n = nullptr, *out_q = nullptr, *out_d = nullptr;
+    int n_outlet = 0;
+    WavePartial *partials = nullptr;
+    int n_partials = 0;
+    SimState *st = nullptr;
+    double *rain_t = nullptr, *rain_v = nullptr;
+    int n_rain = 0;
+    double *hq = nullptr, *ht = nullptr, *hd = nullptr;
+    int n_hydro_max = 0;
+    double *dt_hist = nullptr;
+    int n_dt_max = 4096;
+    double *vol_err = nullptr;
+    int n_rep_max = 256;
+    double sum_pattern = 0, n_valid = 0, drainage_area = 0, datum = 0, I_min = 0;
+    int tj = 16, wpb = 4;
+    // graph
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    // timing of the last run_fixed
+    double li_ms = 0, total_ms = 0;
+    long long li_launches = 0;
+};
+
+static void set_err(hp2d_sim *s, const std::string &m) {
+    if (s) s->err = m;
+    g_last_error = m;
+}
+
+#define STRY(s, expr)                                                                 \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess) {                                                       \
+            set_err(s, std::string(#expr) + ": " + hipGetErrorString(_e));            \
+            return HP2D_E_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
+    LiArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    const hp2d_run_config &c = s->cfg;
+    a.ny = s->ny;
+    a.nx = s->nx;
+    a.tj = s->tj;
+    a.n_slabs = (s->ny + kRowsOut - 1) / kRowsOut;
+    a.waves_per_block = s->wpb;
+    a.R = T(c.resolution);
+    a.R2 = T(c.resolution * c.resolution);
+    a.cell_area = T(c.resolution * c.resolution);
+    a.C_a = T(c.resolution * c.resolution);
+    a.flag_critical = c.flag_critical;
+    a.outlet_type = c.outlet_type;
+    a.adaptive = c.flag_adaptive_timestepping;
+    a.sqrtS_type1 = T(std::sqrt(std::fabs(c.slope_outlet)));
+    a.depth_tol_mm = T(c.depth_tolerance_mm);
+    a.alfa_min = T(c.alfa_min);
+    a.z = (const T *)s->z;
+    a.n2s = (const T *)s->n2s;
+    for (int p = 0; p < 2; ++p) {
+        a.d_buf[p] = (T *)s->d[p];
+        a.qx_buf[p] = (T *)s->qx[p];
+        a.qy_buf[p] = (T *)s->qy[p];
+        a.I_buf[p] = (T *)s->I[p];
+        a.T_buf[p] = (T *)s->Tc[p];
+    }
+    a.st = s->st;
+    a.rain_pattern = (const T *)s->pattern;
+    a.soil_class = s->soil;
+    a.ga = s->ga;
+    a.I_min = T(s->I_min);
+    a.dmax_map = (T *)s->dmax;
+    a.out_idx = s->out_idx;
+    a.out_n = (const T *)s->out_n;
+    a.n_outlet = s->n_outlet;
+    a.out_q = (T *)s->out_q;
+    a.out_d = (T *)s->out_d;
+    a.partials = s->partials;
+    return a;
+}
+
+template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
+    FinArgs<T> f;
+    std::memset(&f, 0, sizeof(f));
+    const hp2d_run_config &c = s->cfg;
+    f.st = s->st;
+    f.partials = s->partials;
+    // only the waves of the current launch shape write partials
+    f.n_partials = ((s->ny + kRowsOut - 1) / kRowsOut) * ((s->nx + s->tj - 1) / s->tj);
+    f.routing_time = c.routing_time_min;
+    f.tsc = c.time_step_change_s;
+    f.inc = c.time_step_increments_s;
+    f.alfa_min = c.alfa_min;
+    f.alfa_max = c.alfa_max;
+    f.R = c.resolution;
+    f.A = c.resolution * c.resolution;
+    f.rec = c.record_time_hydrographs_min;
+    f.report_pct = c.report_every_percent;
+    f.factor_reduction = c.factor_reduction;
+    f.drainage_area = s->drainage_area;
+    f.sum_pattern = s->sum_pattern;
+    f.n_valid = s->n_valid;
+    f.flag_timestep = c.flag_timestep;
+    f.adaptive = c.flag_adaptive_timestepping;
+    f.ga = c.flag_infiltration == 1;
+    f.flag_wq = c.flag_waterquality == 1;
+    f.has_pattern = s->pattern != nullptr;
+    f.n_rec = s->n_hydro_max;
+    f.rain_t = s->rain_t;
+    f.rain_v = s->rain_v;
+    f.n_rain = s->n_rain;
+    f.out_q = (const T *)s->out_q;
+    f.out_d = (const T *)s->out_d;
+    f.n_outlet = s->n_outlet;
+    f.hq = s->hq;
+    f.ht = s->ht;
+    f.hd = s->hd;
+    f.n_hydro_max = s->n_hydro_max;
+    f.dt_hist = s->dt_hist;
+    f.n_dt_max = s->n_dt_max;
+    f.vol_err_hist = s->vol_err;
+    f.n_rep_max = s->n_rep_max;
+    return f;
+}
+
+template <typename T> static hipError_t sim_step_launch(hp2d_sim *s, bool with_finalize) {
+    LiArgs<T> a = sim_li_args<T>(s);
+    hipError_t e = launch_li_step<T>(a, kModeSim, s->stream);
+    if (e != hipSuccess || !with_finalize) return e;
+    return launch_finalize<T>(sim_fin_args<T>(s), s->stream);
+}
+
+static hipError_t step_launch(hp2d_sim *s, bool with_finalize) {
+    return s->elem == 8 ? sim_step_launch<double>(s, with_finalize)
+                        : sim_step_launch<float>(s, with_finalize);
+}
+
+template <typename T> static std::vector<T> cast_vec(const double *src, size_t n, double shift = 0) {
+    std::vector<T> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = T(src[i] - shift);
+    return v;
+}
+
+extern "C" {
+
+hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
+                            hp2d_sim **out) {
+    if (!cfg || !in || !out || cfg->ny < 1 || cfg->nx < 1 || !in->elevation || !in->roughness) {
+        g_last_error = "hp2d_sim_create: bad arguments";
+        return HP2D_E_ARG;
+    }
+    if (cfg->routing != HP2D_ROUTING_INERTIAL || cfg->flag_numerical_scheme != 1) {
+        g_last_error = "hp2d_sim_create: only local-inertial routing with the Bates scheme";
+        return HP2D_E_UNSUPPORTED;
+    }
+    if (cfg->flag_waterquality || cfg->flag_groundwater) {
+        g_last_error = "hp2d_sim_create: water quality / groundwater not enabled in the loop";
+        return HP2D_E_UNSUPPORTED;
+    }
+    if (cfg->time_step_change_s <= 0 || cfg->time_step_increments_s <= 0 ||
+        cfg->record_time_hydrographs_min <= 0 || cfg->routing_time_min <= 0 ||
+        cfg->min_time_step_s <= 0) {
+        g_last_error = "hp2d_sim_create: non-positive time control";
+        return HP2D_E_ARG;
+    }
+    if (hp2d_device_count() < 1) {
+        g_last_error = "hp2d_sim_create: no HIP device visible";
+        return HP2D_E_NO_DEVICE;
+    }
+    hp2d_sim *s = new hp2d_sim();
+    s->cfg = *cfg;
+    s->ny = cfg->ny;
+    s->nx = cfg->nx;
+    s->n = (size_t)cfg->ny * cfg->nx;
+    s->elem = cfg->precision == HP2D_FP64 ? 8 : 4;
+    s->tj = default_tj(cfg->nx);
+    const size_t n = s->n;
+    const int ny = s->ny;
+    const double R = cfg->resolution, A = R * R;
+    auto fail = [&](hp2d_status st, const std::string &m) {
+        g_last_error = m;
+        hp2d_sim_destroy(s);
+        return st;
+    };
+    // ---- host preparation ----
+    std::vector<double> z(in->elevation, in->elevation + n), n2(n), d0(n, 0.0), I0(n, 0.0);
+    std::vector<long long> oidx;
+    double zmin = INFINITY;
+    for (size_t c = 0; c < n; ++c) {
+        const bool nan_cell = std::isnan(z[c]);
+        if (!nan_cell) {
+            zmin = std::min(zmin, z[c]);
+            s->n_valid += 1;
+            s->drainage_area += A;
+        }
+        n2[c] = in->roughness[c] * in->roughness[c];
+        d0[c] = in->d0 ? in->d0[c] : 0.0;
+        if (nan_cell) d0[c] = NAN;
+        else if (!(d0[c] >= 0)) return fail(HP2D_E_ARG, "hp2d_sim_create: negative initial depth");
+        if (in->outlet_mask && in->outlet_mask[c]) oidx.push_back((long long)c);
+    }
+    if (s->n_valid == 0) return fail(HP2D_E_ARG, "hp2d_sim_create: no valid cell");
+    for (int i = 0; i < in->n_rain; ++i)
+        if (!(in->rain_intensity_mmh[i] >= 0))
+            return fail(HP2D_E_ARG, "hp2d_sim_create: negative rainfall intensity");
+    if (in->rain_pattern)
+        for (size_t c = 0; c < n; ++c) {
+            if (std::isnan(z[c])) continue;
+            if (!(in->rain_pattern[c] >= 0))
+                return fail(HP2D_E_ARG, "hp2d_sim_create: negative rainfall pattern");
+            s->sum_pattern += in->rain_pattern[c];
+        }
+    std::vector<double> n2s = n2;
+    std::vector<double> on(oidx.size());
+    for (size_t k = 0; k < oidx.size(); ++k) {
+        n2s[oidx[k]] = std::copysign(std::fabs(n2s[oidx[k]]), -1.0);
+        on[k] = in->roughness[oidx[k]];
+    }
+    // datum shift in FP32 (translation invariant: only differences of z and of
+    // the water surface enter the step)
+    s->datum = (cfg->precision == HP2D_FP32) ? zmin : 0.0;
+    // Green-Ampt classes (HydroPol2D_preprocessing.m:2249-2253)
+    std::vector<GaClass> gac;
+    const bool ga = cfg->flag_infiltration == 1;
+    if (ga) {
+        if (cfg->n_soil_classes < 1 || !in->ksat_mmh || !in->psi_mm || !in->dtheta ||
+            !in->theta_sat_minus_r)
+            return fail(HP2D_E_ARG, "hp2d_sim_create: Green-Ampt tables missing");
+        for (int k = 0; k < cfg->n_soil_classes; ++k) {
+            GaClass g;
+            g.ksat = in->ksat_mmh[k];
+            g.psi = in->psi_mm[k];
+            g.dtheta = in->dtheta[k];
+            const double kr = (1.0 / 75.0 * (std::sqrt(g.ksat / 25.4)));
+            g.Tr = 4.5 / std::sqrt(g.ksat / 25.4);
+            double Lu = 4.0 * std::sqrt(g.ksat / 25.4);
+            Lu = Lu * 2.54 / 100.0;
+            if (in->lu_m && !std::isnan(in->lu_m[k])) Lu = in->lu_m[k];
+            g.k_out = in->theta_sat_minus_r[k] * kr * Lu * 1000.0;
+            g.Lu_mm = Lu * 1000.0;
+            gac.push_back(g);
+        }
+        if (in->soil_class)
+            for (size_t c = 0; c < n; ++c)
+                if (in->soil_class[c] >= cfg->n_soil_classes)
+                    return fail(HP2D_E_ARG, "hp2d_sim_create: soil class out of range");
+        for (size_t c = 0; c < n; ++c) {
+            I0[c] = in->I0_mm ? in->I0_mm[c] : in->I_min_mm;
+            if (std::isnan(z[c])) I0[c] = NAN;
+        }
+    }
+    // initial storage (Main_While.m:248-260), column-major nansum order
+    double S_p = 0, S_UZ = 0;
+    for (int j = 0; j < s->nx; ++j) {
+        double cp = 0, cu = 0;
+        for (int i = 0; i < ny; ++i) {
+            const size_t c = (size_t)i + (size_t)j * ny;
+            const double v = A * d0[c] / 1000.0;
+            if (!std::isnan(v)) cp += v;
+            if (ga) {
+                const double u = A * I0[c] / 1000.0;
+                if (!std::isnan(u)) cu += u;
+            }
+        }
+        S_p += cp;
+        S_UZ += cu;
+    }
+    // ---- device allocation ----
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    const size_t nb = n * s->elem;
+    auto A_ = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, bytes);
+    };
+    A_(&s->z, nb);
+    A_(&s->n2s, nb);
+    for (int p = 0; p < 2; ++p) {
+        A_(&s->d[p], nb);
+        A_(&s->qx[p], nb);
+        A_(&s->qy[p], nb);
+        if (ga) {
+            A_(&s->I[p], nb);
+            A_(&s->Tc[p], nb);
+        }
+    }
+    A_(&s->dmax, nb);
+    if (in->rain_pattern) A_(&s->pattern, nb);
+    if (ga && in->soil_class) A_((void **)&s->soil, n);
+    if (ga) A_((void **)&s->ga, gac.size() * sizeof(GaClass));
+    s->n_outlet = (int)oidx.size();
+    if (s->n_outlet > 0) {
+        A_((void **)&s->out_idx, oidx.size() * sizeof(long long));
+        A_(&s->out_n, oidx.size() * s->elem);
+        A_(&s->out_q, oidx.size() * s->elem);
+        A_(&s->out_d, oidx.size() * s->elem);
+    }
+    const int n_slabs = (ny + kRowsOut - 1) / kRowsOut;
+    s->n_partials = n_slabs * ((s->nx + 16 - 1) / 16);  // capacity for the smallest tj
+    A_((void **)&s->partials, (size_t)s->n_partials * sizeof(WavePartial));
+    A_((void **)&s->st, sizeof(SimState));
+    s->n_rain = in->n_rain;
+    A_((void **)&s->rain_t, std::max(1, in->n_rain) * sizeof(double));
+    A_((void **)&s->rain_v, std::max(1, in->n_rain) * sizeof(double));
+    s->n_hydro_max = (int)std::floor(cfg->routing_time_min / cfg->record_time_hydrographs_min + 1e-12) + 1;
+    A_((void **)&s->hq, s->n_hydro_max * sizeof(double));
+    A_((void **)&s->ht, s->n_hydro_max * sizeof(double));
+    A_((void **)&s->hd, s->n_hydro_max * sizeof(double));
+    A_((void **)&s->dt_hist, s->n_dt_max * sizeof(double));
+    A_((void **)&s->vol_err, s->n_rep_max * sizeof(double));
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? HP2D_E_NOMEM : HP2D_E_HIP,
+                    std::string("hp2d_sim_create: ") + hipGetErrorString(e));
+    // ---- uploads ----
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    auto up_real = [&](void *dst, const std::vector<double> &v, double shift) {
+        if (s->elem == 8) {
+            std::vector<double> w(v.size());
+            for (size_t i = 0; i < v.size(); ++i) w[i] = v[i] - shift;
+            up(dst, w.data(), w.size() * 8);
+        } else {
+            std::vector<float> w = cast_vec<float>(v.data(), v.size(), shift);
+            up(dst, w.data(), w.size() * 4);
+        }
+    };
+    up_real(s->z, z, s->datum);
+    up_real(s->n2s, n2s, 0.0);
+    std::vector<double> zeros(n, 0.0);
+    for (int p = 0; p < 2; ++p) {
+        up_real(s->d[p], d0, 0.0);
+        up_real(s->qx[p], zeros, 0.0);
+        up_real(s->qy[p], zeros, 0.0);
+        if (ga) {
+            up_real(s->I[p], I0, 0.0);
+            up_real(s->Tc[p], zeros, 0.0);
+        }
+    }
+    up_real(s->dmax, d0, 0.0);
+    if (in->rain_pattern) up_real(s->pattern, std::vector<double>(in->rain_pattern, in->rain_pattern + n), 0.0);
+    if (s->soil) up(s->soil, in->soil_class, n);
+    if (ga) up(s->ga, gac.data(), gac.size() * sizeof(GaClass));
+    if (s->n_outlet > 0) {
+        up(s->out_idx, oidx.data(), oidx.size() * sizeof(long long));
+        up_real(s->out_n, on, 0.0);
+        std::vector<double> zq(oidx.size(), 0.0);
+        up_real(s->out_q, zq, 0.0);
+        up_real(s->out_d, zq, 0.0);
+    }
+    if (in->n_rain > 0) {
+        up(s->rain_t, in->rain_time_min, in->n_rain * sizeof(double));
+        up(s->rain_v, in->rain_intensity_mmh, in->n_rain * sizeof(double));
+    }
+    std::vector<double> hz(s->n_hydro_max, 0.0);
+    up(s->hq, hz.data(), hz.size() * 8);
+    up(s->ht, hz.data(), hz.size() * 8);
+    up(s->hd, hz.data(), hz.size() * 8);
+    // ---- initial time control (Main_While.m:109-127, preprocessing.m:2469,2615,2618) ----
+    SimState st;
+    std::memset(&st, 0, sizeof(st));
+    const double min_dt0 = cfg->min_time_step_s;
+    st.k = 1;
+    st.time_step_min = min_dt0 / 60.0;
+    st.t = min_dt0 / 60.0;
+    st.progress = (st.t / cfg->routing_time_min) * 100.0;
+    st.t_previous = 0.0;
+    st.cursor = 1;
+    st.delta_p_agg = rain_depth(in->rain_time_min, in->rain_intensity_mmh, in->n_rain, 0.0, st.t,
+                                &st.cursor);
+    st.t_previous = 0.0;
+    st.max_dt = std::min(cfg->record_time_hydrographs_min * 60.0, cfg->max_time_step_s);
+    st.min_dt = std::min(cfg->record_time_hydrographs_min * 60.0, min_dt0);
+    st.time_save_previous = 0.0;
+    st.S_prev = 0.0 + S_p + S_UZ + 0.0;
+    st.catch_index = 1;
+    st.next_report = cfg->report_every_percent;
+    st.last_record = 1;
+    st.inject_k = -1;
+    st.stop_accepted = -1;
+    st.tmin_wq = INFINITY;
+    st.done = !(st.t <= (cfg->routing_time_min + st.min_dt / 60.0)) ? 1 : 0;
+    up(s->st, &st, sizeof(st));
+    if (e != hipSuccess)
+        return fail(HP2D_E_HIP, std::string("hp2d_sim_create upload: ") + hipGetErrorString(e));
+    s->I_min = in->I_min_mm;
+    s->err.clear();
+    *out = s;
+    return HP2D_OK;
+}
+
+void hp2d_sim_destroy(hp2d_sim *s) {
+    if (!s) return;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->graph) hipGraphExecDestroy(s->graph);
+    void *ps[] = {s->z, s->n2s, s->pattern, s->dmax, s->d[0], s->d[1], s->qx[0], s->qx[1],
+                  s->qy[0], s->qy[1], s->I[0], s->I[1], s->Tc[0], s->Tc[1], s->soil, s->ga,
+                  s->out_idx, s->out_n, s->out_q, s->out_d, s->partials, s->st, s->rain_t,
+                  s->rain_v, s->hq, s->ht, s->hd, s->dt_hist, s->vol_err};
+    for (void *p : ps)
+        if (p) hipFree(p);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+const char *hp2d_sim_last_error(const hp2d_sim *s) { return s ? s->err.c_str() : g_last_error.c_str(); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Running the loop
+// ---------------------------------------------------------------------------
+static hp2d_status read_state(hp2d_sim *s, SimState &st) {
+    STRY(s, hipStreamSynchronize(s->stream));
+    STRY(s, hipMemcpy(&st, s->st, sizeof(SimState), hipMemcpyDeviceToHost));
+    return HP2D_OK;
+}
+
+static hp2d_status write_state(hp2d_sim *s, const SimState &st) {
+    STRY(s, hipMemcpy(s->st, &st, sizeof(SimState), hipMemcpyHostToDevice));
+    return HP2D_OK;
+}
+
+static hp2d_status ensure_graph(hp2d_sim *s, int steps) {
+    if (s->graph && s->graph_steps == steps) return HP2D_OK;
+    if (s->graph) {
+        hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    STRY(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < steps; ++i) {
+        hipError_t e = step_launch(s, true);
+        if (e != hipSuccess) {
+            hipGraph_t dummy = nullptr;
+            hipStreamEndCapture(s->stream, &dummy);
+            if (dummy) hipGraphDestroy(dummy);
+            set_err(s, std::string("graph capture: ") + hipGetErrorString(e));
+            return HP2D_E_HIP;
+        }
+    }
+    STRY(s, hipStreamEndCapture(s->stream, &g));
+    hipError_t e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        set_err(s, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        s->graph = nullptr;
+        return HP2D_E_HIP;
+    }
+    s->graph_steps = steps;
+    return HP2D_OK;
+}
+
+extern "C" {
+
+hp2d_status hp2d_sim_run(hp2d_sim *s, int64_t max_steps, int32_t *done) {
+    if (!s || max_steps < 0) return HP2D_E_ARG;
+    SimState st;
+    hp2d_status rc = read_state(s, st);
+    if (rc) return rc;
+    if (st.done) {
+        if (done) *done = 1;
+        return HP2D_OK;
+    }
+    st.stop_accepted = max_steps > 0 ? st.accepted + max_steps : -1;
+    st.paused = 0;
+    if ((rc = write_state(s, st))) return rc;
+    const int batch = (int)std::min<int64_t>(64, max_steps > 0 ? max_steps : 64);
+    const bool use_graph = batch >= 8;
+    if (use_graph && (rc = ensure_graph(s, batch))) return rc;
+    // every launch either accepts or rejects a step; rejected steps only come
+    // from injected failures, so this bound always ends the loop.
+    const int64_t cap = (max_steps > 0 ? max_steps : (int64_t)1 << 40) + 1024;
+    int64_t launched = 0;
+    while (launched < cap) {
+        if (use_graph) {
+            STRY(s, hipGraphLaunch(s->graph, s->stream));
+        } else {
+            for (int i = 0; i < batch; ++i) STRY(s, step_launch(s, true));
+        }
+        launched += batch;
+        if ((rc = read_state(s, st))) return rc;
+        if (st.done || st.paused) break;
+    }
+    st.stop_accepted = -1;
+    st.paused = 0;
+    if ((rc = write_state(s, st))) return rc;
+    if (st.neg_events > 0 && s->err.empty())
+        s->err = "negative depth below -1e-8 mm met before clean-up (Hydrological_Model.m:45)";
+    if (done) *done = st.done;
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_sim_run_fixed(hp2d_sim *s, int64_t n_steps, int32_t graph_steps) {
+    if (!s || n_steps < 0) return HP2D_E_ARG;
+    hipEvent_t e0, e1, a0, a1;
+    STRY(s, hipEventCreate(&e0));
+    STRY(s, hipEventCreate(&e1));
+    STRY(s, hipEventCreate(&a0));
+    STRY(s, hipEventCreate(&a1));
+    s->li_ms = 0;
+    s->li_launches = 0;
+    if (graph_steps > 0) {
+        hp2d_status rc = ensure_graph(s, graph_steps);
+        if (rc) return rc;
+    }
+    STRY(s, hipEventRecord(e0, s->stream));
+    if (graph_steps > 0) {
+        int64_t done_steps = 0;
+        while (done_steps + graph_steps <= n_steps) {
+            STRY(s, hipGraphLaunch(s->graph, s->stream));
+            done_steps += graph_steps;
+        }
+        for (; done_steps < n_steps; ++done_steps) STRY(s, step_launch(s, true));
+    } else {
+        for (int64_t i = 0; i < n_steps; ++i) {
+            STRY(s, hipEventRecord(a0, s->stream));
+            LiArgs<double> ad;
+            hipError_t e;
+            if (s->elem == 8) e = launch_li_step<double>(sim_li_args<double>(s), kModeSim, s->stream);
+            else e = launch_li_step<float>(sim_li_args<float>(s), kModeSim, s->stream);
+            (void)ad;
+            STRY(s, e);
+            STRY(s, hipEventRecord(a1, s->stream));
+            if (s->elem == 8) e = launch_finalize<double>(sim_fin_args<double>(s), s->stream);
+            else e = launch_finalize<float>(sim_fin_args<float>(s), s->stream);
+            STRY(s, e);
+            STRY(s, hipEventSynchronize(a1));
+            float ms = 0;
+            STRY(s, hipEventElapsedTime(&ms, a0, a1));
+            s->li_ms += ms;
+            s->li_launches += 1;
+        }
+    }
+    STRY(s, hipEventRecord(e1, s->stream));
+    STRY(s, hipEventSynchronize(e1));
+    float tot = 0;
+    STRY(s, hipEventElapsedTime(&tot, e0, e1));
+    s->total_ms = tot;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(a0);
+    hipEventDestroy(a1);
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_sim_kernel_time(hp2d_sim *s, double *li_ms, double *total_ms,
+                                 int64_t *launches) {
+    if (!s) return HP2D_E_ARG;
+    if (li_ms) *li_ms = s->li_launches ? s->li_ms / (double)s->li_launches : 0.0;
+    if (total_ms) *total_ms = s->total_ms;
+    if (launches) *launches = s->li_launches;
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_sim_report(hp2d_sim *s, hp2d_run_report *rep) {
+    if (!s || !rep) return HP2D_E_ARG;
+    SimState st;
+    hp2d_status rc = read_state(s, st);
+    if (rc) return rc;
+    rep->steps = st.accepted;
+    rep->rejected_steps = st.rejected;
+    rep->t_min = st.t;
+    rep->time_step_s = st.time_step_min * 60.0;
+    rep->max_velocity = st.max_velocity;
+    rep->volume_error_m3 = st.volume_error;
+    rep->n_reports = st.n_reports;
+    rep->outflow_volume_m3 = st.outflow_volume;
+    rep->inflow_volume_m3 = st.inflow_volume;
+    rep->outlet_runoff_volume_mm = st.runoff_mm;
+    rep->cum_rain_m3 = st.cum_rain;
+    rep->cum_infiltration_m3 = st.cum_inf;
+    rep->cum_outflow_m3 = st.cum_out;
+    rep->storage_m3 = st.storage;
+    rep->catch_index = st.catch_index;
+    return HP2D_OK;
+}
+
+static hp2d_status fetch_real(hp2d_sim *s, double *dst, const void *src, double shift) {
+    if (!dst || !src) return HP2D_OK;
+    if (s->elem == 8) {
+        STRY(s, hipMemcpy(dst, src, s->n * 8, hipMemcpyDeviceToHost));
+        if (shift != 0)
+            for (size_t i = 0; i < s->n; ++i) dst[i] += shift;
+    } else {
+        std::vector<float> tmp(s->n);
+        STRY(s, hipMemcpy(tmp.data(), src, s->n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < s->n; ++i) dst[i] = (double)tmp[i] + shift;
+    }
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_sim_get_state(hp2d_sim *s, double *d_t, double *outflow, double *Hf,
+                               double *I_t, double *dmax, double *B_t, double *gw_h) {
+    if (!s) return HP2D_E_ARG;
+    SimState st;
+    hp2d_status rc = read_state(s, st);
+    if (rc) return rc;
+    if (Hf || B_t || gw_h) {
+        set_err(s, "Hf, B_t and gw_h are not kept by the device loop");
+        return HP2D_E_UNSUPPORTED;
+    }
+    const int cur = (int)((st.k + 1) & 1);  // buffer holding the last accepted state
+    if ((rc = fetch_real(s, d_t, s->d[cur], 0.0))) return rc;
+    if (outflow) {
+        if ((rc = fetch_real(s, outflow, s->qx[cur], 0.0))) return rc;
+        if ((rc = fetch_real(s, outflow + s->n, s->qy[cur], 0.0))) return rc;
+        std::vector<double> q3(s->n, 0.0);
+        if (s->n_outlet > 0) {
+            std::vector<long long> idx(s->n_outlet);
+            STRY(s, hipMemcpy(idx.data(), s->out_idx, idx.size() * 8, hipMemcpyDeviceToHost));
+            std::vector<double> q(s->n_outlet);
+            if (s->elem == 8) {
+                STRY(s, hipMemcpy(q.data(), s->out_q, q.size() * 8, hipMemcpyDeviceToHost));
+            } else {
+                std::vector<float> qf(s->n_outlet);
+                STRY(s, hipMemcpy(qf.data(), s->out_q, qf.size() * 4, hipMemcpyDeviceToHost));
+                for (size_t k = 0; k < qf.size(); ++k) q[k] = qf[k];
+            }
+            for (size_t k = 0; k < idx.size(); ++k) q3[idx[k]] = q[k];
+        }
+        std::memcpy(outflow + 2 * s->n, q3.data(), s->n * 8);
+    }
+    if (I_t) {
+        if (!s->I[cur]) {
+            set_err(s, "no infiltration state in this run");
+            return HP2D_E_STATE;
+        }
+        if ((rc = fetch_real(s, I_t, s->I[cur], 0.0))) return rc;
+    }
+    if ((rc = fetch_real(s, dmax, s->dmax, 0.0))) return rc;
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_sim_get_hydrograph(hp2d_sim *s, double *q_m3s, double *time_min,
+                                    double *depth_outlet_mm, int32_t n_max, int32_t *n_out) {
+    if (!s) return HP2D_E_ARG;
+    SimState st;
+    hp2d_status rc = read_state(s, st);
+    if (rc) return rc;
+    const int n = std::min(n_max, s->n_hydro_max);
+    if (q_m3s) STRY(s, hipMemcpy(q_m3s, s->hq, n * 8, hipMemcpyDeviceToHost));
+    if (time_min) STRY(s, hipMemcpy(time_min, s->ht, n * 8, hipMemcpyDeviceToHost));
+    if (depth_outlet_mm) STRY(s, hipMemcpy(depth_outlet_mm, s->hd, n * 8, hipMemcpyDeviceToHost));
+    if (n_out) *n_out = std::min(st.n_hydro, n);
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_sim_get_dt_history(hp2d_sim *s, double *dt_s, int32_t n_max, int32_t *n_out) {
+    if (!s || !dt_s) return HP2D_E_ARG;
+    SimState st;
+    hp2d_status rc = read_state(s, st);
+    if (rc) return rc;
+    std::vector<double> ring(s->n_dt_max);
+    STRY(s, hipMemcpy(ring.data(), s->dt_hist, ring.size() * 8, hipMemcpyDeviceToHost));
+    const long long total = st.accepted;
+    const long long avail = std::min<long long>(total, s->n_dt_max);
+    const long long take = std::min<long long>(avail, n_max);
+    for (long long i = 0; i < take; ++i) {
+        const long long step = total - take + i;
+        dt_s[i] = ring[step % s->n_dt_max];
+    }
+    if (n_out) *n_out = (int32_t)take;
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_debug_inject_failure(hp2d_sim *s, int64_t step_k, int32_t kind) {
+    if (!s || kind != 1) return HP2D_E_ARG;
+    SimState st;
+    hp2d_status rc = read_state(s, st);
+    if (rc) return rc;
+    st.inject_k = step_k;
+    return write_state(s, st);
+}
+
+}  // extern "C"
+
+namespace hp2d {
+hipError_t launch_bates(const double *q_p, double dt, const double *Hf, const double *S,
+                        const double *n_sq, long long n_cells, int n_pages, double *q,
+                        hipStream_t s);
+}
+
+extern "C" hp2d_status hp2d_inertial_solver(hp2d_handle *h, int32_t flag_numerical_scheme,
+                                            const double *q_p, double dt, const double *Hf,
+                                            const double *S, const double *n_sq, int32_t n_pages,
+                                            double *q_out) {
+    if (!h || !q_p || !Hf || !S || !n_sq || !q_out || n_pages < 1) return HP2D_E_ARG;
+    if (flag_numerical_scheme != 1) {
+        set_err(h, "only flag_numerical_scheme = 1 (Bates) is implemented");
+        return HP2D_E_UNSUPPORTED;
+    }
+    const size_t n = (size_t)h->ny * h->nx, np = n * n_pages;
+    double *dq = nullptr, *dH = nullptr, *dS = nullptr, *dn = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc((void **)&dq, np * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&dH, np * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&dS, np * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&dn, n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&dout, np * 8);
+    if (e == hipSuccess) e = hipMemcpyAsync(dq, q_p, np * 8, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dH, Hf, np * 8, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dS, S, np * 8, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dn, n_sq, n * 8, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hp2d::launch_bates(dq, dt, dH, dS, dn, (long long)n, n_pages, dout, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(q_out, dout, np * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    for (double *p : {dq, dH, dS, dn, dout})
+        if (p) hipFree(p);
+    if (e != hipSuccess) {
+        set_err(h, std::string("hp2d_inertial_solver: ") + hipGetErrorString(e));
+        return HP2D_E_HIP;
+    }
+    return HP2D_OK;
+}
+
+// Entry points whose kernels live in their own translation units are
+// declared in include/hp2d.h and defined there (hp2d_ca.hip, hp2d_wq.hip,
+// hp2d_gw.hip).

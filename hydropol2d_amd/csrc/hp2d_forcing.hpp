@@ -1,0 +1,54 @@
+// hp2d_forcing.hpp - exact step integral of an end-stamped piecewise-constant
+// forcing series, usable on host and device.
+//
+// Restates localAverageRateOverStep / localAdvanceEndStampedCursor /
+// localEstimateConstantDt of update_spatial_BC.m:1095-1185,1240-1250:
+// times[i] is the END of interval i (interval 0 starts at times[0] minus the
+// series spacing), the cursor (1-based, as in the reference) only moves
+// forward, and the result is the time-averaged rate over [t0, t1].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace hp2d {
+
+__host__ __device__ inline double rain_average(const double *times, const double *values,
+                                               int n, double t0, double t1, int *cursor) {
+    if (n <= 0 || !__builtin_isfinite(t0) || !__builtin_isfinite(t1) || t1 <= t0) return 0.0;
+    const double span = t1 - t0;
+    double acc = 0.0;
+    int cur = *cursor;
+    if (cur < 1) cur = 1;
+    if (cur > n) cur = n;
+    while (cur < n && t0 > times[cur - 1]) ++cur;
+    const double spacing = (n >= 2) ? times[1] - times[0] : 0.0;
+    for (int i = cur; i <= n; ++i) {
+        const double start = (i == 1) ? times[0] - spacing : times[i - 2];
+        const double end = times[i - 1];
+        const double overlap = fmax(0.0, fmin(t1, end) - fmax(t0, start));
+        if (overlap > 0 && __builtin_isfinite(values[i - 1])) acc = acc + values[i - 1] * overlap;
+        if (end >= t1) {
+            cur = i;
+            break;
+        }
+    }
+    *cursor = cur;
+    return acc / span;
+}
+
+// delta_p_agg [mm] of update_spatial_BC.m:422-448 for the step [t_prev, t].
+__host__ __device__ inline double rain_depth(const double *times, const double *values, int n,
+                                             double t_prev, double t, int *cursor) {
+    const double dt_step = fmax(t - t_prev, 0.0);
+    double depth = 0.0;
+    if (dt_step > 0) {
+        double avg = rain_average(times, values, n, t_prev, t, cursor);
+        if (__builtin_isnan(avg)) avg = 0.0;
+        depth = avg * dt_step / 60.0;
+    }
+    return depth;
+}
+
+}  // namespace hp2d

@@ -1,0 +1,94 @@
+// hp2d_internal.hpp - shared device/host definitions of the gfx950 HydroPol2D step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <string>
+
+namespace hp2d {
+
+constexpr int kWave = 64;                  // CDNA wavefront
+constexpr int kHalo = 2;                   // radius of the step stencil (donor limiter)
+constexpr int kRowsOut = kWave - 2 * kHalo; // rows finalised per wave slab
+
+// ----------------------------------------------------------------------------
+// Per-wave partial results of one step, reduced by the finalize kernel in a
+// fixed order (deterministic).  All FP64.
+// ----------------------------------------------------------------------------
+struct WavePartial {
+    double storage;     // sum A*d_t/1000 over valid cells      [m3]
+    double uz;          // sum A*I_t/1000 (Green-Ampt)          [m3]
+    double rain;        // rain volume applied this step        [m3]
+    double infil;       // infiltrated volume this step         [m3]
+    double vmax;        // max total cell velocity              [m/s]  (NaN = none)
+    double dtw;         // min Courant wave candidate           [s]    (NaN = none)
+    double raster;      // max velocity_raster                  [m/s]
+    double dtot_max;    // max d_tot/1000 (rollback celerity)   [m]
+    double dprev_max;   // max d_p/1000                         [m]
+    double neg;         // 1 if a depth < -1e-8 mm was seen before clean-up
+    double washed;      // wash-off mass this step [kg] (WQ)
+    double pad;
+};
+
+// ----------------------------------------------------------------------------
+// Device-resident time-control state of the Main_While loop (FP64).
+// ----------------------------------------------------------------------------
+struct SimState {
+    double t, t_previous, time_step_min, time_save_previous, delta_p_agg;
+    double progress, next_report, S_prev;
+    double max_dt, min_dt;
+    double outflow_volume, inflow_volume, volume_error, runoff_mm;
+    double cum_rain, cum_inf, cum_out, storage, uz;
+    double max_velocity, max_vel_raster, tcr;
+    double tmin_wq;
+    long long k, accepted, rejected, inject_k, stop_accepted;
+    int cursor, catch_index, done, last_record, n_hydro, n_reports, neg_events, paused;
+};
+
+struct GaClass {
+    double ksat, psi, dtheta, Lu_mm, k_out, Tr;
+};
+
+// ----------------------------------------------------------------------------
+// Small math helpers with MATLAB semantics (max/min omit NaN -> fmax/fmin).
+// ----------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T dmax(T a, T b) { return fmax(a, b); }
+template <typename T> __device__ __forceinline__ T dmin(T a, T b) { return fmin(a, b); }
+
+__device__ __forceinline__ double pow73(double h) { return pow(h, 7.0 / 3.0); }
+__device__ __forceinline__ float pow73(float h) { return h * h * cbrtf(h); }
+__device__ __forceinline__ double pow53(double h) { return pow(h, 5.0 / 3.0); }
+__device__ __forceinline__ float pow53(float h) { float c = cbrtf(h); return h * c * c; }
+__device__ __forceinline__ double powm13(double h) { return pow(h, -1.0 / 3.0); }
+__device__ __forceinline__ float powm13(float h) { return 1.0f / cbrtf(h); }
+
+template <typename T> __device__ __forceinline__ T eps_of();
+template <> __device__ __forceinline__ double eps_of<double>() { return 2.220446049250313e-16; }
+template <> __device__ __forceinline__ float eps_of<float>() { return 1.1920929e-07f; }
+
+template <typename T> __device__ __forceinline__ T wave_fmax(T v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+template <typename T> __device__ __forceinline__ T wave_fmin(T v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+}  // namespace hp2d
+
+// Error helpers shared by the C-ABI implementation.
+#define HP2D_HIP_TRY(expr)                                                          \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess) {                                                      \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(_e));            \
+            return HP2D_E_HIP;                                                       \
+        }                                                                            \
+    } while (0)

@@ -1,0 +1,51 @@
+// hp2d_li.hpp - launch arguments of the fused local-inertial step kernel.
+#pragma once
+
+#include "hp2d_internal.hpp"
+
+namespace hp2d {
+
+enum { kModeDropIn = 0, kModeSim = 1 };
+
+template <typename T> struct LiArgs {
+    int ny, nx;
+    int tj;               // columns per wave task
+    int n_slabs;          // ceil(ny / kRowsOut)
+    int waves_per_block;  // 1..4
+    // physics constants
+    T R, R2, cell_area, C_a;
+    int flag_critical, outlet_type, adaptive;
+    T sqrtS_type1;        // sqrt(|slope_outlet|)
+    T depth_tol_mm, alfa_min;
+    T dt_s, dt_h;         // drop-in mode only
+    // static fields
+    const T *z;           // elevation [m] (FP32: shifted to a datum)
+    const T *n2s;         // Manning n^2, sign bit set on outlet cells
+    // drop-in mode state
+    const T *d_src, *qx_src, *qy_src;
+    T *d_dst, *qx_dst, *qy_dst;
+    const T *I_src, *T_src;
+    T *I_dst, *T_dst;
+    // simulation mode ping-pong buffers (index = step parity)
+    T *d_buf[2], *qx_buf[2], *qy_buf[2], *I_buf[2], *T_buf[2];
+    const SimState *st;
+    // hydrology (sim mode)
+    const T *rain_pattern;
+    const uint8_t *soil_class;
+    const GaClass *ga;    // NULL = no infiltration
+    T I_min;
+    T *dmax_map;
+    // outlets (sorted linear indices)
+    const long long *out_idx;
+    const T *out_n;
+    int n_outlet;
+    T *out_q, *out_d;
+    // drop-in optional outputs
+    T *q3, *hfx, *hfy, *hf3, *outlet_flow, *qleft, *qright, *qup, *qdown, *itot;
+    // sim partials
+    WavePartial *partials;
+};
+
+template <typename T> hipError_t launch_li_step(const LiArgs<T> &a, int mode, hipStream_t s);
+
+}  // namespace hp2d

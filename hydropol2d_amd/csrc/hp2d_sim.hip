@@ -1,0 +1,211 @@
+// hp2d_sim.hip - device-side time control of the HydroPol2D step loop.
+//
+// One 256-thread block per step.  It reduces the per-wave partials of the
+// fused step in a fixed order (deterministic), then runs, in FP64 on one
+// lane, the scalar tail of one iteration of HydroPol2D_Main_While.m:403-994:
+//   refreshing_timestep bucket + Courant dt   (:693-705, refreshing_timestep.m:17-355)
+//   first report check                         (:711-715)
+//   update_spatial_BC rainfall integral        (:721, update_spatial_BC.m:422-448)
+//   t_previous / t_save                        (:734-740)
+//   save_output_maps hydrograph, runoff        (save_output_maps.m:266-269,409-411,481-490,611-617)
+//   t += dt, k += 1, report check + mass_balance_check (:779-803, mass_balance_check.m:6-73)
+// or, for a rejected step, the catch block (:926-993).
+#include "hp2d_forcing.hpp"
+#include "hp2d_internal.hpp"
+#include "hp2d_sim.hpp"
+
+namespace hp2d {
+
+constexpr int kFinThreads = 256;
+
+template <typename T>
+__global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
+    SimState *st = a.st;
+    if (st->done || st->paused) return;
+    __shared__ double sh[12][kFinThreads];
+    const int t = threadIdx.x;
+    double storage = 0, uz = 0, rain = 0, infil = 0, vmax = NAN, dtw = NAN, raster = NAN;
+    double dtm = NAN, dpm = NAN, neg = 0, qsum = 0, dsum = 0;
+    for (int i = t; i < a.n_partials; i += kFinThreads) {
+        const WavePartial p = a.partials[i];
+        storage += p.storage;
+        uz += p.uz;
+        rain += p.rain;
+        infil += p.infil;
+        vmax = fmax(vmax, p.vmax);
+        dtw = fmin(dtw, p.dtw);
+        raster = fmax(raster, p.raster);
+        dtm = fmax(dtm, p.dtot_max);
+        dpm = fmax(dpm, p.dprev_max);
+        neg = fmax(neg, p.neg);
+    }
+    for (int i = t; i < a.n_outlet; i += kFinThreads) {
+        qsum += (double)a.out_q[i] * a.A;
+        dsum += (double)a.out_d[i];
+    }
+    sh[0][t] = storage; sh[1][t] = uz; sh[2][t] = rain; sh[3][t] = infil;
+    sh[4][t] = vmax; sh[5][t] = dtw; sh[6][t] = raster; sh[7][t] = dtm;
+    sh[8][t] = dpm; sh[9][t] = neg; sh[10][t] = qsum; sh[11][t] = dsum;
+    __syncthreads();
+    for (int w = kFinThreads / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            sh[0][t] += sh[0][t + w]; sh[1][t] += sh[1][t + w];
+            sh[2][t] += sh[2][t + w]; sh[3][t] += sh[3][t + w];
+            sh[4][t] = fmax(sh[4][t], sh[4][t + w]); sh[5][t] = fmin(sh[5][t], sh[5][t + w]);
+            sh[6][t] = fmax(sh[6][t], sh[6][t + w]); sh[7][t] = fmax(sh[7][t], sh[7][t + w]);
+            sh[8][t] = fmax(sh[8][t], sh[8][t + w]); sh[9][t] = fmax(sh[9][t], sh[9][t + w]);
+            sh[10][t] += sh[10][t + w]; sh[11][t] += sh[11][t + w];
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    storage = sh[0][0]; uz = sh[1][0]; rain = sh[2][0]; infil = sh[3][0];
+    vmax = sh[4][0]; dtw = sh[5][0]; raster = sh[6][0]; dtm = sh[7][0]; dpm = sh[8][0];
+    neg = sh[9][0]; qsum = sh[10][0]; dsum = sh[11][0];
+
+    const long long k = st->k;
+    const bool failed = (st->inject_k == k);
+    const double routing_time = a.routing_time;
+    if (neg > 0) {  // Hydrological_Model.m:45-47
+        st->catch_index += 1;
+        st->neg_events += 1;
+    }
+    if (!failed) {
+        const double dt_used_min = st->time_step_min;
+        double t_now = st->t;
+        // ---------------- refreshing_timestep ----------------
+        const long long pos_save = (long long)ceil((t_now * 60.0) / a.tsc);
+        const double time_save = (double)(pos_save - 1) * a.tsc / 60.0;
+        const double delta_save = time_save - st->time_save_previous;
+        st->time_save_previous = time_save;
+        double tcr;
+        if (delta_save > 0 || k == 1) {
+            const double R = a.R;
+            double dt_adv;
+            if (vmax > 0) {
+                dt_adv = 1.0 * R / fmax(vmax, 1e-10);
+                const double tsf = fmax(a.alfa_max - 2.0 * fmax(vmax - 5.0, 0.0), a.alfa_min);
+                dt_adv = dt_adv * tsf;
+                dt_adv = fmin(dt_adv, st->max_dt);
+            } else {
+                dt_adv = st->max_dt;
+            }
+            double dw = dtw;
+            if (isnan(dw)) dw = st->max_dt;
+            double nt = a.flag_timestep == 1 ? fmin(dt_adv, dw) : dw;
+            if (a.flag_wq) nt = fmin(1.0 * st->tmin_wq, nt);
+            if (isnan(nt) || isinf(nt)) nt = st->max_dt;
+            tcr = fmax(a.inc * floor(nt / a.inc), st->min_dt);
+            tcr = fmin(tcr, st->max_dt);
+            st->time_step_min = tcr / 60.0;
+            st->max_velocity = vmax;
+            st->max_vel_raster = raster;
+        } else {
+            tcr = st->time_step_min * 60.0;
+        }
+        st->tcr = tcr;
+        // ---------------- first report check ----------------
+        if (st->progress >= st->next_report || st->progress >= 100.0)
+            st->next_report += a.report_pct;
+        // ---------------- update_spatial_BC ----------------
+        int cur = st->cursor;
+        st->delta_p_agg = rain_depth(a.rain_t, a.rain_v, a.n_rain, st->t_previous, t_now, &cur);
+        st->cursor = cur;
+        // ---------------- t_previous / t_save ----------------
+        st->t_previous = (k == 1) ? tcr / 60.0 : t_now;
+        const double t_save = t_now + tcr / 60.0;
+        // ---------------- save_output_maps ----------------
+        int actual = (int)floor(t_save / a.rec) + 1;
+        if (actual > a.n_rec) actual = a.n_rec;
+        if (actual < 1) actual = 1;
+        while (actual < a.n_rec && (double)actual * a.rec <= t_save) ++actual;
+        while (actual > 1 && (double)(actual - 1) * a.rec > t_save) --actual;
+        const double dmean = a.n_outlet > 0 ? dsum / a.n_outlet : NAN;
+        if (k == 1) {
+            if (a.n_hydro_max > 0) {
+                a.hq[0] = qsum / 1000.0 / 3600.0;
+                a.ht[0] = tcr / 60.0;
+                a.hd[0] = dmean;
+                if (st->n_hydro < 1) st->n_hydro = 1;
+            }
+        } else if (actual - st->last_record > 0) {
+            if (actual - 1 < a.n_hydro_max) {
+                a.hq[actual - 1] = qsum / 1000.0 / 3600.0;
+                a.ht[actual - 1] = t_now;
+                a.hd[actual - 1] = dmean;
+                if (st->n_hydro < actual) st->n_hydro = actual;
+            }
+        }
+        st->last_record = actual;
+        st->runoff_mm = qsum * st->time_step_min / 60.0 / a.drainage_area + st->runoff_mm;
+        st->cum_rain += rain;
+        st->cum_inf += infil;
+        st->cum_out += qsum / 1000.0 / 3600.0 * (dt_used_min * 60.0);
+        st->storage = storage;
+        st->uz = uz;
+        if (a.n_dt_max > 0) a.dt_hist[st->accepted % a.n_dt_max] = tcr;
+        // ---------------- advance ----------------
+        t_now = tcr / 60.0 + t_now;
+        st->t = t_now;
+        st->k = k + 1;
+        st->accepted += 1;
+        st->progress = (t_now / routing_time) * 100.0;
+        bool do_report = false;
+        if (st->progress >= st->next_report || st->progress >= 100.0) {
+            do_report = true;
+            st->next_report += a.report_pct;
+        }
+        if (do_report) {
+            const double ts = st->time_step_min;
+            st->outflow_volume = qsum / 1000.0 / 3600.0 * ts * 60.0 + st->outflow_volume;
+            const double Pv = a.has_pattern ? st->delta_p_agg * a.sum_pattern / 1000.0 * a.A
+                                            : a.n_valid * (st->delta_p_agg / 1000.0 * a.A);
+            const double inflow_vol = 0.0 + Pv + 0.0;
+            st->inflow_volume = inflow_vol + st->inflow_volume;
+            const double Qin = inflow_vol - Pv;
+            const double Qout = qsum / 1000.0 / 3600.0 * ts * 60.0 + 0.0;
+            const double fluxes = Pv + Qin - 0.0 - 0.0 - 0.0 - Qout - 0.0;
+            const double dS = (0.0 + storage + (a.ga ? uz : 0.0) + 0.0 + 0.0) - st->S_prev;
+            st->S_prev = st->S_prev + dS;
+            st->volume_error = dS - fluxes;
+            if (st->n_reports < a.n_rep_max) a.vol_err_hist[st->n_reports] = st->volume_error;
+            st->n_reports += 1;
+        }
+    } else {
+        // ---------------- catch block (Main_While.m:926-993) ----------------
+        st->inject_k = -1;
+        st->rejected += 1;
+        double t_now = st->t - st->time_step_min;
+        st->t_previous = t_now;
+        const double wave_celerity = sqrt(9.81 * fmax(dtm, dpm));
+        const double max_vel = st->max_vel_raster;
+        st->catch_index += 1;
+        const double factor = 1.0 / st->catch_index / a.factor_reduction;
+        double nt = (a.adaptive == 1) ? factor * (a.alfa_min * a.R / wave_celerity)
+                                      : factor * (a.alfa_min * a.R / (max_vel + wave_celerity));
+        const double dt_wb = nt;
+        nt = fmin(nt, dt_wb);
+        if (st->catch_index > 1) st->max_dt = fmin(nt * 1.5, st->max_dt);
+        nt = fmin(nt, st->max_dt);
+        if (isinf(nt)) nt = st->min_dt;
+        st->time_step_min = nt / 60.0;
+        t_now = t_now + st->time_step_min;
+        st->t = t_now;
+        int cur = st->cursor;
+        st->delta_p_agg = rain_depth(a.rain_t, a.rain_v, a.n_rain, st->t_previous, t_now, &cur);
+        st->cursor = cur;
+    }
+    st->done = !(st->t <= (routing_time + st->min_dt / 60.0)) ? 1 : 0;
+    if (st->stop_accepted >= 0 && st->accepted >= st->stop_accepted) st->paused = 1;
+}
+
+template <typename T>
+hipError_t launch_finalize(const FinArgs<T> &a, hipStream_t s) {
+    hipLaunchKernelGGL((finalize_kernel<T>), dim3(1), dim3(kFinThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+template hipError_t launch_finalize<float>(const FinArgs<float> &, hipStream_t);
+template hipError_t launch_finalize<double>(const FinArgs<double> &, hipStream_t);
+
+}  // namespace hp2d

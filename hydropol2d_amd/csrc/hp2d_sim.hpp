@@ -1,0 +1,33 @@
+// hp2d_sim.hpp - arguments of the per-step finalize kernel.
+#pragma once
+
+#include "hp2d_internal.hpp"
+
+namespace hp2d {
+
+template <typename T> struct FinArgs {
+    SimState *st;
+    const WavePartial *partials;
+    int n_partials;
+    // running_control / flags
+    double routing_time, tsc, inc, alfa_min, alfa_max, R, A, rec, report_pct;
+    double factor_reduction, drainage_area, sum_pattern, n_valid;
+    int flag_timestep, adaptive, ga, flag_wq, has_pattern, n_rec;
+    // lumped rainfall series
+    const double *rain_t, *rain_v;
+    int n_rain;
+    // outlets
+    const T *out_q, *out_d;
+    int n_outlet;
+    // records
+    double *hq, *ht, *hd;
+    int n_hydro_max;
+    double *dt_hist;
+    int n_dt_max;
+    double *vol_err_hist;
+    int n_rep_max;
+};
+
+template <typename T> hipError_t launch_finalize(const FinArgs<T> &a, hipStream_t s);
+
+}  // namespace hp2d

@@ -1,0 +1,127 @@
+"""MATLAB-signature mirrors of the reference's routing functions on the HIP library.
+
+`Local_Inertial_Model_D4` keeps the 45-argument signature and the 14 return
+values of HydroPol2D_Functions/Local_Inertial_Model_D4.m:1-5, and
+`Inertial_Solver` the one of Inertial_Solver.m:1, so a caller written against
+the reference (e.g. the validation driver
+run_local_inertial_ca_hydrodynamics_validation.m:406-411) can switch to this
+module unchanged.  Arrays are column-major (Fortran order) exactly as MATLAB
+holds them; outputs come back as Fortran-ordered numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib, abi
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
+
+
+class _Handle:
+    """A library handle cached per (ny, nx, precision), freed at interpreter exit."""
+
+    _cache: dict = {}
+
+    @classmethod
+    def get(cls, ny, nx, precision):
+        key = (ny, nx, precision)
+        h = cls._cache.get(key)
+        if h is None:
+            L = _lib.load()
+            _lib.require_device()
+            ptr = C.c_void_p()
+            _lib.check(L.hp2d_create(ny, nx, precision, C.byref(ptr)))
+            h = cls._cache[key] = ptr
+        return h
+
+    @classmethod
+    def release_all(cls):
+        L = _lib.load()
+        for h in cls._cache.values():
+            L.hp2d_destroy(h)
+        cls._cache.clear()
+
+
+def _empty_or(x):
+    return x is None or (hasattr(x, "__len__") and len(x) == 0)
+
+
+def Inertial_Solver(flag_numerical_scheme, q_p, dt, Hf, S, n_sq, dx=None, idx_nan=None):
+    """q = Inertial_Solver(flag_numerical_scheme,q_p,dt,Hf,S,n_sq,dx,idx_nan)."""
+    q_p = np.asfortranarray(q_p, dtype=np.float64)
+    Hf = np.asfortranarray(Hf, dtype=np.float64)
+    S = np.asfortranarray(S, dtype=np.float64)
+    n_sq = np.asfortranarray(n_sq, dtype=np.float64)
+    ny, nx = n_sq.shape[:2]
+    n_pages = q_p.size // (ny * nx)
+    h = _Handle.get(ny, nx, abi.HP2D_FP64)
+    out = np.zeros_like(q_p, order="F")
+    L = _lib.load()
+    _lib.check(L.hp2d_inertial_solver(h, int(flag_numerical_scheme), q_p.ctypes.data_as(_dp),
+                                      float(dt), Hf.ctypes.data_as(_dp), S.ctypes.data_as(_dp),
+                                      n_sq.ctypes.data_as(_dp), n_pages, out.ctypes.data_as(_dp)),
+               handle=h)
+    return out
+
+
+def Local_Inertial_Model_D4(flag_numerical_scheme, reservoir_x, reservoir_y, k1, h1, k2, k3, h2,
+                            k4, yds1, xds1, yds2, xds2, flag_reservoir, z, d_tot, d_p,
+                            roughness_cell, roughness_squared, cell_area, time_step, Resolution,
+                            outlet_index, outlet_type, slope_outlet, row_outlet, col_outlet,
+                            d_tolerance, outflow, idx_nan, flag_critical, flag_subgrid, nc, nf,
+                            River_Width, River_Depth, Qc_prev, Qf_prev, Qci_prev, Qfi_prev,
+                            C_a_prev, Subgrid_Properties, flag_overbanks, flag_inflow,
+                            SubgridTables):
+    """Local_Inertial_Model_D4.m:1-5 on the gfx950 kernel.
+
+    Returns (qout_left, qout_right, qout_up, qout_down, outlet_flow, d_t,
+    I_tot_end_cell, outflow, Hf, Qc, Qf, Qci, Qfi, C_a).  The dtype of z
+    selects the precision (float64 = MATLAB double, float32 = flag_single).
+    Subgrid, reservoir and river-channel branches raise HP2DError
+    (status HP2D_E_UNSUPPORTED), as the build covers the coarse-grid path.
+    """
+    if flag_reservoir == 1 or flag_subgrid == 1:
+        raise _lib.HP2DError(abi.HP2D_E_UNSUPPORTED,
+                             "reservoir / subgrid branches are outside this build")
+    if not _empty_or(River_Width) and np.any(np.asarray(River_Width) > 0):
+        outlet_mask = np.asarray(outlet_index, dtype=bool)
+        rw = np.array(River_Width, dtype=float, copy=True)
+        rw[outlet_mask] = 0   # Local_Inertial_Model_D4.m:18
+        if np.any(rw > 0):
+            raise _lib.HP2DError(abi.HP2D_E_UNSUPPORTED, "river channels are outside this build")
+    single = np.asarray(z).dtype == np.float32
+    T = np.float32 if single else np.float64
+    z = np.asfortranarray(z, dtype=T)
+    ny, nx = z.shape
+    d_tot = np.asfortranarray(d_tot, dtype=T)
+    rc = np.asfortranarray(np.broadcast_to(np.asarray(roughness_cell, dtype=T), (ny, nx)))
+    rs = np.asfortranarray(np.broadcast_to(np.asarray(roughness_squared, dtype=T), (ny, nx)))
+    of = np.asfortranarray(np.asarray(outflow, dtype=T).reshape(ny, nx, -1, order="F"))
+    if of.shape[2] < 3:
+        of = np.asfortranarray(np.concatenate([of, np.zeros((ny, nx, 3 - of.shape[2]), T)], axis=2))
+    rows = np.ascontiguousarray(np.asarray(row_outlet, dtype=np.int32).ravel())
+    cols = np.ascontiguousarray(np.asarray(col_outlet, dtype=np.int32).ravel())
+    n_out = int(min(rows.size, cols.size))
+    outs = [np.zeros((ny, nx), dtype=T, order="F") for _ in range(7)]
+    outflow_new = np.zeros((ny, nx, 3), dtype=T, order="F")
+    Hf = np.zeros((ny, nx, 3), dtype=T, order="F")
+    L = _lib.load()
+    prec = abi.HP2D_FP32 if single else abi.HP2D_FP64
+    h = _Handle.get(ny, nx, prec)
+    P = _fp if single else _dp
+    fn = L.hp2d_local_inertial_d4_f32 if single else L.hp2d_local_inertial_d4
+    cast = (lambda a: a.ctypes.data_as(P))
+    status = fn(h, int(flag_numerical_scheme), cast(z), cast(d_tot), cast(rc), cast(rs),
+                float(cell_area), float(time_step), float(Resolution), int(outlet_type),
+                float(slope_outlet), rows.ctypes.data_as(_i32p), cols.ctypes.data_as(_i32p),
+                n_out, cast(of), int(flag_critical), *[cast(o) for o in outs], cast(outflow_new),
+                cast(Hf))
+    _lib.check(status, handle=h)
+    qout_left, qout_right, qout_up, qout_down, outlet_flow, d_t, I_tot_end_cell = outs
+    C_a = T(Resolution) ** 2
+    return (qout_left, qout_right, qout_up, qout_down, outlet_flow, d_t, I_tot_end_cell,
+            outflow_new, Hf, 0, 0, 0, 0, C_a)

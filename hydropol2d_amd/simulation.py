@@ -1,0 +1,103 @@
+"""Device-resident HydroPol2D step loop (the HydroPol2D_Main_While mirror).
+
+`Simulation(spec)` uploads a RunSpec once; `run()` advances the loop with no
+per-step host round-trip (dt, rainfall integral, hydrograph, FP64 mass
+ledgers all live on the device); the getters copy results back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib, abi
+from .runspec import RunSpec
+
+_dp = C.POINTER(C.c_double)
+
+
+class Simulation:
+    def __init__(self, spec: RunSpec):
+        L = _lib.load()
+        _lib.require_device()
+        self.spec = spec
+        self.ny, self.nx = spec.shape
+        cfg, ins, keep = spec.to_c()
+        ptr = C.c_void_p()
+        status = L.hp2d_sim_create(C.byref(cfg), C.byref(ins), C.byref(ptr))
+        del keep
+        _lib.check(status)
+        self._s = ptr
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_s", None):
+            self._L.hp2d_sim_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, status):
+        _lib.check(status, sim=self._s)
+
+    def run(self, max_steps: int = 0) -> bool:
+        """Advance up to max_steps accepted steps (0 = to the end). Returns done."""
+        done = C.c_int32(0)
+        self._check(self._L.hp2d_sim_run(self._s, int(max_steps), C.byref(done)))
+        return bool(done.value)
+
+    def run_fixed(self, n_steps: int, graph_steps: int = 0):
+        self._check(self._L.hp2d_sim_run_fixed(self._s, int(n_steps), int(graph_steps)))
+
+    def kernel_time(self):
+        li, tot, n = C.c_double(), C.c_double(), C.c_int64()
+        self._check(self._L.hp2d_sim_kernel_time(self._s, C.byref(li), C.byref(tot), C.byref(n)))
+        return li.value, tot.value, n.value
+
+    def inject_failure(self, step_k: int):
+        self._check(self._L.hp2d_debug_inject_failure(self._s, int(step_k), 1))
+
+    def report(self) -> dict:
+        rep = abi.RunReport()
+        self._check(self._L.hp2d_sim_report(self._s, C.byref(rep)))
+        return rep.as_dict()
+
+    def state(self, infiltration: bool = False) -> dict:
+        ny, nx = self.ny, self.nx
+        d = np.zeros((ny, nx), order="F")
+        of = np.zeros((ny, nx, 3), order="F")
+        dm = np.zeros((ny, nx), order="F")
+        It = np.zeros((ny, nx), order="F") if infiltration else None
+        p = (lambda a: a.ctypes.data_as(_dp) if a is not None else _dp())
+        self._check(self._L.hp2d_sim_get_state(self._s, p(d), p(of), _dp(), p(It), p(dm), _dp(),
+                                               _dp()))
+        out = {"d_t": d, "outflow": of, "dmax": dm}
+        if infiltration:
+            out["I_t"] = It
+        return out
+
+    def hydrograph(self):
+        n = self.spec.n_hydro_records()
+        q, t, d = np.zeros(n), np.zeros(n), np.zeros(n)
+        got = C.c_int32()
+        self._check(self._L.hp2d_sim_get_hydrograph(self._s, q.ctypes.data_as(_dp),
+                                                    t.ctypes.data_as(_dp), d.ctypes.data_as(_dp),
+                                                    n, C.byref(got)))
+        return {"hydro_q": q, "hydro_t": t, "hydro_depth": d, "n_hydro": got.value}
+
+    def dt_history(self, n_max: int = 4096):
+        buf = np.zeros(n_max)
+        got = C.c_int32()
+        self._check(self._L.hp2d_sim_get_dt_history(self._s, buf.ctypes.data_as(_dp), n_max,
+                                                    C.byref(got)))
+        return buf[: got.value]

@@ -1,0 +1,145 @@
+"""Seeded synthetic inputs of the benchmark configurations.
+
+All generation happens in FP64 on the host; the arrays are column-major
+(ny x nx, MATLAB layout).  Each builder returns a RunSpec.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .runspec import GreenAmpt, RunSpec
+
+
+def tilted_plane_c1(ny=256, nx=256, dx=5.0, seed=1, precision="fp64", routing_time_min=60.0):
+    """Tilted plane: z = 0.01 x + 0.002 y + N(0, 0.02^2), n = 0.02, 50 mm/h for 30 min,
+    Green-Ampt Ks = 5 mm/h, psi = 110 mm, dtheta = 0.3, critical-depth outlet at the
+    lowest cell, alpha = 0.7, 1 h."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(nx)[None, :] * dx            # along columns
+    y = np.arange(ny)[:, None] * dx            # along rows
+    z = 0.01 * x + 0.002 * y + rng.normal(0.0, 0.02, (ny, nx))
+    z = np.asfortranarray(z)
+    outlet = np.zeros((ny, nx), dtype=np.uint8, order="F")
+    # HydroPol2D_preprocessing.m:1708-1711: outlet = cell(s) with the minimum elevation
+    outlet[np.unravel_index(np.argmin(z), z.shape)] = 1
+    ga = GreenAmpt(ksat_mmh=np.array([5.0]), psi_mm=np.array([110.0]), dtheta=np.array([0.3]),
+                   theta_sat_minus_r=np.array([0.4]), I_min_mm=1e-3)
+    return RunSpec(
+        elevation=z, roughness=np.full((ny, nx), 0.02, order="F"), resolution=dx,
+        outlet_mask=outlet, rain_time_min=np.array([30.0, 60.0]),
+        rain_intensity_mmh=np.array([50.0, 0.0]), precision=precision, outlet_type=2,
+        routing_time_min=routing_time_min, min_time_step_s=0.01, max_time_step_s=10.0,
+        time_step_increments_s=0.001, time_step_change_s=1e-4, alfa_min=0.7, alfa_max=0.7,
+        flag_timestep=1, flag_adaptive_timestepping=1, depth_tolerance_mm=1.0,
+        record_time_hydrographs_min=1.0, infiltration=ga)
+
+
+def spectral_dem(ny, nx, dx, hurst=0.8, relief=200.0, seed=2):
+    """Fractal (fBm) surface by spectral synthesis, scaled to `relief` metres."""
+    rng = np.random.default_rng(seed)
+    ky = np.fft.fftfreq(ny, d=dx)[:, None]
+    kx = np.fft.rfftfreq(nx, d=dx)[None, :]
+    k = np.sqrt(kx ** 2 + ky ** 2)
+    k[0, 0] = np.inf
+    amp = k ** (-(hurst + 1.0))
+    phase = rng.uniform(0, 2 * np.pi, amp.shape)
+    spec = amp * (np.cos(phase) + 1j * np.sin(phase))
+    z = np.fft.irfft2(spec, s=(ny, nx))
+    z = (z - z.min()) / (z.max() - z.min()) * relief
+    return z
+
+
+def voronoi_classes(ny, nx, n_classes, n_seeds, seed):
+    rng = np.random.default_rng(seed)
+    pts = np.stack([rng.uniform(0, ny, n_seeds), rng.uniform(0, nx, n_seeds)], axis=1)
+    cls = rng.integers(0, n_classes, n_seeds)
+    out = np.empty((ny, nx), dtype=np.uint8)
+    # nearest seed on a coarse lattice, upsampled (fast, deterministic)
+    step = max(1, min(ny, nx) // 512)
+    yy = np.arange(0, ny, step)
+    xx = np.arange(0, nx, step)
+    best = np.full((yy.size, xx.size), np.inf)
+    lab = np.zeros((yy.size, xx.size), dtype=np.uint8)
+    for (py, px), c in zip(pts, cls):
+        d = (yy[:, None] - py) ** 2 + (xx[None, :] - px) ** 2
+        m = d < best
+        best[m] = d[m]
+        lab[m] = c
+    out[:] = np.repeat(np.repeat(lab, step, axis=0), step, axis=1)[:ny, :nx]
+    return out
+
+
+def gaussian_random_field(ny, nx, dx, mean, cv, corr_m, seed):
+    rng = np.random.default_rng(seed)
+    ky = np.fft.fftfreq(ny, d=dx)[:, None]
+    kx = np.fft.rfftfreq(nx, d=dx)[None, :]
+    k2 = kx ** 2 + ky ** 2
+    filt = np.exp(-2 * (np.pi ** 2) * k2 * corr_m ** 2)
+    noise = np.fft.rfft2(rng.normal(size=(ny, nx)))
+    f = np.fft.irfft2(noise * np.sqrt(filt), s=(ny, nx))
+    f = (f - f.mean()) / (f.std() + 1e-300)
+    field = mean * (1.0 + cv * f)
+    return np.maximum(field, 0.0)
+
+
+def full_physics_c2(ny=4096, nx=4096, dx=2.0, precision="fp32", routing_time_min=180.0,
+                    fill_sinks=None):
+    """Fractal DEM (Hurst 0.8, 200 m relief), 8 LULC classes (n in [0.015, 0.15]), 5 soil
+    classes (Ks in [1, 50] mm/h), GRF rainfall pattern (mean 40 mm/h, CV 0.5, 5 km
+    correlation) on a 2 h triangular hyetograph, perimeter outlets at the lowest edge."""
+    z = spectral_dem(ny, nx, dx, 0.8, 200.0, seed=2)
+    # tilt towards the lowest column so that the catchment drains to one edge
+    z = z + np.linspace(50.0, 0.0, nx)[None, :]
+    if fill_sinks is not None:
+        z = fill_sinks(z)
+    lulc = voronoi_classes(ny, nx, 8, 64, seed=3)
+    n_cls = np.linspace(0.015, 0.15, 8)
+    rough = n_cls[lulc]
+    soil = voronoi_classes(ny, nx, 5, 48, seed=4)
+    pattern = gaussian_random_field(ny, nx, dx, 1.0, 0.5, 5000.0, seed=5)
+    # 2 h triangular hyetograph peaking at 40/mean... mean intensity 40 mm/h
+    t_end = np.arange(10.0, 130.0, 10.0)        # end-stamped 10-min blocks over 2 h
+    mid = t_end - 5.0
+    tri = np.where(mid <= 60.0, mid / 60.0, (120.0 - mid) / 60.0) * 80.0   # mean 40 mm/h
+    t_end = np.append(t_end, 1e9)
+    tri = np.append(tri, 0.0)
+    outlet = np.zeros((ny, nx), dtype=np.uint8, order="F")
+    outlet[:, nx - 1] = 1
+    ga = GreenAmpt(ksat_mmh=np.linspace(1.0, 50.0, 5), psi_mm=np.linspace(200.0, 50.0, 5),
+                   dtheta=np.full(5, 0.3), theta_sat_minus_r=np.full(5, 0.4),
+                   soil_class=np.asfortranarray(soil), I_min_mm=1e-3)
+    return RunSpec(
+        elevation=np.asfortranarray(z), roughness=np.asfortranarray(rough), resolution=dx,
+        outlet_mask=outlet, rain_time_min=t_end, rain_intensity_mmh=tri,
+        rain_pattern=np.asfortranarray(pattern), precision=precision, outlet_type=2,
+        routing_time_min=routing_time_min, min_time_step_s=0.01, max_time_step_s=10.0,
+        time_step_increments_s=0.001, time_step_change_s=1e-4, alfa_min=0.7, alfa_max=0.7,
+        record_time_hydrographs_min=1.0, infiltration=ga)
+
+
+def dam_break_c3(ny=2048, nx=2048, dx=1.0, precision="fp32", routing_time_min=30.0):
+    """V valley (5 % side slope, 0.5 % longitudinal slope along the rows), 10 m reservoir
+    upstream of row 256 behind a removed 50 m breach, n = 0.033, dry bed downstream."""
+    i = np.arange(ny)[:, None] * dx
+    j = np.arange(nx)[None, :] * dx
+    centre = (nx - 1) * dx / 2
+    z = 0.05 * np.abs(j - centre) + 0.005 * (ny * dx - i)
+    wall = 256
+    d0 = np.zeros((ny, nx))
+    pool = z[:wall, :]
+    level = z[wall, int(nx // 2)] + 10.0
+    d0[:wall, :] = np.maximum(level - pool, 0.0) * 1000.0
+    # dam across row `wall` except a 50 m breach around the valley axis
+    half = int(round(25.0 / dx))
+    jc = nx // 2
+    z[wall, :] = level + 1.0
+    z[wall, jc - half:jc + half] = z[wall + 1, jc - half:jc + half]
+    d0[wall, :] = 0.0
+    outlet = np.zeros((ny, nx), dtype=np.uint8, order="F")
+    outlet[ny - 1, :] = 1
+    return RunSpec(
+        elevation=np.asfortranarray(z), roughness=np.full((ny, nx), 0.033, order="F"),
+        resolution=dx, outlet_mask=outlet, d0=np.asfortranarray(d0), precision=precision,
+        outlet_type=2, routing_time_min=routing_time_min, min_time_step_s=0.001,
+        max_time_step_s=10.0, time_step_increments_s=0.001, time_step_change_s=1e-4,
+        alfa_min=0.7, alfa_max=0.7, record_time_hydrographs_min=1.0)

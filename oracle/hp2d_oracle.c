@@ -219,7 +219,10 @@ int32_t oracle_li_d4(int32_t ny, int32_t nx, int32_t scheme, const double *z,
         outflow[2 * n + c] = 0.0;
         Hf[2 * n + c] = 0.0;
     }
+    /* vectorised MATLAB semantics: every outlet value is computed from the
+     * same pre-outlet d_t, then assigned (duplicates collapse). */
     const double dt_h = time_step_min / 60.0;
+    double *qo = zalloc(n_outlet > 0 ? n_outlet : 1);
     for (int32_t k = 0; k < n_outlet; ++k) {
         int64_t c = outlet_idx[k];
         double h_out = fmax(d_t[c], 0.0) / 1000.0;
@@ -237,10 +240,18 @@ int32_t oracle_li_d4(int32_t ny, int32_t nx, int32_t scheme, const double *z,
         double qmax = fmax(d_t[c] - 1000.0 * 0.0, 0.0) / dt_h;
         q_out = fmin(q_out, qmax);
         if (!isfinite(q_out)) q_out = 0.0;
-        if (outlet_flow) outlet_flow[c] = q_out;
-        outflow[2 * n + c] = q_out;
-        d_t[c] = d_t[c] - q_out * dt_h;
+        qo[k] = q_out;
     }
+    for (int32_t k = 0; k < n_outlet; ++k) {
+        int64_t c = outlet_idx[k];
+        if (outlet_flow) outlet_flow[c] = qo[k];
+        outflow[2 * n + c] = qo[k];
+    }
+    double *dnew = zalloc(n_outlet > 0 ? n_outlet : 1);
+    for (int32_t k = 0; k < n_outlet; ++k) dnew[k] = d_t[outlet_idx[k]] - qo[k] * dt_h;
+    for (int32_t k = 0; k < n_outlet; ++k) d_t[outlet_idx[k]] = dnew[k];
+    free(qo);
+    free(dnew);
     /* :513-514 I_tot_end_cell = abs(sum(outflow,3))*dt/1000*1/3600*Resolution^2 */
     if (I_tot_end_cell)
         for (int64_t c = 0; c < n; ++c)

@@ -303,3 +303,15 @@ def test_kinematic_wave_fixture_plane_on_oracle():
             q_model[it] = float(np.sum(oflow)) / 1000 / 3600 * dx * dx
     qm = np.array([q_model[int(t)] for t in t_ref])
     assert li_cases.nse(qm[1:], q_ref[1:]) > 0.9
+
+
+def test_duplicate_outlets_collapse_like_vectorised_assignment():
+    # d_t(outlet_sub) = d_t(outlet_sub) - q_out*dt_h with a repeated index
+    # subtracts once (Local_Inertial_Model_D4.m:496)
+    z = np.zeros((1, 3))
+    d = np.array([[0.0, 0.0, 120.0]])
+    n = np.full((1, 3), 0.04)
+    once = _li(z, d, n, 0.3, 2.0, rows=[1], cols=[3])
+    twice = _li(z, d, n, 0.3, 2.0, rows=[1, 1], cols=[3, 3])
+    np.testing.assert_array_equal(once["d_t"], twice["d_t"])
+    np.testing.assert_array_equal(once["outlet_flow"], twice["outlet_flow"])
