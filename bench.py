@@ -68,7 +68,8 @@ def main():
     sim = Simulation(spec)
     # warm-up (also lets water start moving: rainfall begins at t = 0)
     sim.run_fixed(args.warmup, graph_steps=min(args.graph_steps, max(args.warmup, 1)))
-    print("[bench] warm-up done", file=sys.stderr, flush=True)
+    tune = sim.autotune()
+    print(f"[bench] warm-up done, launch shape {tune}", file=sys.stderr, flush=True)
     # timed region: K whole steps through captured graphs
     t1 = time.perf_counter()
     sim.run_fixed(args.steps, graph_steps=args.graph_steps)
@@ -120,6 +121,7 @@ def main():
                                "on a 2 h triangular hyetograph, critical-depth edge outlets; "
                                "fused rain+GA+local-inertial step + dt control",
                    "cells": cells, "precision": args.precision,
+                   "launch": {"cols_per_wave": tune["tj"], "waves_per_block": tune["waves_per_block"]},
                    "bytes_per_cell_update": rd + wr},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -5,7 +5,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -332,7 +334,8 @@ struct hp2d_sim {
     long long *out_idx = nullptr;
     void *out_n = nullptr, *out_q = nullptr, *out_d = nullptr;
     int n_outlet = 0;
-    WavePartial *partials = nullptr;
+    double *partials = nullptr;
+    long long part_cap = 0;
     int n_partials = 0;
     SimState *st = nullptr;
     double *rain_t = nullptr, *rain_v = nullptr;
@@ -407,6 +410,7 @@ template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
     a.out_q = (T *)s->out_q;
     a.out_d = (T *)s->out_d;
     a.partials = s->partials;
+    a.part_cap = s->part_cap;
     return a;
 }
 
@@ -416,6 +420,7 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
     const hp2d_run_config &c = s->cfg;
     f.st = s->st;
     f.partials = s->partials;
+    f.part_cap = s->part_cap;
     // only the waves of the current launch shape write partials
     f.n_partials = ((s->ny + kRowsOut - 1) / kRowsOut) * ((s->nx + s->tj - 1) / s->tj);
     f.routing_time = c.routing_time_min;
@@ -626,7 +631,8 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     }
     const int n_slabs = (ny + kRowsOut - 1) / kRowsOut;
     s->n_partials = n_slabs * ((s->nx + 16 - 1) / 16);  // capacity for the smallest tj
-    A_((void **)&s->partials, (size_t)s->n_partials * sizeof(WavePartial));
+    s->part_cap = s->n_partials;
+    A_((void **)&s->partials, (size_t)s->n_partials * kPartCount * sizeof(double));
     A_((void **)&s->st, sizeof(SimState));
     s->n_rain = in->n_rain;
     A_((void **)&s->rain_t, std::max(1, in->n_rain) * sizeof(double));
@@ -1045,3 +1051,105 @@ extern "C" hp2d_status hp2d_inertial_solver(hp2d_handle *h, int32_t flag_numeric
 // Entry points whose kernels live in their own translation units are
 // declared in include/hp2d.h and defined there (hp2d_ca.hip, hp2d_wq.hip,
 // hp2d_gw.hip).
+
+// ---------------------------------------------------------------------------
+// Launch-shape autotuner
+// ---------------------------------------------------------------------------
+namespace {
+struct TuneKey {
+    int dev, ny, nx, elem;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(dev, ny, nx, elem) < std::tie(o.dev, o.ny, o.nx, o.elem);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tune_cache;
+}  // namespace
+
+extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t *best_wpb,
+                                         double *best_ms) {
+    if (!s) return HP2D_E_ARG;
+    int dev = 0;
+    STRY(s, hipGetDevice(&dev));
+    const TuneKey key{dev, s->ny, s->nx, (int)s->elem};
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            s->tj = it->second.first;
+            s->wpb = it->second.second;
+            if (s->graph) {
+                hipGraphExecDestroy(s->graph);
+                s->graph = nullptr;
+            }
+            if (best_tj) *best_tj = s->tj;
+            if (best_wpb) *best_wpb = s->wpb;
+            if (best_ms) *best_ms = -1.0;
+            return HP2D_OK;
+        }
+    }
+    SimState st;
+    hp2d_status rc = read_state(s, st);
+    if (rc) return rc;
+    if (st.done) {
+        set_err(s, "autotune needs a sim that has not finished");
+        return HP2D_E_STATE;
+    }
+    // neutralise the step: the kernel treats k == inject_k as a rejected step
+    // (writes only scratch buffers, no ledgers, no dmax) and no finalize runs.
+    const long long saved_inject = st.inject_k;
+    const long long saved_stop = st.stop_accepted;
+    const int saved_paused = st.paused;
+    st.inject_k = st.k;
+    st.stop_accepted = -1;
+    st.paused = 0;
+    if ((rc = write_state(s, st))) return rc;
+    hipEvent_t e0, e1;
+    STRY(s, hipEventCreate(&e0));
+    STRY(s, hipEventCreate(&e1));
+    const int tjs[] = {16, 32, 64, 128};
+    const int wpbs[] = {1, 2, 4};
+    double best = 1e30;
+    int btj = s->tj, bwpb = s->wpb;
+    for (int tj : tjs) {
+        if (tj > 16 && (s->nx + tj - 1) / tj < 2) continue;
+        for (int wpb : wpbs) {
+            s->tj = tj;
+            s->wpb = wpb;
+            for (int i = 0; i < 2; ++i) STRY(s, step_launch(s, false));
+            STRY(s, hipEventRecord(e0, s->stream));
+            const int reps = 5;
+            for (int i = 0; i < reps; ++i) STRY(s, step_launch(s, false));
+            STRY(s, hipEventRecord(e1, s->stream));
+            STRY(s, hipEventSynchronize(e1));
+            float ms = 0;
+            STRY(s, hipEventElapsedTime(&ms, e0, e1));
+            const double per = ms / reps;
+            if (per < best) {
+                best = per;
+                btj = tj;
+                bwpb = wpb;
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    s->tj = btj;
+    s->wpb = bwpb;
+    if (s->graph) {
+        hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    st.inject_k = saved_inject;
+    st.stop_accepted = saved_stop;
+    st.paused = saved_paused;
+    if ((rc = write_state(s, st))) return rc;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune_cache[key] = {btj, bwpb};
+    }
+    if (best_tj) *best_tj = btj;
+    if (best_wpb) *best_wpb = bwpb;
+    if (best_ms) *best_ms = best;
+    return HP2D_OK;
+}

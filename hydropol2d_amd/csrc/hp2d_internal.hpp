@@ -14,22 +14,22 @@ constexpr int kHalo = 2;                   // radius of the step stencil (donor 
 constexpr int kRowsOut = kWave - 2 * kHalo; // rows finalised per wave slab
 
 // ----------------------------------------------------------------------------
-// Per-wave partial results of one step, reduced by the finalize kernel in a
-// fixed order (deterministic).  All FP64.
+// Per-wave partial results of one step (structure of arrays, FP64), reduced
+// by the finalize kernel in a fixed order (deterministic).  Field f of wave w
+// lives at partials[f * cap + w].
 // ----------------------------------------------------------------------------
-struct WavePartial {
-    double storage;     // sum A*d_t/1000 over valid cells      [m3]
-    double uz;          // sum A*I_t/1000 (Green-Ampt)          [m3]
-    double rain;        // rain volume applied this step        [m3]
-    double infil;       // infiltrated volume this step         [m3]
-    double vmax;        // max total cell velocity              [m/s]  (NaN = none)
-    double dtw;         // min Courant wave candidate           [s]    (NaN = none)
-    double raster;      // max velocity_raster                  [m/s]
-    double dtot_max;    // max d_tot/1000 (rollback celerity)   [m]
-    double dprev_max;   // max d_p/1000                         [m]
-    double neg;         // 1 if a depth < -1e-8 mm was seen before clean-up
-    double washed;      // wash-off mass this step [kg] (WQ)
-    double pad;
+enum PartField {
+    kPartStorage = 0,  // sum A*d_t/1000 over valid cells      [m3]
+    kPartUz,           // sum A*I_t/1000 (Green-Ampt)          [m3]
+    kPartRain,         // rain volume applied this step        [m3]
+    kPartInfil,        // infiltrated volume this step         [m3]
+    kPartVmax,         // max total cell velocity              [m/s]  (NaN = none)
+    kPartDtw,          // min Courant wave candidate           [s]    (NaN = none)
+    kPartRaster,       // max velocity_raster                  [m/s]
+    kPartDtotMax,      // max d_tot/1000 (rollback celerity)   [m]
+    kPartDprevMax,     // max d_p/1000                         [m]
+    kPartNeg,          // 1 if a depth < -1e-8 mm was seen before clean-up
+    kPartCount
 };
 
 // ----------------------------------------------------------------------------

@@ -19,15 +19,32 @@
 
 namespace hp2d {
 
+// Face slope and flow depth.  FP64 keeps the reference's operation order
+// (y = z + h, Sx/Sy :123-127, Hf :133-134).  FP32 evaluates the same
+// expressions with the elevation difference taken first, so the flow depth of
+// a shallow face keeps the precision of the cell depths instead of that of
+// the elevations (both are the same real-number expression).
+__device__ __forceinline__ void face_geom(double zA, double zB, double hA, double hB, double R,
+                                          double &S, double &Hf) {
+    const double yA = zA + hA, yB = zB + hB;
+    S = (yB - yA) / R;
+    Hf = fmax(fmax(yA, yB) - fmax(zA, zB), 0.0);
+}
+__device__ __forceinline__ void face_geom(float zA, float zB, float hA, float hB, float R,
+                                          float &S, float &Hf) {
+    const float zm = fmaxf(zA, zB);
+    S = ((zB - zA) + (hB - hA)) / R;
+    Hf = fmaxf(fmaxf(hA + (zA - zm), hB + (zB - zm)), 0.0f);
+}
+
 template <typename T>
-__device__ __forceinline__ T face_flux(T qp_mmh, T yA, T yB, T zA, T zB, T n2, T R, T R2,
-                                       T dt, int crit, T &Hf_out) {
+__device__ __forceinline__ T face_flux(T qp_mmh, T zA, T zB, T hA, T hB, T n2, T R, T R2, T dt,
+                                       int crit, T &Hf_out) {
     // A is the cell that stores the face, B the other side; S = (yB - yA)/R
     // reproduces Sx (:123) for x faces and Sy (:127) for y faces.
     const T g = T(9.81);
-    T S = (yB - yA) / R;
-    T zf = dmax(zA, zB);
-    T Hf = dmax(dmax(yA, yB) - zf, T(0));
+    T S, Hf;
+    face_geom(zA, zB, hA, hB, R, S, Hf);
     T q_p = qp_mmh / T(1000) / T(3600) * R2 / R;
     T num = q_p - g * Hf * dt * S;
     T den = T(1) + g * dt * n2 * fabs(q_p) / pow73(Hf);
@@ -41,6 +58,39 @@ __device__ __forceinline__ T face_flux(T qp_mmh, T yA, T yB, T zA, T zB, T n2, T
     q = dmin(dmax(q, -thr), thr);
     Hf_out = Hf;
     return q * R;  // outflow_rate [m3/s]
+}
+
+// Per-column loads of one lane, fetched one column ahead of the compute.
+template <typename T> struct ColLoad {
+    T z, d, n2, qx, qy, pat, I, Tc;
+    int soil;
+};
+
+template <typename T, int MODE>
+__device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d_src,
+                                                  const T *qx_src, const T *qy_src,
+                                                  const T *I_src, const T *Tc_src, bool in,
+                                                  long long id) {
+    ColLoad<T> v;
+    v.z = v.d = v.n2 = v.qx = v.qy = v.I = v.Tc = T(0);
+    v.pat = T(1);
+    v.soil = 0;
+    if (in) {
+        v.z = a.z[id];
+        v.d = d_src[id];
+        v.n2 = a.n2s[id];
+        v.qx = qx_src[id];
+        v.qy = qy_src[id];
+        if (MODE == kModeSim) {
+            if (a.rain_pattern) v.pat = a.rain_pattern[id];
+            if (a.ga) {
+                v.I = I_src[id];
+                v.Tc = Tc_src[id];
+                if (a.soil_class) v.soil = a.soil_class[id];
+            }
+        }
+    }
+    return v;
 }
 
 template <typename T>
@@ -104,7 +154,7 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
     const T avail_den = dmax(dt_s, eps_of<T>());
 
     // pipeline registers (suffix = column lag behind the load column c)
-    T y1 = 0, z1 = 0, n21 = 0, qxp1 = 0, d1 = 0, n2s1 = 0;   // column c-1
+    T h1 = 0, z1 = 0, n21 = 0, qxp1 = 0, d1 = 0, n2s1 = 0;   // column c-1
     T rawQx2 = 0, rawQy1 = 0, rawQy2 = 0, Hfx2 = 0, Hfy1 = 0, Hfy2 = 0;
     T d2 = 0, scale2 = 0, finQx3 = 0, ox3 = 0, n2s2 = 0;
 
@@ -114,32 +164,33 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
     int neg = 0;
     const T alfaR = a.alfa_min * R;
 
+    ColLoad<T> nxt = load_column<T, MODE>(a, d_src, qx_src, qy_src, I_src, Tc_src,
+                                          row_in && c0 - 2 >= 0 && c0 - 2 < nx,
+                                          (long long)r + (long long)(c0 - 2) * ny);
     for (int c = c0 - 2; c <= c1 + 1; ++c) {
         const bool col_in = (c >= 0) && (c < nx);
         const bool cell_in = row_in && col_in;
         const long long id = (long long)r + (long long)c * ny;
-        T zc = 0, dc = 0, n2c = 0, qxc = 0, qyc = 0, dprev = 0;
-        if (cell_in) {
-            zc = a.z[id];
-            dc = d_src[id];
-            n2c = a.n2s[id];
-            qxc = qx_src[id];
-            qyc = qy_src[id];
-        }
+        const ColLoad<T> cur = nxt;
+        // issue the next column's loads before computing this one
+        if (c + 1 <= c1 + 1)
+            nxt = load_column<T, MODE>(a, d_src, qx_src, qy_src, I_src, Tc_src,
+                                       row_in && c + 1 >= 0 && c + 1 < nx, id + ny);
+        T zc = cur.z, dc = cur.d, n2c = cur.n2, qxc = cur.qx, qyc = cur.qy, dprev = 0;
         // ---- hydrology prologue: Hydrological_Model.m:15-55 (sim mode) ----
         if (MODE == kModeSim && cell_in) {
             dprev = dc;
             const bool nan_cell = isnan(zc);
-            T eff = a.rain_pattern ? T(a.rain_pattern[id]) * rain : rain;
+            T eff = a.rain_pattern ? cur.pat * rain : rain;
             if (nan_cell) eff = T(NAN);
             T dd = dc + eff;
             const bool own = row_own && c >= c0 && c < c1;
             if (own && !nan_cell && !fail_step) s_rain += (double)eff / 1000.0 * (double)A;
             if (a.ga) {
-                const GaClass g = a.ga[a.soil_class ? a.soil_class[id] : 0];
+                const GaClass g = a.ga[cur.soil];
                 const T ksat = T(g.ksat), psi = T(g.psi), dth = T(g.dtheta), Imin = a.I_min;
-                const T Ip = I_src[id];
-                T Tn = Tc_src[id];
+                const T Ip = cur.I;
+                T Tn = cur.Tc;
                 const T i_a = dd / dmax(dt_h, T(1e-12));
                 T C = ksat * (T(1) + ((dd + psi) * dth) / Ip);
                 T f, In;
@@ -179,18 +230,18 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
                 dpmax = dmax(dpmax, dprev / T(1000));
             }
         }
-        const T yc = zc + dmax(dc / T(1000), T(0));
+        const T hc = dmax(dc / T(1000), T(0));  // depth_cell (:31)
 
         // ---- raw y-face flux at (r,c): face between rows r-1 and r ----
-        const T y_up = __shfl_up(yc, 1), z_up = __shfl_up(zc, 1);
+        const T h_up = __shfl_up(hc, 1), z_up = __shfl_up(zc, 1);
         T rawQyc = 0, Hfyc = 0;
         if (cell_in && r >= 1)
-            rawQyc = face_flux<T>(qyc, yc, y_up, zc, z_up, fabs(n2c), R, R2, dt_s, a.flag_critical, Hfyc);
+            rawQyc = face_flux<T>(qyc, zc, z_up, hc, h_up, fabs(n2c), R, R2, dt_s, a.flag_critical, Hfyc);
 
         // ---- raw x-face flux at (r,c-1): face between columns c-1 and c ----
         T rawQx1 = 0, Hfx1 = 0;
         if (row_in && c - 1 >= 0 && c - 1 < nx - 1)
-            rawQx1 = face_flux<T>(qxp1, y1, yc, z1, zc, fabs(n21), R, R2, dt_s, a.flag_critical, Hfx1);
+            rawQx1 = face_flux<T>(qxp1, z1, zc, h1, hc, fabs(n21), R, R2, dt_s, a.flag_critical, Hfx1);
 
         // ---- donor scale of cell (r,c-1): Local_Inertial_Model_D4.m:814-829 ----
         const T qy1_dn = __shfl_down(rawQy1, 1);
@@ -324,7 +375,7 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
         scale2 = s1;
         d2 = d1;
         d1 = dc;
-        y1 = yc;
+        h1 = hc;
         z1 = zc;
         n21 = n2c;
         n2s2 = n2s1;
@@ -340,20 +391,19 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
         const double rn_sum = wave_sum(s_rain), in_sum = wave_sum(s_inf);
         const int any_neg = __any(neg);
         if (lane == 0) {
-            WavePartial p;
-            p.storage = st_sum;
-            p.uz = uz_sum;
-            p.rain = rn_sum;
-            p.infil = in_sum;
-            p.vmax = (double)wv_vmax;
-            p.dtw = (double)wv_dtw;
-            p.raster = (double)wv_r;
-            p.dtot_max = (double)wv_dt;
-            p.dprev_max = (double)wv_dp;
-            p.neg = any_neg ? 1.0 : 0.0;
-            p.washed = 0.0;
-            p.pad = 0.0;
-            a.partials[(long long)blockIdx.y * a.n_slabs + slab] = p;
+            const long long idx = (long long)blockIdx.y * a.n_slabs + slab;
+            double *P = a.partials;
+            const long long cap = a.part_cap;
+            P[kPartStorage * cap + idx] = st_sum;
+            P[kPartUz * cap + idx] = uz_sum;
+            P[kPartRain * cap + idx] = rn_sum;
+            P[kPartInfil * cap + idx] = in_sum;
+            P[kPartVmax * cap + idx] = (double)wv_vmax;
+            P[kPartDtw * cap + idx] = (double)wv_dtw;
+            P[kPartRaster * cap + idx] = (double)wv_r;
+            P[kPartDtotMax * cap + idx] = (double)wv_dt;
+            P[kPartDprevMax * cap + idx] = (double)wv_dp;
+            P[kPartNeg * cap + idx] = any_neg ? 1.0 : 0.0;
         }
     }
 }

@@ -42,8 +42,9 @@ template <typename T> struct LiArgs {
     T *out_q, *out_d;
     // drop-in optional outputs
     T *q3, *hfx, *hfy, *hf3, *outlet_flow, *qleft, *qright, *qup, *qdown, *itot;
-    // sim partials
-    WavePartial *partials;
+    // sim partials (structure of arrays, kPartCount fields of part_cap entries)
+    double *partials;
+    long long part_cap;
 };
 
 template <typename T> hipError_t launch_li_step(const LiArgs<T> &a, int mode, hipStream_t s);

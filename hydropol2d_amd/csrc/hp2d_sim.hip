@@ -16,52 +16,55 @@
 
 namespace hp2d {
 
-constexpr int kFinThreads = 256;
+constexpr int kFinThreads = 1024;
+constexpr int kFinWaves = kFinThreads / kWave;
 
 template <typename T>
 __global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
     SimState *st = a.st;
     if (st->done || st->paused) return;
-    __shared__ double sh[12][kFinThreads];
-    const int t = threadIdx.x;
+    __shared__ double sh[kFinWaves][12];
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    const double *P = a.partials;
+    const long long cap = a.part_cap;
     double storage = 0, uz = 0, rain = 0, infil = 0, vmax = NAN, dtw = NAN, raster = NAN;
     double dtm = NAN, dpm = NAN, neg = 0, qsum = 0, dsum = 0;
     for (int i = t; i < a.n_partials; i += kFinThreads) {
-        const WavePartial p = a.partials[i];
-        storage += p.storage;
-        uz += p.uz;
-        rain += p.rain;
-        infil += p.infil;
-        vmax = fmax(vmax, p.vmax);
-        dtw = fmin(dtw, p.dtw);
-        raster = fmax(raster, p.raster);
-        dtm = fmax(dtm, p.dtot_max);
-        dpm = fmax(dpm, p.dprev_max);
-        neg = fmax(neg, p.neg);
+        storage += P[kPartStorage * cap + i];
+        uz += P[kPartUz * cap + i];
+        rain += P[kPartRain * cap + i];
+        infil += P[kPartInfil * cap + i];
+        vmax = fmax(vmax, P[kPartVmax * cap + i]);
+        dtw = fmin(dtw, P[kPartDtw * cap + i]);
+        raster = fmax(raster, P[kPartRaster * cap + i]);
+        dtm = fmax(dtm, P[kPartDtotMax * cap + i]);
+        dpm = fmax(dpm, P[kPartDprevMax * cap + i]);
+        neg = fmax(neg, P[kPartNeg * cap + i]);
     }
     for (int i = t; i < a.n_outlet; i += kFinThreads) {
         qsum += (double)a.out_q[i] * a.A;
         dsum += (double)a.out_d[i];
     }
-    sh[0][t] = storage; sh[1][t] = uz; sh[2][t] = rain; sh[3][t] = infil;
-    sh[4][t] = vmax; sh[5][t] = dtw; sh[6][t] = raster; sh[7][t] = dtm;
-    sh[8][t] = dpm; sh[9][t] = neg; sh[10][t] = qsum; sh[11][t] = dsum;
-    __syncthreads();
-    for (int w = kFinThreads / 2; w > 0; w >>= 1) {
-        if (t < w) {
-            sh[0][t] += sh[0][t + w]; sh[1][t] += sh[1][t + w];
-            sh[2][t] += sh[2][t + w]; sh[3][t] += sh[3][t + w];
-            sh[4][t] = fmax(sh[4][t], sh[4][t + w]); sh[5][t] = fmin(sh[5][t], sh[5][t + w]);
-            sh[6][t] = fmax(sh[6][t], sh[6][t + w]); sh[7][t] = fmax(sh[7][t], sh[7][t + w]);
-            sh[8][t] = fmax(sh[8][t], sh[8][t + w]); sh[9][t] = fmax(sh[9][t], sh[9][t + w]);
-            sh[10][t] += sh[10][t + w]; sh[11][t] += sh[11][t + w];
-        }
-        __syncthreads();
+    storage = wave_sum(storage); uz = wave_sum(uz); rain = wave_sum(rain);
+    infil = wave_sum(infil); qsum = wave_sum(qsum); dsum = wave_sum(dsum);
+    vmax = wave_fmax(vmax); dtw = wave_fmin(dtw); raster = wave_fmax(raster);
+    dtm = wave_fmax(dtm); dpm = wave_fmax(dpm); neg = wave_fmax(neg);
+    if (lane == 0) {
+        sh[w][0] = storage; sh[w][1] = uz; sh[w][2] = rain; sh[w][3] = infil;
+        sh[w][4] = vmax; sh[w][5] = dtw; sh[w][6] = raster; sh[w][7] = dtm;
+        sh[w][8] = dpm; sh[w][9] = neg; sh[w][10] = qsum; sh[w][11] = dsum;
     }
+    __syncthreads();
     if (t != 0) return;
-    storage = sh[0][0]; uz = sh[1][0]; rain = sh[2][0]; infil = sh[3][0];
-    vmax = sh[4][0]; dtw = sh[5][0]; raster = sh[6][0]; dtm = sh[7][0]; dpm = sh[8][0];
-    neg = sh[9][0]; qsum = sh[10][0]; dsum = sh[11][0];
+    storage = uz = rain = infil = qsum = dsum = 0.0;
+    vmax = dtw = raster = dtm = dpm = NAN;
+    neg = 0.0;
+    for (int v = 0; v < kFinWaves; ++v) {  // fixed order: deterministic
+        storage += sh[v][0]; uz += sh[v][1]; rain += sh[v][2]; infil += sh[v][3];
+        vmax = fmax(vmax, sh[v][4]); dtw = fmin(dtw, sh[v][5]); raster = fmax(raster, sh[v][6]);
+        dtm = fmax(dtm, sh[v][7]); dpm = fmax(dpm, sh[v][8]); neg = fmax(neg, sh[v][9]);
+        qsum += sh[v][10]; dsum += sh[v][11];
+    }
 
     const long long k = st->k;
     const bool failed = (st->inject_k == k);

@@ -7,7 +7,8 @@ namespace hp2d {
 
 template <typename T> struct FinArgs {
     SimState *st;
-    const WavePartial *partials;
+    const double *partials;
+    long long part_cap;
     int n_partials;
     // running_control / flags
     double routing_time, tsc, inc, alfa_min, alfa_max, R, A, rec, report_pct;

@@ -59,6 +59,12 @@ class Simulation:
     def run_fixed(self, n_steps: int, graph_steps: int = 0):
         self._check(self._L.hp2d_sim_run_fixed(self._s, int(n_steps), int(graph_steps)))
 
+    def autotune(self):
+        """Pick the fastest launch shape on this sim's data (does not advance it)."""
+        tj, wpb, ms = C.c_int32(), C.c_int32(), C.c_double()
+        self._check(self._L.hp2d_sim_autotune(self._s, C.byref(tj), C.byref(wpb), C.byref(ms)))
+        return {"tj": tj.value, "waves_per_block": wpb.value, "ms": ms.value}
+
     def kernel_time(self):
         li, tot, n = C.c_double(), C.c_double(), C.c_int64()
         self._check(self._L.hp2d_sim_kernel_time(self._s, C.byref(li), C.byref(tot), C.byref(n)))

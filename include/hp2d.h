@@ -309,6 +309,12 @@ hp2d_status hp2d_sim_get_dt_history(hp2d_sim *s, double *dt_s, int32_t n_max,
  * host rollback path of HydroPol2D_Main_While.m:926-993 without a real fault. */
 hp2d_status hp2d_debug_inject_failure(hp2d_sim *s, int64_t step_k, int32_t kind);
 
+/* Project autotuner of the fused step's launch shape: times the candidate
+ * (columns per wave task, waves per block) pairs on this sim's own data
+ * without advancing it, keeps the fastest, and caches the choice per
+ * (device, grid shape, precision) for the process.  Outputs may be NULL. */
+hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t *best_wpb, double *best_ms);
+
 /* Average duration [ms] of the fused local-inertial kernel over the last
  * hp2d_sim_run_fixed call, measured with HIP events on the sim's stream. */
 hp2d_status hp2d_sim_kernel_time(hp2d_sim *s, double *li_ms, double *total_ms,
