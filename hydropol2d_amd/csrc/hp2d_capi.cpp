@@ -110,8 +110,9 @@ int32_t hp2d_device_count(void) {
 const char *hp2d_global_last_error(void) { return g_last_error.c_str(); }
 
 hp2d_status hp2d_create(int32_t ny, int32_t nx, int32_t precision, hp2d_handle **out) {
-    if (!out || ny < 1 || nx < 1 || (precision != HP2D_FP64 && precision != HP2D_FP32)) {
-        g_last_error = "hp2d_create: bad arguments";
+    if (!out || ny < 1 || nx < 1 || (precision != HP2D_FP64 && precision != HP2D_FP32) ||
+        (double)ny * (double)nx >= 4294967296.0) {
+        g_last_error = "hp2d_create: bad arguments (grids are limited to 2^32 cells)";
         return HP2D_E_ARG;
     }
     if (hp2d_device_count() < 1) {
@@ -214,6 +215,9 @@ static hp2d_status li_dropin(hp2d_handle *h, int scheme, const T *z, const T *d_
     a.R2 = resolution * resolution;
     a.cell_area = cell_area;
     a.C_a = resolution * resolution;
+    a.inv_R = T(1) / resolution;
+    a.k_mmh_q = T((double)resolution / 3.6e6);
+    a.k_q_mmh = T(3.6e6 / (double)cell_area);
     a.flag_critical = critical;
     a.outlet_type = outlet_type;
     a.sqrtS_type1 = std::sqrt(std::fabs(slope_outlet));
@@ -383,6 +387,9 @@ template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
     a.R2 = T(c.resolution * c.resolution);
     a.cell_area = T(c.resolution * c.resolution);
     a.C_a = T(c.resolution * c.resolution);
+    a.inv_R = T(1.0 / c.resolution);
+    a.k_mmh_q = T(c.resolution / 3.6e6);
+    a.k_q_mmh = T(3.6e6 / (c.resolution * c.resolution));
     a.flag_critical = c.flag_critical;
     a.outlet_type = c.outlet_type;
     a.adaptive = c.flag_adaptive_timestepping;
@@ -441,6 +448,7 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
     f.ga = c.flag_infiltration == 1;
     f.flag_wq = c.flag_waterquality == 1;
     f.has_pattern = s->pattern != nullptr;
+    f.vmax_squared = s->elem == 4;
     f.n_rec = s->n_hydro_max;
     f.rain_t = s->rain_t;
     f.rain_v = s->rain_v;
@@ -483,6 +491,10 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
                             hp2d_sim **out) {
     if (!cfg || !in || !out || cfg->ny < 1 || cfg->nx < 1 || !in->elevation || !in->roughness) {
         g_last_error = "hp2d_sim_create: bad arguments";
+        return HP2D_E_ARG;
+    }
+    if ((double)cfg->ny * (double)cfg->nx >= 4294967296.0) {
+        g_last_error = "hp2d_sim_create: grids are limited to 2^32 cells";
         return HP2D_E_ARG;
     }
     if (cfg->routing != HP2D_ROUTING_INERTIAL || cfg->flag_numerical_scheme != 1) {

@@ -5,59 +5,119 @@
 // rows; lanes 0,1,62,63 are the radius-2 halo that the donor draining limiter
 // needs (a face flux depends on its donor cell's scale, which depends on all
 // four faces of that donor).  Every per-cell array is read once per column
-// with a 64-lane coalesced load; the 2-row halo is re-read from L2, and
-// neighbours across lanes come from cross-lane shuffles (no LDS).  Per column
-// the wave computes, with a lag of two columns:
+// with a 64-lane coalesced load issued one column ahead of the compute; the
+// 2-row halo is re-read from L2, and neighbours across lanes come from
+// wavefront-shift DPP moves (no LDS).  Per column the wave computes, with a
+// lag of two columns:
 //   raw face fluxes  (Inertial_Solver.m:23-26 + caps, Local_Inertial_Model_D4.m:123-142,198,263-304)
 //   donor scales     (Local_Inertial_Model_D4.m:799-829)
 //   limited fluxes   (:831-857), continuity (:313-315,:389-415), outlet (:418-498)
-// and, in simulation mode, the Green-Ampt / rainfall prologue of
+// and, in simulation mode, the rainfall / Green-Ampt prologue of
 // Hydrological_Model.m (folded into the column loads) and the per-cell terms of
 // refreshing_timestep.m:63-242 reduced to per-wave partials.
+//
+// Precision policy: the double instantiation keeps the reference's operation
+// order (it is the parity mode); the float instantiation evaluates the same
+// expressions with reciprocal multiplies and the hardware log2/exp2/rcp.
 #include "hp2d_internal.hpp"
 #include "hp2d_li.hpp"
 
 namespace hp2d {
 
-// Face slope and flow depth.  FP64 keeps the reference's operation order
-// (y = z + h, Sx/Sy :123-127, Hf :133-134).  FP32 evaluates the same
-// expressions with the elevation difference taken first, so the flow depth of
-// a shallow face keeps the precision of the cell depths instead of that of
-// the elevations (both are the same real-number expression).
-__device__ __forceinline__ void face_geom(double zA, double zB, double hA, double hB, double R,
-                                          double &S, double &Hf) {
+// ---------------------------------------------------------------------------
+// cross-lane moves: lane i receives lane i-1 (up) / lane i+1 (down)
+// (DPP wave_shr:1 / wave_shl:1; the end lanes keep 0 and are halo lanes)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float lane_up(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_down(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1); }
+__device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1); }
+
+// ---------------------------------------------------------------------------
+// precision-specific arithmetic
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// mm -> m
+__device__ __forceinline__ double mm2m(double x) { return x / 1000.0; }
+__device__ __forceinline__ float mm2m(float x) { return x * 1e-3f; }
+
+// Face slope and flow depth.  FP64: y = z + h, Sx/Sy (:123-127), Hf (:133-134)
+// in the reference's order.  FP32: the same expressions with the elevation
+// difference taken first, so a shallow face keeps the precision of the cell
+// depths instead of that of the elevations.
+__device__ __forceinline__ void face_geom(double zA, double zB, double hA, double hB,
+                                          const LiArgs<double> &a, double &S, double &Hf) {
     const double yA = zA + hA, yB = zB + hB;
-    S = (yB - yA) / R;
+    S = (yB - yA) / a.R;
     Hf = fmax(fmax(yA, yB) - fmax(zA, zB), 0.0);
 }
-__device__ __forceinline__ void face_geom(float zA, float zB, float hA, float hB, float R,
-                                          float &S, float &Hf) {
+__device__ __forceinline__ void face_geom(float zA, float zB, float hA, float hB,
+                                          const LiArgs<float> &a, float &S, float &Hf) {
     const float zm = fmaxf(zA, zB);
-    S = ((zB - zA) + (hB - hA)) / R;
+    S = ((zB - zA) + (hB - hA)) * a.inv_R;
     Hf = fmaxf(fmaxf(hA + (zA - zm), hB + (zB - zm)), 0.0f);
 }
 
+// stored mm/h -> m2/s (:198)
+__device__ __forceinline__ double mmh_to_q(double v, const LiArgs<double> &a) {
+    return v / 1000.0 / 3600.0 * a.R2 / a.R;
+}
+__device__ __forceinline__ float mmh_to_q(float v, const LiArgs<float> &a) { return v * a.k_mmh_q; }
+
+// Bates update (Inertial_Solver.m:24-25)
+__device__ __forceinline__ double bates(double q_p, double Hf, double S, double n2, double dt) {
+    const double g = 9.81;
+    return (q_p - g * Hf * dt * S) / (1.0 + g * dt * n2 * fabs(q_p) / pow(Hf, 7.0 / 3.0));
+}
+__device__ __forceinline__ float bates(float q_p, float Hf, float S, float n2, float dt) {
+    const float g = 9.81f;
+    const float p73 = __builtin_amdgcn_exp2f((7.0f / 3.0f) * __builtin_amdgcn_logf(Hf));
+    const float den = 1.0f + g * dt * n2 * fabsf(q_p) * __builtin_amdgcn_rcpf(p73);
+    return (q_p - g * Hf * dt * S) * __builtin_amdgcn_rcpf(den);
+}
+
+// face discharge [m3/s] -> stored mm/h of cell area (:309)
+__device__ __forceinline__ double q_to_mmh(double Q, const LiArgs<double> &a) {
+    return Q / a.cell_area * 1000.0 * 3600.0;
+}
+__device__ __forceinline__ float q_to_mmh(float Q, const LiArgs<float> &a) { return Q * a.k_q_mmh; }
+
 template <typename T>
-__device__ __forceinline__ T face_flux(T qp_mmh, T zA, T zB, T hA, T hB, T n2, T R, T R2, T dt,
-                                       int crit, T &Hf_out) {
+__device__ __forceinline__ T face_flux(T qp_mmh, T zA, T zB, T hA, T hB, T n2, T dt,
+                                       const LiArgs<T> &a, T &Hf_out) {
     // A is the cell that stores the face, B the other side; S = (yB - yA)/R
     // reproduces Sx (:123) for x faces and Sy (:127) for y faces.
-    const T g = T(9.81);
     T S, Hf;
-    face_geom(zA, zB, hA, hB, R, S, Hf);
-    T q_p = qp_mmh / T(1000) / T(3600) * R2 / R;
-    T num = q_p - g * Hf * dt * S;
-    T den = T(1) + g * dt * n2 * fabs(q_p) / pow73(Hf);
-    T q = num / den;
+    face_geom(zA, zB, hA, hB, a, S, Hf);
+    T q = bates(mmh_to_q(qp_mmh, a), Hf, S, n2, dt);
     if (!isfinite(q)) q = T(0);
-    if (crit) {
-        T cv = Hf * sqrt(T(9.81) * Hf);
+    if (a.flag_critical) {
+        const T cv = Hf * sqrt(T(9.81) * Hf);
         q = dmin(dmax(q, -cv), cv);
     }
-    T thr = Hf * T(10);
+    const T thr = Hf * T(10);
     q = dmin(dmax(q, -thr), thr);
     Hf_out = Hf;
-    return q * R;  // outflow_rate [m3/s]
+    return q * a.R;  // outflow_rate [m3/s]
+}
+
+__device__ __forceinline__ int find_outlet(const long long *idx, int n, long long key) {
+    int lo = 0, hi = n - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const long long v = idx[mid];
+        if (v == key) return mid;
+        if (v < key) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return -1;
 }
 
 // Per-column loads of one lane, fetched one column ahead of the compute.
@@ -70,7 +130,7 @@ template <typename T, int MODE>
 __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d_src,
                                                   const T *qx_src, const T *qy_src,
                                                   const T *I_src, const T *Tc_src, bool in,
-                                                  long long id) {
+                                                  unsigned id) {
     ColLoad<T> v;
     v.z = v.d = v.n2 = v.qx = v.qy = v.I = v.Tc = T(0);
     v.pat = T(1);
@@ -91,19 +151,6 @@ __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d
         }
     }
     return v;
-}
-
-template <typename T>
-__device__ __forceinline__ int find_outlet(const long long *idx, int n, long long key) {
-    int lo = 0, hi = n - 1;
-    while (lo <= hi) {
-        int mid = (lo + hi) >> 1;
-        long long v = idx[mid];
-        if (v == key) return mid;
-        if (v < key) lo = mid + 1;
-        else hi = mid - 1;
-    }
-    return -1;
 }
 
 template <typename T, int MODE>
@@ -145,13 +192,17 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
         dt_h = a.dt_h;
     }
 
-    const T R = a.R, R2 = a.R2, A = a.cell_area, Ca = a.C_a;
+    const T A = a.cell_area;
     const int r = slab * kRowsOut - kHalo + lane;
     const bool row_in = (r >= 0) && (r < ny);
     const bool row_own = row_in && lane >= kHalo && lane < kWave - kHalo;
     const int c0 = blockIdx.y * a.tj;
     const int c1 = min(c0 + a.tj, nx);
     const T avail_den = dmax(dt_s, eps_of<T>());
+    const T inv_dt_h = frcp(dmax(dt_h, T(1e-12)));  // FP32 path only
+    const T k_avail = A * frcp(avail_den);            // FP32 path only
+    const T k_cont = T(1000) * frcp(a.C_a);           // FP32 path only
+    constexpr bool f64 = sizeof(T) == 8;
 
     // pipeline registers (suffix = column lag behind the load column c)
     T h1 = 0, z1 = 0, n21 = 0, qxp1 = 0, d1 = 0, n2s1 = 0;   // column c-1
@@ -159,40 +210,46 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
     T d2 = 0, scale2 = 0, finQx3 = 0, ox3 = 0, n2s2 = 0;
 
     // per-lane reductions
-    T vmax = T(NAN), dtw = T(NAN), rmax = T(NAN), dtmax = T(NAN), dpmax = T(NAN);
+    T vkey = T(NAN), wkey = T(NAN), rmax = T(NAN), dtmax = T(NAN), dpmax = T(NAN);
     double s_store = 0.0, s_uz = 0.0, s_rain = 0.0, s_inf = 0.0;
     int neg = 0;
-    const T alfaR = a.alfa_min * R;
 
+    // 32-bit element offsets (grids are limited to 2^32 cells at creation)
     ColLoad<T> nxt = load_column<T, MODE>(a, d_src, qx_src, qy_src, I_src, Tc_src,
                                           row_in && c0 - 2 >= 0 && c0 - 2 < nx,
-                                          (long long)r + (long long)(c0 - 2) * ny);
+                                          (unsigned)r + (unsigned)(c0 - 2) * (unsigned)ny);
+#pragma unroll 2
     for (int c = c0 - 2; c <= c1 + 1; ++c) {
         const bool col_in = (c >= 0) && (c < nx);
         const bool cell_in = row_in && col_in;
-        const long long id = (long long)r + (long long)c * ny;
+        const unsigned id = (unsigned)r + (unsigned)c * (unsigned)ny;
         const ColLoad<T> cur = nxt;
         // issue the next column's loads before computing this one
         if (c + 1 <= c1 + 1)
             nxt = load_column<T, MODE>(a, d_src, qx_src, qy_src, I_src, Tc_src,
-                                       row_in && c + 1 >= 0 && c + 1 < nx, id + ny);
-        T zc = cur.z, dc = cur.d, n2c = cur.n2, qxc = cur.qx, qyc = cur.qy, dprev = 0;
-        // ---- hydrology prologue: Hydrological_Model.m:15-55 (sim mode) ----
+                                       row_in && c + 1 >= 0 && c + 1 < nx, id + (unsigned)ny);
+        const T zc = cur.z, n2c = cur.n2, qxc = cur.qx, qyc = cur.qy;
+        T dc = cur.d;
+
+        // ---- rainfall + Green-Ampt prologue: Hydrological_Model.m:15-55 ----
         if (MODE == kModeSim && cell_in) {
-            dprev = dc;
+            const T dprev = dc;
             const bool nan_cell = isnan(zc);
             T eff = a.rain_pattern ? cur.pat * rain : rain;
             if (nan_cell) eff = T(NAN);
             T dd = dc + eff;
             const bool own = row_own && c >= c0 && c < c1;
-            if (own && !nan_cell && !fail_step) s_rain += (double)eff / 1000.0 * (double)A;
+            if (own && !nan_cell && !fail_step) s_rain += (double)eff;
             if (a.ga) {
+                // legacy explicit Green-Ampt closure in the Infiltration_Module slot
+                // (HydroPol2D_Main_While_Ensemble.m:243-298)
                 const GaClass g = a.ga[cur.soil];
                 const T ksat = T(g.ksat), psi = T(g.psi), dth = T(g.dtheta), Imin = a.I_min;
                 const T Ip = cur.I;
                 T Tn = cur.Tc;
-                const T i_a = dd / dmax(dt_h, T(1e-12));
-                T C = ksat * (T(1) + ((dd + psi) * dth) / Ip);
+                const T i_a = f64 ? dd / dmax(dt_h, T(1e-12)) : dd * inv_dt_h;
+                T C = f64 ? ksat * (T(1) + ((dd + psi) * dth) / Ip)
+                          : ksat * (T(1) + ((dd + psi) * dth) * frcp(Ip));
                 T f, In;
                 if (kstep == 1) {
                     f = dmin(C, i_a);
@@ -216,8 +273,8 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
                     I_dst[id] = In;
                     Tc_dst[id] = Tn;
                     if (!fail_step) {
-                        if (!isnan(f)) s_inf += (double)(f * dt_h) / 1000.0 * (double)A;
-                        if (!isnan(In)) s_uz += (double)A * (double)In / 1000.0;
+                        if (!isnan(f)) s_inf += (double)(f * dt_h);
+                        if (!isnan(In)) s_uz += (double)In;
                     }
                 }
             }
@@ -226,32 +283,33 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
             if (dd < T(0)) dd = T(0);     // :52
             dc = dd;
             if (own) {  // rollback celerity inputs (HydroPol2D_Main_While.m:932)
-                dtmax = dmax(dtmax, dc / T(1000));
-                dpmax = dmax(dpmax, dprev / T(1000));
+                dtmax = dmax(dtmax, dc);
+                dpmax = dmax(dpmax, dprev);
             }
         }
-        const T hc = dmax(dc / T(1000), T(0));  // depth_cell (:31)
+        const T hc = dmax(mm2m(dc), T(0));  // depth_cell (:31)
 
         // ---- raw y-face flux at (r,c): face between rows r-1 and r ----
-        const T h_up = __shfl_up(hc, 1), z_up = __shfl_up(zc, 1);
+        const T h_up = lane_up(hc), z_up = lane_up(zc);
         T rawQyc = 0, Hfyc = 0;
         if (cell_in && r >= 1)
-            rawQyc = face_flux<T>(qyc, zc, z_up, hc, h_up, fabs(n2c), R, R2, dt_s, a.flag_critical, Hfyc);
+            rawQyc = face_flux<T>(qyc, zc, z_up, hc, h_up, fabs(n2c), dt_s, a, Hfyc);
 
         // ---- raw x-face flux at (r,c-1): face between columns c-1 and c ----
         T rawQx1 = 0, Hfx1 = 0;
         if (row_in && c - 1 >= 0 && c - 1 < nx - 1)
-            rawQx1 = face_flux<T>(qxp1, z1, zc, h1, hc, fabs(n21), R, R2, dt_s, a.flag_critical, Hfx1);
+            rawQx1 = face_flux<T>(qxp1, z1, zc, h1, hc, fabs(n21), dt_s, a, Hfx1);
 
         // ---- donor scale of cell (r,c-1): Local_Inertial_Model_D4.m:814-829 ----
-        const T qy1_dn = __shfl_down(rawQy1, 1);
+        const T qy1_dn = lane_down(rawQy1);
         T outr = T(0) + dmax(rawQx1, T(0));
         if (c - 1 >= 1) outr += dmax(-rawQx2, T(0));
         outr += dmax(rawQy1, T(0));
         if (r < ny - 1) outr += dmax(-qy1_dn, T(0));
-        const T avail = dmax(d1 / T(1000), T(0)) * A / avail_den;
+        const T avail = f64 ? dmax(d1 / T(1000), T(0)) * A / avail_den
+                            : dmax(d1 * T(1e-3), T(0)) * k_avail;
         T s1 = T(1);
-        if (outr > avail && outr > T(0)) s1 = avail / outr;
+        if (outr > avail && outr > T(0)) s1 = f64 ? avail / outr : avail * frcp(outr);
         if (!isfinite(s1)) s1 = T(0);
         s1 = dmax(dmin(s1, T(1)), T(0));
 
@@ -262,27 +320,27 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
         else if (fqx < T(0)) fqx = fqx * s1;
         if (o == nx - 1) fqx = T(0);
         if (!isfinite(fqx)) fqx = T(0);
-        const T s2_up = __shfl_up(scale2, 1);
+        const T s2_up = lane_up(scale2);
         T fqy = rawQy2;
         if (fqy > T(0)) fqy = fqy * scale2;
         else if (fqy < T(0)) fqy = fqy * s2_up;
         if (r == 0) fqy = T(0);
         if (!isfinite(fqy)) fqy = T(0);
-        const T fqy_dn = __shfl_down(fqy, 1);
+        const T fqy_dn = lane_down(fqy);
 
         // ---- continuity (:313-315, :390, :415) ----
         const T west = (o >= 1) ? finQx3 : T(0);
         const T south = (r < ny - 1) ? fqy_dn : T(0);
         const T vol = dt_s * (west - fqx - fqy + south);
-        T dn = d2 + T(1000) * vol / Ca;
+        T dn = f64 ? d2 + T(1000) * vol / a.C_a : d2 + vol * k_cont;
         if (dn < T(0)) dn = T(0);
 
         const bool out_cell = row_own && o >= c0 && o < c1;
-        const long long oid = (long long)r + (long long)o * ny;
+        const unsigned oid = (unsigned)r + (unsigned)o * (unsigned)ny;
         // ---- outlet (:418-498) ----
         T hf3 = T(0), qo = T(0);
         if (out_cell && signbit(n2s2) && a.n_outlet > 0) {
-            const int slot = find_outlet<T>(a.out_idx, a.n_outlet, oid);
+            const int slot = find_outlet(a.out_idx, a.n_outlet, (long long)oid);
             if (slot >= 0) {
                 const T h_out = dmax(dn, T(0)) / T(1000);
                 T sqrtS;
@@ -294,7 +352,7 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
                     if (!isfinite(sqrtS)) sqrtS = T(0);
                 }
                 hf3 = h_out;
-                qo = (T(1) / a.out_n[slot]) * R * pow53(h_out) * sqrtS / A * T(1000) * T(3600);
+                qo = (T(1) / a.out_n[slot]) * a.R * pow53(h_out) * sqrtS / A * T(1000) * T(3600);
                 const T qmax = dmax(dn - T(1000) * T(0), T(0)) / dt_h;
                 qo = dmin(qo, qmax);
                 if (!isfinite(qo)) qo = T(0);
@@ -303,34 +361,61 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
                 if (a.out_d) a.out_d[slot] = dn;
             }
         }
-        const T ox = fqx / A * T(1000) * T(3600);
-        const T oy = fqy / A * T(1000) * T(3600);
+        const T ox = q_to_mmh(fqx, a);
+        const T oy = q_to_mmh(fqy, a);
 
         if (MODE == kModeSim) {
             // ---- refreshing_timestep.m:63-238 per-cell terms ----
-            const T f1 = Hfx2 * T(1000), f2 = Hfy2 * T(1000), f3 = hf3 * T(1000);
             const T tol = a.depth_tol_mm;
-            const T s1d = (f2 < tol) ? T(1e12) : f2;
-            const T vu = (oy / T(1000) / T(3600)) * R / (s1d / T(1000));
-            const T vu_dn = __shfl_down(vu, 1);
+            T vu;
+            if (f64) {
+                const T f2 = Hfy2 * T(1000);
+                const T s2d = (f2 < tol) ? T(1e12) : f2;
+                vu = (oy / T(1000) / T(3600)) * a.R / (s2d / T(1000));
+            } else {
+                const T hy = (Hfy2 * T(1000) < tol) ? T(1e9) : Hfy2;
+                vu = fqy * a.inv_R * frcp(hy);
+            }
+            const T vu_dn = lane_down(vu);
             if (out_cell && !fail_step) {
-                const T dts = dmax(dmax(f1 / T(1000), f2 / T(1000)), f3 / T(1000));
-                const T sxd = (f1 < tol) ? T(1e12) : f1;
-                const T vr = (ox / T(1000) / T(3600)) * R / (sxd / T(1000));
+                T vr, dts;
+                if (f64) {
+                    const T f1 = Hfx2 * T(1000), f2 = Hfy2 * T(1000), f3 = hf3 * T(1000);
+                    dts = dmax(dmax(f1 / T(1000), f2 / T(1000)), f3 / T(1000));
+                    const T s1d = (f1 < tol) ? T(1e12) : f1;
+                    vr = (ox / T(1000) / T(3600)) * a.R / (s1d / T(1000));
+                } else {
+                    dts = dmax(dmax(Hfx2, Hfy2), hf3);
+                    const T hx = (Hfx2 * T(1000) < tol) ? T(1e9) : Hfx2;
+                    vr = fqx * a.inv_R * frcp(hx);
+                }
                 const T vl = (o >= 1) ? -vr : -T(0);  // :73 unshifted quirk
                 const T vd = (r < ny - 1) ? -vu_dn : -T(0);
-                const T raster = dmax(dmax(dmax(fabs(vl), fabs(vr)), fabs(vu)), fabs(vd));
+                // velocity_raster only enters dt (non-adaptive) and the rollback celerity
+                const T raster = a.adaptive ? T(0)
+                                            : dmax(dmax(dmax(fabs(vl), fabs(vr)), fabs(vu)), fabs(vd));
                 const T rc = T(0.5) * (vr - vl), uc = T(0.5) * (vu - vd);
-                const T tot = hypot(rc, uc);
-                vmax = dmax(vmax, tot);
-                rmax = dmax(rmax, raster);
-                T cw = sqrt(T(9.81) * dmax(dts, T(0)));
-                cw = dmax(cw, T(1e-10));
-                if (isinf(cw)) cw = T(NAN);
-                if (tot < T(0.01)) cw = T(NAN);
-                const T cand = a.adaptive ? alfaR / cw : alfaR / (raster + cw);
-                dtw = dmin(dtw, cand);
-                if (!isnan(dn)) s_store += (double)A * (double)dn / 1000.0;
+                bool stagnant;
+                if (f64) {
+                    const T tot = hypot(rc, uc);
+                    vkey = dmax(vkey, tot);
+                    stagnant = tot < T(0.01);
+                } else {
+                    const T tot2 = rc * rc + uc * uc;  // compared squared, rooted once per step
+                    vkey = dmax(vkey, tot2);
+                    stagnant = tot2 < T(1e-4);
+                }
+                if (!a.adaptive) rmax = dmax(rmax, raster);
+                if (!stagnant) {
+                    if (a.adaptive) {
+                        wkey = dmax(wkey, dts);  // min(alfa*dx/c) = alfa*dx/c(max depth)
+                    } else {
+                        T cw = sqrt(T(9.81) * dmax(dts, T(0)));
+                        cw = dmax(cw, T(1e-10));
+                        if (!isinf(cw)) wkey = dmin(wkey, a.alfa_min * a.R / (raster + cw));
+                    }
+                }
+                if (!isnan(dn)) s_store += (double)dn;
             }
             if (out_cell) {
                 d_dst[oid] = dn;
@@ -342,9 +427,8 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
                 }
             }
         } else {
-            const T oy_dn = __shfl_down(oy, 1);
+            const T oy_dn = lane_down(oy);
             if (out_cell) {
-                const long long n = (long long)ny * nx;
                 d_dst[oid] = dn;
                 if (qx_dst) qx_dst[oid] = ox;
                 if (qy_dst) qy_dst[oid] = oy;
@@ -358,8 +442,7 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
                 if (a.qleft) a.qleft[oid] = -((o >= 1) ? ox3 : T(0));
                 if (a.qdown) a.qdown[oid] = -((r < ny - 1) ? oy_dn : T(0));
                 if (a.itot)
-                    a.itot[oid] = fabs(ox + oy + qo) * dt_s / T(1000) * T(1) / T(3600) * R2;
-                (void)n;
+                    a.itot[oid] = fabs(ox + oy + qo) * dt_s / T(1000) * T(1) / T(3600) * a.R2;
             }
         }
 
@@ -384,8 +467,10 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
     }
 
     if (MODE == kModeSim) {
-        // per-wave partials (deterministic butterfly reductions)
-        const T wv_vmax = wave_fmax(vmax), wv_dtw = wave_fmin(dtw), wv_r = wave_fmax(rmax);
+        // per-wave partials (deterministic butterfly reductions); volumes are
+        // reduced as sums of depths and scaled by A/1000 by the finalize kernel
+        const T wv_v = wave_fmax(vkey), wv_r = wave_fmax(rmax);
+        const T wv_w = a.adaptive ? wave_fmax(wkey) : wave_fmin(wkey);
         const T wv_dt = wave_fmax(dtmax), wv_dp = wave_fmax(dpmax);
         const double st_sum = wave_sum(s_store), uz_sum = wave_sum(s_uz);
         const double rn_sum = wave_sum(s_rain), in_sum = wave_sum(s_inf);
@@ -398,8 +483,8 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
             P[kPartUz * cap + idx] = uz_sum;
             P[kPartRain * cap + idx] = rn_sum;
             P[kPartInfil * cap + idx] = in_sum;
-            P[kPartVmax * cap + idx] = (double)wv_vmax;
-            P[kPartDtw * cap + idx] = (double)wv_dtw;
+            P[kPartVmax * cap + idx] = (double)wv_v;
+            P[kPartDtw * cap + idx] = (double)wv_w;
             P[kPartRaster * cap + idx] = (double)wv_r;
             P[kPartDtotMax * cap + idx] = (double)wv_dt;
             P[kPartDprevMax * cap + idx] = (double)wv_dp;

@@ -14,6 +14,7 @@ template <typename T> struct LiArgs {
     int waves_per_block;  // 1..4
     // physics constants
     T R, R2, cell_area, C_a;
+    T inv_R, k_mmh_q, k_q_mmh;  // 1/R, R/3.6e6 (mm/h -> m2/s), 3.6e6/A (m3/s -> mm/h)
     int flag_critical, outlet_type, adaptive;
     T sqrtS_type1;        // sqrt(|slope_outlet|)
     T depth_tol_mm, alfa_min;

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
         rain += P[kPartRain * cap + i];
         infil += P[kPartInfil * cap + i];
         vmax = fmax(vmax, P[kPartVmax * cap + i]);
-        dtw = fmin(dtw, P[kPartDtw * cap + i]);
+        dtw = a.adaptive ? fmax(dtw, P[kPartDtw * cap + i]) : fmin(dtw, P[kPartDtw * cap + i]);
         raster = fmax(raster, P[kPartRaster * cap + i]);
         dtm = fmax(dtm, P[kPartDtotMax * cap + i]);
         dpm = fmax(dpm, P[kPartDprevMax * cap + i]);
@@ -47,7 +47,8 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
     }
     storage = wave_sum(storage); uz = wave_sum(uz); rain = wave_sum(rain);
     infil = wave_sum(infil); qsum = wave_sum(qsum); dsum = wave_sum(dsum);
-    vmax = wave_fmax(vmax); dtw = wave_fmin(dtw); raster = wave_fmax(raster);
+    vmax = wave_fmax(vmax); dtw = a.adaptive ? wave_fmax(dtw) : wave_fmin(dtw);
+    raster = wave_fmax(raster);
     dtm = wave_fmax(dtm); dpm = wave_fmax(dpm); neg = wave_fmax(neg);
     if (lane == 0) {
         sh[w][0] = storage; sh[w][1] = uz; sh[w][2] = rain; sh[w][3] = infil;
@@ -61,9 +62,27 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
     neg = 0.0;
     for (int v = 0; v < kFinWaves; ++v) {  // fixed order: deterministic
         storage += sh[v][0]; uz += sh[v][1]; rain += sh[v][2]; infil += sh[v][3];
-        vmax = fmax(vmax, sh[v][4]); dtw = fmin(dtw, sh[v][5]); raster = fmax(raster, sh[v][6]);
+        vmax = fmax(vmax, sh[v][4]);
+        dtw = a.adaptive ? fmax(dtw, sh[v][5]) : fmin(dtw, sh[v][5]);
+        raster = fmax(raster, sh[v][6]);
         dtm = fmax(dtm, sh[v][7]); dpm = fmax(dpm, sh[v][8]); neg = fmax(neg, sh[v][9]);
         qsum += sh[v][10]; dsum += sh[v][11];
+    }
+    // volumes were reduced as sums of depths [mm]
+    const double a1000 = a.A / 1000.0;
+    storage *= a1000;
+    uz *= a1000;
+    rain *= a1000;
+    infil *= a1000;
+    if (a.vmax_squared) vmax = sqrt(vmax);
+    dtm /= 1000.0;  // max depths were reduced in mm
+    dpm /= 1000.0;
+    if (a.adaptive && !isnan(dtw)) {
+        // refreshing_timestep.m:227-233 on the deepest non-stagnant cell:
+        // alfa*dx/c is monotone in the depth, so its minimum is taken there
+        double cw = sqrt(9.81 * fmax(dtw, 0.0));
+        cw = fmax(cw, 1e-10);
+        dtw = isinf(cw) ? NAN : a.alfa_min * a.R / cw;
     }
 
     const long long k = st->k;
