@@ -33,6 +33,7 @@ def parse():
     p.add_argument("--graph-steps", type=int, default=50)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     return p.parse_args()
 
 
@@ -68,7 +69,7 @@ def main():
     sim = Simulation(spec)
     # warm-up (also lets water start moving: rainfall begins at t = 0)
     sim.run_fixed(args.warmup, graph_steps=min(args.graph_steps, max(args.warmup, 1)))
-    tune = sim.autotune()
+    tune = {"tj": None, "waves_per_block": None} if args.no_tune else sim.autotune()
     print(f"[bench] warm-up done, launch shape {tune}", file=sys.stderr, flush=True)
     # timed region: K whole steps through captured graphs
     t1 = time.perf_counter()

@@ -15,6 +15,7 @@ LIB = os.path.join(HERE, "libhp2d.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
+    "-I" + os.path.join(os.path.dirname(HERE), "include"),
     "-O3", "-std=c++17", "-fPIC", "-shared",
     "--offload-arch=gfx950",
     # no packed FP32 instructions (v_pk_*_f32) anywhere in the device code

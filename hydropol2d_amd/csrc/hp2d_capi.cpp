@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/hp2d.h"
+#include "hp2d.h"
 #include "hp2d_forcing.hpp"
 #include "hp2d_internal.hpp"
 #include "hp2d_li.hpp"
@@ -335,6 +335,7 @@ struct hp2d_sim {
     void *I[2] = {nullptr, nullptr}, *Tc[2] = {nullptr, nullptr};
     uint8_t *soil = nullptr;
     GaClass *ga = nullptr;
+    float *ga_f32 = nullptr;
     long long *out_idx = nullptr;
     void *out_n = nullptr, *out_q = nullptr, *out_d = nullptr;
     int n_outlet = 0;
@@ -349,6 +350,7 @@ struct hp2d_sim {
     double *dt_hist = nullptr;
     int n_dt_max = 4096;
     double *vol_err = nullptr;
+    double *tune_sink = nullptr;
     int n_rep_max = 256;
     double sum_pattern = 0, n_valid = 0, drainage_area = 0, datum = 0, I_min = 0;
     int tj = 16, wpb = 4;
@@ -409,6 +411,9 @@ template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
     a.rain_pattern = (const T *)s->pattern;
     a.soil_class = s->soil;
     a.ga = s->ga;
+    a.ga_f32 = s->ga_f32;
+    a.n_soil_classes = c.flag_infiltration == 1 ? c.n_soil_classes : 0;
+    a.routing_time = c.routing_time_min;
     a.I_min = T(s->I_min);
     a.dmax_map = (T *)s->dmax;
     a.out_idx = s->out_idx;
@@ -428,8 +433,9 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
     f.st = s->st;
     f.partials = s->partials;
     f.part_cap = s->part_cap;
-    // only the waves of the current launch shape write partials
-    f.n_partials = ((s->ny + kRowsOut - 1) / kRowsOut) * ((s->nx + s->tj - 1) / s->tj);
+    // one partial per workgroup of the current launch shape
+    f.n_partials = (((s->ny + kRowsOut - 1) / kRowsOut + s->wpb - 1) / s->wpb) *
+                   ((s->nx + s->tj - 1) / s->tj);
     f.routing_time = c.routing_time_min;
     f.tsc = c.time_step_change_s;
     f.inc = c.time_step_increments_s;
@@ -571,7 +577,8 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     std::vector<GaClass> gac;
     const bool ga = cfg->flag_infiltration == 1;
     if (ga) {
-        if (cfg->n_soil_classes < 1 || !in->ksat_mmh || !in->psi_mm || !in->dtheta ||
+        if (cfg->n_soil_classes < 1 || cfg->n_soil_classes > kMaxSoilClasses || !in->ksat_mmh ||
+            !in->psi_mm || !in->dtheta ||
             !in->theta_sat_minus_r)
             return fail(HP2D_E_ARG, "hp2d_sim_create: Green-Ampt tables missing");
         for (int k = 0; k < cfg->n_soil_classes; ++k) {
@@ -634,6 +641,7 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     if (in->rain_pattern) A_(&s->pattern, nb);
     if (ga && in->soil_class) A_((void **)&s->soil, n);
     if (ga) A_((void **)&s->ga, gac.size() * sizeof(GaClass));
+    if (ga) A_((void **)&s->ga_f32, gac.size() * 8 * sizeof(float));
     s->n_outlet = (int)oidx.size();
     if (s->n_outlet > 0) {
         A_((void **)&s->out_idx, oidx.size() * sizeof(long long));
@@ -655,6 +663,7 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     A_((void **)&s->hd, s->n_hydro_max * sizeof(double));
     A_((void **)&s->dt_hist, s->n_dt_max * sizeof(double));
     A_((void **)&s->vol_err, s->n_rep_max * sizeof(double));
+    A_((void **)&s->tune_sink, sizeof(double));
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? HP2D_E_NOMEM : HP2D_E_HIP,
                     std::string("hp2d_sim_create: ") + hipGetErrorString(e));
@@ -687,7 +696,17 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     up_real(s->dmax, d0, 0.0);
     if (in->rain_pattern) up_real(s->pattern, std::vector<double>(in->rain_pattern, in->rain_pattern + n), 0.0);
     if (s->soil) up(s->soil, in->soil_class, n);
-    if (ga) up(s->ga, gac.data(), gac.size() * sizeof(GaClass));
+    if (ga) {
+        up(s->ga, gac.data(), gac.size() * sizeof(GaClass));
+        std::vector<float> gf(gac.size() * 8, 0.0f);
+        for (size_t k = 0; k < gac.size(); ++k) {
+            const GaClass &g = gac[k];
+            const float v[8] = {(float)g.ksat, (float)g.psi, (float)g.dtheta, (float)g.Lu_mm,
+                                (float)g.k_out, (float)g.Tr, 0.0f, 0.0f};
+            std::memcpy(&gf[k * 8], v, sizeof(v));
+        }
+        up(s->ga_f32, gf.data(), gf.size() * sizeof(float));
+    }
     if (s->n_outlet > 0) {
         up(s->out_idx, oidx.data(), oidx.size() * sizeof(long long));
         up_real(s->out_n, on, 0.0);
@@ -741,9 +760,9 @@ void hp2d_sim_destroy(hp2d_sim *s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     if (s->graph) hipGraphExecDestroy(s->graph);
     void *ps[] = {s->z, s->n2s, s->pattern, s->dmax, s->d[0], s->d[1], s->qx[0], s->qx[1],
-                  s->qy[0], s->qy[1], s->I[0], s->I[1], s->Tc[0], s->Tc[1], s->soil, s->ga,
+                  s->qy[0], s->qy[1], s->I[0], s->I[1], s->Tc[0], s->Tc[1], s->soil, s->ga, s->ga_f32,
                   s->out_idx, s->out_n, s->out_q, s->out_d, s->partials, s->st, s->rain_t,
-                  s->rain_v, s->hq, s->ht, s->hd, s->dt_hist, s->vol_err};
+                  s->rain_v, s->hq, s->ht, s->hd, s->dt_hist, s->vol_err, s->tune_sink};
     for (void *p : ps)
         if (p) hipFree(p);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -918,7 +937,27 @@ hp2d_status hp2d_sim_report(hp2d_sim *s, hp2d_run_report *rep) {
     rep->cum_rain_m3 = st.cum_rain;
     rep->cum_infiltration_m3 = st.cum_inf;
     rep->cum_outflow_m3 = st.cum_out;
-    rep->storage_m3 = st.storage;
+    {
+        // surface storage of the current state, FP64 (mass_balance_check.m:62)
+        const int cur = (int)((st.k + 1) & 1);
+        const int blocks = 1024;
+        double *part = nullptr;
+        STRY(s, hipMalloc((void **)&part, 2 * blocks * sizeof(double)));
+        hipError_t e = s->elem == 8
+                           ? launch_storage<double>((const double *)s->d[cur], nullptr,
+                                                    (long long)s->n, part, blocks, s->stream)
+                           : launch_storage<float>((const float *)s->d[cur], nullptr,
+                                                   (long long)s->n, part, blocks, s->stream);
+        std::vector<double> hp(2 * blocks);
+        if (e == hipSuccess) e = hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        hipFree(part);
+        STRY(s, e);
+        double sum = 0.0;
+        for (int b = 0; b < blocks; ++b) sum += hp[2 * b];
+        const double A = s->cfg.resolution * s->cfg.resolution;
+        rep->storage_m3 = sum * (A / 1000.0);
+    }
     rep->catch_index = st.catch_index;
     return HP2D_OK;
 }
@@ -1128,10 +1167,25 @@ extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t 
         for (int wpb : wpbs) {
             s->tj = tj;
             s->wpb = wpb;
-            for (int i = 0; i < 2; ++i) STRY(s, step_launch(s, false));
+            // time the fused kernel plus the reduction of its partials
+            auto tune_step = [&]() -> hipError_t {
+                hipError_t e = step_launch(s, false);
+                if (e != hipSuccess) return e;
+                if (s->elem == 8) {
+                    FinArgs<double> f = sim_fin_args<double>(s);
+                    f.dry_run = 1;
+                    f.partials_out = s->tune_sink;
+                    return launch_finalize<double>(f, s->stream);
+                }
+                FinArgs<float> f = sim_fin_args<float>(s);
+                f.dry_run = 1;
+                f.partials_out = s->tune_sink;
+                return launch_finalize<float>(f, s->stream);
+            };
+            for (int i = 0; i < 2; ++i) STRY(s, tune_step());
             STRY(s, hipEventRecord(e0, s->stream));
             const int reps = 5;
-            for (int i = 0; i < reps; ++i) STRY(s, step_launch(s, false));
+            for (int i = 0; i < reps; ++i) STRY(s, tune_step());
             STRY(s, hipEventRecord(e1, s->stream));
             STRY(s, hipEventSynchronize(e1));
             float ms = 0;

@@ -12,6 +12,9 @@ namespace hp2d {
 constexpr int kWave = 64;                  // CDNA wavefront
 constexpr int kHalo = 2;                   // radius of the step stencil (donor limiter)
 constexpr int kRowsOut = kWave - 2 * kHalo; // rows finalised per wave slab
+constexpr int kXcds = 8;                    // accelerator complex dies of an MI355X
+constexpr int kMaxWavesPerBlock = 4;
+constexpr int kMaxSoilClasses = 256;        // class index is one byte per cell
 
 // ----------------------------------------------------------------------------
 // Per-wave partial results of one step (structure of arrays, FP64), reduced
@@ -21,7 +24,6 @@ constexpr int kRowsOut = kWave - 2 * kHalo; // rows finalised per wave slab
 enum PartField {
     kPartStorage = 0,  // sum A*d_t/1000 over valid cells      [m3]
     kPartUz,           // sum A*I_t/1000 (Green-Ampt)          [m3]
-    kPartRain,         // rain volume applied this step        [m3]
     kPartInfil,        // infiltrated volume this step         [m3]
     kPartVmax,         // max total cell velocity              [m/s]  (NaN = none)
     kPartDtw,          // min Courant wave candidate           [s]    (NaN = none)

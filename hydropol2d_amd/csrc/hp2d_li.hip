@@ -89,7 +89,7 @@ __device__ __forceinline__ double q_to_mmh(double Q, const LiArgs<double> &a) {
 }
 __device__ __forceinline__ float q_to_mmh(float Q, const LiArgs<float> &a) { return Q * a.k_q_mmh; }
 
-template <typename T>
+template <typename T, bool CRIT>
 __device__ __forceinline__ T face_flux(T qp_mmh, T zA, T zB, T hA, T hB, T n2, T dt,
                                        const LiArgs<T> &a, T &Hf_out) {
     // A is the cell that stores the face, B the other side; S = (yB - yA)/R
@@ -98,7 +98,7 @@ __device__ __forceinline__ T face_flux(T qp_mmh, T zA, T zB, T hA, T hB, T n2, T
     face_geom(zA, zB, hA, hB, a, S, Hf);
     T q = bates(mmh_to_q(qp_mmh, a), Hf, S, n2, dt);
     if (!isfinite(q)) q = T(0);
-    if (a.flag_critical) {
+    if (CRIT) {
         const T cv = Hf * sqrt(T(9.81) * Hf);
         q = dmin(dmax(q, -cv), cv);
     }
@@ -153,17 +153,29 @@ __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d
     return v;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool CRIT>
 __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
+    constexpr bool f64 = sizeof(T) == 8;
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
-    const int slab = blockIdx.x * a.waves_per_block + wv;
-    if (slab >= a.n_slabs) return;
+    // XCD-aware tile order: the hardware deals workgroups round-robin to the 8
+    // XCDs, so workgroup w runs on XCD w % 8.  Give each XCD a contiguous run of
+    // logical tiles (row blocks fastest, then column chunks) so that the halo
+    // rows shared by neighbouring row blocks are fetched into one L2.
+    const int n_bx = (a.n_slabs + a.waves_per_block - 1) / a.waves_per_block;
+    const int n_wg = gridDim.x;
+    const int xcd = blockIdx.x % kXcds, k_in = blockIdx.x / kXcds;
+    const int per = n_wg / kXcds, rem = n_wg % kXcds;
+    const int logical = xcd * per + min(xcd, rem) + k_in;
+    const int bx = logical % n_bx, by = logical / n_bx;
+    const int slab = bx * a.waves_per_block + wv;
+    const bool active = slab < a.n_slabs;
     const int ny = a.ny, nx = a.nx;
 
     long long kstep = 1;
     T dt_s, dt_h, rain = T(0);
     int fail_step = 0;
+    bool may_report = false;
     const T *d_src = a.d_src, *qx_src = a.qx_src, *qy_src = a.qy_src;
     T *d_dst = a.d_dst, *qx_dst = a.qx_dst, *qy_dst = a.qy_dst;
     const T *I_src = a.I_src, *Tc_src = a.T_src;
@@ -176,6 +188,11 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
         dt_h = T(st->time_step_min / 60.0);
         rain = T(st->delta_p_agg);
         fail_step = (st->inject_k == kstep);
+        // a mass-balance report can only follow this step if t + max_dt reaches the
+        // next report threshold (the new dt never exceeds max_dt); only then are the
+        // storage sums needed (HydroPol2D_Main_While.m:787-803)
+        const double prog_max = ((st->t + st->max_dt / 60.0) / a.routing_time) * 100.0;
+        may_report = (prog_max >= st->next_report) || (prog_max >= 100.0);
         const int src = (int)((kstep + 1) & 1), dst = (int)(kstep & 1);
         d_src = a.d_buf[src];
         d_dst = a.d_buf[dst];
@@ -192,17 +209,25 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
         dt_h = a.dt_h;
     }
 
+    // Green-Ampt class table (FP32 path) staged in LDS: the per-cell class
+    // gather then costs an LDS read instead of a dependent global load
+    __shared__ float4 ga_lds[2 * kMaxSoilClasses];
+    if (MODE == kModeSim && !f64 && a.ga) {
+        for (int i = threadIdx.x; i < 2 * a.n_soil_classes; i += blockDim.x)
+            ga_lds[i] = reinterpret_cast<const float4 *>(a.ga_f32)[i];
+        __syncthreads();
+    }
+
     const T A = a.cell_area;
     const int r = slab * kRowsOut - kHalo + lane;
     const bool row_in = (r >= 0) && (r < ny);
     const bool row_own = row_in && lane >= kHalo && lane < kWave - kHalo;
-    const int c0 = blockIdx.y * a.tj;
+    const int c0 = by * a.tj;
     const int c1 = min(c0 + a.tj, nx);
     const T avail_den = dmax(dt_s, eps_of<T>());
     const T inv_dt_h = frcp(dmax(dt_h, T(1e-12)));  // FP32 path only
     const T k_avail = A * frcp(avail_den);            // FP32 path only
     const T k_cont = T(1000) * frcp(a.C_a);           // FP32 path only
-    constexpr bool f64 = sizeof(T) == 8;
 
     // pipeline registers (suffix = column lag behind the load column c)
     T h1 = 0, z1 = 0, n21 = 0, qxp1 = 0, d1 = 0, n2s1 = 0;   // column c-1
@@ -211,15 +236,16 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
 
     // per-lane reductions
     T vkey = T(NAN), wkey = T(NAN), rmax = T(NAN), dtmax = T(NAN), dpmax = T(NAN);
-    double s_store = 0.0, s_uz = 0.0, s_rain = 0.0, s_inf = 0.0;
+    double s_store = 0.0, s_uz = 0.0, s_inf = 0.0;
     int neg = 0;
 
     // 32-bit element offsets (grids are limited to 2^32 cells at creation)
     ColLoad<T> nxt = load_column<T, MODE>(a, d_src, qx_src, qy_src, I_src, Tc_src,
-                                          row_in && c0 - 2 >= 0 && c0 - 2 < nx,
+                                          active && row_in && c0 - 2 >= 0 && c0 - 2 < nx,
                                           (unsigned)r + (unsigned)(c0 - 2) * (unsigned)ny);
-#pragma unroll 2
-    for (int c = c0 - 2; c <= c1 + 1; ++c) {
+    const int c_end = active ? c1 + 1 : c0 - 3;  // inactive waves skip the march
+#pragma clang loop unroll_count(2)
+    for (int c = c0 - 2; c <= c_end; ++c) {
         const bool col_in = (c >= 0) && (c < nx);
         const bool cell_in = row_in && col_in;
         const unsigned id = (unsigned)r + (unsigned)c * (unsigned)ny;
@@ -239,12 +265,20 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
             if (nan_cell) eff = T(NAN);
             T dd = dc + eff;
             const bool own = row_own && c >= c0 && c < c1;
-            if (own && !nan_cell && !fail_step) s_rain += (double)eff;
             if (a.ga) {
                 // legacy explicit Green-Ampt closure in the Infiltration_Module slot
                 // (HydroPol2D_Main_While_Ensemble.m:243-298)
-                const GaClass g = a.ga[cur.soil];
-                const T ksat = T(g.ksat), psi = T(g.psi), dth = T(g.dtheta), Imin = a.I_min;
+                T ksat, psi, dth, Lu_mm, k_out, Tr;
+                if (f64) {
+                    const GaClass g = a.ga[cur.soil];
+                    ksat = T(g.ksat); psi = T(g.psi); dth = T(g.dtheta);
+                    Lu_mm = T(g.Lu_mm); k_out = T(g.k_out); Tr = T(g.Tr);
+                } else {
+                    const float4 g0 = ga_lds[2 * cur.soil];
+                    const float4 g1 = ga_lds[2 * cur.soil + 1];
+                    ksat = g0.x; psi = g0.y; dth = g0.z; Lu_mm = g0.w; k_out = g1.x; Tr = g1.y;
+                }
+                const T Imin = a.I_min;
                 const T Ip = cur.I;
                 T Tn = cur.Tc;
                 const T i_a = f64 ? dd / dmax(dt_h, T(1e-12)) : dd * inv_dt_h;
@@ -254,14 +288,14 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
                 if (kstep == 1) {
                     f = dmin(C, i_a);
                     In = dmax(Ip + f * dt_h, Imin);
-                    Tn = T(g.Tr);
+                    Tn = Tr;
                 } else {
-                    if (Ip > T(g.Lu_mm)) C = ksat;
+                    if (Ip > Lu_mm) C = ksat;
                     const T idxC = (i_a <= C) ? T(1) : T(0);
                     Tn = Tn - dt_h;
-                    if (i_a > C) Tn = T(g.Tr);
+                    if (i_a > C) Tn = Tr;
                     f = dmin(C, i_a);
-                    In = dmax(Ip + f * dt_h - T(g.k_out) * idxC * dt_h, Imin);
+                    In = dmax(Ip + f * dt_h - k_out * idxC * dt_h, Imin);
                     if (Tn < T(0)) In = Imin;
                 }
                 if (nan_cell) {
@@ -274,7 +308,7 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
                     Tc_dst[id] = Tn;
                     if (!fail_step) {
                         if (!isnan(f)) s_inf += (double)(f * dt_h);
-                        if (!isnan(In)) s_uz += (double)In;
+                        if (may_report && !isnan(In)) s_uz += (double)In;
                     }
                 }
             }
@@ -282,7 +316,7 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
             if (dd < T(1e-6)) dd = T(0);  // Hydrological_Model.m:49
             if (dd < T(0)) dd = T(0);     // :52
             dc = dd;
-            if (own) {  // rollback celerity inputs (HydroPol2D_Main_While.m:932)
+            if (own && fail_step) {  // rollback celerity inputs (HydroPol2D_Main_While.m:932)
                 dtmax = dmax(dtmax, dc);
                 dpmax = dmax(dpmax, dprev);
             }
@@ -293,12 +327,12 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
         const T h_up = lane_up(hc), z_up = lane_up(zc);
         T rawQyc = 0, Hfyc = 0;
         if (cell_in && r >= 1)
-            rawQyc = face_flux<T>(qyc, zc, z_up, hc, h_up, fabs(n2c), dt_s, a, Hfyc);
+            rawQyc = face_flux<T, CRIT>(qyc, zc, z_up, hc, h_up, fabs(n2c), dt_s, a, Hfyc);
 
         // ---- raw x-face flux at (r,c-1): face between columns c-1 and c ----
         T rawQx1 = 0, Hfx1 = 0;
         if (row_in && c - 1 >= 0 && c - 1 < nx - 1)
-            rawQx1 = face_flux<T>(qxp1, z1, zc, h1, hc, fabs(n21), dt_s, a, Hfx1);
+            rawQx1 = face_flux<T, CRIT>(qxp1, z1, zc, h1, hc, fabs(n21), dt_s, a, Hfx1);
 
         // ---- donor scale of cell (r,c-1): Local_Inertial_Model_D4.m:814-829 ----
         const T qy1_dn = lane_down(rawQy1);
@@ -315,15 +349,13 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
 
         // ---- limited fluxes of output column o = c-2 (:831-857) ----
         const int o = c - 2;
-        T fqx = rawQx2;
-        if (fqx > T(0)) fqx = fqx * scale2;
-        else if (fqx < T(0)) fqx = fqx * s1;
+        // donor-scaled with a select of the multiplier (a zero flux is never
+        // scaled, as the masked assignments of :834-851 leave it untouched)
+        T fqx = rawQx2 * ((rawQx2 > T(0)) ? scale2 : ((rawQx2 < T(0)) ? s1 : T(1)));
         if (o == nx - 1) fqx = T(0);
         if (!isfinite(fqx)) fqx = T(0);
         const T s2_up = lane_up(scale2);
-        T fqy = rawQy2;
-        if (fqy > T(0)) fqy = fqy * scale2;
-        else if (fqy < T(0)) fqy = fqy * s2_up;
+        T fqy = rawQy2 * ((rawQy2 > T(0)) ? scale2 : ((rawQy2 < T(0)) ? s2_up : T(1)));
         if (r == 0) fqy = T(0);
         if (!isfinite(fqy)) fqy = T(0);
         const T fqy_dn = lane_down(fqy);
@@ -415,7 +447,7 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
                         if (!isinf(cw)) wkey = dmin(wkey, a.alfa_min * a.R / (raster + cw));
                     }
                 }
-                if (!isnan(dn)) s_store += (double)dn;
+                if (may_report && !isnan(dn)) s_store += (double)dn;
             }
             if (out_cell) {
                 d_dst[oid] = dn;
@@ -467,28 +499,38 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
     }
 
     if (MODE == kModeSim) {
-        // per-wave partials (deterministic butterfly reductions); volumes are
-        // reduced as sums of depths and scaled by A/1000 by the finalize kernel
+        // per-wave butterfly reductions, then the waves of the block combined in
+        // a fixed order (deterministic); volumes are reduced as sums of depths
+        // and scaled by A/1000 by the finalize kernel
+        __shared__ double red[kPartCount][kMaxWavesPerBlock];
         const T wv_v = wave_fmax(vkey), wv_r = wave_fmax(rmax);
         const T wv_w = a.adaptive ? wave_fmax(wkey) : wave_fmin(wkey);
         const T wv_dt = wave_fmax(dtmax), wv_dp = wave_fmax(dpmax);
         const double st_sum = wave_sum(s_store), uz_sum = wave_sum(s_uz);
-        const double rn_sum = wave_sum(s_rain), in_sum = wave_sum(s_inf);
+        const double in_sum = wave_sum(s_inf);
         const int any_neg = __any(neg);
         if (lane == 0) {
-            const long long idx = (long long)blockIdx.y * a.n_slabs + slab;
-            double *P = a.partials;
-            const long long cap = a.part_cap;
-            P[kPartStorage * cap + idx] = st_sum;
-            P[kPartUz * cap + idx] = uz_sum;
-            P[kPartRain * cap + idx] = rn_sum;
-            P[kPartInfil * cap + idx] = in_sum;
-            P[kPartVmax * cap + idx] = (double)wv_v;
-            P[kPartDtw * cap + idx] = (double)wv_w;
-            P[kPartRaster * cap + idx] = (double)wv_r;
-            P[kPartDtotMax * cap + idx] = (double)wv_dt;
-            P[kPartDprevMax * cap + idx] = (double)wv_dp;
-            P[kPartNeg * cap + idx] = any_neg ? 1.0 : 0.0;
+            red[kPartStorage][wv] = st_sum;
+            red[kPartUz][wv] = uz_sum;
+            red[kPartInfil][wv] = in_sum;
+            red[kPartVmax][wv] = (double)wv_v;
+            red[kPartDtw][wv] = (double)wv_w;
+            red[kPartRaster][wv] = (double)wv_r;
+            red[kPartDtotMax][wv] = (double)wv_dt;
+            red[kPartDprevMax][wv] = (double)wv_dp;
+            red[kPartNeg][wv] = any_neg ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        if (threadIdx.x < kPartCount) {
+            const int f = threadIdx.x;
+            double acc = red[f][0];
+            for (int w = 1; w < a.waves_per_block; ++w) {
+                const double v = red[f][w];
+                if (f == kPartStorage || f == kPartUz || f == kPartInfil) acc += v;
+                else if (f == kPartDtw && !a.adaptive) acc = fmin(acc, v);
+                else acc = fmax(acc, v);
+            }
+            a.partials[(long long)f * a.part_cap + logical] = acc;
         }
     }
 }
@@ -496,11 +538,18 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
 template <typename T>
 hipError_t launch_li_step(const LiArgs<T> &a, int mode, hipStream_t s) {
     dim3 block(kWave * a.waves_per_block);
-    dim3 grid((a.n_slabs + a.waves_per_block - 1) / a.waves_per_block, (a.nx + a.tj - 1) / a.tj);
-    if (mode == kModeSim)
-        hipLaunchKernelGGL((li_step_kernel<T, kModeSim>), grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn>), grid, block, 0, s, a);
+    dim3 grid(((a.n_slabs + a.waves_per_block - 1) / a.waves_per_block) * ((a.nx + a.tj - 1) / a.tj));
+    if (mode == kModeSim) {
+        if (a.flag_critical)
+            hipLaunchKernelGGL((li_step_kernel<T, kModeSim, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((li_step_kernel<T, kModeSim, false>), grid, block, 0, s, a);
+    } else {
+        if (a.flag_critical)
+            hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn, false>), grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -30,10 +30,13 @@ template <typename T> struct LiArgs {
     // simulation mode ping-pong buffers (index = step parity)
     T *d_buf[2], *qx_buf[2], *qy_buf[2], *I_buf[2], *T_buf[2];
     const SimState *st;
+    double routing_time;  // decides on the device whether a mass-balance report can follow
     // hydrology (sim mode)
     const T *rain_pattern;
     const uint8_t *soil_class;
     const GaClass *ga;    // NULL = no infiltration
+    const float *ga_f32;  // the same table as 8 floats per class (FP32 path)
+    int n_soil_classes;
     T I_min;
     T *dmax_map;
     // outlets (sorted linear indices)

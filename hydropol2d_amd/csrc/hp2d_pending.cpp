@@ -2,7 +2,7 @@
 // are not in this build yet.  They fail loudly (HP2D_E_UNSUPPORTED).
 #include <string>
 
-#include "../../include/hp2d.h"
+#include "hp2d.h"
 
 extern "C" {
 

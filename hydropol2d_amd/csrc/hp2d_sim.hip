@@ -29,10 +29,10 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
     const long long cap = a.part_cap;
     double storage = 0, uz = 0, rain = 0, infil = 0, vmax = NAN, dtw = NAN, raster = NAN;
     double dtm = NAN, dpm = NAN, neg = 0, qsum = 0, dsum = 0;
+#pragma unroll 4
     for (int i = t; i < a.n_partials; i += kFinThreads) {
         storage += P[kPartStorage * cap + i];
         uz += P[kPartUz * cap + i];
-        rain += P[kPartRain * cap + i];
         infil += P[kPartInfil * cap + i];
         vmax = fmax(vmax, P[kPartVmax * cap + i]);
         dtw = a.adaptive ? fmax(dtw, P[kPartDtw * cap + i]) : fmin(dtw, P[kPartDtw * cap + i]);
@@ -72,7 +72,6 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
     const double a1000 = a.A / 1000.0;
     storage *= a1000;
     uz *= a1000;
-    rain *= a1000;
     infil *= a1000;
     if (a.vmax_squared) vmax = sqrt(vmax);
     dtm /= 1000.0;  // max depths were reduced in mm
@@ -85,8 +84,18 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
         dtw = isinf(cw) ? NAN : a.alfa_min * a.R / cw;
     }
 
+    if (a.dry_run) {  // launch-shape timing: reduce only, leave the state alone
+        if (t == 0) a.partials_out[0] = storage + uz + infil + vmax + dtw + raster + qsum;
+        return;
+    }
     const long long k = st->k;
     const bool failed = (st->inject_k == k);
+    // the same decision the step kernel took: storage sums exist only then
+    const double prog_max = ((st->t + st->max_dt / 60.0) / a.routing_time) * 100.0;
+    const bool may_report = (prog_max >= st->next_report) || (prog_max >= 100.0);
+    // rainfall volume of this step: the applied depth over the valid cells
+    rain = failed ? 0.0
+                  : st->delta_p_agg * (a.has_pattern ? a.sum_pattern : a.n_valid) * (a.A / 1000.0);
     const double routing_time = a.routing_time;
     if (neg > 0) {  // Hydrological_Model.m:45-47
         st->catch_index += 1;
@@ -163,8 +172,10 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
         st->cum_rain += rain;
         st->cum_inf += infil;
         st->cum_out += qsum / 1000.0 / 3600.0 * (dt_used_min * 60.0);
-        st->storage = storage;
-        st->uz = uz;
+        if (may_report) {
+            st->storage = storage;
+            st->uz = uz;
+        }
         if (a.n_dt_max > 0) a.dt_hist[st->accepted % a.n_dt_max] = tcr;
         // ---------------- advance ----------------
         t_now = tcr / 60.0 + t_now;
@@ -228,6 +239,46 @@ hipError_t launch_finalize(const FinArgs<T> &a, hipStream_t s) {
 }
 
 template hipError_t launch_finalize<float>(const FinArgs<float> &, hipStream_t);
+
+// Surface (and soil) storage sums of the current state, FP64 per-block partials
+// in a fixed order: mass_balance_check.m:62-64 outside the report cadence.
+template <typename T>
+__global__ __launch_bounds__(256) void storage_kernel(const T *d, const T *I, long long n,
+                                                      double *part) {
+    double sd = 0.0, si = 0.0;
+    for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+        const double v = (double)d[c];
+        if (!isnan(v)) sd += v;
+        if (I) {
+            const double u = (double)I[c];
+            if (!isnan(u)) si += u;
+        }
+    }
+    __shared__ double sh[2][4];
+    sd = wave_sum(sd);
+    si = wave_sum(si);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][w] = sd;
+        sh[1][w] = si;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+        part[2 * blockIdx.x + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+    }
+}
+
+template <typename T>
+hipError_t launch_storage(const T *d, const T *I, long long n, double *part, int blocks,
+                          hipStream_t s) {
+    hipLaunchKernelGGL((storage_kernel<T>), dim3(blocks), dim3(256), 0, s, d, I, n, part);
+    return hipGetLastError();
+}
+template hipError_t launch_storage<float>(const float *, const float *, long long, double *, int,
+                                          hipStream_t);
+template hipError_t launch_storage<double>(const double *, const double *, long long, double *,
+                                           int, hipStream_t);
 template hipError_t launch_finalize<double>(const FinArgs<double> &, hipStream_t);
 
 }  // namespace hp2d

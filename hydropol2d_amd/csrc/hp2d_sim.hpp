@@ -27,8 +27,13 @@ template <typename T> struct FinArgs {
     int n_dt_max;
     double *vol_err_hist;
     int n_rep_max;
+    int dry_run;           // 1: reduce the partials only (autotuner timing)
+    double *partials_out;  // sink of the dry-run reduction
 };
 
 template <typename T> hipError_t launch_finalize(const FinArgs<T> &a, hipStream_t s);
+template <typename T>
+hipError_t launch_storage(const T *d, const T *I, long long n, double *part, int blocks,
+                          hipStream_t s);
 
 }  // namespace hp2d
