@@ -13,6 +13,7 @@
 
 #include "hp2d.h"
 #include "hp2d_forcing.hpp"
+#include "hp2d_handle.hpp"
 #include "hp2d_internal.hpp"
 #include "hp2d_li.hpp"
 #include "hp2d_sim.hpp"
@@ -81,6 +82,8 @@ struct hp2d_handle {
     void *out_n = nullptr;
     int out_cap = 0;
     int tj = 16, wpb = 4;
+    void *ws = nullptr;          // module workspace (wash-off, groundwater)
+    size_t ws_bytes = 0;
 };
 
 static void set_err(hp2d_handle *h, const std::string &m) {
@@ -144,6 +147,7 @@ void hp2d_destroy(hp2d_handle *h) {
         if (p) hipFree(p);
     if (h->out_idx) hipFree(h->out_idx);
     if (h->out_n) hipFree(h->out_n);
+    if (h->ws) hipFree(h->ws);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
 }
@@ -151,6 +155,39 @@ void hp2d_destroy(hp2d_handle *h) {
 const char *hp2d_last_error(const hp2d_handle *h) { return h ? h->err.c_str() : g_last_error.c_str(); }
 
 }  // extern "C"
+
+hp2d_status hp2d_handle_shape(hp2d_handle *h, int32_t *ny, int32_t *nx, int32_t *precision) {
+    if (!h) return HP2D_E_ARG;
+    *ny = h->ny;
+    *nx = h->nx;
+    *precision = h->precision;
+    return HP2D_OK;
+}
+
+hipStream_t hp2d_handle_stream(hp2d_handle *h) { return h->stream; }
+
+hp2d_status hp2d_handle_fail(hp2d_handle *h, const std::string &msg) {
+    set_err(h, msg);
+    return HP2D_E_HIP;
+}
+
+void *hp2d_handle_workspace(hp2d_handle *h, size_t bytes) {
+    if (h->ws_bytes >= bytes) return h->ws;
+    if (h->ws) {
+        hipStreamSynchronize(h->stream);
+        hipFree(h->ws);
+        h->ws = nullptr;
+        h->ws_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&h->ws, bytes);
+    if (e != hipSuccess) {
+        h->ws = nullptr;
+        set_err(h, std::string("workspace hipMalloc: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    h->ws_bytes = bytes;
+    return h->ws;
+}
 
 template <typename T>
 static hp2d_status li_dropin(hp2d_handle *h, int scheme, const T *z, const T *d_tot,

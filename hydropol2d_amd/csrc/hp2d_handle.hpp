@@ -1,0 +1,18 @@
+// hp2d_handle.hpp - access to a drop-in handle from the per-module sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "hp2d.h"
+
+// grid shape and precision of a handle
+hp2d_status hp2d_handle_shape(hp2d_handle *h, int32_t *ny, int32_t *nx, int32_t *precision);
+// the handle's HIP stream
+hipStream_t hp2d_handle_stream(hp2d_handle *h);
+// record an error on the handle and return HP2D_E_HIP
+hp2d_status hp2d_handle_fail(hp2d_handle *h, const std::string &msg);
+// device workspace of at least `bytes` owned by the handle (grown on demand,
+// kept until hp2d_destroy); NULL on allocation failure (error recorded)
+void *hp2d_handle_workspace(hp2d_handle *h, size_t bytes);

@@ -125,3 +125,39 @@ def Local_Inertial_Model_D4(flag_numerical_scheme, reservoir_x, reservoir_y, k1,
     C_a = T(Resolution) ** 2
     return (qout_left, qout_right, qout_up, qout_down, outlet_flow, d_t, I_tot_end_cell,
             outflow_new, Hf, 0, 0, 0, 0, C_a)
+
+
+def CA_Routing(reservoir_x, reservoir_y, k1, h1, k2, k3, h2, k4, yds1, xds1, yds2, xds2,
+               flag_reservoir, elevation_cell, d_tot, roughness_cell, cell_area, time_step,
+               h_0_cell, Resolution, I_tot_end_cell, outlet_check, outlet_type, slope_outlet,
+               row_outlet, col_outlet, idx_nan, flag_critical):
+    """CA_Routing.m:1 on the gfx950 kernel (flag_reservoir = 0).
+
+    Returns (qout_left, qout_right, qout_up, qout_down, outlet_flow, d_t,
+    I_tot_end_cell).  idx_nan is taken from NaN elevations (the reference's
+    preprocessing sets both together); outlet_check is taken from the outlet
+    rows/cols.
+    """
+    if flag_reservoir == 1:
+        raise _lib.HP2DError(abi.HP2D_E_UNSUPPORTED, "reservoir branch is outside this build")
+    z = np.asfortranarray(elevation_cell, dtype=np.float64)
+    ny, nx = z.shape
+    d_tot = np.asfortranarray(d_tot, dtype=np.float64)
+    rc = np.asfortranarray(np.broadcast_to(np.asarray(roughness_cell, dtype=np.float64), (ny, nx)))
+    h0 = None
+    if not _empty_or(h_0_cell):
+        h0 = np.asfortranarray(np.broadcast_to(np.asarray(h_0_cell, dtype=np.float64), (ny, nx)))
+    Ip = np.asfortranarray(np.broadcast_to(np.asarray(I_tot_end_cell, dtype=np.float64), (ny, nx)))
+    rows = np.ascontiguousarray(np.asarray(row_outlet, dtype=np.int32).ravel())
+    cols = np.ascontiguousarray(np.asarray(col_outlet, dtype=np.int32).ravel())
+    outs = [np.zeros((ny, nx), order="F") for _ in range(7)]
+    L = _lib.load()
+    h = _Handle.get(ny, nx, abi.HP2D_FP64)
+    p = (lambda a: a.ctypes.data_as(_dp) if a is not None else _dp())
+    status = L.hp2d_ca_routing(h, p(z), p(d_tot), p(rc), float(cell_area), float(time_step), p(h0),
+                               float(Resolution), p(Ip), int(outlet_type), float(slope_outlet),
+                               rows.ctypes.data_as(_i32p), cols.ctypes.data_as(_i32p),
+                               int(min(rows.size, cols.size)), int(flag_critical),
+                               *[p(o) for o in outs])
+    _lib.check(status, handle=h)
+    return tuple(outs)

@@ -156,17 +156,22 @@ hp2d_status hp2d_ca_routing(hp2d_handle *h, const double *z, const double *d_tot
  *   build_up_wash_off(C_3,C_4,qout_left,qout_right,qout_up,qout_down,
  *                     outlet_flow,B_t,time_step,nx,ny,cell_area,outlet_index,
  *                     idx_nan_5,flag_wq_model,mass_lost,Tot_Washed,Bmin,Bmax,min_Bt)
- * build_up_wash_off.m:1-183.  C_3, C_4, B_t, Tot_Washed, outlet_mask: ny*nx
- * (outlet_mask 0/1); idx_nan given through NaN in B_t. Bmin/Bmax are used only
- * for flag_wq_model = 1.  B_t_out, P_conc, Tot_Washed_out: ny*nx.
- * scalars out: Out_Conc [mg/L], tmin_wq [s], tot_W_out [kg], mass_lost [kg]
- * (mass_lost in = *mass_lost). */
+ * replaces build_up_wash_off.m:1-183 (called at HydroPol2D_Main_While.m:685).
+ * Host arrays, column-major ny*nx doubles: C_3, C_4 (per-cell wash-off
+ * coefficient and exponent), the five flow pages [mm/h], B_t [kg],
+ * Tot_Washed [kg] (NULL = zeros).  nan_mask = idx_nan_5(:,:,1) as 0/1 bytes
+ * (NULL = none); outlet_index is not read by the routine and is not taken.
+ * time_step_min [min], cell_area [m2], Bmin/Bmax [g/m2] (flag_wq_model = 1
+ * only), min_Bt [g/m2].  Outputs ny*nx: B_t_out [kg], P_conc [mg/L],
+ * tot_W_out [kg/h] (may be NULL), Tot_Washed_out [kg]; scalars Out_Conc
+ * [mg/L], tmin_wq [s]; *mass_lost [kg] is read and updated.  More than 1e7
+ * internal sub-steps is refused with HP2D_E_HIP. */
 hp2d_status hp2d_build_up_wash_off(hp2d_handle *h, const double *C_3, const double *C_4,
                                    const double *qout_left, const double *qout_right,
                                    const double *qout_up, const double *qout_down,
                                    const double *outlet_flow, const double *B_t,
                                    double time_step_min, double cell_area,
-                                   const uint8_t *outlet_mask, int32_t flag_wq_model,
+                                   const uint8_t *nan_mask, int32_t flag_wq_model,
                                    const double *Tot_Washed, double Bmin, double Bmax,
                                    double min_Bt,
                                    double *B_t_out, double *P_conc, double *Out_Conc,

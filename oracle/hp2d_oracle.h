@@ -87,6 +87,13 @@ int32_t oracle_ca_routing(int32_t ny, int32_t nx, const double *z, const double 
                           double *qout_right, double *qout_up, double *qout_down,
                           double *outlet_flow, double *d_t, double *I_tot_end_cell);
 
+int32_t oracle_washoff(int32_t ny, int32_t nx, const double *C3, const double *C4,
+                       const double *ql, const double *qr, const double *qu, const double *qd,
+                       const double *qo, double *B_t, double time_step_min, double cell_area,
+                       const uint8_t *nan_mask, int32_t model, double *mass_lost,
+                       double *Tot_Washed, double Bmin, double Bmax, double min_Bt,
+                       double *P_conc, double *Out_Conc, double *tmin_wq, double *tot_W_out);
+
 #ifdef __cplusplus
 }
 #endif

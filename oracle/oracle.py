@@ -73,6 +73,10 @@ def lib():
                                             C.c_double, _dp, C.c_double, _dp, C.c_int32,
                                             C.c_double, _i64p, C.c_int32, C.c_int32] + [_dp] * 7
             L.oracle_ca_routing.restype = C.c_int32
+        L.oracle_washoff.argtypes = [C.c_int32, C.c_int32] + [_dp] * 8 + [C.c_double, C.c_double,
+                                     C.POINTER(C.c_uint8), C.c_int32, _dp, _dp, C.c_double,
+                                     C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.oracle_washoff.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -207,3 +211,31 @@ def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: 
     res["volume_error_hist"] = res["volume_error_hist"][: min(rep["n_reports"], n_report_max)]
     del keep
     return res
+
+
+def washoff(C3, C4, qout_left, qout_right, qout_up, qout_down, outlet_flow, B_t, time_step_min,
+            cell_area, nan_mask=None, flag_wq_model=0, mass_lost=0.0, Tot_Washed=None, Bmin=0.0,
+            Bmax=0.0, min_Bt=0.0):
+    """build_up_wash_off on the oracle; returns a dict (B_t, P_conc, Out_Conc, tmin_wq,
+    tot_W_out, mass_lost, Tot_Washed)."""
+    B = _f(B_t).copy(order="F")
+    if B.ndim == 1:
+        B = B.reshape(1, -1, order="F")
+    ny, nx = B.shape
+    sh = lambda a: np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (ny, nx)))
+    C3, C4 = sh(C3), sh(C4)
+    qs = [sh(q) for q in (qout_left, qout_right, qout_up, qout_down, outlet_flow)]
+    tw = sh(0.0 if Tot_Washed is None else Tot_Washed).copy(order="F")
+    pc = np.zeros((ny, nx), order="F")
+    tot = np.zeros((ny, nx), order="F")
+    ml = C.c_double(mass_lost)
+    oc, tm = C.c_double(), C.c_double()
+    nm = None if nan_mask is None else np.asfortranarray(np.asarray(nan_mask, dtype=np.uint8))
+    nmp = nm.ctypes.data_as(C.POINTER(C.c_uint8)) if nm is not None else C.POINTER(C.c_uint8)()
+    rc = lib().oracle_washoff(ny, nx, _p(C3), _p(C4), *[_p(q) for q in qs], _p(B), time_step_min,
+                              cell_area, nmp, flag_wq_model, C.byref(ml), _p(tw), Bmin, Bmax,
+                              min_Bt, _p(pc), C.byref(oc), C.byref(tm), _p(tot))
+    if rc != 0:
+        raise ValueError("washoff: tmin_wq gives more than 1e7 sub-steps")
+    return {"B_t": B, "P_conc": pc, "Out_Conc": oc.value, "tmin_wq": tm.value, "tot_W_out": tot,
+            "mass_lost": ml.value, "Tot_Washed": tw}

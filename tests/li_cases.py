@@ -164,6 +164,11 @@ def vtilted_case(step, n_steps_limit=None):
 
 
 def ritter_case(step, n_manning=0.0):
+    """n_manning: 0 for local inertial, 0.003 for CA (run_...validation.m:226-230)."""
+    return _ritter_case(step, n_manning)
+
+
+def _ritter_case(step, n_manning):
     """run_ritter_case, :219-296 (registered diagnostic_failed for LI: report only)."""
     dx, nx, ny, dam_x, h0, g = 2.0, 251, 5, 180.0, 1.0, 9.81
     duration_s, dt_s = 20.0, 0.05
@@ -247,5 +252,34 @@ def oracle_li_step():
         out = oracle.li_d4(z, d_tot, n, n ** 2, dx ** 2, dt_s / 60, dx, 1, outlet_slope,
                            rows, cols, state, 0)
         return out["d_t"], out["outlet_flow"], out["outflow"]
+
+    return step
+
+
+def ca_gather(qleft, qright, qup, qdown):
+    """Outflows become inflows (run_...validation.m:431-436, Main_While.m:631-645)."""
+    ny, nx = qleft.shape
+    qin = np.zeros((ny, nx))
+    qin[:, 1:nx] += qright[:, 0:nx - 1]
+    qin[:, 0:nx - 1] += qleft[:, 1:nx]
+    qin[1:ny, :] += qdown[0:ny - 1, :]
+    qin[0:ny - 1, :] += qup[1:ny, :]
+    return qin
+
+
+def oracle_ca_step():
+    """ca_step (:417-440) on the CPU oracle."""
+    from oracle import oracle
+
+    def step(z, d_tot, n, dt_s, dx, state, outlet, outlet_slope):
+        ny, nx = z.shape
+        if state is None:
+            state = np.zeros((ny, nx), order="F")
+        _, rows, cols = outlet
+        out = oracle.ca_routing(z, d_tot, n, dx ** 2, dt_s / 60, np.zeros((ny, nx)), dx, state, 1,
+                                outlet_slope, rows, cols, 0)
+        qin = ca_gather(out["qout_left"], out["qout_right"], out["qout_up"], out["qout_down"])
+        d_t = out["d_t"] + qin * (dt_s / 60) / 60
+        return d_t, out["outlet_flow"], out["I_tot_end_cell"]
 
     return step

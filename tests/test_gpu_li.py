@@ -123,3 +123,60 @@ def test_validation_thresholds_on_gpu():
     assert r["mass_abs_pct"] < 0.1 and r["symmetry_max"] < 1e-8 and r["channel_final"] > 0
     r = li_cases.stage_case(step)
     assert r["max_stage_error"] < 1e-12 and r["mass_abs_pct"] < 0.1 and r["response_amp"] > 1e-4
+
+
+# ------------------------------------------------------ CA routing -----------
+
+CA_NAMES = ["qout_left", "qout_right", "qout_up", "qout_down", "outlet_flow", "d_t",
+            "I_tot_end_cell"]
+
+
+@pytest.mark.parametrize("seed,ny,nx,otype,crit,zero_prev", [
+    (0, 33, 47, 1, 0, True), (1, 80, 61, 2, 1, False), (2, 7, 130, 2, 0, False),
+    (3, 1, 9, 1, 1, True)])
+def test_ca_dropin_matches_oracle(seed, ny, nx, otype, crit, zero_prev):
+    c = _random_case(seed, ny, nx, outlet_type=otype, critical=crit)
+    rng = np.random.default_rng(seed + 100)
+    Ip = np.zeros((ny, nx)) if zero_prev else rng.uniform(0, 5.0, (ny, nx))
+    h0 = rng.uniform(0, 2.0, (ny, nx))
+    print(f"ca case seed={seed} shape={ny}x{nx} outlet={otype} critical={crit}", flush=True)
+    L = _lim()
+    mask = np.zeros((ny, nx), dtype=bool)
+    mask[c["rows"] - 1, c["cols"] - 1] = True
+    g = L.CA_Routing([], [], [], [], [], [], [], [], [], [], [], [], 0, c["z"], c["d"], c["n"], 9.0,
+                     0.4 / 60, h0, 3.0, Ip, mask, otype, 0.004, c["rows"], c["cols"],
+                     np.isnan(c["z"]), crit)
+    o = oracle.ca_routing(c["z"], c["d"], c["n"], 9.0, 0.4 / 60, h0, 3.0, Ip, otype, 0.004,
+                          c["rows"], c["cols"], crit)
+    for k, name in enumerate(CA_NAMES):
+        gv, ov = np.asarray(g[k]), o[name]
+        np.testing.assert_array_equal(np.isnan(gv), np.isnan(ov), err_msg=name)
+        np.testing.assert_allclose(gv, ov, rtol=RTOL64,
+                                   atol=1e-9 * (np.nanmax(np.abs(ov)) + 1e-300), equal_nan=True,
+                                   err_msg=name)
+
+
+def _gpu_ca_step():
+    L = _lim()
+
+    def step(z, d_tot, n, dt_s, dx, state, outlet, outlet_slope):
+        ny, nx = z.shape
+        if state is None:
+            state = np.zeros((ny, nx), order="F")
+        mask, rows, cols = outlet
+        out = L.CA_Routing([], [], [], [], [], [], [], [], [], [], [], [], 0, z, d_tot, n, dx ** 2,
+                           dt_s / 60, np.zeros((ny, nx)), dx, state, mask, 1, outlet_slope, rows,
+                           cols, np.zeros((ny, nx), bool), 0)
+        qin = li_cases.ca_gather(out[0], out[1], out[2], out[3])
+        return out[5] + qin * (dt_s / 60) / 60, out[4], out[6]
+
+    return step
+
+
+@pytest.mark.parametrize("case", ["plane", "vtilted", "stage"])
+def test_ca_validation_cases_gpu_equal_oracle(case):
+    fn = {"plane": li_cases.plane_case, "vtilted": li_cases.vtilted_case,
+          "stage": li_cases.stage_case}[case]
+    g = fn(_gpu_ca_step(), n_steps_limit=600)
+    o = fn(li_cases.oracle_ca_step(), n_steps_limit=600)
+    np.testing.assert_allclose(g["h"], o["h"], rtol=RTOL64, atol=1e-12)

@@ -315,3 +315,46 @@ def test_duplicate_outlets_collapse_like_vectorised_assignment():
     twice = _li(z, d, n, 0.3, 2.0, rows=[1, 1], cols=[3, 3])
     np.testing.assert_array_equal(once["d_t"], twice["d_t"])
     np.testing.assert_array_equal(once["outlet_flow"], twice["outlet_flow"])
+
+
+# ------------------------------------------ A16 CA routing (CA_Routing.m) ----
+
+def test_ca_plane_and_vtilted_meet_reference_thresholds():
+    step = li_cases.oracle_ca_step()
+    r = li_cases.plane_case(step)
+    # run_...validation.m:112-116 (CA branch): mass < 0.1 % and some outflow
+    assert r["mass_abs_pct"] < 0.1 and r["q_model"].max() > 0
+    r = li_cases.vtilted_case(step)
+    assert r["mass_abs_pct"] < 0.1
+    assert r["symmetry_max"] < 1e-8
+    assert r["channel_final"] > 0
+
+
+def test_ca_stage_and_ritter_meet_reference_thresholds():
+    step = li_cases.oracle_ca_step()
+    r = li_cases.stage_case(step)
+    assert r["max_stage_error"] < 1e-12 and r["mass_abs_pct"] < 0.1 and r["response_amp"] > 1e-4
+    r = li_cases.ritter_case(step, n_manning=0.003)
+    assert r["mass_abs_pct"] < 0.1 and r["rmse"] < 0.50
+
+
+def test_ca_flat_pool_does_not_move():
+    # equal water surfaces: every depth difference is below h_min -> no transfer
+    z = np.array([[0.0, 0.25, 0.5], [0.5, 0.0, 0.25]])
+    d = (1.0 - z) * 1000
+    n = np.full(z.shape, 0.03)
+    out = oracle.ca_routing(z, d, n, 4.0, 1 / 60, None, 2.0, np.zeros(z.shape), 1, 0.0, [], [], 0)
+    assert np.all(out["I_tot_end_cell"] == 0)
+    np.testing.assert_array_equal(out["d_t"], d)
+
+
+def test_ca_weights_split_by_available_volume():
+    # single wet cell with two lower neighbours: the outflow splits in
+    # proportion to the depth differences (weights V_k / (sum V + V_min))
+    z = np.array([[0.0, 0.0, 0.0]])
+    d = np.array([[0.0, 200.0, 100.0]])
+    n = np.full(z.shape, 0.03)
+    out = oracle.ca_routing(z, d, n, 1.0, 0.5 / 60, None, 1.0, np.zeros(z.shape), 1, 0.0, [], [], 0)
+    ql, qr = out["qout_left"][0, 1], out["qout_right"][0, 1]
+    assert ql > 0 and qr > 0
+    assert ql / qr == pytest.approx(0.2 / 0.1, rel=1e-12)
