@@ -50,8 +50,9 @@ SIGNATURES = {
     "hp2d_build_up_wash_off": (C.c_int32, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                            C.c_double, C.c_double, _u8p, C.c_int32, _dp,
                                            C.c_double, C.c_double, C.c_double] + [_dp] * 7),
-    "hp2d_boussinesq_2d": (C.c_int32, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _u8p, C.c_double,
-                                       C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _i32p]),
+    "hp2d_boussinesq_2d": (C.c_int32, [_vp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp,
+                                       _dp, _dp, C.c_double, _dp, _u8p, _u8p, _dp, _dp, _dp, _dp,
+                                       _dp, _dp, _dp, _i32p]),
     "hp2d_sim_create": (C.c_int32, [C.POINTER(abi.RunConfig), C.POINTER(abi.RunInputs),
                                     C.POINTER(_vp)]),
     "hp2d_sim_destroy": (None, [_vp]),

@@ -1,0 +1,383 @@
+// hp2d_gw.hip - explicit Boussinesq groundwater solver
+// (Boussinesq_2D_explicit.m:59-195) for gfx950.
+//
+// One sub-step of the reference's predictor-corrector is five elementwise
+// kernels over the column-major grid plus two one-thread control kernels:
+//   ctl_begin  while-test, n_steps = ceil(T/dt), dt = min(dt_GW, dt)
+//   flux       Darcy face fluxes of h_t (donor-capped, left/upper donor)
+//   predict    h_p = max(h_t - dt/Sy div F1, z0), Dirichlet
+//   flux(avg)  fluxes of h_p, averaged with F1 in place
+//   update     h = h_t + dt/Sy (-div F + R), Dirichlet, >= z0, exfiltration,
+//              <= surface
+//   velocity   Darcy velocities of the new head, the catchment mask, and the
+//              next Courant limit as an atomic minimum over positive doubles
+//   ctl_end    kk / trim / t bookkeeping of the reference loop
+// The loop control lives on the device, so the host only launches batches of
+// sub-steps and reads the done flag between batches; kernels of a sub-step
+// past the end return at once.
+// The closing mass-balance sums are per-column FP64 sums in row order,
+// finished on the host in column order: the same association as the
+// reference's nansum(nansum(.)).
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "hp2d.h"
+#include "hp2d_handle.hpp"
+#include "hp2d_internal.hpp"
+
+namespace hp2d {
+
+constexpr int kGwBlock = 256;
+constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
+
+struct GwCtl {
+    double t, T, dt, dt_used, n_steps;
+    unsigned long long dtgw_bits;  // min positive finite Courant candidate, as bits
+    int done, nsub;
+};
+
+struct GwArgs {
+    int ny, nx;
+    long long n;
+    double dx, dy, courant;
+    const double *z0, *Sy, *R, *K, *h_soil, *h_dir;
+    const uint8_t *catchment, *dirichlet;  // NULL = all inside / none
+    GwCtl *ctl;
+};
+
+__device__ __forceinline__ double signum(double F) {
+    return isnan(F) ? NAN : (double)((F > 0.0) - (F < 0.0));
+}
+
+__global__ void gw_ctl_init(GwCtl *ctl, double T) {
+    ctl->t = 0.0;
+    ctl->T = T;
+    ctl->dt = T;
+    ctl->dt_used = T;
+    ctl->n_steps = 1.0;
+    ctl->dtgw_bits = kInfBits;
+    ctl->done = 0;
+    ctl->nsub = 0;
+}
+
+__global__ void gw_ctl_begin(GwCtl *ctl) {
+    if (ctl->done) return;
+    if (!(ctl->t < ctl->T)) {
+        ctl->done = 1;
+        return;
+    }
+    ctl->n_steps = ceil(ctl->T / ctl->dt);
+    const double dt_gw = __longlong_as_double((long long)ctl->dtgw_bits);
+    ctl->dt = fmin(dt_gw, ctl->dt);
+    ctl->dt_used = ctl->dt;
+    ctl->dtgw_bits = kInfBits;
+}
+
+__global__ void gw_ctl_end(GwCtl *ctl) {
+    if (ctl->done) return;
+    ctl->nsub += 1;
+    const int kk = 2;  // reset to 1 at the top of every pass, then incremented
+    if ((double)kk == ctl->n_steps - 1.0) ctl->dt = ctl->T - kk * ctl->dt;
+    ctl->t = ctl->t + ctl->dt;
+}
+
+// compute_fluxes_conservative; with avg, F := 0.5*(F_prev + F)
+__global__ __launch_bounds__(kGwBlock) void gw_flux_kernel(GwArgs a, const double *__restrict__ h,
+                                                           double *Fx, double *Fy, int avg) {
+    if (a.ctl->done) return;
+    const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
+    if (c >= a.n) return;
+    const int ny = a.ny;
+    const int i = (int)(c % ny), j = (int)(c / ny);
+    const double dt = a.ctl->dt_used;
+    const double H = fmax(h[c] - a.z0[c], 0.0);
+    double fx = 0.0, fy = 0.0;
+    if (j < a.nx - 1) {
+        const long long r = c + ny;
+        const double Hr = fmax(h[r] - a.z0[r], 0.0);
+        const double Hx = 0.5 * (H + Hr);
+        const double Kx = 0.5 * (a.K[c] + a.K[r]);
+        const double dHdx = (h[r] - h[c]) / a.dx;
+        double F = -Kx * Hx * dHdx;
+        const double cap = a.Sy[c] * H / dt;
+        F = signum(F) * fmin(fabs(F), cap);
+        if (isnan(Hr)) F = 0.0;
+        fx = F;
+    }
+    if (i < ny - 1) {
+        const long long d = c + 1;
+        const double Hd = fmax(h[d] - a.z0[d], 0.0);
+        const double Hy = 0.5 * (H + Hd);
+        const double Ky = 0.5 * (a.K[c] + a.K[d]);
+        const double dHdy = (h[d] - h[c]) / a.dy;
+        double F = -Ky * Hy * dHdy;
+        const double cap = a.Sy[c] * H / dt;
+        F = signum(F) * fmin(fabs(F), cap);
+        if (isnan(Hd)) F = 0.0;
+        fy = F;
+    }
+    if (avg) {
+        fx = 0.5 * (Fx[c] + fx);
+        fy = 0.5 * (Fy[c] + fy);
+    }
+    Fx[c] = fx;
+    Fy[c] = fy;
+}
+
+// compute_flux_divergence at cell (i, j)
+__device__ __forceinline__ double gw_div(const GwArgs &a, const double *Fx, const double *Fy,
+                                         long long c, int i, int j) {
+    double div_x, div_y;
+    if (j == 0)
+        div_x = Fx[c] / a.dx;
+    else if (j == a.nx - 1)
+        div_x = -Fx[c - a.ny] / a.dx;
+    else
+        div_x = (Fx[c] - Fx[c - a.ny]) / a.dx;
+    if (i == 0)
+        div_y = Fy[c] / a.dy;
+    else if (i == a.ny - 1)
+        div_y = -Fy[c - 1] / a.dy;
+    else
+        div_y = (Fy[c] - Fy[c - 1]) / a.dy;
+    return div_x + div_y;
+}
+
+// predictor (:113-118)
+__global__ __launch_bounds__(kGwBlock) void gw_predict_kernel(GwArgs a, const double *h_t,
+                                                              const double *Fx, const double *Fy,
+                                                              double *h_p) {
+    if (a.ctl->done) return;
+    const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
+    if (c >= a.n) return;
+    const int i = (int)(c % a.ny), j = (int)(c / a.ny);
+    const double dt = a.ctl->dt_used;
+    double v = h_t[c] + dt / a.Sy[c] * (-gw_div(a, Fx, Fy, c, i, j));
+    if (a.dirichlet && a.dirichlet[c]) v = a.h_dir[c];
+    h_p[c] = fmax(v, a.z0[c]);
+}
+
+// corrector update, Dirichlet, bed, exfiltration and surface clamp (:128-148)
+__global__ __launch_bounds__(kGwBlock) void gw_update_kernel(GwArgs a, const double *h_t,
+                                                             const double *Fx, const double *Fy,
+                                                             double *h_out, double *q_exf) {
+    if (a.ctl->done) return;
+    const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
+    if (c >= a.n) return;
+    const int i = (int)(c % a.ny), j = (int)(c / a.ny);
+    const double dt = a.ctl->dt_used;
+    double v = h_t[c] + dt / a.Sy[c] * (-gw_div(a, Fx, Fy, c, i, j) + a.R[c]);
+    if (a.dirichlet && a.dirichlet[c]) v = a.h_dir[c];
+    v = fmax(v, a.z0[c]);
+    const double hs = a.z0[c] + a.h_soil[c];
+    double q = fmax(0.0, v - hs) * a.Sy[c] / dt;
+    if (a.catchment && !a.catchment[c]) q = NAN;
+    q_exf[c] = q;
+    h_out[c] = fmin(v, hs);
+}
+
+// compute_boussinesq_velocities (+ mask) and the Courant candidates of
+// compute_stable_dt; h_masked (if given) receives h with NaN outside
+__global__ __launch_bounds__(kGwBlock) void gw_velocity_kernel(GwArgs a, const double *h,
+                                                               double *ux, double *uy,
+                                                               double *h_masked, int mask_on,
+                                                               int check_done) {
+    if (check_done && a.ctl->done) return;
+    const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
+    double best = INFINITY;
+    if (c < a.n) {
+        const int ny = a.ny;
+        const int i = (int)(c % ny), j = (int)(c / ny);
+        double vx = NAN, vy = NAN;
+        if (j < a.nx - 1) vx = -(0.5 * (a.K[c] + a.K[c + ny])) * ((h[c + ny] - h[c]) / a.dx);
+        if (i < ny - 1) vy = -(0.5 * (a.K[c] + a.K[c + 1])) * ((h[c + 1] - h[c]) / a.dy);
+        const bool out = mask_on && a.catchment && !a.catchment[c];
+        if (out) {
+            vx = NAN;
+            vy = NAN;
+        }
+        ux[c] = vx;
+        uy[c] = vy;
+        if (h_masked) h_masked[c] = out ? NAN : h[c];
+        const double cx = a.courant * a.dx / fabs(vx), cy = a.courant * a.dy / fabs(vy);
+        if (isfinite(cx) && cx > 0.0) best = fmin(best, cx);
+        if (isfinite(cy) && cy > 0.0) best = fmin(best, cy);
+    }
+    best = wave_fmin(best);
+    __shared__ double sh[kGwBlock / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sh[0];
+        for (int k = 1; k < kGwBlock / 64; ++k) m = fmin(m, sh[k]);
+        if (m < INFINITY)
+            atomicMin(&a.ctl->dtgw_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// per-column nansum of R*dt, q_exf*dt, Sy*max(h0 - z0, 0), Sy*max(h - z0, 0)
+__global__ __launch_bounds__(kGwBlock) void gw_colsum_kernel(GwArgs a, const double *q_exf,
+                                                             const double *h0, const double *h,
+                                                             double *out) {
+    const int j = blockIdx.x * kGwBlock + threadIdx.x;
+    if (j >= a.nx) return;
+    const double dt = a.ctl->dt_used;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int i = 0; i < a.ny; ++i) {
+        const long long c = (long long)j * a.ny + i;
+        const double r = a.R[c] * dt;
+        const double q = q_exf[c] * dt;
+        const double p = a.Sy[c] * fmax(h0[c] - a.z0[c], 0.0);
+        const double u = a.Sy[c] * fmax(h[c] - a.z0[c], 0.0);
+        if (!isnan(r)) s0 += r;
+        if (!isnan(q)) s1 += q;
+        if (!isnan(p)) s2 += p;
+        if (!isnan(u)) s3 += u;
+    }
+    out[4 * j + 0] = s0;
+    out[4 * j + 1] = s1;
+    out[4 * j + 2] = s2;
+    out[4 * j + 3] = s3;
+}
+
+}  // namespace hp2d
+
+extern "C" hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, double dt, double dx, double dy,
+                                          const double *h_in, const double *z0, const double *Sy,
+                                          const double *R, const double *K, double courant,
+                                          const double *h_soil, const uint8_t *catchment_mask,
+                                          const uint8_t *dirichlet_mask, const double *h_dirichlet,
+                                          double *h_t, double *u_x, double *u_y, double *q_exf,
+                                          double *q_river, double *error, int32_t *n_substeps) {
+    using namespace hp2d;
+    if (!h || !h_in || !z0 || !Sy || !R || !K || !h_soil || !h_t || !q_exf || !error ||
+        !(dt > 0.0) || !(dx > 0.0) || !(dy > 0.0) || !(courant > 0.0) ||
+        (dirichlet_mask && !h_dirichlet))
+        return HP2D_E_ARG;
+    int32_t ny = 0, nx = 0, prec = 0;
+    hp2d_status st = hp2d_handle_shape(h, &ny, &nx, &prec);
+    if (st != HP2D_OK) return st;
+    if (ny < 2 || nx < 2) return HP2D_E_ARG;  // the reference indexes column/row 2
+    const size_t n = (size_t)ny * nx;
+    const int blocks = (int)((n + kGwBlock - 1) / kGwBlock);
+    const int cblocks = (nx + kGwBlock - 1) / kGwBlock;
+    hipStream_t s = hp2d_handle_stream(h);
+
+    // workspace: z0 Sy R K h_soil h_dir h0 h_t h_p h_tmp Fx Fy ux uy q | colsums | ctl | masks
+    const size_t n_arrays = 15;
+    const size_t col_off = n_arrays * n * sizeof(double);
+    const size_t ctl_off = col_off + 4 * (size_t)nx * sizeof(double);
+    const size_t mask_off = ctl_off + 256;
+    char *ws = (char *)hp2d_handle_workspace(h, mask_off + 2 * n + 256);
+    if (!ws) return HP2D_E_NOMEM;
+    double *p = (double *)ws;
+    auto take = [&]() {
+        double *r = p;
+        p += n;
+        return r;
+    };
+    double *dz0 = take(), *dSy = take(), *dR = take(), *dK = take(), *dhs = take(),
+           *dhd = take(), *dh0 = take(), *dht = take(), *dhp = take(), *dtmp = take(),
+           *dFx = take(), *dFy = take(), *dux = take(), *duy = take(), *dq = take();
+    double *dcol = (double *)(ws + col_off);
+    GwCtl *ctl = (GwCtl *)(ws + ctl_off);
+    uint8_t *dcm = catchment_mask ? (uint8_t *)(ws + mask_off) : nullptr;
+    uint8_t *ddm = dirichlet_mask ? (uint8_t *)(ws + mask_off + n) : nullptr;
+
+    hipError_t e = hipSuccess;
+    auto H2D = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    };
+    auto D2H = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    };
+    const size_t nb = n * sizeof(double);
+    H2D(dz0, z0, nb);
+    H2D(dSy, Sy, nb);
+    H2D(dR, R, nb);
+    H2D(dK, K, nb);
+    H2D(dhs, h_soil, nb);
+    H2D(dh0, h_in, nb);
+    H2D(dht, h_in, nb);
+    if (ddm) {
+        H2D(dhd, h_dirichlet, nb);
+        H2D(ddm, dirichlet_mask, n);
+    }
+    if (dcm) H2D(dcm, catchment_mask, n);
+
+    GwArgs a;
+    a.ny = ny;
+    a.nx = nx;
+    a.n = (long long)n;
+    a.dx = dx;
+    a.dy = dy;
+    a.courant = courant;
+    a.z0 = dz0;
+    a.Sy = dSy;
+    a.R = dR;
+    a.K = dK;
+    a.h_soil = dhs;
+    a.h_dir = dhd;
+    a.catchment = dcm;
+    a.dirichlet = ddm;
+    a.ctl = ctl;
+
+    const dim3 g(blocks), b(kGwBlock);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gw_ctl_init, dim3(1), dim3(1), 0, s, ctl, dt);
+        // velocities of the entry head (not masked, :91)
+        hipLaunchKernelGGL(gw_velocity_kernel, g, b, 0, s, a, dht, dux, duy, (double *)nullptr, 0,
+                           0);
+        e = hipGetLastError();
+    }
+    GwCtl hc{};
+    const int kMaxSub = 1000000;
+    int batch = 1;
+    while (e == hipSuccess) {
+        for (int k = 0; k < batch; ++k) {
+            hipLaunchKernelGGL(gw_ctl_begin, dim3(1), dim3(1), 0, s, ctl);
+            hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)dht, dFx, dFy, 0);
+            hipLaunchKernelGGL(gw_predict_kernel, g, b, 0, s, a, (const double *)dht,
+                               (const double *)dFx, (const double *)dFy, dhp);
+            hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)dhp, dFx, dFy, 1);
+            hipLaunchKernelGGL(gw_update_kernel, g, b, 0, s, a, (const double *)dht,
+                               (const double *)dFx, (const double *)dFy, dtmp, dq);
+            hipLaunchKernelGGL(gw_velocity_kernel, g, b, 0, s, a, (const double *)dtmp, dux, duy,
+                               dht, 1, 1);
+            hipLaunchKernelGGL(gw_ctl_end, dim3(1), dim3(1), 0, s, ctl);
+        }
+        e = hipGetLastError();
+        D2H(&hc, ctl, sizeof(GwCtl));
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess || hc.done) break;
+        if (hc.nsub > kMaxSub)
+            return hp2d_handle_fail(h, "hp2d_boussinesq_2d: more than 1e6 sub-steps");
+        batch = batch < 64 ? batch * 2 : 64;
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gw_colsum_kernel, dim3(cblocks), b, 0, s, a, (const double *)dq,
+                           (const double *)dh0, (const double *)dht, dcol);
+        e = hipGetLastError();
+    }
+    std::vector<double> col(4 * (size_t)nx);
+    D2H(col.data(), dcol, col.size() * sizeof(double));
+    D2H(h_t, dht, nb);
+    D2H(q_exf, dq, nb);
+    if (u_x) D2H(u_x, dux, nb);
+    if (u_y) D2H(u_y, duy, nb);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return hp2d_handle_fail(h, std::string("hp2d_boussinesq_2d: ") + hipGetErrorString(e));
+    double sums[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < nx; ++j)
+        for (int k = 0; k < 4; ++k) sums[k] += col[4 * (size_t)j + k];
+    const double A = dx * dy;
+    const double recharge_mass = sums[0] * A, exfil_mass = sums[1] * A;
+    const double storage_prev = sums[2] * A, storage_curr = sums[3] * A;
+    *error = recharge_mass - exfil_mass - (storage_curr - storage_prev);
+    if (q_river)
+        for (size_t c = 0; c < n; ++c) q_river[c] = 0.0;
+    if (n_substeps) *n_substeps = hc.nsub;
+    return HP2D_OK;
+}

@@ -178,18 +178,26 @@ hp2d_status hp2d_build_up_wash_off(hp2d_handle *h, const double *C_3, const doub
                                    double *tmin_wq, double *tot_W_out, double *mass_lost,
                                    double *Tot_Washed_out);
 
-/* [h_new, q_exf, dt_used, n_sub] = Boussinesq_2D_explicit(...)
- * Boussinesq_2D_explicit.m:1-195 on the coarse grid (no river coupling):
- * h (head) [m], z0 (aquifer base) [m], K [m/s], Sy [-], R (recharge) [m/s],
- * h_surf (surface elevation) [m]: ny*nx; dirichlet_mask 0/1 or NULL; NaN in
- * z0 marks inactive cells.  T_total [s] is integrated with internal
- * Courant-limited sub-steps.  Outputs h_new [m], q_exf [m/s] ny*nx. */
-hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, const double *h_in, const double *z0,
-                               const double *K, const double *Sy, const double *R,
-                               const double *h_surf, const uint8_t *dirichlet_mask,
-                               double dx, double dy, double T_total, double courant,
-                               double *h_out, double *q_exf, double *dt_used_last,
-                               int32_t *n_substeps);
+/* [h_t,u_x,u_y,q_exf,q_river,error] = Boussinesq_2D_explicit(dt,dx,dy,h,z0,Sy,R,K,
+ *     river_mask,K_river,h_river,z_river,Courant,h_soil,catchment_mask,
+ *     dirichlet_mask,h_dirichlet,perimeter)
+ * replaces Boussinesq_2D_explicit.m:1-195 (called at Groundwater_Module.m:229-247 and by
+ * Validation/Groundwater/Hillslope_Analytical/run_hillslope_analytical_validation.m:81,159).
+ * Host arrays, column-major ny*nx doubles (ny, nx >= 2): h [m], z0 (aquifer base) [m],
+ * Sy [-], R [m/s], K [m/s], h_soil [m] (surface = z0 + h_soil).  dt [s] is the span
+ * integrated with the reference's Courant-limited sub-steps.  catchment_mask /
+ * dirichlet_mask: 0/1 bytes or NULL (all inside / none); h_dirichlet required with a
+ * Dirichlet mask.  The river arguments and the perimeter are not read by the reference
+ * (q_river is zeros) and are not taken.  Outputs ny*nx: h_t [m] (NaN outside the
+ * catchment), u_x, u_y [m/s] (may be NULL), q_exf [m/s], q_river (zeros, may be NULL);
+ * error [m3] as the reference computes it; n_substeps (may be NULL). */
+hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, double dt, double dx, double dy,
+                               const double *h_in, const double *z0, const double *Sy,
+                               const double *R, const double *K, double courant,
+                               const double *h_soil, const uint8_t *catchment_mask,
+                               const uint8_t *dirichlet_mask, const double *h_dirichlet,
+                               double *h_t, double *u_x, double *u_y, double *q_exf,
+                               double *q_river, double *error, int32_t *n_substeps);
 
 /* ------------------------------------------------------------------------ */
 /* Device-resident step loop (the performance path)                         */

@@ -94,6 +94,19 @@ int32_t oracle_washoff(int32_t ny, int32_t nx, const double *C3, const double *C
                        double *Tot_Washed, double Bmin, double Bmax, double min_Bt,
                        double *P_conc, double *Out_Conc, double *tmin_wq, double *tot_W_out);
 
+int32_t oracle_boussinesq_2d(int32_t ny, int32_t nx, double dt, double dx, double dy,
+                             const double *h, const double *z0, const double *Sy, const double *R,
+                             const double *K, double courant, const double *h_soil,
+                             const uint8_t *catchment, const uint8_t *dirichlet,
+                             const double *h_dirichlet, double *h_t, double *u_x, double *u_y,
+                             double *q_exf, double *error, int32_t *n_sub);
+
+int32_t oracle_gw_recharge(int64_t n, const double *infiltration_rate, const double *S0,
+                           double dt, const double *min_S, const double *max_S,
+                           const double *theta_s, const double *theta_r, const double *n_vg,
+                           const double *alpha_vg, double l_vg, const double *ksat_mmh,
+                           double *recharge_rate, double *S_out, double *cumulative_recharge);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,14 @@ def lib():
                                      C.POINTER(C.c_uint8), C.c_int32, _dp, _dp, C.c_double,
                                      C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.oracle_washoff.restype = C.c_int32
+        _u8 = C.POINTER(C.c_uint8)
+        L.oracle_boussinesq_2d.argtypes = ([C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                            C.c_double] + [_dp] * 5 + [C.c_double, _dp, _u8, _u8,
+                                            _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)])
+        L.oracle_boussinesq_2d.restype = C.c_int32
+        L.oracle_gw_recharge.argtypes = ([C.c_int64, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp,
+                                          _dp, _dp, C.c_double, _dp, _dp, _dp, _dp])
+        L.oracle_gw_recharge.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -239,3 +247,50 @@ def washoff(C3, C4, qout_left, qout_right, qout_up, qout_down, outlet_flow, B_t,
         raise ValueError("washoff: tmin_wq gives more than 1e7 sub-steps")
     return {"B_t": B, "P_conc": pc, "Out_Conc": oc.value, "tmin_wq": tm.value, "tot_W_out": tot,
             "mass_lost": ml.value, "Tot_Washed": tw}
+
+
+def _mask(m, shape):
+    if m is None:
+        return None, C.POINTER(C.c_uint8)()
+    a = np.asfortranarray(np.broadcast_to(np.asarray(m, dtype=bool), shape).astype(np.uint8))
+    return a, a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def boussinesq_2d(dt, dx, dy, h, z0, Sy, R, K, courant, h_soil, catchment_mask=None,
+                  dirichlet_mask=None, h_dirichlet=None):
+    """Boussinesq_2D_explicit on the oracle; returns a dict (h_t, u_x, u_y, q_exf,
+    error, n_sub).  Scalars broadcast to the grid of h."""
+    h = _f(h)
+    ny, nx = h.shape
+    g = lambda a: np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (ny, nx)))
+    z0, Sy, R, K, hs = g(z0), g(Sy), g(R), g(K), g(h_soil)
+    hd = g(0.0 if h_dirichlet is None else h_dirichlet)
+    cm, cmp = _mask(catchment_mask, (ny, nx))
+    dm, dmp = _mask(dirichlet_mask, (ny, nx))
+    out = {k: np.zeros((ny, nx), order="F") for k in ("h_t", "u_x", "u_y", "q_exf")}
+    err, nsub = C.c_double(), C.c_int32()
+    rc = lib().oracle_boussinesq_2d(ny, nx, dt, dx, dy, _p(h), _p(z0), _p(Sy), _p(R), _p(K),
+                                    courant, _p(hs), cmp, dmp, _p(hd), _p(out["h_t"]),
+                                    _p(out["u_x"]), _p(out["u_y"]), _p(out["q_exf"]),
+                                    C.byref(err), C.byref(nsub))
+    if rc != 0:
+        raise ValueError(f"boussinesq_2d: oracle returned {rc}")
+    out["error"] = err.value
+    out["n_sub"] = nsub.value
+    return out
+
+
+def gw_recharge(infiltration_rate, S0, dt, min_S, max_S, theta_s, theta_r, n_vg, alpha_vg,
+                ksat_mmh, l_vg=0.5, cumulative=None):
+    """simulate_groundwater_recharge on the oracle; returns (R_mb [m/s], S_new [m], cumulative [mm])."""
+    S0 = _f(S0)
+    shape = S0.shape
+    g = lambda a: np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), shape))
+    args = [g(x) for x in (infiltration_rate, S0, min_S, max_S, theta_s, theta_r, n_vg,
+                           alpha_vg, ksat_mmh)]
+    Rr, Sn = np.zeros(shape, order="F"), np.zeros(shape, order="F")
+    cum = g(0.0 if cumulative is None else cumulative).copy(order="F")
+    lib().oracle_gw_recharge(S0.size, _p(args[0]), _p(args[1]), dt, _p(args[2]), _p(args[3]),
+                             _p(args[4]), _p(args[5]), _p(args[6]), _p(args[7]), l_vg,
+                             _p(args[8]), _p(Rr), _p(Sn), _p(cum))
+    return Rr, Sn, cum
