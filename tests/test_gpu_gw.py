@@ -52,7 +52,7 @@ def _same(got, ref):
     for k in ("h_t", "u_x", "u_y", "q_exf"):
         np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
     assert got["n_sub"] == ref["n_sub"]
-    scale = np.nansum(ref["h_t"]) + 1.0
+    scale = np.nansum(np.abs(ref["h_t"])) + 1.0
     assert abs(got["error"] - ref["error"]) <= 1e-12 * scale * 1e3
 
 
