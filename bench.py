@@ -32,9 +32,76 @@ def parse():
     p.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     p.add_argument("--graph-steps", type=int, default=50)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--cpu-steps", type=int, default=6)
     p.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
+    p.add_argument("--ensemble-members", type=int, default=64,
+                   help="members of the batched-ensemble measurement (0 skips it)")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="only run 5 device copies of MIB MiB (the known-size kernel that "
+                        "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return p.parse_args()
+
+
+PROFILE_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "li_step_kernel_4096_fp32.md")
+
+
+def measured_traffic():
+    """HBM bytes per li_step_kernel launch from the committed counter summary
+    (a line `traffic_bytes_per_launch = N`), or None when it is absent."""
+    try:
+        with open(PROFILE_SUMMARY) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1].split()[0])
+    except (OSError, ValueError, IndexError):
+        return None
+    return None
+
+
+def calibrate_copy(mib):
+    import torch
+    n = mib * 1024 * 1024 // 4
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"[bench] copy calibration: 5 copies of {n * 4} bytes", flush=True)
+
+
+def ensemble_throughput(n_members):
+    """Members/hour of the batched ensemble against the plain per-member GPU
+    solver (the same device loop, one member after another), on C4-style
+    members: grid sides log-uniform in [128, 512], dx = 5 m, perturbed n, Ks
+    and rainfall, 2 h simulated each."""
+    from hydropol2d_amd import synthetic
+    from hydropol2d_amd.simulation import Ensemble, Simulation
+    specs = synthetic.ensemble_c4(n_members, seed=1000, min_log2=7, max_log2=9,
+                                  routing_time_min=120.0)
+    cells = sum(sp.shape[0] * sp.shape[1] for sp in specs)
+    with Ensemble(specs) as ens:
+        t0 = time.perf_counter()
+        ens.run(graph_steps=50)
+        t_batch = time.perf_counter() - t0
+        peaks = [ens.member(i).state()["dmax"] for i in range(len(specs))]
+        steps = [ens.member(i).report()["steps"] for i in range(len(specs))]
+    t_plain = 0.0
+    max_diff = 0.0
+    for i, sp in enumerate(specs):
+        with Simulation(sp) as sim:
+            t0 = time.perf_counter()
+            sim.run()
+            t_plain += time.perf_counter() - t0
+            max_diff = max(max_diff, float(np.nanmax(np.abs(sim.state()["dmax"] - peaks[i]))) / 1000)
+    print(f"[bench] ensemble {n_members} members: batched {t_batch:.2f} s, plain {t_plain:.2f} s",
+          file=sys.stderr, flush=True)
+    return {"members": n_members, "cells": cells, "member_steps": int(sum(steps)),
+            "members_per_hour_batched": n_members / t_batch * 3600.0,
+            "members_per_hour_plain": n_members / t_plain * 3600.0,
+            "speedup": t_plain / t_batch, "max_peak_depth_diff_m": max_diff,
+            "workload": "C4-style: sides 2^U(7,9), dx=5 m, fractal DEM on a 1 % tilt, n/Ks/rain "
+                        "x e^U(ln .5, ln 1.5) around C1, 2 h each, FP32"}
 
 
 def algorithmic_bytes_per_cell(spec, elem):
@@ -53,6 +120,9 @@ def algorithmic_bytes_per_cell(spec, elem):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "decomposition is a design note, not a measured path.", file=sys.stderr)
@@ -71,13 +141,14 @@ def main():
     sim.run_fixed(args.warmup, graph_steps=min(args.graph_steps, max(args.warmup, 1)))
     tune = {"tj": None, "waves_per_block": None} if args.no_tune else sim.autotune()
     print(f"[bench] warm-up done, launch shape {tune}", file=sys.stderr, flush=True)
-    # timed region: K whole steps through captured graphs
+    # timed region: K whole steps through captured graphs; run_fixed (like the
+    # autotuner before it) returns only after synchronising the simulation's
+    # stream, so the device is idle on both sides of the wall-clock interval
     t1 = time.perf_counter()
     sim.run_fixed(args.steps, graph_steps=args.graph_steps)
     t2 = time.perf_counter()
-    _, total_ms, _ = sim.kernel_time()
-    wall_s = t2 - t1
-    elapsed_s = total_ms / 1000.0 if total_ms > 0 else wall_s
+    _, graph_ms, _ = sim.kernel_time()
+    elapsed_s = t2 - t1
     ms_per_step = elapsed_s * 1000.0 / args.steps
     value = cells * args.steps / elapsed_s
     # dominant kernel: average duration from HIP events around each launch on the sim stream
@@ -104,6 +175,8 @@ def main():
                          f"oracle, single thread, {cpu_s:.1f} s"}
         print(f"[bench] cpu oracle {cpu['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
 
+    ens = ensemble_throughput(args.ensemble_members) if args.ensemble_members > 0 else None
+
     line = {
         "metric": "cell-updates/s (% HBM roofline)",
         "value": value,
@@ -125,10 +198,12 @@ def main():
                    "launch": {"cols_per_wave": tune["tj"], "waves_per_block": tune["waves_per_block"]},
                    "bytes_per_cell_update": rd + wr},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "algorithmic_bytes": bytes_per_launch,
                      "kernel": "li_step_kernel", "avg_kernel_ms": li_ms,
-                     "launches_timed": launches},
+                     "launches_timed": launches, "graph_event_ms": graph_ms},
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(line), flush=True)
 

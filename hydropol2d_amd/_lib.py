@@ -53,6 +53,16 @@ SIGNATURES = {
     "hp2d_boussinesq_2d": (C.c_int32, [_vp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp,
                                        _dp, _dp, C.c_double, _dp, _u8p, _u8p, _dp, _dp, _dp, _dp,
                                        _dp, _dp, _dp, _i32p]),
+    "hp2d_sim_record_time_to_peak": (C.c_int32, [_vp]),
+    "hp2d_sim_get_time_to_peak": (C.c_int32, [_vp, _dp]),
+    "hp2d_ensemble_create": (C.c_int32, [C.c_int32, C.POINTER(abi.RunConfig),
+                                         C.POINTER(abi.RunInputs), C.c_int32, C.POINTER(_vp)]),
+    "hp2d_ensemble_destroy": (None, [_vp]),
+    "hp2d_ensemble_last_error": (C.c_char_p, [_vp]),
+    "hp2d_ensemble_run": (C.c_int32, [_vp, C.c_int64, C.c_int32, _i32p]),
+    "hp2d_ensemble_member": (C.c_int32, [_vp, C.c_int32, C.POINTER(_vp)]),
+    "hp2d_ensemble_record_time_to_peak": (C.c_int32, [_vp]),
+    "hp2d_ensemble_elapsed": (C.c_int32, [_vp, _dp]),
     "hp2d_sim_create": (C.c_int32, [C.POINTER(abi.RunConfig), C.POINTER(abi.RunInputs),
                                     C.POINTER(_vp)]),
     "hp2d_sim_destroy": (None, [_vp]),

@@ -367,7 +367,7 @@ struct hp2d_sim {
     hipStream_t stream = nullptr;
     std::string err;
     // device memory
-    void *z = nullptr, *n2s = nullptr, *pattern = nullptr, *dmax = nullptr;
+    void *z = nullptr, *n2s = nullptr, *pattern = nullptr, *dmax = nullptr, *tpeak = nullptr;
     void *d[2] = {nullptr, nullptr}, *qx[2] = {nullptr, nullptr}, *qy[2] = {nullptr, nullptr};
     void *I[2] = {nullptr, nullptr}, *Tc[2] = {nullptr, nullptr};
     uint8_t *soil = nullptr;
@@ -453,6 +453,7 @@ template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
     a.routing_time = c.routing_time_min;
     a.I_min = T(s->I_min);
     a.dmax_map = (T *)s->dmax;
+    a.tpeak_map = (T *)s->tpeak;
     a.out_idx = s->out_idx;
     a.out_n = (const T *)s->out_n;
     a.n_outlet = s->n_outlet;
@@ -799,7 +800,8 @@ void hp2d_sim_destroy(hp2d_sim *s) {
     void *ps[] = {s->z, s->n2s, s->pattern, s->dmax, s->d[0], s->d[1], s->qx[0], s->qx[1],
                   s->qy[0], s->qy[1], s->I[0], s->I[1], s->Tc[0], s->Tc[1], s->soil, s->ga, s->ga_f32,
                   s->out_idx, s->out_n, s->out_q, s->out_d, s->partials, s->st, s->rain_t,
-                  s->rain_v, s->hq, s->ht, s->hd, s->dt_hist, s->vol_err, s->tune_sink};
+                  s->rain_v, s->hq, s->ht, s->hd, s->dt_hist, s->vol_err, s->tune_sink,
+                  s->tpeak};
     for (void *p : ps)
         if (p) hipFree(p);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -1256,3 +1258,375 @@ extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t 
     if (best_ms) *best_ms = best;
     return HP2D_OK;
 }
+
+// ============================================================================
+// Time to peak (derived output: no reference counterpart)
+// ============================================================================
+extern "C" {
+
+hp2d_status hp2d_sim_record_time_to_peak(hp2d_sim *s) {
+    if (!s) return HP2D_E_ARG;
+    if (s->tpeak) return HP2D_OK;
+    STRY(s, hipMalloc(&s->tpeak, s->n * s->elem));
+    STRY(s, hipMemsetAsync(s->tpeak, 0, s->n * s->elem, s->stream));
+    if (s->graph) {  // captured launches carry the old arguments
+        hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+        s->graph_steps = 0;
+    }
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_sim_get_time_to_peak(hp2d_sim *s, double *tpeak_min) {
+    if (!s || !tpeak_min) return HP2D_E_ARG;
+    if (!s->tpeak) {
+        set_err(s, "hp2d_sim_record_time_to_peak was not called");
+        return HP2D_E_ARG;
+    }
+    STRY(s, hipStreamSynchronize(s->stream));
+    return fetch_real(s, tpeak_min, s->tpeak, 0.0);
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Ensemble: many independent runs advanced by one batched launch per phase
+// ============================================================================
+struct hp2d_ensemble {
+    std::vector<hp2d_sim *> m;
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t elem = 4;
+    int wpb = 4, critical = 0;
+    void *li_args = nullptr, *fin_args = nullptr;  // per-member argument tables
+    int *tile_grid = nullptr, *first_tile = nullptr, *active_ids = nullptr;
+    int *done = nullptr;
+    SimState **states = nullptr;
+    std::vector<int> active;
+    int n_tiles = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    double last_ms = 0;
+};
+
+static void set_err(hp2d_ensemble *e, const std::string &m) {
+    if (e) e->err = m;
+    g_last_error = m;
+}
+
+#define ETRY(e, expr)                                                                 \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess) {                                                       \
+            set_err(e, std::string(#expr) + ": " + hipGetErrorString(_e));            \
+            return HP2D_E_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+namespace {
+
+__global__ void gather_done_kernel(SimState *const *states, int n, int *done) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) done[i] = (states[i]->done || states[i]->paused) ? 1 : 0;
+}
+
+int member_tiles(const hp2d_sim *s) {
+    return (((s->ny + kRowsOut - 1) / kRowsOut + s->wpb - 1) / s->wpb) *
+           ((s->nx + s->tj - 1) / s->tj);
+}
+
+// tile table of the active members: tiles of one member are contiguous
+hp2d_status build_tiles(hp2d_ensemble *e) {
+    std::vector<int> tg, first(e->m.size(), 0);
+    for (int id : e->active) {
+        first[id] = (int)tg.size();
+        tg.insert(tg.end(), member_tiles(e->m[id]), id);
+    }
+    e->n_tiles = (int)tg.size();
+    if (!tg.empty())
+        ETRY(e, hipMemcpyAsync(e->tile_grid, tg.data(), tg.size() * 4, hipMemcpyHostToDevice,
+                               e->stream));
+    ETRY(e, hipMemcpyAsync(e->first_tile, first.data(), first.size() * 4, hipMemcpyHostToDevice,
+                           e->stream));
+    if (!e->active.empty())
+        ETRY(e, hipMemcpyAsync(e->active_ids, e->active.data(), e->active.size() * 4,
+                               hipMemcpyHostToDevice, e->stream));
+    ETRY(e, hipStreamSynchronize(e->stream));
+    if (e->graph) {
+        hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+        e->graph_steps = 0;
+    }
+    return HP2D_OK;
+}
+
+hipError_t ensemble_step(hp2d_ensemble *e) {
+    hipError_t r;
+    const int na = (int)e->active.size();
+    if (e->elem == 4) {
+        r = launch_li_step_batch<float>((const LiArgs<float> *)e->li_args, e->tile_grid,
+                                        e->first_tile, e->n_tiles, e->wpb, e->critical, e->stream);
+        if (r == hipSuccess)
+            r = launch_finalize_batch<float>((const FinArgs<float> *)e->fin_args, e->active_ids, na,
+                                             e->stream);
+    } else {
+        r = launch_li_step_batch<double>((const LiArgs<double> *)e->li_args, e->tile_grid,
+                                         e->first_tile, e->n_tiles, e->wpb, e->critical, e->stream);
+        if (r == hipSuccess)
+            r = launch_finalize_batch<double>((const FinArgs<double> *)e->fin_args, e->active_ids,
+                                              na, e->stream);
+    }
+    return r;
+}
+
+hp2d_status ensemble_graph(hp2d_ensemble *e, int steps) {
+    if (e->graph && e->graph_steps == steps) return HP2D_OK;
+    if (e->graph) {
+        hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    ETRY(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < steps; ++i) {
+        hipError_t r = ensemble_step(e);
+        if (r != hipSuccess) {
+            hipGraph_t dummy = nullptr;
+            hipStreamEndCapture(e->stream, &dummy);
+            if (dummy) hipGraphDestroy(dummy);
+            set_err(e, std::string("ensemble graph capture: ") + hipGetErrorString(r));
+            return HP2D_E_HIP;
+        }
+    }
+    ETRY(e, hipStreamEndCapture(e->stream, &g));
+    hipError_t r = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (r != hipSuccess) {
+        e->graph = nullptr;
+        set_err(e, std::string("hipGraphInstantiate: ") + hipGetErrorString(r));
+        return HP2D_E_HIP;
+    }
+    e->graph_steps = steps;
+    return HP2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void hp2d_ensemble_destroy(hp2d_ensemble *e) {
+    if (!e) return;
+    if (e->stream) hipStreamSynchronize(e->stream);
+    if (e->graph) hipGraphExecDestroy(e->graph);
+    void *ps[] = {e->li_args, e->fin_args, e->tile_grid, e->first_tile, e->active_ids, e->done,
+                  e->states};
+    for (void *p : ps)
+        if (p) hipFree(p);
+    for (hp2d_sim *s : e->m) hp2d_sim_destroy(s);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+}
+
+const char *hp2d_ensemble_last_error(const hp2d_ensemble *e) {
+    return e ? e->err.c_str() : g_last_error.c_str();
+}
+
+hp2d_status hp2d_ensemble_create(int32_t n_members, const hp2d_run_config *cfgs,
+                                 const hp2d_run_inputs *inputs, int32_t cols_per_wave,
+                                 hp2d_ensemble **out) {
+    if (!out || n_members < 1 || !cfgs || !inputs) {
+        g_last_error = "hp2d_ensemble_create: bad arguments";
+        return HP2D_E_ARG;
+    }
+    if (cols_per_wave != 0 && cols_per_wave != 16 && cols_per_wave != 32 && cols_per_wave != 64 &&
+        cols_per_wave != 128) {
+        g_last_error = "hp2d_ensemble_create: cols_per_wave must be 0, 16, 32, 64 or 128";
+        return HP2D_E_ARG;
+    }
+    for (int i = 1; i < n_members; ++i) {
+        if (cfgs[i].precision != cfgs[0].precision ||
+            cfgs[i].flag_critical != cfgs[0].flag_critical) {
+            g_last_error = "hp2d_ensemble_create: members must share precision and flag_critical";
+            return HP2D_E_ARG;
+        }
+    }
+    hp2d_ensemble *e = new hp2d_ensemble();
+    e->elem = cfgs[0].precision == HP2D_FP64 ? 8 : 4;
+    e->critical = cfgs[0].flag_critical;
+    for (int i = 0; i < n_members; ++i) {
+        hp2d_sim *s = nullptr;
+        hp2d_status rc = hp2d_sim_create(&cfgs[i], &inputs[i], &s);
+        if (rc != HP2D_OK) {
+            e->err = "member " + std::to_string(i) + ": " + g_last_error;
+            g_last_error = e->err;
+            hp2d_ensemble_destroy(e);
+            return rc;
+        }
+        s->wpb = e->wpb;
+        if (cols_per_wave) s->tj = cols_per_wave;
+        e->m.push_back(s);
+    }
+    const size_t nm = e->m.size();
+    size_t tiles = 0;
+    for (hp2d_sim *s : e->m) {
+        const int nt = member_tiles(s);
+        if ((long long)nt > s->part_cap) {
+            set_err(e, "member partials smaller than its tile count");
+            hp2d_ensemble_destroy(e);
+            return HP2D_E_ARG;
+        }
+        tiles += nt;
+    }
+    const size_t la = e->elem == 4 ? sizeof(LiArgs<float>) : sizeof(LiArgs<double>);
+    const size_t fa = e->elem == 4 ? sizeof(FinArgs<float>) : sizeof(FinArgs<double>);
+    std::vector<char> hl(la * nm), hf(fa * nm);
+    std::vector<SimState *> hs(nm);
+    for (size_t i = 0; i < nm; ++i) {
+        if (e->elem == 4) {
+            LiArgs<float> a = sim_li_args<float>(e->m[i]);
+            FinArgs<float> f = sim_fin_args<float>(e->m[i]);
+            std::memcpy(&hl[i * la], &a, la);
+            std::memcpy(&hf[i * fa], &f, fa);
+        } else {
+            LiArgs<double> a = sim_li_args<double>(e->m[i]);
+            FinArgs<double> f = sim_fin_args<double>(e->m[i]);
+            std::memcpy(&hl[i * la], &a, la);
+            std::memcpy(&hf[i * fa], &f, fa);
+        }
+        hs[i] = e->m[i]->st;
+    }
+    hipError_t r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (r == hipSuccess) r = hipMalloc(&e->li_args, la * nm);
+    if (r == hipSuccess) r = hipMalloc(&e->fin_args, fa * nm);
+    if (r == hipSuccess) r = hipMalloc((void **)&e->tile_grid, std::max<size_t>(tiles, 1) * 4);
+    if (r == hipSuccess) r = hipMalloc((void **)&e->first_tile, nm * 4);
+    if (r == hipSuccess) r = hipMalloc((void **)&e->active_ids, nm * 4);
+    if (r == hipSuccess) r = hipMalloc((void **)&e->done, nm * 4);
+    if (r == hipSuccess) r = hipMalloc((void **)&e->states, nm * sizeof(SimState *));
+    // member streams may still be uploading
+    for (size_t i = 0; i < nm && r == hipSuccess; ++i) r = hipStreamSynchronize(e->m[i]->stream);
+    if (r == hipSuccess) r = hipMemcpy(e->li_args, hl.data(), la * nm, hipMemcpyHostToDevice);
+    if (r == hipSuccess) r = hipMemcpy(e->fin_args, hf.data(), fa * nm, hipMemcpyHostToDevice);
+    if (r == hipSuccess)
+        r = hipMemcpy(e->states, hs.data(), nm * sizeof(SimState *), hipMemcpyHostToDevice);
+    if (r != hipSuccess) {
+        set_err(e, std::string("hp2d_ensemble_create: ") + hipGetErrorString(r));
+        hp2d_ensemble_destroy(e);
+        return r == hipErrorOutOfMemory ? HP2D_E_NOMEM : HP2D_E_HIP;
+    }
+    for (size_t i = 0; i < nm; ++i) e->active.push_back((int)i);
+    hp2d_status rc = build_tiles(e);
+    if (rc != HP2D_OK) {
+        hp2d_ensemble_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_ensemble_run(hp2d_ensemble *e, int64_t max_steps, int32_t graph_steps,
+                              int32_t *n_done) {
+    if (!e || max_steps < 0 || graph_steps < 0) return HP2D_E_ARG;
+    hipEvent_t e0, e1;
+    ETRY(e, hipEventCreate(&e0));
+    ETRY(e, hipEventCreate(&e1));
+    ETRY(e, hipEventRecord(e0, e->stream));
+    int64_t steps = 0;
+    std::vector<int> done(e->m.size(), 0);
+    // drop members that finished before this call
+    {
+        hipLaunchKernelGGL(gather_done_kernel, dim3(((int)e->m.size() + 255) / 256), dim3(256), 0,
+                           e->stream, (SimState *const *)e->states, (int)e->m.size(), e->done);
+        ETRY(e, hipGetLastError());
+        ETRY(e, hipMemcpyAsync(done.data(), e->done, done.size() * 4, hipMemcpyDeviceToHost,
+                               e->stream));
+        ETRY(e, hipStreamSynchronize(e->stream));
+        std::vector<int> still;
+        for (int id : e->active)
+            if (!done[id]) still.push_back(id);
+        if (still.size() != e->active.size()) {
+            e->active.swap(still);
+            hp2d_status rc = build_tiles(e);
+            if (rc) return rc;
+        }
+    }
+    while (steps < max_steps && !e->active.empty()) {
+        const int64_t chunk = graph_steps > 0 ? std::min<int64_t>(graph_steps, max_steps - steps)
+                                              : std::min<int64_t>(64, max_steps - steps);
+        if (graph_steps > 0 && chunk == graph_steps) {
+            hp2d_status rc = ensemble_graph(e, graph_steps);
+            if (rc) return rc;
+            ETRY(e, hipGraphLaunch(e->graph, e->stream));
+        } else {
+            for (int64_t k = 0; k < chunk; ++k) ETRY(e, ensemble_step(e));
+        }
+        steps += chunk;
+        hipLaunchKernelGGL(gather_done_kernel, dim3(((int)e->m.size() + 255) / 256), dim3(256), 0,
+                           e->stream, (SimState *const *)e->states, (int)e->m.size(), e->done);
+        ETRY(e, hipGetLastError());
+        ETRY(e, hipMemcpyAsync(done.data(), e->done, done.size() * 4, hipMemcpyDeviceToHost,
+                               e->stream));
+        ETRY(e, hipStreamSynchronize(e->stream));
+        std::vector<int> still;
+        for (int id : e->active)
+            if (!done[id]) still.push_back(id);
+        if (still.size() != e->active.size()) {
+            e->active.swap(still);
+            if (!e->active.empty()) {
+                hp2d_status rc = build_tiles(e);
+                if (rc) return rc;
+            }
+        }
+    }
+    ETRY(e, hipEventRecord(e1, e->stream));
+    ETRY(e, hipEventSynchronize(e1));
+    float ms = 0;
+    ETRY(e, hipEventElapsedTime(&ms, e0, e1));
+    e->last_ms = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (n_done) *n_done = (int32_t)(e->m.size() - e->active.size());
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_ensemble_member(hp2d_ensemble *e, int32_t i, hp2d_sim **sim) {
+    if (!e || !sim || i < 0 || i >= (int32_t)e->m.size()) return HP2D_E_ARG;
+    ETRY(e, hipStreamSynchronize(e->stream));
+    *sim = e->m[i];
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_ensemble_record_time_to_peak(hp2d_ensemble *e) {
+    if (!e) return HP2D_E_ARG;
+    const size_t la = e->elem == 4 ? sizeof(LiArgs<float>) : sizeof(LiArgs<double>);
+    std::vector<char> hl(la * e->m.size());
+    for (size_t i = 0; i < e->m.size(); ++i) {
+        hp2d_status rc = hp2d_sim_record_time_to_peak(e->m[i]);
+        if (rc) {
+            set_err(e, e->m[i]->err);
+            return rc;
+        }
+        ETRY(e, hipStreamSynchronize(e->m[i]->stream));
+        if (e->elem == 4) {
+            LiArgs<float> a = sim_li_args<float>(e->m[i]);
+            std::memcpy(&hl[i * la], &a, la);
+        } else {
+            LiArgs<double> a = sim_li_args<double>(e->m[i]);
+            std::memcpy(&hl[i * la], &a, la);
+        }
+    }
+    ETRY(e, hipMemcpy(e->li_args, hl.data(), hl.size(), hipMemcpyHostToDevice));
+    if (e->graph) {
+        hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+        e->graph_steps = 0;
+    }
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_ensemble_elapsed(hp2d_ensemble *e, double *ms) {
+    if (!e || !ms) return HP2D_E_ARG;
+    *ms = e->last_ms;
+    return HP2D_OK;
+}
+
+}  // extern "C"

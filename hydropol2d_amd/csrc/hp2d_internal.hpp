@@ -48,6 +48,7 @@ struct SimState {
     long long k, accepted, rejected, inject_k, stop_accepted;
     int cursor, catch_index, done, last_record, n_hydro, n_reports, neg_events, paused;
 };
+static_assert(sizeof(SimState) % 4 == 0, "SimState is copied as 32-bit words");
 
 struct GaClass {
     double ksat, psi, dtheta, Lu_mm, k_out, Tr;

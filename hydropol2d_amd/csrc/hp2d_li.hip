@@ -153,27 +153,32 @@ __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d
     return v;
 }
 
-template <typename T, int MODE, bool CRIT>
-__global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
-    constexpr bool f64 = sizeof(T) == 8;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x / kWave;
-    // XCD-aware tile order: the hardware deals workgroups round-robin to the 8
-    // XCDs, so workgroup w runs on XCD w % 8.  Give each XCD a contiguous run of
-    // logical tiles (row blocks fastest, then column chunks) so that the halo
-    // rows shared by neighbouring row blocks are fetched into one L2.
-    const int n_bx = (a.n_slabs + a.waves_per_block - 1) / a.waves_per_block;
+// XCD-aware tile order: the hardware deals workgroups round-robin to the 8
+// XCDs, so workgroup w runs on XCD w % 8.  Give each XCD a contiguous run of
+// logical tiles (row blocks fastest, then column chunks) so that the halo rows
+// shared by neighbouring row blocks are fetched into one L2.
+__device__ __forceinline__ int xcd_logical_tile() {
     const int n_wg = gridDim.x;
     const int xcd = blockIdx.x % kXcds, k_in = blockIdx.x / kXcds;
     const int per = n_wg / kXcds, rem = n_wg % kXcds;
-    const int logical = xcd * per + min(xcd, rem) + k_in;
+    return xcd * per + min(xcd, rem) + k_in;
+}
+
+// One workgroup's tile of one grid: `logical` counts the grid's tiles, row
+// blocks fastest.
+template <typename T, int MODE, bool CRIT>
+__device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logical) {
+    constexpr bool f64 = sizeof(T) == 8;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int n_bx = (a.n_slabs + a.waves_per_block - 1) / a.waves_per_block;
     const int bx = logical % n_bx, by = logical / n_bx;
     const int slab = bx * a.waves_per_block + wv;
     const bool active = slab < a.n_slabs;
     const int ny = a.ny, nx = a.nx;
 
     long long kstep = 1;
-    T dt_s, dt_h, rain = T(0);
+    T dt_s, dt_h, rain = T(0), t_end_min = T(0);
     int fail_step = 0;
     bool may_report = false;
     const T *d_src = a.d_src, *qx_src = a.qx_src, *qy_src = a.qy_src;
@@ -187,6 +192,7 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
         dt_s = T(st->time_step_min * 60.0);
         dt_h = T(st->time_step_min / 60.0);
         rain = T(st->delta_p_agg);
+        t_end_min = T(st->t + st->time_step_min);
         fail_step = (st->inject_k == kstep);
         // a mass-balance report can only follow this step if t + max_dt reaches the
         // next report threshold (the new dt never exceeds max_dt); only then are the
@@ -456,6 +462,8 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
                 if (a.dmax_map && !fail_step) {
                     const T prevmax = a.dmax_map[oid];
                     a.dmax_map[oid] = (kstep == 1) ? dn : dmax(dn, prevmax);
+                    // time to peak: end time of the step that last raised the maximum
+                    if (a.tpeak_map && (kstep == 1 || dn > prevmax)) a.tpeak_map[oid] = t_end_min;
                 }
             }
         } else {
@@ -535,6 +543,26 @@ __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
     }
 }
 
+template <typename T, int MODE, bool CRIT>
+__global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
+    li_step_tile<T, MODE, CRIT>(a, xcd_logical_tile());
+}
+
+// Many grids in one launch (an ensemble): tile t of the launch belongs to grid
+// tile_grid[t] and is that grid's tile t - grid_first_tile[grid].  Every grid
+// keeps its own arguments, state, partials and dt; the block size is shared.
+template <typename T, bool CRIT>
+__global__ __launch_bounds__(256) void li_step_batch_kernel(const LiArgs<T> *__restrict__ args,
+                                                            const int *__restrict__ tile_grid,
+                                                            const int *__restrict__ grid_first_tile) {
+    const int t = xcd_logical_tile();
+    const int g = tile_grid[t];
+    // read in place: the fields are wave-uniform scalar loads from a read-only
+    // table, kept in the scalar cache
+    const LiArgs<T> &a = args[g];
+    li_step_tile<T, kModeSim, CRIT>(a, t - grid_first_tile[g]);
+}
+
 template <typename T>
 hipError_t launch_li_step(const LiArgs<T> &a, int mode, hipStream_t s) {
     dim3 block(kWave * a.waves_per_block);
@@ -553,6 +581,24 @@ hipError_t launch_li_step(const LiArgs<T> &a, int mode, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_li_step_batch(const LiArgs<T> *args, const int *tile_grid,
+                                const int *grid_first_tile, int n_tiles, int waves_per_block,
+                                int critical, hipStream_t s) {
+    dim3 block(kWave * waves_per_block), grid(n_tiles);
+    if (critical)
+        hipLaunchKernelGGL((li_step_batch_kernel<T, true>), grid, block, 0, s, args, tile_grid,
+                           grid_first_tile);
+    else
+        hipLaunchKernelGGL((li_step_batch_kernel<T, false>), grid, block, 0, s, args, tile_grid,
+                           grid_first_tile);
+    return hipGetLastError();
+}
+
+template hipError_t launch_li_step_batch<float>(const LiArgs<float> *, const int *, const int *,
+                                                int, int, int, hipStream_t);
+template hipError_t launch_li_step_batch<double>(const LiArgs<double> *, const int *, const int *,
+                                                 int, int, int, hipStream_t);
 template hipError_t launch_li_step<float>(const LiArgs<float> &, int, hipStream_t);
 template hipError_t launch_li_step<double>(const LiArgs<double> &, int, hipStream_t);
 

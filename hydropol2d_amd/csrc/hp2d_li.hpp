@@ -39,6 +39,7 @@ template <typename T> struct LiArgs {
     int n_soil_classes;
     T I_min;
     T *dmax_map;
+    T *tpeak_map;         // optional: model time [min] at which dmax was last raised
     // outlets (sorted linear indices)
     const long long *out_idx;
     const T *out_n;
@@ -52,5 +53,12 @@ template <typename T> struct LiArgs {
 };
 
 template <typename T> hipError_t launch_li_step(const LiArgs<T> &a, int mode, hipStream_t s);
+
+// n_tiles tiles of several grids in one launch (simulation mode); all grids
+// share waves_per_block and the critical-velocity flag
+template <typename T>
+hipError_t launch_li_step_batch(const LiArgs<T> *args, const int *tile_grid,
+                                const int *grid_first_tile, int n_tiles, int waves_per_block,
+                                int critical, hipStream_t s);
 
 }  // namespace hp2d

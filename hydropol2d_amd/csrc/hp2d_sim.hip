@@ -20,8 +20,18 @@ constexpr int kFinThreads = 1024;
 constexpr int kFinWaves = kFinThreads / kWave;
 
 template <typename T>
-__global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
-    SimState *st = a.st;
+__device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
+    // The scalar tail below is one thread's chain of dependent reads and
+    // writes of the run state.  It works on a copy in LDS, loaded and stored
+    // back in one pass, so the chain pays LDS latency instead of L2 latency.
+    __shared__ SimState sst;
+    {
+        const unsigned *src = reinterpret_cast<const unsigned *>(a.st);
+        unsigned *dst = reinterpret_cast<unsigned *>(&sst);
+        for (int i = threadIdx.x; i < (int)(sizeof(SimState) / 4); i += kFinThreads) dst[i] = src[i];
+    }
+    __syncthreads();
+    SimState *st = &sst;
     if (st->done || st->paused) return;
     __shared__ double sh[kFinWaves][12];
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
@@ -230,6 +240,21 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
     }
     st->done = !(st->t <= (routing_time + st->min_dt / 60.0)) ? 1 : 0;
     if (st->stop_accepted >= 0 && st->accepted >= st->stop_accepted) st->paused = 1;
+    unsigned *gdst = reinterpret_cast<unsigned *>(a.st);
+    const unsigned *lsrc = reinterpret_cast<const unsigned *>(&sst);
+    for (int i = 0; i < (int)(sizeof(SimState) / 4); ++i) gdst[i] = lsrc[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kFinThreads) void finalize_kernel(FinArgs<T> a) {
+    finalize_body<T>(a);
+}
+
+// one block per grid of an ensemble: block b finalises grid ids[b]
+template <typename T>
+__global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(const FinArgs<T> *args,
+                                                                     const int *ids) {
+    finalize_body<T>(args[ids[blockIdx.x]]);
 }
 
 template <typename T>
@@ -237,6 +262,18 @@ hipError_t launch_finalize(const FinArgs<T> &a, hipStream_t s) {
     hipLaunchKernelGGL((finalize_kernel<T>), dim3(1), dim3(kFinThreads), 0, s, a);
     return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_finalize_batch(const FinArgs<T> *args, const int *ids, int n_grids,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL((finalize_batch_kernel<T>), dim3(n_grids), dim3(kFinThreads), 0, s, args,
+                       ids);
+    return hipGetLastError();
+}
+template hipError_t launch_finalize_batch<float>(const FinArgs<float> *, const int *, int,
+                                                 hipStream_t);
+template hipError_t launch_finalize_batch<double>(const FinArgs<double> *, const int *, int,
+                                                  hipStream_t);
 
 template hipError_t launch_finalize<float>(const FinArgs<float> &, hipStream_t);
 

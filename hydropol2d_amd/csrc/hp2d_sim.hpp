@@ -32,6 +32,9 @@ template <typename T> struct FinArgs {
 };
 
 template <typename T> hipError_t launch_finalize(const FinArgs<T> &a, hipStream_t s);
+// one block per grid of an ensemble; block b finalises grid ids[b]
+template <typename T>
+hipError_t launch_finalize_batch(const FinArgs<T> *args, const int *ids, int n_grids, hipStream_t s);
 template <typename T>
 hipError_t launch_storage(const T *d, const T *I, long long n, double *part, int blocks,
                           hipStream_t s);

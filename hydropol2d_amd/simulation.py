@@ -101,9 +101,106 @@ class Simulation:
                                                     n, C.byref(got)))
         return {"hydro_q": q, "hydro_t": t, "hydro_depth": d, "n_hydro": got.value}
 
+    def record_time_to_peak(self):
+        """Start recording, per cell, the model time [min] of the step that last
+        raised its maximum depth (a derived output; the reference keeps dmax only)."""
+        self._check(self._L.hp2d_sim_record_time_to_peak(self._s))
+
+    def time_to_peak(self):
+        t = np.zeros((self.ny, self.nx), order="F")
+        self._check(self._L.hp2d_sim_get_time_to_peak(self._s, t.ctypes.data_as(_dp)))
+        return t
+
     def dt_history(self, n_max: int = 4096):
         buf = np.zeros(n_max)
         got = C.c_int32()
         self._check(self._L.hp2d_sim_get_dt_history(self._s, buf.ctypes.data_as(_dp), n_max,
                                                     C.byref(got)))
         return buf[: got.value]
+
+
+class _Member(Simulation):
+    """A member of an Ensemble: the Simulation getters on a borrowed handle."""
+
+    def __init__(self, ens, i, spec, ptr):
+        self.spec = spec
+        self.ny, self.nx = spec.shape
+        self._s = ptr
+        self._L = ens._L
+        self._ens = ens          # keeps the owner alive
+
+    def close(self):             # the ensemble owns the handle
+        self._s = None
+
+
+class Ensemble:
+    """Independent runs (members) advanced together: one batched launch of the
+    fused step over every unfinished member's tiles and one batched launch of
+    the members' dt / bookkeeping tails per step, each member with its own grid
+    size, inputs, clock and adaptive dt (the member loops of
+    HydroPol2D_Main_While_Ensemble.m)."""
+
+    def __init__(self, specs, cols_per_wave: int = 0):
+        L = _lib.load()
+        _lib.require_device()
+        self.specs = list(specs)
+        n = len(self.specs)
+        cfgs = (abi.RunConfig * n)()
+        ins = (abi.RunInputs * n)()
+        keep = []
+        for i, sp in enumerate(self.specs):
+            c, inp, k = sp.to_c()
+            cfgs[i] = c
+            ins[i] = inp
+            keep.append(k)
+        ptr = C.c_void_p()
+        status = L.hp2d_ensemble_create(n, cfgs, ins, int(cols_per_wave), C.byref(ptr))
+        del keep
+        _lib.check(status)
+        self._e = ptr
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_e", None):
+            self._L.hp2d_ensemble_destroy(self._e)
+            self._e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.specs)
+
+    def _check(self, status):
+        if status != abi.HP2D_OK:
+            msg = self._L.hp2d_ensemble_last_error(self._e)
+            raise _lib.HP2DError(status, (msg or b"").decode(errors="replace"))
+
+    def run(self, max_steps: int = 1 << 62, graph_steps: int = 50) -> int:
+        """Advance unfinished members by up to max_steps steps; returns how many are done."""
+        done = C.c_int32(0)
+        self._check(self._L.hp2d_ensemble_run(self._e, int(max_steps), int(graph_steps),
+                                              C.byref(done)))
+        return done.value
+
+    def elapsed_ms(self) -> float:
+        ms = C.c_double()
+        self._check(self._L.hp2d_ensemble_elapsed(self._e, C.byref(ms)))
+        return ms.value
+
+    def record_time_to_peak(self):
+        self._check(self._L.hp2d_ensemble_record_time_to_peak(self._e))
+
+    def member(self, i: int) -> Simulation:
+        ptr = C.c_void_p()
+        self._check(self._L.hp2d_ensemble_member(self._e, int(i), C.byref(ptr)))
+        return _Member(self, i, self.specs[i], ptr)

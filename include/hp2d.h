@@ -333,6 +333,42 @@ hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t *best_wpb, 
 hp2d_status hp2d_sim_kernel_time(hp2d_sim *s, double *li_ms, double *total_ms,
                                  int64_t *launches);
 
+/* Time to peak: once enabled, every accepted step whose depth raises a cell's
+ * dmax also records the model time [min] at the end of that step.  Derived
+ * output with no counterpart in the reference (save_output_maps.m:588-600
+ * keeps only the peak).  Enabling it costs one conditional 4/8-byte write
+ * per raised cell. */
+hp2d_status hp2d_sim_record_time_to_peak(hp2d_sim *s);
+hp2d_status hp2d_sim_get_time_to_peak(hp2d_sim *s, double *tpeak_min);
+
+/* ------------------------------------------------------------------------ */
+/* Ensemble: independent runs advanced together                              */
+/* ------------------------------------------------------------------------ */
+
+/* Members of an ensemble / calibration / probabilistic study (the member loops
+ * of HydroPol2D_Main_While_Ensemble.m and Calibration_Water_Quality.m, each a
+ * full HydroPol2D_Main_While run) with their own grid sizes, inputs, clocks
+ * and adaptive dt.  One batched launch of the fused step covers the tiles of
+ * every unfinished member and one batched launch runs every member's dt /
+ * bookkeeping tail; finished members drop out of the tile table.  Members
+ * share precision and flag_critical.  cols_per_wave: 16, 32, 64, 128 for all
+ * members, or 0 for each member's own default. */
+typedef struct hp2d_ensemble hp2d_ensemble;
+hp2d_status hp2d_ensemble_create(int32_t n_members, const hp2d_run_config *cfgs,
+                                 const hp2d_run_inputs *inputs, int32_t cols_per_wave,
+                                 hp2d_ensemble **out);
+void hp2d_ensemble_destroy(hp2d_ensemble *e);
+const char *hp2d_ensemble_last_error(const hp2d_ensemble *e);
+/* Advance every unfinished member by up to max_steps steps (hipGraphs of
+ * graph_steps batched steps when graph_steps > 0).  *n_done = members finished. */
+hp2d_status hp2d_ensemble_run(hp2d_ensemble *e, int64_t max_steps, int32_t graph_steps,
+                              int32_t *n_done);
+/* Borrow member i's simulation for the hp2d_sim_* getters (owned by the ensemble). */
+hp2d_status hp2d_ensemble_member(hp2d_ensemble *e, int32_t i, hp2d_sim **sim);
+hp2d_status hp2d_ensemble_record_time_to_peak(hp2d_ensemble *e);
+/* Device time [ms] of the last hp2d_ensemble_run call. */
+hp2d_status hp2d_ensemble_elapsed(hp2d_ensemble *e, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,3 +116,51 @@ def test_closed_basin_conserves_volume_at_full_size():
     assert rep["steps"] == 20
     assert d.min() >= 0
     assert abs(d.sum(dtype=np.float64) - v0) / v0 < 1e-5
+
+
+def test_fp32_error_per_output_field():
+    # The FP32 state path against the FP64 oracle on the C1 plane (GA, rain,
+    # critical outlet) at 128 x 128 over 20 min: the error of every output
+    # field, printed as a table, with the bound each one is held to.
+    spec = synthetic.tilted_plane_c1(128, 128, routing_time_min=20.0)
+    o = oracle.run(spec)
+    spec32 = synthetic.tilted_plane_c1(128, 128, routing_time_min=20.0, precision="fp32")
+    with _sim(spec32) as s:
+        s.run()
+        st = s.state(infiltration=True)
+        hy = s.hydrograph()
+        rep = s.report()
+    orep = o["report"]
+    n_h = min(hy["n_hydro"], o["n_hydro"])
+    qo, qg = o["hydro_q"][:n_h], hy["hydro_q"][:n_h]
+    rows = [
+        ("d_t [m]", np.nanmax(np.abs(st["d_t"] - o["d_t"])) / 1000, 1e-3),
+        ("dmax [m]", np.nanmax(np.abs(st["dmax"] - o["dmax"])) / 1000, 1e-3),
+        ("I_t [mm]", np.nanmax(np.abs(st["I_t"] - o["I_t"])), 0.05),
+        ("outflow x [mm/h], rel. to max", np.nanmax(np.abs(st["outflow"][:, :, 0] - o["outflow"][:, :, 0]))
+         / np.nanmax(np.abs(o["outflow"][:, :, 0])), 2e-2),
+        ("outflow y [mm/h], rel. to max", np.nanmax(np.abs(st["outflow"][:, :, 1] - o["outflow"][:, :, 1]))
+         / np.nanmax(np.abs(o["outflow"][:, :, 1])), 2e-2),
+        ("outlet hydrograph [m3/s], rel. to peak", np.max(np.abs(qg - qo)) / max(np.max(qo), 1e-12), 1e-2),
+        ("outflow volume [m3], rel.", abs(rep["outflow_volume_m3"] - orep["outflow_volume_m3"])
+         / max(orep["outflow_volume_m3"], 1e-12), 1e-2),
+        # the FP64 ledgers integrate over the FP32 run's own dt sequence, whose
+        # last step ends at a slightly different time
+        ("cum. rain [m3], rel.", abs(rep["cum_rain_m3"] - orep["cum_rain_m3"]) / orep["cum_rain_m3"], 1e-4),
+        ("cum. infiltration [m3], rel.", abs(rep["cum_infiltration_m3"] - orep["cum_infiltration_m3"])
+         / orep["cum_infiltration_m3"], 1e-3),
+        ("storage [m3], rel.", abs(rep["storage_m3"] - orep["storage_m3"]) / orep["storage_m3"], 1e-3),
+        ("max velocity [m/s], rel.", abs(rep["max_velocity"] - orep["max_velocity"])
+         / orep["max_velocity"], 2e-2),
+        ("steps, rel.", abs(rep["steps"] - orep["steps"]) / orep["steps"], 1e-2),
+        # mass_balance_check's volume_error (a step's fluxes against the storage
+        # change since the last report) is large by construction; it is compared
+        # with the oracle's, relative to the storage
+        ("volume error [m3], rel. to storage", abs(rep["volume_error_m3"] - orep["volume_error_m3"])
+         / orep["storage_m3"], 1e-3),
+    ]
+    print("\nFP32 vs FP64 oracle, 128x128 C1 plane, 20 min:")
+    for name, err, bound in rows:
+        print(f"  {name:45s} {err:.3e}  (bound {bound:.1e})")
+    for name, err, bound in rows:
+        assert err <= bound, (name, err, bound)
