@@ -107,6 +107,12 @@ int32_t oracle_gw_recharge(int64_t n, const double *infiltration_rate, const dou
                            const double *alpha_vg, double l_vg, const double *ksat_mmh,
                            double *recharge_rate, double *S_out, double *cumulative_recharge);
 
+void oracle_conv3x3_fwd(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
+                        const float *x, const float *w, const float *b, int32_t relu, float *y);
+void oracle_conv3x3_bwd(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
+                        const float *x, const float *w, const float *gy, float *gx, float *gw,
+                        float *gb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,11 @@ def lib():
         L.oracle_gw_recharge.argtypes = ([C.c_int64, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp,
                                           _dp, _dp, C.c_double, _dp, _dp, _dp, _dp])
         L.oracle_gw_recharge.restype = C.c_int32
+        _fp = C.POINTER(C.c_float)
+        L.oracle_conv3x3_fwd.argtypes = [C.c_int32] * 5 + [_fp, _fp, _fp, C.c_int32, _fp]
+        L.oracle_conv3x3_fwd.restype = None
+        L.oracle_conv3x3_bwd.argtypes = [C.c_int32] * 5 + [_fp] * 6
+        L.oracle_conv3x3_bwd.restype = None
         _lib = L
     return _lib
 
@@ -294,3 +299,35 @@ def gw_recharge(infiltration_rate, S0, dt, min_S, max_S, theta_s, theta_r, n_vg,
                              _p(args[4]), _p(args[5]), _p(args[6]), _p(args[7]), l_vg,
                              _p(args[8]), _p(Rr), _p(Sn), _p(cum))
     return Rr, Sn, cum
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else C.POINTER(C.c_float)()
+
+
+def conv3x3(x, w, b=None, relu=False):
+    """3x3 zero-padded convolution, NHWC x [N,H,W,Cin], w [Cout,3,3,Cin] -> [N,H,W,Cout]."""
+    x, w = _f32(x), _f32(w)
+    N, H, W, Cin = x.shape
+    Cout = w.shape[0]
+    b = None if b is None else _f32(b)
+    y = np.zeros((N, H, W, Cout), dtype=np.float32)
+    lib().oracle_conv3x3_fwd(N, H, W, Cin, Cout, _fp(x), _fp(w), _fp(b), int(relu), _fp(y))
+    return y
+
+
+def conv3x3_grad(x, w, gy):
+    """Gradients (dx, dw, db) of conv3x3 for the output gradient gy (before activation)."""
+    x, w, gy = _f32(x), _f32(w), _f32(gy)
+    N, H, W, Cin = x.shape
+    Cout = w.shape[0]
+    gx = np.zeros_like(x)
+    gw = np.zeros_like(w)
+    gb = np.zeros(Cout, dtype=np.float32)
+    lib().oracle_conv3x3_bwd(N, H, W, Cin, Cout, _fp(x), _fp(w), _fp(gy), _fp(gx), _fp(gw),
+                             _fp(gb))
+    return gx, gw, gb
