@@ -63,6 +63,20 @@ SIGNATURES = {
     "hp2d_ensemble_member": (C.c_int32, [_vp, C.c_int32, C.POINTER(_vp)]),
     "hp2d_ensemble_record_time_to_peak": (C.c_int32, [_vp]),
     "hp2d_ensemble_elapsed": (C.c_int32, [_vp, _dp]),
+    "hp2d_nn_conv3x3_fwd": (C.c_int32, [C.c_int32] * 5 + [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp,
+                                         _vp]),
+    "hp2d_nn_conv3x3_dgrad": (C.c_int32, [C.c_int32] * 5 + [_vp] * 5),
+    "hp2d_nn_conv3x3_wgrad": (C.c_int32, [C.c_int32] * 5 + [_vp] * 5),
+    "hp2d_nn_relu_mask": (C.c_int32, [_vp, _vp, C.c_int64, _vp]),
+    "hp2d_nn_add": (C.c_int32, [_vp, _vp, _vp, C.c_int64, _vp]),
+    "hp2d_nn_avgpool2": (C.c_int32, [_vp] + [C.c_int32] * 4 + [_vp, C.c_int32, _vp]),
+    "hp2d_nn_upsample2": (C.c_int32, [_vp] + [C.c_int32] * 4 + [_vp, C.c_int32, _vp]),
+    "hp2d_nn_head": (C.c_int32, [_vp, C.c_int64, C.c_int32, _vp, _vp, C.c_int32, _vp, _vp, _vp,
+                                 _vp, _vp, _vp]),
+    "hp2d_nn_mse": (C.c_int32, [_vp, _vp, _vp, C.c_int64, C.c_int32, C.c_float, _vp, _vp, _vp]),
+    "hp2d_nn_adam": (C.c_int32, [_vp, _vp, _vp, _vp, C.c_int64, C.c_float, C.c_float, C.c_float,
+                                 C.c_float, C.c_int32, _vp, _vp]),
+    "hp2d_nn_to_bf16": (C.c_int32, [_vp, C.c_int64, _vp, _vp]),
     "hp2d_sim_create": (C.c_int32, [C.POINTER(abi.RunConfig), C.POINTER(abi.RunInputs),
                                     C.POINTER(_vp)]),
     "hp2d_sim_destroy": (None, [_vp]),

@@ -146,7 +146,7 @@ def dam_break_c3(ny=2048, nx=2048, dx=1.0, precision="fp32", routing_time_min=30
 
 
 def ensemble_c4(n_members=512, seed=1000, min_log2=7, max_log2=10, dx=5.0, precision="fp32",
-                routing_time_min=120.0):
+                routing_time_min=120.0, vary_soil_moisture=False):
     """Ensemble members around the C1 plane: grid side 2^U(min_log2, max_log2)
     (log-uniform), a seeded fractal DEM (Hurst 0.8, relief 5 % of the domain
     length) on a 1 % tilt towards the last column (critical-depth outlets
@@ -157,13 +157,15 @@ def ensemble_c4(n_members=512, seed=1000, min_log2=7, max_log2=10, dx=5.0, preci
     for m in range(n_members):
         side = int(round(2.0 ** rng.uniform(min_log2, max_log2)))
         f_n, f_ks, f_rain = np.exp(rng.uniform(np.log(0.5), np.log(1.5), 3))
+        # initial soil moisture deficit theta_s - theta_i (fixed 0.3 unless varied)
+        dtheta = rng.uniform(0.15, 0.35) if vary_soil_moisture else 0.3
         ny = nx = side
         z = spectral_dem(ny, nx, dx, 0.8, 0.05 * side * dx, seed=seed + m)
         z = z + 0.01 * (nx - 1 - np.arange(nx))[None, :] * dx
         outlet = np.zeros((ny, nx), dtype=np.uint8, order="F")
         outlet[:, nx - 1] = 1
         ga = GreenAmpt(ksat_mmh=np.array([5.0 * f_ks]), psi_mm=np.array([110.0]),
-                       dtheta=np.array([0.3]), theta_sat_minus_r=np.array([0.4]), I_min_mm=1e-3)
+                       dtheta=np.array([dtheta]), theta_sat_minus_r=np.array([0.4]), I_min_mm=1e-3)
         specs.append(RunSpec(
             elevation=np.asfortranarray(z), roughness=np.full((ny, nx), 0.02 * f_n, order="F"),
             resolution=dx, outlet_mask=outlet, rain_time_min=np.array([30.0, 1e9]),

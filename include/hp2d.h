@@ -369,6 +369,59 @@ hp2d_status hp2d_ensemble_record_time_to_peak(hp2d_ensemble *e);
 /* Device time [ms] of the last hp2d_ensemble_run call. */
 hp2d_status hp2d_ensemble_elapsed(hp2d_ensemble *e, double *ms);
 
+/* ------------------------------------------------------------------------ */
+/* Flood surrogate kernels (no counterpart in the reference)                 */
+/* ------------------------------------------------------------------------ */
+/* A convolutional encoder-decoder maps per-member input maps to peak-depth
+ * and time-to-peak maps; its layers run on these kernels.  All pointers are
+ * device pointers; `stream` is a hipStream_t (NULL = default stream).
+ * Activations are NHWC bf16 stored as uint16 (channels a multiple of 32);
+ * 3x3 weights are [Cout][3][3][Cin] bf16; biases, weight gradients, optimiser
+ * state and the head are FP32.  Every 3x3 convolution is an implicit GEMM on
+ * gfx950 BF16 MFMA with FP32 accumulation. */
+
+/* y = act(conv3x3(x, w) + bias) (act = ReLU when relu != 0); with skip != NULL
+ * also y_skip = y + skip.  bias may be NULL. */
+hp2d_status hp2d_nn_conv3x3_fwd(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
+                                const uint16_t *x, const uint16_t *w, const float *bias,
+                                int32_t relu, const uint16_t *skip, uint16_t *y, uint16_t *y_skip,
+                                void *stream);
+/* gx = d/dx of conv3x3 for the output gradient gy (before the activation);
+ * w_flip is a 9*Cin*Cout bf16 workspace. */
+hp2d_status hp2d_nn_conv3x3_dgrad(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
+                                  const uint16_t *gy, const uint16_t *w, uint16_t *w_flip,
+                                  uint16_t *gx, void *stream);
+/* gw [Cout][9*Cin] += d/dw, gb [Cout] += d/db (gb may be NULL), FP32. */
+hp2d_status hp2d_nn_conv3x3_wgrad(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
+                                  const uint16_t *x, const uint16_t *gy, float *gw, float *gb,
+                                  void *stream);
+/* g[i] = 0 where y[i] <= 0 (ReLU backward on the stored activation). */
+hp2d_status hp2d_nn_relu_mask(uint16_t *g, const uint16_t *y, int64_t n, void *stream);
+/* out = a + b (bf16). */
+hp2d_status hp2d_nn_add(const uint16_t *a, const uint16_t *b, uint16_t *out, int64_t n,
+                        void *stream);
+/* 2x2 average pooling of x [N,H,W,C] into y [N,H/2,W/2,C]; with backward != 0,
+ * x is the gradient of the pooled map and y the gradient of the input [N,H,W,C]. */
+hp2d_status hp2d_nn_avgpool2(const uint16_t *x, int32_t N, int32_t H, int32_t W, int32_t C,
+                             uint16_t *y, int32_t backward, void *stream);
+/* nearest 2x upsampling of x [N,H/2,W/2,C] into y [N,H,W,C]; with backward != 0,
+ * x is the gradient [N,H,W,C] and y receives [N,H/2,W/2,C]. */
+hp2d_status hp2d_nn_upsample2(const uint16_t *x, int32_t N, int32_t H, int32_t W, int32_t C,
+                              uint16_t *y, int32_t backward, void *stream);
+/* 1x1 head: out [P][O] = b + x [P][C] w^T (w [O][C]); with gy != NULL the
+ * backward instead: gx (bf16), gw, gb accumulated. */
+hp2d_status hp2d_nn_head(const uint16_t *x, int64_t P, int32_t C, const float *w, const float *b,
+                         int32_t O, float *out, const float *gy, uint16_t *gx, float *gw, float *gb,
+                         void *stream);
+/* masked MSE: *loss += sum(mask*(out-target)^2)*inv_count, gy = 2(out-target)*mask*inv_count */
+hp2d_status hp2d_nn_mse(const float *out, const float *target, const uint8_t *mask, int64_t P,
+                        int32_t O, float inv_count, float *gy, float *loss, void *stream);
+/* one Adam step (Kingma & Ba 2015) of n FP32 weights; w_bf16 (may be NULL) gets the bf16 copy */
+hp2d_status hp2d_nn_adam(float *w, float *m, float *v, const float *g, int64_t n, float lr,
+                         float beta1, float beta2, float eps, int32_t step, uint16_t *w_bf16,
+                         void *stream);
+hp2d_status hp2d_nn_to_bf16(const float *x, int64_t n, uint16_t *y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
