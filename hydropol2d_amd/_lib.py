@@ -63,6 +63,10 @@ SIGNATURES = {
     "hp2d_ensemble_member": (C.c_int32, [_vp, C.c_int32, C.POINTER(_vp)]),
     "hp2d_ensemble_record_time_to_peak": (C.c_int32, [_vp]),
     "hp2d_ensemble_elapsed": (C.c_int32, [_vp, _dp]),
+    "hp2d_infiltration_module": (C.c_int32, [_vp, C.c_double, C.c_double, C.c_double] + [_dp] * 8
+                                 + [_u8p, C.c_double, C.c_double, C.c_double, C.c_double]
+                                 + [_dp] * 9),
+    "hp2d_fill_sinks": (C.c_int32, [C.c_int32, C.c_int32, _dp, _u8p, C.c_double, _dp]),
     "hp2d_nn_conv3x3_fwd": (C.c_int32, [C.c_int32] * 5 + [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp,
                                          _vp]),
     "hp2d_nn_conv3x3_dgrad": (C.c_int32, [C.c_int32] * 5 + [_vp] * 5),

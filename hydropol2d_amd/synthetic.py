@@ -34,6 +34,24 @@ def tilted_plane_c1(ny=256, nx=256, dx=5.0, seed=1, precision="fp64", routing_ti
         record_time_hydrographs_min=1.0, infiltration=ga)
 
 
+def fill_sinks(z, seed_mask=None, epsilon=1e-4):
+    """Depression-filled copy of z (Priority-Flood+epsilon in libhp2d, host code)."""
+    import ctypes as C
+    from . import _lib
+    L = _lib.load()
+    z = np.asfortranarray(z, dtype=np.float64)
+    out = np.empty_like(z, order="F")
+    m = None
+    if seed_mask is not None:
+        m = np.asfortranarray(np.asarray(seed_mask, dtype=np.uint8))
+    st = L.hp2d_fill_sinks(z.shape[0], z.shape[1], z.ctypes.data_as(C.POINTER(C.c_double)),
+                           m.ctypes.data_as(C.POINTER(C.c_uint8)) if m is not None
+                           else C.POINTER(C.c_uint8)(), float(epsilon),
+                           out.ctypes.data_as(C.POINTER(C.c_double)))
+    _lib.check(st)
+    return out
+
+
 def spectral_dem(ny, nx, dx, hurst=0.8, relief=200.0, seed=2):
     """Fractal (fBm) surface by spectral synthesis, scaled to `relief` metres."""
     rng = np.random.default_rng(seed)
@@ -146,7 +164,7 @@ def dam_break_c3(ny=2048, nx=2048, dx=1.0, precision="fp32", routing_time_min=30
 
 
 def ensemble_c4(n_members=512, seed=1000, min_log2=7, max_log2=10, dx=5.0, precision="fp32",
-                routing_time_min=120.0, vary_soil_moisture=False):
+                routing_time_min=120.0, vary_soil_moisture=False, filled=False):
     """Ensemble members around the C1 plane: grid side 2^U(min_log2, max_log2)
     (log-uniform), a seeded fractal DEM (Hurst 0.8, relief 5 % of the domain
     length) on a 1 % tilt towards the last column (critical-depth outlets
@@ -164,6 +182,8 @@ def ensemble_c4(n_members=512, seed=1000, min_log2=7, max_log2=10, dx=5.0, preci
         z = z + 0.01 * (nx - 1 - np.arange(nx))[None, :] * dx
         outlet = np.zeros((ny, nx), dtype=np.uint8, order="F")
         outlet[:, nx - 1] = 1
+        if filled:  # every depression drains to the outlet column
+            z = fill_sinks(z, seed_mask=outlet)
         ga = GreenAmpt(ksat_mmh=np.array([5.0 * f_ks]), psi_mm=np.array([110.0]),
                        dtheta=np.array([dtheta]), theta_sat_minus_r=np.array([0.4]), I_min_mm=1e-3)
         specs.append(RunSpec(

@@ -199,6 +199,35 @@ hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, double dt, double dx, double dy,
                                double *h_t, double *u_x, double *u_y, double *q_exf,
                                double *q_river, double *error, int32_t *n_substeps);
 
+/* Infiltration_Module.m:101-583 (flag_infiltration = 1, no subgrid, no
+ * layered soil; called from Hydrological_Model.m): van Genuchten / Mualem
+ * surface-interface capacity with a Darcy driving head, limited by the supply
+ * d_t/dt and the remaining unsaturated-zone storage.  Host arrays ny*nx:
+ * elevation [m], h_t (groundwater head) [m], soil_depth [m], theta_sat,
+ * theta_r, n_vg, alpha_vg [1/m], ksat_mmh [mm/h], idx_imp (0/1 bytes or NULL),
+ * d_t [mm], I_t [mm] (read-only).  Scalars: time_step [min], resolution [m],
+ * C_a [m2] (continuity area), l_vg, Ltop [m], dh_max [m], Ksurf_weight
+ * (the reference's defaults are 0.5, 0.05, 1.0, 0.5).  Outputs ny*nx: d_t_out,
+ * I_t_out [mm], I_p [mm] (= I_t in, may be NULL), f [mm/h]; cum_inf [m] is
+ * read and updated when given; inf_volume [m3] and dS_inf (errors(4)) [m3]. */
+hp2d_status hp2d_infiltration_module(hp2d_handle *h, double time_step_min, double resolution,
+                                     double C_a, const double *elevation, const double *h_t,
+                                     const double *soil_depth, const double *theta_sat,
+                                     const double *theta_r, const double *n_vg,
+                                     const double *alpha_vg, const double *ksat_mmh,
+                                     const uint8_t *idx_imp, double l_vg, double Ltop,
+                                     double dh_max, double Ksurf_weight, const double *d_t,
+                                     const double *I_t, double *d_t_out, double *I_t_out,
+                                     double *I_p, double *f, double *cum_inf, double *inf_volume,
+                                     double *dS_inf);
+
+/* Depression filling of a DEM (host preprocessing; the reference fills with
+ * TopoToolbox): Priority-Flood+epsilon (Barnes et al., Computers & Geosciences
+ * 62, 2014) seeded at the seed_mask cells (NULL = the grid perimeter and the
+ * edge of the NaN mask), D8 neighbours.  z, out: ny*nx [m] (may alias). */
+hp2d_status hp2d_fill_sinks(int32_t ny, int32_t nx, const double *z, const uint8_t *seed_mask,
+                            double epsilon, double *out);
+
 /* ------------------------------------------------------------------------ */
 /* Device-resident step loop (the performance path)                         */
 /* ------------------------------------------------------------------------ */

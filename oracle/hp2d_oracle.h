@@ -113,6 +113,15 @@ void oracle_conv3x3_bwd(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Co
                         const float *x, const float *w, const float *gy, float *gx, float *gw,
                         float *gb);
 
+int32_t oracle_infiltration(int32_t ny, int32_t nx, double time_step_min, double resolution,
+                            double C_a, const double *elevation, const double *h_t,
+                            const double *soil_depth, const double *theta_sat,
+                            const double *theta_r, const double *n_vg, const double *alpha_vg,
+                            const double *ksat_mmh, const uint8_t *idx_imp, double l_vg,
+                            double Ltop_in, double dh_max, double Ksurf_weight, double *d_t,
+                            double *I_t, double *I_p, double *f_out, double *cum_inf,
+                            double *inf_volume, double *dS_inf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,10 @@ def lib():
         L.oracle_conv3x3_fwd.restype = None
         L.oracle_conv3x3_bwd.argtypes = [C.c_int32] * 5 + [_fp] * 6
         L.oracle_conv3x3_bwd.restype = None
+        L.oracle_infiltration.argtypes = ([C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double]
+                                          + [_dp] * 8 + [_u8, C.c_double, C.c_double, C.c_double,
+                                                         C.c_double] + [_dp] * 7)
+        L.oracle_infiltration.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -331,3 +335,24 @@ def conv3x3_grad(x, w, gy):
     lib().oracle_conv3x3_bwd(N, H, W, Cin, Cout, _fp(x), _fp(w), _fp(gy), _fp(gx), _fp(gw),
                              _fp(gb))
     return gx, gw, gb
+
+
+def infiltration(time_step_min, resolution, d_t, I_t, elevation, h_t, soil_depth, theta_sat,
+                 theta_r, n_vg, alpha_vg, ksat_mmh, idx_imp=None, C_a=None, l_vg=0.5, Ltop=0.05,
+                 dh_max=1.0, Ksurf_weight=0.5, cum_inf=None):
+    """Infiltration_Module (flag_infiltration = 1, no subgrid) on the oracle; returns a dict."""
+    d = _f(d_t).copy(order="F")
+    ny, nx = d.shape
+    g = lambda a: np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (ny, nx)))
+    I = g(I_t).copy(order="F")
+    args = [g(a) for a in (elevation, h_t, soil_depth, theta_sat, theta_r, n_vg, alpha_vg, ksat_mmh)]
+    im, imp = _mask(idx_imp, (ny, nx))
+    Ip, f = np.zeros((ny, nx), order="F"), np.zeros((ny, nx), order="F")
+    cum = None if cum_inf is None else g(cum_inf).copy(order="F")
+    vol, dS = C.c_double(), C.c_double()
+    Ca = resolution * resolution if C_a is None else C_a
+    lib().oracle_infiltration(ny, nx, time_step_min, resolution, Ca, *[_p(a) for a in args], imp,
+                              l_vg, Ltop, dh_max, Ksurf_weight, _p(d), _p(I), _p(Ip), _p(f),
+                              _p(cum), C.byref(vol), C.byref(dS))
+    return {"d_t": d, "I_t": I, "I_p": Ip, "f": f, "cum_inf": cum, "inf_volume": vol.value,
+            "dS_inf": dS.value}
