@@ -66,6 +66,7 @@ SIGNATURES = {
     "hp2d_infiltration_module": (C.c_int32, [_vp, C.c_double, C.c_double, C.c_double] + [_dp] * 8
                                  + [_u8p, C.c_double, C.c_double, C.c_double, C.c_double]
                                  + [_dp] * 9),
+    "hp2d_groundwater_module": (C.c_int32, [_vp, _vp, _vp] + [_dp] * 7 + [_i32p]),
     "hp2d_fill_sinks": (C.c_int32, [C.c_int32, C.c_int32, _dp, _u8p, C.c_double, _dp]),
     "hp2d_nn_conv3x3_fwd": (C.c_int32, [C.c_int32] * 5 + [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp,
                                          _vp]),

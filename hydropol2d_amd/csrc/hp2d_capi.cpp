@@ -84,6 +84,8 @@ struct hp2d_handle {
     int tj = 16, wpb = 4;
     void *ws = nullptr;          // module workspace (wash-off, groundwater)
     size_t ws_bytes = 0;
+    void *ws2 = nullptr;         // second workspace (groundwater module around Boussinesq)
+    size_t ws2_bytes = 0;
 };
 
 static void set_err(hp2d_handle *h, const std::string &m) {
@@ -148,6 +150,7 @@ void hp2d_destroy(hp2d_handle *h) {
     if (h->out_idx) hipFree(h->out_idx);
     if (h->out_n) hipFree(h->out_n);
     if (h->ws) hipFree(h->ws);
+    if (h->ws2) hipFree(h->ws2);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
 }
@@ -187,6 +190,24 @@ void *hp2d_handle_workspace(hp2d_handle *h, size_t bytes) {
     }
     h->ws_bytes = bytes;
     return h->ws;
+}
+
+void *hp2d_handle_workspace2(hp2d_handle *h, size_t bytes) {
+    if (h->ws2_bytes >= bytes) return h->ws2;
+    if (h->ws2) {
+        hipStreamSynchronize(h->stream);
+        hipFree(h->ws2);
+        h->ws2 = nullptr;
+        h->ws2_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&h->ws2, bytes);
+    if (e != hipSuccess) {
+        h->ws2 = nullptr;
+        set_err(h, std::string("workspace hipMalloc: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    h->ws2_bytes = bytes;
+    return h->ws2;
 }
 
 template <typename T>

@@ -25,26 +25,9 @@
 #include "hp2d.h"
 #include "hp2d_handle.hpp"
 #include "hp2d_internal.hpp"
+#include "hp2d_gw.hpp"
 
 namespace hp2d {
-
-constexpr int kGwBlock = 256;
-constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
-
-struct GwCtl {
-    double t, T, dt, dt_used, n_steps;
-    unsigned long long dtgw_bits;  // min positive finite Courant candidate, as bits
-    int done, nsub;
-};
-
-struct GwArgs {
-    int ny, nx;
-    long long n;
-    double dx, dy, courant;
-    const double *z0, *Sy, *R, *K, *h_soil, *h_dir;
-    const uint8_t *catchment, *dirichlet;  // NULL = all inside / none
-    GwCtl *ctl;
-};
 
 __device__ __forceinline__ double signum(double F) {
     return isnan(F) ? NAN : (double)((F > 0.0) - (F < 0.0));
@@ -241,6 +224,90 @@ __global__ __launch_bounds__(kGwBlock) void gw_colsum_kernel(GwArgs a, const dou
     out[4 * j + 3] = s3;
 }
 
+hp2d_status gw_run(hp2d_handle *h, const GwArgs &a_in, double dt, const double *dh0,
+                   const GwScratch &w, double *error, int32_t *n_substeps) {
+    const int ny = a_in.ny, nx = a_in.nx;
+    const size_t n = (size_t)ny * nx, nb = n * sizeof(double);
+    const int blocks = (int)((n + kGwBlock - 1) / kGwBlock);
+    const int cblocks = (nx + kGwBlock - 1) / kGwBlock;
+    hipStream_t s = hp2d_handle_stream(h);
+    GwArgs a = a_in;
+    a.ctl = w.ctl;
+    hipError_t e = hipMemcpyAsync(w.ht, dh0, nb, hipMemcpyDeviceToDevice, s);
+    const dim3 g(blocks), b(kGwBlock);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gw_ctl_init, dim3(1), dim3(1), 0, s, w.ctl, dt);
+        // velocities of the entry head (not masked, Boussinesq_2D_explicit.m:91)
+        hipLaunchKernelGGL(gw_velocity_kernel, g, b, 0, s, a, (const double *)w.ht, w.ux, w.uy,
+                           (double *)nullptr, 0, 0);
+        e = hipGetLastError();
+    }
+    GwCtl hc{};
+    const int kMaxSub = 1000000;
+    int batch = 1;
+    while (e == hipSuccess) {
+        for (int k = 0; k < batch; ++k) {
+            hipLaunchKernelGGL(gw_ctl_begin, dim3(1), dim3(1), 0, s, w.ctl);
+            hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)w.ht, w.Fx, w.Fy, 0);
+            hipLaunchKernelGGL(gw_predict_kernel, g, b, 0, s, a, (const double *)w.ht,
+                               (const double *)w.Fx, (const double *)w.Fy, w.hp);
+            hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)w.hp, w.Fx, w.Fy, 1);
+            hipLaunchKernelGGL(gw_update_kernel, g, b, 0, s, a, (const double *)w.ht,
+                               (const double *)w.Fx, (const double *)w.Fy, w.tmp, w.q);
+            hipLaunchKernelGGL(gw_velocity_kernel, g, b, 0, s, a, (const double *)w.tmp, w.ux, w.uy,
+                               w.ht, 1, 1);
+            hipLaunchKernelGGL(gw_ctl_end, dim3(1), dim3(1), 0, s, w.ctl);
+        }
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&hc, w.ctl, sizeof(GwCtl), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess || hc.done) break;
+        if (hc.nsub > kMaxSub) return hp2d_handle_fail(h, "Boussinesq: more than 1e6 sub-steps");
+        batch = batch < 64 ? batch * 2 : 64;
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gw_colsum_kernel, dim3(cblocks), b, 0, s, a, (const double *)w.q, dh0,
+                           (const double *)w.ht, w.col);
+        e = hipGetLastError();
+    }
+    std::vector<double> col(4 * (size_t)nx);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(col.data(), w.col, col.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hp2d_handle_fail(h, std::string("Boussinesq: ") + hipGetErrorString(e));
+    double sums[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < nx; ++j)
+        for (int k = 0; k < 4; ++k) sums[k] += col[4 * (size_t)j + k];
+    const double A = a.dx * a.dy;
+    const double recharge_mass = sums[0] * A, exfil_mass = sums[1] * A;
+    const double storage_prev = sums[2] * A, storage_curr = sums[3] * A;
+    *error = recharge_mass - exfil_mass - (storage_curr - storage_prev);
+    if (n_substeps) *n_substeps = hc.nsub;
+    return HP2D_OK;
+}
+
+size_t gw_scratch_bytes(int ny, int nx) {
+    const size_t n = (size_t)ny * nx;
+    return 8 * n * sizeof(double) + 4 * (size_t)nx * sizeof(double) + 256;
+}
+
+GwScratch gw_scratch(char *base, int ny, int nx) {
+    const size_t n = (size_t)ny * nx;
+    GwScratch w;
+    double *p = (double *)base;
+    w.ht = p; p += n;
+    w.hp = p; p += n;
+    w.tmp = p; p += n;
+    w.Fx = p; p += n;
+    w.Fy = p; p += n;
+    w.ux = p; p += n;
+    w.uy = p; p += n;
+    w.q = p; p += n;
+    w.col = p;
+    w.ctl = (GwCtl *)(base + 8 * n * sizeof(double) + 4 * (size_t)nx * sizeof(double));
+    return w;
+}
+
 }  // namespace hp2d
 
 extern "C" hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, double dt, double dx, double dy,
@@ -259,32 +326,20 @@ extern "C" hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, double dt, double dx, 
     hp2d_status st = hp2d_handle_shape(h, &ny, &nx, &prec);
     if (st != HP2D_OK) return st;
     if (ny < 2 || nx < 2) return HP2D_E_ARG;  // the reference indexes column/row 2
-    const size_t n = (size_t)ny * nx;
-    const int blocks = (int)((n + kGwBlock - 1) / kGwBlock);
-    const int cblocks = (nx + kGwBlock - 1) / kGwBlock;
+    const size_t n = (size_t)ny * nx, nb = n * sizeof(double);
     hipStream_t s = hp2d_handle_stream(h);
-
-    // workspace: z0 Sy R K h_soil h_dir h0 h_t h_p h_tmp Fx Fy ux uy q | colsums | ctl | masks
-    const size_t n_arrays = 15;
-    const size_t col_off = n_arrays * n * sizeof(double);
-    const size_t ctl_off = col_off + 4 * (size_t)nx * sizeof(double);
-    const size_t mask_off = ctl_off + 256;
-    char *ws = (char *)hp2d_handle_workspace(h, mask_off + 2 * n + 256);
+    // workspace: inputs z0 Sy R K h_soil h_dir h0 | masks | Boussinesq scratch
+    const size_t in_bytes = 7 * nb;
+    const size_t mask_off = in_bytes;
+    const size_t scr_off = ((mask_off + 2 * n + 255) / 256) * 256;
+    char *ws = (char *)hp2d_handle_workspace(h, scr_off + gw_scratch_bytes(ny, nx) + 256);
     if (!ws) return HP2D_E_NOMEM;
     double *p = (double *)ws;
-    auto take = [&]() {
-        double *r = p;
-        p += n;
-        return r;
-    };
-    double *dz0 = take(), *dSy = take(), *dR = take(), *dK = take(), *dhs = take(),
-           *dhd = take(), *dh0 = take(), *dht = take(), *dhp = take(), *dtmp = take(),
-           *dFx = take(), *dFy = take(), *dux = take(), *duy = take(), *dq = take();
-    double *dcol = (double *)(ws + col_off);
-    GwCtl *ctl = (GwCtl *)(ws + ctl_off);
+    double *dz0 = p, *dSy = p + n, *dR = p + 2 * n, *dK = p + 3 * n, *dhs = p + 4 * n,
+           *dhd = p + 5 * n, *dh0 = p + 6 * n;
     uint8_t *dcm = catchment_mask ? (uint8_t *)(ws + mask_off) : nullptr;
     uint8_t *ddm = dirichlet_mask ? (uint8_t *)(ws + mask_off + n) : nullptr;
-
+    const GwScratch w = gw_scratch(ws + scr_off, ny, nx);
     hipError_t e = hipSuccess;
     auto H2D = [&](void *dst, const void *src, size_t bytes) {
         if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
@@ -292,20 +347,19 @@ extern "C" hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, double dt, double dx, 
     auto D2H = [&](void *dst, const void *src, size_t bytes) {
         if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
     };
-    const size_t nb = n * sizeof(double);
     H2D(dz0, z0, nb);
     H2D(dSy, Sy, nb);
     H2D(dR, R, nb);
     H2D(dK, K, nb);
     H2D(dhs, h_soil, nb);
     H2D(dh0, h_in, nb);
-    H2D(dht, h_in, nb);
     if (ddm) {
         H2D(dhd, h_dirichlet, nb);
         H2D(ddm, dirichlet_mask, n);
     }
     if (dcm) H2D(dcm, catchment_mask, n);
-
+    if (e != hipSuccess)
+        return hp2d_handle_fail(h, std::string("hp2d_boussinesq_2d: ") + hipGetErrorString(e));
     GwArgs a;
     a.ny = ny;
     a.nx = nx;
@@ -321,63 +375,17 @@ extern "C" hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, double dt, double dx, 
     a.h_dir = dhd;
     a.catchment = dcm;
     a.dirichlet = ddm;
-    a.ctl = ctl;
-
-    const dim3 g(blocks), b(kGwBlock);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(gw_ctl_init, dim3(1), dim3(1), 0, s, ctl, dt);
-        // velocities of the entry head (not masked, :91)
-        hipLaunchKernelGGL(gw_velocity_kernel, g, b, 0, s, a, dht, dux, duy, (double *)nullptr, 0,
-                           0);
-        e = hipGetLastError();
-    }
-    GwCtl hc{};
-    const int kMaxSub = 1000000;
-    int batch = 1;
-    while (e == hipSuccess) {
-        for (int k = 0; k < batch; ++k) {
-            hipLaunchKernelGGL(gw_ctl_begin, dim3(1), dim3(1), 0, s, ctl);
-            hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)dht, dFx, dFy, 0);
-            hipLaunchKernelGGL(gw_predict_kernel, g, b, 0, s, a, (const double *)dht,
-                               (const double *)dFx, (const double *)dFy, dhp);
-            hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)dhp, dFx, dFy, 1);
-            hipLaunchKernelGGL(gw_update_kernel, g, b, 0, s, a, (const double *)dht,
-                               (const double *)dFx, (const double *)dFy, dtmp, dq);
-            hipLaunchKernelGGL(gw_velocity_kernel, g, b, 0, s, a, (const double *)dtmp, dux, duy,
-                               dht, 1, 1);
-            hipLaunchKernelGGL(gw_ctl_end, dim3(1), dim3(1), 0, s, ctl);
-        }
-        e = hipGetLastError();
-        D2H(&hc, ctl, sizeof(GwCtl));
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess || hc.done) break;
-        if (hc.nsub > kMaxSub)
-            return hp2d_handle_fail(h, "hp2d_boussinesq_2d: more than 1e6 sub-steps");
-        batch = batch < 64 ? batch * 2 : 64;
-    }
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(gw_colsum_kernel, dim3(cblocks), b, 0, s, a, (const double *)dq,
-                           (const double *)dh0, (const double *)dht, dcol);
-        e = hipGetLastError();
-    }
-    std::vector<double> col(4 * (size_t)nx);
-    D2H(col.data(), dcol, col.size() * sizeof(double));
-    D2H(h_t, dht, nb);
-    D2H(q_exf, dq, nb);
-    if (u_x) D2H(u_x, dux, nb);
-    if (u_y) D2H(u_y, duy, nb);
+    a.ctl = w.ctl;
+    st = gw_run(h, a, dt, dh0, w, error, n_substeps);
+    if (st != HP2D_OK) return st;
+    D2H(h_t, w.ht, nb);
+    D2H(q_exf, w.q, nb);
+    if (u_x) D2H(u_x, w.ux, nb);
+    if (u_y) D2H(u_y, w.uy, nb);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess)
         return hp2d_handle_fail(h, std::string("hp2d_boussinesq_2d: ") + hipGetErrorString(e));
-    double sums[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int j = 0; j < nx; ++j)
-        for (int k = 0; k < 4; ++k) sums[k] += col[4 * (size_t)j + k];
-    const double A = dx * dy;
-    const double recharge_mass = sums[0] * A, exfil_mass = sums[1] * A;
-    const double storage_prev = sums[2] * A, storage_curr = sums[3] * A;
-    *error = recharge_mass - exfil_mass - (storage_curr - storage_prev);
     if (q_river)
         for (size_t c = 0; c < n; ++c) q_river[c] = 0.0;
-    if (n_substeps) *n_substeps = hc.nsub;
     return HP2D_OK;
 }

@@ -1,0 +1,377 @@
+// hp2d_gwm.hip - the groundwater module step (Groundwater_Module.m:37-404, no
+// layered soil) for gfx950.
+//
+// Per-cell work runs in elementwise kernels: the van Genuchten / Mualem
+// recharge bucket (simulate_groundwater_recharge.m:55-215) with the pending
+// exchange, the scheduler's per-cell terms (head change, diffusivity), the
+// local head update, the return of exfiltrated and excess water to the
+// surface.  Reductions are per-column FP64 sums / NaN-omitting maxima, folded
+// on the host in column order, where the scheduler's scalar decision is taken.
+// The Boussinesq update runs the device solver of hp2d_gw.hip in place.
+#include <cfloat>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "hp2d.h"
+#include "hp2d_gw.hpp"
+#include "hp2d_handle.hpp"
+
+namespace hp2d {
+
+struct GwmArgs {
+    int ny, nx;
+    long long n;
+    double dt_s, A, l_vg;
+    const double *elev, *soil_depth, *ts, *tr, *nv, *alpha, *ksat, *ksat_gw, *Sy;
+    double *h, *I, *d, *cum, *maxd, *pn, *pr, *pc;
+    double *z_bed, *Kgw, *rfg, *q, *t1, *t2, *t3;
+};
+
+// recharge bucket, pending exchange and the scheduler's per-cell terms
+__global__ __launch_bounds__(256) void gwm_recharge_kernel(GwmArgs a) {
+    const long long c = blockIdx.x * 256LL + threadIdx.x;
+    if (c >= a.n) return;
+    const bool nan = isnan(a.elev[c]);
+    const double zero = nan ? NAN : 0.0;
+    const double gw_depth = a.h[c] - (a.elev[c] - a.soil_depth[c]);
+    double zwt = a.soil_depth[c] - gw_depth;
+    zwt = fmax(zwt, 0.0);
+    zwt = fmin(zwt, a.soil_depth[c]);
+    const double ts = a.ts[c], tr = a.tr[c];
+    const double uzmax = zwt * (ts - tr);
+    // simulate_groundwater_recharge with infiltration = zero_matrix, min = 0
+    const double dt = fmax(a.dt_s, 1e-12);
+    const double nvg = a.nv[c], m = 1.0 - 1.0 / nvg, al = a.alpha[c];
+    const double Ks = a.ksat[c] / 1000.0 / 3600.0;
+    const double inf_r = isfinite(zero) ? zero : 0.0;
+    const double S0 = a.I[c] / 1000.0;
+    const double s0 = isfinite(S0) ? S0 : 0.0;
+    const double mx = isfinite(uzmax) ? uzmax : 0.0;
+    const double min_eff = fmin(0.0, mx);
+    double S = s0;
+    S = fmax(S, min_eff);
+    S = fmin(S, mx);
+    const double por_eff = fmax(ts - tr, 1e-12);
+    double zb = mx / por_eff;
+    zb = fmax(zb, 0.0);
+    const double Lgw = fmax(0.5 * zb, 1e-6);
+    const double zb_safe = fmax(zb, 1e-6);
+    double theta = tr + S / zb_safe;
+    theta = fmax(theta, tr);
+    theta = fmin(theta, ts);
+    double Se = (theta - tr) / fmax(ts - tr, 1e-12);
+    Se = fmin(fmax(Se, 1e-6), 1.0);
+    double hs = -(1.0 / al) * pow(pow(Se, -1.0 / m) - 1.0, 1.0 / nvg);
+    if (Se >= 0.999999) hs = 0.0;
+    const double term = 1.0 - pow(Se, 1.0 / m);
+    double Kr = pow(Se, a.l_vg) * pow(1.0 - pow(term, m), 2.0);
+    Kr = fmin(fmax(Kr, 0.0), 1.0);
+    double grad = 1.0 + hs / Lgw;
+    grad = fmax(grad, -1.0);
+    grad = fmin(grad, 2.0);
+    double Rr = (Ks * Kr) * grad;
+    const bool no_uz = mx <= 0.0;
+    if (no_uz) Rr = 0.0;
+    double S_new = S + dt * (inf_r - Rr);
+    S_new = fmax(S_new, min_eff);
+    S_new = fmin(S_new, mx);
+    if (no_uz) S_new = 0.0;
+    double R_mb = inf_r - (S_new - S) / dt;
+    if (no_uz) R_mb = 0.0;
+    a.I[c] = S_new * 1000.0;
+    if (a.cum) a.cum[c] = a.cum[c] + fmax(R_mb, 0.0) * dt * 1000.0;
+    // exchange (the capillary rate is the zero matrix without layered soil)
+    double rr = R_mb - zero;
+    if (nan) rr = NAN;
+    const double down = fmax(R_mb, 0.0) * a.dt_s;
+    a.pn[c] = a.pn[c] + rr * a.dt_s;
+    a.pr[c] = a.pr[c] + down;
+    a.pc[c] = a.pc[c] + fmax(zero, 0.0) * a.dt_s;
+    a.t1[c] = down * (nan ? NAN : a.A);
+    // scheduler terms: pending head change and diffusivity K H / Sy
+    const double Sy_head = fmax(a.Sy[c], 1e-6);
+    a.t2[c] = fabs(a.pn[c]) / fmax(Sy_head, 1e-9);
+    const double zbed = a.elev[c] - a.soil_depth[c];
+    double K = a.ksat_gw[c] / 1000.0 / 3600.0;
+    if (!isfinite(K)) K = NAN;
+    double sat = fmax(a.h[c] - zbed, 1e-4);
+    if (nan) sat = NAN;
+    a.t3[c] = K * sat / Sy_head;
+    a.z_bed[c] = zbed;
+    a.Kgw[c] = a.ksat_gw[c] / 1000.0 / 3600.0;
+}
+
+// per column: [0] nansum(t1), [1] max(t2) omitting NaN, [2] max(t3) omitting NaN
+__global__ __launch_bounds__(256) void gwm_colred_kernel(int ny, int nx, const double *t1,
+                                                         const double *t2, const double *t3,
+                                                         double *out) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= nx) return;
+    double s = 0.0, m2 = NAN, m3 = NAN;
+    for (int i = 0; i < ny; ++i) {
+        const long long c = (long long)j * ny + i;
+        if (!isnan(t1[c])) s += t1[c];
+        if (t2) m2 = fmax(m2, t2[c]);
+        if (t3) m3 = fmax(m3, t3[c]);
+    }
+    out[3 * j] = s;
+    out[3 * j + 1] = m2;
+    out[3 * j + 2] = m3;
+}
+
+// recharge_for_gw (:199-203) and q_exf = zero_matrix
+__global__ __launch_bounds__(256) void gwm_rfg_kernel(GwmArgs a, double dt_gw) {
+    const long long c = blockIdx.x * 256LL + threadIdx.x;
+    if (c >= a.n) return;
+    const bool nan = isnan(a.elev[c]);
+    a.rfg[c] = nan ? NAN : a.pn[c] / dt_gw;
+    a.q[c] = nan ? NAN : 0.0;
+}
+
+// after Boussinesq: take the head and q_exf, exfiltration onto the surface (:248-250)
+__global__ __launch_bounds__(256) void gwm_after_bq_kernel(GwmArgs a, const double *h_new,
+                                                           const double *q_new, double dt_gw) {
+    const long long c = blockIdx.x * 256LL + threadIdx.x;
+    if (c >= a.n) return;
+    a.h[c] = h_new[c];
+    const double q = q_new[c];
+    a.q[c] = q;
+    double dn = a.d[c] + dt_gw * (q * 1000.0);
+    if (dn < 0.0) dn = 0.0;
+    a.d[c] = dn;
+}
+
+// local water-table update without baseflow (:252-263)
+__global__ __launch_bounds__(256) void gwm_local_kernel(GwmArgs a, double dt_gw) {
+    const long long c = blockIdx.x * 256LL + threadIdx.x;
+    if (c >= a.n) return;
+    const double Sy_l = fmax(a.Sy[c], 1e-6);
+    const double dh = a.rfg[c] * dt_gw / Sy_l;
+    double hc = a.h[c] + dh;
+    hc = fmax(hc, a.z_bed[c]);
+    const double ex_water = fmax(hc - a.elev[c], 0.0) * Sy_l;
+    a.h[c] = fmin(hc, a.elev[c]);
+    double dn = a.d[c] + ex_water * 1000.0;
+    if (dn < 0.0) dn = 0.0;
+    a.d[c] = dn;
+}
+
+// after an update: exfiltrated volume, pending reset, unsaturated-zone excess (:266-395)
+__global__ __launch_bounds__(256) void gwm_post_kernel(GwmArgs a, double dt_gw) {
+    const long long c = blockIdx.x * 256LL + threadIdx.x;
+    if (c >= a.n) return;
+    const bool nan = isnan(a.elev[c]);
+    a.t1[c] = fmax(a.q[c], 0.0) * dt_gw * (nan ? NAN : a.A);
+    a.pn[c] = 0.0;
+    a.pr[c] = 0.0;
+    a.pc[c] = 0.0;
+    double zwt = a.elev[c] - a.h[c];
+    zwt = fmax(zwt, 0.0);
+    zwt = fmin(zwt, a.soil_depth[c]);
+    const double uz_new = zwt * (a.ts[c] - a.tr[c]);
+    a.t2[c] = fmax(a.I[c] / 1000.0 - uz_new, 0.0);  // excess [m]
+}
+
+__global__ __launch_bounds__(256) void gwm_excess_kernel(GwmArgs a) {
+    const long long c = blockIdx.x * 256LL + threadIdx.x;
+    if (c >= a.n) return;
+    const double Sm = a.I[c] / 1000.0, ex = a.t2[c];
+    a.I[c] = fmax((Sm - ex) * 1000.0, 0.0);
+    double dn = a.d[c] + ex * 1000.0;
+    if (dn < 0.0) dn = 0.0;
+    a.d[c] = dn;
+}
+
+__global__ __launch_bounds__(256) void gwm_maxdepth_kernel(GwmArgs a) {
+    const long long c = blockIdx.x * 256LL + threadIdx.x;
+    if (c >= a.n || !a.maxd) return;
+    a.maxd[c] = fmax(a.maxd[c], a.h[c] - (a.elev[c] - a.soil_depth[c]));
+}
+
+}  // namespace hp2d
+
+extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_params *p,
+                                               hp2d_gw_state *g, double *h_t, double *I_t,
+                                               double *d_t, double *cumulative_recharge,
+                                               double *max_gw_depth, double *q_exf, double *error,
+                                               int32_t *ran) {
+    using namespace hp2d;
+    if (!h || !p || !g || !h_t || !I_t || !d_t || !q_exf || !error || !ran || !p->elevation ||
+        !p->soil_depth || !p->theta_sat || !p->theta_r || !p->n_vg || !p->alpha_vg ||
+        !p->ksat_mmh || !p->ksat_gw_mmh || !p->Sy || !g->pending_net || !g->pending_recharge ||
+        !g->pending_capillary || !(p->resolution > 0.0))
+        return HP2D_E_ARG;
+    int32_t ny = 0, nx = 0, prec = 0;
+    hp2d_status st = hp2d_handle_shape(h, &ny, &nx, &prec);
+    if (st != HP2D_OK) return st;
+    if (ny != p->ny || nx != p->nx || (p->flag_baseflow == 1 && (ny < 2 || nx < 2))) return HP2D_E_ARG;
+    const size_t n = (size_t)ny * nx, nb = n * sizeof(double);
+    hipStream_t s = hp2d_handle_stream(h);
+    // module arrays in the second workspace: 9 inputs, 8 state, 7 temporaries | colsums | mask
+    const size_t n_arr = 24;
+    const size_t col_off = n_arr * nb;
+    const size_t mask_off = col_off + 3 * (size_t)nx * sizeof(double);
+    char *ws = (char *)hp2d_handle_workspace2(h, mask_off + n + 256);
+    if (!ws) return HP2D_E_NOMEM;
+    double *q0 = (double *)ws;
+    auto arr = [&](int k) { return q0 + (size_t)k * n; };
+    GwmArgs a;
+    a.ny = ny;
+    a.nx = nx;
+    a.n = (long long)n;
+    a.dt_s = p->time_step_min * 60.0;
+    a.A = p->resolution * p->resolution;
+    a.l_vg = p->l_vg;
+    double *in[9] = {arr(0), arr(1), arr(2), arr(3), arr(4), arr(5), arr(6), arr(7), arr(8)};
+    a.elev = in[0]; a.soil_depth = in[1]; a.ts = in[2]; a.tr = in[3]; a.nv = in[4]; a.alpha = in[5];
+    a.ksat = in[6]; a.ksat_gw = in[7]; a.Sy = in[8];
+    a.h = arr(9); a.I = arr(10); a.d = arr(11); a.cum = cumulative_recharge ? arr(12) : nullptr;
+    a.maxd = max_gw_depth ? arr(13) : nullptr; a.pn = arr(14); a.pr = arr(15); a.pc = arr(16);
+    a.z_bed = arr(17); a.Kgw = arr(18); a.rfg = arr(19); a.q = arr(20); a.t1 = arr(21);
+    a.t2 = arr(22); a.t3 = arr(23);
+    double *col = (double *)(ws + col_off);
+    uint8_t *dom = p->domain ? (uint8_t *)(ws + mask_off) : nullptr;
+    hipError_t e = hipSuccess;
+    auto H2D = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    };
+    auto D2H = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    };
+    const double *hin[9] = {p->elevation, p->soil_depth, p->theta_sat, p->theta_r, p->n_vg,
+                            p->alpha_vg, p->ksat_mmh, p->ksat_gw_mmh, p->Sy};
+    for (int k = 0; k < 9; ++k) H2D(in[k], hin[k], nb);
+    H2D(a.h, h_t, nb);
+    H2D(a.I, I_t, nb);
+    H2D(a.d, d_t, nb);
+    if (a.cum) H2D(a.cum, cumulative_recharge, nb);
+    if (a.maxd) H2D(a.maxd, max_gw_depth, nb);
+    H2D(a.pn, g->pending_net, nb);
+    H2D(a.pr, g->pending_recharge, nb);
+    H2D(a.pc, g->pending_capillary, nb);
+    if (dom) H2D(dom, p->domain, n);
+    const dim3 G((unsigned)((n + 255) / 256)), B(256), CB((unsigned)((nx + 255) / 256));
+    std::vector<double> hc(3 * (size_t)nx);
+    auto fold = [&](double &sum, double &m2, double &m3) {
+        sum = 0.0;
+        m2 = NAN;
+        m3 = NAN;
+        for (int j = 0; j < nx; ++j) {
+            sum += hc[3 * (size_t)j];
+            m2 = std::fmax(m2, hc[3 * (size_t)j + 1]);
+            m3 = std::fmax(m3, hc[3 * (size_t)j + 2]);
+        }
+    };
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gwm_recharge_kernel, G, B, 0, s, a);
+        hipLaunchKernelGGL(gwm_colred_kernel, CB, B, 0, s, ny, nx, (const double *)a.t1,
+                           (const double *)a.t2, (const double *)a.t3, col);
+        e = hipGetLastError();
+    }
+    D2H(hc.data(), col, hc.size() * 8);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return hp2d_handle_fail(h, std::string("hp2d_groundwater_module: ") + hipGetErrorString(e));
+    double rech_sum, hc_max, maxD;
+    fold(rech_sum, hc_max, maxD);
+    g->elapsed_s = g->elapsed_s + a.dt_s;
+    g->total_recharge_m3 = g->total_recharge_m3 + rech_sum;
+    g->total_capillary_m3 = g->total_capillary_m3 + 0.0;
+    // scheduler (initialize_groundwater_scheduler.m:62-65, estimate_...m, :166-194)
+    const double target_dt_s = std::fmax(p->target_dt_min * 60.0, DBL_EPSILON);
+    const double min_dt_s = std::fmax(p->min_dt_min * 60.0, DBL_EPSILON);
+    const double max_head_change = std::fmax(p->max_head_change_m, DBL_EPSILON);
+    const double courant_c = std::fmax(std::fmin(p->courant, 1.0), DBL_EPSILON);
+    double dt_stable = target_dt_s;
+    if (p->flag_baseflow == 1 && std::isfinite(maxD) && maxD > 0) {
+        const double L = p->resolution;
+        dt_stable = courant_c * (L * L) / maxD / 4.0;
+        dt_stable = std::fmin(std::fmax(dt_stable, 1.0), target_dt_s);
+    }
+    double target = std::fmin(target_dt_s, dt_stable);
+    target = std::fmax(target, min_dt_s);
+    const bool head_due = hc_max >= max_head_change;
+    const bool final_step = p->t_min >= (p->routing_time_min - 1e-9);
+    const bool due = p->flag_async == 0 ? true : (g->elapsed_s >= target || head_due || final_step);
+    const double dt_gw = std::fmax(g->elapsed_s, DBL_EPSILON);
+    double err = 0.0;
+    if (due) {
+        hipLaunchKernelGGL(gwm_rfg_kernel, G, B, 0, s, a, dt_gw);
+        e = hipGetLastError();
+        if (p->flag_baseflow == 1) {
+            char *wsb = (char *)hp2d_handle_workspace(h, gw_scratch_bytes(ny, nx) + 256);
+            if (!wsb) return HP2D_E_NOMEM;
+            const GwScratch w = gw_scratch(wsb, ny, nx);
+            GwArgs ga;
+            ga.ny = ny;
+            ga.nx = nx;
+            ga.n = (long long)n;
+            ga.dx = p->resolution;
+            ga.dy = p->resolution;
+            ga.courant = p->courant;
+            ga.z0 = a.z_bed;
+            ga.Sy = a.Sy;
+            ga.R = a.rfg;
+            ga.K = a.Kgw;
+            ga.h_soil = a.soil_depth;
+            ga.h_dir = nullptr;
+            ga.catchment = dom;
+            ga.dirichlet = nullptr;
+            ga.ctl = w.ctl;
+            if (e == hipSuccess) {
+                st = gw_run(h, ga, dt_gw, a.h, w, &err, nullptr);
+                if (st != HP2D_OK) return st;
+                hipLaunchKernelGGL(gwm_after_bq_kernel, G, B, 0, s, a, (const double *)w.ht,
+                                   (const double *)w.q, dt_gw);
+                e = hipGetLastError();
+            }
+        } else if (e == hipSuccess) {
+            hipLaunchKernelGGL(gwm_local_kernel, G, B, 0, s, a, dt_gw);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(gwm_post_kernel, G, B, 0, s, a, dt_gw);
+            hipLaunchKernelGGL(gwm_colred_kernel, CB, B, 0, s, ny, nx, (const double *)a.t1,
+                               (const double *)a.t2, (const double *)nullptr, col);
+            e = hipGetLastError();
+        }
+        D2H(hc.data(), col, hc.size() * 8);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess)
+            return hp2d_handle_fail(h, std::string("hp2d_groundwater_module: ") + hipGetErrorString(e));
+        double exf, ex_max, unused;
+        fold(exf, ex_max, unused);
+        g->total_exfiltration_m3 = g->total_exfiltration_m3 + exf;
+        g->n_updates += 1;
+        g->elapsed_s = 0.0;
+        g->last_dt_s = dt_gw;
+        if (ex_max > 0) {  // any(excess_m(:) > 0)
+            hipLaunchKernelGGL(gwm_excess_kernel, G, B, 0, s, a);
+            e = hipGetLastError();
+        }
+    } else {
+        g->last_dt_s = 0.0;
+        hipLaunchKernelGGL(gwm_rfg_kernel, G, B, 0, s, a, 1.0);  // q_exf = zero_matrix
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gwm_maxdepth_kernel, G, B, 0, s, a);
+        e = hipGetLastError();
+    }
+    D2H(h_t, a.h, nb);
+    D2H(I_t, a.I, nb);
+    D2H(d_t, a.d, nb);
+    D2H(q_exf, a.q, nb);
+    if (a.cum) D2H(cumulative_recharge, a.cum, nb);
+    if (a.maxd) D2H(max_gw_depth, a.maxd, nb);
+    D2H(g->pending_net, a.pn, nb);
+    D2H(g->pending_recharge, a.pr, nb);
+    D2H(g->pending_capillary, a.pc, nb);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return hp2d_handle_fail(h, std::string("hp2d_groundwater_module: ") + hipGetErrorString(e));
+    *error = err;
+    *ran = due ? 1 : 0;
+    return HP2D_OK;
+}

@@ -16,3 +16,6 @@ hp2d_status hp2d_handle_fail(hp2d_handle *h, const std::string &msg);
 // device workspace of at least `bytes` owned by the handle (grown on demand,
 // kept until hp2d_destroy); NULL on allocation failure (error recorded)
 void *hp2d_handle_workspace(hp2d_handle *h, size_t bytes);
+// a second, independent workspace (a module that calls another module keeps
+// its own arrays here while the callee uses the first)
+void *hp2d_handle_workspace2(hp2d_handle *h, size_t bytes);

@@ -320,13 +320,17 @@ def stack_targets(net, peak_m, tpeak_h):
     return torch.from_numpy(t).to(net.dev)
 
 
-def train(net, x, y, epochs=10, batch=32, lr=1e-3, seed=0, log=None):
-    """Mini-batch Adam over (x, y) device tensors; returns the loss of each epoch."""
+def train(net, x, y, epochs=10, batch=32, lr=1e-3, seed=0, log=None, lr_final=None):
+    """Mini-batch Adam over (x, y) device tensors; returns the loss of each epoch.
+    With lr_final the learning rate follows a cosine from lr to lr_final."""
     torch = _torch()
     rng = np.random.default_rng(seed)
     n = x.shape[0]
     hist = []
+    lr0 = lr
     for ep in range(epochs):
+        if lr_final is not None and epochs > 1:
+            lr = lr_final + 0.5 * (lr0 - lr_final) * (1 + np.cos(np.pi * ep / (epochs - 1)))
         order = rng.permutation(n)
         tot, cnt = 0.0, 0
         for i in range(0, n - batch + 1, batch):

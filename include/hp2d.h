@@ -221,6 +221,41 @@ hp2d_status hp2d_infiltration_module(hp2d_handle *h, double time_step_min, doubl
                                      double *I_p, double *f, double *cum_inf, double *inf_volume,
                                      double *dS_inf);
 
+/* Groundwater_Module.m:37-404 (flag_groundwater_modeling = 1, no layered
+ * soil, so no capillary rise) with initialize_groundwater_scheduler.m,
+ * simulate_groundwater_recharge.m:55-215 and
+ * estimate_groundwater_stable_timestep.m: one call per Main_While step.  The
+ * unsaturated-zone bucket drains into the pending exchange; when the
+ * scheduler's update is due (flag_groundwater_async = 0, or the elapsed time
+ * reaches min(target, stable dt), or the pending head change reaches the cap,
+ * or the final step) the exchange drives Boussinesq_2D_explicit
+ * (flag_baseflow = 1) or a local head change, and the water above the ground
+ * and the unsaturated-zone excess return to the surface. */
+typedef struct hp2d_gw_params {
+    int32_t ny, nx;
+    double time_step_min, resolution, t_min, routing_time_min;
+    int32_t flag_baseflow, flag_async;
+    double target_dt_min, min_dt_min, max_head_change_m, courant, l_vg; /* 1440, 1, 0.25, 0.25, 0.5 */
+    const double *elevation, *soil_depth, *theta_sat, *theta_r, *n_vg, *alpha_vg;
+    const double *ksat_mmh, *ksat_gw_mmh, *Sy;   /* ny*nx; NaN elevation = outside */
+    const uint8_t *domain;                        /* catchment mask (NULL = all) */
+} hp2d_gw_params;
+
+/* GW_States: pending exchange maps [m] (ny*nx, NaN outside at first) and scalars */
+typedef struct hp2d_gw_state {
+    double *pending_net, *pending_recharge, *pending_capillary;
+    double elapsed_s, total_recharge_m3, total_capillary_m3, total_exfiltration_m3, last_dt_s;
+    int32_t n_updates;
+} hp2d_gw_state;
+
+/* Host arrays ny*nx updated in place: h_t [m], I_t [mm], d_t [mm],
+ * cumulative_recharge [mm], max_gw_depth [m]; state updated; q_exf [m/s] out;
+ * *error = errors(5) [m3]; *ran = 1 when the update ran. */
+hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *state,
+                                    double *h_t, double *I_t, double *d_t,
+                                    double *cumulative_recharge, double *max_gw_depth,
+                                    double *q_exf, double *error, int32_t *ran);
+
 /* Depression filling of a DEM (host preprocessing; the reference fills with
  * TopoToolbox): Priority-Flood+epsilon (Barnes et al., Computers & Geosciences
  * 62, 2014) seeded at the seed_mask cells (NULL = the grid perimeter and the

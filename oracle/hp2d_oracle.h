@@ -122,6 +122,28 @@ int32_t oracle_infiltration(int32_t ny, int32_t nx, double time_step_min, double
                             double *I_t, double *I_p, double *f_out, double *cum_inf,
                             double *inf_volume, double *dS_inf);
 
+/* Groundwater_Module step (no layered soil) */
+typedef struct oracle_gwm_params {
+    int32_t ny, nx;
+    double time_step_min, resolution, t_min, routing_time_min;
+    int32_t flag_baseflow, flag_async;
+    double target_dt_min, min_dt_min, max_head_change_m, courant, l_vg;
+    const double *elevation, *soil_depth, *theta_sat, *theta_r, *n_vg, *alpha_vg;
+    const double *ksat_mmh, *ksat_gw_mmh, *Sy;
+    const uint8_t *domain;   /* catchment mask for the Boussinesq update, NULL = all */
+} oracle_gwm_params;
+
+typedef struct oracle_gwm_state {
+    double *pending_net, *pending_recharge, *pending_capillary;  /* ny*nx [m] */
+    double elapsed_s, total_recharge_m3, total_capillary_m3, total_exfiltration_m3, last_dt_s;
+    int32_t n_updates;
+} oracle_gwm_state;
+
+int32_t oracle_groundwater_module(const oracle_gwm_params *p, oracle_gwm_state *g, double *h_t,
+                                  double *I_t, double *d_t, double *cumulative_recharge,
+                                  double *max_gw_depth, double *q_exf, double *error,
+                                  int32_t *ran, double *dt_gw_out);
+
 #ifdef __cplusplus
 }
 #endif

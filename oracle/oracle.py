@@ -94,8 +94,30 @@ def lib():
                                           + [_dp] * 8 + [_u8, C.c_double, C.c_double, C.c_double,
                                                          C.c_double] + [_dp] * 7)
         L.oracle_infiltration.restype = C.c_int32
+        L.oracle_groundwater_module.argtypes = [C.POINTER(GwmParams), C.POINTER(GwmState)] + \
+            [_dp] * 6 + [_dp, C.POINTER(C.c_int32), _dp]
+        L.oracle_groundwater_module.restype = C.c_int32
         _lib = L
     return _lib
+
+
+class GwmParams(C.Structure):
+    _fields_ = [("ny", C.c_int32), ("nx", C.c_int32), ("time_step_min", C.c_double),
+                ("resolution", C.c_double), ("t_min", C.c_double), ("routing_time_min", C.c_double),
+                ("flag_baseflow", C.c_int32), ("flag_async", C.c_int32),
+                ("target_dt_min", C.c_double), ("min_dt_min", C.c_double),
+                ("max_head_change_m", C.c_double), ("courant", C.c_double), ("l_vg", C.c_double)] + \
+               [(k, C.POINTER(C.c_double)) for k in ("elevation", "soil_depth", "theta_sat", "theta_r",
+                                                     "n_vg", "alpha_vg", "ksat_mmh", "ksat_gw_mmh",
+                                                     "Sy")] + [("domain", C.POINTER(C.c_uint8))]
+
+
+class GwmState(C.Structure):
+    _fields_ = [("pending_net", C.POINTER(C.c_double)), ("pending_recharge", C.POINTER(C.c_double)),
+                ("pending_capillary", C.POINTER(C.c_double)), ("elapsed_s", C.c_double),
+                ("total_recharge_m3", C.c_double), ("total_capillary_m3", C.c_double),
+                ("total_exfiltration_m3", C.c_double), ("last_dt_s", C.c_double),
+                ("n_updates", C.c_int32)]
 
 
 def _f(a):
@@ -356,3 +378,47 @@ def infiltration(time_step_min, resolution, d_t, I_t, elevation, h_t, soil_depth
                               _p(cum), C.byref(vol), C.byref(dS))
     return {"d_t": d, "I_t": I, "I_p": Ip, "f": f, "cum_inf": cum, "inf_volume": vol.value,
             "dS_inf": dS.value}
+
+
+class GroundwaterModule:
+    """Groundwater_Module (no layered soil) on the oracle, keeping GW_States
+    between calls like the reference's workspace does."""
+
+    def __init__(self, elevation, soil_depth, theta_sat, theta_r, n_vg, alpha_vg, ksat_mmh,
+                 ksat_gw_mmh, Sy, resolution, domain=None, flag_baseflow=1, flag_async=1,
+                 target_dt_min=1440.0, min_dt_min=1.0, max_head_change_m=0.25, courant=0.25,
+                 l_vg=0.5):
+        z = _f(elevation)
+        self.shape = z.shape
+        g = lambda a: np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), z.shape))
+        self._arr = {k: g(v) for k, v in dict(elevation=z, soil_depth=soil_depth, theta_sat=theta_sat,
+                                              theta_r=theta_r, n_vg=n_vg, alpha_vg=alpha_vg,
+                                              ksat_mmh=ksat_mmh, ksat_gw_mmh=ksat_gw_mmh,
+                                              Sy=Sy).items()}
+        self._dom, dp = _mask(domain, z.shape)
+        self.p = GwmParams(z.shape[0], z.shape[1], 0.0, resolution, 0.0, 0.0, flag_baseflow,
+                           flag_async, target_dt_min, min_dt_min, max_head_change_m, courant, l_vg,
+                           *[_p(self._arr[k]) for k in ("elevation", "soil_depth", "theta_sat",
+                                                         "theta_r", "n_vg", "alpha_vg", "ksat_mmh",
+                                                         "ksat_gw_mmh", "Sy")], dp)
+        nanm = np.isnan(z)
+        self.pending = {k: np.where(nanm, np.nan, 0.0).copy(order="F")
+                        for k in ("net", "recharge", "capillary")}
+        self.s = GwmState(_p(self.pending["net"]), _p(self.pending["recharge"]),
+                          _p(self.pending["capillary"]), 0.0, 0.0, 0.0, 0.0, 0.0, 0)
+
+    def step(self, time_step_min, t_min, routing_time_min, h_t, I_t, d_t, cumulative_recharge,
+             max_gw_depth):
+        """One call; h_t, I_t, d_t, cumulative_recharge, max_gw_depth (Fortran float64
+        arrays) are updated in place.  Returns (q_exf, error, ran, dt_gw)."""
+        self.p.time_step_min = time_step_min
+        self.p.t_min = t_min
+        self.p.routing_time_min = routing_time_min
+        q = np.zeros(self.shape, order="F")
+        err, ran, dtg = C.c_double(), C.c_int32(), C.c_double()
+        rc = lib().oracle_groundwater_module(C.byref(self.p), C.byref(self.s), _p(h_t), _p(I_t),
+                                             _p(d_t), _p(cumulative_recharge), _p(max_gw_depth),
+                                             _p(q), C.byref(err), C.byref(ran), C.byref(dtg))
+        if rc != 0:
+            raise ValueError(f"groundwater module: oracle returned {rc}")
+        return q, err.value, bool(ran.value), dtg.value
