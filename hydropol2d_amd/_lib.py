@@ -90,6 +90,7 @@ SIGNATURES = {
     "hp2d_sim_run_fixed": (C.c_int32, [_vp, C.c_int64, C.c_int32]),
     "hp2d_sim_report": (C.c_int32, [_vp, C.POINTER(abi.RunReport)]),
     "hp2d_sim_get_state": (C.c_int32, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "hp2d_sim_get_wq_maps": (C.c_int32, [_vp, _dp, _dp]),
     "hp2d_sim_get_hydrograph": (C.c_int32, [_vp, _dp, _dp, _dp, C.c_int32, _i32p]),
     "hp2d_sim_get_dt_history": (C.c_int32, [_vp, _dp, C.c_int32, _i32p]),
     "hp2d_debug_inject_failure": (C.c_int32, [_vp, C.c_int64, C.c_int32]),

@@ -91,9 +91,14 @@ class RunInputs(C.Structure):
         ("wq_Bmax", C.c_double),
         ("wq_min_Bt", C.c_double),
         ("gw_h0", _dp),
-        ("gw_z0", _dp),
-        ("gw_K", C.c_double),
+        ("gw_soil_depth", _dp),
+        ("gw_ksat_mmh", C.c_double),
         ("gw_Sy", C.c_double),
+        ("gw_theta_sat", _dp),
+        ("gw_theta_r", _dp),
+        ("gw_n_vg", _dp),
+        ("gw_alpha_vg", _dp),
+        ("gw_l_vg", C.c_double),
     ]
 
 
@@ -116,6 +121,18 @@ class RunReport(C.Structure):
         ("cum_outflow_m3", C.c_double),
         ("storage_m3", C.c_double),
         ("catch_index", C.c_int32),
+        ("wq_mass_kg", C.c_double),
+        ("wq_mass_lost_kg", C.c_double),
+        ("wq_out_conc_mgl", C.c_double),
+        ("wq_tmin_s", C.c_double),
+        ("wq_mass_outlet_mg", C.c_double),
+        ("wq_vol_outlet", C.c_double),
+        ("wq_substep_steps", C.c_int64),
+        ("gw_storage_m3", C.c_double),
+        ("gw_total_recharge_m3", C.c_double),
+        ("gw_total_exfiltration_m3", C.c_double),
+        ("gw_error_m3", C.c_double),
+        ("gw_updates", C.c_int32),
     ]
 
     def as_dict(self):

@@ -13,10 +13,12 @@
 
 #include "hp2d.h"
 #include "hp2d_forcing.hpp"
+#include "hp2d_gw.hpp"
 #include "hp2d_handle.hpp"
 #include "hp2d_internal.hpp"
 #include "hp2d_li.hpp"
 #include "hp2d_sim.hpp"
+#include "hp2d_simwq.hpp"
 
 using namespace hp2d;
 
@@ -418,6 +420,27 @@ struct hp2d_sim {
     // timing of the last run_fixed
     double li_ms = 0, total_ms = 0;
     long long li_launches = 0;
+    // water quality (build_up_wash_off in the loop)
+    bool wq = false;
+    void *wqB[2] = {nullptr, nullptr}, *wqBs = nullptr, *wqBx[2] = {nullptr, nullptr};
+    void *wqOm = nullptr, *wqTW[2] = {nullptr, nullptr}, *wqPc = nullptr;
+    void *wqC3 = nullptr, *wqC4 = nullptr;
+    uint16_t *wq_cell = nullptr;
+    double *wq_part = nullptr;
+    WqCtl *wq_ctl = nullptr;
+    int wq_blocks = 0, wq_scap = 2;
+    double wq_min_Bt = 0, wq_Bmin = 0, wq_Bmax = 0;
+    // groundwater (Groundwater_Module every gw_every_steps accepted steps)
+    bool gw = false;
+    hp2d_handle *gwh = nullptr;        // stream + workspace of the module's kernels
+    double *gw_mem = nullptr;          // the module's device arrays (GwmArgs)
+    GwmArgs gwa{};
+    double *gw_col = nullptr;
+    uint8_t *gw_dom = nullptr;
+    hp2d_gw_params gwp{};
+    hp2d_gw_state gws{};
+    double gw_error = 0.0;
+    long long gw_last = 0;             // accepted steps at the last module call
 };
 
 static void set_err(hp2d_sim *s, const std::string &m) {
@@ -482,6 +505,43 @@ template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
     a.out_d = (T *)s->out_d;
     a.partials = s->partials;
     a.part_cap = s->part_cap;
+    a.track_dmax = s->wq ? 1 : 0;
+    return a;
+}
+
+template <typename T> static WqSimArgs<T> sim_wq_args(hp2d_sim *s) {
+    WqSimArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    const hp2d_run_config &c = s->cfg;
+    a.ny = s->ny;
+    a.nx = s->nx;
+    a.n = (long long)s->n;
+    a.A = c.resolution * c.resolution;
+    a.min_Bt = s->wq_min_Bt;
+    a.Bmin = s->wq_Bmin;
+    a.Bmax = s->wq_Bmax;
+    a.model = c.flag_wq_model;
+    a.n_blocks = s->wq_blocks;
+    a.s_cap = s->wq_scap;
+    a.cell = s->wq_cell;
+    a.out_idx = s->out_idx;
+    a.out_q = (const T *)s->out_q;
+    a.n_outlet = s->n_outlet;
+    for (int p = 0; p < 2; ++p) {
+        a.qx_buf[p] = (T *)s->qx[p];
+        a.qy_buf[p] = (T *)s->qy[p];
+        a.B[p] = (T *)s->wqB[p];
+        a.Bx[p] = (T *)s->wqBx[p];
+        a.TW[p] = (T *)s->wqTW[p];
+    }
+    a.C3 = (const T *)s->wqC3;
+    a.C4 = (const T *)s->wqC4;
+    a.Bs = (T *)s->wqBs;
+    a.omass = (T *)s->wqOm;
+    a.Pc = (T *)s->wqPc;
+    a.part = s->wq_part;
+    a.ctl = s->wq_ctl;
+    a.st = s->st;
     return a;
 }
 
@@ -511,7 +571,8 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
     f.flag_timestep = c.flag_timestep;
     f.adaptive = c.flag_adaptive_timestepping;
     f.ga = c.flag_infiltration == 1;
-    f.flag_wq = c.flag_waterquality == 1;
+    f.flag_wq = s->wq ? 1 : 0;
+    f.flag_gw = s->gw ? 1 : 0;
     f.has_pattern = s->pattern != nullptr;
     f.vmax_squared = s->elem == 4;
     f.n_rec = s->n_hydro_max;
@@ -532,11 +593,21 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
     return f;
 }
 
+// the launches that follow the fused step: the wash-off (when enabled) and
+// the dt / bookkeeping tail
+template <typename T> static hipError_t sim_after_li(hp2d_sim *s) {
+    if (s->wq) {
+        hipError_t e = launch_wq_step<T>(sim_wq_args<T>(s), s->stream);
+        if (e != hipSuccess) return e;
+    }
+    return launch_finalize<T>(sim_fin_args<T>(s), s->stream);
+}
+
 template <typename T> static hipError_t sim_step_launch(hp2d_sim *s, bool with_finalize) {
     LiArgs<T> a = sim_li_args<T>(s);
     hipError_t e = launch_li_step<T>(a, kModeSim, s->stream);
     if (e != hipSuccess || !with_finalize) return e;
-    return launch_finalize<T>(sim_fin_args<T>(s), s->stream);
+    return sim_after_li<T>(s);
 }
 
 static hipError_t step_launch(hp2d_sim *s, bool with_finalize) {
@@ -566,8 +637,8 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         g_last_error = "hp2d_sim_create: only local-inertial routing with the Bates scheme";
         return HP2D_E_UNSUPPORTED;
     }
-    if (cfg->flag_waterquality || cfg->flag_groundwater) {
-        g_last_error = "hp2d_sim_create: water quality / groundwater not enabled in the loop";
+    if (cfg->flag_groundwater == 1 && cfg->flag_infiltration != 1) {
+        g_last_error = "hp2d_sim_create: the groundwater module needs the infiltration state I_t";
         return HP2D_E_UNSUPPORTED;
     }
     if (cfg->time_step_change_s <= 0 || cfg->time_step_increments_s <= 0 ||
@@ -663,6 +734,70 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
             if (std::isnan(z[c])) I0[c] = NAN;
         }
     }
+    // water quality: class tables, exponential build-up (HydroPol2D_preprocessing.m:2170-2174)
+    // and initial_mass (:2319)
+    s->wq = cfg->flag_waterquality == 1;
+    std::vector<double> B0, wc3, wc4;
+    std::vector<uint16_t> wcell;
+    double initial_mass = 0.0;
+    if (s->wq) {
+        const int k = cfg->n_lulc_classes;
+        if (k < 1 || k > 256 || !in->wq_C1 || !in->wq_C2 || !in->wq_C3 || !in->wq_C4)
+            return fail(HP2D_E_ARG, "hp2d_sim_create: water-quality class tables missing");
+        if (cfg->flag_wq_model != 0 && cfg->flag_wq_model != 1)
+            return fail(HP2D_E_ARG, "hp2d_sim_create: flag_wq_model must be 0 or 1");
+        B0.resize(n);
+        wcell.resize(n);
+        wc3.assign(in->wq_C3, in->wq_C3 + k);
+        wc4.assign(in->wq_C4, in->wq_C4 + k);
+        std::vector<uint8_t> is_out(n, 0);
+        for (long long c : oidx) is_out[(size_t)c] = 1;
+        for (size_t c = 0; c < n; ++c) {
+            const int cls = in->lulc_class ? in->lulc_class[c] : 0;
+            if (cls >= k) return fail(HP2D_E_ARG, "hp2d_sim_create: LULC class out of range");
+            const bool nan_cell = std::isnan(z[c]);
+            const double b0 = in->wq_C1[cls] * (1.0 - std::pow(2.718281828459045, -in->wq_C2[cls] * in->wq_ADD_days));
+            B0[c] = b0 * A / 1e4 * (nan_cell ? 0.0 : 1.0);
+            wcell[c] = (uint16_t)(cls | (nan_cell ? kWqNanBit : 0u) | (is_out[c] ? kWqOutletBit : 0u));
+        }
+        for (size_t c = 0; c < n; ++c)  // sum(sum(B_t(~isinf(B_t)))), column-major
+            if (!std::isinf(B0[c])) initial_mass += B0[c];
+    }
+    // groundwater module: per-cell van Genuchten maps from the soil classes
+    s->gw = cfg->flag_groundwater == 1;
+    double S_GW = 0.0;
+    std::vector<double> gw_host;  // 9 input maps, then h0
+    if (s->gw) {
+        const int k = cfg->n_soil_classes;
+        if (!in->gw_h0 || !in->gw_soil_depth || !in->gw_theta_sat || !in->gw_theta_r ||
+            !in->gw_n_vg || !in->gw_alpha_vg || cfg->gw_every_steps < 1 || ny < 2 || s->nx < 2)
+            return fail(HP2D_E_ARG, "hp2d_sim_create: groundwater inputs missing");
+        gw_host.resize(10 * n);
+        for (size_t c = 0; c < n; ++c) {
+            const int cls = in->soil_class ? in->soil_class[c] : 0;
+            (void)k;
+            gw_host[0 * n + c] = z[c];
+            gw_host[1 * n + c] = in->gw_soil_depth[c];
+            gw_host[2 * n + c] = in->gw_theta_sat[cls];
+            gw_host[3 * n + c] = in->gw_theta_r[cls];
+            gw_host[4 * n + c] = in->gw_n_vg[cls];
+            gw_host[5 * n + c] = in->gw_alpha_vg[cls];
+            gw_host[6 * n + c] = in->ksat_mmh[cls];
+            gw_host[7 * n + c] = in->gw_ksat_mmh;
+            gw_host[8 * n + c] = in->gw_Sy;
+            gw_host[9 * n + c] = in->gw_h0[c];
+        }
+        // Main_While.m:258 S_GW = nansum(nansum(A .* Sy .* (h_t - (z - D))))
+        for (int j = 0; j < s->nx; ++j) {
+            double col = 0.0;
+            for (int i = 0; i < ny; ++i) {
+                const size_t c = (size_t)i + (size_t)j * ny;
+                const double v = A * in->gw_Sy * (in->gw_h0[c] - (z[c] - in->gw_soil_depth[c]));
+                if (!std::isnan(v)) col += v;
+            }
+            S_GW += col;
+        }
+    }
     // initial storage (Main_While.m:248-260), column-major nansum order
     double S_p = 0, S_UZ = 0;
     for (int j = 0; j < s->nx; ++j) {
@@ -723,9 +858,34 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     A_((void **)&s->dt_hist, s->n_dt_max * sizeof(double));
     A_((void **)&s->vol_err, s->n_rep_max * sizeof(double));
     A_((void **)&s->tune_sink, sizeof(double));
+    if (s->wq) {
+        for (int p = 0; p < 2; ++p) {
+            A_(&s->wqB[p], nb);
+            A_(&s->wqTW[p], nb);
+            if (cfg->flag_wq_model == 1) A_(&s->wqBx[p], nb);
+        }
+        A_(&s->wqBs, nb);
+        A_(&s->wqOm, nb);
+        A_(&s->wqPc, nb);
+        A_(&s->wqC3, wc3.size() * s->elem);
+        A_(&s->wqC4, wc4.size() * s->elem);
+        A_((void **)&s->wq_cell, n * sizeof(uint16_t));
+        s->wq_blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+        A_((void **)&s->wq_part, (size_t)s->wq_blocks * kWqFieldCount * sizeof(double));
+        A_((void **)&s->wq_ctl, sizeof(WqCtl));
+    }
+    if (s->gw) {
+        A_((void **)&s->gw_mem, 24 * n * sizeof(double));
+        A_((void **)&s->gw_col, 3 * (size_t)s->nx * sizeof(double));
+        A_((void **)&s->gw_dom, n);
+    }
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? HP2D_E_NOMEM : HP2D_E_HIP,
                     std::string("hp2d_sim_create: ") + hipGetErrorString(e));
+    if (s->gw) {
+        const hp2d_status hs = hp2d_create(ny, s->nx, HP2D_FP64, &s->gwh);
+        if (hs != HP2D_OK) return fail(hs, "hp2d_sim_create: groundwater handle");
+    }
     // ---- uploads ----
     auto up = [&](void *dst, const void *src, size_t bytes) {
         if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
@@ -777,6 +937,55 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         up(s->rain_t, in->rain_time_min, in->n_rain * sizeof(double));
         up(s->rain_v, in->rain_intensity_mmh, in->n_rain * sizeof(double));
     }
+    if (s->wq) {
+        for (int p = 0; p < 2; ++p) {
+            up_real(s->wqB[p], B0, 0.0);
+            up_real(s->wqTW[p], zeros, 0.0);
+        }
+        up_real(s->wqPc, zeros, 0.0);
+        up_real(s->wqC3, wc3, 0.0);
+        up_real(s->wqC4, wc4, 0.0);
+        up(s->wq_cell, wcell.data(), n * sizeof(uint16_t));
+        s->wq_min_Bt = in->wq_min_Bt;
+        s->wq_Bmin = in->wq_Bmin;
+        s->wq_Bmax = in->wq_Bmax;
+    }
+    if (s->gw) {
+        double *g = s->gw_mem;
+        auto arr = [&](int k) { return g + (size_t)k * n; };
+        up(g, gw_host.data(), 9 * n * sizeof(double));          // elev .. Sy
+        up(arr(9), gw_host.data() + 9 * n, n * sizeof(double));  // h
+        std::vector<double> pend(n);
+        for (size_t c = 0; c < n; ++c) pend[c] = std::isnan(z[c]) ? NAN : 0.0;
+        up(arr(14), pend.data(), n * sizeof(double));
+        up(arr(15), pend.data(), n * sizeof(double));
+        up(arr(16), pend.data(), n * sizeof(double));
+        up(arr(12), zeros.data(), n * sizeof(double));  // cumulative recharge
+        std::vector<uint8_t> dom(n);
+        for (size_t c = 0; c < n; ++c) dom[c] = std::isnan(z[c]) ? 0 : 1;
+        up(s->gw_dom, dom.data(), n);
+        GwmArgs &a = s->gwa;
+        a.ny = ny;
+        a.nx = s->nx;
+        a.n = (long long)n;
+        a.elev = arr(0); a.soil_depth = arr(1); a.ts = arr(2); a.tr = arr(3); a.nv = arr(4);
+        a.alpha = arr(5); a.ksat = arr(6); a.ksat_gw = arr(7); a.Sy = arr(8);
+        a.h = arr(9); a.I = arr(10); a.d = arr(11); a.cum = arr(12); a.maxd = nullptr;
+        a.pn = arr(14); a.pr = arr(15); a.pc = arr(16); a.z_bed = arr(17); a.Kgw = arr(18);
+        a.rfg = arr(19); a.q = arr(20); a.t1 = arr(21); a.t2 = arr(22); a.t3 = arr(23);
+        hp2d_gw_params &p = s->gwp;
+        p.ny = ny;
+        p.nx = s->nx;
+        p.resolution = R;
+        p.routing_time_min = cfg->routing_time_min;
+        p.flag_baseflow = 1;
+        p.flag_async = 0;
+        p.target_dt_min = 1440.0;
+        p.min_dt_min = 1.0;
+        p.max_head_change_m = 0.25;
+        p.courant = cfg->gw_courant;
+        p.l_vg = in->gw_l_vg;
+    }
     std::vector<double> hz(s->n_hydro_max, 0.0);
     up(s->hq, hz.data(), hz.size() * 8);
     up(s->ht, hz.data(), hz.size() * 8);
@@ -797,7 +1006,9 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     st.max_dt = std::min(cfg->record_time_hydrographs_min * 60.0, cfg->max_time_step_s);
     st.min_dt = std::min(cfg->record_time_hydrographs_min * 60.0, min_dt0);
     st.time_save_previous = 0.0;
-    st.S_prev = 0.0 + S_p + S_UZ + 0.0;
+    st.S_prev = 0.0 + S_p + S_UZ + S_GW;
+    st.gw_storage = S_GW;
+    st.wq_initial_mass = initial_mass;
     st.catch_index = 1;
     st.next_report = cfg->report_every_percent;
     st.last_record = 1;
@@ -822,9 +1033,12 @@ void hp2d_sim_destroy(hp2d_sim *s) {
                   s->qy[0], s->qy[1], s->I[0], s->I[1], s->Tc[0], s->Tc[1], s->soil, s->ga, s->ga_f32,
                   s->out_idx, s->out_n, s->out_q, s->out_d, s->partials, s->st, s->rain_t,
                   s->rain_v, s->hq, s->ht, s->hd, s->dt_hist, s->vol_err, s->tune_sink,
-                  s->tpeak};
+                  s->tpeak, s->wqB[0], s->wqB[1], s->wqBs, s->wqBx[0], s->wqBx[1], s->wqOm,
+                  s->wqTW[0], s->wqTW[1], s->wqPc, s->wqC3, s->wqC4, s->wq_cell, s->wq_part,
+                  s->wq_ctl, s->gw_mem, s->gw_col, s->gw_dom};
     for (void *p : ps)
         if (p) hipFree(p);
+    if (s->gwh) hp2d_destroy(s->gwh);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
 }
@@ -877,6 +1091,124 @@ static hp2d_status ensure_graph(hp2d_sim *s, int steps) {
     return HP2D_OK;
 }
 
+__global__ void cast_to_f64_kernel(const float *src, double *dst, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        dst[i] = (double)src[i];
+}
+__global__ void cast_to_f32_kernel(const double *src, float *dst, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        dst[i] = (float)src[i];
+}
+
+// A step whose wash-off split exceeded the captured sub-steps left the hold
+// flag set (every later launch was a no-op): run its remaining sub-steps, the
+// fold (which clears the flag) and its finalize.
+static hp2d_status sim_release_hold(hp2d_sim *s, SimState &st) {
+    if (st.wq_err) {
+        set_err(s, "build_up_wash_off: tmin_wq asks for more than 1e7 internal sub-steps "
+                   "(the reference would not terminate)");
+        return HP2D_E_ARG;
+    }
+    WqCtl ctl;
+    STRY(s, hipMemcpy(&ctl, s->wq_ctl, sizeof(WqCtl), hipMemcpyDeviceToHost));
+    hipError_t e;
+    if (s->elem == 8) {
+        e = launch_wq_finish<double>(sim_wq_args<double>(s), ctl.steps, s->stream);
+        if (e == hipSuccess) e = launch_finalize<double>(sim_fin_args<double>(s), s->stream);
+    } else {
+        e = launch_wq_finish<float>(sim_wq_args<float>(s), ctl.steps, s->stream);
+        if (e == hipSuccess) e = launch_finalize<float>(sim_fin_args<float>(s), s->stream);
+    }
+    STRY(s, e);
+    return read_state(s, st);
+}
+
+// Groundwater_Module over the model time since its last call, on the current
+// depth and infiltration buffers (FP32 state is widened to FP64 around it).
+static hp2d_status sim_gw_update(hp2d_sim *s, SimState &st) {
+    const int cur = (int)((st.k + 1) & 1);  // the buffers the next step reads
+    hipStream_t gs = hp2d_handle_stream(s->gwh);
+    GwmArgs a = s->gwa;
+    const long long n = (long long)s->n;
+    const dim3 G((unsigned)std::min<long long>((n + 255) / 256, 8192)), B(256);
+    if (s->elem == 8) {
+        a.d = (double *)s->d[cur];
+        a.I = (double *)s->I[cur];
+    } else {
+        hipLaunchKernelGGL(cast_to_f64_kernel, G, B, 0, gs, (const float *)s->d[cur], a.d, n);
+        hipLaunchKernelGGL(cast_to_f64_kernel, G, B, 0, gs, (const float *)s->I[cur], a.I, n);
+        STRY(s, hipGetLastError());
+    }
+    s->gwp.time_step_min = st.gw_elapsed_min;
+    s->gwp.t_min = st.t;
+    double err = 0.0, sgw = 0.0;
+    int32_t ran = 0;
+    const hp2d_status rc = gwm_run(s->gwh, &s->gwp, &s->gws, a, s->gw_col, s->gw_dom, &err, &ran, &sgw);
+    if (rc != HP2D_OK) {
+        set_err(s, std::string("groundwater module: ") + hp2d_last_error(s->gwh));
+        return rc;
+    }
+    if (s->elem == 4) {
+        hipLaunchKernelGGL(cast_to_f32_kernel, G, B, 0, gs, (const double *)a.d, (float *)s->d[cur], n);
+        hipLaunchKernelGGL(cast_to_f32_kernel, G, B, 0, gs, (const double *)a.I, (float *)s->I[cur], n);
+        STRY(s, hipGetLastError());
+        STRY(s, hipStreamSynchronize(gs));
+    }
+    s->gw_error = err;
+    s->gw_last = st.accepted;
+    st.gw_elapsed_min = 0.0;
+    st.gw_storage = sgw;
+    return write_state(s, st);
+}
+
+// Advance by n_accept accepted steps (0: to the end of the run) in captured
+// graphs of `batch` steps, pausing for the groundwater calls and releasing
+// wash-off holds between graphs.
+static hp2d_status sim_advance(hp2d_sim *s, int64_t n_accept, int batch, int32_t *done) {
+    SimState st;
+    hp2d_status rc = read_state(s, st);
+    if (rc) return rc;
+    const int64_t target = n_accept > 0 ? st.accepted + n_accept : -1;
+    const int every = s->gw ? s->cfg.gw_every_steps : 0;
+    if (every > 0) batch = std::min(batch, every);
+    const bool use_graph = batch >= 8 || (batch >= 1 && (s->wq || s->gw));
+    if (use_graph && (rc = ensure_graph(s, batch))) return rc;
+    // every launch of a graph accepts, rejects or skips (paused / held) each of
+    // its steps, and a release always completes the held step; a run longer
+    // than this bound is a failure of the time control, not an endless loop
+    const int64_t cap = (n_accept > 0 ? n_accept : (int64_t)1 << 40) * 64 + (1 << 20);
+    int64_t launches = 0;
+    while (!st.done && (target < 0 || st.accepted < target)) {
+        if (++launches > cap) {
+            set_err(s, "sim: the step loop made no progress");
+            return HP2D_E_HIP;
+        }
+        if (every > 0 && st.accepted - s->gw_last >= every) {
+            if ((rc = sim_gw_update(s, st))) return rc;
+        }
+        int64_t stop = target;
+        if (every > 0) {
+            const int64_t g = s->gw_last + every;
+            stop = stop < 0 ? g : std::min(stop, g);
+        }
+        st.stop_accepted = stop;
+        st.paused = 0;
+        if ((rc = write_state(s, st))) return rc;
+        if (use_graph) {
+            STRY(s, hipGraphLaunch(s->graph, s->stream));
+        } else {
+            for (int i = 0; i < batch; ++i) STRY(s, step_launch(s, true));
+        }
+        if ((rc = read_state(s, st))) return rc;
+        if (st.hold && (rc = sim_release_hold(s, st))) return rc;
+    }
+    st.stop_accepted = -1;
+    st.paused = 0;
+    if ((rc = write_state(s, st))) return rc;
+    if (done) *done = st.done;
+    return HP2D_OK;
+}
+
 extern "C" {
 
 hp2d_status hp2d_sim_run(hp2d_sim *s, int64_t max_steps, int32_t *done) {
@@ -886,6 +1218,14 @@ hp2d_status hp2d_sim_run(hp2d_sim *s, int64_t max_steps, int32_t *done) {
     if (rc) return rc;
     if (st.done) {
         if (done) *done = 1;
+        return HP2D_OK;
+    }
+    if (s->wq || s->gw) {
+        const int batch = (int)std::min<int64_t>(64, max_steps > 0 ? max_steps : 64);
+        if ((rc = sim_advance(s, max_steps, batch, done))) return rc;
+        if ((rc = read_state(s, st))) return rc;
+        if (st.neg_events > 0 && s->err.empty())
+            s->err = "negative depth below -1e-8 mm met before clean-up (Hydrological_Model.m:45)";
         return HP2D_OK;
     }
     st.stop_accepted = max_steps > 0 ? st.accepted + max_steps : -1;
@@ -931,7 +1271,10 @@ hp2d_status hp2d_sim_run_fixed(hp2d_sim *s, int64_t n_steps, int32_t graph_steps
         if (rc) return rc;
     }
     STRY(s, hipEventRecord(e0, s->stream));
-    if (graph_steps > 0) {
+    if (graph_steps > 0 && (s->wq || s->gw)) {
+        hp2d_status rc = sim_advance(s, n_steps, graph_steps, nullptr);
+        if (rc) return rc;
+    } else if (graph_steps > 0) {
         int64_t done_steps = 0;
         while (done_steps + graph_steps <= n_steps) {
             STRY(s, hipGraphLaunch(s->graph, s->stream));
@@ -940,6 +1283,12 @@ hp2d_status hp2d_sim_run_fixed(hp2d_sim *s, int64_t n_steps, int32_t graph_steps
         for (; done_steps < n_steps; ++done_steps) STRY(s, step_launch(s, true));
     } else {
         for (int64_t i = 0; i < n_steps; ++i) {
+            if (s->gw) {  // a groundwater call opens the step that follows every N accepted ones
+                SimState st;
+                hp2d_status rc = read_state(s, st);
+                if (!rc && st.accepted - s->gw_last >= s->cfg.gw_every_steps) rc = sim_gw_update(s, st);
+                if (rc) return rc;
+            }
             STRY(s, hipEventRecord(a0, s->stream));
             LiArgs<double> ad;
             hipError_t e;
@@ -948,10 +1297,16 @@ hp2d_status hp2d_sim_run_fixed(hp2d_sim *s, int64_t n_steps, int32_t graph_steps
             (void)ad;
             STRY(s, e);
             STRY(s, hipEventRecord(a1, s->stream));
-            if (s->elem == 8) e = launch_finalize<double>(sim_fin_args<double>(s), s->stream);
-            else e = launch_finalize<float>(sim_fin_args<float>(s), s->stream);
+            if (s->elem == 8) e = sim_after_li<double>(s);
+            else e = sim_after_li<float>(s);
             STRY(s, e);
             STRY(s, hipEventSynchronize(a1));
+            if (s->wq) {  // a held wash-off split completes before the next launch
+                SimState st;
+                hp2d_status rc = read_state(s, st);
+                if (!rc && st.hold) rc = sim_release_hold(s, st);
+                if (rc) return rc;
+            }
             float ms = 0;
             STRY(s, hipEventElapsedTime(&ms, a0, a1));
             s->li_ms += ms;
@@ -1017,10 +1372,41 @@ hp2d_status hp2d_sim_report(hp2d_sim *s, hp2d_run_report *rep) {
         for (int b = 0; b < blocks; ++b) sum += hp[2 * b];
         const double A = s->cfg.resolution * s->cfg.resolution;
         rep->storage_m3 = sum * (A / 1000.0);
+        if (s->wq) {  // sum of the current B_t
+            const void *Bc = s->wqB[st.wq_n & 1];
+            STRY(s, hipMalloc((void **)&part, 2 * blocks * sizeof(double)));
+            e = s->elem == 8 ? launch_storage<double>((const double *)Bc, nullptr, (long long)s->n,
+                                                      part, blocks, s->stream)
+                             : launch_storage<float>((const float *)Bc, nullptr, (long long)s->n,
+                                                     part, blocks, s->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            hipFree(part);
+            STRY(s, e);
+            double sb = 0.0;
+            for (int b = 0; b < blocks; ++b) sb += hp[2 * b];
+            rep->wq_mass_kg = sb;
+        }
     }
     rep->catch_index = st.catch_index;
+    if (s->wq) {
+        rep->wq_mass_lost_kg = st.wq_mass_lost;
+        rep->wq_out_conc_mgl = st.wq_out_conc;
+        rep->wq_tmin_s = st.tmin_wq;
+        rep->wq_mass_outlet_mg = st.wq_mass_outlet;
+        rep->wq_vol_outlet = st.wq_vol_outlet;
+        rep->wq_substep_steps = st.wq_split_steps;
+    }
+    if (s->gw) {
+        rep->gw_storage_m3 = st.gw_storage;
+        rep->gw_total_recharge_m3 = s->gws.total_recharge_m3;
+        rep->gw_total_exfiltration_m3 = s->gws.total_exfiltration_m3;
+        rep->gw_error_m3 = s->gw_error;
+        rep->gw_updates = s->gws.n_updates;
+    }
     return HP2D_OK;
 }
+
 
 static hp2d_status fetch_real(hp2d_sim *s, double *dst, const void *src, double shift) {
     if (!dst || !src) return HP2D_OK;
@@ -1042,10 +1428,12 @@ hp2d_status hp2d_sim_get_state(hp2d_sim *s, double *d_t, double *outflow, double
     SimState st;
     hp2d_status rc = read_state(s, st);
     if (rc) return rc;
-    if (Hf || B_t || gw_h) {
-        set_err(s, "Hf, B_t and gw_h are not kept by the device loop");
+    if (Hf || (B_t && !s->wq) || (gw_h && !s->gw)) {
+        set_err(s, "Hf is not kept by the device loop; B_t / gw_h need water quality / groundwater");
         return HP2D_E_UNSUPPORTED;
     }
+    if (B_t && (rc = fetch_real(s, B_t, s->wqB[st.wq_n & 1], 0.0))) return rc;
+    if (gw_h) STRY(s, hipMemcpy(gw_h, s->gwa.h, s->n * 8, hipMemcpyDeviceToHost));
     const int cur = (int)((st.k + 1) & 1);  // buffer holding the last accepted state
     if ((rc = fetch_real(s, d_t, s->d[cur], 0.0))) return rc;
     if (outflow) {
@@ -1075,6 +1463,20 @@ hp2d_status hp2d_sim_get_state(hp2d_sim *s, double *d_t, double *outflow, double
         if ((rc = fetch_real(s, I_t, s->I[cur], 0.0))) return rc;
     }
     if ((rc = fetch_real(s, dmax, s->dmax, 0.0))) return rc;
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_sim_get_wq_maps(hp2d_sim *s, double *P_conc, double *Tot_Washed) {
+    if (!s) return HP2D_E_ARG;
+    if (!s->wq) {
+        set_err(s, "hp2d_sim_get_wq_maps: water quality is off for this run");
+        return HP2D_E_UNSUPPORTED;
+    }
+    SimState st;
+    hp2d_status rc = read_state(s, st);
+    if (rc) return rc;
+    if (P_conc && (rc = fetch_real(s, P_conc, s->wqPc, 0.0))) return rc;
+    if (Tot_Washed && (rc = fetch_real(s, Tot_Washed, s->wqTW[st.wq_n & 1], 0.0))) return rc;
     return HP2D_OK;
 }
 
@@ -1462,6 +1864,13 @@ hp2d_status hp2d_ensemble_create(int32_t n_members, const hp2d_run_config *cfgs,
         cols_per_wave != 128) {
         g_last_error = "hp2d_ensemble_create: cols_per_wave must be 0, 16, 32, 64 or 128";
         return HP2D_E_ARG;
+    }
+    for (int i = 0; i < n_members; ++i) {
+        if (cfgs[i].flag_waterquality == 1 || cfgs[i].flag_groundwater == 1) {
+            g_last_error = "hp2d_ensemble_create: members run the surface step and Green-Ampt "
+                           "(no wash-off or groundwater module in the batched launch)";
+            return HP2D_E_UNSUPPORTED;
+        }
     }
     for (int i = 1; i < n_members; ++i) {
         if (cfgs[i].precision != cfgs[0].precision ||

@@ -45,4 +45,24 @@ GwScratch gw_scratch(char *base, int ny, int nx);
 hp2d_status gw_run(hp2d_handle *h, const GwArgs &a, double dt, const double *dh0,
                    const GwScratch &w, double *error, int32_t *n_substeps);
 
+// Device fields of the groundwater module step (Groundwater_Module.m:37-404):
+// soil / aquifer maps, the workspace state it updates in place (head h [m],
+// vadose storage I [mm], surface depth d [mm], cumulative recharge [mm],
+// maximum water-table depth, pending exchange maps [m]) and temporaries.
+struct GwmArgs {
+    int ny, nx;
+    long long n;
+    double dt_s, A, l_vg;
+    const double *elev, *soil_depth, *ts, *tr, *nv, *alpha, *ksat, *ksat_gw, *Sy;
+    double *h, *I, *d, *cum, *maxd, *pn, *pr, *pc;
+    double *z_bed, *Kgw, *rfg, *q, *t1, *t2, *t3;
+};
+
+// One module call on device arrays (the scalars of p, the state g updated);
+// col: 3*nx doubles of scratch; dom: catchment mask or NULL.  Returns after
+// the stream has drained; *s_gw (optional) receives the saturated storage
+// plus the pending exchange [m3] after the call.
+hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, GwmArgs a,
+                    double *col, const uint8_t *dom, double *error, int32_t *ran, double *s_gw);
+
 }  // namespace hp2d

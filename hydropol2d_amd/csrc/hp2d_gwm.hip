@@ -19,15 +19,6 @@
 
 namespace hp2d {
 
-struct GwmArgs {
-    int ny, nx;
-    long long n;
-    double dt_s, A, l_vg;
-    const double *elev, *soil_depth, *ts, *tr, *nv, *alpha, *ksat, *ksat_gw, *Sy;
-    double *h, *I, *d, *cum, *maxd, *pn, *pr, *pc;
-    double *z_bed, *Kgw, *rfg, *q, *t1, *t2, *t3;
-};
-
 // recharge bucket, pending exchange and the scheduler's per-cell terms
 __global__ __launch_bounds__(256) void gwm_recharge_kernel(GwmArgs a) {
     const long long c = blockIdx.x * 256LL + threadIdx.x;
@@ -189,68 +180,35 @@ __global__ __launch_bounds__(256) void gwm_maxdepth_kernel(GwmArgs a) {
     a.maxd[c] = fmax(a.maxd[c], a.h[c] - (a.elev[c] - a.soil_depth[c]));
 }
 
-}  // namespace hp2d
+// per column: sum of A*Sy*(h - z_bed) + A*pending_net (mass_balance_check.m:66-70)
+__global__ __launch_bounds__(256) void gwm_storage_kernel(GwmArgs a, double *out) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.nx) return;
+    double s = 0.0, sp = 0.0;
+    for (int i = 0; i < a.ny; ++i) {
+        const long long c = (long long)j * a.ny + i;
+        const double area = isnan(a.elev[c]) ? NAN : a.A;
+        const double v = area * a.Sy[c] * (a.h[c] - (a.elev[c] - a.soil_depth[c]));
+        if (!isnan(v)) s += v;
+        const double w = area * a.pn[c];
+        if (!isnan(w)) sp += w;
+    }
+    out[2 * j] = s;
+    out[2 * j + 1] = sp;
+}
 
-extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_params *p,
-                                               hp2d_gw_state *g, double *h_t, double *I_t,
-                                               double *d_t, double *cumulative_recharge,
-                                               double *max_gw_depth, double *q_exf, double *error,
-                                               int32_t *ran) {
-    using namespace hp2d;
-    if (!h || !p || !g || !h_t || !I_t || !d_t || !q_exf || !error || !ran || !p->elevation ||
-        !p->soil_depth || !p->theta_sat || !p->theta_r || !p->n_vg || !p->alpha_vg ||
-        !p->ksat_mmh || !p->ksat_gw_mmh || !p->Sy || !g->pending_net || !g->pending_recharge ||
-        !g->pending_capillary || !(p->resolution > 0.0))
-        return HP2D_E_ARG;
-    int32_t ny = 0, nx = 0, prec = 0;
-    hp2d_status st = hp2d_handle_shape(h, &ny, &nx, &prec);
-    if (st != HP2D_OK) return st;
-    if (ny != p->ny || nx != p->nx || (p->flag_baseflow == 1 && (ny < 2 || nx < 2))) return HP2D_E_ARG;
-    const size_t n = (size_t)ny * nx, nb = n * sizeof(double);
+hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, GwmArgs a,
+                    double *col, const uint8_t *dom, double *error, int32_t *ran, double *s_gw) {
+    const int ny = a.ny, nx = a.nx;
+    const size_t n = (size_t)a.n;
     hipStream_t s = hp2d_handle_stream(h);
-    // module arrays in the second workspace: 9 inputs, 8 state, 7 temporaries | colsums | mask
-    const size_t n_arr = 24;
-    const size_t col_off = n_arr * nb;
-    const size_t mask_off = col_off + 3 * (size_t)nx * sizeof(double);
-    char *ws = (char *)hp2d_handle_workspace2(h, mask_off + n + 256);
-    if (!ws) return HP2D_E_NOMEM;
-    double *q0 = (double *)ws;
-    auto arr = [&](int k) { return q0 + (size_t)k * n; };
-    GwmArgs a;
-    a.ny = ny;
-    a.nx = nx;
-    a.n = (long long)n;
     a.dt_s = p->time_step_min * 60.0;
     a.A = p->resolution * p->resolution;
     a.l_vg = p->l_vg;
-    double *in[9] = {arr(0), arr(1), arr(2), arr(3), arr(4), arr(5), arr(6), arr(7), arr(8)};
-    a.elev = in[0]; a.soil_depth = in[1]; a.ts = in[2]; a.tr = in[3]; a.nv = in[4]; a.alpha = in[5];
-    a.ksat = in[6]; a.ksat_gw = in[7]; a.Sy = in[8];
-    a.h = arr(9); a.I = arr(10); a.d = arr(11); a.cum = cumulative_recharge ? arr(12) : nullptr;
-    a.maxd = max_gw_depth ? arr(13) : nullptr; a.pn = arr(14); a.pr = arr(15); a.pc = arr(16);
-    a.z_bed = arr(17); a.Kgw = arr(18); a.rfg = arr(19); a.q = arr(20); a.t1 = arr(21);
-    a.t2 = arr(22); a.t3 = arr(23);
-    double *col = (double *)(ws + col_off);
-    uint8_t *dom = p->domain ? (uint8_t *)(ws + mask_off) : nullptr;
     hipError_t e = hipSuccess;
-    auto H2D = [&](void *dst, const void *src, size_t bytes) {
-        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
-    };
     auto D2H = [&](void *dst, const void *src, size_t bytes) {
         if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
     };
-    const double *hin[9] = {p->elevation, p->soil_depth, p->theta_sat, p->theta_r, p->n_vg,
-                            p->alpha_vg, p->ksat_mmh, p->ksat_gw_mmh, p->Sy};
-    for (int k = 0; k < 9; ++k) H2D(in[k], hin[k], nb);
-    H2D(a.h, h_t, nb);
-    H2D(a.I, I_t, nb);
-    H2D(a.d, d_t, nb);
-    if (a.cum) H2D(a.cum, cumulative_recharge, nb);
-    if (a.maxd) H2D(a.maxd, max_gw_depth, nb);
-    H2D(a.pn, g->pending_net, nb);
-    H2D(a.pr, g->pending_recharge, nb);
-    H2D(a.pc, g->pending_capillary, nb);
-    if (dom) H2D(dom, p->domain, n);
     const dim3 G((unsigned)((n + 255) / 256)), B(256), CB((unsigned)((nx + 255) / 256));
     std::vector<double> hc(3 * (size_t)nx);
     auto fold = [&](double &sum, double &m2, double &m3) {
@@ -263,16 +221,14 @@ extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_par
             m3 = std::fmax(m3, hc[3 * (size_t)j + 2]);
         }
     };
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(gwm_recharge_kernel, G, B, 0, s, a);
-        hipLaunchKernelGGL(gwm_colred_kernel, CB, B, 0, s, ny, nx, (const double *)a.t1,
-                           (const double *)a.t2, (const double *)a.t3, col);
-        e = hipGetLastError();
-    }
+    hipLaunchKernelGGL(gwm_recharge_kernel, G, B, 0, s, a);
+    hipLaunchKernelGGL(gwm_colred_kernel, CB, B, 0, s, ny, nx, (const double *)a.t1,
+                       (const double *)a.t2, (const double *)a.t3, col);
+    e = hipGetLastError();
     D2H(hc.data(), col, hc.size() * 8);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess)
-        return hp2d_handle_fail(h, std::string("hp2d_groundwater_module: ") + hipGetErrorString(e));
+        return hp2d_handle_fail(h, std::string("groundwater module: ") + hipGetErrorString(e));
     double rech_sum, hc_max, maxD;
     fold(rech_sum, hc_max, maxD);
     g->elapsed_s = g->elapsed_s + a.dt_s;
@@ -320,7 +276,7 @@ extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_par
             ga.dirichlet = nullptr;
             ga.ctl = w.ctl;
             if (e == hipSuccess) {
-                st = gw_run(h, ga, dt_gw, a.h, w, &err, nullptr);
+                hp2d_status st = gw_run(h, ga, dt_gw, a.h, w, &err, nullptr);
                 if (st != HP2D_OK) return st;
                 hipLaunchKernelGGL(gwm_after_bq_kernel, G, B, 0, s, a, (const double *)w.ht,
                                    (const double *)w.q, dt_gw);
@@ -339,7 +295,7 @@ extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_par
         D2H(hc.data(), col, hc.size() * 8);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess)
-            return hp2d_handle_fail(h, std::string("hp2d_groundwater_module: ") + hipGetErrorString(e));
+            return hp2d_handle_fail(h, std::string("groundwater module: ") + hipGetErrorString(e));
         double exf, ex_max, unused;
         fold(exf, ex_max, unused);
         g->total_exfiltration_m3 = g->total_exfiltration_m3 + exf;
@@ -359,6 +315,92 @@ extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_par
         hipLaunchKernelGGL(gwm_maxdepth_kernel, G, B, 0, s, a);
         e = hipGetLastError();
     }
+    if (s_gw && e == hipSuccess) {
+        hipLaunchKernelGGL(gwm_storage_kernel, CB, B, 0, s, a, col);
+        e = hipGetLastError();
+        D2H(hc.data(), col, 2 * (size_t)nx * 8);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return hp2d_handle_fail(h, std::string("groundwater module: ") + hipGetErrorString(e));
+    if (s_gw) {  // nansum(nansum(A.*Sy.*(h - z_bed))) + nansum(nansum(A.*pending))
+        double sg = 0.0, sp = 0.0;
+        for (int j = 0; j < nx; ++j) {
+            sg += hc[2 * (size_t)j];
+            sp += hc[2 * (size_t)j + 1];
+        }
+        *s_gw = sg + sp;
+    }
+    *error = err;
+    *ran = due ? 1 : 0;
+    return HP2D_OK;
+}
+
+}  // namespace hp2d
+
+extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_params *p,
+                                               hp2d_gw_state *g, double *h_t, double *I_t,
+                                               double *d_t, double *cumulative_recharge,
+                                               double *max_gw_depth, double *q_exf, double *error,
+                                               int32_t *ran) {
+    using namespace hp2d;
+    if (!h || !p || !g || !h_t || !I_t || !d_t || !q_exf || !error || !ran || !p->elevation ||
+        !p->soil_depth || !p->theta_sat || !p->theta_r || !p->n_vg || !p->alpha_vg ||
+        !p->ksat_mmh || !p->ksat_gw_mmh || !p->Sy || !g->pending_net || !g->pending_recharge ||
+        !g->pending_capillary || !(p->resolution > 0.0))
+        return HP2D_E_ARG;
+    int32_t ny = 0, nx = 0, prec = 0;
+    hp2d_status st = hp2d_handle_shape(h, &ny, &nx, &prec);
+    if (st != HP2D_OK) return st;
+    if (ny != p->ny || nx != p->nx || (p->flag_baseflow == 1 && (ny < 2 || nx < 2))) return HP2D_E_ARG;
+    const size_t n = (size_t)ny * nx, nb = n * sizeof(double);
+    hipStream_t s = hp2d_handle_stream(h);
+    // module arrays in the second workspace: 9 inputs, 8 state, 7 temporaries | colsums | mask
+    const size_t n_arr = 24;
+    const size_t col_off = n_arr * nb;
+    const size_t mask_off = col_off + 3 * (size_t)nx * sizeof(double);
+    char *ws = (char *)hp2d_handle_workspace2(h, mask_off + n + 256);
+    if (!ws) return HP2D_E_NOMEM;
+    double *q0 = (double *)ws;
+    auto arr = [&](int k) { return q0 + (size_t)k * n; };
+    GwmArgs a;
+    a.ny = ny;
+    a.nx = nx;
+    a.n = (long long)n;
+    double *in[9] = {arr(0), arr(1), arr(2), arr(3), arr(4), arr(5), arr(6), arr(7), arr(8)};
+    a.elev = in[0]; a.soil_depth = in[1]; a.ts = in[2]; a.tr = in[3]; a.nv = in[4]; a.alpha = in[5];
+    a.ksat = in[6]; a.ksat_gw = in[7]; a.Sy = in[8];
+    a.h = arr(9); a.I = arr(10); a.d = arr(11); a.cum = cumulative_recharge ? arr(12) : nullptr;
+    a.maxd = max_gw_depth ? arr(13) : nullptr; a.pn = arr(14); a.pr = arr(15); a.pc = arr(16);
+    a.z_bed = arr(17); a.Kgw = arr(18); a.rfg = arr(19); a.q = arr(20); a.t1 = arr(21);
+    a.t2 = arr(22); a.t3 = arr(23);
+    double *col = (double *)(ws + col_off);
+    uint8_t *dom = p->domain ? (uint8_t *)(ws + mask_off) : nullptr;
+    hipError_t e = hipSuccess;
+    auto H2D = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    };
+    auto D2H = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    };
+    const double *hin[9] = {p->elevation, p->soil_depth, p->theta_sat, p->theta_r, p->n_vg,
+                            p->alpha_vg, p->ksat_mmh, p->ksat_gw_mmh, p->Sy};
+    for (int k = 0; k < 9; ++k) H2D(in[k], hin[k], nb);
+    H2D(a.h, h_t, nb);
+    H2D(a.I, I_t, nb);
+    H2D(a.d, d_t, nb);
+    if (a.cum) H2D(a.cum, cumulative_recharge, nb);
+    if (a.maxd) H2D(a.maxd, max_gw_depth, nb);
+    H2D(a.pn, g->pending_net, nb);
+    H2D(a.pr, g->pending_recharge, nb);
+    H2D(a.pc, g->pending_capillary, nb);
+    if (dom) H2D(dom, p->domain, n);
+    if (e != hipSuccess)
+        return hp2d_handle_fail(h, std::string("hp2d_groundwater_module: ") + hipGetErrorString(e));
+    double err = 0.0;
+    int32_t did = 0;
+    st = gwm_run(h, p, g, a, col, dom, &err, &did, nullptr);
+    if (st != HP2D_OK) return st;
     D2H(h_t, a.h, nb);
     D2H(I_t, a.I, nb);
     D2H(d_t, a.d, nb);
@@ -372,6 +414,6 @@ extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_par
     if (e != hipSuccess)
         return hp2d_handle_fail(h, std::string("hp2d_groundwater_module: ") + hipGetErrorString(e));
     *error = err;
-    *ran = due ? 1 : 0;
+    *ran = did;
     return HP2D_OK;
 }

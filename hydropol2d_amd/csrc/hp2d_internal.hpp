@@ -31,6 +31,7 @@ enum PartField {
     kPartDtotMax,      // max d_tot/1000 (rollback celerity)   [m]
     kPartDprevMax,     // max d_p/1000                         [m]
     kPartNeg,          // 1 if a depth < -1e-8 mm was seen before clean-up
+    kPartDnewMax,      // max new d_t/1000 (late rollback celerity)  [m]
     kPartCount
 };
 
@@ -45,8 +46,16 @@ struct SimState {
     double cum_rain, cum_inf, cum_out, storage, uz;
     double max_velocity, max_vel_raster, tcr;
     double tmin_wq;
+    // water quality (WQ_States and the per-step outcome of build_up_wash_off)
+    double wq_initial_mass, wq_mass_lost, wq_sum_wout, wq_sum_b, wq_out_conc;
+    double wq_mass_outlet, wq_vol_outlet;
+    // groundwater: saturated storage + pending exchange [m3], model time since the last call
+    double gw_storage, gw_elapsed_min;
     long long k, accepted, rejected, inject_k, stop_accepted;
+    long long wq_n, wq_split_steps;  // wash-off calls (B_t buffer parity), split calls
     int cursor, catch_index, done, last_record, n_hydro, n_reports, neg_events, paused;
+    int hold;    // a wash-off split longer than the captured sub-steps waits for the host
+    int wq_err;  // the wash-off refused a split beyond 1e7 sub-steps
 };
 static_assert(sizeof(SimState) % 4 == 0, "SimState is copied as 32-bit words");
 

@@ -187,7 +187,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     T *I_dst = a.I_dst, *Tc_dst = a.T_dst;
     if (MODE == kModeSim) {
         const SimState *st = a.st;
-        if (st->done || st->paused) return;
+        if (st->done || st->paused || st->hold) return;
         kstep = st->k;
         dt_s = T(st->time_step_min * 60.0);
         dt_h = T(st->time_step_min / 60.0);
@@ -241,7 +241,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     T d2 = 0, scale2 = 0, finQx3 = 0, ox3 = 0, n2s2 = 0;
 
     // per-lane reductions
-    T vkey = T(NAN), wkey = T(NAN), rmax = T(NAN), dtmax = T(NAN), dpmax = T(NAN);
+    T vkey = T(NAN), wkey = T(NAN), rmax = T(NAN), dtmax = T(NAN), dpmax = T(NAN), dnmax = T(NAN);
     double s_store = 0.0, s_uz = 0.0, s_inf = 0.0;
     int neg = 0;
 
@@ -322,7 +322,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
             if (dd < T(1e-6)) dd = T(0);  // Hydrological_Model.m:49
             if (dd < T(0)) dd = T(0);     // :52
             dc = dd;
-            if (own && fail_step) {  // rollback celerity inputs (HydroPol2D_Main_While.m:932)
+            if (own && (fail_step || a.track_dmax)) {  // rollback celerity inputs (HydroPol2D_Main_While.m:932)
                 dtmax = dmax(dtmax, dc);
                 dpmax = dmax(dpmax, dprev);
             }
@@ -454,6 +454,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
                     }
                 }
                 if (may_report && !isnan(dn)) s_store += (double)dn;
+                if (a.track_dmax) dnmax = dmax(dnmax, dn);
             }
             if (out_cell) {
                 d_dst[oid] = dn;
@@ -513,7 +514,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
         __shared__ double red[kPartCount][kMaxWavesPerBlock];
         const T wv_v = wave_fmax(vkey), wv_r = wave_fmax(rmax);
         const T wv_w = a.adaptive ? wave_fmax(wkey) : wave_fmin(wkey);
-        const T wv_dt = wave_fmax(dtmax), wv_dp = wave_fmax(dpmax);
+        const T wv_dt = wave_fmax(dtmax), wv_dp = wave_fmax(dpmax), wv_dn = wave_fmax(dnmax);
         const double st_sum = wave_sum(s_store), uz_sum = wave_sum(s_uz);
         const double in_sum = wave_sum(s_inf);
         const int any_neg = __any(neg);
@@ -527,6 +528,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
             red[kPartDtotMax][wv] = (double)wv_dt;
             red[kPartDprevMax][wv] = (double)wv_dp;
             red[kPartNeg][wv] = any_neg ? 1.0 : 0.0;
+            red[kPartDnewMax][wv] = (double)wv_dn;
         }
         __syncthreads();
         if (threadIdx.x < kPartCount) {

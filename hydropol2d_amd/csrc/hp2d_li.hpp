@@ -38,6 +38,7 @@ template <typename T> struct LiArgs {
     const float *ga_f32;  // the same table as 8 floats per class (FP32 path)
     int n_soil_classes;
     T I_min;
+    int track_dmax;       // reduce the depth maxima every step (a late rollback can follow)
     T *dmax_map;
     T *tpeak_map;         // optional: model time [min] at which dmax was last raised
     // outlets (sorted linear indices)

@@ -32,13 +32,13 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     }
     __syncthreads();
     SimState *st = &sst;
-    if (st->done || st->paused) return;
-    __shared__ double sh[kFinWaves][12];
+    if (st->done || st->paused || st->hold) return;
+    __shared__ double sh[kFinWaves][14];
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
     const double *P = a.partials;
     const long long cap = a.part_cap;
     double storage = 0, uz = 0, rain = 0, infil = 0, vmax = NAN, dtw = NAN, raster = NAN;
-    double dtm = NAN, dpm = NAN, neg = 0, qsum = 0, dsum = 0;
+    double dtm = NAN, dpm = NAN, dnm = NAN, neg = 0, qsum = 0, dsum = 0, qraw = 0;
 #pragma unroll 4
     for (int i = t; i < a.n_partials; i += kFinThreads) {
         storage += P[kPartStorage * cap + i];
@@ -50,25 +50,29 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         dtm = fmax(dtm, P[kPartDtotMax * cap + i]);
         dpm = fmax(dpm, P[kPartDprevMax * cap + i]);
         neg = fmax(neg, P[kPartNeg * cap + i]);
+        if (a.flag_wq) dnm = fmax(dnm, P[kPartDnewMax * cap + i]);
     }
     for (int i = t; i < a.n_outlet; i += kFinThreads) {
         qsum += (double)a.out_q[i] * a.A;
         dsum += (double)a.out_d[i];
+        qraw += (double)a.out_q[i];
     }
     storage = wave_sum(storage); uz = wave_sum(uz); rain = wave_sum(rain);
     infil = wave_sum(infil); qsum = wave_sum(qsum); dsum = wave_sum(dsum);
     vmax = wave_fmax(vmax); dtw = a.adaptive ? wave_fmax(dtw) : wave_fmin(dtw);
     raster = wave_fmax(raster);
     dtm = wave_fmax(dtm); dpm = wave_fmax(dpm); neg = wave_fmax(neg);
+    dnm = wave_fmax(dnm); qraw = wave_sum(qraw);
     if (lane == 0) {
         sh[w][0] = storage; sh[w][1] = uz; sh[w][2] = rain; sh[w][3] = infil;
         sh[w][4] = vmax; sh[w][5] = dtw; sh[w][6] = raster; sh[w][7] = dtm;
         sh[w][8] = dpm; sh[w][9] = neg; sh[w][10] = qsum; sh[w][11] = dsum;
+        sh[w][12] = dnm; sh[w][13] = qraw;
     }
     __syncthreads();
     if (t != 0) return;
-    storage = uz = rain = infil = qsum = dsum = 0.0;
-    vmax = dtw = raster = dtm = dpm = NAN;
+    storage = uz = rain = infil = qsum = dsum = qraw = 0.0;
+    vmax = dtw = raster = dtm = dpm = dnm = NAN;
     neg = 0.0;
     for (int v = 0; v < kFinWaves; ++v) {  // fixed order: deterministic
         storage += sh[v][0]; uz += sh[v][1]; rain += sh[v][2]; infil += sh[v][3];
@@ -77,6 +81,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         raster = fmax(raster, sh[v][6]);
         dtm = fmax(dtm, sh[v][7]); dpm = fmax(dpm, sh[v][8]); neg = fmax(neg, sh[v][9]);
         qsum += sh[v][10]; dsum += sh[v][11];
+        dnm = fmax(dnm, sh[v][12]); qraw += sh[v][13];
     }
     // volumes were reduced as sums of depths [mm]
     const double a1000 = a.A / 1000.0;
@@ -86,6 +91,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     if (a.vmax_squared) vmax = sqrt(vmax);
     dtm /= 1000.0;  // max depths were reduced in mm
     dpm /= 1000.0;
+    dnm /= 1000.0;
     if (a.adaptive && !isnan(dtw)) {
         // refreshing_timestep.m:227-233 on the deepest non-stagnant cell:
         // alfa*dx/c is monotone in the depth, so its minimum is taken there
@@ -99,7 +105,15 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         return;
     }
     const long long k = st->k;
-    const bool failed = (st->inject_k == k);
+    bool failed = (st->inject_k == k);
+    if (a.flag_wq && !failed) {
+        // the wash-off of this step ran (HydroPol2D_Main_While.m:683-689): its
+        // buffers become current whatever follows, B_t is not rolled back
+        st->wq_n += 1;
+        st->wq_out_conc = fmax(1000.0 * st->wq_sum_wout / (qraw / 1000.0 * a.A), 0.0);
+        // :691-695: more than 1.2 x the initial mass raises an error (rollback)
+        if (st->wq_sum_b > 1.2 * st->wq_initial_mass) failed = true;
+    }
     // the same decision the step kernel took: storage sums exist only then
     const double prog_max = ((st->t + st->max_dt / 60.0) / a.routing_time) * 100.0;
     const bool may_report = (prog_max >= st->next_report) || (prog_max >= 100.0);
@@ -152,6 +166,12 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         int cur = st->cursor;
         st->delta_p_agg = rain_depth(a.rain_t, a.rain_v, a.n_rain, st->t_previous, t_now, &cur);
         st->cursor = cur;
+        // ---------------- EMC (:724-731), with the refreshed time_step ----------------
+        if (a.flag_wq) {
+            const double qo = qraw / 1000.0 / 3600.0 * 1000.0;
+            st->wq_mass_outlet = fmax(st->wq_mass_outlet + st->wq_out_conc * qo * (st->time_step_min * 60.0), 0.0);
+            st->wq_vol_outlet = fmax(qo * (st->time_step_min * 60.0) + st->wq_vol_outlet, 0.0);
+        }
         // ---------------- t_previous / t_save ----------------
         st->t_previous = (k == 1) ? tcr / 60.0 : t_now;
         const double t_save = t_now + tcr / 60.0;
@@ -182,6 +202,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         st->cum_rain += rain;
         st->cum_inf += infil;
         st->cum_out += qsum / 1000.0 / 3600.0 * (dt_used_min * 60.0);
+        if (a.flag_gw) st->gw_elapsed_min += dt_used_min;
         if (may_report) {
             st->storage = storage;
             st->uz = uz;
@@ -208,15 +229,22 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
             const double Qin = inflow_vol - Pv;
             const double Qout = qsum / 1000.0 / 3600.0 * ts * 60.0 + 0.0;
             const double fluxes = Pv + Qin - 0.0 - 0.0 - 0.0 - Qout - 0.0;
-            const double dS = (0.0 + storage + (a.ga ? uz : 0.0) + 0.0 + 0.0) - st->S_prev;
+            const double S_GW = a.flag_gw ? st->gw_storage : 0.0;
+            const double dS = (0.0 + storage + (a.ga ? uz : 0.0) + S_GW + 0.0) - st->S_prev;
             st->S_prev = st->S_prev + dS;
             st->volume_error = dS - fluxes;
             if (st->n_reports < a.n_rep_max) a.vol_err_hist[st->n_reports] = st->volume_error;
             st->n_reports += 1;
         }
-    } else {
+    }
+    // tmin_wq < 0, NaN or Inf raises at the end of the try block (:919-922):
+    // the catch block then runs on the state of the accepted step (d_p = d_t)
+    const bool late = !failed && a.flag_wq &&
+                      (st->tmin_wq < 0 || isnan(st->tmin_wq) || isinf(st->tmin_wq));
+    if (failed || late) {
         // ---------------- catch block (Main_While.m:926-993) ----------------
-        st->inject_k = -1;
+        if (late) dpm = dnm;
+        if (st->inject_k == k) st->inject_k = -1;
         st->rejected += 1;
         double t_now = st->t - st->time_step_min;
         st->t_previous = t_now;

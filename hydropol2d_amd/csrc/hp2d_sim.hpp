@@ -13,7 +13,7 @@ template <typename T> struct FinArgs {
     // running_control / flags
     double routing_time, tsc, inc, alfa_min, alfa_max, R, A, rec, report_pct;
     double factor_reduction, drainage_area, sum_pattern, n_valid;
-    int flag_timestep, adaptive, ga, flag_wq, has_pattern, n_rec, vmax_squared;
+    int flag_timestep, adaptive, ga, flag_wq, flag_gw, has_pattern, n_rec, vmax_squared;
     // lumped rainfall series
     const double *rain_t, *rain_v;
     int n_rain;

@@ -1,0 +1,380 @@
+// hp2d_simwq.hip - build_up_wash_off (build_up_wash_off.m:11-183) inside the
+// device-resident step loop, after the local-inertial update of
+// HydroPol2D_Main_While.m:683-689.
+//
+// The five face pages of the reference (left, right, up, down, outlet) are
+// not stored: a cell's pages are its own face fluxes and its neighbours'
+// (qout_left = -q_x of the left cell, qout_down = -q_y of the cell below,
+// Local_Inertial_Model_D4.m:347-350), and a neighbour's rate towards the cell
+// is recomputed from the neighbour's mass, class and the shared face flux, so
+// one grid-stride pass yields dW without a W buffer.
+//
+// Per step: wq_first_kernel (rates, dW, B_t_mid, tentative outputs of the
+// unsplit case, block partials) -> wq_ctl_kernel (one block: the tmin_wq and
+// split decision, the closing scalars of the unsplit case) -> s_cap guarded
+// wq_sub_kernel launches (the internal sub-steps, :80-132; the last one
+// closes the call: clamp, tmin_wq, P_conc, Tot_Washed) -> wq_fold_kernel (one
+// block: the closing scalars of a split call).  A split longer than s_cap
+// raises SimState::hold, which makes every later kernel of the captured step
+// sequence a no-op until the host has run the remaining sub-steps.
+//
+// Face rates below zero carry no wash-off (q < 0 is taken as 0 before the
+// power law, as in the per-call drop-in); tot_q_out keeps the signed sum.
+#include "hp2d_simwq.hpp"
+
+namespace hp2d {
+
+constexpr int kWqThreads = 256;
+constexpr int kWqFoldThreads = 1024;
+
+__device__ __forceinline__ double wq_pow(double x, double c) { return pow(x, c); }
+__device__ __forceinline__ float wq_pow(float x, float c) {
+    return c == 0.0f ? 1.0f : exp2f(c * log2f(x));  // pow(x, 0) = 1 as for every x
+}
+
+// wash-off parameters of one cell for the current pass
+template <typename T> struct WqCell {
+    T B, f, C3, C4;
+    bool zero;  // idx_bt (or the idx_nan_5 mask of the first pass): no flux
+};
+
+template <typename T>
+__device__ __forceinline__ WqCell<T> wq_cell(const WqSimArgs<T> &a, long long c, const T *Bin,
+                                             const T *Bx_in, bool first, T *Bx_out) {
+    WqCell<T> w;
+    const unsigned fl = a.cell[c];
+    const int cls = (int)(fl & 0xffu);
+    const T B = Bin[c];
+    w.B = B;
+    w.C3 = a.C3[cls];
+    w.C4 = a.C4[cls];
+    T f = B;
+    if (a.model == 1) {  // rating curve (:23-28, :89-94)
+        const T Br = T(a.Bmin / 1000.0 * a.A), Bm = T(a.Bmax / 1000.0 * a.A);
+        T bx = first ? B : Bx_in[c];
+        if (B * T(1000) / T(a.A) <= T(a.Bmin)) bx = T(0);
+        if (B * T(1000) / T(a.A) >= T(a.Bmax)) bx = Bm;
+        if (Bx_out) Bx_out[c] = bx;
+        f = T(1) + dmax(bx - Br, T(0));
+    }
+    w.f = f;
+    const bool low = (B * T(1000) / T(a.A) < T(a.min_Bt)) || isnan(B) || isinf(B);
+    w.zero = low || (first && (fl & kWqNanBit));
+    return w;
+}
+
+// W of one page: C3 (q/1000/3600*A)^C4 f(B) [kg/h]
+template <typename T>
+__device__ __forceinline__ T wq_rate(const WqCell<T> &w, T q, T A) {
+    const T qq = q < T(0) ? T(0) : q;  // NaN stays NaN
+    const T r = w.C3 * wq_pow(qq / T(1000) / T(3600) * A, w.C4) * w.f;
+    return w.zero ? T(0) : r;
+}
+
+template <typename T>
+__device__ __forceinline__ T outlet_q(const WqSimArgs<T> &a, long long c) {
+    if (!(a.cell[c] & kWqOutletBit)) return T(0);
+    int lo = 0, hi = a.n_outlet - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const long long v = a.out_idx[mid];
+        if (v == c) return a.out_q[mid];
+        if (v < c) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return T(0);
+}
+
+// the pass is skipped when the run is over / paused, when the step was
+// rejected before the wash-off (injected failure), or on hold
+__device__ __forceinline__ bool wq_skip(const SimState *st, bool force) {
+    return st->done || st->paused || st->inject_k == st->k || (st->hold && !force);
+}
+
+// fixed-order block reduction of up to 8 fields (mins or sums)
+template <int NF>
+__device__ __forceinline__ void block_store(double (&v)[NF], const bool (&is_min)[NF],
+                                            double *part, int nb) {
+    __shared__ double sh[NF][kWqThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        v[f] = is_min[f] ? wave_fmin(v[f]) : wave_sum(v[f]);
+        if (lane == 0) sh[f][w] = v[f];
+    }
+    __syncthreads();
+    if (threadIdx.x < NF) {
+        const int f = threadIdx.x;
+        double acc = sh[f][0];
+        for (int k = 1; k < kWqThreads / kWave; ++k) acc = is_min[f] ? fmin(acc, sh[f][k]) : acc + sh[f][k];
+        part[(long long)f * nb + blockIdx.x] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// first pass (:13-79): rates of the entry B_t, dW, B_t_mid, tmin candidates;
+// writes the unsplit outcome (B_t, Tot_Washed, P_conc), overwritten by the
+// last sub-step when the call splits
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kWqThreads) void wq_first_kernel(WqSimArgs<T> a) {
+    const SimState *st = a.st;
+    if (wq_skip(st, false)) return;
+    const int qp = (int)(st->k & 1);          // fluxes just written by the step
+    const int bp = (int)(st->wq_n & 1);       // entry B_t of this call
+    const T *qx = a.qx_buf[qp], *qy = a.qy_buf[qp];
+    const T *Bin = a.B[bp];
+    T *Bout = a.B[bp ^ 1];
+    const T *TWin = a.TW[bp];
+    T *TWout = a.TW[bp ^ 1];
+    const double dt_min = st->time_step_min;
+    const T A = T(a.A);
+    const int ny = a.ny, nx = a.nx;
+    double v[6] = {NAN, NAN, NAN, NAN, 0.0, 0.0};
+    const bool is_min[6] = {true, true, true, true, false, false};
+    for (long long c = blockIdx.x * (long long)kWqThreads + threadIdx.x; c < a.n;
+         c += (long long)gridDim.x * kWqThreads) {
+        const int i = (int)(c % ny), j = (int)(c / ny);
+        const WqCell<T> w = wq_cell(a, c, Bin, (const T *)nullptr, true,
+                                    a.model == 1 ? a.Bx[0] : (T *)nullptr);
+        const T qr = qx[c], qu = qy[c];
+        const T ql = j > 0 ? -qx[c - ny] : T(0);
+        const T qd = i < ny - 1 ? -qy[c + 1] : T(0);
+        const T qo = outlet_q(a, c);
+        const T W0 = wq_rate(w, ql, A), W1 = wq_rate(w, qr, A), W2 = wq_rate(w, qu, A);
+        const T W3 = wq_rate(w, qd, A), W4 = wq_rate(w, qo, A);
+        const T totW = W0 + W1 + W2 + W3 + W4;
+        const T totQ = (a.cell[c] & kWqNanBit) ? T(0) : ql + qr + qu + qd + qo;
+        // the neighbours' pages towards this cell (:51-55)
+        T in_l = T(0), in_r = T(0), in_u = T(0), in_d = T(0);
+        if (j > 0) in_l = wq_rate(wq_cell(a, c - ny, Bin, (const T *)nullptr, true, (T *)nullptr), qx[c - ny], A);
+        if (j < nx - 1) in_r = wq_rate(wq_cell(a, c + ny, Bin, (const T *)nullptr, true, (T *)nullptr), -qr, A);
+        if (i > 0) in_u = wq_rate(wq_cell(a, c - 1, Bin, (const T *)nullptr, true, (T *)nullptr), -qu, A);
+        if (i < ny - 1) in_d = wq_rate(wq_cell(a, c + 1, Bin, (const T *)nullptr, true, (T *)nullptr), qy[c + 1], A);
+        const T dW = totW - (in_l + in_r + in_u + in_d);
+        const T Bmid = w.B - dW * T(dt_min) / T(60);
+        // tmin candidates (:61-76, :144-159): m_calc / (|min(dW,0)| + 1e-16)
+        const T d0 = dW > T(0) ? T(0) : dW;
+        const T mc0 = (w.B * T(1000) / A < T(a.min_Bt)) ? T(INFINITY) : w.B;
+        const T mce = (Bmid * T(1000) / A < T(a.min_Bt)) ? T(INFINITY) : Bmid;
+        const T cand0 = mc0 / (fabs(d0) + T(1e-16));
+        const T cande = mce / (fabs(d0) + T(1e-16));
+        const T Bfin = fabs(Bmid) < T(1e-12) ? T(0) : Bmid;  // :171
+        Bout[c] = Bfin;
+        TWout[c] = totW * T(dt_min) / T(60) + TWin[c];
+        const T tq = dmax(totQ, T(10));
+        a.Pc[c] = dmax(T(1e6) * totW / (tq * A), T(0));
+        v[kWqMinDW] = fmin(v[kWqMinDW], (double)dW);
+        v[kWqMinMid] = fmin(v[kWqMinMid], (double)Bmid);
+        v[kWqCand0] = fmin(v[kWqCand0], (double)cand0);
+        v[kWqCandEnd] = fmin(v[kWqCandEnd], (double)cande);
+        v[kWqSumWout] += (double)W4;
+        if (!isinf(Bfin)) v[kWqSumB] += (double)Bfin;
+    }
+    block_store<6>(v, is_min, a.part, a.n_blocks);
+}
+
+// one block: folds the first pass; decides the split; closes an unsplit call
+__device__ __forceinline__ void fold_fields(const double *part, int nb, int f0, int nf,
+                                            const bool *is_min, double *out) {
+    __shared__ double sh[8][kWqFoldThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    for (int f = 0; f < nf; ++f) {
+        double acc = is_min[f] ? NAN : 0.0;
+        for (int b = threadIdx.x; b < nb; b += kWqFoldThreads) {
+            const double x = part[(long long)(f0 + f) * nb + b];
+            acc = is_min[f] ? fmin(acc, x) : acc + x;
+        }
+        acc = is_min[f] ? wave_fmin(acc) : wave_sum(acc);
+        if (lane == 0) sh[f][w] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int f = 0; f < nf; ++f) {
+            double acc = sh[f][0];
+            for (int k = 1; k < kWqFoldThreads / kWave; ++k)
+                acc = is_min[f] ? fmin(acc, sh[f][k]) : acc + sh[f][k];
+            out[f] = acc;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWqFoldThreads) void wq_ctl_kernel(WqSimArgs<T> a) {
+    SimState *st = a.st;
+    if (wq_skip(st, false)) return;
+    double r[6];
+    const bool is_min[6] = {true, true, true, true, false, false};
+    fold_fields(a.part, a.n_blocks, 0, 6, is_min, r);
+    if (threadIdx.x != 0) return;
+    const double dt_s = st->time_step_min * 60.0;  // large_timestep
+    const double tmin0 = r[kWqMinDW] >= 0 ? dt_s : 3600.0 * r[kWqCand0];
+    WqCtl *ctl = a.ctl;
+    ctl->tmin0 = tmin0;
+    ctl->dt_s = dt_s;
+    const bool split = r[kWqMinMid] < 0 || tmin0 < dt_s;
+    ctl->split = split ? 1 : 0;
+    if (!split) {
+        // B_t = B_t_mid: nothing negative to clamp, mass_lost += 0
+        st->tmin_wq = r[kWqMinDW] >= 0 ? dt_s : 3600.0 * r[kWqCandEnd];
+        st->wq_sum_wout = r[kWqSumWout];
+        st->wq_sum_b = r[kWqSumB];
+        ctl->steps = 0;
+        return;
+    }
+    const double n_dec = dt_s / tmin0;
+    const double n_floor = floor(n_dec);
+    ctl->n_dec = n_dec;
+    ctl->n_floor = n_floor;
+    st->wq_split_steps += 1;
+    if (!(n_floor < 1e7)) {  // tmin_wq = 0 would loop for ever in the reference
+        st->wq_err = 1;
+        st->hold = 1;
+        ctl->steps = 0;
+        return;
+    }
+    ctl->steps = (long long)n_floor + 1;
+    if (ctl->steps > a.s_cap) st->hold = 1;
+}
+
+// ---------------------------------------------------------------------------
+// sub-step k of a split call (:85-125); the last one closes the call
+// (:128-140 clamp and mass_lost, :144-159 tmin_wq, :171, :175-182)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kWqThreads) void wq_sub_kernel(WqSimArgs<T> a, int k, int force) {
+    const SimState *st = a.st;
+    if (wq_skip(st, force != 0)) return;
+    const WqCtl *ctl = a.ctl;
+    if (!ctl->split) return;
+    const long long steps = ctl->steps;
+    if (k > steps) return;
+    const int qp = (int)(st->k & 1), bp = (int)(st->wq_n & 1);
+    const T *qx = a.qx_buf[qp], *qy = a.qy_buf[qp];
+    // ping-pong so that the last sub-step lands in the call's output buffer
+    T *dst_final = a.B[bp ^ 1];
+    auto buf = [&](long long s) { return ((steps - s) % 2 == 0) ? dst_final : a.Bs; };
+    const T *Bin = (k == 1) ? a.B[bp] : buf(k - 1);
+    T *Bout = buf(k);
+    const T *Bx_in = a.model == 1 ? a.Bx[(k - 1) & 1] : nullptr;
+    T *Bx_out = a.model == 1 ? a.Bx[k & 1] : nullptr;
+    const double dt = (k == steps) ? (ctl->n_dec - ctl->n_floor) * ctl->tmin0
+                                   : ctl->dt_s / (double)steps;
+    const bool last = (k == steps);
+    const double dt_min = st->time_step_min;
+    const T A = T(a.A), dtT = T(dt);
+    const int ny = a.ny, nx = a.nx;
+    const T *TWin = a.TW[bp];
+    T *TWout = a.TW[bp ^ 1];
+    double v[5] = {NAN, NAN, 0.0, 0.0, 0.0};  // minDW, candEnd, sumNeg, outMass, sumB
+    const bool is_min[5] = {true, true, false, false, false};
+    for (long long c = blockIdx.x * (long long)kWqThreads + threadIdx.x; c < a.n;
+         c += (long long)gridDim.x * kWqThreads) {
+        const int i = (int)(c % ny), j = (int)(c / ny);
+        const WqCell<T> w = wq_cell(a, c, Bin, Bx_in, false, Bx_out);
+        const T qr = qx[c], qu = qy[c];
+        const T ql = j > 0 ? -qx[c - ny] : T(0);
+        const T qd = i < ny - 1 ? -qy[c + 1] : T(0);
+        const T qo = outlet_q(a, c);
+        const T W0 = wq_rate(w, ql, A), W1 = wq_rate(w, qr, A), W2 = wq_rate(w, qu, A);
+        const T W3 = wq_rate(w, qd, A), W4 = wq_rate(w, qo, A);
+        const T totW = W0 + W1 + W2 + W3 + W4;
+        T in_l = T(0), in_r = T(0), in_u = T(0), in_d = T(0);
+        if (j > 0) in_l = wq_rate(wq_cell(a, c - ny, Bin, Bx_in, false, (T *)nullptr), qx[c - ny], A);
+        if (j < nx - 1) in_r = wq_rate(wq_cell(a, c + ny, Bin, Bx_in, false, (T *)nullptr), -qr, A);
+        if (i > 0) in_u = wq_rate(wq_cell(a, c - 1, Bin, Bx_in, false, (T *)nullptr), -qu, A);
+        if (i < ny - 1) in_d = wq_rate(wq_cell(a, c + 1, Bin, Bx_in, false, (T *)nullptr), qy[c + 1], A);
+        const T dW = totW - (in_l + in_r + in_u + in_d);
+        const T om = (k == 1 ? T(0) : a.omass[c]) + totW * dtT / T(3600);
+        T Bn = w.B - dW * dtT / T(3600);
+        v[3] += (double)(W4 * dtT / T(3600));
+        if (!last) {
+            a.omass[c] = om;
+            Bout[c] = Bn;
+            continue;
+        }
+        if (Bn < T(0)) v[2] += (double)Bn;
+        if (Bn < T(0)) Bn = T(0);
+        const T d0 = dW > T(0) ? T(0) : dW;
+        const T mce = (Bn * T(1000) / A < T(a.min_Bt)) ? T(INFINITY) : Bn;
+        v[0] = fmin(v[0], (double)dW);
+        v[1] = fmin(v[1], (double)(mce / (fabs(d0) + T(1e-16))));
+        if (fabs(Bn) < T(1e-12)) Bn = T(0);
+        Bout[c] = Bn;
+        if (!isinf(Bn)) v[4] += (double)Bn;
+        const T totWa = om / (T(dt_min) / T(60));
+        const T totQ = (a.cell[c] & kWqNanBit) ? T(0) : ql + qr + qu + qd + qo;
+        TWout[c] = totWa * T(dt_min) / T(60) + TWin[c];
+        a.Pc[c] = dmax(T(1e6) * totWa / (dmax(totQ, T(10)) * A), T(0));
+    }
+    // partials: the outlet mass accumulates over the sub-steps in this block's slot
+    __shared__ double prev;
+    if (threadIdx.x == 0) prev = (k == 1) ? 0.0 : a.part[(long long)kWqOutMass * a.n_blocks + blockIdx.x];
+    __syncthreads();
+    const double prev_out = prev;
+    __syncthreads();
+    double u[5] = {v[0], v[1], v[2], v[3], v[4]};
+    const bool m[5] = {is_min[0], is_min[1], is_min[2], is_min[3], is_min[4]};
+    // store as fields kWqMinDW, kWqCandEnd, kWqSumNeg, kWqOutMass, kWqSumB
+    __shared__ double sh[5][kWqThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        u[f] = m[f] ? wave_fmin(u[f]) : wave_sum(u[f]);
+        if (lane == 0) sh[f][wv] = u[f];
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        const int f = threadIdx.x;
+        double acc = sh[f][0];
+        for (int q = 1; q < kWqThreads / kWave; ++q) acc = m[f] ? fmin(acc, sh[f][q]) : acc + sh[f][q];
+        const int field[5] = {kWqMinDW, kWqCandEnd, kWqSumNeg, kWqOutMass, kWqSumB};
+        if (f == 3) acc = prev_out + acc;
+        if (f == 3 || last) a.part[(long long)field[f] * a.n_blocks + blockIdx.x] = acc;
+    }
+}
+
+// one block: the closing scalars of a split call
+template <typename T>
+__global__ __launch_bounds__(kWqFoldThreads) void wq_fold_kernel(WqSimArgs<T> a, int force) {
+    SimState *st = a.st;
+    if (wq_skip(st, force != 0)) return;
+    const WqCtl *ctl = a.ctl;
+    if (!ctl->split) return;
+    double r[8];
+    const bool is_min[8] = {true, false, false, true, false, false, false, false};
+    // fields kWqMinDW .. kWqOutMass are read; kWqMinMid, kWqCand0, kWqSumWout are stale
+    fold_fields(a.part, a.n_blocks, 0, 8, is_min, r);
+    if (threadIdx.x != 0) return;
+    const double dt_s = ctl->dt_s;
+    st->tmin_wq = r[kWqMinDW] >= 0 ? dt_s : 3600.0 * r[kWqCandEnd];
+    st->wq_mass_lost = r[kWqSumNeg] + st->wq_mass_lost;
+    st->wq_sum_wout = r[kWqOutMass] / (st->time_step_min / 60.0);
+    st->wq_sum_b = r[kWqSumB];
+    if (force) st->hold = 0;
+}
+
+template <typename T> hipError_t launch_wq_step(const WqSimArgs<T> &a, hipStream_t s) {
+    hipLaunchKernelGGL((wq_first_kernel<T>), dim3(a.n_blocks), dim3(kWqThreads), 0, s, a);
+    hipLaunchKernelGGL((wq_ctl_kernel<T>), dim3(1), dim3(kWqFoldThreads), 0, s, a);
+    for (int k = 1; k <= a.s_cap; ++k)
+        hipLaunchKernelGGL((wq_sub_kernel<T>), dim3(a.n_blocks), dim3(kWqThreads), 0, s, a, k, 0);
+    hipLaunchKernelGGL((wq_fold_kernel<T>), dim3(1), dim3(kWqFoldThreads), 0, s, a, 0);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_wq_finish(const WqSimArgs<T> &a, long long steps, hipStream_t s) {
+    for (long long k = 1; k <= steps; ++k)
+        hipLaunchKernelGGL((wq_sub_kernel<T>), dim3(a.n_blocks), dim3(kWqThreads), 0, s, a,
+                           (int)k, 1);
+    hipLaunchKernelGGL((wq_fold_kernel<T>), dim3(1), dim3(kWqFoldThreads), 0, s, a, 1);
+    return hipGetLastError();
+}
+
+template hipError_t launch_wq_step<float>(const WqSimArgs<float> &, hipStream_t);
+template hipError_t launch_wq_step<double>(const WqSimArgs<double> &, hipStream_t);
+template hipError_t launch_wq_finish<float>(const WqSimArgs<float> &, long long, hipStream_t);
+template hipError_t launch_wq_finish<double>(const WqSimArgs<double> &, long long, hipStream_t);
+
+}  // namespace hp2d

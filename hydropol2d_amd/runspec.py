@@ -63,14 +63,25 @@ class WaterQuality:
 
 @dataclass
 class Groundwater:
-    """Boussinesq aquifer (Boussinesq_2D_explicit.m) run every `every_steps` steps."""
+    """Groundwater_Module (vadose bucket on I_t, Boussinesq baseflow) run every
+    `every_steps` accepted surface steps over the interval since its last call.
+
+    h0: initial water-table head [m]; soil_depth: soil above the aquifer base
+    [m] (grid or scalar); ksat_mmh: aquifer conductivity [mm/h]; Sy: specific
+    yield; theta_sat / theta_r / n_vg / alpha_vg [1/m]: van Genuchten
+    parameters per soil class (the Green-Ampt classes carry the soil Ks)."""
 
     h0: np.ndarray
-    z0: np.ndarray
-    K: float = 1e-5
+    soil_depth: object
+    theta_sat: np.ndarray
+    theta_r: np.ndarray
+    n_vg: np.ndarray
+    alpha_vg: np.ndarray
+    ksat_mmh: float = 36.0
     Sy: float = 0.2
     every_steps: int = 10
-    courant: float = 0.5
+    courant: float = 0.25
+    l_vg: float = 0.5
 
 
 @dataclass
@@ -157,7 +168,7 @@ class RunSpec:
         cfg.flag_wq_model = self.water_quality.flag_wq_model if self.water_quality else 0
         cfg.flag_groundwater = 1 if self.groundwater is not None else 0
         cfg.gw_every_steps = self.groundwater.every_steps if self.groundwater else 0
-        cfg.gw_courant = self.groundwater.courant if self.groundwater else 0.5
+        cfg.gw_courant = self.groundwater.courant if self.groundwater else 0.25
 
         ins = abi.RunInputs()
         ins.elevation = _ptr(f(self.elevation), C.c_double)
@@ -195,8 +206,15 @@ class RunSpec:
             ins.wq_min_Bt = wq.min_Bt
         gw = self.groundwater
         if gw is not None:
+            ny, nx = np.shape(self.elevation)
             ins.gw_h0 = _ptr(f(gw.h0), C.c_double)
-            ins.gw_z0 = _ptr(f(gw.z0), C.c_double)
-            ins.gw_K = gw.K
+            sd = np.broadcast_to(np.asarray(gw.soil_depth, dtype=np.float64), (ny, nx))
+            ins.gw_soil_depth = _ptr(f(sd), C.c_double)
+            ins.gw_ksat_mmh = gw.ksat_mmh
             ins.gw_Sy = gw.Sy
+            ins.gw_theta_sat = _ptr(f(np.atleast_1d(gw.theta_sat)), C.c_double)
+            ins.gw_theta_r = _ptr(f(np.atleast_1d(gw.theta_r)), C.c_double)
+            ins.gw_n_vg = _ptr(f(np.atleast_1d(gw.n_vg)), C.c_double)
+            ins.gw_alpha_vg = _ptr(f(np.atleast_1d(gw.alpha_vg)), C.c_double)
+            ins.gw_l_vg = gw.l_vg
         return cfg, ins, keep

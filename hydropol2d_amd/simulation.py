@@ -78,18 +78,31 @@ class Simulation:
         self._check(self._L.hp2d_sim_report(self._s, C.byref(rep)))
         return rep.as_dict()
 
-    def state(self, infiltration: bool = False) -> dict:
+    def state(self, infiltration: bool = False, water_quality: bool = False,
+              groundwater: bool = False) -> dict:
+        """Current state: d_t [mm], outflow pages [mm/h], dmax [mm]; optionally I_t
+        [mm], the water-quality maps (B_t [kg], P_conc [mg/L], Tot_Washed [kg]) and
+        the groundwater head gw_h [m]."""
         ny, nx = self.ny, self.nx
         d = np.zeros((ny, nx), order="F")
         of = np.zeros((ny, nx, 3), order="F")
         dm = np.zeros((ny, nx), order="F")
         It = np.zeros((ny, nx), order="F") if infiltration else None
+        Bt = np.zeros((ny, nx), order="F") if water_quality else None
+        gh = np.zeros((ny, nx), order="F") if groundwater else None
         p = (lambda a: a.ctypes.data_as(_dp) if a is not None else _dp())
-        self._check(self._L.hp2d_sim_get_state(self._s, p(d), p(of), _dp(), p(It), p(dm), _dp(),
-                                               _dp()))
+        self._check(self._L.hp2d_sim_get_state(self._s, p(d), p(of), _dp(), p(It), p(dm), p(Bt),
+                                               p(gh)))
         out = {"d_t": d, "outflow": of, "dmax": dm}
         if infiltration:
             out["I_t"] = It
+        if water_quality:
+            pc = np.zeros((ny, nx), order="F")
+            tw = np.zeros((ny, nx), order="F")
+            self._check(self._L.hp2d_sim_get_wq_maps(self._s, p(pc), p(tw)))
+            out.update(B_t=Bt, P_conc=pc, Tot_Washed=tw)
+        if groundwater:
+            out["gw_h"] = gh
         return out
 
     def hydrograph(self):

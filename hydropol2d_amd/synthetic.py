@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .runspec import GreenAmpt, RunSpec
+from .runspec import GreenAmpt, Groundwater, RunSpec, WaterQuality
 
 
 def tilted_plane_c1(ny=256, nx=256, dx=5.0, seed=1, precision="fp64", routing_time_min=60.0):
@@ -100,11 +100,34 @@ def gaussian_random_field(ny, nx, dx, mean, cv, corr_m, seed):
     return np.maximum(field, 0.0)
 
 
+def c2_water_quality(lulc):
+    """One pollutant with exponential build-up (7 dry days) and a mass-based
+    power-law wash-off, parameters per LULC class; cells under 0.01 g/m2 carry
+    no wash-off (min_Bt; with 0 a washed-out cell would give tmin_wq = 0)."""
+    k = int(np.max(lulc)) + 1
+    return WaterQuality(C1=np.linspace(20.0, 120.0, k), C2=np.full(k, 0.3),
+                        C3=np.linspace(5.0, 30.0, k), C4=np.full(k, 1.1),
+                        lulc_class=np.asfortranarray(lulc.astype(np.uint8)), ADD_days=7.0,
+                        min_Bt=0.01)
+
+
+def c2_groundwater(z, n_soil, soil_depth=2.0, wt_depth=1.0, every_steps=10):
+    """Aquifer under a 2 m soil (Sy = 0.2, K = 1e-5 m/s = 36 mm/h), water table 1 m
+    below the ground, van Genuchten soils from sand to clay over the soil classes."""
+    return Groundwater(h0=np.asfortranarray(z - wt_depth), soil_depth=soil_depth,
+                       theta_sat=np.linspace(0.40, 0.47, n_soil),
+                       theta_r=np.linspace(0.05, 0.09, n_soil),
+                       n_vg=np.linspace(2.6, 1.3, n_soil), alpha_vg=np.linspace(14.0, 1.0, n_soil),
+                       ksat_mmh=36.0, Sy=0.2, every_steps=every_steps)
+
+
 def full_physics_c2(ny=4096, nx=4096, dx=2.0, precision="fp32", routing_time_min=180.0,
-                    fill_sinks=None):
+                    fill_sinks=None, water_quality=True, groundwater=True):
     """Fractal DEM (Hurst 0.8, 200 m relief), 8 LULC classes (n in [0.015, 0.15]), 5 soil
     classes (Ks in [1, 50] mm/h), GRF rainfall pattern (mean 40 mm/h, CV 0.5, 5 km
-    correlation) on a 2 h triangular hyetograph, perimeter outlets at the lowest edge."""
+    correlation) on a 2 h triangular hyetograph, perimeter outlets at the lowest edge,
+    one pollutant with build-up / wash-off per LULC class and the groundwater module
+    (Sy = 0.2, K = 1e-5 m/s) every 10 surface steps."""
     z = spectral_dem(ny, nx, dx, 0.8, 200.0, seed=2)
     # tilt towards the lowest column so that the catchment drains to one edge
     z = z + np.linspace(50.0, 0.0, nx)[None, :]
@@ -132,7 +155,9 @@ def full_physics_c2(ny=4096, nx=4096, dx=2.0, precision="fp32", routing_time_min
         rain_pattern=np.asfortranarray(pattern), precision=precision, outlet_type=2,
         routing_time_min=routing_time_min, min_time_step_s=0.01, max_time_step_s=10.0,
         time_step_increments_s=0.001, time_step_change_s=1e-4, alfa_min=0.7, alfa_max=0.7,
-        record_time_hydrographs_min=1.0, infiltration=ga)
+        record_time_hydrographs_min=1.0, infiltration=ga,
+        water_quality=c2_water_quality(lulc) if water_quality else None,
+        groundwater=c2_groundwater(z, 5) if groundwater else None)
 
 
 def dam_break_c3(ny=2048, nx=2048, dx=1.0, precision="fp32", routing_time_min=30.0):

@@ -291,9 +291,10 @@ typedef struct hp2d_run_config {
     int32_t flag_infiltration;         /* 1 = Green-Ampt */
     int32_t flag_waterquality;         /* 1 = build-up / wash-off */
     int32_t flag_wq_model;             /* 0 mass model, 1 rating model */
-    int32_t flag_groundwater;          /* 1 = Boussinesq every gw_every_steps */
+    int32_t flag_groundwater;          /* 1 = Groundwater_Module (Boussinesq baseflow) every
+                                          gw_every_steps accepted steps; needs flag_infiltration */
     int32_t gw_every_steps;
-    double gw_courant;
+    double gw_courant;                 /* flags.groundwater_courant */
     int32_t n_soil_classes;            /* GA class tables below */
     int32_t n_lulc_classes;            /* WQ class tables below */
 } hp2d_run_config;
@@ -324,10 +325,14 @@ typedef struct hp2d_run_inputs {
     const double *wq_C1, *wq_C2, *wq_C3, *wq_C4; /* [n_lulc_classes] */
     double wq_ADD_days;
     double wq_Bmin, wq_Bmax, wq_min_Bt;
-    /* Groundwater (Boussinesq) */
-    const double *gw_h0;               /* initial head [m] */
-    const double *gw_z0;               /* aquifer base [m] */
-    double gw_K, gw_Sy;
+    /* Groundwater (Groundwater_Module.m with simulate_groundwater_recharge.m and
+     * Boussinesq_2D_explicit.m): the vadose bucket is I_t, the soil classes
+     * carry the van Genuchten parameters, the soil Ks is ksat_mmh above */
+    const double *gw_h0;               /* initial water-table head [m] */
+    const double *gw_soil_depth;       /* soil depth above the aquifer base [m] */
+    double gw_ksat_mmh, gw_Sy;         /* aquifer conductivity [mm/h], specific yield */
+    const double *gw_theta_sat, *gw_theta_r, *gw_n_vg, *gw_alpha_vg; /* [n_soil_classes] */
+    double gw_l_vg;                    /* Mualem pore-connectivity (0.5) */
 } hp2d_run_inputs;
 
 /* Results of hp2d_sim_run, accumulated on the device in FP64. */
@@ -345,6 +350,19 @@ typedef struct hp2d_run_report {
     double cum_rain_m3, cum_infiltration_m3, cum_outflow_m3; /* every-step FP64 ledgers */
     double storage_m3;                 /* surface storage at exit */
     int32_t catch_index;
+    /* water quality (WQ_States, build_up_wash_off outputs of the last step) */
+    double wq_mass_kg;                 /* sum of B_t over the domain */
+    double wq_mass_lost_kg;            /* WQ_States.mass_lost */
+    double wq_out_conc_mgl;            /* Out_Conc */
+    double wq_tmin_s;                  /* tmin_wq */
+    double wq_mass_outlet_mg;          /* WQ_States.mass_outlet (Main_While.m:728) */
+    double wq_vol_outlet;              /* WQ_States.vol_outlet (Main_While.m:729) */
+    int64_t wq_substep_steps;          /* steps whose wash-off was split internally */
+    /* groundwater (GW_States) */
+    double gw_storage_m3;              /* sum A*Sy*(h_t - z_bed) */
+    double gw_total_recharge_m3, gw_total_exfiltration_m3;
+    double gw_error_m3;                /* errors(5) of the last update */
+    int32_t gw_updates;
 } hp2d_run_report;
 
 typedef struct hp2d_sim hp2d_sim;
@@ -372,6 +390,10 @@ hp2d_status hp2d_sim_report(hp2d_sim *s, hp2d_run_report *rep);
  * outflow: ny*nx*3 [mm/h]; Hf: ny*nx*3 [m] (of the last step). */
 hp2d_status hp2d_sim_get_state(hp2d_sim *s, double *d_t, double *outflow, double *Hf,
                                double *I_t, double *dmax, double *B_t, double *gw_h);
+
+/* Water-quality maps of the last step: P_conc [mg/L] and Tot_Washed [kg]
+ * (build_up_wash_off.m:177,182); ny*nx each, NULL skips. */
+hp2d_status hp2d_sim_get_wq_maps(hp2d_sim *s, double *P_conc, double *Tot_Washed);
 
 /* Hydrograph records (save_output_maps.m:266-269,409,481-490):
  * q_m3s[n], time_min[n], depth_outlet_mm[n]; n_max capacity, *n_out filled. */

@@ -271,7 +271,7 @@ int32_t oracle_li_d4(int32_t ny, int32_t nx, int32_t scheme, const double *z,
 /* ------------------------------------------------------------------------ */
 double oracle_refresh_dt(int32_t ny, int32_t nx, const double *qout_right,
                          const double *qout_up, const double *Hf,
-                         const oracle_dt_params *p, double *max_velocity,
+                         oracle_dt_params *p, double *max_velocity,
                          double *velocity_raster) {
     const int64_t n = (int64_t)ny * nx;
     const double R = p->resolution;
@@ -322,6 +322,7 @@ double oracle_refresh_dt(int32_t ny, int32_t nx, const double *qout_right,
     double nt = (p->flag_timestep == 1) ? fmin(dt_adv, dtw) : dtw;
     if (p->flag_waterquality == 1) nt = fmin(1.0 * p->tmin_wq, nt);
     if (isnan(nt) || isinf(nt)) nt = p->max_time_step;
+    p->nt_raw = nt;
     /* :344-353 */
     double tcr = fmax(p->time_step_increments * floor(nt / p->time_step_increments), p->min_time_step);
     tcr = fmin(tcr, p->max_time_step);
@@ -388,6 +389,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     const int64_t n = (int64_t)ny * nx;
     const double R = cfg->resolution, A = R * R;
     if (cfg->routing != HP2D_ROUTING_INERTIAL) return HP2D_E_UNSUPPORTED;
+    if (cfg->flag_groundwater == 1 && cfg->flag_infiltration != 1) return HP2D_E_UNSUPPORTED;
 
     double *z = zalloc(n), *nman = zalloc(n), *n2 = zalloc(n);
     double *d_t = zalloc(n), *d_p = zalloc(n), *d_tot = zalloc(n);
@@ -421,6 +423,84 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     memcpy(d_p, d_t, (size_t)n * sizeof(double));
     memcpy(I_p, I_t, (size_t)n * sizeof(double));
 
+    /* Water quality: class-indexed C1..C4 and the exponential build-up of
+     * HydroPol2D_preprocessing.m:2170-2174, initial_mass of :2319 */
+    const int do_wq = cfg->flag_waterquality == 1;
+    double *B = NULL, *C3m = NULL, *C4m = NULL, *Pc = NULL, *TW = NULL, *ql = NULL, *qd = NULL;
+    double initial_mass = 0.0, mass_lost = 0.0, out_conc = 0.0, tmin_wq = 0.0;
+    double mass_outlet = 0.0, vol_outlet = 0.0;
+    int64_t wq_split = 0;
+    if (do_wq) {
+        B = zalloc(n); C3m = zalloc(n); C4m = zalloc(n); Pc = zalloc(n); TW = zalloc(n);
+        ql = zalloc(n); qd = zalloc(n);
+        for (int64_t c = 0; c < n; ++c) {
+            const int32_t cls = in->lulc_class ? in->lulc_class[c] : 0;
+            const double B0 = in->wq_C1[cls] * (1.0 - pow(2.718281828459045, -in->wq_C2[cls] * in->wq_ADD_days));
+            B[c] = B0 * A / 1e4 * (is_nan[c] ? 0.0 : 1.0);
+            C3m[c] = in->wq_C3[cls];
+            C4m[c] = in->wq_C4[cls];
+        }
+        for (int64_t c = 0; c < n; ++c) /* sum(sum(B_t(~isinf(B_t)))), column-major */
+            if (!isinf(B[c])) initial_mass += B[c];
+    }
+
+    /* Groundwater: Groundwater_Module (synchronous, Boussinesq baseflow) called
+     * every gw_every_steps accepted steps over the interval since its last call,
+     * before the next step's hydrology; per-cell van Genuchten parameters from
+     * the soil classes (initialize_groundwater_scheduler.m:38-66 state) */
+    const int do_gw = cfg->flag_groundwater == 1 && do_ga;
+    double *gw_h = NULL, *gw_sd = NULL, *gw_ts = NULL, *gw_tr = NULL, *gw_nv = NULL;
+    double *gw_al = NULL, *gw_ks = NULL, *gw_kg = NULL, *gw_Sy = NULL, *gw_q = NULL;
+    double *gw_pn = NULL, *gw_pr = NULL, *gw_pc = NULL, *gw_cum = NULL;
+    uint8_t *gw_dom = NULL;
+    oracle_gwm_params gp;
+    oracle_gwm_state gs;
+    memset(&gp, 0, sizeof(gp));
+    memset(&gs, 0, sizeof(gs));
+    int64_t gw_last = 0;
+    double gw_elapsed_min = 0.0, gw_err = 0.0;
+    if (do_gw) {
+        gw_h = zalloc(n); gw_sd = zalloc(n); gw_ts = zalloc(n); gw_tr = zalloc(n);
+        gw_nv = zalloc(n); gw_al = zalloc(n); gw_ks = zalloc(n); gw_kg = zalloc(n);
+        gw_Sy = zalloc(n); gw_q = zalloc(n); gw_pn = zalloc(n); gw_pr = zalloc(n);
+        gw_pc = zalloc(n); gw_cum = zalloc(n);
+        gw_dom = (uint8_t *)calloc((size_t)n, 1);
+        for (int64_t c = 0; c < n; ++c) {
+            const int32_t cls = in->soil_class ? in->soil_class[c] : 0;
+            gw_h[c] = in->gw_h0[c];
+            gw_sd[c] = in->gw_soil_depth[c];
+            gw_ts[c] = in->gw_theta_sat[cls];
+            gw_tr[c] = in->gw_theta_r[cls];
+            gw_nv[c] = in->gw_n_vg[cls];
+            gw_al[c] = in->gw_alpha_vg[cls];
+            gw_ks[c] = in->ksat_mmh[cls];
+            gw_kg[c] = in->gw_ksat_mmh;
+            gw_Sy[c] = in->gw_Sy;
+            gw_dom[c] = is_nan[c] ? 0 : 1;
+            const double p0 = is_nan[c] ? NAN : 0.0;
+            gw_pn[c] = p0;
+            gw_pr[c] = p0;
+            gw_pc[c] = p0;
+        }
+        gp.ny = ny;
+        gp.nx = nx;
+        gp.resolution = R;
+        gp.routing_time_min = cfg->routing_time_min;
+        gp.flag_baseflow = 1;
+        gp.flag_async = 0;
+        gp.target_dt_min = 1440.0;
+        gp.min_dt_min = 1.0;
+        gp.max_head_change_m = 0.25;
+        gp.courant = cfg->gw_courant;
+        gp.l_vg = in->gw_l_vg;
+        gp.elevation = z; gp.soil_depth = gw_sd; gp.theta_sat = gw_ts; gp.theta_r = gw_tr;
+        gp.n_vg = gw_nv; gp.alpha_vg = gw_al; gp.ksat_mmh = gw_ks; gp.ksat_gw_mmh = gw_kg;
+        gp.Sy = gw_Sy; gp.domain = gw_dom;
+        gs.pending_net = gw_pn;
+        gs.pending_recharge = gw_pr;
+        gs.pending_capillary = gw_pc;
+    }
+
     /* running_control */
     double min_dt0 = cfg->min_time_step_s, max_dt = cfg->max_time_step_s;
     double min_dt = min_dt0;
@@ -428,7 +508,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     double routing_time = cfg->routing_time_min;
     oracle_dt_params dp = {R, cfg->alfa_min, cfg->alfa_max, 0, 0, cfg->time_step_increments_s,
                            cfg->depth_tolerance_mm, cfg->flag_timestep,
-                           cfg->flag_adaptive_timestepping, 0, 0.0};
+                           cfg->flag_adaptive_timestepping, 0, 0.0, 0.0};
 
     /* Main_While.m:109-127 + preprocessing.m:2469,2615,2618 */
     int64_t k = 1;
@@ -465,7 +545,12 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             for (int64_t c = 0; c < n; ++c) tmp[c] = A * I_t[c] / 1000.0;
             S_UZ = nansum2(ny, nx, tmp);
         }
-        S_prev = 0.0 + S_p + S_UZ + 0.0;
+        double S_GW = 0.0;
+        if (do_gw) { /* Main_While.m:258 */
+            for (int64_t c = 0; c < n; ++c) tmp[c] = A * gw_Sy[c] * (gw_h[c] - (z[c] - gw_sd[c]));
+            S_GW = nansum2(ny, nx, tmp);
+        }
+        S_prev = 0.0 + S_p + S_UZ + S_GW;
         free(tmp);
     }
     int32_t catch_index = 1;
@@ -486,7 +571,21 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
 
     while (t <= (routing_time + min_dt / 60.0)) {
         if (out->max_steps > 0 && accepted >= out->max_steps) break;
-        int failed = 0;
+        int failed = 0, late = 0;
+        /* ------- groundwater update at the end of every gw_every_steps accepted steps ------- */
+        if (do_gw && cfg->gw_every_steps > 0 && accepted - gw_last >= cfg->gw_every_steps) {
+            int32_t ran = 0;
+            double dtg = 0.0;
+            gp.time_step_min = gw_elapsed_min;
+            gp.t_min = t;
+            int32_t rc = oracle_groundwater_module(&gp, &gs, gw_h, I_t, d_t, gw_cum, NULL, gw_q,
+                                                   &gw_err, &ran, &dtg);
+            if (rc != 0) return rc;
+            memcpy(d_p, d_t, (size_t)n * sizeof(double));
+            memcpy(I_p, I_t, (size_t)n * sizeof(double));
+            gw_last = accepted;
+            gw_elapsed_min = 0.0;
+        }
         /* ---------------- Hydrological_Model.m:15-55 ---------------- */
         /* interception_module.m:59-71, Snow_Module.m:97-99: d_t += Eff_Rainfall */
         double rain_vol_step = 0.0, inf_vol_step = 0.0;
@@ -547,10 +646,23 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
         /* ---------------- Local_Inertial_Model_D4 (:596-615) ---------------- */
         oracle_li_d4(ny, nx, cfg->flag_numerical_scheme, z, d_tot, nman, n2, A, time_step, R,
                      cfg->outlet_type, cfg->slope_outlet, oidx, n_out, outflow_prev,
-                     cfg->flag_critical, NULL, qr, qu, NULL, qof, d_t, NULL, outflow, Hf);
+                     cfg->flag_critical, ql, qr, qu, qd, qof, d_t, NULL, outflow, Hf);
         if (inject > 0 && k == inject) {
             failed = 1;
             inject = 0;
+        }
+        /* ---------------- build_up_wash_off (:683-695) ---------------- */
+        if (do_wq && !failed) {
+            int32_t rc = oracle_washoff(ny, nx, C3m, C4m, ql, qr, qu, qd, qof, B, time_step, A,
+                                        is_nan, cfg->flag_wq_model, &mass_lost, TW,
+                                        in->wq_Bmin, in->wq_Bmax, in->wq_min_Bt, Pc, &out_conc,
+                                        &tmin_wq, NULL);
+            if (rc < 0) return HP2D_E_ARG;
+            wq_split += rc;
+            double sB = 0.0; /* sum(sum(B_t(~isinf(B_t)))) > 1.2*initial_mass: error (:691-695) */
+            for (int64_t c = 0; c < n; ++c)
+                if (!isinf(B[c])) sB += B[c];
+            if (sB > 1.2 * initial_mass) failed = 1;
         }
         if (!failed) {
             /* ---------------- refreshing_timestep (:693-705) ---------------- */
@@ -561,7 +673,21 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             if (delta_time_save > 0 || k == 1) {
                 dp.max_time_step = max_dt;
                 dp.min_time_step = min_dt;
+                dp.flag_waterquality = do_wq;  /* refreshing_timestep.m:329-332 */
+                dp.tmin_wq = tmin_wq;
                 tcr = oracle_refresh_dt(ny, nx, qr, qu, Hf, &dp, &max_velocity, vraster);
+                if (out->dt_follow && n_dt < out->n_follow && out->dt_follow[n_dt] != tcr) {
+                    const double inc = cfg->time_step_increments_s;
+                    const double u = dp.nt_raw / inc;
+                    const int tie = fabs(u - nearbyint(u)) <= 1e-9 * fmax(1.0, fabs(u)) &&
+                                    fabs(out->dt_follow[n_dt] - tcr) <= inc * (1.0 + 1e-9);
+                    if (tie) {
+                        tcr = out->dt_follow[n_dt];
+                        out->n_dt_ties++;
+                    } else {
+                        out->n_dt_mismatch++;
+                    }
+                }
                 time_step = tcr / 60.0;
             } else {
                 tcr = time_step * 60.0;
@@ -578,6 +704,19 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                     if (isnan(avg)) avg = 0.0;
                     delta_p_agg = avg * dt_step / 60.0;
                 }
+            }
+            /* EMC (:724-731), with the time_step refreshing_timestep just set */
+            if (do_wq) {
+                double q_cols = 0.0, q_all = 0.0;
+                for (int32_t j = 0; j < nx; ++j) {
+                    double col = 0.0;
+                    for (int32_t i = 0; i < ny; ++i)
+                        if (!isnan(qof[IX(i, j)])) col += qof[IX(i, j)];
+                    q_cols += col / 1000.0 / 3600.0 * 1000.0;
+                    q_all += col;
+                }
+                mass_outlet = fmax(mass_outlet + out_conc * q_cols * (time_step * 60.0), 0.0);
+                vol_outlet = fmax(q_all / 1000.0 / 3600.0 * 1000.0 * (time_step * 60.0) + vol_outlet, 0.0);
             }
             /* :734-740 */
             if (k == 1) t_previous = tcr / 60.0;
@@ -620,6 +759,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             cum_rain += rain_vol_step;
             cum_inf += inf_vol_step;
             cum_out += qsum / 1000.0 / 3600.0 * (dt_used_min * 60.0);
+            gw_elapsed_min += dt_used_min;
             /* :766-774 */
             memcpy(d_p, d_t, (size_t)n * sizeof(double));
             memcpy(I_p, I_t, (size_t)n * sizeof(double));
@@ -659,8 +799,16 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                     for (int64_t c = 0; c < n; ++c) tmp[c] = area_nan[c] * I_t[c] / 1000.0;
                     S_UZ = nansum2(ny, nx, tmp);
                 }
+                double S_GW = 0.0;
+                if (do_gw) { /* mass_balance_check.m:66-70 */
+                    for (int64_t c = 0; c < n; ++c)
+                        tmp[c] = area_nan[c] * gw_Sy[c] * (gw_h[c] - (z[c] - gw_sd[c]));
+                    S_GW = nansum2(ny, nx, tmp);
+                    for (int64_t c = 0; c < n; ++c) tmp[c] = area_nan[c] * gw_pn[c];
+                    S_GW = S_GW + nansum2(ny, nx, tmp);
+                }
                 double fluxes = Pv + Qin - 0.0 - 0.0 - 0.0 - Qout - 0.0;
-                double dS = (0.0 + S_p + S_UZ + 0.0 + 0.0) - S_prev;
+                double dS = (0.0 + S_p + S_UZ + S_GW + 0.0) - S_prev;
                 S_prev = S_prev + dS;
                 volume_error = dS - fluxes;
                 if (out->volume_error_hist && n_reports < out->n_report_max)
@@ -668,7 +816,11 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                 n_reports++;
                 free(tmp);
             }
-        } else {
+            /* water-quality instability check at the end of the try block (:919-922):
+             * the error lands in the catch block after the step was accepted */
+            late = do_wq && (tmin_wq < 0 || isnan(tmin_wq) || isinf(tmin_wq));
+        }
+        if (failed || late) {
             /* catch block, Main_While.m:926-993 */
             rejected++;
             t = t - time_step;
@@ -716,6 +868,12 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     if (out->outflow) memcpy(out->outflow, outflow, (size_t)3 * n * sizeof(double));
     if (out->I_t && do_ga) memcpy(out->I_t, I_t, (size_t)n * sizeof(double));
     if (out->dmax) memcpy(out->dmax, dmax, (size_t)n * sizeof(double));
+    if (do_wq) {
+        if (out->B_t) memcpy(out->B_t, B, (size_t)n * sizeof(double));
+        if (out->P_conc) memcpy(out->P_conc, Pc, (size_t)n * sizeof(double));
+        if (out->Tot_Washed) memcpy(out->Tot_Washed, TW, (size_t)n * sizeof(double));
+    }
+    if (do_gw && out->gw_h) memcpy(out->gw_h, gw_h, (size_t)n * sizeof(double));
     out->n_hydro = n_hydro;
     hp2d_run_report *rp = &out->rep;
     rp->steps = accepted;
@@ -738,9 +896,35 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
         free(tmp);
     }
     rp->catch_index = catch_index;
+    if (do_wq) {
+        double sB = 0.0;
+        for (int64_t c = 0; c < n; ++c)
+            if (!isnan(B[c])) sB += B[c];
+        rp->wq_mass_kg = sB;
+        rp->wq_mass_lost_kg = mass_lost;
+        rp->wq_out_conc_mgl = out_conc;
+        rp->wq_tmin_s = tmin_wq;
+        rp->wq_mass_outlet_mg = mass_outlet;
+        rp->wq_vol_outlet = vol_outlet;
+        rp->wq_substep_steps = wq_split;
+    }
+    if (do_gw) {
+        double *tmp = zalloc(n);
+        for (int64_t c = 0; c < n; ++c) tmp[c] = area_nan[c] * gw_Sy[c] * (gw_h[c] - (z[c] - gw_sd[c]));
+        rp->gw_storage_m3 = nansum2(ny, nx, tmp);
+        free(tmp);
+        rp->gw_total_recharge_m3 = gs.total_recharge_m3;
+        rp->gw_total_exfiltration_m3 = gs.total_exfiltration_m3;
+        rp->gw_error_m3 = gw_err;
+        rp->gw_updates = gs.n_updates;
+    }
     free(z); free(nman); free(n2); free(d_t); free(d_p); free(d_tot);
     free(outflow); free(outflow_prev); free(Hf); free(qr); free(qu); free(qof); free(dmax);
     free(I_t); free(I_p); free(Tclk); free(vraster); free(area_nan); free(is_nan); free(oidx);
     free(ga);
+    free(B); free(C3m); free(C4m); free(Pc); free(TW); free(ql); free(qd);
+    free(gw_h); free(gw_sd); free(gw_ts); free(gw_tr); free(gw_nv); free(gw_al); free(gw_ks);
+    free(gw_kg); free(gw_Sy); free(gw_q); free(gw_pn); free(gw_pr); free(gw_pc); free(gw_cum);
+    free(gw_dom);
     return 0;
 }

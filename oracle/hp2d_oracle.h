@@ -41,13 +41,14 @@ typedef struct oracle_dt_params {
     double time_step_increments, depth_tolerance_mm;
     int32_t flag_timestep, flag_adaptive_timestepping, flag_waterquality;
     double tmin_wq;
+    double nt_raw;  /* out: the new time step before the snap to the increments */
 } oracle_dt_params;
 
 /* returns time_calculation_routing [s]; writes max_velocity and optionally
  * velocity_raster (ny*nx). */
 double oracle_refresh_dt(int32_t ny, int32_t nx, const double *qout_right,
                          const double *qout_up, const double *Hf,
-                         const oracle_dt_params *p, double *max_velocity,
+                         oracle_dt_params *p, double *max_velocity,
                          double *velocity_raster);
 
 double oracle_rain_avg(const double *times, const double *values, int32_t n, double t0,
@@ -73,6 +74,15 @@ typedef struct oracle_run_out {
     hp2d_run_report rep;
     double *volume_error_hist; /* n_report_max */
     int32_t n_report_max;
+    double *P_conc, *Tot_Washed;  /* ny*nx or NULL (water quality) */
+    /* Optional accepted-step dt sequence of another implementation.  Where the
+     * oracle's own pre-snap dt lies within 1e-9 (relative) of an increment
+     * boundary and the given dt is one increment away, the floor() of
+     * refreshing_timestep.m:344 is a floating-point tie and the given dt is
+     * adopted (n_dt_ties counts them); any other difference is counted in
+     * n_dt_mismatch and the oracle keeps its own dt. */
+    const double *dt_follow;
+    int64_t n_follow, n_dt_ties, n_dt_mismatch;
 } oracle_run_out;
 
 int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
@@ -87,6 +97,8 @@ int32_t oracle_ca_routing(int32_t ny, int32_t nx, const double *z, const double 
                           double *qout_right, double *qout_up, double *qout_down,
                           double *outlet_flow, double *d_t, double *I_tot_end_cell);
 
+/* returns 1 when the step was split internally, 0 when not, -1 when the split
+ * would exceed 1e7 sub-steps (tmin_wq = 0 loops for ever in the reference) */
 int32_t oracle_washoff(int32_t ny, int32_t nx, const double *C3, const double *C4,
                        const double *ql, const double *qr, const double *qu, const double *qd,
                        const double *qo, double *B_t, double time_step_min, double cell_area,

@@ -111,7 +111,8 @@ int32_t oracle_washoff(int32_t ny, int32_t nx, const double *C3, const double *C
     for (int64_t c = 0; c < n; ++c) Bmid[c] = B_t[c] - dW[c] * time_step_min / 60.0;
     for (int64_t c = 0; c < n; ++c) Wout5[c] = W[4 * n + c];
     double lost = 0.0;
-    if (nanmin_all(Bmid, n) < 0 || tmin < time_step_min * 60.0) { /* :80-135 */
+    const int32_t split = nanmin_all(Bmid, n) < 0 || tmin < time_step_min * 60.0;
+    if (split) { /* :80-135 */
         double n_dec = time_step_min * 60.0 / tmin;
         double n_floor = floor(n_dec);
         if (!(n_floor < 1e7)) { /* the reference would loop for ever (tmin_wq = 0) */
@@ -167,5 +168,5 @@ int32_t oracle_washoff(int32_t ny, int32_t nx, const double *C3, const double *C
     *Out_Conc = fmax(1000.0 * sum_wo / (sum_qo / 1000.0 * cell_area), 0.0);
     *tmin_wq = tmin;
     free(q5); free(W); free(totW); free(totQ); free(dW); free(Bx); free(Wout5); free(Bmid);
-    return 0;
+    return split;
 }

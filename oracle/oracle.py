@@ -34,7 +34,7 @@ class DtParams(C.Structure):
         ("max_time_step", C.c_double), ("min_time_step", C.c_double),
         ("time_step_increments", C.c_double), ("depth_tolerance_mm", C.c_double),
         ("flag_timestep", C.c_int32), ("flag_adaptive_timestepping", C.c_int32),
-        ("flag_waterquality", C.c_int32), ("tmin_wq", C.c_double),
+        ("flag_waterquality", C.c_int32), ("tmin_wq", C.c_double), ("nt_raw", C.c_double),
     ]
 
 
@@ -45,6 +45,9 @@ class RunOut(C.Structure):
         ("n_hydro_max", C.c_int32), ("n_hydro", C.c_int32), ("dt_hist", _dp),
         ("n_dt_max", C.c_int32), ("max_steps", C.c_int64), ("inject_fail_step", C.c_int64),
         ("rep", abi.RunReport), ("volume_error_hist", _dp), ("n_report_max", C.c_int32),
+        ("P_conc", _dp), ("Tot_Washed", _dp),
+        ("dt_follow", _dp), ("n_follow", C.c_int64), ("n_dt_ties", C.c_int64),
+        ("n_dt_mismatch", C.c_int64),
     ]
 
 
@@ -212,8 +215,12 @@ def rain_avg(times, values, t0, t1, cursor=1):
 
 
 def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: int = 200000,
-        n_report_max: int = 64):
-    """Full Main_While run on the oracle.  Returns a dict of final state + ledgers."""
+        n_report_max: int = 64, dt_follow=None):
+    """Full Main_While run on the oracle.  Returns a dict of final state + ledgers.
+
+    dt_follow: the accepted-step dt sequence [s] of another implementation; the
+    oracle adopts it only at floating-point ties of the dt snap (see
+    oracle_run_out) and reports n_dt_ties / n_dt_mismatch."""
     ny, nx = spec.shape
     cfg, ins, keep = spec.to_c()
     cfg.precision = abi.HP2D_FP64
@@ -223,6 +230,10 @@ def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: 
         "outflow": np.zeros((ny, nx, 3), order="F"),
         "I_t": np.zeros((ny, nx), order="F"),
         "dmax": np.zeros((ny, nx), order="F"),
+        "B_t": np.zeros((ny, nx), order="F"),
+        "P_conc": np.zeros((ny, nx), order="F"),
+        "Tot_Washed": np.zeros((ny, nx), order="F"),
+        "gw_h": np.zeros((ny, nx), order="F"),
         "hydro_q": np.zeros(n_h),
         "hydro_t": np.zeros(n_h),
         "hydro_depth": np.zeros(n_h),
@@ -231,6 +242,8 @@ def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: 
     }
     out = RunOut()
     out.d_t, out.outflow, out.I_t, out.dmax = (_p(res[k]) for k in ("d_t", "outflow", "I_t", "dmax"))
+    out.B_t, out.P_conc, out.Tot_Washed, out.gw_h = (_p(res[k]) for k in
+                                                     ("B_t", "P_conc", "Tot_Washed", "gw_h"))
     out.hydro_q, out.hydro_t, out.hydro_depth = (_p(res[k]) for k in
                                                  ("hydro_q", "hydro_t", "hydro_depth"))
     out.n_hydro_max = n_h
@@ -240,12 +253,18 @@ def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: 
     out.inject_fail_step = inject_fail_step
     out.volume_error_hist = _p(res["volume_error_hist"])
     out.n_report_max = n_report_max
+    if dt_follow is not None:
+        follow = np.ascontiguousarray(dt_follow, dtype=np.float64)
+        out.dt_follow = _p(follow)
+        out.n_follow = follow.size
     rc = lib().oracle_run(C.byref(cfg), C.byref(ins), C.byref(out))
     if rc != 0:
         raise RuntimeError(f"oracle_run failed with status {rc}")
     rep = out.rep.as_dict()
     res["report"] = rep
     res["n_hydro"] = out.n_hydro
+    res["n_dt_ties"] = out.n_dt_ties
+    res["n_dt_mismatch"] = out.n_dt_mismatch
     res["dt_hist"] = res["dt_hist"][: min(rep["steps"], n_dt_max)]
     res["volume_error_hist"] = res["volume_error_hist"][: min(rep["n_reports"], n_report_max)]
     del keep
@@ -274,7 +293,7 @@ def washoff(C3, C4, qout_left, qout_right, qout_up, qout_down, outlet_flow, B_t,
     rc = lib().oracle_washoff(ny, nx, _p(C3), _p(C4), *[_p(q) for q in qs], _p(B), time_step_min,
                               cell_area, nmp, flag_wq_model, C.byref(ml), _p(tw), Bmin, Bmax,
                               min_Bt, _p(pc), C.byref(oc), C.byref(tm), _p(tot))
-    if rc != 0:
+    if rc < 0:
         raise ValueError("washoff: tmin_wq gives more than 1e7 sub-steps")
     return {"B_t": B, "P_conc": pc, "Out_Conc": oc.value, "tmin_wq": tm.value, "tot_W_out": tot,
             "mass_lost": ml.value, "Tot_Washed": tw}
