@@ -4,9 +4,12 @@
 A "step" is one iteration of the device-resident Main_While loop on the
 4096 x 4096 full-physics workload (fractal DEM, 8 LULC classes, 5 Green-Ampt
 soil classes, Gaussian-random-field rainfall on a triangular hyetograph,
-critical-depth outlets on one edge): the fused rainfall + Green-Ampt +
-local-inertial kernel followed by the one-block dt/bookkeeping kernel.
-Inputs are resident in HBM before timing; steps run as captured hipGraphs.
+critical-depth outlets on one edge, one pollutant with build-up / wash-off,
+the groundwater module every 10 surface steps): the fused rainfall +
+Green-Ampt + local-inertial kernel, the wash-off passes, the one-block
+dt/bookkeeping kernel, and every 10th step the groundwater module (recharge
+bucket + Boussinesq).  Inputs are resident in HBM before timing; steps run as
+captured hipGraphs.
 
 Prints one JSON line (value = cell-updates/s of the whole job).
 """
@@ -32,7 +35,8 @@ def parse():
     p.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     p.add_argument("--graph-steps", type=int, default=50)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-steps", type=int, default=6)
+    p.add_argument("--cpu-steps", type=int, default=11,
+                   help="steps of the CPU oracle sample (11 includes one groundwater call)")
     p.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     p.add_argument("--ensemble-members", type=int, default=64,
                    help="members of the batched-ensemble measurement (0 skips it)")
@@ -192,8 +196,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"full-physics single domain {n}x{n} dx=2 m: fractal DEM, "
                                "8 LULC Manning classes, 5 Green-Ampt soil classes, GRF rainfall "
-                               "on a 2 h triangular hyetograph, critical-depth edge outlets; "
-                               "fused rain+GA+local-inertial step + dt control",
+                               "on a 2 h triangular hyetograph, critical-depth edge outlets, one "
+                               "pollutant (build-up / wash-off), groundwater module (Sy=0.2, "
+                               "K=1e-5 m/s) every 10 steps; fused rain+GA+local-inertial step, "
+                               "wash-off passes, dt control",
                    "cells": cells, "precision": args.precision,
                    "launch": {"cols_per_wave": tune["tj"], "waves_per_block": tune["waves_per_block"]},
                    "bytes_per_cell_update": rd + wr},

@@ -536,6 +536,7 @@ template <typename T> static WqSimArgs<T> sim_wq_args(hp2d_sim *s) {
     }
     a.C3 = (const T *)s->wqC3;
     a.C4 = (const T *)s->wqC4;
+    a.n_classes = c.n_lulc_classes;
     a.Bs = (T *)s->wqBs;
     a.omass = (T *)s->wqOm;
     a.Pc = (T *)s->wqPc;
@@ -971,7 +972,9 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         a.elev = arr(0); a.soil_depth = arr(1); a.ts = arr(2); a.tr = arr(3); a.nv = arr(4);
         a.alpha = arr(5); a.ksat = arr(6); a.ksat_gw = arr(7); a.Sy = arr(8);
         a.h = arr(9); a.I = arr(10); a.d = arr(11); a.cum = arr(12); a.maxd = nullptr;
-        a.pn = arr(14); a.pr = arr(15); a.pc = arr(16); a.z_bed = arr(17); a.Kgw = arr(18);
+        // the pending recharge / capillary maps are returned by the per-call entry
+        // point but read by nothing in the loop: only the net exchange map is kept
+        a.pn = arr(14); a.pr = nullptr; a.pc = nullptr; a.z_bed = arr(17); a.Kgw = arr(18);
         a.rfg = arr(19); a.q = arr(20); a.t1 = arr(21); a.t2 = arr(22); a.t3 = arr(23);
         hp2d_gw_params &p = s->gwp;
         p.ny = ny;

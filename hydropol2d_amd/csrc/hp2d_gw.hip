@@ -18,6 +18,7 @@
 // The closing mass-balance sums are per-column FP64 sums in row order,
 // finished on the host in column order: the same association as the
 // reference's nansum(nansum(.)).
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -25,9 +26,22 @@
 #include "hp2d.h"
 #include "hp2d_handle.hpp"
 #include "hp2d_internal.hpp"
+#include "hp2d_colred.hpp"
 #include "hp2d_gw.hpp"
 
 namespace hp2d {
+
+// (row, column) of a linear cell index; 32-bit arithmetic below 2^32 cells
+__device__ __forceinline__ void gw_rc(long long c, int ny, int &i, int &j) {
+    if (c < 0x100000000LL) {
+        const unsigned u = (unsigned)c;
+        i = (int)(u % (unsigned)ny);
+        j = (int)(u / (unsigned)ny);
+    } else {
+        i = (int)(c % ny);
+        j = (int)(c / ny);
+    }
+}
 
 __device__ __forceinline__ double signum(double F) {
     return isnan(F) ? NAN : (double)((F > 0.0) - (F < 0.0));
@@ -63,6 +77,9 @@ __global__ void gw_ctl_end(GwCtl *ctl) {
     const int kk = 2;  // reset to 1 at the top of every pass, then incremented
     if ((double)kk == ctl->n_steps - 1.0) ctl->dt = ctl->T - kk * ctl->dt;
     ctl->t = ctl->t + ctl->dt;
+    // the loop test of the next pass, taken now so that the host sees the end
+    // without launching a pass that would only find it
+    if (!(ctl->t < ctl->T)) ctl->done = 1;
 }
 
 // compute_fluxes_conservative; with avg, F := 0.5*(F_prev + F)
@@ -72,7 +89,8 @@ __global__ __launch_bounds__(kGwBlock) void gw_flux_kernel(GwArgs a, const doubl
     const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
     if (c >= a.n) return;
     const int ny = a.ny;
-    const int i = (int)(c % ny), j = (int)(c / ny);
+    int i, j;
+    gw_rc(c, ny, i, j);
     const double dt = a.ctl->dt_used;
     const double H = fmax(h[c] - a.z0[c], 0.0);
     double fx = 0.0, fy = 0.0;
@@ -134,7 +152,8 @@ __global__ __launch_bounds__(kGwBlock) void gw_predict_kernel(GwArgs a, const do
     if (a.ctl->done) return;
     const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
     if (c >= a.n) return;
-    const int i = (int)(c % a.ny), j = (int)(c / a.ny);
+    int i, j;
+    gw_rc(c, a.ny, i, j);
     const double dt = a.ctl->dt_used;
     double v = h_t[c] + dt / a.Sy[c] * (-gw_div(a, Fx, Fy, c, i, j));
     if (a.dirichlet && a.dirichlet[c]) v = a.h_dir[c];
@@ -148,7 +167,8 @@ __global__ __launch_bounds__(kGwBlock) void gw_update_kernel(GwArgs a, const dou
     if (a.ctl->done) return;
     const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
     if (c >= a.n) return;
-    const int i = (int)(c % a.ny), j = (int)(c / a.ny);
+    int i, j;
+    gw_rc(c, a.ny, i, j);
     const double dt = a.ctl->dt_used;
     double v = h_t[c] + dt / a.Sy[c] * (-gw_div(a, Fx, Fy, c, i, j) + a.R[c]);
     if (a.dirichlet && a.dirichlet[c]) v = a.h_dir[c];
@@ -167,11 +187,13 @@ __global__ __launch_bounds__(kGwBlock) void gw_velocity_kernel(GwArgs a, const d
                                                                double *h_masked, int mask_on,
                                                                int check_done) {
     if (check_done && a.ctl->done) return;
-    const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
     double best = INFINITY;
-    if (c < a.n) {
-        const int ny = a.ny;
-        const int i = (int)(c % ny), j = (int)(c / ny);
+    const int ny = a.ny;
+    // grid-stride over the cells: one atomic per workgroup of a bounded grid
+    for (long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x; c < a.n;
+         c += (long long)gridDim.x * kGwBlock) {
+        int i, j;
+        gw_rc(c, ny, i, j);
         double vx = NAN, vy = NAN;
         if (j < a.nx - 1) vx = -(0.5 * (a.K[c] + a.K[c + ny])) * ((h[c + ny] - h[c]) / a.dx);
         if (i < ny - 1) vy = -(0.5 * (a.K[c] + a.K[c + 1])) * ((h[c + 1] - h[c]) / a.dy);
@@ -200,28 +222,17 @@ __global__ __launch_bounds__(kGwBlock) void gw_velocity_kernel(GwArgs a, const d
 }
 
 // per-column nansum of R*dt, q_exf*dt, Sy*max(h0 - z0, 0), Sy*max(h - z0, 0)
-__global__ __launch_bounds__(kGwBlock) void gw_colsum_kernel(GwArgs a, const double *q_exf,
-                                                             const double *h0, const double *h,
-                                                             double *out) {
-    const int j = blockIdx.x * kGwBlock + threadIdx.x;
-    if (j >= a.nx) return;
+__global__ __launch_bounds__(kColThreads) void gw_colsum_kernel(GwArgs a, const double *q_exf,
+                                                                const double *h0, const double *h,
+                                                                double *out) {
     const double dt = a.ctl->dt_used;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    for (int i = 0; i < a.ny; ++i) {
-        const long long c = (long long)j * a.ny + i;
-        const double r = a.R[c] * dt;
-        const double q = q_exf[c] * dt;
-        const double p = a.Sy[c] * fmax(h0[c] - a.z0[c], 0.0);
-        const double u = a.Sy[c] * fmax(h[c] - a.z0[c], 0.0);
-        if (!isnan(r)) s0 += r;
-        if (!isnan(q)) s1 += q;
-        if (!isnan(p)) s2 += p;
-        if (!isnan(u)) s3 += u;
-    }
-    out[4 * j + 0] = s0;
-    out[4 * j + 1] = s1;
-    out[4 * j + 2] = s2;
-    out[4 * j + 3] = s3;
+    const bool is_max[4] = {false, false, false, false};
+    column_reduce<4>(a.ny, blockIdx.x, is_max, [&](long long c, double (&v)[4]) {
+        v[0] = a.R[c] * dt;
+        v[1] = q_exf[c] * dt;
+        v[2] = a.Sy[c] * fmax(h0[c] - a.z0[c], 0.0);
+        v[3] = a.Sy[c] * fmax(h[c] - a.z0[c], 0.0);
+    }, out);
 }
 
 hp2d_status gw_run(hp2d_handle *h, const GwArgs &a_in, double dt, const double *dh0,
@@ -229,16 +240,15 @@ hp2d_status gw_run(hp2d_handle *h, const GwArgs &a_in, double dt, const double *
     const int ny = a_in.ny, nx = a_in.nx;
     const size_t n = (size_t)ny * nx, nb = n * sizeof(double);
     const int blocks = (int)((n + kGwBlock - 1) / kGwBlock);
-    const int cblocks = (nx + kGwBlock - 1) / kGwBlock;
     hipStream_t s = hp2d_handle_stream(h);
     GwArgs a = a_in;
     a.ctl = w.ctl;
     hipError_t e = hipMemcpyAsync(w.ht, dh0, nb, hipMemcpyDeviceToDevice, s);
-    const dim3 g(blocks), b(kGwBlock);
+    const dim3 g(blocks), b(kGwBlock), gv(std::min(blocks, 2048));
     if (e == hipSuccess) {
         hipLaunchKernelGGL(gw_ctl_init, dim3(1), dim3(1), 0, s, w.ctl, dt);
         // velocities of the entry head (not masked, Boussinesq_2D_explicit.m:91)
-        hipLaunchKernelGGL(gw_velocity_kernel, g, b, 0, s, a, (const double *)w.ht, w.ux, w.uy,
+        hipLaunchKernelGGL(gw_velocity_kernel, gv, b, 0, s, a, (const double *)w.ht, w.ux, w.uy,
                            (double *)nullptr, 0, 0);
         e = hipGetLastError();
     }
@@ -254,7 +264,7 @@ hp2d_status gw_run(hp2d_handle *h, const GwArgs &a_in, double dt, const double *
             hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)w.hp, w.Fx, w.Fy, 1);
             hipLaunchKernelGGL(gw_update_kernel, g, b, 0, s, a, (const double *)w.ht,
                                (const double *)w.Fx, (const double *)w.Fy, w.tmp, w.q);
-            hipLaunchKernelGGL(gw_velocity_kernel, g, b, 0, s, a, (const double *)w.tmp, w.ux, w.uy,
+            hipLaunchKernelGGL(gw_velocity_kernel, gv, b, 0, s, a, (const double *)w.tmp, w.ux, w.uy,
                                w.ht, 1, 1);
             hipLaunchKernelGGL(gw_ctl_end, dim3(1), dim3(1), 0, s, w.ctl);
         }
@@ -266,7 +276,7 @@ hp2d_status gw_run(hp2d_handle *h, const GwArgs &a_in, double dt, const double *
         batch = batch < 64 ? batch * 2 : 64;
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(gw_colsum_kernel, dim3(cblocks), b, 0, s, a, (const double *)w.q, dh0,
+        hipLaunchKernelGGL(gw_colsum_kernel, dim3(nx), dim3(kColThreads), 0, s, a, (const double *)w.q, dh0,
                            (const double *)w.ht, w.col);
         e = hipGetLastError();
     }

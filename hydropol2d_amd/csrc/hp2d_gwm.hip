@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "hp2d.h"
+#include "hp2d_colred.hpp"
 #include "hp2d_gw.hpp"
 #include "hp2d_handle.hpp"
 
@@ -56,7 +57,8 @@ __global__ __launch_bounds__(256) void gwm_recharge_kernel(GwmArgs a) {
     double hs = -(1.0 / al) * pow(pow(Se, -1.0 / m) - 1.0, 1.0 / nvg);
     if (Se >= 0.999999) hs = 0.0;
     const double term = 1.0 - pow(Se, 1.0 / m);
-    double Kr = pow(Se, a.l_vg) * pow(1.0 - pow(term, m), 2.0);
+    const double y = 1.0 - pow(term, m);
+    double Kr = pow(Se, a.l_vg) * (y * y);  // (.)^2: y*y is the correctly rounded square
     Kr = fmin(fmax(Kr, 0.0), 1.0);
     double grad = 1.0 + hs / Lgw;
     grad = fmax(grad, -1.0);
@@ -77,8 +79,8 @@ __global__ __launch_bounds__(256) void gwm_recharge_kernel(GwmArgs a) {
     if (nan) rr = NAN;
     const double down = fmax(R_mb, 0.0) * a.dt_s;
     a.pn[c] = a.pn[c] + rr * a.dt_s;
-    a.pr[c] = a.pr[c] + down;
-    a.pc[c] = a.pc[c] + fmax(zero, 0.0) * a.dt_s;
+    if (a.pr) a.pr[c] = a.pr[c] + down;
+    if (a.pc) a.pc[c] = a.pc[c] + fmax(zero, 0.0) * a.dt_s;
     a.t1[c] = down * (nan ? NAN : a.A);
     // scheduler terms: pending head change and diffusivity K H / Sy
     const double Sy_head = fmax(a.Sy[c], 1e-6);
@@ -94,21 +96,15 @@ __global__ __launch_bounds__(256) void gwm_recharge_kernel(GwmArgs a) {
 }
 
 // per column: [0] nansum(t1), [1] max(t2) omitting NaN, [2] max(t3) omitting NaN
-__global__ __launch_bounds__(256) void gwm_colred_kernel(int ny, int nx, const double *t1,
-                                                         const double *t2, const double *t3,
-                                                         double *out) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= nx) return;
-    double s = 0.0, m2 = NAN, m3 = NAN;
-    for (int i = 0; i < ny; ++i) {
-        const long long c = (long long)j * ny + i;
-        if (!isnan(t1[c])) s += t1[c];
-        if (t2) m2 = fmax(m2, t2[c]);
-        if (t3) m3 = fmax(m3, t3[c]);
-    }
-    out[3 * j] = s;
-    out[3 * j + 1] = m2;
-    out[3 * j + 2] = m3;
+__global__ __launch_bounds__(kColThreads) void gwm_colred_kernel(int ny, int nx, const double *t1,
+                                                                 const double *t2, const double *t3,
+                                                                 double *out) {
+    const bool is_max[3] = {false, true, true};
+    column_reduce<3>(ny, blockIdx.x, is_max, [&](long long c, double (&v)[3]) {
+        v[0] = t1[c];
+        v[1] = t2 ? t2[c] : NAN;
+        v[2] = t3 ? t3[c] : NAN;
+    }, out);
 }
 
 // recharge_for_gw (:199-203) and q_exf = zero_matrix
@@ -155,8 +151,8 @@ __global__ __launch_bounds__(256) void gwm_post_kernel(GwmArgs a, double dt_gw) 
     const bool nan = isnan(a.elev[c]);
     a.t1[c] = fmax(a.q[c], 0.0) * dt_gw * (nan ? NAN : a.A);
     a.pn[c] = 0.0;
-    a.pr[c] = 0.0;
-    a.pc[c] = 0.0;
+    if (a.pr) a.pr[c] = 0.0;
+    if (a.pc) a.pc[c] = 0.0;
     double zwt = a.elev[c] - a.h[c];
     zwt = fmax(zwt, 0.0);
     zwt = fmin(zwt, a.soil_depth[c]);
@@ -180,21 +176,14 @@ __global__ __launch_bounds__(256) void gwm_maxdepth_kernel(GwmArgs a) {
     a.maxd[c] = fmax(a.maxd[c], a.h[c] - (a.elev[c] - a.soil_depth[c]));
 }
 
-// per column: sum of A*Sy*(h - z_bed) + A*pending_net (mass_balance_check.m:66-70)
-__global__ __launch_bounds__(256) void gwm_storage_kernel(GwmArgs a, double *out) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= a.nx) return;
-    double s = 0.0, sp = 0.0;
-    for (int i = 0; i < a.ny; ++i) {
-        const long long c = (long long)j * a.ny + i;
+// per column: sum of A*Sy*(h - z_bed) and of A*pending_net (mass_balance_check.m:66-70)
+__global__ __launch_bounds__(kColThreads) void gwm_storage_kernel(GwmArgs a, double *out) {
+    const bool is_max[2] = {false, false};
+    column_reduce<2>(a.ny, blockIdx.x, is_max, [&](long long c, double (&v)[2]) {
         const double area = isnan(a.elev[c]) ? NAN : a.A;
-        const double v = area * a.Sy[c] * (a.h[c] - (a.elev[c] - a.soil_depth[c]));
-        if (!isnan(v)) s += v;
-        const double w = area * a.pn[c];
-        if (!isnan(w)) sp += w;
-    }
-    out[2 * j] = s;
-    out[2 * j + 1] = sp;
+        v[0] = area * a.Sy[c] * (a.h[c] - (a.elev[c] - a.soil_depth[c]));
+        v[1] = area * a.pn[c];
+    }, out);
 }
 
 hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, GwmArgs a,
@@ -209,7 +198,7 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
     auto D2H = [&](void *dst, const void *src, size_t bytes) {
         if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
     };
-    const dim3 G((unsigned)((n + 255) / 256)), B(256), CB((unsigned)((nx + 255) / 256));
+    const dim3 G((unsigned)((n + 255) / 256)), B(256);
     std::vector<double> hc(3 * (size_t)nx);
     auto fold = [&](double &sum, double &m2, double &m3) {
         sum = 0.0;
@@ -222,7 +211,7 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
         }
     };
     hipLaunchKernelGGL(gwm_recharge_kernel, G, B, 0, s, a);
-    hipLaunchKernelGGL(gwm_colred_kernel, CB, B, 0, s, ny, nx, (const double *)a.t1,
+    hipLaunchKernelGGL(gwm_colred_kernel, dim3(nx), dim3(kColThreads), 0, s, ny, nx, (const double *)a.t1,
                        (const double *)a.t2, (const double *)a.t3, col);
     e = hipGetLastError();
     D2H(hc.data(), col, hc.size() * 8);
@@ -288,7 +277,7 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
         }
         if (e == hipSuccess) {
             hipLaunchKernelGGL(gwm_post_kernel, G, B, 0, s, a, dt_gw);
-            hipLaunchKernelGGL(gwm_colred_kernel, CB, B, 0, s, ny, nx, (const double *)a.t1,
+            hipLaunchKernelGGL(gwm_colred_kernel, dim3(nx), dim3(kColThreads), 0, s, ny, nx, (const double *)a.t1,
                                (const double *)a.t2, (const double *)nullptr, col);
             e = hipGetLastError();
         }
@@ -316,7 +305,7 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
         e = hipGetLastError();
     }
     if (s_gw && e == hipSuccess) {
-        hipLaunchKernelGGL(gwm_storage_kernel, CB, B, 0, s, a, col);
+        hipLaunchKernelGGL(gwm_storage_kernel, dim3(nx), dim3(kColThreads), 0, s, a, col);
         e = hipGetLastError();
         D2H(hc.data(), col, 2 * (size_t)nx * 8);
     }

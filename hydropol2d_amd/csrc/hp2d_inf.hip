@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "hp2d.h"
+#include "hp2d_colred.hpp"
 #include "hp2d_handle.hpp"
 
 namespace hp2d {
@@ -100,22 +101,15 @@ __global__ __launch_bounds__(256) void infiltration_kernel(InfArgs a) {
 
 // per column j: [0] nansum(A*I/1000), [1] nansum(A*d/1000) of the given
 // arrays; with v != NULL also [2] nansum(v)
-__global__ __launch_bounds__(256) void inf_colsum_kernel(int ny, int nx, double A, const double *I,
-                                                         const double *d, const double *v,
-                                                         double *out) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= nx) return;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int i = 0; i < ny; ++i) {
-        const long long c = (long long)j * ny + i;
-        const double u = A * I[c] / 1000.0, w = A * d[c] / 1000.0;
-        if (!isnan(u)) s0 += u;
-        if (!isnan(w)) s1 += w;
-        if (v && !isnan(v[c])) s2 += v[c];
-    }
-    out[3 * j] = s0;
-    out[3 * j + 1] = s1;
-    out[3 * j + 2] = s2;
+__global__ __launch_bounds__(kColThreads) void inf_colsum_kernel(int ny, int nx, double A,
+                                                                 const double *I, const double *d,
+                                                                 const double *v, double *out) {
+    const bool is_max[3] = {false, false, false};
+    column_reduce<3>(ny, blockIdx.x, is_max, [&](long long c, double (&x)[3]) {
+        x[0] = A * I[c] / 1000.0;
+        x[1] = A * d[c] / 1000.0;
+        x[2] = v ? v[c] : 0.0;
+    }, out);
 }
 
 }  // namespace hp2d
@@ -181,12 +175,11 @@ extern "C" hp2d_status hp2d_infiltration_module(
     H2D(dd, d_t, nb); H2D(dI, I_t, nb);
     if (cum_inf) H2D(dcum, cum_inf, nb);
     if (dimp) H2D(dimp, idx_imp, n);
-    const int cb = (nx + 255) / 256;
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(inf_colsum_kernel, dim3(cb), dim3(256), 0, s, ny, nx, a.A,
+        hipLaunchKernelGGL(inf_colsum_kernel, dim3(nx), dim3(kColThreads), 0, s, ny, nx, a.A,
                            (const double *)dI, (const double *)dd, (const double *)nullptr, col0);
         hipLaunchKernelGGL(infiltration_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(inf_colsum_kernel, dim3(cb), dim3(256), 0, s, ny, nx, a.A,
+        hipLaunchKernelGGL(inf_colsum_kernel, dim3(nx), dim3(kColThreads), 0, s, ny, nx, a.A,
                            (const double *)dI, (const double *)dd, (const double *)dinf, col1);
         e = hipGetLastError();
     }

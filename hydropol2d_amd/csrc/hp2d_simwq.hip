@@ -27,9 +27,40 @@ namespace hp2d {
 constexpr int kWqThreads = 256;
 constexpr int kWqFoldThreads = 1024;
 
-__device__ __forceinline__ double wq_pow(double x, double c) { return pow(x, c); }
+// pow(x, c) for x >= 0 or NaN: the zero base (a face without flow in the
+// page's direction, the common case) costs no transcendental, and gives
+// exactly what pow gives: +0 for c > 0, 1 for c == 0, +inf for c < 0
+template <typename T> __device__ __forceinline__ T wq_pow0(T c) {
+    return isnan(c) ? c : (c > T(0) ? T(0) : (c == T(0) ? T(1) : T(INFINITY)));
+}
+__device__ __forceinline__ double wq_pow(double x, double c) { return x == 0.0 ? wq_pow0(c) : pow(x, c); }
 __device__ __forceinline__ float wq_pow(float x, float c) {
-    return c == 0.0f ? 1.0f : exp2f(c * log2f(x));  // pow(x, 0) = 1 as for every x
+    if (x == 0.0f) return wq_pow0(c);
+    if (c == 0.0f) return 1.0f;  // pow(x, 0) = 1 for every x
+    return __builtin_amdgcn_exp2f(c * __builtin_amdgcn_logf(x));
+}
+
+// FP64 keeps the reference's operation order; FP32 multiplies by hardware
+// reciprocals (the precision policy of the FP32 step kernel)
+__device__ __forceinline__ double wq_div(double x, double y) { return x / y; }
+__device__ __forceinline__ float wq_div(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+__device__ __forceinline__ double wq_gm2(double B, double A) { return B * 1000.0 / A; }  // [g/m2]
+__device__ __forceinline__ float wq_gm2(float B, float A) { return B * (1000.0f * __builtin_amdgcn_rcpf(A)); }
+__device__ __forceinline__ double wq_qA(double q, double A) { return q / 1000.0 / 3600.0 * A; }  // [m3/s]
+__device__ __forceinline__ float wq_qA(float q, float A) { return q * (A * 2.77777778e-7f); }
+
+// class tables of a launch, staged in LDS
+template <typename T> struct WqLds {
+    T C3[256], C4[256];
+};
+
+template <typename T>
+__device__ __forceinline__ void wq_stage_classes(const WqSimArgs<T> &a, WqLds<T> &lds) {
+    for (int k = threadIdx.x; k < 256; k += kWqThreads) {
+        lds.C3[k] = a.C3[k < a.n_classes ? k : 0];
+        lds.C4[k] = a.C4[k < a.n_classes ? k : 0];
+    }
+    __syncthreads();
 }
 
 // wash-off parameters of one cell for the current pass
@@ -39,26 +70,26 @@ template <typename T> struct WqCell {
 };
 
 template <typename T>
-__device__ __forceinline__ WqCell<T> wq_cell(const WqSimArgs<T> &a, long long c, const T *Bin,
-                                             const T *Bx_in, bool first, T *Bx_out) {
+__device__ __forceinline__ WqCell<T> wq_cell(const WqSimArgs<T> &a, const WqLds<T> &lds, long long c,
+                                             const T *Bin, const T *Bx_in, bool first, T *Bx_out) {
     WqCell<T> w;
     const unsigned fl = a.cell[c];
     const int cls = (int)(fl & 0xffu);
     const T B = Bin[c];
     w.B = B;
-    w.C3 = a.C3[cls];
-    w.C4 = a.C4[cls];
+    w.C3 = lds.C3[cls];
+    w.C4 = lds.C4[cls];
     T f = B;
     if (a.model == 1) {  // rating curve (:23-28, :89-94)
         const T Br = T(a.Bmin / 1000.0 * a.A), Bm = T(a.Bmax / 1000.0 * a.A);
         T bx = first ? B : Bx_in[c];
-        if (B * T(1000) / T(a.A) <= T(a.Bmin)) bx = T(0);
-        if (B * T(1000) / T(a.A) >= T(a.Bmax)) bx = Bm;
+        if (wq_gm2(B, T(a.A)) <= T(a.Bmin)) bx = T(0);
+        if (wq_gm2(B, T(a.A)) >= T(a.Bmax)) bx = Bm;
         if (Bx_out) Bx_out[c] = bx;
         f = T(1) + dmax(bx - Br, T(0));
     }
     w.f = f;
-    const bool low = (B * T(1000) / T(a.A) < T(a.min_Bt)) || isnan(B) || isinf(B);
+    const bool low = (wq_gm2(B, T(a.A)) < T(a.min_Bt)) || isnan(B) || isinf(B);
     w.zero = low || (first && (fl & kWqNanBit));
     return w;
 }
@@ -67,8 +98,18 @@ __device__ __forceinline__ WqCell<T> wq_cell(const WqSimArgs<T> &a, long long c,
 template <typename T>
 __device__ __forceinline__ T wq_rate(const WqCell<T> &w, T q, T A) {
     const T qq = q < T(0) ? T(0) : q;  // NaN stays NaN
-    const T r = w.C3 * wq_pow(qq / T(1000) / T(3600) * A, w.C4) * w.f;
+    const T r = w.C3 * wq_pow(wq_qA(qq, A), w.C4) * w.f;
     return w.zero ? T(0) : r;
+}
+
+// a neighbour's page towards the cell: its parameters are only read when the
+// face carries flow in that direction (or C4 = 0 makes a zero flux count)
+template <typename T>
+__device__ __forceinline__ T wq_inflow(const WqSimArgs<T> &a, const WqLds<T> &lds, long long nb,
+                                       T q, T A, const T *Bin, const T *Bx_in, bool first) {
+    const T qq = q < T(0) ? T(0) : q;
+    if (qq == T(0) && wq_pow0(lds.C4[a.cell[nb] & 0xffu]) == T(0)) return T(0);
+    return wq_rate(wq_cell(a, lds, nb, Bin, Bx_in, first, (T *)nullptr), q, A);
 }
 
 template <typename T>
@@ -111,6 +152,41 @@ __device__ __forceinline__ void block_store(double (&v)[NF], const bool (&is_min
     }
 }
 
+// rates of one cell against its current B: the five outflow pages, their
+// sum, the signed flux sum and the net rate dW (:24-59, :89-120)
+template <typename T> struct WqRates {
+    T W4, totW, totQ, dW;
+    WqCell<T> own;
+};
+
+template <typename T>
+__device__ __forceinline__ WqRates<T> wq_rates(const WqSimArgs<T> &a, const WqLds<T> &lds, int i,
+                                               int j, long long c, const T *qx, const T *qy,
+                                               const T *Bin, const T *Bx_in, T *Bx_out, bool first) {
+    const int ny = a.ny, nx = a.nx;
+    const T A = T(a.A);
+    WqRates<T> r;
+    const WqCell<T> w = wq_cell(a, lds, c, Bin, Bx_in, first, Bx_out);
+    r.own = w;
+    const T qr = qx[c], qu = qy[c];
+    const T ql = j > 0 ? -qx[c - ny] : T(0);
+    const T qd = i < ny - 1 ? -qy[c + 1] : T(0);
+    const T qo = outlet_q(a, c);
+    const T W0 = wq_rate(w, ql, A), W1 = wq_rate(w, qr, A), W2 = wq_rate(w, qu, A);
+    const T W3 = wq_rate(w, qd, A), W4 = wq_rate(w, qo, A);
+    r.W4 = W4;
+    r.totW = W0 + W1 + W2 + W3 + W4;
+    r.totQ = (a.cell[c] & kWqNanBit) ? T(0) : ql + qr + qu + qd + qo;
+    // the neighbours' pages towards this cell (:51-55)
+    T in_l = T(0), in_r = T(0), in_u = T(0), in_d = T(0);
+    if (j > 0) in_l = wq_inflow(a, lds, c - ny, -ql, A, Bin, Bx_in, first);
+    if (j < nx - 1) in_r = wq_inflow(a, lds, c + ny, -qr, A, Bin, Bx_in, first);
+    if (i > 0) in_u = wq_inflow(a, lds, c - 1, -qu, A, Bin, Bx_in, first);
+    if (i < ny - 1) in_d = wq_inflow(a, lds, c + 1, -qd, A, Bin, Bx_in, first);
+    r.dW = r.totW - (in_l + in_r + in_u + in_d);
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // first pass (:13-79): rates of the entry B_t, dW, B_t_mid, tmin candidates;
 // writes the unsplit outcome (B_t, Tot_Washed, P_conc), overwritten by the
@@ -120,6 +196,8 @@ template <typename T>
 __global__ __launch_bounds__(kWqThreads) void wq_first_kernel(WqSimArgs<T> a) {
     const SimState *st = a.st;
     if (wq_skip(st, false)) return;
+    __shared__ WqLds<T> lds;
+    wq_stage_classes(a, lds);
     const int qp = (int)(st->k & 1);          // fluxes just written by the step
     const int bp = (int)(st->wq_n & 1);       // entry B_t of this call
     const T *qx = a.qx_buf[qp], *qy = a.qy_buf[qp];
@@ -129,46 +207,35 @@ __global__ __launch_bounds__(kWqThreads) void wq_first_kernel(WqSimArgs<T> a) {
     T *TWout = a.TW[bp ^ 1];
     const double dt_min = st->time_step_min;
     const T A = T(a.A);
-    const int ny = a.ny, nx = a.nx;
     double v[6] = {NAN, NAN, NAN, NAN, 0.0, 0.0};
     const bool is_min[6] = {true, true, true, true, false, false};
-    for (long long c = blockIdx.x * (long long)kWqThreads + threadIdx.x; c < a.n;
-         c += (long long)gridDim.x * kWqThreads) {
-        const int i = (int)(c % ny), j = (int)(c / ny);
-        const WqCell<T> w = wq_cell(a, c, Bin, (const T *)nullptr, true,
-                                    a.model == 1 ? a.Bx[0] : (T *)nullptr);
-        const T qr = qx[c], qu = qy[c];
-        const T ql = j > 0 ? -qx[c - ny] : T(0);
-        const T qd = i < ny - 1 ? -qy[c + 1] : T(0);
-        const T qo = outlet_q(a, c);
-        const T W0 = wq_rate(w, ql, A), W1 = wq_rate(w, qr, A), W2 = wq_rate(w, qu, A);
-        const T W3 = wq_rate(w, qd, A), W4 = wq_rate(w, qo, A);
-        const T totW = W0 + W1 + W2 + W3 + W4;
-        const T totQ = (a.cell[c] & kWqNanBit) ? T(0) : ql + qr + qu + qd + qo;
-        // the neighbours' pages towards this cell (:51-55)
-        T in_l = T(0), in_r = T(0), in_u = T(0), in_d = T(0);
-        if (j > 0) in_l = wq_rate(wq_cell(a, c - ny, Bin, (const T *)nullptr, true, (T *)nullptr), qx[c - ny], A);
-        if (j < nx - 1) in_r = wq_rate(wq_cell(a, c + ny, Bin, (const T *)nullptr, true, (T *)nullptr), -qr, A);
-        if (i > 0) in_u = wq_rate(wq_cell(a, c - 1, Bin, (const T *)nullptr, true, (T *)nullptr), -qu, A);
-        if (i < ny - 1) in_d = wq_rate(wq_cell(a, c + 1, Bin, (const T *)nullptr, true, (T *)nullptr), qy[c + 1], A);
-        const T dW = totW - (in_l + in_r + in_u + in_d);
-        const T Bmid = w.B - dW * T(dt_min) / T(60);
+    // tiles of 256 rows of one column: tile t is (row block t % n_rb, column t / n_rb)
+    const int n_rb = (a.ny + kWqThreads - 1) / kWqThreads;
+    const long long n_tiles = (long long)n_rb * a.nx;
+    for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int j = (int)(t / n_rb), i = (int)(t % n_rb) * kWqThreads + (int)threadIdx.x;
+        if (i >= a.ny) continue;
+        const long long c = (long long)j * a.ny + i;
+        const WqRates<T> r = wq_rates(a, lds, i, j, c, qx, qy, Bin, (const T *)nullptr,
+                                      a.model == 1 ? a.Bx[0] : (T *)nullptr, true);
+        const T B = r.own.B, dW = r.dW, totW = r.totW;
+        const T Bmid = B - wq_div(dW * T(dt_min), T(60));
         // tmin candidates (:61-76, :144-159): m_calc / (|min(dW,0)| + 1e-16)
         const T d0 = dW > T(0) ? T(0) : dW;
-        const T mc0 = (w.B * T(1000) / A < T(a.min_Bt)) ? T(INFINITY) : w.B;
-        const T mce = (Bmid * T(1000) / A < T(a.min_Bt)) ? T(INFINITY) : Bmid;
-        const T cand0 = mc0 / (fabs(d0) + T(1e-16));
-        const T cande = mce / (fabs(d0) + T(1e-16));
+        const T mc0 = (wq_gm2(B, A) < T(a.min_Bt)) ? T(INFINITY) : B;
+        const T mce = (wq_gm2(Bmid, A) < T(a.min_Bt)) ? T(INFINITY) : Bmid;
+        const T cand0 = wq_div(mc0, fabs(d0) + T(1e-16));
+        const T cande = wq_div(mce, fabs(d0) + T(1e-16));
         const T Bfin = fabs(Bmid) < T(1e-12) ? T(0) : Bmid;  // :171
         Bout[c] = Bfin;
-        TWout[c] = totW * T(dt_min) / T(60) + TWin[c];
-        const T tq = dmax(totQ, T(10));
-        a.Pc[c] = dmax(T(1e6) * totW / (tq * A), T(0));
+        TWout[c] = wq_div(totW * T(dt_min), T(60)) + TWin[c];
+        const T tq = dmax(r.totQ, T(10));
+        a.Pc[c] = dmax(wq_div(T(1e6) * totW, tq * A), T(0));
         v[kWqMinDW] = fmin(v[kWqMinDW], (double)dW);
         v[kWqMinMid] = fmin(v[kWqMinMid], (double)Bmid);
         v[kWqCand0] = fmin(v[kWqCand0], (double)cand0);
         v[kWqCandEnd] = fmin(v[kWqCandEnd], (double)cande);
-        v[kWqSumWout] += (double)W4;
+        v[kWqSumWout] += (double)r.W4;
         if (!isinf(Bfin)) v[kWqSumB] += (double)Bfin;
     }
     block_store<6>(v, is_min, a.part, a.n_blocks);
@@ -249,6 +316,8 @@ __global__ __launch_bounds__(kWqThreads) void wq_sub_kernel(WqSimArgs<T> a, int 
     if (!ctl->split) return;
     const long long steps = ctl->steps;
     if (k > steps) return;
+    __shared__ WqLds<T> lds;
+    wq_stage_classes(a, lds);
     const int qp = (int)(st->k & 1), bp = (int)(st->wq_n & 1);
     const T *qx = a.qx_buf[qp], *qy = a.qy_buf[qp];
     // ping-pong so that the last sub-step lands in the call's output buffer
@@ -263,31 +332,22 @@ __global__ __launch_bounds__(kWqThreads) void wq_sub_kernel(WqSimArgs<T> a, int 
     const bool last = (k == steps);
     const double dt_min = st->time_step_min;
     const T A = T(a.A), dtT = T(dt);
-    const int ny = a.ny, nx = a.nx;
     const T *TWin = a.TW[bp];
     T *TWout = a.TW[bp ^ 1];
     double v[5] = {NAN, NAN, 0.0, 0.0, 0.0};  // minDW, candEnd, sumNeg, outMass, sumB
     const bool is_min[5] = {true, true, false, false, false};
-    for (long long c = blockIdx.x * (long long)kWqThreads + threadIdx.x; c < a.n;
-         c += (long long)gridDim.x * kWqThreads) {
-        const int i = (int)(c % ny), j = (int)(c / ny);
-        const WqCell<T> w = wq_cell(a, c, Bin, Bx_in, false, Bx_out);
-        const T qr = qx[c], qu = qy[c];
-        const T ql = j > 0 ? -qx[c - ny] : T(0);
-        const T qd = i < ny - 1 ? -qy[c + 1] : T(0);
-        const T qo = outlet_q(a, c);
-        const T W0 = wq_rate(w, ql, A), W1 = wq_rate(w, qr, A), W2 = wq_rate(w, qu, A);
-        const T W3 = wq_rate(w, qd, A), W4 = wq_rate(w, qo, A);
-        const T totW = W0 + W1 + W2 + W3 + W4;
-        T in_l = T(0), in_r = T(0), in_u = T(0), in_d = T(0);
-        if (j > 0) in_l = wq_rate(wq_cell(a, c - ny, Bin, Bx_in, false, (T *)nullptr), qx[c - ny], A);
-        if (j < nx - 1) in_r = wq_rate(wq_cell(a, c + ny, Bin, Bx_in, false, (T *)nullptr), -qr, A);
-        if (i > 0) in_u = wq_rate(wq_cell(a, c - 1, Bin, Bx_in, false, (T *)nullptr), -qu, A);
-        if (i < ny - 1) in_d = wq_rate(wq_cell(a, c + 1, Bin, Bx_in, false, (T *)nullptr), qy[c + 1], A);
-        const T dW = totW - (in_l + in_r + in_u + in_d);
-        const T om = (k == 1 ? T(0) : a.omass[c]) + totW * dtT / T(3600);
-        T Bn = w.B - dW * dtT / T(3600);
-        v[3] += (double)(W4 * dtT / T(3600));
+    // tiles of 256 rows of one column: tile t is (row block t % n_rb, column t / n_rb)
+    const int n_rb = (a.ny + kWqThreads - 1) / kWqThreads;
+    const long long n_tiles = (long long)n_rb * a.nx;
+    for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int j = (int)(t / n_rb), i = (int)(t % n_rb) * kWqThreads + (int)threadIdx.x;
+        if (i >= a.ny) continue;
+        const long long c = (long long)j * a.ny + i;
+        const WqRates<T> r = wq_rates(a, lds, i, j, c, qx, qy, Bin, Bx_in, Bx_out, false);
+        const T dW = r.dW, totW = r.totW;
+        const T om = (k == 1 ? T(0) : a.omass[c]) + wq_div(totW * dtT, T(3600));
+        T Bn = r.own.B - wq_div(dW * dtT, T(3600));
+        v[3] += (double)wq_div(r.W4 * dtT, T(3600));
         if (!last) {
             a.omass[c] = om;
             Bout[c] = Bn;
@@ -296,16 +356,15 @@ __global__ __launch_bounds__(kWqThreads) void wq_sub_kernel(WqSimArgs<T> a, int 
         if (Bn < T(0)) v[2] += (double)Bn;
         if (Bn < T(0)) Bn = T(0);
         const T d0 = dW > T(0) ? T(0) : dW;
-        const T mce = (Bn * T(1000) / A < T(a.min_Bt)) ? T(INFINITY) : Bn;
+        const T mce = (wq_gm2(Bn, A) < T(a.min_Bt)) ? T(INFINITY) : Bn;
         v[0] = fmin(v[0], (double)dW);
-        v[1] = fmin(v[1], (double)(mce / (fabs(d0) + T(1e-16))));
+        v[1] = fmin(v[1], (double)wq_div(mce, fabs(d0) + T(1e-16)));
         if (fabs(Bn) < T(1e-12)) Bn = T(0);
         Bout[c] = Bn;
         if (!isinf(Bn)) v[4] += (double)Bn;
-        const T totWa = om / (T(dt_min) / T(60));
-        const T totQ = (a.cell[c] & kWqNanBit) ? T(0) : ql + qr + qu + qd + qo;
-        TWout[c] = totWa * T(dt_min) / T(60) + TWin[c];
-        a.Pc[c] = dmax(T(1e6) * totWa / (dmax(totQ, T(10)) * A), T(0));
+        const T totWa = wq_div(om, wq_div(T(dt_min), T(60)));
+        TWout[c] = wq_div(totWa * T(dt_min), T(60)) + TWin[c];
+        a.Pc[c] = dmax(wq_div(T(1e6) * totWa, dmax(r.totQ, T(10)) * A), T(0));
     }
     // partials: the outlet mass accumulates over the sub-steps in this block's slot
     __shared__ double prev;

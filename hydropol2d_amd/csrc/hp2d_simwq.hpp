@@ -42,6 +42,7 @@ template <typename T> struct WqSimArgs {
     int n_outlet;
     T *qx_buf[2], *qy_buf[2];   // face fluxes [mm/h] by step parity
     const T *C3, *C4;           // class tables
+    int n_classes;
     T *B[2];                    // B_t [kg] by wash-off call parity
     T *Bs;                      // second buffer of the sub-step ping-pong
     T *Bx[2];                   // B_t_extra of the rating model (sub-steps)
