@@ -68,6 +68,7 @@ SIGNATURES = {
                                  + [_dp] * 9),
     "hp2d_groundwater_module": (C.c_int32, [_vp, _vp, _vp] + [_dp] * 7 + [_i32p]),
     "hp2d_fill_sinks": (C.c_int32, [C.c_int32, C.c_int32, _dp, _u8p, C.c_double, _dp]),
+    "hp2d_flow_accumulation": (C.c_int32, [C.c_int32, C.c_int32, _dp, _dp]),
     "hp2d_nn_conv3x3_fwd": (C.c_int32, [C.c_int32] * 5 + [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp,
                                          _vp]),
     "hp2d_nn_conv3x3_dgrad": (C.c_int32, [C.c_int32] * 5 + [_vp] * 5),
@@ -106,6 +107,14 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise HP2DError(-1, f"{path} is not built: run `python -m hydropol2d_amd.build`")
+    # PyTorch ships its own HIP runtime under the same soname as the system's
+    # (libamdhip64.so.7): whichever loads first serves the whole process, and
+    # PyTorch finds no device on another build of it.  Loading PyTorch first
+    # makes libhp2d bind to PyTorch's runtime, so both can share the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)  # AttributeError = missing export: fail loudly

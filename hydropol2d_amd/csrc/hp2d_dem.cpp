@@ -10,7 +10,9 @@
 // domain) are left NaN and never entered.
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <functional>
+#include <numeric>
 #include <queue>
 #include <vector>
 
@@ -77,6 +79,42 @@ extern "C" hp2d_status hp2d_fill_sinks(int32_t ny, int32_t nx, const double *z,
                 if (out[q] <= out[c]) out[q] = epsilon > 0.0 ? lift : out[c];
                 open.push({out[q], q});
             }
+    }
+    return HP2D_OK;
+}
+
+extern "C" hp2d_status hp2d_flow_accumulation(int32_t ny, int32_t nx, const double *z,
+                                              double *out) {
+    if (ny < 1 || nx < 1 || !z || !out) return HP2D_E_ARG;
+    const long long n = (long long)ny * nx;
+    std::vector<long long> order;
+    order.reserve((size_t)n);
+    for (long long c = 0; c < n; ++c) {
+        out[c] = std::isnan(z[c]) ? 0.0 : 1.0;
+        if (!std::isnan(z[c])) order.push_back(c);
+    }
+    // highest first; ties in index order (deterministic)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](long long a, long long b) { return z[a] > z[b]; });
+    const double diag = std::sqrt(2.0);
+    for (long long c : order) {
+        const int i = (int)(c % ny), j = (int)(c / ny);
+        double best = 0.0;
+        long long to = -1;
+        for (int di = -1; di <= 1; ++di)
+            for (int dj = -1; dj <= 1; ++dj) {
+                if (!di && !dj) continue;
+                const int a = i + di, b = j + dj;
+                if (a < 0 || a >= ny || b < 0 || b >= nx) continue;
+                const long long q = (long long)b * ny + a;
+                if (std::isnan(z[q])) continue;
+                const double s = (z[c] - z[q]) / ((di && dj) ? diag : 1.0);
+                if (s > best) {
+                    best = s;
+                    to = q;
+                }
+            }
+        if (to >= 0) out[to] += out[c];
     }
     return HP2D_OK;
 }

@@ -4,8 +4,12 @@ and time-to-peak maps and judged by the solver.
 
 The network has no counterpart in the reference.  Inputs per member are six
 maps [elevation, Manning n, Green-Ampt Ks, initial soil moisture, total
-rainfall depth, peak rainfall intensity], normalised and zero-padded to 32
-channels; outputs are [peak depth, time to peak].
+rainfall depth, peak rainfall intensity] and, derived from the elevation, the
+D8 upslope contributing area (log-scaled), normalised and zero-padded to 32
+channels; outputs are [peak depth, time to peak].  The contributing area is a
+non-local function of the DEM that a 4-level network of 3x3 convolutions
+sees only over its receptive field; handing it over as a map lets the
+network place the channels of a 256 x 256 member.
 
     E0 conv 32->32  @H     relu                    (skip to D0)
     E1 conv 32->64  @H/2   relu  after avg-pool    (skip to D1)
@@ -30,7 +34,7 @@ from . import _lib
 
 ENC = [(32, 32), (32, 64), (64, 128), (128, 256)]
 DEC = [(256, 128), (128, 64), (64, 32)]
-N_IN, N_PAD, N_OUT = 6, 32, 2
+N_IN, N_PAD, N_OUT = 7, 32, 2
 
 
 def _torch():
@@ -259,10 +263,11 @@ THETA_S = 0.4
 
 
 def member_features(spec):
-    """The six input maps of one member [H, W, 6] (FP32, normalised):
+    """The input maps of one member [H, W, 7] (FP32, normalised):
     elevation (minus its mean, / 10 m), Manning n / 0.02, Ks / 5 mm/h, initial
     soil moisture theta_s - dtheta, total rainfall depth / 25 mm, peak intensity
-    / 50 mm/h."""
+    / 50 mm/h, log10 of the D8 upslope contributing cells / 4."""
+    from .synthetic import flow_accumulation
     z = np.asarray(spec.elevation, dtype=np.float64)
     H, W = z.shape
     ga = spec.infiltration
@@ -274,13 +279,14 @@ def member_features(spec):
     dur = np.minimum(t, spec.routing_time_min) - np.minimum(t0, spec.routing_time_min)
     total = float(np.sum(v * np.maximum(dur, 0.0)) / 60.0)
     peak = float(v.max())
-    f = np.empty((H, W, 6), dtype=np.float32)
+    f = np.empty((H, W, N_IN), dtype=np.float32)
     f[..., 0] = (z - z.mean()) / 10.0
     f[..., 1] = np.asarray(spec.roughness) / 0.02
     f[..., 2] = ks / 5.0
     f[..., 3] = (THETA_S - dth) / THETA_S
     f[..., 4] = total / 25.0
     f[..., 5] = peak / 50.0
+    f[..., 6] = np.log10(np.maximum(flow_accumulation(z), 1.0)) / 4.0
     return f
 
 
@@ -306,7 +312,7 @@ def solve_members(specs, graph_steps=50):
 
 
 def stack_inputs(net, specs):
-    """Device bf16 input tensor [n, H, W, 32] of the members (6 maps, zero padding)."""
+    """Device bf16 input tensor [n, H, W, 32] of the members (N_IN maps, zero padding)."""
     torch = _torch()
     x = np.zeros((len(specs), net.H, net.W, N_PAD), dtype=np.float32)
     for i, sp in enumerate(specs):

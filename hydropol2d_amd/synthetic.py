@@ -52,6 +52,19 @@ def fill_sinks(z, seed_mask=None, epsilon=1e-4):
     return out
 
 
+def flow_accumulation(z):
+    """D8 upslope contributing cells of every cell (libhp2d, host code)."""
+    import ctypes as C
+    from . import _lib
+    L = _lib.load()
+    z = np.asfortranarray(z, dtype=np.float64)
+    out = np.empty_like(z, order="F")
+    _lib.check(L.hp2d_flow_accumulation(z.shape[0], z.shape[1],
+                                        z.ctypes.data_as(C.POINTER(C.c_double)),
+                                        out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
 def spectral_dem(ny, nx, dx, hurst=0.8, relief=200.0, seed=2):
     """Fractal (fBm) surface by spectral synthesis, scaled to `relief` metres."""
     rng = np.random.default_rng(seed)

@@ -263,6 +263,13 @@ hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_params *p, hp2
 hp2d_status hp2d_fill_sinks(int32_t ny, int32_t nx, const double *z, const uint8_t *seed_mask,
                             double epsilon, double *out);
 
+/* D8 upslope contributing cells of every cell (host preprocessing; the
+ * reference derives flow accumulation with TopoToolbox FLOWobj): each valid
+ * cell passes its count, itself included, to its steepest-descent neighbour
+ * (drop / distance, diagonals at sqrt(2)), cells visited from high to low;
+ * cells without a lower neighbour keep theirs.  NaN cells get 0.  out: ny*nx. */
+hp2d_status hp2d_flow_accumulation(int32_t ny, int32_t nx, const double *z, double *out);
+
 /* ------------------------------------------------------------------------ */
 /* Device-resident step loop (the performance path)                         */
 /* ------------------------------------------------------------------------ */

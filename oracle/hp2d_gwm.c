@@ -36,6 +36,7 @@ int32_t oracle_groundwater_module(const oracle_gwm_params *p, oracle_gwm_state *
                                   int32_t *ran, double *dt_gw_out) {
     const int32_t ny = p->ny, nx = p->nx;
     const int64_t n = (int64_t)ny * nx;
+    if (n <= 0) return -1;
     const double dt_s = p->time_step_min * 60.0;
     const double A = p->resolution * p->resolution;
     double *tmp = malloc(sizeof(double) * n), *Rdown = malloc(sizeof(double) * n);
