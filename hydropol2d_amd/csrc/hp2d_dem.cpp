@@ -87,22 +87,19 @@ extern "C" hp2d_status hp2d_flow_accumulation(int32_t ny, int32_t nx, const doub
                                               double *out) {
     if (ny < 1 || nx < 1 || !z || !out) return HP2D_E_ARG;
     const long long n = (long long)ny * nx;
-    std::vector<long long> order;
-    order.reserve((size_t)n);
+    // D8 receiver of every valid cell: the steepest descent (drop / distance,
+    // diagonals at sqrt(2)); -1 where no neighbour is lower
+    std::vector<long long> recv((size_t)n, -1);
+    std::vector<int> donors((size_t)n, 0);
+    const double diag = std::sqrt(2.0);
     for (long long c = 0; c < n; ++c) {
         out[c] = std::isnan(z[c]) ? 0.0 : 1.0;
-        if (!std::isnan(z[c])) order.push_back(c);
-    }
-    // highest first; ties in index order (deterministic)
-    std::stable_sort(order.begin(), order.end(),
-                     [&](long long a, long long b) { return z[a] > z[b]; });
-    const double diag = std::sqrt(2.0);
-    for (long long c : order) {
+        if (std::isnan(z[c])) continue;
         const int i = (int)(c % ny), j = (int)(c / ny);
         double best = 0.0;
         long long to = -1;
-        for (int di = -1; di <= 1; ++di)
-            for (int dj = -1; dj <= 1; ++dj) {
+        for (int dj = -1; dj <= 1; ++dj)
+            for (int di = -1; di <= 1; ++di) {
                 if (!di && !dj) continue;
                 const int a = i + di, b = j + dj;
                 if (a < 0 || a >= ny || b < 0 || b >= nx) continue;
@@ -114,7 +111,21 @@ extern "C" hp2d_status hp2d_flow_accumulation(int32_t ny, int32_t nx, const doub
                     to = q;
                 }
             }
-        if (to >= 0) out[to] += out[c];
+        recv[(size_t)c] = to;
+        if (to >= 0) donors[(size_t)to] += 1;
+    }
+    // pass counts downstream in topological order (a cell goes once all its
+    // donors have); counts are integers, so the order does not change the sums
+    std::vector<long long> ready;
+    ready.reserve((size_t)n);
+    for (long long c = 0; c < n; ++c)
+        if (!std::isnan(z[c]) && donors[(size_t)c] == 0) ready.push_back(c);
+    for (size_t k = 0; k < ready.size(); ++k) {
+        const long long c = ready[k];
+        const long long to = recv[(size_t)c];
+        if (to < 0) continue;
+        out[to] += out[c];
+        if (--donors[(size_t)to] == 0) ready.push_back(to);
     }
     return HP2D_OK;
 }

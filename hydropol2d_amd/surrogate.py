@@ -311,12 +311,19 @@ def solve_members(specs, graph_steps=50):
             time.perf_counter() - t0)
 
 
-def stack_inputs(net, specs):
-    """Device bf16 input tensor [n, H, W, 32] of the members (N_IN maps, zero padding)."""
+def stack_inputs(net, specs, workers=8):
+    """Device bf16 input tensor [n, H, W, 32] of the members (N_IN maps, zero padding).
+    The host maps of the members are built on `workers` threads (the library's
+    host routines release the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
     torch = _torch()
     x = np.zeros((len(specs), net.H, net.W, N_PAD), dtype=np.float32)
-    for i, sp in enumerate(specs):
-        x[i, :, :, :N_IN] = member_features(sp)
+
+    def fill(i):
+        x[i, :, :, :N_IN] = member_features(specs[i])
+
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as pool:
+        list(pool.map(fill, range(len(specs))))
     return net.to_bf16(torch.from_numpy(x).to(net.dev))
 
 
