@@ -51,3 +51,39 @@ def test_nan_cells_stay_outside():
     z[:5, :5] = np.nan
     f = synthetic.fill_sinks(z)
     assert np.isnan(f[:5, :5]).all() and np.isfinite(f[5:, 5:]).all()
+
+
+def test_flow_accumulation_on_a_plane_and_a_valley():
+    from hydropol2d_amd import synthetic
+    # a plane falling along the columns: every row drains straight to the last column
+    z = np.asfortranarray(100.0 - 0.05 * np.arange(12)[None, :] + 0.0 * np.arange(7)[:, None])
+    a = synthetic.flow_accumulation(z)
+    np.testing.assert_array_equal(a, np.tile(np.arange(1, 13, dtype=float), (7, 1)))
+    # a V valley draining along its axis: the axis cell of the last column
+    # collects the whole grid, the side cells collect only themselves
+    i = np.arange(9)[:, None]
+    j = np.arange(15)[None, :]
+    z = np.asfortranarray(0.1 * np.abs(i - 4) + 0.01 * (14 - j))
+    a = synthetic.flow_accumulation(z)
+    assert a[4, 14] == z.size
+    assert a[0, 0] == 1 and a[8, 7] == 1
+
+
+def test_flow_accumulation_counts_every_cell_once():
+    from hydropol2d_amd import synthetic
+    z = synthetic.fill_sinks(synthetic.spectral_dem(48, 64, 5.0, 0.8, 20.0, seed=4))
+    z[:5, :7] = np.nan
+    a = synthetic.flow_accumulation(z)
+    assert np.all(a[np.isnan(z)] == 0)
+    # the cells that pass their count on are those with a lower D8 neighbour;
+    # the counts kept at the others (the drainage points) add up to the valid cells
+    ny, nx = z.shape
+    keeps = np.zeros_like(a, dtype=bool)
+    for ii in range(ny):
+        for jj in range(nx):
+            if np.isnan(z[ii, jj]):
+                continue
+            nb = z[max(ii - 1, 0):ii + 2, max(jj - 1, 0):jj + 2]
+            keeps[ii, jj] = not np.any(nb < z[ii, jj])
+    assert a[keeps].sum() == np.count_nonzero(~np.isnan(z))
+    assert np.all(a[~np.isnan(z)] >= 1)
