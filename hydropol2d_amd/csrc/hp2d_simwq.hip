@@ -27,17 +27,19 @@ namespace hp2d {
 constexpr int kWqThreads = 256;
 constexpr int kWqFoldThreads = 1024;
 
-// pow(x, c) for x >= 0 or NaN: the zero base (a face without flow in the
-// page's direction, the common case) costs no transcendental, and gives
-// exactly what pow gives: +0 for c > 0, 1 for c == 0, +inf for c < 0
+// pow(0, c) as pow gives it: +0 for c > 0, 1 for c == 0, +inf for c < 0.  The
+// FP64 power law takes this case without a call (a face without flow in the
+// page's direction, the common case there)
 template <typename T> __device__ __forceinline__ T wq_pow0(T c) {
     return isnan(c) ? c : (c > T(0) ? T(0) : (c == T(0) ? T(1) : T(INFINITY)));
 }
 __device__ __forceinline__ double wq_pow(double x, double c) { return x == 0.0 ? wq_pow0(c) : pow(x, c); }
+// FP32: exp2(c log2 x) without a branch.  The zero base needs no case of its
+// own: log2(0) = -inf, so c > 0 gives exp2(-inf) = +0 and c < 0 gives +inf, as
+// pow does; only c = 0 (pow(x, 0) = 1 for every x, NaN included) is selected.
 __device__ __forceinline__ float wq_pow(float x, float c) {
-    if (x == 0.0f) return wq_pow0(c);
-    if (c == 0.0f) return 1.0f;  // pow(x, 0) = 1 for every x
-    return __builtin_amdgcn_exp2f(c * __builtin_amdgcn_logf(x));
+    const float r = __builtin_amdgcn_exp2f(c * __builtin_amdgcn_logf(x));
+    return c == 0.0f ? 1.0f : r;
 }
 
 // FP64 keeps the reference's operation order; FP32 multiplies by hardware
@@ -102,13 +104,19 @@ __device__ __forceinline__ T wq_rate(const WqCell<T> &w, T q, T A) {
     return w.zero ? T(0) : r;
 }
 
-// a neighbour's page towards the cell: its parameters are only read when the
-// face carries flow in that direction (or C4 = 0 makes a zero flux count)
+// a neighbour's page towards the cell.  FP64: the neighbour's parameters are
+// only read when the face carries flow in that direction (or C4 = 0 makes a
+// zero flux count), which skips the software pow of the dry faces.  FP32: the
+// rate is evaluated on every face without a branch (the hardware exp2/log2
+// cost less than the divergent branch); a dry face gives C3 * 0 * f = 0 as the
+// skipped case does, f being finite wherever the cell is not masked.
 template <typename T>
 __device__ __forceinline__ T wq_inflow(const WqSimArgs<T> &a, const WqLds<T> &lds, long long nb,
                                        T q, T A, const T *Bin, const T *Bx_in, bool first) {
-    const T qq = q < T(0) ? T(0) : q;
-    if (qq == T(0) && wq_pow0(lds.C4[a.cell[nb] & 0xffu]) == T(0)) return T(0);
+    if constexpr (sizeof(T) == 8) {
+        const T qq = q < T(0) ? T(0) : q;
+        if (qq == T(0) && wq_pow0(lds.C4[a.cell[nb] & 0xffu]) == T(0)) return T(0);
+    }
     return wq_rate(wq_cell(a, lds, nb, Bin, Bx_in, first, (T *)nullptr), q, A);
 }
 
@@ -207,12 +215,15 @@ __global__ __launch_bounds__(kWqThreads) void wq_first_kernel(WqSimArgs<T> a) {
     T *TWout = a.TW[bp ^ 1];
     const double dt_min = st->time_step_min;
     const T A = T(a.A);
-    double v[6] = {NAN, NAN, NAN, NAN, 0.0, 0.0};
-    const bool is_min[6] = {true, true, true, true, false, false};
+    // minima in the working precision (the conversion to FP64 is exact and
+    // monotone, so the minimum converts unchanged), sums in FP64
+    T mn[4] = {T(NAN), T(NAN), T(NAN), T(NAN)};
+    double s_wout = 0.0, s_b = 0.0;
     // tiles of 256 rows of one column: tile t is (row block t % n_rb, column t / n_rb)
-    const int n_rb = (a.ny + kWqThreads - 1) / kWqThreads;
-    const long long n_tiles = (long long)n_rb * a.nx;
-    for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    // (fewer than 2^32 cells, so fewer than 2^24 tiles)
+    const unsigned n_rb = (unsigned)(a.ny + kWqThreads - 1) / kWqThreads;
+    const unsigned n_tiles = n_rb * (unsigned)a.nx;
+    for (unsigned t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const int j = (int)(t / n_rb), i = (int)(t % n_rb) * kWqThreads + (int)threadIdx.x;
         if (i >= a.ny) continue;
         const long long c = (long long)j * a.ny + i;
@@ -231,13 +242,15 @@ __global__ __launch_bounds__(kWqThreads) void wq_first_kernel(WqSimArgs<T> a) {
         TWout[c] = wq_div(totW * T(dt_min), T(60)) + TWin[c];
         const T tq = dmax(r.totQ, T(10));
         a.Pc[c] = dmax(wq_div(T(1e6) * totW, tq * A), T(0));
-        v[kWqMinDW] = fmin(v[kWqMinDW], (double)dW);
-        v[kWqMinMid] = fmin(v[kWqMinMid], (double)Bmid);
-        v[kWqCand0] = fmin(v[kWqCand0], (double)cand0);
-        v[kWqCandEnd] = fmin(v[kWqCandEnd], (double)cande);
-        v[kWqSumWout] += (double)r.W4;
-        if (!isinf(Bfin)) v[kWqSumB] += (double)Bfin;
+        mn[kWqMinDW] = dmin(mn[kWqMinDW], dW);
+        mn[kWqMinMid] = dmin(mn[kWqMinMid], Bmid);
+        mn[kWqCand0] = dmin(mn[kWqCand0], cand0);
+        mn[kWqCandEnd] = dmin(mn[kWqCandEnd], cande);
+        s_wout += (double)r.W4;
+        if (!isinf(Bfin)) s_b += (double)Bfin;
     }
+    double v[6] = {(double)mn[0], (double)mn[1], (double)mn[2], (double)mn[3], s_wout, s_b};
+    const bool is_min[6] = {true, true, true, true, false, false};
     block_store<6>(v, is_min, a.part, a.n_blocks);
 }
 
@@ -334,12 +347,13 @@ __global__ __launch_bounds__(kWqThreads) void wq_sub_kernel(WqSimArgs<T> a, int 
     const T A = T(a.A), dtT = T(dt);
     const T *TWin = a.TW[bp];
     T *TWout = a.TW[bp ^ 1];
+    T mn_dw = T(NAN), mn_cand = T(NAN);       // minima in the working precision
     double v[5] = {NAN, NAN, 0.0, 0.0, 0.0};  // minDW, candEnd, sumNeg, outMass, sumB
     const bool is_min[5] = {true, true, false, false, false};
     // tiles of 256 rows of one column: tile t is (row block t % n_rb, column t / n_rb)
-    const int n_rb = (a.ny + kWqThreads - 1) / kWqThreads;
-    const long long n_tiles = (long long)n_rb * a.nx;
-    for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const unsigned n_rb = (unsigned)(a.ny + kWqThreads - 1) / kWqThreads;
+    const unsigned n_tiles = n_rb * (unsigned)a.nx;
+    for (unsigned t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const int j = (int)(t / n_rb), i = (int)(t % n_rb) * kWqThreads + (int)threadIdx.x;
         if (i >= a.ny) continue;
         const long long c = (long long)j * a.ny + i;
@@ -357,8 +371,8 @@ __global__ __launch_bounds__(kWqThreads) void wq_sub_kernel(WqSimArgs<T> a, int 
         if (Bn < T(0)) Bn = T(0);
         const T d0 = dW > T(0) ? T(0) : dW;
         const T mce = (wq_gm2(Bn, A) < T(a.min_Bt)) ? T(INFINITY) : Bn;
-        v[0] = fmin(v[0], (double)dW);
-        v[1] = fmin(v[1], (double)wq_div(mce, fabs(d0) + T(1e-16)));
+        mn_dw = dmin(mn_dw, dW);
+        mn_cand = dmin(mn_cand, wq_div(mce, fabs(d0) + T(1e-16)));
         if (fabs(Bn) < T(1e-12)) Bn = T(0);
         Bout[c] = Bn;
         if (!isinf(Bn)) v[4] += (double)Bn;
@@ -366,6 +380,8 @@ __global__ __launch_bounds__(kWqThreads) void wq_sub_kernel(WqSimArgs<T> a, int 
         TWout[c] = wq_div(totWa * T(dt_min), T(60)) + TWin[c];
         a.Pc[c] = dmax(wq_div(T(1e6) * totWa, dmax(r.totQ, T(10)) * A), T(0));
     }
+    v[0] = (double)mn_dw;
+    v[1] = (double)mn_cand;
     // partials: the outlet mass accumulates over the sub-steps in this block's slot
     __shared__ double prev;
     if (threadIdx.x == 0) prev = (k == 1) ? 0.0 : a.part[(long long)kWqOutMass * a.n_blocks + blockIdx.x];
