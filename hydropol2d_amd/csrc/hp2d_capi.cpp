@@ -116,10 +116,15 @@ int32_t hp2d_device_count(void) {
 
 const char *hp2d_global_last_error(void) { return g_last_error.c_str(); }
 
+// Largest grid: cell offsets are 32-bit, and a one-thread-per-cell launch of
+// 256-thread blocks must stay under 2^32 work-items (a dispatch counts them in
+// 32 bits)
+constexpr double kMaxCells = 4294967296.0 - 65536.0;
+
 hp2d_status hp2d_create(int32_t ny, int32_t nx, int32_t precision, hp2d_handle **out) {
     if (!out || ny < 1 || nx < 1 || (precision != HP2D_FP64 && precision != HP2D_FP32) ||
-        (double)ny * (double)nx >= 4294967296.0) {
-        g_last_error = "hp2d_create: bad arguments (grids are limited to 2^32 cells)";
+        (double)ny * (double)nx > kMaxCells) {
+        g_last_error = "hp2d_create: bad arguments (grids are limited to 2^32 - 2^16 cells)";
         return HP2D_E_ARG;
     }
     if (hp2d_device_count() < 1) {
@@ -630,8 +635,8 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         g_last_error = "hp2d_sim_create: bad arguments";
         return HP2D_E_ARG;
     }
-    if ((double)cfg->ny * (double)cfg->nx >= 4294967296.0) {
-        g_last_error = "hp2d_sim_create: grids are limited to 2^32 cells";
+    if ((double)cfg->ny * (double)cfg->nx > kMaxCells) {
+        g_last_error = "hp2d_sim_create: grids are limited to 2^32 - 2^16 cells";
         return HP2D_E_ARG;
     }
     if (cfg->routing != HP2D_ROUTING_INERTIAL || cfg->flag_numerical_scheme != 1) {

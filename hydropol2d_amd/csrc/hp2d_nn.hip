@@ -208,16 +208,20 @@ __global__ __launch_bounds__(kBlock) void conv3x3_wgrad_kernel(ConvArgs a) {
         }
 }
 
+// grid-stride loop bounds of the element-wise kernels
+__device__ __forceinline__ long long gs_first() { return blockIdx.x * (long long)blockDim.x + threadIdx.x; }
+__device__ __forceinline__ long long gs_stride() { return (long long)gridDim.x * blockDim.x; }
+
 // W'[ci][kh][kw][co] = W[co][2-kh][2-kw][ci]: the data-gradient weights
 __global__ void flip_transpose_kernel(const uint16_t *w, int Cin, int Cout, uint16_t *wt) {
-    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const long long total = 9LL * Cin * Cout;
-    if (e >= total) return;
-    const int ci = (int)(e % Cin);
-    const int tap = (int)((e / Cin) % 9);
-    const int co = (int)(e / (9LL * Cin));
-    const int kh = tap / 3, kw = tap % 3;
-    wt[((long long)ci * 9 + (2 - kh) * 3 + (2 - kw)) * Cout + co] = w[e];
+    for (long long e = gs_first(); e < total; e += gs_stride()) {
+        const int ci = (int)(e % Cin);
+        const int tap = (int)((e / Cin) % 9);
+        const int co = (int)(e / (9LL * Cin));
+        const int kh = tap / 3, kw = tap % 3;
+        wt[((long long)ci * 9 + (2 - kh) * 3 + (2 - kw)) * Cout + co] = w[e];
+    }
 }
 
 // bias gradient: gb[c] += sum over pixels of g[p][c]
@@ -234,84 +238,83 @@ __global__ void bias_grad_kernel(const uint16_t *g, long long P, int C, float *g
 
 // g *= (y > 0): the ReLU backward on the stored activation
 __global__ void relu_mask_kernel(uint16_t *g, const uint16_t *y, long long n) {
-    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i < n && !(bf2f(y[i]) > 0.f)) g[i] = 0;
+    for (long long i = gs_first(); i < n; i += gs_stride())
+        if (!(bf2f(y[i]) > 0.f)) g[i] = 0;
 }
 
 // out = a + b (bf16, for gradients meeting at a skip connection)
 __global__ void add_kernel(const uint16_t *a, const uint16_t *b, uint16_t *out, long long n) {
-    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i < n) out[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+    for (long long i = gs_first(); i < n; i += gs_stride()) out[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
 }
 
 // 2x2 average pooling (H, W even) and its backward
 __global__ void avgpool2_kernel(const uint16_t *x, int N, int H, int W, int C, uint16_t *y) {
-    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const int Ho = H / 2, Wo = W / 2;
     const long long total = (long long)N * Ho * Wo * C;
-    if (e >= total) return;
-    const int c = (int)(e % C);
-    long long r = e / C;
-    const int wo = (int)(r % Wo);
-    r /= Wo;
-    const int ho = (int)(r % Ho);
-    const int n = (int)(r / Ho);
-    auto at = [&](int h, int w) { return bf2f(x[(((long long)n * H + h) * W + w) * C + c]); };
-    const float s = at(2 * ho, 2 * wo) + at(2 * ho, 2 * wo + 1) + at(2 * ho + 1, 2 * wo) +
-                    at(2 * ho + 1, 2 * wo + 1);
-    y[e] = f2bf(0.25f * s);
+    for (long long e = gs_first(); e < total; e += gs_stride()) {
+        const int c = (int)(e % C);
+        long long r = e / C;
+        const int wo = (int)(r % Wo);
+        r /= Wo;
+        const int ho = (int)(r % Ho);
+        const int n = (int)(r / Ho);
+        auto at = [&](int h, int w) { return bf2f(x[(((long long)n * H + h) * W + w) * C + c]); };
+        const float s = at(2 * ho, 2 * wo) + at(2 * ho, 2 * wo + 1) + at(2 * ho + 1, 2 * wo) +
+                        at(2 * ho + 1, 2 * wo + 1);
+        y[e] = f2bf(0.25f * s);
+    }
 }
 __global__ void avgpool2_bwd_kernel(const uint16_t *gy, int N, int H, int W, int C, uint16_t *gx) {
-    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const long long total = (long long)N * H * W * C;
-    if (e >= total) return;
-    const int c = (int)(e % C);
-    long long r = e / C;
-    const int w = (int)(r % W);
-    r /= W;
-    const int h = (int)(r % H);
-    const int n = (int)(r / H);
-    gx[e] = f2bf(0.25f * bf2f(gy[(((long long)n * (H / 2) + h / 2) * (W / 2) + w / 2) * C + c]));
+    for (long long e = gs_first(); e < total; e += gs_stride()) {
+        const int c = (int)(e % C);
+        long long r = e / C;
+        const int w = (int)(r % W);
+        r /= W;
+        const int h = (int)(r % H);
+        const int n = (int)(r / H);
+        gx[e] = f2bf(0.25f * bf2f(gy[(((long long)n * (H / 2) + h / 2) * (W / 2) + w / 2) * C + c]));
+    }
 }
 
 // nearest 2x upsampling (output H, W) and its backward (sum over the 2x2 block)
 __global__ void upsample2_kernel(const uint16_t *x, int N, int H, int W, int C, uint16_t *y) {
-    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const long long total = (long long)N * H * W * C;
-    if (e >= total) return;
-    const int c = (int)(e % C);
-    long long r = e / C;
-    const int w = (int)(r % W);
-    r /= W;
-    const int h = (int)(r % H);
-    const int n = (int)(r / H);
-    y[e] = x[(((long long)n * (H / 2) + h / 2) * (W / 2) + w / 2) * C + c];
+    for (long long e = gs_first(); e < total; e += gs_stride()) {
+        const int c = (int)(e % C);
+        long long r = e / C;
+        const int w = (int)(r % W);
+        r /= W;
+        const int h = (int)(r % H);
+        const int n = (int)(r / H);
+        y[e] = x[(((long long)n * (H / 2) + h / 2) * (W / 2) + w / 2) * C + c];
+    }
 }
 __global__ void upsample2_bwd_kernel(const uint16_t *gy, int N, int H, int W, int C, uint16_t *gx) {
-    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const int Ho = H / 2, Wo = W / 2;
     const long long total = (long long)N * Ho * Wo * C;
-    if (e >= total) return;
-    const int c = (int)(e % C);
-    long long r = e / C;
-    const int wo = (int)(r % Wo);
-    r /= Wo;
-    const int ho = (int)(r % Ho);
-    const int n = (int)(r / Ho);
-    auto at = [&](int h, int w) { return bf2f(gy[(((long long)n * H + h) * W + w) * C + c]); };
-    gx[e] = f2bf(at(2 * ho, 2 * wo) + at(2 * ho, 2 * wo + 1) + at(2 * ho + 1, 2 * wo) +
-                 at(2 * ho + 1, 2 * wo + 1));
+    for (long long e = gs_first(); e < total; e += gs_stride()) {
+        const int c = (int)(e % C);
+        long long r = e / C;
+        const int wo = (int)(r % Wo);
+        r /= Wo;
+        const int ho = (int)(r % Ho);
+        const int n = (int)(r / Ho);
+        auto at = [&](int h, int w) { return bf2f(gy[(((long long)n * H + h) * W + w) * C + c]); };
+        gx[e] = f2bf(at(2 * ho, 2 * wo) + at(2 * ho, 2 * wo + 1) + at(2 * ho + 1, 2 * wo) +
+                     at(2 * ho + 1, 2 * wo + 1));
+    }
 }
 
 // 1x1 head: out[p][o] = b[o] + sum_c x[p][c] w[o][c] (FP32 out, O outputs)
 __global__ void head_fwd_kernel(const uint16_t *x, long long P, int C, const float *w,
                                 const float *b, int O, float *out) {
-    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    for (int o = 0; o < O; ++o) {
-        float s = b[o];
-        for (int c = 0; c < C; ++c) s += bf2f(x[p * C + c]) * w[o * C + c];
-        out[p * O + o] = s;
+    for (long long p = gs_first(); p < P; p += gs_stride()) {
+        for (int o = 0; o < O; ++o) {
+            float s = b[o];
+            for (int c = 0; c < C; ++c) s += bf2f(x[p * C + c]) * w[o * C + c];
+            out[p * O + o] = s;
+        }
     }
 }
 // its backward: gx[p][c] = sum_o gy[p][o] w[o][c] (bf16); gw, gb accumulated
@@ -369,9 +372,8 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const uint16_t *x, long l
 // masked mean squared error over the O output maps; gy = d loss / d out
 __global__ void mse_kernel(const float *out, const float *target, const uint8_t *mask, long long P,
                            int O, float inv_count, float *gy, float *loss) {
-    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     float l = 0.f;
-    if (p < P) {
+    for (long long p = gs_first(); p < P; p += gs_stride()) {
         const bool on = !mask || mask[p];
         for (int o = 0; o < O; ++o) {
             const float d = on ? out[p * O + o] - target[p * O + o] : 0.f;
@@ -386,24 +388,32 @@ __global__ void mse_kernel(const float *out, const float *target, const uint8_t 
 // Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the bf16 copy
 __global__ void adam_kernel(float *wt, float *m, float *v, const float *g, long long n, float lr,
                             float b1, float b2, float eps, float c1, float c2, uint16_t *wb) {
-    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float gi = g[i];
-    const float mi = b1 * m[i] + (1.f - b1) * gi;
-    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    const float w = wt[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-    wt[i] = w;
-    if (wb) wb[i] = f2bf(w);
+    for (long long i = gs_first(); i < n; i += gs_stride()) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float w = wt[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        wt[i] = w;
+        if (wb) wb[i] = f2bf(w);
+    }
 }
 
 __global__ void f32_to_bf16_kernel(const float *x, long long n, uint16_t *y) {
-    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i < n) y[i] = f2bf(x[i]);
+    for (long long i = gs_first(); i < n; i += gs_stride()) y[i] = f2bf(x[i]);
 }
 
-inline dim3 grid1(long long n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
+// Grid of an element-wise launch: one thread per element up to kMaxGridBlocks
+// blocks, the kernels' grid-stride loops take the rest.  A dispatch counts its
+// work-items in 32 bits, so an uncapped grid over more than 2^32 elements
+// would silently cover only part of them.
+constexpr long long kMaxGridBlocks = 1 << 16;
+inline dim3 grid1(long long n, int b = 256) {
+    long long blocks = (n + b - 1) / b;
+    if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
+    return dim3((unsigned)(blocks > 0 ? blocks : 1));
+}
 
 }  // namespace nn
 }  // namespace hp2d

@@ -148,3 +148,24 @@ def test_training_fits_a_smooth_map():
     losses = [float(net.train_step(dx, dt, lr=3e-3).item()) for _ in range(150)]
     assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
     assert np.isfinite(losses).all()
+
+
+def test_elementwise_kernels_cover_more_than_2_32_elements():
+    # a dispatch counts its work-items in 32 bits: the element-wise kernels run
+    # a capped grid with grid-stride loops, so the last elements of a tensor of
+    # more than 2^32 elements (a 4352-member stack of 256 x 256 x 32 inputs is
+    # 9.1e9) are converted too (17 GiB + 8.6 GiB of device memory)
+    torch = _torch()
+    n = (1 << 32) + 4096
+    x = torch.full((n,), 1.5, dtype=torch.float32, device="cuda")
+    x[-4096:] = 2.5
+    x[(1 << 31):(1 << 31) + 8] = -3.0
+    y = torch.zeros((n,), dtype=torch.bfloat16, device="cuda")
+    assert _lib().hp2d_nn_to_bf16(_ptr(x), n, _ptr(y), None) == 0
+    torch.cuda.synchronize()
+    del x
+    assert bool((y[-4096:] == 2.5).all())
+    assert bool((y[(1 << 31):(1 << 31) + 8] == -3.0).all())
+    assert bool((y[:4096] == 1.5).all()) and float(y[(1 << 32) - 1]) == 1.5
+    del y
+    torch.cuda.empty_cache()
