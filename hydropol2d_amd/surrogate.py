@@ -4,12 +4,16 @@ and time-to-peak maps and judged by the solver.
 
 The network has no counterpart in the reference.  Inputs per member are six
 maps [elevation, Manning n, Green-Ampt Ks, initial soil moisture, total
-rainfall depth, peak rainfall intensity] and, derived from the elevation, the
-D8 upslope contributing area (log-scaled), normalised and zero-padded to 32
-channels; outputs are [peak depth, time to peak].  The contributing area is a
-non-local function of the DEM that a 4-level network of 3x3 convolutions
-sees only over its receptive field; handing it over as a map lets the
-network place the channels of a 256 x 256 member.
+rainfall depth, peak rainfall intensity] and two derived from them: the D8
+upslope contributing area (log-scaled) and the kinematic normal depth of the
+peak excess rainfall over that area, (n q / sqrt(S))^(3/5) with
+q = (i_peak - Ks) A_up / dx and S the steepest D8 slope; normalised and
+zero-padded to 32 channels.  Outputs are [peak depth, time to peak].  The
+contributing area is a non-local function of the DEM that a 4-level network
+of 3x3 convolutions sees only over its receptive field; handing it over as a
+map, with the steady-state depth it implies, lets the network place and
+size the channels of a 256 x 256 member and learn the corrections (finite
+rain duration, storage, infiltration) on top.
 
     E0 conv 32->32  @H     relu                    (skip to D0)
     E1 conv 32->64  @H/2   relu  after avg-pool    (skip to D1)
@@ -34,7 +38,7 @@ from . import _lib
 
 ENC = [(32, 32), (32, 64), (64, 128), (128, 256)]
 DEC = [(256, 128), (128, 64), (64, 32)]
-N_IN, N_PAD, N_OUT = 7, 32, 2
+N_IN, N_PAD, N_OUT = 8, 32, 2
 
 
 def _torch():
@@ -263,11 +267,12 @@ THETA_S = 0.4
 
 
 def member_features(spec):
-    """The input maps of one member [H, W, 7] (FP32, normalised):
+    """The input maps of one member [H, W, 8] (FP32, normalised):
     elevation (minus its mean, / 10 m), Manning n / 0.02, Ks / 5 mm/h, initial
     soil moisture theta_s - dtheta, total rainfall depth / 25 mm, peak intensity
-    / 50 mm/h, log10 of the D8 upslope contributing cells / 4."""
-    from .synthetic import flow_accumulation
+    / 50 mm/h, log10 of the D8 upslope contributing cells / 4, kinematic normal
+    depth of the peak excess rainfall [dm] (slopes floored at 1e-3)."""
+    from .synthetic import d8_slope, flow_accumulation
     z = np.asarray(spec.elevation, dtype=np.float64)
     H, W = z.shape
     ga = spec.infiltration
@@ -286,11 +291,21 @@ def member_features(spec):
     f[..., 3] = (THETA_S - dth) / THETA_S
     f[..., 4] = total / 25.0
     f[..., 5] = peak / 50.0
-    f[..., 6] = np.log10(np.maximum(flow_accumulation(z), 1.0)) / 4.0
+    area = np.maximum(flow_accumulation(z), 1.0)
+    f[..., 6] = np.log10(area) / 4.0
+    dx = float(spec.resolution)
+    q = max(peak - ks, 0.0) / 3.6e6 * area * dx  # [m2/s] per unit width
+    slope = np.maximum(np.nan_to_num(d8_slope(z, dx), nan=0.0), 1e-3)
+    h_kin = (np.asarray(spec.roughness, dtype=np.float64) * q / np.sqrt(slope)) ** 0.6
+    f[..., 7] = h_kin * 10.0
     return f
 
 
-DEPTH_SCALE = 10.0   # target units: peak depth [m] * 10, time to peak [h]
+# target units: peak depth in cm, time to peak in tenths of the hour, so that
+# the squared error the network is trained on is mostly the depth's (the judge
+# measures peak-depth RMSE)
+DEPTH_SCALE = 100.0
+TPEAK_SCALE = 0.1
 
 
 def solve_members(specs, graph_steps=50):
@@ -329,7 +344,7 @@ def stack_inputs(net, specs, workers=8):
 
 def stack_targets(net, peak_m, tpeak_h):
     torch = _torch()
-    t = np.stack([peak_m * DEPTH_SCALE, tpeak_h], axis=-1).astype(np.float32)
+    t = np.stack([peak_m * DEPTH_SCALE, tpeak_h * TPEAK_SCALE], axis=-1).astype(np.float32)
     return torch.from_numpy(t).to(net.dev)
 
 

@@ -65,6 +65,24 @@ def flow_accumulation(z):
     return out
 
 
+def d8_slope(z, dx):
+    """Steepest downhill D8 slope of every cell [m/m] (0 at pits and flats,
+    NaN at NaN cells); diagonal neighbours at dx * sqrt(2)."""
+    z = np.asarray(z, dtype=np.float64)
+    ny, nx = z.shape
+    zp = np.pad(z, 1, constant_values=np.nan)
+    s = np.zeros_like(z)
+    for di in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            if di == 0 and dj == 0:
+                continue
+            nb = zp[1 + di:1 + di + ny, 1 + dj:1 + dj + nx]
+            drop = (z - nb) / (dx * (np.sqrt(2.0) if di and dj else 1.0))
+            s = np.fmax(s, np.where(np.isnan(drop), 0.0, drop))
+    s[np.isnan(z)] = np.nan
+    return s
+
+
 def spectral_dem(ny, nx, dx, hurst=0.8, relief=200.0, seed=2):
     """Fractal (fBm) surface by spectral synthesis, scaled to `relief` metres."""
     rng = np.random.default_rng(seed)

@@ -87,3 +87,21 @@ def test_flow_accumulation_counts_every_cell_once():
             keeps[ii, jj] = not np.any(nb < z[ii, jj])
     assert a[keeps].sum() == np.count_nonzero(~np.isnan(z))
     assert np.all(a[~np.isnan(z)] >= 1)
+
+
+def test_d8_slope_closed_forms():
+    from hydropol2d_amd import synthetic
+    # plane falling 0.05 m per column at dx = 2: steepest drop along the row, 0.025;
+    # the last column has no lower neighbour (0)
+    z = 100.0 - 0.05 * np.arange(10)[None, :] + 0.0 * np.arange(6)[:, None]
+    s = synthetic.d8_slope(z, 2.0)
+    np.testing.assert_allclose(s[:, :-1], 0.025, rtol=1e-12)
+    assert np.all(s[:, -1] == 0.0)
+    # a cone: the centre is a pit (0); the cells next to it fall straight in
+    i, j = np.meshgrid(np.arange(7), np.arange(7), indexing="ij")
+    z = np.hypot(i - 3.0, j - 3.0)
+    s = synthetic.d8_slope(z, 1.0)
+    assert s[3, 3] == 0.0
+    assert s[3, 4] == pytest.approx(1.0) and s[4, 4] == pytest.approx(1.0)
+    z[0, 0] = np.nan
+    assert np.isnan(synthetic.d8_slope(z, 1.0)[0, 0])
