@@ -11,7 +11,10 @@ dt/bookkeeping kernel, and every 10th step the groundwater module (recharge
 bucket + Boussinesq).  Inputs are resident in HBM before timing; steps run as
 captured hipGraphs.
 
-Prints one JSON line (value = cell-updates/s of the whole job).
+Prints one JSON line (value = cell-updates/s of the whole job).  Beside it:
+the batched-ensemble study (members/hour against the plain per-member loop,
+--ensemble-members) and, on request, the surrogate study (members/hour of
+the surrogate with its 5 % solver judge, --surrogate-train).
 """
 from __future__ import annotations
 
@@ -40,6 +43,10 @@ def parse():
     p.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     p.add_argument("--ensemble-members", type=int, default=64,
                    help="members of the batched-ensemble measurement (0 skips it)")
+    p.add_argument("--surrogate-train", type=int, default=0, metavar="N",
+                   help="also run the surrogate study: train on N solved 256 x 256 members, judge "
+                        "256 more (0 skips it; about 2.5 min at N = 4096)")
+    p.add_argument("--surrogate-epochs", type=int, default=40)
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run 5 device copies of MIB MiB (the known-size kernel that "
                         "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -106,6 +113,68 @@ def ensemble_throughput(n_members):
             "speedup": t_plain / t_batch, "max_peak_depth_diff_m": max_diff,
             "workload": "C4-style: sides 2^U(7,9), dx=5 m, fractal DEM on a 1 % tilt, n/Ks/rain "
                         "x e^U(ln .5, ln 1.5) around C1, 2 h each, FP32"}
+
+
+def surrogate_study(n_train, epochs, n_eval=256, n_plain=8):
+    """Members/hour of the surrogate with its 5 % solver judge against the
+    plain per-member solver and the batched ensemble, on 256 x 256 C4-style
+    members (2 h each, FP32).  The training data are the solver's own peak
+    depth and time-to-peak maps of n_train members; the n_eval held-out members
+    give the peak-depth RMSE against the solver and the judge's verdict."""
+    import torch
+    from hydropol2d_amd import surrogate as S
+    from hydropol2d_amd import synthetic
+    from hydropol2d_amd.simulation import Simulation
+    specs = synthetic.ensemble_c4(n_train + n_eval, seed=5000, min_log2=8, max_log2=8,
+                                  routing_time_min=120.0, vary_soil_moisture=True, filled=True)
+    pk, tp, t_data = S.solve_members(specs)
+    net = S.UNetSurrogate(256, 256, batch=32, seed=1)
+    t0 = time.perf_counter()
+    x = S.stack_inputs(net, specs[:n_train])
+    y = S.stack_targets(net, pk[:n_train], tp[:n_train])
+    S.train(net, x, y, epochs=epochs, batch=32, lr=1e-3, lr_final=5e-5)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del x, y
+    res = S.judged_predictions(net, specs[n_train:], judge_fraction=0.05, tol_m=0.02)
+    ref = pk[n_train:]
+    rmse = float(np.sqrt(np.mean((res["peak_m"] - ref) ** 2)))
+    unjudged = np.setdiff1d(np.arange(n_eval), res["judged"])
+    rmse_net = float(np.sqrt(np.mean((res["peak_m"][unjudged] - ref[unjudged]) ** 2)))
+    t_pred = res["t_inference_s"] + res["t_judge_s"] + res["t_fallback_s"]
+    t_plain = 0.0
+    for sp in specs[n_train:n_train + n_plain]:
+        with Simulation(sp) as sim:
+            t1 = time.perf_counter()
+            sim.run()
+            t_plain += time.perf_counter() - t1
+    per_plain = t_plain / n_plain
+    per_batch = t_data / len(specs)
+    study = 20000  # members of a probabilistic study (C5 scale)
+    t_study = t_data + t_train + study / n_eval * t_pred
+    print(f"[bench] surrogate: data {t_data:.1f} s, train {t_train:.1f} s, {n_eval} members in "
+          f"{t_pred:.3f} s, judge RMSE {res['judge_rmse_m'] * 100:.2f} cm", file=sys.stderr,
+          flush=True)
+    return {
+        "members_train": n_train, "members_eval": n_eval, "epochs": epochs,
+        "peak_depth_rmse_m": rmse, "surrogate_rmse_m_unjudged": rmse_net,
+        "judge_rmse_m": res["judge_rmse_m"],
+        "judge_members": int(len(res["judged"])), "fallback": bool(res["fallback"]),
+        "rmse_mean_map_m": float(np.sqrt(np.mean((pk[:n_train].mean(axis=0)[None] - ref) ** 2))),
+        "rmse_zero_map_m": float(np.sqrt(np.mean(ref ** 2))),
+        "s_inference": res["t_inference_s"], "s_judge": res["t_judge_s"],
+        "s_training_data": t_data, "s_training": t_train,
+        "members_per_hour_surrogate": n_eval / t_pred * 3600.0,
+        "members_per_hour_batched": 3600.0 / per_batch,
+        "members_per_hour_plain": 3600.0 / per_plain,
+        "speedup_vs_plain": per_plain * n_eval / t_pred,
+        "speedup_vs_batched": per_batch * n_eval / t_pred,
+        "speedup_vs_plain_20000_member_study_incl_training": study * per_plain / t_study,
+        "workload": "256x256 C4-style members, dx=5 m, sink-filled fractal DEM on a 1 % tilt, "
+                    "n/Ks/rain x e^U(ln .5, ln 1.5), soil moisture deficit U(0.15, 0.35), 2 h "
+                    "each, FP32; judge = 5 % of the members re-run by the solver, fallback above "
+                    "2 cm",
+    }
 
 
 def algorithmic_bytes_per_cell(spec, elem):
@@ -180,6 +249,8 @@ def main():
         print(f"[bench] cpu oracle {cpu['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
 
     ens = ensemble_throughput(args.ensemble_members) if args.ensemble_members > 0 else None
+    sur = (surrogate_study(args.surrogate_train, args.surrogate_epochs)
+           if args.surrogate_train > 0 else None)
 
     line = {
         "metric": "cell-updates/s (% HBM roofline)",
@@ -210,6 +281,7 @@ def main():
                      "launches_timed": launches, "graph_event_ms": graph_ms},
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "surrogate": sur,
     }
     print(json.dumps(line), flush=True)
 
