@@ -71,28 +71,39 @@ template <typename T> struct WqCell {
     bool zero;  // idx_bt (or the idx_nan_5 mask of the first pass): no flux
 };
 
+// the parameters of a cell from its flag word fl, its mass B and (rating
+// model) the entry B_t_extra bx; bx_out receives the updated B_t_extra
 template <typename T>
-__device__ __forceinline__ WqCell<T> wq_cell(const WqSimArgs<T> &a, const WqLds<T> &lds, long long c,
-                                             const T *Bin, const T *Bx_in, bool first, T *Bx_out) {
+__device__ __forceinline__ WqCell<T> wq_cell_of(const WqSimArgs<T> &a, const WqLds<T> &lds, unsigned fl,
+                                                T B, T bx, bool first, T &bx_out) {
     WqCell<T> w;
-    const unsigned fl = a.cell[c];
     const int cls = (int)(fl & 0xffu);
-    const T B = Bin[c];
     w.B = B;
     w.C3 = lds.C3[cls];
     w.C4 = lds.C4[cls];
     T f = B;
+    bx_out = bx;
     if (a.model == 1) {  // rating curve (:23-28, :89-94)
         const T Br = T(a.Bmin / 1000.0 * a.A), Bm = T(a.Bmax / 1000.0 * a.A);
-        T bx = first ? B : Bx_in[c];
         if (wq_gm2(B, T(a.A)) <= T(a.Bmin)) bx = T(0);
         if (wq_gm2(B, T(a.A)) >= T(a.Bmax)) bx = Bm;
-        if (Bx_out) Bx_out[c] = bx;
+        bx_out = bx;
         f = T(1) + dmax(bx - Br, T(0));
     }
     w.f = f;
     const bool low = (wq_gm2(B, T(a.A)) < T(a.min_Bt)) || isnan(B) || isinf(B);
     w.zero = low || (first && (fl & kWqNanBit));
+    return w;
+}
+
+template <typename T>
+__device__ __forceinline__ WqCell<T> wq_cell(const WqSimArgs<T> &a, const WqLds<T> &lds, long long c,
+                                             const T *Bin, const T *Bx_in, bool first, T *Bx_out) {
+    const T B = Bin[c];
+    const T bx_in = (a.model == 1 && !first) ? Bx_in[c] : B;
+    T bx;
+    const WqCell<T> w = wq_cell_of(a, lds, (unsigned)a.cell[c], B, bx_in, first, bx);
+    if (a.model == 1 && Bx_out) Bx_out[c] = bx;
     return w;
 }
 
@@ -121,8 +132,8 @@ __device__ __forceinline__ T wq_inflow(const WqSimArgs<T> &a, const WqLds<T> &ld
 }
 
 template <typename T>
-__device__ __forceinline__ T outlet_q(const WqSimArgs<T> &a, long long c) {
-    if (!(a.cell[c] & kWqOutletBit)) return T(0);
+__device__ __forceinline__ T outlet_q(const WqSimArgs<T> &a, long long c, unsigned fl) {
+    if (!(fl & kWqOutletBit)) return T(0);
     int lo = 0, hi = a.n_outlet - 1;
     while (lo <= hi) {
         const int mid = (lo + hi) >> 1;
@@ -179,7 +190,7 @@ __device__ __forceinline__ WqRates<T> wq_rates(const WqSimArgs<T> &a, const WqLd
     const T qr = qx[c], qu = qy[c];
     const T ql = j > 0 ? -qx[c - ny] : T(0);
     const T qd = i < ny - 1 ? -qy[c + 1] : T(0);
-    const T qo = outlet_q(a, c);
+    const T qo = outlet_q(a, c, (unsigned)a.cell[c]);
     const T W0 = wq_rate(w, ql, A), W1 = wq_rate(w, qr, A), W2 = wq_rate(w, qu, A);
     const T W3 = wq_rate(w, qd, A), W4 = wq_rate(w, qo, A);
     r.W4 = W4;
@@ -192,6 +203,68 @@ __device__ __forceinline__ WqRates<T> wq_rates(const WqSimArgs<T> &a, const WqLd
     if (i > 0) in_u = wq_inflow(a, lds, c - 1, -qu, A, Bin, Bx_in, first);
     if (i < ny - 1) in_d = wq_inflow(a, lds, c + 1, -qd, A, Bin, Bx_in, first);
     r.dW = r.totW - (in_l + in_r + in_u + in_d);
+    return r;
+}
+
+// Loads of one cell of the first pass: its flag word and mass and its four
+// neighbours' (left, right, up, down; a neighbour outside the grid reads the
+// cell itself and is masked), its two face fluxes and the neighbours' shared
+// ones, and its entry Tot_Washed.  The FP32 pass fetches them one tile ahead.
+template <typename T> struct WqIn {
+    T B, Bn[4], qr, qu, qxl, qyd, tw;
+    unsigned fl, fln[4];
+};
+
+template <typename T>
+__device__ __forceinline__ void wq_load(const WqSimArgs<T> &a, int i, int j, long long c, const T *qx,
+                                        const T *qy, const T *Bin, const T *TWin, WqIn<T> &in) {
+    const int ny = a.ny, nx = a.nx;
+    const long long nb[4] = {j > 0 ? c - ny : c, j < nx - 1 ? c + ny : c, i > 0 ? c - 1 : c,
+                             i < ny - 1 ? c + 1 : c};
+    in.fl = a.cell[c];
+    in.B = Bin[c];
+    in.qr = qx[c];
+    in.qu = qy[c];
+    in.qxl = qx[nb[0]];
+    in.qyd = qy[nb[3]];
+    in.tw = TWin[c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        in.fln[k] = a.cell[nb[k]];
+        in.Bn[k] = Bin[nb[k]];
+    }
+}
+
+// wq_rates of the first pass from preloaded inputs (the same arithmetic)
+template <typename T>
+__device__ __forceinline__ WqRates<T> wq_rates_in(const WqSimArgs<T> &a, const WqLds<T> &lds, int i,
+                                                  int j, long long c, const WqIn<T> &in, T *Bx_out) {
+    const int ny = a.ny, nx = a.nx;
+    const T A = T(a.A);
+    WqRates<T> r;
+    T bx;
+    const WqCell<T> w = wq_cell_of(a, lds, in.fl, in.B, in.B, true, bx);
+    if (a.model == 1 && Bx_out) Bx_out[c] = bx;
+    r.own = w;
+    const T qr = in.qr, qu = in.qu;
+    const T ql = j > 0 ? -in.qxl : T(0);
+    const T qd = i < ny - 1 ? -in.qyd : T(0);
+    const T qo = outlet_q(a, c, in.fl);
+    const T W0 = wq_rate(w, ql, A), W1 = wq_rate(w, qr, A), W2 = wq_rate(w, qu, A);
+    const T W3 = wq_rate(w, qd, A), W4 = wq_rate(w, qo, A);
+    r.W4 = W4;
+    r.totW = W0 + W1 + W2 + W3 + W4;
+    r.totQ = (in.fl & kWqNanBit) ? T(0) : ql + qr + qu + qd + qo;
+    const T qn[4] = {-ql, -qr, -qu, -qd};
+    const bool has[4] = {j > 0, j < nx - 1, i > 0, i < ny - 1};
+    T inr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        T unused;
+        const T v = wq_rate(wq_cell_of(a, lds, in.fln[k], in.Bn[k], in.Bn[k], true, unused), qn[k], A);
+        inr[k] = has[k] ? v : T(0);
+    }
+    r.dW = r.totW - (inr[0] + inr[1] + inr[2] + inr[3]);
     return r;
 }
 
@@ -223,12 +296,42 @@ __global__ __launch_bounds__(kWqThreads) void wq_first_kernel(WqSimArgs<T> a) {
     // (fewer than 2^32 cells, so fewer than 2^24 tiles)
     const unsigned n_rb = (unsigned)(a.ny + kWqThreads - 1) / kWqThreads;
     const unsigned n_tiles = n_rb * (unsigned)a.nx;
+    T *Bx_out = a.model == 1 ? a.Bx[0] : (T *)nullptr;
+    auto cell_of_tile = [&](unsigned t, int &i, int &j) {
+        j = (int)(t / n_rb);
+        i = (int)(t % n_rb) * kWqThreads + (int)threadIdx.x;
+        return (long long)j * a.ny + i;
+    };
+    // FP32: the loads of tile t + gridDim.x are issued before tile t is
+    // computed (the FP64 pass is bound by its software pow, not by latency)
+    constexpr bool kPrefetch = sizeof(T) == 4;
+    WqIn<T> nxt;
+    if (kPrefetch && blockIdx.x < n_tiles) {
+        int i0, j0;
+        const long long c0 = cell_of_tile(blockIdx.x, i0, j0);
+        if (i0 < a.ny) wq_load(a, i0, j0, c0, qx, qy, Bin, TWin, nxt);
+    }
     for (unsigned t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        const int j = (int)(t / n_rb), i = (int)(t % n_rb) * kWqThreads + (int)threadIdx.x;
-        if (i >= a.ny) continue;
-        const long long c = (long long)j * a.ny + i;
-        const WqRates<T> r = wq_rates(a, lds, i, j, c, qx, qy, Bin, (const T *)nullptr,
-                                      a.model == 1 ? a.Bx[0] : (T *)nullptr, true);
+        int i, j;
+        const long long c = cell_of_tile(t, i, j);
+        WqRates<T> r;
+        T tw_in;
+        if constexpr (kPrefetch) {
+            const WqIn<T> cur = nxt;
+            const unsigned tn = t + gridDim.x;
+            if (tn < n_tiles) {
+                int in_i, in_j;
+                const long long cn = cell_of_tile(tn, in_i, in_j);
+                if (in_i < a.ny) wq_load(a, in_i, in_j, cn, qx, qy, Bin, TWin, nxt);
+            }
+            if (i >= a.ny) continue;
+            r = wq_rates_in(a, lds, i, j, c, cur, Bx_out);
+            tw_in = cur.tw;
+        } else {
+            if (i >= a.ny) continue;
+            r = wq_rates(a, lds, i, j, c, qx, qy, Bin, (const T *)nullptr, Bx_out, true);
+            tw_in = TWin[c];
+        }
         const T B = r.own.B, dW = r.dW, totW = r.totW;
         const T Bmid = B - wq_div(dW * T(dt_min), T(60));
         // tmin candidates (:61-76, :144-159): m_calc / (|min(dW,0)| + 1e-16)
@@ -239,7 +342,7 @@ __global__ __launch_bounds__(kWqThreads) void wq_first_kernel(WqSimArgs<T> a) {
         const T cande = wq_div(mce, fabs(d0) + T(1e-16));
         const T Bfin = fabs(Bmid) < T(1e-12) ? T(0) : Bmid;  // :171
         Bout[c] = Bfin;
-        TWout[c] = wq_div(totW * T(dt_min), T(60)) + TWin[c];
+        TWout[c] = wq_div(totW * T(dt_min), T(60)) + tw_in;
         const T tq = dmax(r.totQ, T(10));
         a.Pc[c] = dmax(wq_div(T(1e6) * totW, tq * A), T(0));
         mn[kWqMinDW] = dmin(mn[kWqMinDW], dW);
