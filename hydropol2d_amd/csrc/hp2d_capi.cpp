@@ -881,7 +881,7 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         A_((void **)&s->wq_ctl, sizeof(WqCtl));
     }
     if (s->gw) {
-        A_((void **)&s->gw_mem, 24 * n * sizeof(double));
+        A_((void **)&s->gw_mem, 22 * n * sizeof(double));
         A_((void **)&s->gw_col, 3 * (size_t)s->nx * sizeof(double));
         A_((void **)&s->gw_dom, n);
     }
@@ -966,7 +966,15 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         up(arr(14), pend.data(), n * sizeof(double));
         up(arr(15), pend.data(), n * sizeof(double));
         up(arr(16), pend.data(), n * sizeof(double));
-        up(arr(12), zeros.data(), n * sizeof(double));  // cumulative recharge
+        // the aquifer bed and the aquifer conductivity [m/s] are static: set once
+        // here, so the module's recharge pass does not rewrite them every call
+        std::vector<double> zbed(n), kgw(n);
+        for (size_t c = 0; c < n; ++c) {
+            zbed[c] = gw_host[0 * n + c] - gw_host[1 * n + c];
+            kgw[c] = gw_host[7 * n + c] / 1000.0 / 3600.0;
+        }
+        up(arr(17), zbed.data(), n * sizeof(double));
+        up(arr(18), kgw.data(), n * sizeof(double));
         std::vector<uint8_t> dom(n);
         for (size_t c = 0; c < n; ++c) dom[c] = std::isnan(z[c]) ? 0 : 1;
         up(s->gw_dom, dom.data(), n);
@@ -976,11 +984,13 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         a.n = (long long)n;
         a.elev = arr(0); a.soil_depth = arr(1); a.ts = arr(2); a.tr = arr(3); a.nv = arr(4);
         a.alpha = arr(5); a.ksat = arr(6); a.ksat_gw = arr(7); a.Sy = arr(8);
-        a.h = arr(9); a.I = arr(10); a.d = arr(11); a.cum = arr(12); a.maxd = nullptr;
+        // the cumulative recharge map is returned by the per-call entry point only
+        a.h = arr(9); a.I = arr(10); a.d = arr(11); a.cum = nullptr; a.maxd = nullptr;
+        a.statics_ready = 1;
         // the pending recharge / capillary maps are returned by the per-call entry
         // point but read by nothing in the loop: only the net exchange map is kept
         a.pn = arr(14); a.pr = nullptr; a.pc = nullptr; a.z_bed = arr(17); a.Kgw = arr(18);
-        a.rfg = arr(19); a.q = arr(20); a.t1 = arr(21); a.t2 = arr(22); a.t3 = arr(23);
+        a.rfg = arr(19); a.q = arr(20); a.t2 = arr(21);
         hp2d_gw_params &p = s->gwp;
         p.ny = ny;
         p.nx = s->nx;
@@ -1099,15 +1109,6 @@ static hp2d_status ensure_graph(hp2d_sim *s, int steps) {
     return HP2D_OK;
 }
 
-__global__ void cast_to_f64_kernel(const float *src, double *dst, long long n) {
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        dst[i] = (double)src[i];
-}
-__global__ void cast_to_f32_kernel(const double *src, float *dst, long long n) {
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        dst[i] = (float)src[i];
-}
-
 // A step whose wash-off split exceeded the captured sub-steps left the hold
 // flag set (every later launch was a no-op): run its remaining sub-steps, the
 // fold (which clears the flag) and its finalize.
@@ -1132,20 +1133,17 @@ static hp2d_status sim_release_hold(hp2d_sim *s, SimState &st) {
 }
 
 // Groundwater_Module over the model time since its last call, on the current
-// depth and infiltration buffers (FP32 state is widened to FP64 around it).
+// depth and infiltration buffers (an FP32 run's buffers are read as FP64 and
+// rounded back by the module's kernels; the head and its ledgers stay FP64).
 static hp2d_status sim_gw_update(hp2d_sim *s, SimState &st) {
     const int cur = (int)((st.k + 1) & 1);  // the buffers the next step reads
-    hipStream_t gs = hp2d_handle_stream(s->gwh);
     GwmArgs a = s->gwa;
-    const long long n = (long long)s->n;
-    const dim3 G((unsigned)std::min<long long>((n + 255) / 256, 8192)), B(256);
     if (s->elem == 8) {
         a.d = (double *)s->d[cur];
         a.I = (double *)s->I[cur];
-    } else {
-        hipLaunchKernelGGL(cast_to_f64_kernel, G, B, 0, gs, (const float *)s->d[cur], a.d, n);
-        hipLaunchKernelGGL(cast_to_f64_kernel, G, B, 0, gs, (const float *)s->I[cur], a.I, n);
-        STRY(s, hipGetLastError());
+    } else {  // the module reads and writes the FP32 state in place
+        a.d32 = (float *)s->d[cur];
+        a.I32 = (float *)s->I[cur];
     }
     s->gwp.time_step_min = st.gw_elapsed_min;
     s->gwp.t_min = st.t;
@@ -1155,12 +1153,6 @@ static hp2d_status sim_gw_update(hp2d_sim *s, SimState &st) {
     if (rc != HP2D_OK) {
         set_err(s, std::string("groundwater module: ") + hp2d_last_error(s->gwh));
         return rc;
-    }
-    if (s->elem == 4) {
-        hipLaunchKernelGGL(cast_to_f32_kernel, G, B, 0, gs, (const double *)a.d, (float *)s->d[cur], n);
-        hipLaunchKernelGGL(cast_to_f32_kernel, G, B, 0, gs, (const double *)a.I, (float *)s->I[cur], n);
-        STRY(s, hipGetLastError());
-        STRY(s, hipStreamSynchronize(gs));
     }
     s->gw_error = err;
     s->gw_last = st.accepted;

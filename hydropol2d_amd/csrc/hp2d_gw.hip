@@ -202,8 +202,8 @@ __global__ __launch_bounds__(kGwBlock) void gw_velocity_kernel(GwArgs a, const d
             vx = NAN;
             vy = NAN;
         }
-        ux[c] = vx;
-        uy[c] = vy;
+        if (ux) ux[c] = vx;  // the Darcy velocities, when the caller returns them
+        if (uy) uy[c] = vy;
         if (h_masked) h_masked[c] = out ? NAN : h[c];
         const double cx = a.courant * a.dx / fabs(vx), cy = a.courant * a.dy / fabs(vy);
         if (isfinite(cx) && cx > 0.0) best = fmin(best, cx);
@@ -349,7 +349,9 @@ extern "C" hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, double dt, double dx, 
            *dhd = p + 5 * n, *dh0 = p + 6 * n;
     uint8_t *dcm = catchment_mask ? (uint8_t *)(ws + mask_off) : nullptr;
     uint8_t *ddm = dirichlet_mask ? (uint8_t *)(ws + mask_off + n) : nullptr;
-    const GwScratch w = gw_scratch(ws + scr_off, ny, nx);
+    GwScratch w = gw_scratch(ws + scr_off, ny, nx);
+    if (!u_x) w.ux = nullptr;  // velocities only where the caller returns them
+    if (!u_y) w.uy = nullptr;
     hipError_t e = hipSuccess;
     auto H2D = [&](void *dst, const void *src, size_t bytes) {
         if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);

@@ -55,7 +55,13 @@ struct GwmArgs {
     double dt_s, A, l_vg;
     const double *elev, *soil_depth, *ts, *tr, *nv, *alpha, *ksat, *ksat_gw, *Sy;
     double *h, *I, *d, *cum, *maxd, *pn, *pr, *pc;
-    double *z_bed, *Kgw, *rfg, *q, *t1, *t2, *t3;
+    double *z_bed, *Kgw, *rfg, *q;
+    double *t2;  // unsaturated-zone excess of the last update [m]
+    // FP32 runs: d and I in the run's own buffers (NULL: the FP64 arrays above)
+    float *d32 = nullptr, *I32 = nullptr;
+    // z_bed and Kgw already hold elev - soil_depth and ksat_gw / 3.6e6 (the
+    // recharge pass then leaves them alone)
+    int statics_ready = 0;
 };
 
 // One module call on device arrays (the scalars of p, the state g updated);

@@ -20,10 +20,27 @@
 
 namespace hp2d {
 
-// recharge bucket, pending exchange and the scheduler's per-cell terms
-__global__ __launch_bounds__(256) void gwm_recharge_kernel(GwmArgs a) {
-    const long long c = blockIdx.x * 256LL + threadIdx.x;
-    if (c >= a.n) return;
+// The surface depth d [mm] and the vadose storage I [mm] of a cell: the
+// module's FP64 arrays, or an FP32 run's own buffers (read as FP64, rounded
+// once per kernel on the way back)
+__device__ __forceinline__ double gwm_I(const GwmArgs &a, long long c) {
+    return a.I32 ? (double)a.I32[c] : a.I[c];
+}
+__device__ __forceinline__ void gwm_set_I(const GwmArgs &a, long long c, double v) {
+    if (a.I32) a.I32[c] = (float)v;
+    else a.I[c] = v;
+}
+__device__ __forceinline__ double gwm_d(const GwmArgs &a, long long c) {
+    return a.d32 ? (double)a.d32[c] : a.d[c];
+}
+__device__ __forceinline__ void gwm_set_d(const GwmArgs &a, long long c, double v) {
+    if (a.d32) a.d32[c] = (float)v;
+    else a.d[c] = v;
+}
+
+// recharge bucket, pending exchange and the scheduler's per-cell terms of
+// cell c; v receives [recharge volume, pending head change, diffusivity]
+__device__ __forceinline__ void gwm_recharge_cell(const GwmArgs &a, long long c, double (&v)[3]) {
     const bool nan = isnan(a.elev[c]);
     const double zero = nan ? NAN : 0.0;
     const double gw_depth = a.h[c] - (a.elev[c] - a.soil_depth[c]);
@@ -37,7 +54,7 @@ __global__ __launch_bounds__(256) void gwm_recharge_kernel(GwmArgs a) {
     const double nvg = a.nv[c], m = 1.0 - 1.0 / nvg, al = a.alpha[c];
     const double Ks = a.ksat[c] / 1000.0 / 3600.0;
     const double inf_r = isfinite(zero) ? zero : 0.0;
-    const double S0 = a.I[c] / 1000.0;
+    const double S0 = gwm_I(a, c) / 1000.0;
     const double s0 = isfinite(S0) ? S0 : 0.0;
     const double mx = isfinite(uzmax) ? uzmax : 0.0;
     const double min_eff = fmin(0.0, mx);
@@ -54,11 +71,33 @@ __global__ __launch_bounds__(256) void gwm_recharge_kernel(GwmArgs a) {
     theta = fmin(theta, ts);
     double Se = (theta - tr) / fmax(ts - tr, 1e-12);
     Se = fmin(fmax(Se, 1e-6), 1.0);
-    double hs = -(1.0 / al) * pow(pow(Se, -1.0 / m) - 1.0, 1.0 / nvg);
+    double hs, Kr;
+    if (a.I32) {
+        // FP32 run: the van Genuchten head and the Mualem conductivity in FP32
+        // with the hardware log2 / exp2 (their results feed a storage that is
+        // itself FP32; the bucket balance below stays FP64)
+        const float sf = (float)Se, mf = (float)m;
+        const float l2s = __builtin_amdgcn_logf(sf);
+        const float se_inv_m = __builtin_amdgcn_exp2f((-1.0f / mf) * l2s);
+        const float hsf = -(1.0f / (float)al) *
+                          __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(se_inv_m - 1.0f) / (float)nvg);
+        const float term = 1.0f - 1.0f / se_inv_m;
+        const float y = 1.0f - __builtin_amdgcn_exp2f(mf * __builtin_amdgcn_logf(term));
+        const float se_l = a.l_vg == 0.5 ? sqrtf(sf) : __builtin_amdgcn_exp2f((float)a.l_vg * l2s);
+        hs = (double)hsf;
+        Kr = (double)(se_l * (y * y));
+    } else {
+        // Se^(-1/m) is taken once and Se^(1/m) is its reciprocal; the Mualem
+        // factor Se^l is a square root for the usual l = 1/2 (each within an ulp
+        // of pow: the recharge pass is bound by its FP64 powers)
+        const double se_inv_m = pow(Se, -1.0 / m);
+        hs = -(1.0 / al) * pow(se_inv_m - 1.0, 1.0 / nvg);
+        const double term = 1.0 - 1.0 / se_inv_m;
+        const double y = 1.0 - pow(term, m);
+        const double se_l = a.l_vg == 0.5 ? sqrt(Se) : pow(Se, a.l_vg);
+        Kr = se_l * (y * y);  // (.)^2: y*y is the correctly rounded square
+    }
     if (Se >= 0.999999) hs = 0.0;
-    const double term = 1.0 - pow(Se, 1.0 / m);
-    const double y = 1.0 - pow(term, m);
-    double Kr = pow(Se, a.l_vg) * (y * y);  // (.)^2: y*y is the correctly rounded square
     Kr = fmin(fmax(Kr, 0.0), 1.0);
     double grad = 1.0 + hs / Lgw;
     grad = fmax(grad, -1.0);
@@ -72,7 +111,7 @@ __global__ __launch_bounds__(256) void gwm_recharge_kernel(GwmArgs a) {
     if (no_uz) S_new = 0.0;
     double R_mb = inf_r - (S_new - S) / dt;
     if (no_uz) R_mb = 0.0;
-    a.I[c] = S_new * 1000.0;
+    gwm_set_I(a, c, S_new * 1000.0);
     if (a.cum) a.cum[c] = a.cum[c] + fmax(R_mb, 0.0) * dt * 1000.0;
     // exchange (the capillary rate is the zero matrix without layered soil)
     double rr = R_mb - zero;
@@ -81,30 +120,29 @@ __global__ __launch_bounds__(256) void gwm_recharge_kernel(GwmArgs a) {
     a.pn[c] = a.pn[c] + rr * a.dt_s;
     if (a.pr) a.pr[c] = a.pr[c] + down;
     if (a.pc) a.pc[c] = a.pc[c] + fmax(zero, 0.0) * a.dt_s;
-    a.t1[c] = down * (nan ? NAN : a.A);
+    v[0] = down * (nan ? NAN : a.A);
     // scheduler terms: pending head change and diffusivity K H / Sy
     const double Sy_head = fmax(a.Sy[c], 1e-6);
-    a.t2[c] = fabs(a.pn[c]) / fmax(Sy_head, 1e-9);
+    v[1] = fabs(a.pn[c]) / fmax(Sy_head, 1e-9);
     const double zbed = a.elev[c] - a.soil_depth[c];
     double K = a.ksat_gw[c] / 1000.0 / 3600.0;
     if (!isfinite(K)) K = NAN;
     double sat = fmax(a.h[c] - zbed, 1e-4);
     if (nan) sat = NAN;
-    a.t3[c] = K * sat / Sy_head;
-    a.z_bed[c] = zbed;
-    a.Kgw[c] = a.ksat_gw[c] / 1000.0 / 3600.0;
+    v[2] = K * sat / Sy_head;
+    if (!a.statics_ready) {
+        a.z_bed[c] = zbed;
+        a.Kgw[c] = a.ksat_gw[c] / 1000.0 / 3600.0;
+    }
 }
 
-// per column: [0] nansum(t1), [1] max(t2) omitting NaN, [2] max(t3) omitting NaN
-__global__ __launch_bounds__(kColThreads) void gwm_colred_kernel(int ny, int nx, const double *t1,
-                                                                 const double *t2, const double *t3,
-                                                                 double *out) {
+// One block per column: the recharge of its cells and the column's
+// [0] nansum of the recharge volume, [1] NaN-omitting max of the pending head
+// change, [2] that of the diffusivity (the scheduler's inputs)
+__global__ __launch_bounds__(kColThreads) void gwm_recharge_kernel(GwmArgs a, double *col) {
     const bool is_max[3] = {false, true, true};
-    column_reduce<3>(ny, blockIdx.x, is_max, [&](long long c, double (&v)[3]) {
-        v[0] = t1[c];
-        v[1] = t2 ? t2[c] : NAN;
-        v[2] = t3 ? t3[c] : NAN;
-    }, out);
+    column_reduce<3>(a.ny, blockIdx.x, is_max,
+                     [&](long long c, double (&v)[3]) { gwm_recharge_cell(a, c, v); }, col);
 }
 
 // recharge_for_gw (:199-203) and q_exf = zero_matrix
@@ -124,9 +162,9 @@ __global__ __launch_bounds__(256) void gwm_after_bq_kernel(GwmArgs a, const doub
     a.h[c] = h_new[c];
     const double q = q_new[c];
     a.q[c] = q;
-    double dn = a.d[c] + dt_gw * (q * 1000.0);
+    double dn = gwm_d(a, c) + dt_gw * (q * 1000.0);
     if (dn < 0.0) dn = 0.0;
-    a.d[c] = dn;
+    gwm_set_d(a, c, dn);
 }
 
 // local water-table update without baseflow (:252-263)
@@ -139,35 +177,41 @@ __global__ __launch_bounds__(256) void gwm_local_kernel(GwmArgs a, double dt_gw)
     hc = fmax(hc, a.z_bed[c]);
     const double ex_water = fmax(hc - a.elev[c], 0.0) * Sy_l;
     a.h[c] = fmin(hc, a.elev[c]);
-    double dn = a.d[c] + ex_water * 1000.0;
+    double dn = gwm_d(a, c) + ex_water * 1000.0;
     if (dn < 0.0) dn = 0.0;
-    a.d[c] = dn;
+    gwm_set_d(a, c, dn);
 }
 
-// after an update: exfiltrated volume, pending reset, unsaturated-zone excess (:266-395)
-__global__ __launch_bounds__(256) void gwm_post_kernel(GwmArgs a, double dt_gw) {
-    const long long c = blockIdx.x * 256LL + threadIdx.x;
-    if (c >= a.n) return;
-    const bool nan = isnan(a.elev[c]);
-    a.t1[c] = fmax(a.q[c], 0.0) * dt_gw * (nan ? NAN : a.A);
-    a.pn[c] = 0.0;
-    if (a.pr) a.pr[c] = 0.0;
-    if (a.pc) a.pc[c] = 0.0;
-    double zwt = a.elev[c] - a.h[c];
-    zwt = fmax(zwt, 0.0);
-    zwt = fmin(zwt, a.soil_depth[c]);
-    const double uz_new = zwt * (a.ts[c] - a.tr[c]);
-    a.t2[c] = fmax(a.I[c] / 1000.0 - uz_new, 0.0);  // excess [m]
+// after an update: exfiltrated volume, pending reset, unsaturated-zone excess
+// (:266-395), one block per column with the column's [0] nansum of the
+// exfiltrated volume and [1] NaN-omitting max of the excess ([2] unused)
+__global__ __launch_bounds__(kColThreads) void gwm_post_kernel(GwmArgs a, double dt_gw, double *col) {
+    const bool is_max[3] = {false, true, true};
+    column_reduce<3>(a.ny, blockIdx.x, is_max, [&](long long c, double (&v)[3]) {
+        const bool nan = isnan(a.elev[c]);
+        v[0] = fmax(a.q[c], 0.0) * dt_gw * (nan ? NAN : a.A);
+        a.pn[c] = 0.0;
+        if (a.pr) a.pr[c] = 0.0;
+        if (a.pc) a.pc[c] = 0.0;
+        double zwt = a.elev[c] - a.h[c];
+        zwt = fmax(zwt, 0.0);
+        zwt = fmin(zwt, a.soil_depth[c]);
+        const double uz_new = zwt * (a.ts[c] - a.tr[c]);
+        const double ex = fmax(gwm_I(a, c) / 1000.0 - uz_new, 0.0);  // excess [m]
+        a.t2[c] = ex;
+        v[1] = ex;
+        v[2] = NAN;
+    }, col);
 }
 
 __global__ __launch_bounds__(256) void gwm_excess_kernel(GwmArgs a) {
     const long long c = blockIdx.x * 256LL + threadIdx.x;
     if (c >= a.n) return;
-    const double Sm = a.I[c] / 1000.0, ex = a.t2[c];
-    a.I[c] = fmax((Sm - ex) * 1000.0, 0.0);
-    double dn = a.d[c] + ex * 1000.0;
+    const double Sm = gwm_I(a, c) / 1000.0, ex = a.t2[c];
+    gwm_set_I(a, c, fmax((Sm - ex) * 1000.0, 0.0));
+    double dn = gwm_d(a, c) + ex * 1000.0;
     if (dn < 0.0) dn = 0.0;
-    a.d[c] = dn;
+    gwm_set_d(a, c, dn);
 }
 
 __global__ __launch_bounds__(256) void gwm_maxdepth_kernel(GwmArgs a) {
@@ -210,9 +254,7 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
             m3 = std::fmax(m3, hc[3 * (size_t)j + 2]);
         }
     };
-    hipLaunchKernelGGL(gwm_recharge_kernel, G, B, 0, s, a);
-    hipLaunchKernelGGL(gwm_colred_kernel, dim3(nx), dim3(kColThreads), 0, s, ny, nx, (const double *)a.t1,
-                       (const double *)a.t2, (const double *)a.t3, col);
+    hipLaunchKernelGGL(gwm_recharge_kernel, dim3(nx), dim3(kColThreads), 0, s, a, col);
     e = hipGetLastError();
     D2H(hc.data(), col, hc.size() * 8);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -247,7 +289,8 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
         if (p->flag_baseflow == 1) {
             char *wsb = (char *)hp2d_handle_workspace(h, gw_scratch_bytes(ny, nx) + 256);
             if (!wsb) return HP2D_E_NOMEM;
-            const GwScratch w = gw_scratch(wsb, ny, nx);
+            GwScratch w = gw_scratch(wsb, ny, nx);
+            w.ux = w.uy = nullptr;  // the module does not return the Darcy velocities
             GwArgs ga;
             ga.ny = ny;
             ga.nx = nx;
@@ -276,9 +319,7 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
             e = hipGetLastError();
         }
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(gwm_post_kernel, G, B, 0, s, a, dt_gw);
-            hipLaunchKernelGGL(gwm_colred_kernel, dim3(nx), dim3(kColThreads), 0, s, ny, nx, (const double *)a.t1,
-                               (const double *)a.t2, (const double *)nullptr, col);
+            hipLaunchKernelGGL(gwm_post_kernel, dim3(nx), dim3(kColThreads), 0, s, a, dt_gw, col);
             e = hipGetLastError();
         }
         D2H(hc.data(), col, hc.size() * 8);
@@ -344,8 +385,8 @@ extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_par
     if (ny != p->ny || nx != p->nx || (p->flag_baseflow == 1 && (ny < 2 || nx < 2))) return HP2D_E_ARG;
     const size_t n = (size_t)ny * nx, nb = n * sizeof(double);
     hipStream_t s = hp2d_handle_stream(h);
-    // module arrays in the second workspace: 9 inputs, 8 state, 7 temporaries | colsums | mask
-    const size_t n_arr = 24;
+    // module arrays in the second workspace: 9 inputs, 8 state, 5 temporaries | colsums | mask
+    const size_t n_arr = 22;
     const size_t col_off = n_arr * nb;
     const size_t mask_off = col_off + 3 * (size_t)nx * sizeof(double);
     char *ws = (char *)hp2d_handle_workspace2(h, mask_off + n + 256);
@@ -361,8 +402,7 @@ extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_par
     a.ksat = in[6]; a.ksat_gw = in[7]; a.Sy = in[8];
     a.h = arr(9); a.I = arr(10); a.d = arr(11); a.cum = cumulative_recharge ? arr(12) : nullptr;
     a.maxd = max_gw_depth ? arr(13) : nullptr; a.pn = arr(14); a.pr = arr(15); a.pc = arr(16);
-    a.z_bed = arr(17); a.Kgw = arr(18); a.rfg = arr(19); a.q = arr(20); a.t1 = arr(21);
-    a.t2 = arr(22); a.t3 = arr(23);
+    a.z_bed = arr(17); a.Kgw = arr(18); a.rfg = arr(19); a.q = arr(20); a.t2 = arr(21);
     double *col = (double *)(ws + col_off);
     uint8_t *dom = p->domain ? (uint8_t *)(ws + mask_off) : nullptr;
     hipError_t e = hipSuccess;

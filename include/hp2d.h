@@ -74,6 +74,7 @@ const char *hp2d_global_last_error(void);
 
 /* Create a handle for grids of ny x nx cells computing in `precision`
  * (HP2D_FP64 = MATLAB double, HP2D_FP32 = MATLAB single / flag_single).
+ * ny * nx must stay below 2^32 - 2^16 (HP2D_E_ARG otherwise).
  * The handle owns device buffers sized for the grid and one HIP stream.
  * Replaces the per-call allocations of Local_Inertial_Model_D4.m:73-84,145-149. */
 hp2d_status hp2d_create(int32_t ny, int32_t nx, int32_t precision, hp2d_handle **out);
@@ -374,7 +375,8 @@ typedef struct hp2d_run_report {
 
 typedef struct hp2d_sim hp2d_sim;
 
-/* Upload a run; allocates every device buffer once. */
+/* Upload a run; allocates every device buffer once.  ny * nx must stay below
+ * 2^32 - 2^16 (HP2D_E_ARG otherwise). */
 hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                             hp2d_sim **out);
 void hp2d_sim_destroy(hp2d_sim *s);
