@@ -40,7 +40,8 @@ __device__ __forceinline__ void gwm_set_d(const GwmArgs &a, long long c, double 
 
 // recharge bucket, pending exchange and the scheduler's per-cell terms of
 // cell c; v receives [recharge volume, pending head change, diffusivity]
-__device__ __forceinline__ void gwm_recharge_cell(const GwmArgs &a, long long c, double (&v)[3]) {
+__device__ __forceinline__ void gwm_recharge_cell(const GwmArgs &a, long long c, double dt_rfg,
+                                                  double (&v)[3]) {
     const bool nan = isnan(a.elev[c]);
     const double zero = nan ? NAN : 0.0;
     const double gw_depth = a.h[c] - (a.elev[c] - a.soil_depth[c]);
@@ -117,13 +118,18 @@ __device__ __forceinline__ void gwm_recharge_cell(const GwmArgs &a, long long c,
     double rr = R_mb - zero;
     if (nan) rr = NAN;
     const double down = fmax(R_mb, 0.0) * a.dt_s;
-    a.pn[c] = a.pn[c] + rr * a.dt_s;
+    const double pn = a.pn[c] + rr * a.dt_s;
+    a.pn[c] = pn;
     if (a.pr) a.pr[c] = a.pr[c] + down;
     if (a.pc) a.pc[c] = a.pc[c] + fmax(zero, 0.0) * a.dt_s;
+    if (dt_rfg > 0.0) {  // the update is known to follow: recharge_for_gw and q_exf (:199-203)
+        a.rfg[c] = nan ? NAN : pn / dt_rfg;
+        a.q[c] = nan ? NAN : 0.0;
+    }
     v[0] = down * (nan ? NAN : a.A);
     // scheduler terms: pending head change and diffusivity K H / Sy
     const double Sy_head = fmax(a.Sy[c], 1e-6);
-    v[1] = fabs(a.pn[c]) / fmax(Sy_head, 1e-9);
+    v[1] = fabs(pn) / fmax(Sy_head, 1e-9);
     const double zbed = a.elev[c] - a.soil_depth[c];
     double K = a.ksat_gw[c] / 1000.0 / 3600.0;
     if (!isfinite(K)) K = NAN;
@@ -138,11 +144,14 @@ __device__ __forceinline__ void gwm_recharge_cell(const GwmArgs &a, long long c,
 
 // One block per column: the recharge of its cells and the column's
 // [0] nansum of the recharge volume, [1] NaN-omitting max of the pending head
-// change, [2] that of the diffusivity (the scheduler's inputs)
-__global__ __launch_bounds__(kColThreads) void gwm_recharge_kernel(GwmArgs a, double *col) {
+// change, [2] that of the diffusivity (the scheduler's inputs).  dt_rfg > 0:
+// the update's interval when the update is known to follow (synchronous
+// module), whose recharge rate and zeroed exfiltration are then written here.
+__global__ __launch_bounds__(kColThreads) void gwm_recharge_kernel(GwmArgs a, double dt_rfg,
+                                                                  double *col) {
     const bool is_max[3] = {false, true, true};
     column_reduce<3>(a.ny, blockIdx.x, is_max,
-                     [&](long long c, double (&v)[3]) { gwm_recharge_cell(a, c, v); }, col);
+                     [&](long long c, double (&v)[3]) { gwm_recharge_cell(a, c, dt_rfg, v); }, col);
 }
 
 // recharge_for_gw (:199-203) and q_exf = zero_matrix
@@ -184,9 +193,11 @@ __global__ __launch_bounds__(256) void gwm_local_kernel(GwmArgs a, double dt_gw)
 
 // after an update: exfiltrated volume, pending reset, unsaturated-zone excess
 // (:266-395), one block per column with the column's [0] nansum of the
-// exfiltrated volume and [1] NaN-omitting max of the excess ([2] unused)
+// exfiltrated volume, [1] NaN-omitting max of the excess and [2] nansum of the
+// aquifer storage A*Sy*(h - z_bed) (the head is final here; the pending term
+// of the storage is zero after the reset)
 __global__ __launch_bounds__(kColThreads) void gwm_post_kernel(GwmArgs a, double dt_gw, double *col) {
-    const bool is_max[3] = {false, true, true};
+    const bool is_max[3] = {false, true, false};
     column_reduce<3>(a.ny, blockIdx.x, is_max, [&](long long c, double (&v)[3]) {
         const bool nan = isnan(a.elev[c]);
         v[0] = fmax(a.q[c], 0.0) * dt_gw * (nan ? NAN : a.A);
@@ -200,7 +211,8 @@ __global__ __launch_bounds__(kColThreads) void gwm_post_kernel(GwmArgs a, double
         const double ex = fmax(gwm_I(a, c) / 1000.0 - uz_new, 0.0);  // excess [m]
         a.t2[c] = ex;
         v[1] = ex;
-        v[2] = NAN;
+        const double area = nan ? NAN : a.A;
+        v[2] = area * a.Sy[c] * (a.h[c] - (a.elev[c] - a.soil_depth[c]));
     }, col);
 }
 
@@ -254,7 +266,11 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
             m3 = std::fmax(m3, hc[3 * (size_t)j + 2]);
         }
     };
-    hipLaunchKernelGGL(gwm_recharge_kernel, dim3(nx), dim3(kColThreads), 0, s, a, col);
+    // synchronous module: the update follows every call, over the interval known now
+    const bool sync = p->flag_async == 0;
+    const double dt_sync = std::fmax(g->elapsed_s + a.dt_s, DBL_EPSILON);
+    hipLaunchKernelGGL(gwm_recharge_kernel, dim3(nx), dim3(kColThreads), 0, s, a, sync ? dt_sync : 0.0,
+                       col);
     e = hipGetLastError();
     D2H(hc.data(), col, hc.size() * 8);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -283,9 +299,13 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
     const bool due = p->flag_async == 0 ? true : (g->elapsed_s >= target || head_due || final_step);
     const double dt_gw = std::fmax(g->elapsed_s, DBL_EPSILON);
     double err = 0.0;
+    double s_after = 0.0;       // aquifer storage after the update (post pass)
+    bool have_storage = false;
     if (due) {
-        hipLaunchKernelGGL(gwm_rfg_kernel, G, B, 0, s, a, dt_gw);
-        e = hipGetLastError();
+        if (!sync) {
+            hipLaunchKernelGGL(gwm_rfg_kernel, G, B, 0, s, a, dt_gw);
+            e = hipGetLastError();
+        }
         if (p->flag_baseflow == 1) {
             char *wsb = (char *)hp2d_handle_workspace(h, gw_scratch_bytes(ny, nx) + 256);
             if (!wsb) return HP2D_E_NOMEM;
@@ -326,8 +346,13 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess)
             return hp2d_handle_fail(h, std::string("groundwater module: ") + hipGetErrorString(e));
-        double exf, ex_max, unused;
-        fold(exf, ex_max, unused);
+        double exf = 0.0, ex_max = NAN;
+        for (int j = 0; j < nx; ++j) {
+            exf += hc[3 * (size_t)j];
+            ex_max = std::fmax(ex_max, hc[3 * (size_t)j + 1]);
+            s_after += hc[3 * (size_t)j + 2];
+        }
+        have_storage = true;
         g->total_exfiltration_m3 = g->total_exfiltration_m3 + exf;
         g->n_updates += 1;
         g->elapsed_s = 0.0;
@@ -341,11 +366,11 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
         hipLaunchKernelGGL(gwm_rfg_kernel, G, B, 0, s, a, 1.0);  // q_exf = zero_matrix
         e = hipGetLastError();
     }
-    if (e == hipSuccess) {
+    if (e == hipSuccess && a.maxd) {
         hipLaunchKernelGGL(gwm_maxdepth_kernel, G, B, 0, s, a);
         e = hipGetLastError();
     }
-    if (s_gw && e == hipSuccess) {
+    if (s_gw && !have_storage && e == hipSuccess) {
         hipLaunchKernelGGL(gwm_storage_kernel, dim3(nx), dim3(kColThreads), 0, s, a, col);
         e = hipGetLastError();
         D2H(hc.data(), col, 2 * (size_t)nx * 8);
@@ -353,7 +378,9 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess)
         return hp2d_handle_fail(h, std::string("groundwater module: ") + hipGetErrorString(e));
-    if (s_gw) {  // nansum(nansum(A.*Sy.*(h - z_bed))) + nansum(nansum(A.*pending))
+    if (s_gw && have_storage) {  // the pending term is zero after the update
+        *s_gw = s_after + 0.0;
+    } else if (s_gw) {  // nansum(nansum(A.*Sy.*(h - z_bed))) + nansum(nansum(A.*pending))
         double sg = 0.0, sp = 0.0;
         for (int j = 0; j < nx; ++j) {
             sg += hc[2 * (size_t)j];
