@@ -43,6 +43,12 @@ def parse():
     p.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     p.add_argument("--ensemble-members", type=int, default=64,
                    help="members of the batched-ensemble measurement (0 skips it)")
+    p.add_argument("--ensemble-max-log2", type=int, default=9,
+                   help="largest member side 2^k (sides log-uniform from 2^7; 10 = the 1024^2 of "
+                        "the 512-member configuration)")
+    p.add_argument("--ensemble-plain-sample", type=int, default=0, metavar="K",
+                   help="time the plain per-member solver on every (members/K)-th member only and "
+                        "scale by cell-steps (0: every member)")
     p.add_argument("--surrogate-train", type=int, default=0, metavar="N",
                    help="also run the surrogate study: train on N solved 256 x 256 members, judge "
                         "256 more (0 skips it; about 2.5 min at N = 4096)")
@@ -81,38 +87,54 @@ def calibrate_copy(mib):
     print(f"[bench] copy calibration: 5 copies of {n * 4} bytes", flush=True)
 
 
-def ensemble_throughput(n_members):
+def ensemble_throughput(n_members, max_log2=9, plain_sample=0):
     """Members/hour of the batched ensemble against the plain per-member GPU
     solver (the same device loop, one member after another), on C4-style
-    members: grid sides log-uniform in [128, 512], dx = 5 m, perturbed n, Ks
-    and rainfall, 2 h simulated each."""
+    members: grid sides log-uniform in [128, 2^max_log2], dx = 5 m, perturbed
+    n, Ks and rainfall, 2 h simulated each.  With plain_sample = K the plain
+    solver runs every (n/K)-th member only, and its time for all members is
+    that sample's time scaled by the cell-steps of all members over the
+    sample's (both counted from the batched run)."""
     from hydropol2d_amd import synthetic
     from hydropol2d_amd.simulation import Ensemble, Simulation
-    specs = synthetic.ensemble_c4(n_members, seed=1000, min_log2=7, max_log2=9,
+    specs = synthetic.ensemble_c4(n_members, seed=1000, min_log2=7, max_log2=max_log2,
                                   routing_time_min=120.0)
-    cells = sum(sp.shape[0] * sp.shape[1] for sp in specs)
+    sizes = [sp.shape[0] * sp.shape[1] for sp in specs]
     with Ensemble(specs) as ens:
         t0 = time.perf_counter()
         ens.run(graph_steps=50)
         t_batch = time.perf_counter() - t0
         peaks = [ens.member(i).state()["dmax"] for i in range(len(specs))]
         steps = [ens.member(i).report()["steps"] for i in range(len(specs))]
-    t_plain = 0.0
+    cell_steps = [c * k for c, k in zip(sizes, steps)]
+    stride = max(1, n_members // plain_sample) if plain_sample > 0 else 1
+    sample = list(range(0, n_members, stride))
+    t_sample = 0.0
     max_diff = 0.0
-    for i, sp in enumerate(specs):
-        with Simulation(sp) as sim:
+    print(f"[bench] ensemble: batched run of {n_members} members in {t_batch:.2f} s",
+          file=sys.stderr, flush=True)
+    for k, i in enumerate(sample):
+        with Simulation(specs[i]) as sim:
             t0 = time.perf_counter()
             sim.run()
-            t_plain += time.perf_counter() - t0
-            max_diff = max(max_diff, float(np.nanmax(np.abs(sim.state()["dmax"] - peaks[i]))) / 1000)
-    print(f"[bench] ensemble {n_members} members: batched {t_batch:.2f} s, plain {t_plain:.2f} s",
-          file=sys.stderr, flush=True)
-    return {"members": n_members, "cells": cells, "member_steps": int(sum(steps)),
+            t_sample += time.perf_counter() - t0
+            max_diff = max(max_diff,
+                           float(np.nanmax(np.abs(sim.state()["dmax"] - peaks[i]))) / 1000)
+        if (k + 1) % 16 == 0:
+            print(f"[bench] ensemble: {k + 1} of {len(sample)} members run alone ({t_sample:.1f} s)",
+                  file=sys.stderr, flush=True)
+    t_plain = t_sample * sum(cell_steps) / sum(cell_steps[i] for i in sample)
+    print(f"[bench] ensemble {n_members} members: batched {t_batch:.2f} s, plain {t_plain:.2f} s "
+          f"({len(sample)} members timed)", file=sys.stderr, flush=True)
+    return {"members": n_members, "cells": int(sum(sizes)), "member_steps": int(sum(steps)),
+            "cell_updates": int(sum(cell_steps)),
+            "cell_updates_per_s_batched": sum(cell_steps) / t_batch,
             "members_per_hour_batched": n_members / t_batch * 3600.0,
             "members_per_hour_plain": n_members / t_plain * 3600.0,
-            "speedup": t_plain / t_batch, "max_peak_depth_diff_m": max_diff,
-            "workload": "C4-style: sides 2^U(7,9), dx=5 m, fractal DEM on a 1 % tilt, n/Ks/rain "
-                        "x e^U(ln .5, ln 1.5) around C1, 2 h each, FP32"}
+            "speedup": t_plain / t_batch, "plain_members_timed": len(sample),
+            "max_peak_depth_diff_m": max_diff,
+            "workload": f"C4-style: sides 2^U(7,{max_log2}), dx=5 m, fractal DEM on a 1 % tilt, "
+                        "n/Ks/rain x e^U(ln .5, ln 1.5) around C1, 2 h each, FP32"}
 
 
 def surrogate_study(n_train, epochs, n_eval=256, n_plain=8):
@@ -125,14 +147,18 @@ def surrogate_study(n_train, epochs, n_eval=256, n_plain=8):
     from hydropol2d_amd import surrogate as S
     from hydropol2d_amd import synthetic
     from hydropol2d_amd.simulation import Simulation
+    def log(msg):
+        print(f"[bench] surrogate: {msg}", file=sys.stderr, flush=True)
+
     specs = synthetic.ensemble_c4(n_train + n_eval, seed=5000, min_log2=8, max_log2=8,
                                   routing_time_min=120.0, vary_soil_moisture=True, filled=True)
-    pk, tp, t_data = S.solve_members(specs)
+    pk, tp, t_data = S.solve_members(specs, log=log)
     net = S.UNetSurrogate(256, 256, batch=32, seed=1)
     t0 = time.perf_counter()
     x = S.stack_inputs(net, specs[:n_train])
     y = S.stack_targets(net, pk[:n_train], tp[:n_train])
-    S.train(net, x, y, epochs=epochs, batch=32, lr=1e-3, lr_final=5e-5)
+    log(f"input maps of {n_train} members in {time.perf_counter() - t0:.1f} s")
+    S.train(net, x, y, epochs=epochs, batch=32, lr=1e-3, lr_final=5e-5, log=log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     del x, y
@@ -248,7 +274,9 @@ def main():
                          f"oracle, single thread, {cpu_s:.1f} s"}
         print(f"[bench] cpu oracle {cpu['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
 
-    ens = ensemble_throughput(args.ensemble_members) if args.ensemble_members > 0 else None
+    ens = (ensemble_throughput(args.ensemble_members, args.ensemble_max_log2,
+                               args.ensemble_plain_sample)
+           if args.ensemble_members > 0 else None)
     sur = (surrogate_study(args.surrogate_train, args.surrogate_epochs)
            if args.surrogate_train > 0 else None)
 

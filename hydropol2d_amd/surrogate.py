@@ -106,6 +106,12 @@ class UNetSurrogate:
         self.G = {}      # gradient buffers, allocated on first use
         self._x = None   # the input of the last forward pass
 
+    def reserve(self, batch):
+        """Grow the activation buffers to hold `batch` members per forward pass
+        (inference in large batches after training in small ones)."""
+        if batch > self.batch_cap:
+            self._alloc(batch)
+
     def _g(self, name, shape):
         torch = _torch()
         t = self.G.get(name)
@@ -308,38 +314,50 @@ DEPTH_SCALE = 100.0
 TPEAK_SCALE = 0.1
 
 
-def solve_members(specs, graph_steps=50):
-    """Run the members as one batched ensemble; returns FP32 arrays
-    (peak depth [m], time to peak [h]) of shape [n, H, W] and the wall time."""
+def solve_members(specs, graph_steps=50, chunk=4096, log=None):
+    """Run the members as batched ensembles of up to `chunk` members; returns
+    FP32 arrays (peak depth [m], time to peak [h]) of shape [n, H, W] and the
+    wall time.  `log` (optional) receives a line after each batch."""
     import time
     from .simulation import Ensemble
     t0 = time.perf_counter()
-    with Ensemble(specs) as ens:
-        ens.record_time_to_peak()
-        ens.run(graph_steps=graph_steps)
-        peaks, tpk = [], []
-        for i in range(len(specs)):
-            m = ens.member(i)
-            peaks.append(m.state()["dmax"] / 1000.0)
-            tpk.append(m.time_to_peak() / 60.0)
+    peaks, tpk = [], []
+    for c0 in range(0, len(specs), chunk):
+        with Ensemble(specs[c0:c0 + chunk]) as ens:
+            ens.record_time_to_peak()
+            ens.run(graph_steps=graph_steps)
+            for i in range(min(chunk, len(specs) - c0)):
+                m = ens.member(i)
+                peaks.append(m.state()["dmax"] / 1000.0)
+                tpk.append(m.time_to_peak() / 60.0)
+        if log:
+            log(f"solved {len(peaks)} of {len(specs)} members ({time.perf_counter() - t0:.1f} s)")
     return (np.asarray(peaks, dtype=np.float32), np.asarray(tpk, dtype=np.float32),
             time.perf_counter() - t0)
 
 
-def stack_inputs(net, specs, workers=8):
+def stack_inputs(net, specs, workers=8, chunk=512):
     """Device bf16 input tensor [n, H, W, 32] of the members (N_IN maps, zero padding).
     The host maps of the members are built on `workers` threads (the library's
-    host routines release the interpreter lock)."""
+    host routines release the interpreter lock), `chunk` members at a time, so
+    that only the bf16 stack is held whole (a 20 000-member stack at 256 x 256
+    is 84 GB in bf16, twice that in FP32)."""
     from concurrent.futures import ThreadPoolExecutor
     torch = _torch()
-    x = np.zeros((len(specs), net.H, net.W, N_PAD), dtype=np.float32)
-
-    def fill(i):
-        x[i, :, :, :N_IN] = member_features(specs[i])
-
+    n = len(specs)
+    out = torch.empty((n, net.H, net.W, N_PAD), dtype=torch.bfloat16, device=net.dev)
+    x = np.zeros((min(chunk, n), net.H, net.W, N_PAD), dtype=np.float32)
     with ThreadPoolExecutor(max_workers=max(1, workers)) as pool:
-        list(pool.map(fill, range(len(specs))))
-    return net.to_bf16(torch.from_numpy(x).to(net.dev))
+        for c0 in range(0, n, chunk):
+            m = min(chunk, n - c0)
+
+            def fill(i):
+                x[i, :, :, :N_IN] = member_features(specs[c0 + i])
+
+            list(pool.map(fill, range(m)))
+            xd = torch.from_numpy(x[:m]).to(net.dev)
+            net._chk(net.L.hp2d_nn_to_bf16(_p(xd), xd.numel(), _p(out[c0:c0 + m]), net._s()))
+    return out
 
 
 def stack_targets(net, peak_m, tpeak_h):
@@ -372,7 +390,7 @@ def train(net, x, y, epochs=10, batch=32, lr=1e-3, seed=0, log=None, lr_final=No
     return hist
 
 
-def judged_predictions(net, specs, judge_fraction=0.05, tol_m=0.02, seed=0):
+def judged_predictions(net, specs, judge_fraction=0.05, tol_m=0.02, seed=0, batch=1024):
     """Surrogate peak-depth maps for all members, judged by the solver.
 
     A `judge_fraction` sample of the members is re-run by the plain solver;
@@ -385,6 +403,7 @@ def judged_predictions(net, specs, judge_fraction=0.05, tol_m=0.02, seed=0):
     n = len(specs)
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
+    net.reserve(min(batch, n))  # inference in batches of up to `batch` members
     x = stack_inputs(net, specs)
     pred = net.predict(x)
     torch.cuda.synchronize()
