@@ -881,7 +881,8 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         A_((void **)&s->wq_ctl, sizeof(WqCtl));
     }
     if (s->gw) {
-        A_((void **)&s->gw_mem, 22 * n * sizeof(double));
+        // 22 FP64 maps, then the soil-class tables (5 x kMaxSoilClasses)
+        A_((void **)&s->gw_mem, (22 * n + 5 * (size_t)kMaxSoilClasses) * sizeof(double));
         A_((void **)&s->gw_col, 3 * (size_t)s->nx * sizeof(double));
         A_((void **)&s->gw_dom, n);
     }
@@ -966,15 +967,11 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         up(arr(14), pend.data(), n * sizeof(double));
         up(arr(15), pend.data(), n * sizeof(double));
         up(arr(16), pend.data(), n * sizeof(double));
-        // the aquifer bed and the aquifer conductivity [m/s] are static: set once
-        // here, so the module's recharge pass does not rewrite them every call
-        std::vector<double> zbed(n), kgw(n);
-        for (size_t c = 0; c < n; ++c) {
-            zbed[c] = gw_host[0 * n + c] - gw_host[1 * n + c];
-            kgw[c] = gw_host[7 * n + c] / 1000.0 / 3600.0;
-        }
+        // the aquifer bed is static: set once here, so the module's recharge pass
+        // does not rewrite it every call
+        std::vector<double> zbed(n);
+        for (size_t c = 0; c < n; ++c) zbed[c] = gw_host[0 * n + c] - gw_host[1 * n + c];
         up(arr(17), zbed.data(), n * sizeof(double));
-        up(arr(18), kgw.data(), n * sizeof(double));
         std::vector<uint8_t> dom(n);
         for (size_t c = 0; c < n; ++c) dom[c] = std::isnan(z[c]) ? 0 : 1;
         up(s->gw_dom, dom.data(), n);
@@ -983,13 +980,47 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         a.nx = s->nx;
         a.n = (long long)n;
         a.elev = arr(0); a.soil_depth = arr(1); a.ts = arr(2); a.tr = arr(3); a.nv = arr(4);
-        a.alpha = arr(5); a.ksat = arr(6); a.ksat_gw = arr(7); a.Sy = arr(8);
+        a.alpha = arr(5); a.ksat = arr(6);
+        // a soil depth that is one value over the grid is passed as that value
+        bool uniform_depth = true;
+        for (size_t c = 1; c < n && uniform_depth; ++c)
+            uniform_depth = std::memcmp(&in->gw_soil_depth[c], &in->gw_soil_depth[0], sizeof(double)) == 0;
+        if (uniform_depth) {
+            a.soil_depth = nullptr;
+            a.soil_depth_u = in->gw_soil_depth[0];
+        }
+        if (s->soil) {
+            // soil-class runs: the recharge and post passes read the class byte and
+            // five per-class values instead of five FP64 maps
+            double *tab = g + 22 * n;
+            std::vector<double> t(5 * (size_t)kMaxSoilClasses, 0.0);
+            for (int k = 0; k < cfg->n_soil_classes; ++k) {
+                t[0 * kMaxSoilClasses + k] = in->gw_theta_sat[k];
+                t[1 * kMaxSoilClasses + k] = in->gw_theta_r[k];
+                t[2 * kMaxSoilClasses + k] = in->gw_n_vg[k];
+                t[3 * kMaxSoilClasses + k] = in->gw_alpha_vg[k];
+                t[4 * kMaxSoilClasses + k] = in->ksat_mmh[k];
+            }
+            up(tab, t.data(), t.size() * sizeof(double));
+            a.soil_class = s->soil;
+            a.cls_ts = tab;
+            a.cls_tr = tab + kMaxSoilClasses;
+            a.cls_nv = tab + 2 * kMaxSoilClasses;
+            a.cls_alpha = tab + 3 * kMaxSoilClasses;
+            a.cls_ksat = tab + 4 * kMaxSoilClasses;
+        }
+        // the aquifer's conductivity and specific yield are uniform in a run:
+        // the kernels take the values instead of reading two FP64 maps per pass
+        a.ksat_gw = nullptr; a.Sy = nullptr;
+        a.ksat_gw_u = in->gw_ksat_mmh;
+        a.Sy_u = in->gw_Sy;
+        a.Kgw_u = in->gw_ksat_mmh / 1000.0 / 3600.0;
         // the cumulative recharge map is returned by the per-call entry point only
         a.h = arr(9); a.I = arr(10); a.d = arr(11); a.cum = nullptr; a.maxd = nullptr;
         a.statics_ready = 1;
         // the pending recharge / capillary maps are returned by the per-call entry
         // point but read by nothing in the loop: only the net exchange map is kept
-        a.pn = arr(14); a.pr = nullptr; a.pc = nullptr; a.z_bed = arr(17); a.Kgw = arr(18);
+        a.pn = arr(14); a.pr = nullptr; a.pc = nullptr; a.z_bed = arr(17); a.Kgw = nullptr;
         a.rfg = arr(19); a.q = arr(20); a.t2 = arr(21);
         hp2d_gw_params &p = s->gwp;
         p.ny = ny;

@@ -98,10 +98,10 @@ __global__ __launch_bounds__(kGwBlock) void gw_flux_kernel(GwArgs a, const doubl
         const long long r = c + ny;
         const double Hr = fmax(h[r] - a.z0[r], 0.0);
         const double Hx = 0.5 * (H + Hr);
-        const double Kx = 0.5 * (a.K[c] + a.K[r]);
+        const double Kx = 0.5 * (gw_K(a, c) + gw_K(a, r));
         const double dHdx = (h[r] - h[c]) / a.dx;
         double F = -Kx * Hx * dHdx;
-        const double cap = a.Sy[c] * H / dt;
+        const double cap = gw_Sy(a, c) * H / dt;
         F = signum(F) * fmin(fabs(F), cap);
         if (isnan(Hr)) F = 0.0;
         fx = F;
@@ -110,10 +110,10 @@ __global__ __launch_bounds__(kGwBlock) void gw_flux_kernel(GwArgs a, const doubl
         const long long d = c + 1;
         const double Hd = fmax(h[d] - a.z0[d], 0.0);
         const double Hy = 0.5 * (H + Hd);
-        const double Ky = 0.5 * (a.K[c] + a.K[d]);
+        const double Ky = 0.5 * (gw_K(a, c) + gw_K(a, d));
         const double dHdy = (h[d] - h[c]) / a.dy;
         double F = -Ky * Hy * dHdy;
-        const double cap = a.Sy[c] * H / dt;
+        const double cap = gw_Sy(a, c) * H / dt;
         F = signum(F) * fmin(fabs(F), cap);
         if (isnan(Hd)) F = 0.0;
         fy = F;
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kGwBlock) void gw_predict_kernel(GwArgs a, const do
     int i, j;
     gw_rc(c, a.ny, i, j);
     const double dt = a.ctl->dt_used;
-    double v = h_t[c] + dt / a.Sy[c] * (-gw_div(a, Fx, Fy, c, i, j));
+    double v = h_t[c] + dt / gw_Sy(a, c) * (-gw_div(a, Fx, Fy, c, i, j));
     if (a.dirichlet && a.dirichlet[c]) v = a.h_dir[c];
     h_p[c] = fmax(v, a.z0[c]);
 }
@@ -170,11 +170,11 @@ __global__ __launch_bounds__(kGwBlock) void gw_update_kernel(GwArgs a, const dou
     int i, j;
     gw_rc(c, a.ny, i, j);
     const double dt = a.ctl->dt_used;
-    double v = h_t[c] + dt / a.Sy[c] * (-gw_div(a, Fx, Fy, c, i, j) + a.R[c]);
+    double v = h_t[c] + dt / gw_Sy(a, c) * (-gw_div(a, Fx, Fy, c, i, j) + a.R[c]);
     if (a.dirichlet && a.dirichlet[c]) v = a.h_dir[c];
     v = fmax(v, a.z0[c]);
-    const double hs = a.z0[c] + a.h_soil[c];
-    double q = fmax(0.0, v - hs) * a.Sy[c] / dt;
+    const double hs = a.z0[c] + (a.h_soil ? a.h_soil[c] : a.h_soil_u);
+    double q = fmax(0.0, v - hs) * gw_Sy(a, c) / dt;
     if (a.catchment && !a.catchment[c]) q = NAN;
     q_exf[c] = q;
     h_out[c] = fmin(v, hs);
@@ -195,8 +195,8 @@ __global__ __launch_bounds__(kGwBlock) void gw_velocity_kernel(GwArgs a, const d
         int i, j;
         gw_rc(c, ny, i, j);
         double vx = NAN, vy = NAN;
-        if (j < a.nx - 1) vx = -(0.5 * (a.K[c] + a.K[c + ny])) * ((h[c + ny] - h[c]) / a.dx);
-        if (i < ny - 1) vy = -(0.5 * (a.K[c] + a.K[c + 1])) * ((h[c + 1] - h[c]) / a.dy);
+        if (j < a.nx - 1) vx = -(0.5 * (gw_K(a, c) + gw_K(a, c + ny))) * ((h[c + ny] - h[c]) / a.dx);
+        if (i < ny - 1) vy = -(0.5 * (gw_K(a, c) + gw_K(a, c + 1))) * ((h[c + 1] - h[c]) / a.dy);
         const bool out = mask_on && a.catchment && !a.catchment[c];
         if (out) {
             vx = NAN;
@@ -230,8 +230,8 @@ __global__ __launch_bounds__(kColThreads) void gw_colsum_kernel(GwArgs a, const 
     column_reduce<4>(a.ny, blockIdx.x, is_max, [&](long long c, double (&v)[4]) {
         v[0] = a.R[c] * dt;
         v[1] = q_exf[c] * dt;
-        v[2] = a.Sy[c] * fmax(h0[c] - a.z0[c], 0.0);
-        v[3] = a.Sy[c] * fmax(h[c] - a.z0[c], 0.0);
+        v[2] = gw_Sy(a, c) * fmax(h0[c] - a.z0[c], 0.0);
+        v[3] = gw_Sy(a, c) * fmax(h[c] - a.z0[c], 0.0);
     }, out);
 }
 

@@ -28,7 +28,14 @@ struct GwArgs {
     const double *z0, *Sy, *R, *K, *h_soil, *h_dir;
     const uint8_t *catchment, *dirichlet;  // NULL = all inside / none
     GwCtl *ctl;
+    // a uniform specific yield / conductivity [m/s] / soil depth [m] in place
+    // of the Sy / K / h_soil maps when those are NULL
+    double Sy_u = 0.0, K_u = 0.0, h_soil_u = 0.0;
 };
+
+// Sy and K of cell c: the map, or the uniform value
+__device__ __forceinline__ double gw_Sy(const GwArgs &a, long long c) { return a.Sy ? a.Sy[c] : a.Sy_u; }
+__device__ __forceinline__ double gw_K(const GwArgs &a, long long c) { return a.K ? a.K[c] : a.K_u; }
 
 // device buffers of one solve (carved from a workspace)
 struct GwScratch {
@@ -54,6 +61,15 @@ struct GwmArgs {
     long long n;
     double dt_s, A, l_vg;
     const double *elev, *soil_depth, *ts, *tr, *nv, *alpha, *ksat, *ksat_gw, *Sy;
+    // uniform aquifer conductivity [mm/h] and specific yield when ksat_gw / Sy
+    // are NULL (Kgw is then NULL too and Kgw_u = ksat_gw_u / 3.6e6)
+    double ksat_gw_u = 0.0, Sy_u = 0.0, Kgw_u = 0.0;
+    double soil_depth_u = 0.0;  // the soil depth [m] when soil_depth is NULL
+    // soil classes in place of the ts / tr / nv / alpha / ksat maps when
+    // soil_class is set: a class byte per cell and per-class tables
+    const uint8_t *soil_class = nullptr;
+    const double *cls_ts = nullptr, *cls_tr = nullptr, *cls_nv = nullptr, *cls_alpha = nullptr,
+                 *cls_ksat = nullptr;
     double *h, *I, *d, *cum, *maxd, *pn, *pr, *pc;
     double *z_bed, *Kgw, *rfg, *q;
     double *t2;  // unsaturated-zone excess of the last update [m]

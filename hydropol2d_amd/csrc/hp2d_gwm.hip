@@ -20,6 +20,44 @@
 
 namespace hp2d {
 
+// the aquifer's specific yield and conductivity [mm/h] of a cell: the map,
+// or the uniform value
+__device__ __forceinline__ double gwm_Sy(const GwmArgs &a, long long c) {
+    return a.Sy ? a.Sy[c] : a.Sy_u;
+}
+__device__ __forceinline__ double gwm_ksat_gw(const GwmArgs &a, long long c) {
+    return a.ksat_gw ? a.ksat_gw[c] : a.ksat_gw_u;
+}
+
+// soil depth of a cell [m]: the map, or the uniform value
+__device__ __forceinline__ double gwm_soil_depth(const GwmArgs &a, long long c) {
+    return a.soil_depth ? a.soil_depth[c] : a.soil_depth_u;
+}
+
+// the soil's van Genuchten parameters and saturated conductivity [mm/h] of a
+// cell: the per-cell maps, or its soil class's entries of the class tables
+struct GwmSoil {
+    double ts, tr, nv, alpha, ksat;
+};
+__device__ __forceinline__ GwmSoil gwm_soil(const GwmArgs &a, long long c) {
+    GwmSoil s;
+    if (a.soil_class) {
+        const int k = a.soil_class[c];
+        s.ts = a.cls_ts[k];
+        s.tr = a.cls_tr[k];
+        s.nv = a.cls_nv[k];
+        s.alpha = a.cls_alpha[k];
+        s.ksat = a.cls_ksat[k];
+    } else {
+        s.ts = a.ts[c];
+        s.tr = a.tr[c];
+        s.nv = a.nv[c];
+        s.alpha = a.alpha[c];
+        s.ksat = a.ksat[c];
+    }
+    return s;
+}
+
 // The surface depth d [mm] and the vadose storage I [mm] of a cell: the
 // module's FP64 arrays, or an FP32 run's own buffers (read as FP64, rounded
 // once per kernel on the way back)
@@ -44,16 +82,18 @@ __device__ __forceinline__ void gwm_recharge_cell(const GwmArgs &a, long long c,
                                                   double (&v)[3]) {
     const bool nan = isnan(a.elev[c]);
     const double zero = nan ? NAN : 0.0;
-    const double gw_depth = a.h[c] - (a.elev[c] - a.soil_depth[c]);
-    double zwt = a.soil_depth[c] - gw_depth;
+    const double sd = gwm_soil_depth(a, c);
+    const double gw_depth = a.h[c] - (a.elev[c] - sd);
+    double zwt = sd - gw_depth;
     zwt = fmax(zwt, 0.0);
-    zwt = fmin(zwt, a.soil_depth[c]);
-    const double ts = a.ts[c], tr = a.tr[c];
+    zwt = fmin(zwt, sd);
+    const GwmSoil soil = gwm_soil(a, c);
+    const double ts = soil.ts, tr = soil.tr;
     const double uzmax = zwt * (ts - tr);
     // simulate_groundwater_recharge with infiltration = zero_matrix, min = 0
     const double dt = fmax(a.dt_s, 1e-12);
-    const double nvg = a.nv[c], m = 1.0 - 1.0 / nvg, al = a.alpha[c];
-    const double Ks = a.ksat[c] / 1000.0 / 3600.0;
+    const double nvg = soil.nv, m = 1.0 - 1.0 / nvg, al = soil.alpha;
+    const double Ks = soil.ksat / 1000.0 / 3600.0;
     const double inf_r = isfinite(zero) ? zero : 0.0;
     const double S0 = gwm_I(a, c) / 1000.0;
     const double s0 = isfinite(S0) ? S0 : 0.0;
@@ -128,17 +168,17 @@ __device__ __forceinline__ void gwm_recharge_cell(const GwmArgs &a, long long c,
     }
     v[0] = down * (nan ? NAN : a.A);
     // scheduler terms: pending head change and diffusivity K H / Sy
-    const double Sy_head = fmax(a.Sy[c], 1e-6);
+    const double Sy_head = fmax(gwm_Sy(a, c), 1e-6);
     v[1] = fabs(pn) / fmax(Sy_head, 1e-9);
-    const double zbed = a.elev[c] - a.soil_depth[c];
-    double K = a.ksat_gw[c] / 1000.0 / 3600.0;
+    const double zbed = a.elev[c] - sd;
+    double K = gwm_ksat_gw(a, c) / 1000.0 / 3600.0;
     if (!isfinite(K)) K = NAN;
     double sat = fmax(a.h[c] - zbed, 1e-4);
     if (nan) sat = NAN;
     v[2] = K * sat / Sy_head;
     if (!a.statics_ready) {
         a.z_bed[c] = zbed;
-        a.Kgw[c] = a.ksat_gw[c] / 1000.0 / 3600.0;
+        if (a.Kgw) a.Kgw[c] = gwm_ksat_gw(a, c) / 1000.0 / 3600.0;
     }
 }
 
@@ -180,7 +220,7 @@ __global__ __launch_bounds__(256) void gwm_after_bq_kernel(GwmArgs a, const doub
 __global__ __launch_bounds__(256) void gwm_local_kernel(GwmArgs a, double dt_gw) {
     const long long c = blockIdx.x * 256LL + threadIdx.x;
     if (c >= a.n) return;
-    const double Sy_l = fmax(a.Sy[c], 1e-6);
+    const double Sy_l = fmax(gwm_Sy(a, c), 1e-6);
     const double dh = a.rfg[c] * dt_gw / Sy_l;
     double hc = a.h[c] + dh;
     hc = fmax(hc, a.z_bed[c]);
@@ -206,13 +246,14 @@ __global__ __launch_bounds__(kColThreads) void gwm_post_kernel(GwmArgs a, double
         if (a.pc) a.pc[c] = 0.0;
         double zwt = a.elev[c] - a.h[c];
         zwt = fmax(zwt, 0.0);
-        zwt = fmin(zwt, a.soil_depth[c]);
-        const double uz_new = zwt * (a.ts[c] - a.tr[c]);
+        zwt = fmin(zwt, gwm_soil_depth(a, c));
+        const GwmSoil soil = gwm_soil(a, c);
+        const double uz_new = zwt * (soil.ts - soil.tr);
         const double ex = fmax(gwm_I(a, c) / 1000.0 - uz_new, 0.0);  // excess [m]
         a.t2[c] = ex;
         v[1] = ex;
         const double area = nan ? NAN : a.A;
-        v[2] = area * a.Sy[c] * (a.h[c] - (a.elev[c] - a.soil_depth[c]));
+        v[2] = area * gwm_Sy(a, c) * (a.h[c] - (a.elev[c] - gwm_soil_depth(a, c)));
     }, col);
 }
 
@@ -229,7 +270,7 @@ __global__ __launch_bounds__(256) void gwm_excess_kernel(GwmArgs a) {
 __global__ __launch_bounds__(256) void gwm_maxdepth_kernel(GwmArgs a) {
     const long long c = blockIdx.x * 256LL + threadIdx.x;
     if (c >= a.n || !a.maxd) return;
-    a.maxd[c] = fmax(a.maxd[c], a.h[c] - (a.elev[c] - a.soil_depth[c]));
+    a.maxd[c] = fmax(a.maxd[c], a.h[c] - (a.elev[c] - gwm_soil_depth(a, c)));
 }
 
 // per column: sum of A*Sy*(h - z_bed) and of A*pending_net (mass_balance_check.m:66-70)
@@ -237,7 +278,7 @@ __global__ __launch_bounds__(kColThreads) void gwm_storage_kernel(GwmArgs a, dou
     const bool is_max[2] = {false, false};
     column_reduce<2>(a.ny, blockIdx.x, is_max, [&](long long c, double (&v)[2]) {
         const double area = isnan(a.elev[c]) ? NAN : a.A;
-        v[0] = area * a.Sy[c] * (a.h[c] - (a.elev[c] - a.soil_depth[c]));
+        v[0] = area * gwm_Sy(a, c) * (a.h[c] - (a.elev[c] - gwm_soil_depth(a, c)));
         v[1] = area * a.pn[c];
     }, out);
 }
@@ -320,9 +361,12 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
             ga.courant = p->courant;
             ga.z0 = a.z_bed;
             ga.Sy = a.Sy;
+            ga.Sy_u = a.Sy_u;
             ga.R = a.rfg;
             ga.K = a.Kgw;
+            ga.K_u = a.Kgw_u;
             ga.h_soil = a.soil_depth;
+            ga.h_soil_u = a.soil_depth_u;
             ga.h_dir = nullptr;
             ga.catchment = dom;
             ga.dirichlet = nullptr;
