@@ -14,21 +14,24 @@
 
 namespace hp2d {
 
-__host__ __device__ inline double rain_average(const double *times, const double *values,
-                                               int n, double t0, double t1, int *cursor) {
+// The series read through an accessor: times(k), values(k) for 0-based k.
+template <typename Series>
+__host__ __device__ inline double rain_average(const Series &ser, int n, double t0, double t1,
+                                               int *cursor) {
     if (n <= 0 || !__builtin_isfinite(t0) || !__builtin_isfinite(t1) || t1 <= t0) return 0.0;
     const double span = t1 - t0;
     double acc = 0.0;
     int cur = *cursor;
     if (cur < 1) cur = 1;
     if (cur > n) cur = n;
-    while (cur < n && t0 > times[cur - 1]) ++cur;
-    const double spacing = (n >= 2) ? times[1] - times[0] : 0.0;
+    while (cur < n && t0 > ser.time(cur - 1)) ++cur;
+    const double spacing = (n >= 2) ? ser.time(1) - ser.time(0) : 0.0;
     for (int i = cur; i <= n; ++i) {
-        const double start = (i == 1) ? times[0] - spacing : times[i - 2];
-        const double end = times[i - 1];
+        const double start = (i == 1) ? ser.time(0) - spacing : ser.time(i - 2);
+        const double end = ser.time(i - 1);
+        const double v = ser.value(i - 1);
         const double overlap = fmax(0.0, fmin(t1, end) - fmax(t0, start));
-        if (overlap > 0 && __builtin_isfinite(values[i - 1])) acc = acc + values[i - 1] * overlap;
+        if (overlap > 0 && __builtin_isfinite(v)) acc = acc + v * overlap;
         if (end >= t1) {
             cur = i;
             break;
@@ -38,17 +41,29 @@ __host__ __device__ inline double rain_average(const double *times, const double
     return acc / span;
 }
 
+// the series as two arrays
+struct SeriesArrays {
+    const double *t, *v;
+    __host__ __device__ double time(int k) const { return t[k]; }
+    __host__ __device__ double value(int k) const { return v[k]; }
+};
+
 // delta_p_agg [mm] of update_spatial_BC.m:422-448 for the step [t_prev, t].
-__host__ __device__ inline double rain_depth(const double *times, const double *values, int n,
-                                             double t_prev, double t, int *cursor) {
+template <typename Series>
+__host__ __device__ inline double rain_depth(const Series &ser, int n, double t_prev, double t,
+                                             int *cursor) {
     const double dt_step = fmax(t - t_prev, 0.0);
     double depth = 0.0;
     if (dt_step > 0) {
-        double avg = rain_average(times, values, n, t_prev, t, cursor);
+        double avg = rain_average(ser, n, t_prev, t, cursor);
         if (__builtin_isnan(avg)) avg = 0.0;
         depth = avg * dt_step / 60.0;
     }
     return depth;
+}
+__host__ __device__ inline double rain_depth(const double *times, const double *values, int n,
+                                             double t_prev, double t, int *cursor) {
+    return rain_depth(SeriesArrays{times, values}, n, t_prev, t, cursor);
 }
 
 }  // namespace hp2d

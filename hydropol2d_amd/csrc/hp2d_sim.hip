@@ -1,6 +1,6 @@
 // hp2d_sim.hip - device-side time control of the HydroPol2D step loop.
 //
-// One 256-thread block per step.  It reduces the per-wave partials of the
+// One 1024-thread block per step.  It reduces the per-wave partials of the
 // fused step in a fixed order (deterministic), then runs, in FP64 on one
 // lane, the scalar tail of one iteration of HydroPol2D_Main_While.m:403-994:
 //   refreshing_timestep bucket + Courant dt   (:693-705, refreshing_timestep.m:17-355)
@@ -18,13 +18,29 @@ namespace hp2d {
 
 constexpr int kFinThreads = 1024;
 constexpr int kFinWaves = kFinThreads / kWave;
+constexpr int kRainWindow = 256;  // rainfall-series entries staged in LDS per step
+
+// The rainfall series with entries [w0, w0 + wn) (and the first two, for the
+// spacing) staged in LDS; the serial lookup reaches HBM only past the window.
+struct SeriesWindow {
+    const double *t, *v;        // the whole series in device memory
+    const double *lt, *lv;      // the window in LDS
+    const double *head;         // t[0], t[1] in LDS
+    int w0, wn;
+    __device__ double time(int k) const {
+        if (k - w0 >= 0 && k - w0 < wn) return lt[k - w0];
+        if (k < 2) return head[k];
+        return t[k];
+    }
+    __device__ double value(int k) const { return (k - w0 >= 0 && k - w0 < wn) ? lv[k - w0] : v[k]; }
+};
 
 template <typename T>
 __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     // The scalar tail below is one thread's chain of dependent reads and
     // writes of the run state.  It works on a copy in LDS, loaded and stored
     // back in one pass, so the chain pays LDS latency instead of L2 latency.
-    __shared__ SimState sst;
+    __shared__ __attribute__((aligned(16))) SimState sst;
     {
         const unsigned *src = reinterpret_cast<const unsigned *>(a.st);
         unsigned *dst = reinterpret_cast<unsigned *>(&sst);
@@ -33,6 +49,19 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     __syncthreads();
     SimState *st = &sst;
     if (st->done || st->paused || st->hold) return;
+    // the rainfall series around the cursor, for the serial lookup below (the
+    // block barrier after the reductions orders these writes before it)
+    __shared__ double rain_lt[kRainWindow], rain_lv[kRainWindow], rain_head[2];
+    SeriesWindow ser{a.rain_t, a.rain_v, rain_lt, rain_lv, rain_head, 0, 0};
+    if (a.n_rain > 0) {
+        ser.w0 = st->cursor - 2 > 0 ? st->cursor - 2 : 0;
+        ser.wn = a.n_rain - ser.w0 < kRainWindow ? a.n_rain - ser.w0 : kRainWindow;
+        for (int i = threadIdx.x; i < ser.wn; i += kFinThreads) {
+            rain_lt[i] = a.rain_t[ser.w0 + i];
+            rain_lv[i] = a.rain_v[ser.w0 + i];
+        }
+        if (threadIdx.x < 2 && threadIdx.x < a.n_rain) rain_head[threadIdx.x] = a.rain_t[threadIdx.x];
+    }
     __shared__ double sh[kFinWaves][14];
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
     const double *P = a.partials;
@@ -164,7 +193,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
             st->next_report += a.report_pct;
         // ---------------- update_spatial_BC ----------------
         int cur = st->cursor;
-        st->delta_p_agg = rain_depth(a.rain_t, a.rain_v, a.n_rain, st->t_previous, t_now, &cur);
+        st->delta_p_agg = rain_depth(ser, a.n_rain, st->t_previous, t_now, &cur);
         st->cursor = cur;
         // ---------------- EMC (:724-731), with the refreshed time_step ----------------
         if (a.flag_wq) {
@@ -263,14 +292,17 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         t_now = t_now + st->time_step_min;
         st->t = t_now;
         int cur = st->cursor;
-        st->delta_p_agg = rain_depth(a.rain_t, a.rain_v, a.n_rain, st->t_previous, t_now, &cur);
+        st->delta_p_agg = rain_depth(ser, a.n_rain, st->t_previous, t_now, &cur);
         st->cursor = cur;
     }
     st->done = !(st->t <= (routing_time + st->min_dt / 60.0)) ? 1 : 0;
     if (st->stop_accepted >= 0 && st->accepted >= st->stop_accepted) st->paused = 1;
-    unsigned *gdst = reinterpret_cast<unsigned *>(a.st);
-    const unsigned *lsrc = reinterpret_cast<const unsigned *>(&sst);
-    for (int i = 0; i < (int)(sizeof(SimState) / 4); ++i) gdst[i] = lsrc[i];
+    // back to device memory in 16-byte pieces (one lane: the others have left)
+    uint4 *gdst = reinterpret_cast<uint4 *>(a.st);
+    const uint4 *lsrc = reinterpret_cast<const uint4 *>(&sst);
+    for (int i = 0; i < (int)(sizeof(SimState) / 16); ++i) gdst[i] = lsrc[i];
+    for (int i = (int)(sizeof(SimState) / 16) * 4; i < (int)(sizeof(SimState) / 4); ++i)
+        reinterpret_cast<unsigned *>(a.st)[i] = reinterpret_cast<const unsigned *>(&sst)[i];
 }
 
 template <typename T>
