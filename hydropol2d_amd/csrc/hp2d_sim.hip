@@ -1,6 +1,6 @@
 // hp2d_sim.hip - device-side time control of the HydroPol2D step loop.
 //
-// One 1024-thread block per step.  It reduces the per-wave partials of the
+// One 256-thread block per step.  It reduces the per-wave partials of the
 // fused step in a fixed order (deterministic), then runs, in FP64 on one
 // lane, the scalar tail of one iteration of HydroPol2D_Main_While.m:403-994:
 //   refreshing_timestep bucket + Courant dt   (:693-705, refreshing_timestep.m:17-355)
@@ -16,7 +16,7 @@
 
 namespace hp2d {
 
-constexpr int kFinThreads = 1024;
+constexpr int kFinThreads = 256;
 constexpr int kFinWaves = kFinThreads / kWave;
 constexpr int kRainWindow = 256;  // rainfall-series entries staged in LDS per step
 
@@ -100,6 +100,10 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     }
     __syncthreads();
     if (t != 0) return;
+    // the serial tail works on a private copy of the run state (registers:
+    // each dependent read-modify-write of a field costs no LDS round trip)
+    SimState loc = sst;
+    st = &loc;
     storage = uz = rain = infil = qsum = dsum = qraw = 0.0;
     vmax = dtw = raster = dtm = dpm = dnm = NAN;
     neg = 0.0;
@@ -297,12 +301,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     }
     st->done = !(st->t <= (routing_time + st->min_dt / 60.0)) ? 1 : 0;
     if (st->stop_accepted >= 0 && st->accepted >= st->stop_accepted) st->paused = 1;
-    // back to device memory in 16-byte pieces (one lane: the others have left)
-    uint4 *gdst = reinterpret_cast<uint4 *>(a.st);
-    const uint4 *lsrc = reinterpret_cast<const uint4 *>(&sst);
-    for (int i = 0; i < (int)(sizeof(SimState) / 16); ++i) gdst[i] = lsrc[i];
-    for (int i = (int)(sizeof(SimState) / 16) * 4; i < (int)(sizeof(SimState) / 4); ++i)
-        reinterpret_cast<unsigned *>(a.st)[i] = reinterpret_cast<const unsigned *>(&sst)[i];
+    *a.st = loc;
 }
 
 template <typename T>
