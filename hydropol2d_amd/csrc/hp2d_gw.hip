@@ -8,16 +8,16 @@
 //   predict    h_p = max(h_t - dt/Sy div F1, z0), Dirichlet
 //   flux(avg)  fluxes of h_p, averaged with F1 in place
 //   update     h = h_t + dt/Sy (-div F + R), Dirichlet, >= z0, exfiltration,
-//              <= surface
+//              <= surface, and the column sums of the closing mass balance
 //   velocity   Darcy velocities of the new head, the catchment mask, and the
 //              next Courant limit as an atomic minimum over positive doubles
 //   ctl_end    kk / trim / t bookkeeping of the reference loop
 // The loop control lives on the device, so the host only launches batches of
 // sub-steps and reads the done flag between batches; kernels of a sub-step
 // past the end return at once.
-// The closing mass-balance sums are per-column FP64 sums in row order,
-// finished on the host in column order: the same association as the
-// reference's nansum(nansum(.)).
+// The closing mass-balance sums are per-column FP64 sums in row order, taken
+// by the last sub-step's update pass and finished on the host in column
+// order: the same association as the reference's nansum(nansum(.)).
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -160,24 +160,36 @@ __global__ __launch_bounds__(kGwBlock) void gw_predict_kernel(GwArgs a, const do
     h_p[c] = fmax(v, a.z0[c]);
 }
 
-// corrector update, Dirichlet, bed, exfiltration and surface clamp (:128-148)
-__global__ __launch_bounds__(kGwBlock) void gw_update_kernel(GwArgs a, const double *h_t,
-                                                             const double *Fx, const double *Fy,
-                                                             double *h_out, double *q_exf) {
+// corrector update, Dirichlet, bed, exfiltration and surface clamp (:128-148),
+// one block per column.  The block also leaves the column's sums of the
+// closing mass balance (R*dt, q_exf*dt, Sy*max(h0 - z0, 0), Sy*max(h - z0, 0)
+// with h masked outside the catchment, as the velocity pass masks it): the
+// last sub-step's sums are the call's.
+__global__ __launch_bounds__(kColThreads) void gw_update_kernel(GwArgs a, const double *h_t,
+                                                                const double *Fx, const double *Fy,
+                                                                double *h_out, double *q_exf,
+                                                                const double *h0, double *col) {
     if (a.ctl->done) return;
-    const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
-    if (c >= a.n) return;
-    int i, j;
-    gw_rc(c, a.ny, i, j);
     const double dt = a.ctl->dt_used;
-    double v = h_t[c] + dt / gw_Sy(a, c) * (-gw_div(a, Fx, Fy, c, i, j) + a.R[c]);
-    if (a.dirichlet && a.dirichlet[c]) v = a.h_dir[c];
-    v = fmax(v, a.z0[c]);
-    const double hs = a.z0[c] + (a.h_soil ? a.h_soil[c] : a.h_soil_u);
-    double q = fmax(0.0, v - hs) * gw_Sy(a, c) / dt;
-    if (a.catchment && !a.catchment[c]) q = NAN;
-    q_exf[c] = q;
-    h_out[c] = fmin(v, hs);
+    const int j = blockIdx.x;
+    const bool is_max[4] = {false, false, false, false};
+    column_reduce<4>(a.ny, j, is_max, [&](long long c, double (&s)[4]) {
+        const int i = (int)(c - (long long)j * a.ny);
+        double v = h_t[c] + dt / gw_Sy(a, c) * (-gw_div(a, Fx, Fy, c, i, j) + a.R[c]);
+        if (a.dirichlet && a.dirichlet[c]) v = a.h_dir[c];
+        v = fmax(v, a.z0[c]);
+        const double hs = a.z0[c] + (a.h_soil ? a.h_soil[c] : a.h_soil_u);
+        const bool out = a.catchment && !a.catchment[c];
+        double q = fmax(0.0, v - hs) * gw_Sy(a, c) / dt;
+        if (out) q = NAN;
+        q_exf[c] = q;
+        const double hn = fmin(v, hs);
+        h_out[c] = hn;
+        s[0] = a.R[c] * dt;
+        s[1] = q * dt;
+        s[2] = gw_Sy(a, c) * fmax(h0[c] - a.z0[c], 0.0);
+        s[3] = gw_Sy(a, c) * fmax((out ? NAN : hn) - a.z0[c], 0.0);
+    }, col);
 }
 
 // compute_boussinesq_velocities (+ mask) and the Courant candidates of
@@ -243,27 +255,32 @@ hp2d_status gw_run(hp2d_handle *h, const GwArgs &a_in, double dt, const double *
     hipStream_t s = hp2d_handle_stream(h);
     GwArgs a = a_in;
     a.ctl = w.ctl;
-    hipError_t e = hipMemcpyAsync(w.ht, dh0, nb, hipMemcpyDeviceToDevice, s);
+    // the first sub-step reads the entry head in place; the velocity pass of
+    // each sub-step leaves the (masked) head of the next in w.ht
+    hipError_t e = hipSuccess;
     const dim3 g(blocks), b(kGwBlock), gv(std::min(blocks, 2048));
-    if (e == hipSuccess) {
+    {
         hipLaunchKernelGGL(gw_ctl_init, dim3(1), dim3(1), 0, s, w.ctl, dt);
         // velocities of the entry head (not masked, Boussinesq_2D_explicit.m:91)
-        hipLaunchKernelGGL(gw_velocity_kernel, gv, b, 0, s, a, (const double *)w.ht, w.ux, w.uy,
-                           (double *)nullptr, 0, 0);
+        hipLaunchKernelGGL(gw_velocity_kernel, gv, b, 0, s, a, dh0, w.ux, w.uy, (double *)nullptr, 0,
+                           0);
         e = hipGetLastError();
     }
+    bool first = true;
     GwCtl hc{};
     const int kMaxSub = 1000000;
     int batch = 1;
     while (e == hipSuccess) {
         for (int k = 0; k < batch; ++k) {
+            const double *ht = first ? dh0 : (const double *)w.ht;
+            first = false;
             hipLaunchKernelGGL(gw_ctl_begin, dim3(1), dim3(1), 0, s, w.ctl);
-            hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)w.ht, w.Fx, w.Fy, 0);
-            hipLaunchKernelGGL(gw_predict_kernel, g, b, 0, s, a, (const double *)w.ht,
-                               (const double *)w.Fx, (const double *)w.Fy, w.hp);
+            hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, ht, w.Fx, w.Fy, 0);
+            hipLaunchKernelGGL(gw_predict_kernel, g, b, 0, s, a, ht, (const double *)w.Fx,
+                               (const double *)w.Fy, w.hp);
             hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)w.hp, w.Fx, w.Fy, 1);
-            hipLaunchKernelGGL(gw_update_kernel, g, b, 0, s, a, (const double *)w.ht,
-                               (const double *)w.Fx, (const double *)w.Fy, w.tmp, w.q);
+            hipLaunchKernelGGL(gw_update_kernel, dim3(nx), dim3(kColThreads), 0, s, a, ht,
+                               (const double *)w.Fx, (const double *)w.Fy, w.tmp, w.q, dh0, w.col);
             hipLaunchKernelGGL(gw_velocity_kernel, gv, b, 0, s, a, (const double *)w.tmp, w.ux, w.uy,
                                w.ht, 1, 1);
             hipLaunchKernelGGL(gw_ctl_end, dim3(1), dim3(1), 0, s, w.ctl);
@@ -275,7 +292,8 @@ hp2d_status gw_run(hp2d_handle *h, const GwArgs &a_in, double dt, const double *
         if (hc.nsub > kMaxSub) return hp2d_handle_fail(h, "Boussinesq: more than 1e6 sub-steps");
         batch = batch < 64 ? batch * 2 : 64;
     }
-    if (e == hipSuccess) {
+    if (e == hipSuccess && hc.nsub == 0) {  // no sub-step ran: the entry head, and its sums
+        e = hipMemcpyAsync(w.ht, dh0, nb, hipMemcpyDeviceToDevice, s);
         hipLaunchKernelGGL(gw_colsum_kernel, dim3(nx), dim3(kColThreads), 0, s, a, (const double *)w.q, dh0,
                            (const double *)w.ht, w.col);
         e = hipGetLastError();
