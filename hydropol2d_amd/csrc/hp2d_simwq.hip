@@ -357,27 +357,44 @@ __global__ __launch_bounds__(kWqThreads) void wq_first_kernel(WqSimArgs<T> a) {
     block_store<6>(v, is_min, a.part, a.n_blocks);
 }
 
-// one block: folds the first pass; decides the split; closes an unsplit call
-__device__ __forceinline__ void fold_fields(const double *part, int nb, int f0, int nf,
-                                            const bool *is_min, double *out) {
-    __shared__ double sh[8][kWqFoldThreads / kWave];
+// one block: folds the first pass; decides the split; closes an unsplit call.
+// Each thread issues all of its loads (NF fields x 4 partials) before it
+// combines any of them, so the fold costs one memory latency, not 4*NF.
+template <int NF>
+__device__ __forceinline__ void fold_fields(const double *part, int nb, const bool (&is_min)[NF],
+                                            double *out) {
+    constexpr int kU = 4;
+    __shared__ double sh[NF][kWqFoldThreads / kWave];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    for (int f = 0; f < nf; ++f) {
-        double acc = is_min[f] ? NAN : 0.0;
-        for (int b = threadIdx.x; b < nb; b += kWqFoldThreads) {
-            const double x = part[(long long)(f0 + f) * nb + b];
-            acc = is_min[f] ? fmin(acc, x) : acc + x;
-        }
-        acc = is_min[f] ? wave_fmin(acc) : wave_sum(acc);
-        if (lane == 0) sh[f][w] = acc;
+    double acc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] = is_min[f] ? NAN : 0.0;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += kU * kWqFoldThreads) {
+        double x[NF][kU];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int b = b0 + u * kWqFoldThreads;
+                x[f][u] = b < nb ? part[(long long)f * nb + b] : (is_min[f] ? NAN : 0.0);
+            }
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int u = 0; u < kU; ++u) acc[f] = is_min[f] ? fmin(acc[f], x[f][u]) : acc[f] + x[f][u];
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const double v = is_min[f] ? wave_fmin(acc[f]) : wave_sum(acc[f]);
+        if (lane == 0) sh[f][w] = v;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int f = 0; f < nf; ++f) {
-            double acc = sh[f][0];
+        for (int f = 0; f < NF; ++f) {
+            double r = sh[f][0];
             for (int k = 1; k < kWqFoldThreads / kWave; ++k)
-                acc = is_min[f] ? fmin(acc, sh[f][k]) : acc + sh[f][k];
-            out[f] = acc;
+                r = is_min[f] ? fmin(r, sh[f][k]) : r + sh[f][k];
+            out[f] = r;
         }
     }
 }
@@ -388,7 +405,7 @@ __global__ __launch_bounds__(kWqFoldThreads) void wq_ctl_kernel(WqSimArgs<T> a) 
     if (wq_skip(st, false)) return;
     double r[6];
     const bool is_min[6] = {true, true, true, true, false, false};
-    fold_fields(a.part, a.n_blocks, 0, 6, is_min, r);
+    fold_fields<6>(a.part, a.n_blocks, is_min, r);
     if (threadIdx.x != 0) return;
     const double dt_s = st->time_step_min * 60.0;  // large_timestep
     const double tmin0 = r[kWqMinDW] >= 0 ? dt_s : 3600.0 * r[kWqCand0];
@@ -522,7 +539,7 @@ __global__ __launch_bounds__(kWqFoldThreads) void wq_fold_kernel(WqSimArgs<T> a,
     double r[8];
     const bool is_min[8] = {true, false, false, true, false, false, false, false};
     // fields kWqMinDW .. kWqOutMass are read; kWqMinMid, kWqCand0, kWqSumWout are stale
-    fold_fields(a.part, a.n_blocks, 0, 8, is_min, r);
+    fold_fields<8>(a.part, a.n_blocks, is_min, r);
     if (threadIdx.x != 0) return;
     const double dt_s = ctl->dt_s;
     st->tmin_wq = r[kWqMinDW] >= 0 ? dt_s : 3600.0 * r[kWqCandEnd];
