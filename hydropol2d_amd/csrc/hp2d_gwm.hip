@@ -78,6 +78,7 @@ __device__ __forceinline__ void gwm_set_d(const GwmArgs &a, long long c, double 
 
 // recharge bucket, pending exchange and the scheduler's per-cell terms of
 // cell c; v receives [recharge volume, pending head change, diffusivity]
+template <bool F32>
 __device__ __forceinline__ void gwm_recharge_cell(const GwmArgs &a, long long c, double dt_rfg,
                                                   double (&v)[3]) {
     const bool nan = isnan(a.elev[c]);
@@ -113,7 +114,7 @@ __device__ __forceinline__ void gwm_recharge_cell(const GwmArgs &a, long long c,
     double Se = (theta - tr) / fmax(ts - tr, 1e-12);
     Se = fmin(fmax(Se, 1e-6), 1.0);
     double hs, Kr;
-    if (a.I32) {
+    if constexpr (F32) {
         // FP32 run: the van Genuchten head and the Mualem conductivity in FP32
         // with the hardware log2 / exp2 (their results feed a storage that is
         // itself FP32; the bucket balance below stays FP64)
@@ -187,11 +188,14 @@ __device__ __forceinline__ void gwm_recharge_cell(const GwmArgs &a, long long c,
 // change, [2] that of the diffusivity (the scheduler's inputs).  dt_rfg > 0:
 // the update's interval when the update is known to follow (synchronous
 // module), whose recharge rate and zeroed exfiltration are then written here.
+// F32: an FP32 run (I32 set), whose variant holds no FP64 powers and so
+// needs fewer registers.
+template <bool F32>
 __global__ __launch_bounds__(kColThreads) void gwm_recharge_kernel(GwmArgs a, double dt_rfg,
                                                                   double *col) {
     const bool is_max[3] = {false, true, true};
     column_reduce<3>(a.ny, blockIdx.x, is_max,
-                     [&](long long c, double (&v)[3]) { gwm_recharge_cell(a, c, dt_rfg, v); }, col);
+                     [&](long long c, double (&v)[3]) { gwm_recharge_cell<F32>(a, c, dt_rfg, v); }, col);
 }
 
 // recharge_for_gw (:199-203) and q_exf = zero_matrix
@@ -310,8 +314,12 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
     // synchronous module: the update follows every call, over the interval known now
     const bool sync = p->flag_async == 0;
     const double dt_sync = std::fmax(g->elapsed_s + a.dt_s, DBL_EPSILON);
-    hipLaunchKernelGGL(gwm_recharge_kernel, dim3(nx), dim3(kColThreads), 0, s, a, sync ? dt_sync : 0.0,
-                       col);
+    if (a.I32)
+        hipLaunchKernelGGL(gwm_recharge_kernel<true>, dim3(nx), dim3(kColThreads), 0, s, a,
+                           sync ? dt_sync : 0.0, col);
+    else
+        hipLaunchKernelGGL(gwm_recharge_kernel<false>, dim3(nx), dim3(kColThreads), 0, s, a,
+                           sync ? dt_sync : 0.0, col);
     e = hipGetLastError();
     D2H(hc.data(), col, hc.size() * 8);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
