@@ -462,7 +462,10 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
                 qy_dst[oid] = oy;
                 if (a.dmax_map && !fail_step) {
                     const T prevmax = a.dmax_map[oid];
-                    a.dmax_map[oid] = (kstep == 1) ? dn : dmax(dn, prevmax);
+                    const T newmax = (kstep == 1) ? dn : dmax(dn, prevmax);
+                    // stored only when it changes (a NaN old value always differs):
+                    // a cell that is not rising costs no write
+                    if (kstep == 1 || newmax != prevmax) a.dmax_map[oid] = newmax;
                     // time to peak: end time of the step that last raised the maximum
                     if (a.tpeak_map && (kstep == 1 || dn > prevmax)) a.tpeak_map[oid] = t_end_min;
                 }
