@@ -212,9 +212,8 @@ def algorithmic_bytes_per_cell(spec, elem):
     if spec.infiltration is not None:
         reads += 2 * elem + (1 if spec.infiltration.soil_class is not None else 0)  # I, T, class
         writes += 2 * elem      # I, T
-    reads += elem               # dmax (save_output_maps.m:588-600)
-    writes += elem
-    return reads, writes
+    reads += elem               # dmax (save_output_maps.m:588-600); it is stored only where
+    return reads, writes        # the maximum rises, so its write is not counted
 
 
 def main():
