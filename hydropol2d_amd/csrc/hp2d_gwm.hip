@@ -207,19 +207,6 @@ __global__ __launch_bounds__(256) void gwm_rfg_kernel(GwmArgs a, double dt_gw) {
     a.q[c] = nan ? NAN : 0.0;
 }
 
-// after Boussinesq: take the head and q_exf, exfiltration onto the surface (:248-250)
-__global__ __launch_bounds__(256) void gwm_after_bq_kernel(GwmArgs a, const double *h_new,
-                                                           const double *q_new, double dt_gw) {
-    const long long c = blockIdx.x * 256LL + threadIdx.x;
-    if (c >= a.n) return;
-    a.h[c] = h_new[c];
-    const double q = q_new[c];
-    a.q[c] = q;
-    double dn = gwm_d(a, c) + dt_gw * (q * 1000.0);
-    if (dn < 0.0) dn = 0.0;
-    gwm_set_d(a, c, dn);
-}
-
 // local water-table update without baseflow (:252-263)
 __global__ __launch_bounds__(256) void gwm_local_kernel(GwmArgs a, double dt_gw) {
     const long long c = blockIdx.x * 256LL + threadIdx.x;
@@ -240,15 +227,33 @@ __global__ __launch_bounds__(256) void gwm_local_kernel(GwmArgs a, double dt_gw)
 // exfiltrated volume, [1] NaN-omitting max of the excess and [2] nansum of the
 // aquifer storage A*Sy*(h - z_bed) (the head is final here; the pending term
 // of the storage is zero after the reset)
-__global__ __launch_bounds__(kColThreads) void gwm_post_kernel(GwmArgs a, double dt_gw, double *col) {
+// With h_new / q_new (after a Boussinesq update) it first takes over the new
+// head and q_exf and moves the exfiltration onto the surface (:248-250), so
+// the head and q_exf are not read back.
+__global__ __launch_bounds__(kColThreads) void gwm_post_kernel(GwmArgs a, double dt_gw, double *col,
+                                                              const double *h_new,
+                                                              const double *q_new) {
     const bool is_max[3] = {false, true, false};
     column_reduce<3>(a.ny, blockIdx.x, is_max, [&](long long c, double (&v)[3]) {
         const bool nan = isnan(a.elev[c]);
-        v[0] = fmax(a.q[c], 0.0) * dt_gw * (nan ? NAN : a.A);
+        double q, h;
+        if (h_new) {
+            h = h_new[c];
+            q = q_new[c];
+            a.h[c] = h;
+            a.q[c] = q;
+            double dn = gwm_d(a, c) + dt_gw * (q * 1000.0);
+            if (dn < 0.0) dn = 0.0;
+            gwm_set_d(a, c, dn);
+        } else {
+            h = a.h[c];
+            q = a.q[c];
+        }
+        v[0] = fmax(q, 0.0) * dt_gw * (nan ? NAN : a.A);
         a.pn[c] = 0.0;
         if (a.pr) a.pr[c] = 0.0;
         if (a.pc) a.pc[c] = 0.0;
-        double zwt = a.elev[c] - a.h[c];
+        double zwt = a.elev[c] - h;
         zwt = fmax(zwt, 0.0);
         zwt = fmin(zwt, gwm_soil_depth(a, c));
         const GwmSoil soil = gwm_soil(a, c);
@@ -257,7 +262,7 @@ __global__ __launch_bounds__(kColThreads) void gwm_post_kernel(GwmArgs a, double
         a.t2[c] = ex;
         v[1] = ex;
         const double area = nan ? NAN : a.A;
-        v[2] = area * gwm_Sy(a, c) * (a.h[c] - (a.elev[c] - gwm_soil_depth(a, c)));
+        v[2] = area * gwm_Sy(a, c) * (h - (a.elev[c] - gwm_soil_depth(a, c)));
     }, col);
 }
 
@@ -349,6 +354,7 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
     const double dt_gw = std::fmax(g->elapsed_s, DBL_EPSILON);
     double err = 0.0;
     double s_after = 0.0;       // aquifer storage after the update (post pass)
+    const double *h_new = nullptr, *q_new = nullptr;  // Boussinesq head and q_exf
     bool have_storage = false;
     if (due) {
         if (!sync) {
@@ -382,16 +388,16 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
             if (e == hipSuccess) {
                 hp2d_status st = gw_run(h, ga, dt_gw, a.h, w, &err, nullptr);
                 if (st != HP2D_OK) return st;
-                hipLaunchKernelGGL(gwm_after_bq_kernel, G, B, 0, s, a, (const double *)w.ht,
-                                   (const double *)w.q, dt_gw);
-                e = hipGetLastError();
+                h_new = w.ht;  // taken over by the post pass
+                q_new = w.q;
             }
         } else if (e == hipSuccess) {
             hipLaunchKernelGGL(gwm_local_kernel, G, B, 0, s, a, dt_gw);
             e = hipGetLastError();
         }
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(gwm_post_kernel, dim3(nx), dim3(kColThreads), 0, s, a, dt_gw, col);
+            hipLaunchKernelGGL(gwm_post_kernel, dim3(nx), dim3(kColThreads), 0, s, a, dt_gw, col,
+                               (const double *)h_new, (const double *)q_new);
             e = hipGetLastError();
         }
         D2H(hc.data(), col, hc.size() * 8);
