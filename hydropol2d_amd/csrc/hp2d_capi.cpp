@@ -1298,6 +1298,9 @@ hp2d_status hp2d_sim_run_fixed(hp2d_sim *s, int64_t n_steps, int32_t graph_steps
     s->li_ms = 0;
     s->li_launches = 0;
     if (graph_steps > 0) {
+        // the batch sim_advance uses (a groundwater call cuts every N steps), so
+        // that the graph built here is the one it launches
+        if (s->gw) graph_steps = std::min<int32_t>(graph_steps, s->cfg.gw_every_steps);
         hp2d_status rc = ensure_graph(s, graph_steps);
         if (rc) return rc;
     }

@@ -153,8 +153,8 @@ def c2_groundwater(z, n_soil, soil_depth=2.0, wt_depth=1.0, every_steps=10):
 
 
 def full_physics_c2(ny=4096, nx=4096, dx=2.0, precision="fp32", routing_time_min=180.0,
-                    fill_sinks=None, water_quality=True, groundwater=True):
-    """Fractal DEM (Hurst 0.8, 200 m relief), 8 LULC classes (n in [0.015, 0.15]), 5 soil
+                    fill=True, water_quality=True, groundwater=True):
+    """Sink-filled fractal DEM (Hurst 0.8, 200 m relief), 8 LULC classes (n in [0.015, 0.15]), 5 soil
     classes (Ks in [1, 50] mm/h), GRF rainfall pattern (mean 40 mm/h, CV 0.5, 5 km
     correlation) on a 2 h triangular hyetograph, perimeter outlets at the lowest edge,
     one pollutant with build-up / wash-off per LULC class and the groundwater module
@@ -162,8 +162,10 @@ def full_physics_c2(ny=4096, nx=4096, dx=2.0, precision="fp32", routing_time_min
     z = spectral_dem(ny, nx, dx, 0.8, 200.0, seed=2)
     # tilt towards the lowest column so that the catchment drains to one edge
     z = z + np.linspace(50.0, 0.0, nx)[None, :]
-    if fill_sinks is not None:
-        z = fill_sinks(z)
+    outlet = np.zeros((ny, nx), dtype=np.uint8, order="F")
+    outlet[:, nx - 1] = 1
+    if fill:  # sinks filled: every depression drains to the outlet column
+        z = fill_sinks(z, seed_mask=outlet)
     lulc = voronoi_classes(ny, nx, 8, 64, seed=3)
     n_cls = np.linspace(0.015, 0.15, 8)
     rough = n_cls[lulc]
@@ -175,8 +177,6 @@ def full_physics_c2(ny=4096, nx=4096, dx=2.0, precision="fp32", routing_time_min
     tri = np.where(mid <= 60.0, mid / 60.0, (120.0 - mid) / 60.0) * 80.0   # mean 40 mm/h
     t_end = np.append(t_end, 1e9)
     tri = np.append(tri, 0.0)
-    outlet = np.zeros((ny, nx), dtype=np.uint8, order="F")
-    outlet[:, nx - 1] = 1
     ga = GreenAmpt(ksat_mmh=np.linspace(1.0, 50.0, 5), psi_mm=np.linspace(200.0, 50.0, 5),
                    dtheta=np.full(5, 0.3), theta_sat_minus_r=np.full(5, 0.4),
                    soil_class=np.asfortranarray(soil), I_min_mm=1e-3)

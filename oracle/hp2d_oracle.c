@@ -367,20 +367,76 @@ typedef struct ga_tables {
     double ksat, psi, dtheta, Lu_mm, k_out, Tr;
 } ga_tables;
 
-/* GA derived parameters, HydroPol2D_preprocessing.m:2249-2253 */
-static ga_tables ga_class(const hp2d_run_inputs *in, int32_t k) {
+/* GA derived parameters, HydroPol2D_preprocessing.m:2249-2253 (lu_m NaN: derived) */
+static ga_tables ga_derive(double ksat, double psi, double dtheta, double theta_sat_minus_r,
+                           double lu_m) {
     ga_tables g;
-    g.ksat = in->ksat_mmh[k];
-    g.psi = in->psi_mm[k];
-    g.dtheta = in->dtheta[k];
+    g.ksat = ksat;
+    g.psi = psi;
+    g.dtheta = dtheta;
     double kr = (1.0 / 75.0 * (sqrt(g.ksat / 25.4)));
     g.Tr = 4.5 / sqrt(g.ksat / 25.4);
     double Lu = 4.0 * sqrt(g.ksat / 25.4);
     Lu = Lu * 2.54 / 100.0;
-    if (in->lu_m && !isnan(in->lu_m[k])) Lu = in->lu_m[k];
-    g.k_out = in->theta_sat_minus_r[k] * kr * Lu * 1000.0;
+    if (!isnan(lu_m)) Lu = lu_m;
+    g.k_out = theta_sat_minus_r * kr * Lu * 1000.0;
     g.Lu_mm = Lu * 1000.0;
     return g;
+}
+
+static ga_tables ga_class(const hp2d_run_inputs *in, int32_t k) {
+    return ga_derive(in->ksat_mmh[k], in->psi_mm[k], in->dtheta[k], in->theta_sat_minus_r[k],
+                     in->lu_m ? in->lu_m[k] : NAN);
+}
+
+void oracle_ga_params(double ksat_mmh, double psi_mm, double dtheta, double theta_sat_minus_r,
+                      double lu_m, double *out6) {
+    const ga_tables g = ga_derive(ksat_mmh, psi_mm, dtheta, theta_sat_minus_r, lu_m);
+    out6[0] = g.ksat; out6[1] = g.psi; out6[2] = g.dtheta;
+    out6[3] = g.Lu_mm; out6[4] = g.k_out; out6[5] = g.Tr;
+}
+
+/* The legacy explicit Green-Ampt closure of one cell, in the
+ * Infiltration_Module slot (HydroPol2D_Main_While_Ensemble.m:243-266 on the
+ * first step, :274-298 afterwards; eff_depth as in Infiltration_Module.m:128-172):
+ * eff is the post-rain depth [mm], Ip the infiltrated depth before the step
+ * [mm], *Tc the recovery clock [h].  Writes *It and returns the infiltration
+ * rate f [mm/h]; the caller takes f*dt_h from the depth. */
+static double ga_cell(const ga_tables *g, int first, double dt_h, double I_min, double eff,
+                      double Ip, double *Tc, double *It) {
+    double i_a = eff / fmax(dt_h, 1e-12);
+    double f;
+    if (first) {
+        double C = g->ksat * (1.0 + ((eff + g->psi) * g->dtheta) / Ip);
+        f = fmin(C, i_a);
+        *It = fmax(Ip + f * dt_h, I_min);
+        *Tc = g->Tr;
+    } else {
+        double C = g->ksat * (1.0 + ((eff + g->psi) * g->dtheta) / Ip);
+        if (Ip > g->Lu_mm) C = g->ksat;
+        int idx_C = (i_a <= C);
+        *Tc = *Tc - dt_h;
+        if (i_a > C) *Tc = g->Tr;
+        f = fmin(C, i_a);
+        *It = fmax(Ip + f * dt_h - g->k_out * (double)idx_C * dt_h, I_min);
+        if (*Tc < 0) *It = I_min;
+    }
+    return f;
+}
+
+/* n cells of one soil class (parameters as returned by oracle_ga_params):
+ * d [mm] in = post-rain depth, out = depth after infiltration; f_out [mm/h]. */
+void oracle_ga_cells(int64_t n, int32_t first_step, double dt_h, double I_min_mm,
+                     const double *params6, double *d, const double *I_p, double *I_t,
+                     double *T, double *f_out) {
+    ga_tables g;
+    g.ksat = params6[0]; g.psi = params6[1]; g.dtheta = params6[2];
+    g.Lu_mm = params6[3]; g.k_out = params6[4]; g.Tr = params6[5];
+    for (int64_t c = 0; c < n; ++c) {
+        const double f = ga_cell(&g, first_step, dt_h, I_min_mm, d[c], I_p[c], &T[c], &I_t[c]);
+        d[c] = d[c] - f * dt_h;
+        if (f_out) f_out[c] = f;
+    }
 }
 
 int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
@@ -603,25 +659,8 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             double dt_h = time_step / 60.0;
             for (int64_t c = 0; c < n; ++c) {
                 int32_t cls = in->soil_class ? in->soil_class[c] : 0;
-                const ga_tables *g = &ga[cls];
-                double eff = d_t[c];
-                double i_a = eff / fmax(dt_h, 1e-12);
-                double f;
-                if (k == 1) {
-                    double C = g->ksat * (1.0 + ((eff + g->psi) * g->dtheta) / I_p[c]);
-                    f = fmin(C, i_a);
-                    I_t[c] = fmax(I_p[c] + f * dt_h, in->I_min_mm);
-                    Tclk[c] = g->Tr;
-                } else {
-                    double C = g->ksat * (1.0 + ((eff + g->psi) * g->dtheta) / I_p[c]);
-                    if (I_p[c] > g->Lu_mm) C = g->ksat;
-                    int idx_C = (i_a <= C);
-                    Tclk[c] = Tclk[c] - dt_h;
-                    if (i_a > C) Tclk[c] = g->Tr;
-                    f = fmin(C, i_a);
-                    I_t[c] = fmax(I_p[c] + f * dt_h - g->k_out * (double)idx_C * dt_h, in->I_min_mm);
-                    if (Tclk[c] < 0) I_t[c] = in->I_min_mm;
-                }
+                double f = ga_cell(&ga[cls], k == 1, dt_h, in->I_min_mm, d_t[c], I_p[c], &Tclk[c],
+                                   &I_t[c]);
                 if (is_nan[c]) {
                     I_t[c] = NAN;
                     f = NAN;
@@ -850,6 +889,9 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             t = t + time_step;
             memcpy(d_t, d_p, (size_t)n * sizeof(double));
             memcpy(I_t, I_p, (size_t)n * sizeof(double));
+            out->rollback_dt_s = nt;
+            out->rollback_max_dt_s = max_dt;
+            if (out->stop_after_rollback) break;
             {
                 double dt_step = fmax(t - t_previous, 0.0);
                 delta_p_agg = 0.0;

@@ -83,7 +83,24 @@ typedef struct oracle_run_out {
      * n_dt_mismatch and the oracle keeps its own dt. */
     const double *dt_follow;
     int64_t n_follow, n_dt_ties, n_dt_mismatch;
+    /* Stop right after the first catch block (the rollback of
+     * HydroPol2D_Main_While.m:926-993) has run, so that the restored state and
+     * the new dt can be read; rollback_max_dt_s is the max_dt it leaves. */
+    int32_t stop_after_rollback;
+    double rollback_dt_s, rollback_max_dt_s;
 } oracle_run_out;
+
+/* Green-Ampt derived parameters of one soil class (HydroPol2D_preprocessing.m:
+ * 2249-2253): out6 = {ksat [mm/h], psi [mm], dtheta, Lu [mm], k_out [mm/h], Tr [h]};
+ * lu_m NaN derives Lu from ksat. */
+void oracle_ga_params(double ksat_mmh, double psi_mm, double dtheta, double theta_sat_minus_r,
+                      double lu_m, double *out6);
+
+/* The legacy explicit Green-Ampt closure on n cells of one class, the same
+ * function the loop applies (HydroPol2D_Main_While_Ensemble.m:243-298). */
+void oracle_ga_cells(int64_t n, int32_t first_step, double dt_h, double I_min_mm,
+                     const double *params6, double *d, const double *I_p, double *I_t,
+                     double *T, double *f_out);
 
 int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                    oracle_run_out *out);

@@ -47,7 +47,8 @@ class RunOut(C.Structure):
         ("rep", abi.RunReport), ("volume_error_hist", _dp), ("n_report_max", C.c_int32),
         ("P_conc", _dp), ("Tot_Washed", _dp),
         ("dt_follow", _dp), ("n_follow", C.c_int64), ("n_dt_ties", C.c_int64),
-        ("n_dt_mismatch", C.c_int64),
+        ("n_dt_mismatch", C.c_int64), ("stop_after_rollback", C.c_int32),
+        ("rollback_dt_s", C.c_double), ("rollback_max_dt_s", C.c_double),
     ]
 
 
@@ -71,6 +72,11 @@ def lib():
         L.oracle_run.argtypes = [C.POINTER(abi.RunConfig), C.POINTER(abi.RunInputs),
                                  C.POINTER(RunOut)]
         L.oracle_run.restype = C.c_int32
+        L.oracle_ga_params.argtypes = [C.c_double] * 5 + [_dp]
+        L.oracle_ga_params.restype = None
+        L.oracle_ga_cells.argtypes = [C.c_int64, C.c_int32, C.c_double, C.c_double, _dp, _dp,
+                                      _dp, _dp, _dp, _dp]
+        L.oracle_ga_cells.restype = None
         if hasattr(L, "oracle_ca_routing"):
             L.oracle_ca_routing.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp, C.c_double,
                                             C.c_double, _dp, C.c_double, _dp, C.c_int32,
@@ -214,8 +220,31 @@ def rain_avg(times, values, t0, t1, cursor=1):
     return v, cur.value
 
 
+def ga_params(ksat_mmh, psi_mm, dtheta, theta_sat_minus_r, lu_m=np.nan):
+    """Derived Green-Ampt parameters of one soil class: dict of ksat, psi,
+    dtheta, Lu_mm, k_out, Tr (HydroPol2D_preprocessing.m:2249-2253)."""
+    out = np.zeros(6)
+    lib().oracle_ga_params(ksat_mmh, psi_mm, dtheta, theta_sat_minus_r, lu_m, _p(out))
+    return dict(zip(("ksat", "psi", "dtheta", "Lu_mm", "k_out", "Tr"), out.tolist()))
+
+
+def ga_step(params, first_step, dt_h, d, I_p, T, I_min_mm=1e-3):
+    """One legacy explicit Green-Ampt update of the cells d (post-rain depth,
+    mm), I_p (mm), T (recovery clock, h) with one class's parameters (the
+    dict of ga_params).  Returns (d_after, I_t, T, f)."""
+    d = np.array(d, dtype=np.float64, ndmin=1)
+    I_p = np.ascontiguousarray(np.broadcast_to(np.asarray(I_p, dtype=np.float64), d.shape))
+    T = np.array(np.broadcast_to(np.asarray(T, dtype=np.float64), d.shape))
+    I_t = np.zeros_like(d)
+    f = np.zeros_like(d)
+    p6 = np.array([params[k] for k in ("ksat", "psi", "dtheta", "Lu_mm", "k_out", "Tr")])
+    lib().oracle_ga_cells(d.size, int(bool(first_step)), dt_h, I_min_mm, _p(p6), _p(d), _p(I_p),
+                          _p(I_t), _p(T), _p(f))
+    return d, I_t, T, f
+
+
 def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: int = 200000,
-        n_report_max: int = 64, dt_follow=None):
+        n_report_max: int = 64, dt_follow=None, stop_after_rollback: bool = False):
     """Full Main_While run on the oracle.  Returns a dict of final state + ledgers.
 
     dt_follow: the accepted-step dt sequence [s] of another implementation; the
@@ -257,11 +286,14 @@ def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: 
         follow = np.ascontiguousarray(dt_follow, dtype=np.float64)
         out.dt_follow = _p(follow)
         out.n_follow = follow.size
+    out.stop_after_rollback = int(stop_after_rollback)
     rc = lib().oracle_run(C.byref(cfg), C.byref(ins), C.byref(out))
     if rc != 0:
         raise RuntimeError(f"oracle_run failed with status {rc}")
     rep = out.rep.as_dict()
     res["report"] = rep
+    res["rollback_dt_s"] = out.rollback_dt_s
+    res["rollback_max_dt_s"] = out.rollback_max_dt_s
     res["n_hydro"] = out.n_hydro
     res["n_dt_ties"] = out.n_dt_ties
     res["n_dt_mismatch"] = out.n_dt_mismatch
