@@ -37,21 +37,27 @@ def parse():
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     p.add_argument("--graph-steps", type=int, default=50)
+    p.add_argument("--spinup-min", type=float, default=10.0,
+                   help="untimed model minutes run before the timed window")
+    p.add_argument("--spinup-max-steps", type=int, default=60000,
+                   help="cap on the spin-up steps (bounds the run's wall time)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=11,
                    help="steps of the CPU oracle sample (11 includes one groundwater call)")
     p.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
-    p.add_argument("--ensemble-members", type=int, default=64,
+    p.add_argument("--cpu-threads", type=int, default=0,
+                   help="threads of the all-core CPU leg (0: OMP_NUM_THREADS, else every core)")
+    p.add_argument("--ensemble-members", type=int, default=512,
                    help="members of the batched-ensemble measurement (0 skips it)")
-    p.add_argument("--ensemble-max-log2", type=int, default=9,
+    p.add_argument("--ensemble-max-log2", type=int, default=10,
                    help="largest member side 2^k (sides log-uniform from 2^7; 10 = the 1024^2 of "
                         "the 512-member configuration)")
-    p.add_argument("--ensemble-plain-sample", type=int, default=0, metavar="K",
+    p.add_argument("--ensemble-plain-sample", type=int, default=64, metavar="K",
                    help="time the plain per-member solver on every (members/K)-th member only and "
                         "scale by cell-steps (0: every member)")
-    p.add_argument("--surrogate-train", type=int, default=0, metavar="N",
-                   help="also run the surrogate study: train on N solved 256 x 256 members, judge "
-                        "256 more (0 skips it; about 2.5 min at N = 4096)")
+    p.add_argument("--surrogate-train", type=int, default=1536, metavar="N",
+                   help="the surrogate study: train on N solved 256 x 256 members, judge 256 "
+                        "more (0 skips it; about 1 min at N = 1536, 2.5 min at N = 4096)")
     p.add_argument("--surrogate-epochs", type=int, default=40)
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run 5 device copies of MIB MiB (the known-size kernel that "
@@ -203,6 +209,48 @@ def surrogate_study(n_train, epochs, n_eval=256, n_plain=8):
     }
 
 
+def step_bytes_per_cell(spec, elem, li_bytes):
+    """Algorithmic bytes per cell of one whole step of the full-physics loop
+    (FP32 state; the groundwater module keeps its head and ledgers in FP64),
+    kernel by kernel; each distinct per-cell array read once and written once
+    per launch, class tables and per-block partials left out."""
+    parts = {"li_step_kernel": li_bytes}
+    if spec.water_quality is not None:
+        # wq_first_kernel: read the LULC/flag word 2, B 4, q_x 4, q_y 4, Tot_Washed 4;
+        # write B 4, Tot_Washed 4, P_conc 4 (the internal sub-steps are guarded no-ops
+        # unless a split occurs)
+        parts["wq_first_kernel"] = 30
+    if spec.groundwater is not None:
+        every = spec.groundwater.every_steps
+        per_call = {
+            # bucket: read elevation 8, head 8, soil class 1, I 4, pending net / recharge /
+            # capillary 24, cumulative recharge 8; write I 4, the three pending 24,
+            # cumulative 8, recharge rate 8, q_exf 8
+            "gwm_recharge_kernel": 105,
+            # Boussinesq predictor faces: read h 8, z_bed 8; write Fx 8, Fy 8
+            "gw_flux_kernel (predictor)": 32,
+            # h_p = h + dt/Sy (-div F): read h, Fx, Fy, z_bed; write h_p
+            "gw_predict_kernel": 40,
+            # corrector faces averaged with the predictor's: read h_p, z_bed, Fx, Fy; write Fx, Fy
+            "gw_flux_kernel (corrector)": 48,
+            # head update + exfiltration: read h, Fx, Fy, R, z_bed; write q_exf, h
+            "gw_update_kernel": 56,
+            # Courant dt of the entry head (read h) and of the new head (read h and the
+            # catchment byte, write the masked head)
+            "gw_velocity_kernel x2": 25,
+            # takes over head and q_exf, returns exfiltration to the surface, resets the
+            # pending sums, excess: read elevation 8, h 8, q 8, d 4, class 1, I 4; write h 8,
+            # q 8, d 4, pending 24, excess 8
+            "gwm_post_kernel": 85,
+            # unsaturated-zone excess back to the surface: read I 4, excess 8, d 4; write I, d
+            "gwm_excess_kernel": 24,
+        }
+        parts["groundwater module per call"] = sum(per_call.values())
+        parts["groundwater module per step"] = sum(per_call.values()) / every
+    total = sum(v for k, v in parts.items() if k != "groundwater module per call")
+    return total, parts
+
+
 def algorithmic_bytes_per_cell(spec, elem):
     """Distinct per-cell arrays read once and written once per step."""
     reads = 5 * elem            # d, z, n^2 (outlet flag in the sign), q_x, q_y
@@ -239,6 +287,23 @@ def main():
     sim.run_fixed(args.warmup, graph_steps=min(args.graph_steps, max(args.warmup, 1)))
     tune = {"tj": None, "waves_per_block": None} if args.no_tune else sim.autotune()
     print(f"[bench] warm-up done, launch shape {tune}", file=sys.stderr, flush=True)
+    # untimed spin-up: the rain fills the domain and water starts to move; the
+    # same graph size as the timed window, so its graph is built here
+    ts = time.perf_counter()
+    spin = 0
+    while spin < args.spinup_max_steps:
+        chunk = min(2000, args.spinup_max_steps - spin)
+        sim.run_fixed(chunk, graph_steps=args.graph_steps)
+        spin += chunk
+        t_model = sim.report()["t_min"]
+        print(f"[bench] spin-up {spin} steps, t = {t_model:.3f} min, "
+              f"{time.perf_counter() - ts:.1f} s", file=sys.stderr, flush=True)
+        if t_model >= args.spinup_min:
+            break
+    # the graph of the timed window exists now: run one more graph's worth so
+    # that the timed interval holds launches only
+    sim.run_fixed(args.graph_steps, graph_steps=args.graph_steps)
+    t_start_min = sim.report()["t_min"]
     # timed region: K whole steps through captured graphs; run_fixed (like the
     # autotuner before it) returns only after synchronising the simulation's
     # stream, so the device is idle on both sides of the wall-clock interval
@@ -255,23 +320,39 @@ def main():
     rd, wr = algorithmic_bytes_per_cell(spec, elem)
     bytes_per_launch = (rd + wr) * cells
     achieved = bytes_per_launch / (li_ms / 1000.0) / 1e9
+    step_b, step_parts = step_bytes_per_cell(spec, elem, rd + wr)
+    step_gbs = step_b * cells / (ms_per_step / 1000.0) / 1e9
     rep = sim.report()
     sim.close()
     print(f"[bench] {args.steps} steps, {ms_per_step:.4f} ms/step, LI kernel {li_ms:.4f} ms "
-          f"({achieved:.0f} GB/s), t={rep['t_min']:.3f} min", file=sys.stderr, flush=True)
+          f"({achieved:.0f} GB/s), window t={t_start_min:.3f}..{rep['t_min']:.3f} min",
+          file=sys.stderr, flush=True)
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu_baseline:
         from oracle import oracle
         cpu_spec = synthetic.full_physics_c2(n, n, 2.0, precision="fp64", routing_time_min=1e6)
-        tc = time.perf_counter()
-        oracle.run(cpu_spec, max_steps=args.cpu_steps, n_dt_max=16)
-        cpu_s = time.perf_counter() - tc
-        cpu = {"value": cells * args.cpu_steps / cpu_s, "unit": "cell-updates/s", "cores": 1,
-               "kind": "port",
-               "sample": f"first {args.cpu_steps} steps of the same {n}x{n} workload, FP64 C "
-                         f"oracle, single thread, {cpu_s:.1f} s"}
-        print(f"[bench] cpu oracle {cpu['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", 0) or 0) or \
+            (os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        legs = {}
+        for mode, nthreads in (("st", 1), ("omp", threads)):
+            oracle.use_openmp(mode == "omp")
+            tc = time.perf_counter()
+            oracle.run(cpu_spec, max_steps=args.cpu_steps, n_dt_max=16)
+            legs[mode] = (time.perf_counter() - tc, nthreads)
+            print(f"[bench] cpu oracle ({nthreads} thread(s)): "
+                  f"{cells * args.cpu_steps / legs[mode][0]:.3e} cell-updates/s",
+                  file=sys.stderr, flush=True)
+        oracle.use_openmp(False)
+        sample = f"first {args.cpu_steps} steps (one groundwater call) of the same {n}x{n} workload"
+        cpu = {"value": cells * args.cpu_steps / legs["omp"][0], "unit": "cell-updates/s",
+               "cores": legs["omp"][1], "kind": "port", "nproc": os.cpu_count(),
+               "sample": f"{sample}, FP64 C oracle with OpenMP on its element-wise loops "
+                         f"({legs['omp'][1]} threads, reductions serial), {legs['omp'][0]:.1f} s"}
+        cpu1 = {"value": cells * args.cpu_steps / legs["st"][0], "unit": "cell-updates/s",
+                "cores": 1, "kind": "port",
+                "sample": f"{sample}, FP64 C oracle, one thread, {legs['st'][0]:.1f} s"}
 
     ens = (ensemble_throughput(args.ensemble_members, args.ensemble_max_log2,
                                args.ensemble_plain_sample)
@@ -292,12 +373,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"full-physics single domain {n}x{n} dx=2 m: fractal DEM, "
+        "config": {"workload": f"full-physics single domain {n}x{n} dx=2 m: sink-filled fractal DEM, "
                                "8 LULC Manning classes, 5 Green-Ampt soil classes, GRF rainfall "
                                "on a 2 h triangular hyetograph, critical-depth edge outlets, one "
                                "pollutant (build-up / wash-off), groundwater module (Sy=0.2, "
                                "K=1e-5 m/s) every 10 steps; fused rain+GA+local-inertial step, "
-                               "wash-off passes, dt control",
+                               "wash-off passes, dt control; timed window after "
+                               f"{t_start_min:.1f} min of model time",
                    "cells": cells, "precision": args.precision,
                    "launch": {"cols_per_wave": tune["tj"], "waves_per_block": tune["waves_per_block"]},
                    "bytes_per_cell_update": rd + wr},
@@ -306,7 +388,12 @@ def main():
                      "algorithmic_bytes": bytes_per_launch,
                      "kernel": "li_step_kernel", "avg_kernel_ms": li_ms,
                      "launches_timed": launches, "graph_event_ms": graph_ms},
+        "step_roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
+                          "bytes_per_cell_step": step_b, "per_kernel_bytes_per_cell": step_parts,
+                          "ms_per_step": ms_per_step},
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "ensemble": ens,
         "surrogate": sur,
     }

@@ -31,6 +31,7 @@
  * (i,j)|(i+1,j); the last column / row hold 0 and are never read. */
 static void gw_fluxes(int32_t ny, int32_t nx, const double *h, const double *z0, const double *K,
                       const double *Sy, double dx, double dy, double dt, double *Fx, double *Fy) {
+    ORACLE_PAR
     for (int32_t j = 0; j < nx; ++j)
         for (int32_t i = 0; i < ny; ++i) {
             const int64_t c = IX(i, j);
@@ -150,6 +151,7 @@ int32_t oracle_boussinesq_2d(int32_t ny, int32_t nx, double dt_in, double dx, do
         dt = fmin(dt_gw, dt);
         /* predictor */
         gw_fluxes(ny, nx, h_t, z0, K, Sy, dx, dy, dt, Fx1, Fy1);
+        ORACLE_PAR
         for (int32_t j = 0; j < nx; ++j)
             for (int32_t i = 0; i < ny; ++i) {
                 const int64_t c = IX(i, j);
@@ -159,10 +161,12 @@ int32_t oracle_boussinesq_2d(int32_t ny, int32_t nx, double dt_in, double dx, do
             }
         /* corrector and averaged fluxes */
         gw_fluxes(ny, nx, hp, z0, K, Sy, dx, dy, dt, Fx2, Fy2);
+        ORACLE_PAR
         for (int64_t c = 0; c < n; ++c) {
             Fx2[c] = 0.5 * (Fx1[c] + Fx2[c]);
             Fy2[c] = 0.5 * (Fy1[c] + Fy2[c]);
         }
+        ORACLE_PAR
         for (int32_t j = 0; j < nx; ++j)
             for (int32_t i = 0; i < ny; ++i) {
                 const int64_t c = IX(i, j);
@@ -173,6 +177,7 @@ int32_t oracle_boussinesq_2d(int32_t ny, int32_t nx, double dt_in, double dx, do
                 tmp[c] = fmin(v, h_surf[c]);
             }
         dt_gw = gw_velocities(ny, nx, tmp, K, dx, dy, courant, catchment, u_x, u_y);
+        ORACLE_PAR
         for (int64_t c = 0; c < n; ++c) {
             h_t[c] = (catchment && !catchment[c]) ? NAN : tmp[c];
             if (catchment && !catchment[c]) q_exf[c] = NAN;
@@ -206,6 +211,7 @@ int32_t oracle_gw_recharge(int64_t n, const double *infiltration_rate, const dou
                            const double *alpha_vg, double l_vg, const double *ksat_mmh,
                            double *recharge_rate, double *S_out, double *cumulative_recharge) {
     dt = fmax(dt, 1e-12);
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         const double nv = n_vg[c], m = 1.0 - 1.0 / nv, a = alpha_vg[c];
         const double ts = theta_s[c], tr = theta_r[c];

@@ -50,6 +50,7 @@ int32_t oracle_groundwater_module(const oracle_gwm_params *p, oracle_gwm_state *
     const double max_head_change = fmax(p->max_head_change_m, DBL_EPSILON);
     const double courant = fmax(fmin(p->courant, 1.0), DBL_EPSILON);
     /* :42-62 zero_matrix, zwt, UZ capacity */
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         const int nan = isnan(p->elevation[c]);
         zero[c] = nan ? NAN : 0.0;
@@ -68,8 +69,10 @@ int32_t oracle_groundwater_module(const oracle_gwm_params *p, oracle_gwm_state *
     oracle_gw_recharge(n, zero, S0, dt_s, minS, uzmax, p->theta_sat, p->theta_r, p->n_vg,
                        p->alpha_vg, p->l_vg, p->ksat_mmh, Rdown, S_new, cumulative_recharge);
     free(minS);
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) I_t[c] = S_new[c] * 1000.0;
     /* :124-150 exchange (capillary rate = 0 without layered soil) and pending state */
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         const int nan = isnan(p->elevation[c]);
         const double cap = zero[c];
@@ -118,6 +121,7 @@ int32_t oracle_groundwater_module(const oracle_gwm_params *p, oracle_gwm_state *
     const int final_step = p->t_min >= (p->routing_time_min - 1e-9);
     const int due = p->flag_async == 0 ? 1 : (g->elapsed_s >= target || head_due || final_step);
     const double dt_gw = fmax(g->elapsed_s, DBL_EPSILON);
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         rfg[c] = due ? g->pending_net[c] / dt_gw : zero[c];
         if (due && isnan(p->elevation[c])) rfg[c] = NAN;

@@ -44,6 +44,7 @@ void oracle_inertial_solver(int32_t scheme, int64_t n_cells, int32_t n_pages,
     (void)scheme;
     const double e73 = 7.0 / 3.0;
     for (int32_t p = 0; p < n_pages; ++p) {
+        ORACLE_PAR
         for (int64_t c = 0; c < n_cells; ++c) {
             int64_t o = c + (int64_t)p * n_cells;
             double num = q_p[o] - G * Hf[o] * dt * S[o];
@@ -63,13 +64,19 @@ static void coarse_draining_limiter(int32_t ny, int32_t nx, double *Qx, double *
     int64_t n = (int64_t)ny * nx;
     double *avail = zalloc(n), *out = zalloc(n), *scale = zalloc(n);
     double dtm = fmax(dt, DBL_EPSILON);
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) avail[c] = fmax(d_tot[c] / 1000.0, 0.0) * cell_area / dtm;
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) out[c] = 0.0 + fmax(Qx[c], 0.0);
+    ORACLE_PAR
     for (int32_t j = 1; j < nx; ++j)
         for (int32_t i = 0; i < ny; ++i) out[IX(i, j)] += fmax(-Qx[IX(i, j - 1)], 0.0);
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) out[c] += fmax(Qy[c], 0.0);
+    ORACLE_PAR
     for (int32_t j = 0; j < nx; ++j)
         for (int32_t i = 0; i < ny - 1; ++i) out[IX(i, j)] += fmax(-Qy[IX(i + 1, j)], 0.0);
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         double s = 1.0;
         if (out[c] > avail[c] && out[c] > 0.0) s = avail[c] / out[c];
@@ -78,6 +85,7 @@ static void coarse_draining_limiter(int32_t ny, int32_t nx, double *Qx, double *
         scale[c] = s;
     }
     if (nx > 1) {
+        ORACLE_PAR
         for (int32_t j = 0; j < nx - 1; ++j)
             for (int32_t i = 0; i < ny; ++i) {
                 double q = Qx[IX(i, j)];
@@ -85,9 +93,11 @@ static void coarse_draining_limiter(int32_t ny, int32_t nx, double *Qx, double *
                 else if (q < 0) q = q * scale[IX(i, j + 1)];
                 Qx[IX(i, j)] = q;
             }
+        ORACLE_PAR
         for (int32_t i = 0; i < ny; ++i) Qx[IX(i, nx - 1)] = 0.0;
     }
     if (ny > 1) {
+        ORACLE_PAR
         for (int32_t j = 0; j < nx; ++j)
             for (int32_t i = 1; i < ny; ++i) {
                 double q = Qy[IX(i, j)];
@@ -95,8 +105,10 @@ static void coarse_draining_limiter(int32_t ny, int32_t nx, double *Qx, double *
                 else if (q < 0) q = q * scale[IX(i - 1, j)];
                 Qy[IX(i, j)] = q;
             }
+        ORACLE_PAR
         for (int32_t j = 0; j < nx; ++j) Qy[IX(0, j)] = 0.0;
     }
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         if (!isfinite(Qx[c])) Qx[c] = 0.0;
         if (!isfinite(Qy[c])) Qy[c] = 0.0;
@@ -129,17 +141,20 @@ int32_t oracle_li_d4(int32_t ny, int32_t nx, int32_t scheme, const double *z,
     double *q = zalloc(3 * n);
 
     /* :31 depth_cell = max(d_tot/1000,0); :43 y = z_dem + depth_cell */
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         depth[c] = fmax(d_tot[c] / 1000.0, 0.0);
         y[c] = z[c] + depth[c];
     }
     /* :145-154 matrix_store = 0*outflow; Hf = 0*outflow (page 3 keeps 0*outflow) */
+    ORACLE_PAR
     for (int64_t c = 0; c < 3 * n; ++c) {
         S[c] = 0.0 * outflow_prev[c];
         Hf[c] = 0.0 * outflow_prev[c];
     }
     double *Sx = S, *Sy = S + n, *Hfx = Hf, *Hfy = Hf + n;
     /* :91-92, :123-128, :133-142 */
+    ORACLE_PAR
     for (int32_t j = 0; j < nx; ++j)
         for (int32_t i = 0; i < ny; ++i) {
             int64_t c = IX(i, j);
@@ -163,14 +178,17 @@ int32_t oracle_li_d4(int32_t ny, int32_t nx, int32_t scheme, const double *z,
             }
         }
     /* :184 Hf(Hf<0)=0 */
+    ORACLE_PAR
     for (int64_t c = 0; c < 3 * n; ++c)
         if (Hf[c] < 0) Hf[c] = 0.0;
     /* :198 outflow = outflow/1000/3600*Resolution^2/Resolution  [m2/s] */
     const double R2 = R * R;
+    ORACLE_PAR
     for (int64_t c = 0; c < 3 * n; ++c) qprev[c] = outflow_prev[c] / 1000.0 / 3600.0 * R2 / R;
     const double dt = time_step_min * 60.0; /* :201 */
     /* :263-264 */
     oracle_inertial_solver(scheme, n, 3, qprev, dt, Hf, S, roughness_squared, q);
+    ORACLE_PAR
     for (int64_t c = 0; c < 3 * n; ++c)
         if (!isfinite(q[c])) q[c] = 0.0;
     /* :272-275 critical velocity cap */
@@ -181,19 +199,23 @@ int32_t oracle_li_d4(int32_t ny, int32_t nx, int32_t scheme, const double *z,
         }
     }
     /* :292-294 velocity cap: q = min(max(q,-10 Hf),10 Hf) */
+    ORACLE_PAR
     for (int64_t c = 0; c < 3 * n; ++c) {
         double thr = Hf[c] * 10.0;
         q[c] = fmin(fmax(q[c], -thr), thr);
     }
     /* :304-306 outflow_rate = q.*cell_width; limiter */
     double *Q = q; /* reuse: Q = q*R in place */
+    ORACLE_PAR
     for (int64_t c = 0; c < 3 * n; ++c) Q[c] = q[c] * R;
     coarse_draining_limiter(ny, nx, Q, Q + n, Q + 2 * n, d_tot, dt, cell_area);
     /* :309 outflow = outflow_rate./cell_area*1000*3600 */
+    ORACLE_PAR
     for (int64_t c = 0; c < 3 * n; ++c) outflow[c] = Q[c] / cell_area * 1000.0 * 3600.0;
     const double *Qx = Q, *Qy = Q + n;
     /* :313-315 Vol_Flux; :390 d_t = d_tot + 1000*Vol_Flux/C_a; :415 */
     const double C_a = R * R; /* :266 */
+    ORACLE_PAR
     for (int32_t j = 0; j < nx; ++j)
         for (int32_t i = 0; i < ny; ++i) {
             int64_t c = IX(i, j);
@@ -205,6 +227,7 @@ int32_t oracle_li_d4(int32_t ny, int32_t nx, int32_t scheme, const double *z,
             d_t[c] = dn;
         }
     /* :347-350 output fluxes (from matrix_store = outflow) */
+    ORACLE_PAR
     for (int32_t j = 0; j < nx; ++j)
         for (int32_t i = 0; i < ny; ++i) {
             int64_t c = IX(i, j);
@@ -214,6 +237,7 @@ int32_t oracle_li_d4(int32_t ny, int32_t nx, int32_t scheme, const double *z,
             if (qout_down) qout_down[c] = -((i < ny - 1) ? outflow[n + IX(i + 1, j)] : 0.0);
         }
     /* :418-498 outlet */
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         if (outlet_flow) outlet_flow[c] = 0.0;
         outflow[2 * n + c] = 0.0;
@@ -277,6 +301,7 @@ double oracle_refresh_dt(int32_t ny, int32_t nx, const double *qout_right,
     const double R = p->resolution;
     double *vr = zalloc(n), *vu = zalloc(n), *dts = zalloc(n);
     /* :63-77 */
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         double f1 = Hf[c] * 1000.0, f2 = Hf[n + c] * 1000.0, f3 = Hf[2 * n + c] * 1000.0;
         dts[c] = fmax(fmax(f1 / 1000.0, f2 / 1000.0), f3 / 1000.0);
@@ -452,7 +477,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     double *outflow = zalloc(3 * n), *outflow_prev = zalloc(3 * n), *Hf = zalloc(3 * n);
     double *qr = zalloc(n), *qu = zalloc(n), *qof = zalloc(n), *dmax = zalloc(n);
     double *I_t = zalloc(n), *I_p = zalloc(n), *Tclk = zalloc(n), *vraster = zalloc(n);
-    double *area_nan = zalloc(n);
+    double *area_nan = zalloc(n), *ga_f = zalloc(n);
     uint8_t *is_nan = (uint8_t *)calloc((size_t)n, 1);
     int64_t *oidx = (int64_t *)calloc((size_t)n, sizeof(int64_t));
     int32_t n_out = 0;
@@ -657,6 +682,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
          * Infiltration_Module.m:128-172) */
         if (do_ga) {
             double dt_h = time_step / 60.0;
+            ORACLE_PAR
             for (int64_t c = 0; c < n; ++c) {
                 int32_t cls = in->soil_class ? in->soil_class[c] : 0;
                 double f = ga_cell(&ga[cls], k == 1, dt_h, in->I_min_mm, d_t[c], I_p[c], &Tclk[c],
@@ -666,8 +692,10 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                     f = NAN;
                 }
                 d_t[c] = d_t[c] - f * dt_h;
-                if (!isnan(f)) inf_vol_step += f * dt_h / 1000.0 * A;
+                ga_f[c] = f;
             }
+            for (int64_t c = 0; c < n; ++c)
+                if (!isnan(ga_f[c])) inf_vol_step += ga_f[c] * dt_h / 1000.0 * A;
         }
         /* Hydrological_Model.m:30 (no inflow), :45-55 */
         double dmin = NAN;
@@ -791,6 +819,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             }
             last_record = actual;
             /* :588-600 dmax */
+            ORACLE_PAR
             for (int64_t c = 0; c < n; ++c) dmax[c] = (k == 1) ? d_t[c] : fmax(d_t[c], dmax[c]);
             /* :611-617 */
             runoff_mm = qsum * time_step / 60.0 / drainage_area + runoff_mm;
@@ -962,7 +991,8 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     }
     free(z); free(nman); free(n2); free(d_t); free(d_p); free(d_tot);
     free(outflow); free(outflow_prev); free(Hf); free(qr); free(qu); free(qof); free(dmax);
-    free(I_t); free(I_p); free(Tclk); free(vraster); free(area_nan); free(is_nan); free(oidx);
+    free(I_t); free(I_p); free(Tclk); free(vraster); free(area_nan); free(ga_f); free(is_nan);
+    free(oidx);
     free(ga);
     free(B); free(C3m); free(C4m); free(Pc); free(TW); free(ql); free(qd);
     free(gw_h); free(gw_sd); free(gw_ts); free(gw_tr); free(gw_nv); free(gw_al); free(gw_ks);

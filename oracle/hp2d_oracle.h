@@ -18,6 +18,15 @@
 #include <stdint.h>
 #include "../include/hp2d.h"
 
+/* Element-wise loops (no accumulation across cells) may run on several
+ * threads when the oracle is built with OpenMP (the all-core CPU baseline);
+ * results are bit-identical either way, since no summation order changes. */
+#ifdef _OPENMP
+#define ORACLE_PAR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORACLE_PAR
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif

@@ -29,6 +29,7 @@ static void washoff_rates(int32_t ny, int32_t nx, const double *C3, const double
                           double min_Bt, double *W, double *totW) {
     const int64_t n = (int64_t)ny * nx;
     const double Br_kg = Bmin / 1000.0 * cell_area, Bm_kg = Bmax / 1000.0 * cell_area;
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         double f_bt = B_t[c];
         if (model == 1) {
@@ -52,6 +53,7 @@ static void washoff_rates(int32_t ny, int32_t nx, const double *C3, const double
 /* dW = tot_W_out - sum of the four inflows (:51-59) */
 static void net_rate(int32_t ny, int32_t nx, const double *W, const double *totW, double *dW) {
     const int64_t n = (int64_t)ny * nx;
+    ORACLE_PAR
     for (int32_t j = 0; j < nx; ++j)
         for (int32_t i = 0; i < ny; ++i) {
             int64_t c = IX(i, j);
@@ -95,6 +97,7 @@ int32_t oracle_washoff(int32_t ny, int32_t nx, const double *C3, const double *C
     const double large_timestep = time_step_min * 60.0;
     memcpy(Bx, B_t, sizeof(double) * n); /* B_t_extra = B_t (:19) */
     washoff_rates(ny, nx, C3, C4, q5, B_t, Bx, cell_area, model, Bmin, Bmax, min_Bt, W, totW);
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) {
         double tq = 0.0;
         for (int p = 0; p < 5; ++p) tq += q5[p * n + c]; /* sum(q_out_t,3), signed */
@@ -108,7 +111,9 @@ int32_t oracle_washoff(int32_t ny, int32_t nx, const double *C3, const double *C
     net_rate(ny, nx, W, totW, dW);
     double tmin = tmin_of(B_t, dW, n, cell_area, min_Bt, large_timestep); /* :61-76 */
     double *Bmid = malloc(sizeof(double) * n);
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) Bmid[c] = B_t[c] - dW[c] * time_step_min / 60.0;
+    ORACLE_PAR
     for (int64_t c = 0; c < n; ++c) Wout5[c] = W[4 * n + c];
     double lost = 0.0;
     const int32_t split = nanmin_all(Bmid, n) < 0 || tmin < time_step_min * 60.0;

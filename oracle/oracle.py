@@ -16,16 +16,29 @@ from hydropol2d_amd.runspec import RunSpec
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _SO = os.path.join(_HERE, "_build", "libhp2d_oracle.so")
-_lib = None
+_SO_OMP = os.path.join(_HERE, "_build", "libhp2d_oracle_omp.so")
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(_SO):
+    if force or not os.path.exists(_SO) or not os.path.exists(_SO_OMP):
         subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _SO
+
+
+def use_openmp(on: bool = True):
+    """Route the following calls to the OpenMP build of the oracle (the
+    all-core CPU baseline: its element-wise loops run on OMP_NUM_THREADS
+    threads, with results bit-identical to the single-threaded build), or
+    back to the single-threaded one."""
+    global _mode
+    _mode = "omp" if on else "st"
+
+
+_mode = "st"
+_libs = {}
 
 
 class DtParams(C.Structure):
@@ -53,10 +66,15 @@ class RunOut(C.Structure):
 
 
 def lib():
-    global _lib
-    if _lib is None:
+    if _mode not in _libs:
         build()
-        L = C.CDLL(_SO)
+        _libs[_mode] = _load(_SO_OMP if _mode == "omp" else _SO)
+    return _libs[_mode]
+
+
+def _load(path):
+    if True:
+        L = C.CDLL(path)
         L.oracle_inertial_solver.argtypes = [C.c_int32, C.c_int64, C.c_int32, _dp, C.c_double,
                                              _dp, _dp, _dp, _dp]
         L.oracle_li_d4.argtypes = [C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp,
@@ -106,8 +124,7 @@ def lib():
         L.oracle_groundwater_module.argtypes = [C.POINTER(GwmParams), C.POINTER(GwmState)] + \
             [_dp] * 6 + [_dp, C.POINTER(C.c_int32), _dp]
         L.oracle_groundwater_module.restype = C.c_int32
-        _lib = L
-    return _lib
+    return L
 
 
 class GwmParams(C.Structure):
