@@ -2,19 +2,22 @@
 """Throughput of the HydroPol2D explicit surface-water step on one MI355X.
 
 A "step" is one iteration of the device-resident Main_While loop on the
-4096 x 4096 full-physics workload (fractal DEM, 8 LULC classes, 5 Green-Ampt
+4096 x 4096 full-physics workload (sink-filled fractal DEM, 8 LULC classes, 5 Green-Ampt
 soil classes, Gaussian-random-field rainfall on a triangular hyetograph,
 critical-depth outlets on one edge, one pollutant with build-up / wash-off,
 the groundwater module every 10 surface steps): the fused rainfall +
 Green-Ampt + local-inertial kernel, the wash-off passes, the one-block
 dt/bookkeeping kernel, and every 10th step the groundwater module (recharge
-bucket + Boussinesq).  Inputs are resident in HBM before timing; steps run as
-captured hipGraphs.
+bucket + Boussinesq).  Inputs are resident in HBM before timing; the model
+first spins up untimed for 10 min of model time; steps run as captured
+hipGraphs, built before the timed window.
 
-Prints one JSON line (value = cell-updates/s of the whole job).  Beside it:
-the batched-ensemble study (members/hour against the plain per-member loop,
---ensemble-members) and, on request, the surrogate study (members/hour of
-the surrogate with its 5 % solver judge, --surrogate-train).
+Prints one JSON line (value = cell-updates/s of the whole job), with the
+roofline of the dominant kernel, that of the whole step, and the FP64 CPU
+oracle on one thread and on all host cores.  Beside it: the batched-ensemble
+study (members/hour against the plain per-member loop, --ensemble-members)
+and the surrogate study (members/hour of the surrogate with its 5 % solver
+judge, --surrogate-train).
 """
 from __future__ import annotations
 
