@@ -104,6 +104,11 @@ static void set_err(hp2d_handle *h, const std::string &m) {
         }                                                                             \
     } while (0)
 
+namespace hp2d {
+// the thread's last error, for entry points defined in other translation units
+void set_global_error(const std::string &m) { g_last_error = m; }
+}  // namespace hp2d
+
 extern "C" {
 
 const char *hp2d_version(void) { return "hp2d-gfx950 0.1.0"; }

@@ -17,8 +17,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <string>
 
 #include "hp2d.h"
+
+namespace hp2d {
+void set_global_error(const std::string &m);
+}
 
 namespace hp2d {
 namespace nn {
@@ -452,6 +458,67 @@ hp2d_status hp2d_nn_conv3x3_dgrad(int32_t N, int32_t H, int32_t W, int32_t Cin, 
     if (hipGetLastError() != hipSuccess) return HP2D_E_HIP;
     return hp2d_nn_conv3x3_fwd(N, H, W, Cout, Cin, gy, w_flip, nullptr, 0, nullptr, gx, nullptr,
                                stream);
+}
+
+}  // extern "C"
+
+// Device scratch of the allocation-free data gradient: the flipped, transposed
+// weights of one layer.  Every GPU holds its own copy of the array until the
+// process ends; its device address differs per GPU and is taken with
+// hipGetSymbolAddress on the first call on each device (the array's name is a
+// host-side stand-in and never reaches a kernel).
+static constexpr long long kNoAllocFlipMax = 9LL * 512 * 256;  // bf16 elements, 2.25 MiB
+static __device__ __align__(256) uint16_t scratch[kNoAllocFlipMax];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+uint16_t *scratch_ptr[kMaxDevices] = {};
+
+// the current device's address of `scratch`, or nullptr with the error set
+uint16_t *scratch_on_current_device() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        hp2d::set_global_error("hp2d_nn_conv3x3_dgrad_noalloc: no current HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            hp2d::set_global_error(std::string("hp2d_nn_conv3x3_dgrad_noalloc: "
+                                               "hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        scratch_ptr[dev] = static_cast<uint16_t *>(p);
+    }
+    return scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+hp2d_status hp2d_nn_conv3x3_dgrad_noalloc(int32_t N, int32_t H, int32_t W, int32_t Cin,
+                                          int32_t Cout, const uint16_t *gy, const uint16_t *w,
+                                          uint16_t *gx, void *stream) {
+    if (!gy || !w || !gx || N < 1 || H < 1 || W < 1 || Cin < 32 || Cout < 32 || Cin % 32 ||
+        Cout % 32) {
+        hp2d::set_global_error("hp2d_nn_conv3x3_dgrad_noalloc: bad arguments");
+        return HP2D_E_ARG;
+    }
+    if (9LL * Cin * Cout > kNoAllocFlipMax) {
+        hp2d::set_global_error("hp2d_nn_conv3x3_dgrad_noalloc: 9*Cin*Cout = " +
+                               std::to_string(9LL * Cin * Cout) + " exceeds the " +
+                               std::to_string(kNoAllocFlipMax) +
+                               " bf16 weights of the device scratch (Cin*Cout <= 131072)");
+        return HP2D_E_ARG;
+    }
+    uint16_t *w_flip = scratch_on_current_device();
+    if (!w_flip) return HP2D_E_HIP;
+    // the same kernels as hp2d_nn_conv3x3_dgrad, with the workspace in scratch;
+    // the flip overwrites every element the forward GEMM then reads
+    return hp2d_nn_conv3x3_dgrad(N, H, W, Cin, Cout, gy, w, w_flip, gx, stream);
 }
 
 hp2d_status hp2d_nn_conv3x3_wgrad(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,

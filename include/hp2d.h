@@ -486,6 +486,16 @@ hp2d_status hp2d_nn_conv3x3_fwd(int32_t N, int32_t H, int32_t W, int32_t Cin, in
 hp2d_status hp2d_nn_conv3x3_dgrad(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
                                   const uint16_t *gy, const uint16_t *w, uint16_t *w_flip,
                                   uint16_t *gx, void *stream);
+/* hp2d_nn_conv3x3_dgrad without the w_flip workspace: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops).  It allocates nothing: the flipped weights
+ * live in a fixed per-GPU device array of 9*512*256 bf16 (2.25 MiB, held
+ * until the process ends), so 9*Cin*Cout must not exceed 1179648
+ * (HP2D_E_ARG otherwise).  Calls that overlap on one GPU share that array
+ * and race; the results equal hp2d_nn_conv3x3_dgrad's bit for bit. */
+hp2d_status hp2d_nn_conv3x3_dgrad_noalloc(int32_t N, int32_t H, int32_t W, int32_t Cin,
+                                          int32_t Cout, const uint16_t *gy, const uint16_t *w,
+                                          uint16_t *gx, void *stream);
 /* gw [Cout][9*Cin] += d/dw, gb [Cout] += d/db (gb may be NULL), FP32. */
 hp2d_status hp2d_nn_conv3x3_wgrad(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
                                   const uint16_t *x, const uint16_t *gy, float *gw, float *gb,
