@@ -256,7 +256,9 @@ def step_bytes_per_cell(spec, elem, li_bytes):
 
 def algorithmic_bytes_per_cell(spec, elem):
     """Distinct per-cell arrays read once and written once per step."""
-    reads = 5 * elem            # d, z, n^2 (outlet flag in the sign), q_x, q_y
+    # d, n^2 (outlet flag in the sign), q_x, q_y and the elevation: FP64 reads z,
+    # FP32 the two face elevation differences (dz_x, dz_y)
+    reads = 4 * elem + (elem if elem == 8 else 2 * elem)
     writes = 3 * elem           # d, q_x, q_y
     if spec.rain_pattern is not None:
         reads += elem           # rainfall multiplier map

@@ -79,7 +79,8 @@ struct hp2d_handle {
     hipStream_t stream = nullptr;
     std::string err;
     size_t elem = 8;
-    void *dev[20] = {nullptr};   // z, d, n2s, qx, qy, d_t, ox, oy, q3, hfx, hfy, hf3, qof, ql, qr, qu, qd, itot
+    void *dev[20] = {nullptr};   // z (FP32: dzx), d, n2s, qx, qy, d_t, ox, oy, q3, hfx, hfy, hf3, qof,
+                                 // ql, qr, qu, qd, itot, dzy (FP32)
     long long *out_idx = nullptr;
     void *out_n = nullptr;
     int out_cap = 0;
@@ -144,7 +145,7 @@ hp2d_status hp2d_create(int32_t ny, int32_t nx, int32_t precision, hp2d_handle *
     h->tj = default_tj(nx);
     const size_t n = (size_t)ny * nx;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-    for (int i = 0; i < 18 && e == hipSuccess; ++i) e = hipMalloc(&h->dev[i], n * h->elem);
+    for (int i = 0; i < 19 && e == hipSuccess; ++i) e = hipMalloc(&h->dev[i], n * h->elem);
     if (e != hipSuccess) {
         g_last_error = std::string("hp2d_create: ") + hipGetErrorString(e);
         hp2d_destroy(h);
@@ -222,6 +223,24 @@ void *hp2d_handle_workspace2(hp2d_handle *h, size_t bytes) {
     return h->ws2;
 }
 
+// Elevation differences of the x and y faces a cell stores, for the FP32 step
+// kernel: dzx(i,j) = z(i,j+1) - z(i,j) (0 in the last column), dzy(i,j) =
+// z(i-1,j) - z(i,j) (0 in the first row), each taken in FP64 and rounded once,
+// NaN when either cell lies outside the domain.
+template <typename Z>
+static void face_dz(int ny, int nx, const Z *z, std::vector<float> &dzx, std::vector<float> &dzy) {
+    const size_t n = (size_t)ny * nx;
+    dzx.assign(n, 0.0f);
+    dzy.assign(n, 0.0f);
+    for (int j = 0; j < nx; ++j)
+        for (int i = 0; i < ny; ++i) {
+            const size_t c = (size_t)i + (size_t)j * ny;
+            if (j < nx - 1) dzx[c] = (float)((double)z[c + ny] - (double)z[c]);
+            if (i > 0) dzy[c] = (float)((double)z[c - 1] - (double)z[c]);
+            if (std::isnan((double)z[c])) dzx[c] = dzy[c] = NAN;
+        }
+}
+
 template <typename T>
 static hp2d_status li_dropin(hp2d_handle *h, int scheme, const T *z, const T *d_tot,
                              const T *rough, const T *rough2, T cell_area, T time_step_min,
@@ -253,6 +272,14 @@ static hp2d_status li_dropin(hp2d_handle *h, int scheme, const T *z, const T *d_
         n2s[oidx[k]] = std::copysign(std::fabs(n2s[oidx[k]]), T(-1));
         on[k] = rough[oidx[k]];
     }
+    // the FP32 kernel reads the face elevation differences, and marks the cells
+    // outside the domain by a NaN n^2
+    std::vector<float> dzx, dzy;
+    if (sizeof(T) == 4) {
+        face_dz(ny, nx, z, dzx, dzy);
+        for (size_t c = 0; c < n; ++c)
+            if (std::isnan((double)z[c])) n2s[c] = T(NAN);
+    }
     if ((int)oidx.size() > h->out_cap) {
         if (h->out_idx) hipFree(h->out_idx);
         if (h->out_n) hipFree(h->out_n);
@@ -264,7 +291,12 @@ static hp2d_status li_dropin(hp2d_handle *h, int scheme, const T *z, const T *d_
     }
     hipStream_t s = h->stream;
     T **D = (T **)h->dev;
-    HTRY(h, hipMemcpyAsync(D[0], z, n * sizeof(T), hipMemcpyHostToDevice, s));
+    if (sizeof(T) == 8) {
+        HTRY(h, hipMemcpyAsync(D[0], z, n * sizeof(T), hipMemcpyHostToDevice, s));
+    } else {
+        HTRY(h, hipMemcpyAsync(D[0], dzx.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+        HTRY(h, hipMemcpyAsync(D[18], dzy.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    }
     HTRY(h, hipMemcpyAsync(D[1], d_tot, n * sizeof(T), hipMemcpyHostToDevice, s));
     HTRY(h, hipMemcpyAsync(D[2], n2s.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
     HTRY(h, hipMemcpyAsync(D[3], outflow_prev, n * sizeof(T), hipMemcpyHostToDevice, s));
@@ -294,6 +326,8 @@ static hp2d_status li_dropin(hp2d_handle *h, int scheme, const T *z, const T *d_
     a.dt_s = time_step_min * T(60);
     a.dt_h = time_step_min / T(60);
     a.z = D[0];
+    a.dzx = D[0];
+    a.dzy = D[18];
     a.d_src = D[1];
     a.n2s = D[2];
     a.qx_src = D[3];
@@ -401,6 +435,7 @@ struct hp2d_sim {
     std::string err;
     // device memory
     void *z = nullptr, *n2s = nullptr, *pattern = nullptr, *dmax = nullptr, *tpeak = nullptr;
+    void *zy = nullptr;  // FP32: dzy (z holds dzx)
     void *d[2] = {nullptr, nullptr}, *qx[2] = {nullptr, nullptr}, *qy[2] = {nullptr, nullptr};
     void *I[2] = {nullptr, nullptr}, *Tc[2] = {nullptr, nullptr};
     uint8_t *soil = nullptr;
@@ -422,7 +457,7 @@ struct hp2d_sim {
     double *vol_err = nullptr;
     double *tune_sink = nullptr;
     int n_rep_max = 256;
-    double sum_pattern = 0, n_valid = 0, drainage_area = 0, datum = 0, I_min = 0;
+    double sum_pattern = 0, n_valid = 0, drainage_area = 0, I_min = 0;
     int tj = 16, wpb = 4;
     // graph
     hipGraphExec_t graph = nullptr;
@@ -490,6 +525,8 @@ template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
     a.depth_tol_mm = T(c.depth_tolerance_mm);
     a.alfa_min = T(c.alfa_min);
     a.z = (const T *)s->z;
+    a.dzx = (const T *)s->z;
+    a.dzy = (const T *)s->zy;
     a.n2s = (const T *)s->n2s;
     for (int p = 0; p < 2; ++p) {
         a.d_buf[p] = (T *)s->d[p];
@@ -711,9 +748,6 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         n2s[oidx[k]] = std::copysign(std::fabs(n2s[oidx[k]]), -1.0);
         on[k] = in->roughness[oidx[k]];
     }
-    // datum shift in FP32 (translation invariant: only differences of z and of
-    // the water surface enter the step)
-    s->datum = (cfg->precision == HP2D_FP32) ? zmin : 0.0;
     // Green-Ampt classes (HydroPol2D_preprocessing.m:2249-2253)
     std::vector<GaClass> gac;
     const bool ga = cfg->flag_infiltration == 1;
@@ -832,6 +866,7 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         if (e == hipSuccess) e = hipMalloc(p, bytes);
     };
     A_(&s->z, nb);
+    if (s->elem == 4) A_(&s->zy, nb);
     A_(&s->n2s, nb);
     for (int p = 0; p < 2; ++p) {
         A_(&s->d[p], nb);
@@ -912,8 +947,20 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
             up(dst, w.data(), w.size() * 4);
         }
     };
-    up_real(s->z, z, s->datum);
-    up_real(s->n2s, n2s, 0.0);
+    if (s->elem == 8) {
+        up_real(s->z, z, 0.0);
+        up_real(s->n2s, n2s, 0.0);
+    } else {
+        // FP32: the face elevation differences, and NaN n^2 outside the domain
+        std::vector<float> dzx, dzy;
+        face_dz(ny, s->nx, z.data(), dzx, dzy);
+        up(s->z, dzx.data(), n * 4);
+        up(s->zy, dzy.data(), n * 4);
+        std::vector<double> n2m = n2s;
+        for (size_t c = 0; c < n; ++c)
+            if (std::isnan(z[c])) n2m[c] = NAN;
+        up_real(s->n2s, n2m, 0.0);
+    }
     std::vector<double> zeros(n, 0.0);
     for (int p = 0; p < 2; ++p) {
         up_real(s->d[p], d0, 0.0);
@@ -1083,7 +1130,7 @@ void hp2d_sim_destroy(hp2d_sim *s) {
     if (!s) return;
     if (s->stream) hipStreamSynchronize(s->stream);
     if (s->graph) hipGraphExecDestroy(s->graph);
-    void *ps[] = {s->z, s->n2s, s->pattern, s->dmax, s->d[0], s->d[1], s->qx[0], s->qx[1],
+    void *ps[] = {s->z, s->zy, s->n2s, s->pattern, s->dmax, s->d[0], s->d[1], s->qx[0], s->qx[1],
                   s->qy[0], s->qy[1], s->I[0], s->I[1], s->Tc[0], s->Tc[1], s->soil, s->ga, s->ga_f32,
                   s->out_idx, s->out_n, s->out_q, s->out_d, s->partials, s->st, s->rain_t,
                   s->rain_v, s->hq, s->ht, s->hd, s->dt_hist, s->vol_err, s->tune_sink,

@@ -65,6 +65,18 @@ __device__ __forceinline__ void face_geom(float zA, float zB, float hA, float hB
     Hf = fmaxf(fmaxf(hA + (zA - zm), hB + (zB - zm)), 0.0f);
 }
 
+// FP32 from the face's elevation difference dz = zB - zA, taken once in FP64
+// when the run is uploaded (the elevations themselves never enter FP32, so a
+// 200 m relief does not cost the face its millimetres): S = (dz + hB - hA)/R,
+// Hf = max(hA + (zA - zm), hB + (zB - zm), 0) with zm = max(zA, zB).  A NaN
+// dz (a cell outside the domain) gives a NaN slope, whose flux the caller
+// zeroes, and the depth of the valid side, as the elevation form does.
+__device__ __forceinline__ void face_geom_dz(float dz, float hA, float hB,
+                                             const LiArgs<float> &a, float &S, float &Hf) {
+    S = (dz + (hB - hA)) * a.inv_R;
+    Hf = fmaxf(fmaxf(hA - fmaxf(dz, 0.0f), hB + fminf(dz, 0.0f)), 0.0f);
+}
+
 // stored mm/h -> m2/s (:198)
 __device__ __forceinline__ double mmh_to_q(double v, const LiArgs<double> &a) {
     return v / 1000.0 / 3600.0 * a.R2 / a.R;
@@ -89,13 +101,15 @@ __device__ __forceinline__ double q_to_mmh(double Q, const LiArgs<double> &a) {
 }
 __device__ __forceinline__ float q_to_mmh(float Q, const LiArgs<float> &a) { return Q * a.k_q_mmh; }
 
+// FP64: zA, zB are the elevations; FP32: zA is unused and zB is dz = zB - zA
 template <typename T, bool CRIT>
 __device__ __forceinline__ T face_flux(T qp_mmh, T zA, T zB, T hA, T hB, T n2, T dt,
                                        const LiArgs<T> &a, T &Hf_out) {
     // A is the cell that stores the face, B the other side; S = (yB - yA)/R
     // reproduces Sx (:123) for x faces and Sy (:127) for y faces.
     T S, Hf;
-    face_geom(zA, zB, hA, hB, a, S, Hf);
+    if constexpr (sizeof(T) == 8) face_geom(zA, zB, hA, hB, a, S, Hf);
+    else face_geom_dz(zB, hA, hB, a, S, Hf);
     T q = bates(mmh_to_q(qp_mmh, a), Hf, S, n2, dt);
     if (!isfinite(q)) q = T(0);
     if (CRIT) {
@@ -121,8 +135,10 @@ __device__ __forceinline__ int find_outlet(const long long *idx, int n, long lon
 }
 
 // Per-column loads of one lane, fetched one column ahead of the compute.
+// z: the elevation (FP64), or dz of the cell's x face (FP32); zy: dz of its
+// y face (FP32 only)
 template <typename T> struct ColLoad {
-    T z, d, n2, qx, qy, pat, I, Tc;
+    T z, zy, d, n2, qx, qy, pat, I, Tc;
     int soil;
 };
 
@@ -132,11 +148,16 @@ __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d
                                                   const T *I_src, const T *Tc_src, bool in,
                                                   unsigned id) {
     ColLoad<T> v;
-    v.z = v.d = v.n2 = v.qx = v.qy = v.I = v.Tc = T(0);
+    v.z = v.zy = v.d = v.n2 = v.qx = v.qy = v.I = v.Tc = T(0);
     v.pat = T(1);
     v.soil = 0;
     if (in) {
-        v.z = a.z[id];
+        if constexpr (sizeof(T) == 8) {
+            v.z = a.z[id];
+        } else {
+            v.z = a.dzx[id];
+            v.zy = a.dzy[id];
+        }
         v.d = d_src[id];
         v.n2 = a.n2s[id];
         v.qx = qx_src[id];
@@ -262,11 +283,12 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
                                        row_in && c + 1 >= 0 && c + 1 < nx, id + (unsigned)ny);
         const T zc = cur.z, n2c = cur.n2, qxc = cur.qx, qyc = cur.qy;
         T dc = cur.d;
+        // cells outside the domain: NaN elevation (FP64) or NaN n^2 (FP32)
+        const bool nan_cell = f64 ? isnan(zc) : isnan(n2c);
 
         // ---- rainfall + Green-Ampt prologue: Hydrological_Model.m:15-55 ----
         if (MODE == kModeSim && cell_in) {
             const T dprev = dc;
-            const bool nan_cell = isnan(zc);
             T eff = a.rain_pattern ? cur.pat * rain : rain;
             if (nan_cell) eff = T(NAN);
             T dd = dc + eff;
@@ -330,15 +352,17 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
         const T hc = dmax(mm2m(dc), T(0));  // depth_cell (:31)
 
         // ---- raw y-face flux at (r,c): face between rows r-1 and r ----
-        const T h_up = lane_up(hc), z_up = lane_up(zc);
+        const T h_up = lane_up(hc);
+        const T z_up = f64 ? lane_up(zc) : cur.zy;  // FP32: dz = z(r-1) - z(r)
         T rawQyc = 0, Hfyc = 0;
         if (cell_in && r >= 1)
             rawQyc = face_flux<T, CRIT>(qyc, zc, z_up, hc, h_up, fabs(n2c), dt_s, a, Hfyc);
 
         // ---- raw x-face flux at (r,c-1): face between columns c-1 and c ----
         T rawQx1 = 0, Hfx1 = 0;
+        // (FP32: z1 is dz = z(c) - z(c-1), stored with column c-1)
         if (row_in && c - 1 >= 0 && c - 1 < nx - 1)
-            rawQx1 = face_flux<T, CRIT>(qxp1, z1, zc, h1, hc, fabs(n21), dt_s, a, Hfx1);
+            rawQx1 = face_flux<T, CRIT>(qxp1, z1, f64 ? zc : z1, h1, hc, fabs(n21), dt_s, a, Hfx1);
 
         // ---- donor scale of cell (r,c-1): Local_Inertial_Model_D4.m:814-829 ----
         const T qy1_dn = lane_down(rawQy1);

@@ -20,7 +20,9 @@ template <typename T> struct LiArgs {
     T depth_tol_mm, alfa_min;
     T dt_s, dt_h;         // drop-in mode only
     // static fields
-    const T *z;           // elevation [m] (FP32: shifted to a datum)
+    const T *z;           // elevation [m] (FP64 kernels)
+    const T *dzx, *dzy;   // FP32 kernels: z(i,j+1) - z(i,j) and z(i-1,j) - z(i,j) [m],
+                          // taken in FP64 and rounded once (NaN outside the domain)
     const T *n2s;         // Manning n^2, sign bit set on outlet cells
     // drop-in mode state
     const T *d_src, *qx_src, *qy_src;

@@ -464,6 +464,64 @@ void oracle_ga_cells(int64_t n, int32_t first_step, double dt_h, double I_min_mm
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* update_spatial_BC.m:362-398, 469-518: the inflow hydrographs and the      */
+/* rainfall maps of the next step over (t_prev, t).                          */
+/* ------------------------------------------------------------------------ */
+typedef struct forcing_state {
+    int32_t n_maps, n_gauges, cursor_maps, cursor_inflow;
+    double *dp_map;              /* ny*nx rain depth of the next step [mm] (maps) */
+    double agg[HP2D_MAX_INFLOW_GAUGES]; /* BC_States.delta_inflow_agg [mm] */
+} forcing_state;
+
+/* localAverageSpatialGaugeRainOverStep / localAverageInputRainMapOverStep's
+ * slow path: accum = sum overlap_i * map_i, avg = accum / dt_total, then
+ * delta_p_agg = avg * dt / 60 (:486, :514) */
+static void forcing_rain_maps(const hp2d_run_inputs *in, int64_t n, const uint8_t *is_nan,
+                              double t0, double t1, forcing_state *fs) {
+    const double dt_step = fmax(t1 - t0, 0.0);
+    const int32_t K = fs->n_maps;
+    const double *times = in->rain_map_time_min;
+    for (int64_t c = 0; c < n; ++c) fs->dp_map[c] = is_nan[c] ? NAN : 0.0;
+    if (K <= 0 || !(t1 > t0)) return;
+    const double dt_total = t1 - t0;
+    int32_t cur = fs->cursor_maps;
+    if (cur < 1) cur = 1;
+    if (cur > K) cur = K;
+    while (cur < K && t0 > times[cur - 1]) cur++;
+    const double dt_force = (K >= 2 && times[1] - times[0] > 0) ? times[1] - times[0] : 0.0;
+    for (int64_t c = 0; c < n; ++c) fs->dp_map[c] = 0.0;
+    for (int32_t i = cur; i <= K; ++i) {
+        const double ts = (i == 1) ? times[0] - dt_force : times[i - 2];
+        const double te = times[i - 1];
+        const double overlap = fmax(0.0, fmin(t1, te) - fmax(t0, ts));
+        if (overlap > 0) {
+            const double *m = in->rain_maps_mmh + (int64_t)(i - 1) * n;
+            for (int64_t c = 0; c < n; ++c) fs->dp_map[c] = fs->dp_map[c] + m[c] * overlap;
+        }
+        if (te >= t1) {
+            cur = i;
+            break;
+        }
+    }
+    fs->cursor_maps = cur;
+    for (int64_t c = 0; c < n; ++c)
+        fs->dp_map[c] = is_nan[c] ? NAN : fs->dp_map[c] / dt_total * (dt_step / 60.0);
+}
+
+/* :362-378: one shared cursor over the gauges' common time stamps */
+static void forcing_inflow(const hp2d_run_inputs *in, double t0, double t1, forcing_state *fs) {
+    for (int32_t g = 0; g < fs->n_gauges; ++g) fs->agg[g] = 0.0;
+    const double dt_step = fmax(t1 - t0, 0.0);
+    if (!(dt_step > 0)) return;
+    for (int32_t g = 0; g < fs->n_gauges; ++g) {
+        const double avg = oracle_rain_avg(in->inflow_time_min,
+                                           in->inflow_rate_mmh + (int64_t)g * in->n_inflow_times,
+                                           in->n_inflow_times, t0, t1, &fs->cursor_inflow);
+        fs->agg[g] = avg * dt_step / 60.0;
+    }
+}
+
 int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                    oracle_run_out *out) {
     const int32_t ny = cfg->ny, nx = cfg->nx;
@@ -599,6 +657,15 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     double t_previous = 0.0;
     int32_t cursor = 1;
     double delta_p_agg = 0.0;
+    forcing_state fs;
+    memset(&fs, 0, sizeof(fs));
+    fs.n_maps = cfg->n_rain_maps > 0 ? cfg->n_rain_maps : 0;
+    fs.n_gauges = cfg->flag_inflow == 1 ? cfg->n_inflow_gauges : 0;
+    fs.cursor_maps = fs.cursor_inflow = 1;
+    if (fs.n_gauges > HP2D_MAX_INFLOW_GAUGES || (fs.n_gauges > 0 && !in->inflow_gauge) ||
+        (fs.n_maps > 0 && (!in->rain_maps_mmh || !in->rain_map_time_min)))
+        return HP2D_E_ARG;
+    if (fs.n_maps > 0) fs.dp_map = zalloc(n);
     {
         double dt_step = fmax(t - t_previous, 0.0);
         double avg = 0.0;
@@ -608,7 +675,10 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             if (isnan(avg)) avg = 0.0;
             delta_p_agg = avg * dt_step / 60.0;
         }
+        if (fs.n_maps > 0) forcing_rain_maps(in, n, is_nan, t_previous, t, &fs);
+        if (fs.n_gauges > 0) forcing_inflow(in, t_previous, t, &fs);
     }
+    double cum_inflow = 0.0;
     t_previous = 0.0;
     max_dt = fmin(rec * 60.0, max_dt);
     min_dt = fmin(rec * 60.0, min_dt);
@@ -672,7 +742,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
         double rain_vol_step = 0.0, inf_vol_step = 0.0;
         const double dt_used_min = time_step;
         for (int64_t c = 0; c < n; ++c) {
-            double eff = pat ? pat[c] * delta_p_agg : delta_p_agg;
+            double eff = fs.n_maps > 0 ? fs.dp_map[c] : (pat ? pat[c] * delta_p_agg : delta_p_agg);
             if (is_nan[c]) eff = NAN;
             else rain_vol_step += eff / 1000.0 * A;
             d_t[c] = d_t[c] + eff;
@@ -697,7 +767,18 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             for (int64_t c = 0; c < n; ++c)
                 if (!isnan(ga_f[c])) inf_vol_step += ga_f[c] * dt_h / 1000.0 * A;
         }
-        /* Hydrological_Model.m:30 (no inflow), :45-55 */
+        /* Hydrological_Model.m:30: d_t += BC_States.inflow (NaN outside the domain) */
+        double inflow_vol_step = 0.0;
+        if (fs.n_gauges > 0) {
+            for (int64_t c = 0; c < n; ++c) {
+                const int32_t g = in->inflow_gauge[c];
+                double q = (g >= 1 && g <= fs.n_gauges) ? fs.agg[g - 1] : 0.0;
+                if (is_nan[c]) q = NAN;
+                else inflow_vol_step += q / 1000.0 * A;
+                d_t[c] = d_t[c] + q;
+            }
+        }
+        /* :45-55 */
         double dmin = NAN;
         for (int64_t c = 0; c < n; ++c) dmin = fmin(dmin, d_t[c]);
         if (dmin < -1e-8) {
@@ -753,6 +834,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                         out->n_dt_ties++;
                     } else {
                         out->n_dt_mismatch++;
+                        if (out->dt_force) tcr = out->dt_follow[n_dt];
                     }
                 }
                 time_step = tcr / 60.0;
@@ -771,6 +853,8 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                     if (isnan(avg)) avg = 0.0;
                     delta_p_agg = avg * dt_step / 60.0;
                 }
+                if (fs.n_maps > 0) forcing_rain_maps(in, n, is_nan, t_previous, t, &fs);
+                if (fs.n_gauges > 0) forcing_inflow(in, t_previous, t, &fs);
             }
             /* EMC (:724-731), with the time_step refreshing_timestep just set */
             if (do_wq) {
@@ -825,6 +909,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             runoff_mm = qsum * time_step / 60.0 / drainage_area + runoff_mm;
             /* FP64 running ledgers (every step) */
             cum_rain += rain_vol_step;
+            cum_inflow += inflow_vol_step;
             cum_inf += inf_vol_step;
             cum_out += qsum / 1000.0 / 3600.0 * (dt_used_min * 60.0);
             gw_elapsed_min += dt_used_min;
@@ -849,10 +934,16 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                 double *tmp = zalloc(n);
                 for (int64_t c = 0; c < n; ++c) tmp[c] = qof[c] * area_nan[c];
                 outflow_volume = nansum2(ny, nx, tmp) / 1000.0 / 3600.0 * time_step * 60.0 + outflow_volume;
-                for (int64_t c = 0; c < n; ++c) tmp[c] = 0.0 / 1000.0 * area_nan[c];
+                for (int64_t c = 0; c < n; ++c) {  /* nansum(BC_States.inflow/1000 .* C_a) */
+                    const int32_t g = fs.n_gauges > 0 ? in->inflow_gauge[c] : 0;
+                    const double q = (g >= 1 && g <= fs.n_gauges) ? fs.agg[g - 1] : 0.0;
+                    tmp[c] = q / 1000.0 * area_nan[c];
+                }
                 double inflow_part = nansum2(ny, nx, tmp);
                 for (int64_t c = 0; c < n; ++c)
-                    tmp[c] = (pat ? pat[c] * delta_p_agg : delta_p_agg) / 1000.0 * area_nan[c];
+                    tmp[c] = (fs.n_maps > 0 ? fs.dp_map[c]
+                                            : (pat ? pat[c] * delta_p_agg : delta_p_agg)) /
+                             1000.0 * area_nan[c];
                 double Pv = nansum2(ny, nx, tmp);
                 double inflow_vol = inflow_part + Pv + 0.0;
                 inflow_volume = inflow_vol + inflow_volume;
@@ -930,6 +1021,8 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                     if (isnan(avg)) avg = 0.0;
                     delta_p_agg = avg * dt_step / 60.0;
                 }
+                if (fs.n_maps > 0) forcing_rain_maps(in, n, is_nan, t_previous, t, &fs);
+                if (fs.n_gauges > 0) forcing_inflow(in, t_previous, t, &fs);
             }
         }
     }
@@ -958,6 +1051,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     rp->inflow_volume_m3 = inflow_volume;
     rp->outlet_runoff_volume_mm = runoff_mm;
     rp->cum_rain_m3 = cum_rain;
+    rp->cum_inflow_m3 = cum_inflow;
     rp->cum_infiltration_m3 = cum_inf;
     rp->cum_outflow_m3 = cum_out;
     {
@@ -998,5 +1092,6 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     free(gw_h); free(gw_sd); free(gw_ts); free(gw_tr); free(gw_nv); free(gw_al); free(gw_ks);
     free(gw_kg); free(gw_Sy); free(gw_q); free(gw_pn); free(gw_pr); free(gw_pc); free(gw_cum);
     free(gw_dom);
+    free(fs.dp_map);
     return 0;
 }

@@ -97,6 +97,10 @@ typedef struct oracle_run_out {
      * the new dt can be read; rollback_max_dt_s is the max_dt it leaves. */
     int32_t stop_after_rollback;
     double rollback_dt_s, rollback_max_dt_s;
+    /* With dt_force set, the oracle takes every refreshed dt from dt_follow
+     * (n_dt_mismatch still counts the steps where its own differed): the FP32
+     * device path is then compared with it on the same time levels. */
+    int32_t dt_force;
 } oracle_run_out;
 
 /* Green-Ampt derived parameters of one soil class (HydroPol2D_preprocessing.m:

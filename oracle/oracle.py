@@ -62,6 +62,7 @@ class RunOut(C.Structure):
         ("dt_follow", _dp), ("n_follow", C.c_int64), ("n_dt_ties", C.c_int64),
         ("n_dt_mismatch", C.c_int64), ("stop_after_rollback", C.c_int32),
         ("rollback_dt_s", C.c_double), ("rollback_max_dt_s", C.c_double),
+        ("dt_force", C.c_int32),
     ]
 
 
@@ -261,12 +262,14 @@ def ga_step(params, first_step, dt_h, d, I_p, T, I_min_mm=1e-3):
 
 
 def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: int = 200000,
-        n_report_max: int = 64, dt_follow=None, stop_after_rollback: bool = False):
+        n_report_max: int = 64, dt_follow=None, stop_after_rollback: bool = False,
+        dt_force: bool = False):
     """Full Main_While run on the oracle.  Returns a dict of final state + ledgers.
 
     dt_follow: the accepted-step dt sequence [s] of another implementation; the
     oracle adopts it only at floating-point ties of the dt snap (see
-    oracle_run_out) and reports n_dt_ties / n_dt_mismatch."""
+    oracle_run_out) and reports n_dt_ties / n_dt_mismatch; with dt_force it
+    adopts every one of them (an FP32 run compared on its own time levels)."""
     ny, nx = spec.shape
     cfg, ins, keep = spec.to_c()
     cfg.precision = abi.HP2D_FP64
@@ -304,6 +307,7 @@ def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: 
         out.dt_follow = _p(follow)
         out.n_follow = follow.size
     out.stop_after_rollback = int(stop_after_rollback)
+    out.dt_force = int(dt_force)
     rc = lib().oracle_run(C.byref(cfg), C.byref(ins), C.byref(out))
     if rc != 0:
         raise RuntimeError(f"oracle_run failed with status {rc}")

@@ -133,3 +133,15 @@ def test_dt_follow_adopts_only_ties():
     r = oracle.run(spec, dt_follow=bad)
     assert r["n_dt_mismatch"] == 1 and r["n_dt_ties"] == 0
     np.testing.assert_array_equal(r["dt_hist"], own["dt_hist"])
+
+
+def test_dt_force_adopts_every_dt():
+    # dt_force: the oracle runs on a given dt sequence (an FP32 run's) and
+    # counts where its own refreshed dt differed
+    spec = loop_cases.plane(C3=3e5, min_Bt=5.0)
+    own = oracle.run(spec, max_steps=40)
+    given = own["dt_hist"].copy()
+    given[5:] = np.maximum(given[5:] - 2 * spec.time_step_increments_s, spec.min_time_step_s)
+    r = oracle.run(spec, max_steps=40, dt_follow=given, dt_force=True)
+    np.testing.assert_array_equal(r["dt_hist"], given)
+    assert r["n_dt_mismatch"] >= 1
