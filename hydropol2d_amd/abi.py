@@ -58,6 +58,9 @@ class RunConfig(C.Structure):
         ("gw_courant", C.c_double),
         ("n_soil_classes", C.c_int32),
         ("n_lulc_classes", C.c_int32),
+        ("flag_inflow", C.c_int32),
+        ("n_inflow_gauges", C.c_int32),
+        ("n_rain_maps", C.c_int32),
     ]
 
 
@@ -99,6 +102,12 @@ class RunInputs(C.Structure):
         ("gw_n_vg", _dp),
         ("gw_alpha_vg", _dp),
         ("gw_l_vg", C.c_double),
+        ("inflow_time_min", _dp),
+        ("inflow_rate_mmh", _dp),
+        ("n_inflow_times", C.c_int32),
+        ("inflow_gauge", _u8p),
+        ("rain_map_time_min", _dp),
+        ("rain_maps_mmh", _dp),
     ]
 
 
@@ -133,6 +142,7 @@ class RunReport(C.Structure):
         ("gw_total_exfiltration_m3", C.c_double),
         ("gw_error_m3", C.c_double),
         ("gw_updates", C.c_int32),
+        ("cum_inflow_m3", C.c_double),
     ]
 
     def as_dict(self):

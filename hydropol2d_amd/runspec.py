@@ -85,6 +85,31 @@ class Groundwater:
 
 
 @dataclass
+class Inflow:
+    """Inflow hydrographs (update_spatial_BC.m:362-398): end-stamped times
+    [min] shared by the gauges, rates [mm/h over a gauge's cells] as an
+    n_times x n_gauges array, and the gauge cells, either as an index map
+    (ny x nx, 1..n_gauges, 0 = none) or as the reference's ny x nx x n_gauges
+    stack of 0/1 masks (Wshed_Properties.inflow_cells; gauges may not share a
+    cell)."""
+
+    time_min: np.ndarray
+    rate_mmh: np.ndarray
+    cells: np.ndarray
+
+    def gauge_map(self):
+        c = np.asarray(self.cells)
+        if c.ndim == 3:
+            if np.any(np.sum(c != 0, axis=2) > 1):
+                raise ValueError("Inflow: a cell belongs to more than one gauge")
+            idx = np.zeros(c.shape[:2], dtype=np.uint8)
+            for g in range(c.shape[2]):
+                idx[c[:, :, g] != 0] = g + 1
+            return np.asfortranarray(idx)
+        return np.asfortranarray(c.astype(np.uint8))
+
+
+@dataclass
 class RunSpec:
     elevation: np.ndarray
     roughness: np.ndarray
@@ -116,6 +141,11 @@ class RunSpec:
     infiltration: Optional[GreenAmpt] = None
     water_quality: Optional[WaterQuality] = None
     groundwater: Optional[Groundwater] = None
+    inflow: Optional[Inflow] = None
+    # rainfall as rate maps [mm/h] (ny x nx x K) ending at rain_map_time_min [min];
+    # replaces the lumped series and rain_pattern when set
+    rain_maps_mmh: Optional[np.ndarray] = None
+    rain_map_time_min: Optional[np.ndarray] = None
 
     @property
     def shape(self):
@@ -204,6 +234,24 @@ class RunSpec:
             ins.wq_Bmin = wq.Bmin
             ins.wq_Bmax = wq.Bmax
             ins.wq_min_Bt = wq.min_Bt
+        fl = self.inflow
+        if fl is not None:
+            rate = np.asarray(fl.rate_mmh, dtype=np.float64)
+            if rate.ndim == 1:
+                rate = rate[:, None]
+            cfg.flag_inflow = 1
+            cfg.n_inflow_gauges = int(rate.shape[1])
+            ins.inflow_time_min = _ptr(f(np.atleast_1d(fl.time_min)), C.c_double)
+            ins.inflow_rate_mmh = _ptr(f(rate), C.c_double)
+            ins.n_inflow_times = int(rate.shape[0])
+            ins.inflow_gauge = _ptr(u(fl.gauge_map()), C.c_uint8)
+        if self.rain_maps_mmh is not None:
+            maps = np.asarray(self.rain_maps_mmh, dtype=np.float64)
+            if maps.ndim == 2:
+                maps = maps[:, :, None]
+            cfg.n_rain_maps = int(maps.shape[2])
+            ins.rain_maps_mmh = _ptr(f(maps), C.c_double)
+            ins.rain_map_time_min = _ptr(f(np.atleast_1d(self.rain_map_time_min)), C.c_double)
         gw = self.groundwater
         if gw is not None:
             ny, nx = np.shape(self.elevation)

@@ -305,7 +305,14 @@ typedef struct hp2d_run_config {
     double gw_courant;                 /* flags.groundwater_courant */
     int32_t n_soil_classes;            /* GA class tables below */
     int32_t n_lulc_classes;            /* WQ class tables below */
+    int32_t flag_inflow;               /* 1 = inflow hydrographs on gauge cells
+                                          (update_spatial_BC.m:362-398, Hydrological_Model.m:30) */
+    int32_t n_inflow_gauges;           /* 1..HP2D_MAX_INFLOW_GAUGES */
+    int32_t n_rain_maps;               /* > 0: rainfall as a series of rate maps instead of the
+                                          lumped series (update_spatial_BC.m:469-518) */
 } hp2d_run_config;
+
+#define HP2D_MAX_INFLOW_GAUGES 16
 
 /* Host inputs of a run; all grids ny*nx column-major.  Pointers may be NULL
  * where marked optional. */
@@ -341,6 +348,18 @@ typedef struct hp2d_run_inputs {
     double gw_ksat_mmh, gw_Sy;         /* aquifer conductivity [mm/h], specific yield */
     const double *gw_theta_sat, *gw_theta_r, *gw_n_vg, *gw_alpha_vg; /* [n_soil_classes] */
     double gw_l_vg;                    /* Mualem pore-connectivity (0.5) */
+    /* Inflow hydrographs (flag_inflow): end-stamped series of rates [mm/h over
+     * the gauge's cells] shared time stamps, one column per gauge */
+    const double *inflow_time_min;     /* [n_inflow_times] end stamps [min] */
+    const double *inflow_rate_mmh;     /* [n_inflow_times x n_inflow_gauges], times fastest */
+    int32_t n_inflow_times;
+    const uint8_t *inflow_gauge;       /* ny*nx: the cell's gauge 1..n_inflow_gauges, 0 = none
+                                          (Wshed_Properties.inflow_cells as one index map) */
+    /* Rainfall maps (n_rain_maps > 0): piecewise-constant rate maps [mm/h],
+     * end-stamped, from interpolated gauges (hp2d_idw_interpolate) or input
+     * rasters; they replace rain_time_min / rain_intensity_mmh / rain_pattern */
+    const double *rain_map_time_min;   /* [n_rain_maps] end stamps [min] */
+    const double *rain_maps_mmh;       /* [ny*nx x n_rain_maps], NaN outside the domain */
 } hp2d_run_inputs;
 
 /* Results of hp2d_sim_run, accumulated on the device in FP64. */
@@ -371,6 +390,7 @@ typedef struct hp2d_run_report {
     double gw_total_recharge_m3, gw_total_exfiltration_m3;
     double gw_error_m3;                /* errors(5) of the last update */
     int32_t gw_updates;
+    double cum_inflow_m3;              /* every-step FP64 ledger of the inflow hydrographs */
 } hp2d_run_report;
 
 typedef struct hp2d_sim hp2d_sim;
