@@ -72,6 +72,8 @@ SIGNATURES = {
     "hp2d_nn_conv3x3_fwd": (C.c_int32, [C.c_int32] * 5 + [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp,
                                          _vp]),
     "hp2d_nn_conv3x3_dgrad": (C.c_int32, [C.c_int32] * 5 + [_vp] * 5),
+    "hp2d_idw_interpolate": (C.c_int32, [C.c_int32, _dp, _dp, _dp, C.c_int32, _dp, C.c_int32,
+                                         _dp, C.c_double, C.c_double, _dp]),
     "hp2d_nn_conv3x3_dgrad_noalloc": (C.c_int32, [C.c_int32] * 5 + [_vp] * 4),
     "hp2d_nn_conv3x3_wgrad": (C.c_int32, [C.c_int32] * 5 + [_vp] * 5),
     "hp2d_nn_relu_mask": (C.c_int32, [_vp, _vp, C.c_int64, _vp]),

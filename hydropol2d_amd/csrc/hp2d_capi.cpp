@@ -450,6 +450,13 @@ struct hp2d_sim {
     SimState *st = nullptr;
     double *rain_t = nullptr, *rain_v = nullptr;
     int n_rain = 0;
+    // rainfall maps and inflow hydrographs (update_spatial_BC.m:362-398, 469-518)
+    void *maps = nullptr;
+    double *map_t = nullptr, *map_sum = nullptr;
+    int n_maps = 0;
+    double *inflow_t = nullptr, *inflow_v = nullptr, *gauge_cells = nullptr;
+    uint8_t *gauge = nullptr;
+    int n_inflow_times = 0, n_gauges = 0;
     double *hq = nullptr, *ht = nullptr, *hd = nullptr;
     int n_hydro_max = 0;
     double *dt_hist = nullptr;
@@ -524,6 +531,8 @@ template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
     a.sqrtS_type1 = T(std::sqrt(std::fabs(c.slope_outlet)));
     a.depth_tol_mm = T(c.depth_tolerance_mm);
     a.alfa_min = T(c.alfa_min);
+    a.rain_maps = (const T *)s->maps;
+    a.inflow_gauge = s->gauge;
     a.z = (const T *)s->z;
     a.dzx = (const T *)s->z;
     a.dzy = (const T *)s->zy;
@@ -627,6 +636,14 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
     f.rain_t = s->rain_t;
     f.rain_v = s->rain_v;
     f.n_rain = s->n_rain;
+    f.map_t = s->map_t;
+    f.map_sum = s->map_sum;
+    f.n_maps = s->n_maps;
+    f.inflow_t = s->inflow_t;
+    f.inflow_v = s->inflow_v;
+    f.gauge_cells = s->gauge_cells;
+    f.n_inflow_times = s->n_inflow_times;
+    f.n_gauges = s->n_gauges;
     f.out_q = (const T *)s->out_q;
     f.out_d = (const T *)s->out_d;
     f.n_outlet = s->n_outlet;
@@ -735,6 +752,42 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     for (int i = 0; i < in->n_rain; ++i)
         if (!(in->rain_intensity_mmh[i] >= 0))
             return fail(HP2D_E_ARG, "hp2d_sim_create: negative rainfall intensity");
+    // rainfall maps: their sums over the valid cells (the step's rain volume and
+    // the report's P), strictly increasing stamps spaced at least max_dt / 3 so
+    // that no step spans more than kMaxMapTerms of them
+    std::vector<double> map_sum;
+    if (cfg->n_rain_maps > 0) {
+        if (!in->rain_maps_mmh || !in->rain_map_time_min)
+            return fail(HP2D_E_ARG, "hp2d_sim_create: n_rain_maps > 0 without the maps");
+        const int K = cfg->n_rain_maps;
+        for (int k = 1; k < K; ++k)
+            if (!((in->rain_map_time_min[k] - in->rain_map_time_min[k - 1]) * 60.0 >=
+                  cfg->max_time_step_s / 3.0))
+                return fail(HP2D_E_ARG, "hp2d_sim_create: rainfall map stamps must increase by at "
+                                        "least max_time_step / 3");
+        map_sum.assign(K, 0.0);
+        for (int k = 0; k < K; ++k)
+            for (size_t c = 0; c < n; ++c) {
+                if (std::isnan(z[c])) continue;
+                const double v = in->rain_maps_mmh[(size_t)k * n + c];
+                if (!(v >= 0)) return fail(HP2D_E_ARG, "hp2d_sim_create: negative or NaN rainfall map value");
+                map_sum[k] += v;
+            }
+    }
+    // inflow hydrographs: the valid cells of each gauge
+    std::vector<double> gauge_cells;
+    if (cfg->flag_inflow == 1) {
+        const int G = cfg->n_inflow_gauges;
+        if (G < 1 || G > HP2D_MAX_INFLOW_GAUGES || !in->inflow_gauge || !in->inflow_time_min ||
+            !in->inflow_rate_mmh || in->n_inflow_times < 1)
+            return fail(HP2D_E_ARG, "hp2d_sim_create: inflow hydrographs incomplete (1 to 16 gauges)");
+        gauge_cells.assign(G, 0.0);
+        for (size_t c = 0; c < n; ++c) {
+            const int g = in->inflow_gauge[c];
+            if (g > G) return fail(HP2D_E_ARG, "hp2d_sim_create: inflow gauge index beyond n_inflow_gauges");
+            if (g > 0 && !std::isnan(z[c])) gauge_cells[g - 1] += 1.0;
+        }
+    }
     if (in->rain_pattern)
         for (size_t c = 0; c < n; ++c) {
             if (std::isnan(z[c])) continue;
@@ -867,6 +920,20 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     };
     A_(&s->z, nb);
     if (s->elem == 4) A_(&s->zy, nb);
+    if (cfg->n_rain_maps > 0) {
+        s->n_maps = cfg->n_rain_maps;
+        A_(&s->maps, nb * (size_t)s->n_maps);
+        A_((void **)&s->map_t, (size_t)s->n_maps * 8);
+        A_((void **)&s->map_sum, (size_t)s->n_maps * 8);
+    }
+    if (cfg->flag_inflow == 1) {
+        s->n_gauges = cfg->n_inflow_gauges;
+        s->n_inflow_times = in->n_inflow_times;
+        A_((void **)&s->inflow_t, (size_t)s->n_inflow_times * 8);
+        A_((void **)&s->inflow_v, (size_t)s->n_inflow_times * s->n_gauges * 8);
+        A_((void **)&s->gauge_cells, (size_t)s->n_gauges * 8);
+        A_((void **)&s->gauge, n);
+    }
     A_(&s->n2s, nb);
     for (int p = 0; p < 2; ++p) {
         A_(&s->d[p], nb);
@@ -996,6 +1063,22 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         up(s->rain_t, in->rain_time_min, in->n_rain * sizeof(double));
         up(s->rain_v, in->rain_intensity_mmh, in->n_rain * sizeof(double));
     }
+    if (s->n_maps > 0) {
+        for (int k = 0; k < s->n_maps; ++k) {
+            std::vector<double> m(in->rain_maps_mmh + (size_t)k * n, in->rain_maps_mmh + (size_t)(k + 1) * n);
+            for (size_t c = 0; c < n; ++c)
+                if (std::isnan(z[c])) m[c] = NAN;
+            up_real((char *)s->maps + (size_t)k * nb, m, 0.0);
+        }
+        up(s->map_t, in->rain_map_time_min, (size_t)s->n_maps * 8);
+        up(s->map_sum, map_sum.data(), (size_t)s->n_maps * 8);
+    }
+    if (s->n_gauges > 0) {
+        up(s->inflow_t, in->inflow_time_min, (size_t)s->n_inflow_times * 8);
+        up(s->inflow_v, in->inflow_rate_mmh, (size_t)s->n_inflow_times * s->n_gauges * 8);
+        up(s->gauge_cells, gauge_cells.data(), (size_t)s->n_gauges * 8);
+        up(s->gauge, in->inflow_gauge, n);
+    }
     if (s->wq) {
         for (int p = 0; p < 2; ++p) {
             up_real(s->wqB[p], B0, 0.0);
@@ -1103,6 +1186,22 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     st.cursor = 1;
     st.delta_p_agg = rain_depth(in->rain_time_min, in->rain_intensity_mmh, in->n_rain, 0.0, st.t,
                                 &st.cursor);
+    st.map_cursor = st.inflow_cursor = 1;
+    if (s->n_maps > 0) {
+        MapTerms mt;
+        if (!map_terms(in->rain_map_time_min, s->n_maps, 0.0, st.t, &st.map_cursor, &mt))
+            return fail(HP2D_E_ARG, "hp2d_sim_create: too many rainfall maps in the first step");
+        st.map_n = mt.n;
+        for (int j = 0; j < kMaxMapTerms; ++j) {
+            st.map_idx[j] = j < mt.n ? mt.idx[j] : 0;
+            st.map_w[j] = j < mt.n ? mt.w[j] : 0.0;
+        }
+        st.map_dtot = mt.dt_total;
+        st.map_k = mt.k;
+    }
+    if (s->n_gauges > 0)
+        inflow_depths(in->inflow_time_min, in->inflow_rate_mmh, in->n_inflow_times, s->n_gauges,
+                      0.0, st.t, &st.inflow_cursor, st.inflow_agg);
     st.t_previous = 0.0;
     st.max_dt = std::min(cfg->record_time_hydrographs_min * 60.0, cfg->max_time_step_s);
     st.min_dt = std::min(cfg->record_time_hydrographs_min * 60.0, min_dt0);
@@ -1136,7 +1235,8 @@ void hp2d_sim_destroy(hp2d_sim *s) {
                   s->rain_v, s->hq, s->ht, s->hd, s->dt_hist, s->vol_err, s->tune_sink,
                   s->tpeak, s->wqB[0], s->wqB[1], s->wqBs, s->wqBx[0], s->wqBx[1], s->wqOm,
                   s->wqTW[0], s->wqTW[1], s->wqPc, s->wqC3, s->wqC4, s->wq_cell, s->wq_part,
-                  s->wq_ctl, s->gw_mem, s->gw_col, s->gw_dom};
+                  s->wq_ctl, s->gw_mem, s->gw_col, s->gw_dom, s->maps, s->map_t, s->map_sum,
+                  s->inflow_t, s->inflow_v, s->gauge_cells, s->gauge};
     for (void *p : ps)
         if (p) hipFree(p);
     if (s->gwh) hp2d_destroy(s->gwh);
@@ -1154,6 +1254,10 @@ const char *hp2d_sim_last_error(const hp2d_sim *s) { return s ? s->err.c_str() :
 static hp2d_status read_state(hp2d_sim *s, SimState &st) {
     STRY(s, hipStreamSynchronize(s->stream));
     STRY(s, hipMemcpy(&st, s->st, sizeof(SimState), hipMemcpyDeviceToHost));
+    if (st.map_err) {
+        set_err(s, "rainfall maps: a step spans more than 4 map intervals");
+        return HP2D_E_ARG;
+    }
     return HP2D_OK;
 }
 
@@ -1436,6 +1540,7 @@ hp2d_status hp2d_sim_report(hp2d_sim *s, hp2d_run_report *rep) {
     rep->inflow_volume_m3 = st.inflow_volume;
     rep->outlet_runoff_volume_mm = st.runoff_mm;
     rep->cum_rain_m3 = st.cum_rain;
+    rep->cum_inflow_m3 = st.cum_inflow;
     rep->cum_infiltration_m3 = st.cum_inf;
     rep->cum_outflow_m3 = st.cum_out;
     {

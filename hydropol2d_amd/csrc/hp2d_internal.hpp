@@ -56,6 +56,14 @@ struct SimState {
     int cursor, catch_index, done, last_record, n_hydro, n_reports, neg_events, paused;
     int hold;    // a wash-off split longer than the captured sub-steps waits for the host
     int wq_err;  // the wash-off refused a split beyond 1e7 sub-steps
+    // rainfall maps of the next step (update_spatial_BC.m:469-518): the maps
+    // overlapping it, their overlaps [min], (t - t_previous) and dt/60
+    int map_n, map_cursor, map_err, inflow_cursor;
+    int map_idx[4];
+    double map_w[4], map_dtot, map_k;
+    // inflow hydrographs of the next step: BC_States.delta_inflow_agg [mm]
+    double inflow_agg[16];
+    double cum_inflow;
 };
 static_assert(sizeof(SimState) % 4 == 0, "SimState is copied as 32-bit words");
 

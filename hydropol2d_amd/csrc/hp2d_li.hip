@@ -19,6 +19,7 @@
 // Precision policy: the double instantiation keeps the reference's operation
 // order (it is the parity mode); the float instantiation evaluates the same
 // expressions with reciprocal multiplies and the hardware log2/exp2/rcp.
+#include "hp2d_forcing.hpp"
 #include "hp2d_internal.hpp"
 #include "hp2d_li.hpp"
 
@@ -139,18 +140,27 @@ __device__ __forceinline__ int find_outlet(const long long *idx, int n, long lon
 // y face (FP32 only)
 template <typename T> struct ColLoad {
     T z, zy, d, n2, qx, qy, pat, I, Tc;
-    int soil;
+    int soil, gauge;
+};
+
+// The rainfall maps' terms of this step (wave-uniform): the offsets of the
+// overlapping maps, their overlaps [min], t - t_previous [min] and dt/60 [h].
+template <typename T> struct MapStep {
+    int n;
+    unsigned off[kMaxMapTerms];
+    T w[kMaxMapTerms];
+    T dtot, k;
 };
 
 template <typename T, int MODE>
 __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d_src,
                                                   const T *qx_src, const T *qy_src,
                                                   const T *I_src, const T *Tc_src, bool in,
-                                                  unsigned id) {
+                                                  unsigned id, const MapStep<T> &ms) {
     ColLoad<T> v;
     v.z = v.zy = v.d = v.n2 = v.qx = v.qy = v.I = v.Tc = T(0);
     v.pat = T(1);
-    v.soil = 0;
+    v.soil = v.gauge = 0;
     if (in) {
         if constexpr (sizeof(T) == 8) {
             v.z = a.z[id];
@@ -163,7 +173,16 @@ __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d
         v.qx = qx_src[id];
         v.qy = qy_src[id];
         if (MODE == kModeSim) {
-            if (a.rain_pattern) v.pat = a.rain_pattern[id];
+            if (a.rain_maps) {
+                // the step's rain depth from the maps (update_spatial_BC.m:1361-1408):
+                // (sum_i overlap_i * map_i) / (t - t_prev) * dt / 60
+                T acc = T(0);
+                for (int j = 0; j < ms.n; ++j) acc = acc + ms.w[j] * a.rain_maps[ms.off[j] + id];
+                v.pat = ms.n > 0 ? acc / ms.dtot * ms.k : T(0);
+            } else if (a.rain_pattern) {
+                v.pat = a.rain_pattern[id];
+            }
+            if (a.inflow_gauge) v.gauge = a.inflow_gauge[id];
             if (a.ga) {
                 v.I = I_src[id];
                 v.Tc = Tc_src[id];
@@ -200,6 +219,9 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
 
     long long kstep = 1;
     T dt_s, dt_h, rain = T(0), t_end_min = T(0);
+    MapStep<T> ms;
+    ms.n = 0;
+    ms.dtot = ms.k = T(0);
     int fail_step = 0;
     bool may_report = false;
     const T *d_src = a.d_src, *qx_src = a.qx_src, *qy_src = a.qy_src;
@@ -215,6 +237,16 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
         rain = T(st->delta_p_agg);
         t_end_min = T(st->t + st->time_step_min);
         fail_step = (st->inject_k == kstep);
+        if (a.rain_maps) {
+            const unsigned cells = (unsigned)ny * (unsigned)nx;
+            ms.n = st->map_n;
+            for (int j = 0; j < kMaxMapTerms; ++j) {
+                ms.off[j] = (unsigned)st->map_idx[j] * cells;
+                ms.w[j] = T(st->map_w[j]);
+            }
+            ms.dtot = T(st->map_dtot);
+            ms.k = T(st->map_k);
+        }
         // a mass-balance report can only follow this step if t + max_dt reaches the
         // next report threshold (the new dt never exceeds max_dt); only then are the
         // storage sums needed (HydroPol2D_Main_While.m:787-803)
@@ -269,7 +301,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     // 32-bit element offsets (grids are limited to 2^32 cells at creation)
     ColLoad<T> nxt = load_column<T, MODE>(a, d_src, qx_src, qy_src, I_src, Tc_src,
                                           active && row_in && c0 - 2 >= 0 && c0 - 2 < nx,
-                                          (unsigned)r + (unsigned)(c0 - 2) * (unsigned)ny);
+                                          (unsigned)r + (unsigned)(c0 - 2) * (unsigned)ny, ms);
     const int c_end = active ? c1 + 1 : c0 - 3;  // inactive waves skip the march
 #pragma clang loop unroll_count(2)
     for (int c = c0 - 2; c <= c_end; ++c) {
@@ -280,7 +312,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
         // issue the next column's loads before computing this one
         if (c + 1 <= c1 + 1)
             nxt = load_column<T, MODE>(a, d_src, qx_src, qy_src, I_src, Tc_src,
-                                       row_in && c + 1 >= 0 && c + 1 < nx, id + (unsigned)ny);
+                                       row_in && c + 1 >= 0 && c + 1 < nx, id + (unsigned)ny, ms);
         const T zc = cur.z, n2c = cur.n2, qxc = cur.qx, qyc = cur.qy;
         T dc = cur.d;
         // cells outside the domain: NaN elevation (FP64) or NaN n^2 (FP32)
@@ -289,7 +321,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
         // ---- rainfall + Green-Ampt prologue: Hydrological_Model.m:15-55 ----
         if (MODE == kModeSim && cell_in) {
             const T dprev = dc;
-            T eff = a.rain_pattern ? cur.pat * rain : rain;
+            T eff = a.rain_maps ? cur.pat : (a.rain_pattern ? cur.pat * rain : rain);
             if (nan_cell) eff = T(NAN);
             T dd = dc + eff;
             const bool own = row_own && c >= c0 && c < c1;
@@ -340,6 +372,8 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
                     }
                 }
             }
+            // Hydrological_Model.m:30: the inflow hydrographs on their gauge cells
+            if (a.inflow_gauge && cur.gauge > 0) dd = dd + T(a.st->inflow_agg[cur.gauge - 1]);
             if (dd < T(-1e-8)) neg = 1;
             if (dd < T(1e-6)) dd = T(0);  // Hydrological_Model.m:49
             if (dd < T(0)) dd = T(0);     // :52

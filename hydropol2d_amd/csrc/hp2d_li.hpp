@@ -34,6 +34,8 @@ template <typename T> struct LiArgs {
     const SimState *st;
     double routing_time;  // decides on the device whether a mass-balance report can follow
     // hydrology (sim mode)
+    const T *rain_maps;           // rainfall maps [mm/h], map k at k * ny * nx (or NULL)
+    const uint8_t *inflow_gauge;  // the cell's inflow gauge 1..n, 0 = none (or NULL)
     const T *rain_pattern;
     const uint8_t *soil_class;
     const GaClass *ga;    // NULL = no infiltration

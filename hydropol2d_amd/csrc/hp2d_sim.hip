@@ -35,6 +35,48 @@ struct SeriesWindow {
     __device__ double value(int k) const { return (k - w0 >= 0 && k - w0 < wn) ? lv[k - w0] : v[k]; }
 };
 
+// sum over the valid cells of the rainfall maps' depth of the step whose
+// terms the state holds [mm x cells]
+template <typename T>
+__device__ inline double map_depth_sum(const FinArgs<T> &a, const SimState *st) {
+    if (st->map_n == 0) return 0.0;
+    double acc = 0.0;
+    for (int j = 0; j < st->map_n; ++j) acc = acc + st->map_w[j] * a.map_sum[st->map_idx[j]];
+    return acc / st->map_dtot * st->map_k;
+}
+
+// sum over the valid cells of the inflow depths the state holds [mm x cells]
+template <typename T>
+__device__ inline double inflow_depth_sum(const FinArgs<T> &a, const SimState *st) {
+    double acc = 0.0;
+    for (int g = 0; g < a.n_gauges; ++g) acc += st->inflow_agg[g] * a.gauge_cells[g];
+    return acc;
+}
+
+// update_spatial_BC's rainfall maps and inflow hydrographs for (t0, t1]
+template <typename T>
+__device__ inline void update_maps_inflow(const FinArgs<T> &a, SimState *st, double t0, double t1) {
+    if (a.n_maps > 0) {
+        MapTerms mt;
+        int cur = st->map_cursor;
+        if (!map_terms(a.map_t, a.n_maps, t0, t1, &cur, &mt)) st->map_err = 1;
+        st->map_cursor = cur;
+        st->map_n = mt.n;
+        for (int j = 0; j < kMaxMapTerms; ++j) {
+            st->map_idx[j] = j < mt.n ? mt.idx[j] : 0;
+            st->map_w[j] = j < mt.n ? mt.w[j] : 0.0;
+        }
+        st->map_dtot = mt.dt_total;
+        st->map_k = mt.k;
+    }
+    if (a.n_gauges > 0) {
+        int cur = st->inflow_cursor;
+        inflow_depths(a.inflow_t, a.inflow_v, a.n_inflow_times, a.n_gauges, t0, t1, &cur,
+                      st->inflow_agg);
+        st->inflow_cursor = cur;
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     // The scalar tail below is one thread's chain of dependent reads and
@@ -152,7 +194,10 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     const bool may_report = (prog_max >= st->next_report) || (prog_max >= 100.0);
     // rainfall volume of this step: the applied depth over the valid cells
     rain = failed ? 0.0
-                  : st->delta_p_agg * (a.has_pattern ? a.sum_pattern : a.n_valid) * (a.A / 1000.0);
+                  : (a.n_maps > 0 ? map_depth_sum(a, st)
+                                  : st->delta_p_agg * (a.has_pattern ? a.sum_pattern : a.n_valid)) *
+                        (a.A / 1000.0);
+    const double inflow_step = failed ? 0.0 : inflow_depth_sum(a, st) * (a.A / 1000.0);
     const double routing_time = a.routing_time;
     if (neg > 0) {  // Hydrological_Model.m:45-47
         st->catch_index += 1;
@@ -199,6 +244,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         int cur = st->cursor;
         st->delta_p_agg = rain_depth(ser, a.n_rain, st->t_previous, t_now, &cur);
         st->cursor = cur;
+        update_maps_inflow(a, st, st->t_previous, t_now);
         // ---------------- EMC (:724-731), with the refreshed time_step ----------------
         if (a.flag_wq) {
             const double qo = qraw / 1000.0 / 3600.0 * 1000.0;
@@ -233,6 +279,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         st->last_record = actual;
         st->runoff_mm = qsum * st->time_step_min / 60.0 / a.drainage_area + st->runoff_mm;
         st->cum_rain += rain;
+        st->cum_inflow += inflow_step;
         st->cum_inf += infil;
         st->cum_out += qsum / 1000.0 / 3600.0 * (dt_used_min * 60.0);
         if (a.flag_gw) st->gw_elapsed_min += dt_used_min;
@@ -255,9 +302,11 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         if (do_report) {
             const double ts = st->time_step_min;
             st->outflow_volume = qsum / 1000.0 / 3600.0 * ts * 60.0 + st->outflow_volume;
-            const double Pv = a.has_pattern ? st->delta_p_agg * a.sum_pattern / 1000.0 * a.A
-                                            : a.n_valid * (st->delta_p_agg / 1000.0 * a.A);
-            const double inflow_vol = 0.0 + Pv + 0.0;
+            const double Pv = a.n_maps > 0 ? map_depth_sum(a, st) / 1000.0 * a.A
+                              : a.has_pattern ? st->delta_p_agg * a.sum_pattern / 1000.0 * a.A
+                                              : a.n_valid * (st->delta_p_agg / 1000.0 * a.A);
+            const double inflow_part = inflow_depth_sum(a, st) / 1000.0 * a.A;
+            const double inflow_vol = inflow_part + Pv + 0.0;
             st->inflow_volume = inflow_vol + st->inflow_volume;
             const double Qin = inflow_vol - Pv;
             const double Qout = qsum / 1000.0 / 3600.0 * ts * 60.0 + 0.0;
@@ -298,6 +347,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         int cur = st->cursor;
         st->delta_p_agg = rain_depth(ser, a.n_rain, st->t_previous, t_now, &cur);
         st->cursor = cur;
+        update_maps_inflow(a, st, st->t_previous, t_now);
     }
     st->done = !(st->t <= (routing_time + st->min_dt / 60.0)) ? 1 : 0;
     if (st->stop_accepted >= 0 && st->accepted >= st->stop_accepted) st->paused = 1;

@@ -17,6 +17,12 @@ template <typename T> struct FinArgs {
     // lumped rainfall series
     const double *rain_t, *rain_v;
     int n_rain;
+    // rainfall maps: end stamps [min] and each map's sum over the valid cells [mm/h]
+    const double *map_t, *map_sum;
+    int n_maps;
+    // inflow hydrographs: shared end stamps, rates [n_times x n_gauges], valid cells per gauge
+    const double *inflow_t, *inflow_v, *gauge_cells;
+    int n_inflow_times, n_gauges;
     // outlets
     const T *out_q, *out_d;
     int n_outlet;

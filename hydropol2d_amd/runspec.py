@@ -100,12 +100,8 @@ class Inflow:
     def gauge_map(self):
         c = np.asarray(self.cells)
         if c.ndim == 3:
-            if np.any(np.sum(c != 0, axis=2) > 1):
-                raise ValueError("Inflow: a cell belongs to more than one gauge")
-            idx = np.zeros(c.shape[:2], dtype=np.uint8)
-            for g in range(c.shape[2]):
-                idx[c[:, :, g] != 0] = g + 1
-            return np.asfortranarray(idx)
+            from .forcing import inflow_cells_to_gauge_map
+            return inflow_cells_to_gauge_map(c)
         return np.asfortranarray(c.astype(np.uint8))
 
 

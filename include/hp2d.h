@@ -362,6 +362,17 @@ typedef struct hp2d_run_inputs {
     const double *rain_maps_mmh;       /* [ny*nx x n_rain_maps], NaN outside the domain */
 } hp2d_run_inputs;
 
+/* Rainfall_Interpolator.m:40-52 / idw_function.m:18-63: inverse-distance
+ * weighting of n_obs gauges (x_obs, y_obs [m], values f_obs) onto the grid
+ * meshgrid(x_grid, y_grid), out ny_grid x nx_grid column-major:
+ * out = sum W f / sum W with W = 1 / D^p, D the Euclidean distance, D = 0
+ * taken as eps and D > rad as Inf (the reference's defaults: p = 2, rad = Inf).
+ * Builds the spatial rainfall maps of update_spatial_BC.m:469-490. */
+hp2d_status hp2d_idw_interpolate(int32_t n_obs, const double *x_obs, const double *y_obs,
+                                 const double *f_obs, int32_t nx_grid, const double *x_grid,
+                                 int32_t ny_grid, const double *y_grid, double p, double rad,
+                                 double *out);
+
 /* Results of hp2d_sim_run, accumulated on the device in FP64. */
 typedef struct hp2d_run_report {
     int64_t steps;                     /* accepted steps (k - 1) */

@@ -1095,3 +1095,29 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     free(fs.dp_map);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Rainfall_Interpolator.m:40-52 with idw_function.m:18-63 (p = 2, rad = Inf, */
+/* L = 2): out(i,j) = sum_g W_g F_g / sum_g W_g, W_g = 1 / D_g^p, D_g the     */
+/* distance from (x_grid(j), y_grid(i)) to gauge g, D = 0 -> eps, D > rad ->  */
+/* Inf.  out is ny_grid x nx_grid column-major.                              */
+/* ------------------------------------------------------------------------ */
+void oracle_idw(int32_t n_obs, const double *x_obs, const double *y_obs, const double *f_obs,
+                int32_t nx_grid, const double *x_grid, int32_t ny_grid, const double *y_grid,
+                double p, double rad, double *out) {
+    for (int32_t j = 0; j < nx_grid; ++j)
+        for (int32_t i = 0; i < ny_grid; ++i) {
+            double num = 0.0, den = 0.0;
+            for (int32_t g = 0; g < n_obs; ++g) {
+                const double dx = fabs(x_grid[j] - x_obs[g]);
+                const double dy = fabs(y_grid[i] - y_obs[g]);
+                double D = pow(pow(dx, 2.0) + pow(dy, 2.0), 1.0 / 2.0);
+                if (D == 0) D = DBL_EPSILON;
+                if (D > rad) D = INFINITY;
+                const double W = 1.0 / pow(D, p);
+                num += W * f_obs[g];
+                den += W;
+            }
+            out[(int64_t)i + (int64_t)j * ny_grid] = num / den;
+        }
+}

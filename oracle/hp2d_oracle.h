@@ -186,6 +186,11 @@ int32_t oracle_groundwater_module(const oracle_gwm_params *p, oracle_gwm_state *
                                   double *max_gw_depth, double *q_exf, double *error,
                                   int32_t *ran, double *dt_gw_out);
 
+/* Rainfall_Interpolator / idw_function (inverse-distance weighting) */
+void oracle_idw(int32_t n_obs, const double *x_obs, const double *y_obs, const double *f_obs,
+                int32_t nx_grid, const double *x_grid, int32_t ny_grid, const double *y_grid,
+                double p, double rad, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,6 +96,9 @@ def _load(path):
         L.oracle_ga_cells.argtypes = [C.c_int64, C.c_int32, C.c_double, C.c_double, _dp, _dp,
                                       _dp, _dp, _dp, _dp]
         L.oracle_ga_cells.restype = None
+        L.oracle_idw.argtypes = [C.c_int32, _dp, _dp, _dp, C.c_int32, _dp, C.c_int32, _dp,
+                                 C.c_double, C.c_double, _dp]
+        L.oracle_idw.restype = None
         if hasattr(L, "oracle_ca_routing"):
             L.oracle_ca_routing.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp, C.c_double,
                                             C.c_double, _dp, C.c_double, _dp, C.c_int32,
@@ -494,3 +497,14 @@ class GroundwaterModule:
         if rc != 0:
             raise ValueError(f"groundwater module: oracle returned {rc}")
         return q, err.value, bool(ran.value), dtg.value
+
+
+def idw(x_obs, y_obs, f_obs, x_grid, y_grid, p=2.0, rad=np.inf):
+    """idw_function on the grid meshgrid(x_grid, y_grid) (ny x nx)."""
+    xo, yo, fo = (np.ascontiguousarray(np.ravel(a), dtype=np.float64) for a in (x_obs, y_obs, f_obs))
+    xg = np.ascontiguousarray(np.ravel(x_grid), dtype=np.float64)
+    yg = np.ascontiguousarray(np.ravel(y_grid), dtype=np.float64)
+    out = np.zeros((yg.size, xg.size), order="F")
+    lib().oracle_idw(xo.size, _p(xo), _p(yo), _p(fo), xg.size, _p(xg), yg.size, _p(yg), p, rad,
+                     _p(out))
+    return out
