@@ -72,6 +72,9 @@ SIGNATURES = {
     "hp2d_nn_conv3x3_fwd": (C.c_int32, [C.c_int32] * 5 + [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp,
                                          _vp]),
     "hp2d_nn_conv3x3_dgrad": (C.c_int32, [C.c_int32] * 5 + [_vp] * 5),
+    "hp2d_sim_get_max_risk": (C.c_int32, [_vp, _dp]),
+    "hp2d_sim_get_map_records": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp,
+                                             C.POINTER(C.c_int32)]),
     "hp2d_idw_interpolate": (C.c_int32, [C.c_int32, _dp, _dp, _dp, C.c_int32, _dp, C.c_int32,
                                          _dp, C.c_double, C.c_double, _dp]),
     "hp2d_nn_conv3x3_dgrad_noalloc": (C.c_int32, [C.c_int32] * 5 + [_vp] * 4),

@@ -61,7 +61,16 @@ class RunConfig(C.Structure):
         ("flag_inflow", C.c_int32),
         ("n_inflow_gauges", C.c_int32),
         ("n_rain_maps", C.c_int32),
+        ("flag_human_instability", C.c_int32),
+        ("hi_weight_kg", C.c_double), ("hi_width1_m", C.c_double), ("hi_width2_m", C.c_double),
+        ("hi_mu", C.c_double), ("hi_Cd", C.c_double), ("hi_ro_water", C.c_double),
+        ("hi_gravity", C.c_double),
+        ("record_time_maps_min", C.c_double),
     ]
+
+
+MAP_FIELDS = ("d", "velocity", "hazard_dv", "risk", "I_t", "GWdepth", "Pol_Conc_Map",
+              "Pol_Mass_Map", "Pol_Load_Map")
 
 
 class RunInputs(C.Structure):
@@ -143,6 +152,7 @@ class RunReport(C.Structure):
         ("gw_error_m3", C.c_double),
         ("gw_updates", C.c_int32),
         ("cum_inflow_m3", C.c_double),
+        ("n_map_records", C.c_int32),
     ]
 
     def as_dict(self):

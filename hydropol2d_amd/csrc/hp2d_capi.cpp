@@ -17,12 +17,17 @@
 #include "hp2d_handle.hpp"
 #include "hp2d_internal.hpp"
 #include "hp2d_li.hpp"
+#include "hp2d_maps.hpp"
 #include "hp2d_sim.hpp"
 #include "hp2d_simwq.hpp"
 
 using namespace hp2d;
 
 namespace {
+
+// device map-stack records between host drains: a step takes one record at
+// most, and a graph holds at most kSnapSlots steps while map stacks are on
+constexpr int kSnapSlots = 16;
 
 thread_local std::string g_last_error;
 
@@ -451,6 +456,13 @@ struct hp2d_sim {
     double *rain_t = nullptr, *rain_v = nullptr;
     int n_rain = 0;
     // rainfall maps and inflow hydrographs (update_spatial_BC.m:362-398, 469-518)
+    // Human_Instability_Module max_risk and the map stacks (save_output_maps.m)
+    void *max_risk = nullptr, *snap_vras = nullptr, *snap_vtot = nullptr;
+    void *stack[HP2D_MAP_FIELDS] = {nullptr};
+    double *snap_t = nullptr;
+    int snap_slots = 0;
+    std::vector<std::vector<double>> records[HP2D_MAP_FIELDS];
+    std::vector<double> record_t;
     void *maps = nullptr;
     double *map_t = nullptr, *map_sum = nullptr;
     int n_maps = 0;
@@ -562,6 +574,17 @@ template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
     a.partials = s->partials;
     a.part_cap = s->part_cap;
     a.track_dmax = s->wq ? 1 : 0;
+    a.max_risk = (T *)s->max_risk;
+    a.hi_weight = T(c.hi_weight_kg);
+    a.hi_w1 = T(c.hi_width1_m);
+    a.hi_w2 = T(c.hi_width2_m);
+    a.hi_mu = T(c.hi_mu);
+    a.hi_Cd = T(c.hi_Cd);
+    a.hi_ro = T(c.hi_ro_water);
+    a.hi_g = T(c.hi_gravity);
+    a.snap_vras = (T *)s->snap_vras;
+    a.snap_vtot = (T *)s->snap_vtot;
+    a.rec_maps = c.record_time_maps_min;
     return a;
 }
 
@@ -644,6 +667,9 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
     f.gauge_cells = s->gauge_cells;
     f.n_inflow_times = s->n_inflow_times;
     f.n_gauges = s->n_gauges;
+    f.rec_maps = s->snap_slots > 0 ? c.record_time_maps_min : 0.0;
+    f.snap_slots = s->snap_slots;
+    f.snap_t = s->snap_t;
     f.out_q = (const T *)s->out_q;
     f.out_d = (const T *)s->out_d;
     f.n_outlet = s->n_outlet;
@@ -660,12 +686,56 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
 
 // the launches that follow the fused step: the wash-off (when enabled) and
 // the dt / bookkeeping tail
+template <typename T> static SnapArgs<T> sim_snap_args(hp2d_sim *s) {
+    SnapArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    const hp2d_run_config &c = s->cfg;
+    a.st = s->st;
+    a.n = (long long)s->n;
+    a.ny = s->ny;
+    a.slots = s->snap_slots;
+    for (int f = 0; f < HP2D_MAP_FIELDS; ++f) a.stack[f] = (T *)s->stack[f];
+    for (int p = 0; p < 2; ++p) {
+        a.d[p] = (const T *)s->d[p];
+        a.I[p] = (const T *)s->I[p];
+        a.wqB[p] = (const T *)s->wqB[p];
+        a.qx[p] = (const T *)s->qx[p];
+        a.qy[p] = (const T *)s->qy[p];
+    }
+    a.vras = (const T *)s->snap_vras;
+    a.vtot = (const T *)s->snap_vtot;
+    a.n2s = (const T *)s->n2s;
+    a.elev = s->elem == 8 ? (const double *)s->z : nullptr;
+    a.hi = c.flag_human_instability == 1;
+    a.hi_weight = c.hi_weight_kg;
+    a.hi_w1 = c.hi_width1_m;
+    a.hi_w2 = c.hi_width2_m;
+    a.hi_mu = c.hi_mu;
+    a.hi_Cd = c.hi_Cd;
+    a.hi_ro = c.hi_ro_water;
+    a.hi_g = c.hi_gravity;
+    if (s->gw) {
+        a.gw_h = s->gwa.h;
+        a.gw_elev = s->gwa.elev;
+        a.gw_sd = s->gwa.soil_depth;
+        a.gw_sd_u = s->gwa.soil_depth_u;
+    }
+    a.wqPc = (const T *)s->wqPc;
+    a.out_idx = s->out_idx;
+    a.out_q = (const T *)s->out_q;
+    a.n_outlet = s->n_outlet;
+    a.R = c.resolution;
+    return a;
+}
+
 template <typename T> static hipError_t sim_after_li(hp2d_sim *s) {
     if (s->wq) {
         hipError_t e = launch_wq_step<T>(sim_wq_args<T>(s), s->stream);
         if (e != hipSuccess) return e;
     }
-    return launch_finalize<T>(sim_fin_args<T>(s), s->stream);
+    hipError_t e = launch_finalize<T>(sim_fin_args<T>(s), s->stream);
+    if (e != hipSuccess || s->snap_slots == 0) return e;
+    return launch_snapshot<T>(sim_snap_args<T>(s), s->stream);
 }
 
 template <typename T> static hipError_t sim_step_launch(hp2d_sim *s, bool with_finalize) {
@@ -925,6 +995,18 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         A_(&s->maps, nb * (size_t)s->n_maps);
         A_((void **)&s->map_t, (size_t)s->n_maps * 8);
         A_((void **)&s->map_sum, (size_t)s->n_maps * 8);
+    }
+    if (cfg->flag_human_instability == 1) A_(&s->max_risk, nb);
+    if (cfg->record_time_maps_min > 0) {
+        // one record at most per step, and at most kSnapSlots steps per graph
+        s->snap_slots = kSnapSlots;
+        A_(&s->snap_vras, nb);
+        A_(&s->snap_vtot, nb);
+        A_((void **)&s->snap_t, kSnapSlots * sizeof(double));
+        const bool on[HP2D_MAP_FIELDS] = {true, true, true, cfg->flag_human_instability == 1,
+                                          cfg->flag_infiltration == 1, s->gw, s->wq, s->wq, s->wq};
+        for (int f = 0; f < HP2D_MAP_FIELDS; ++f)
+            if (on[f]) A_(&s->stack[f], nb * kSnapSlots);
     }
     if (cfg->flag_inflow == 1) {
         s->n_gauges = cfg->n_inflow_gauges;
@@ -1187,6 +1269,7 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     st.delta_p_agg = rain_depth(in->rain_time_min, in->rain_intensity_mmh, in->n_rain, 0.0, st.t,
                                 &st.cursor);
     st.map_cursor = st.inflow_cursor = 1;
+    st.map_last = 1;  // recording_parameters.last_record_maps (HydroPol2D_preprocessing.m:2396)
     if (s->n_maps > 0) {
         MapTerms mt;
         if (!map_terms(in->rain_map_time_min, s->n_maps, 0.0, st.t, &st.map_cursor, &mt))
@@ -1236,7 +1319,9 @@ void hp2d_sim_destroy(hp2d_sim *s) {
                   s->tpeak, s->wqB[0], s->wqB[1], s->wqBs, s->wqBx[0], s->wqBx[1], s->wqOm,
                   s->wqTW[0], s->wqTW[1], s->wqPc, s->wqC3, s->wqC4, s->wq_cell, s->wq_part,
                   s->wq_ctl, s->gw_mem, s->gw_col, s->gw_dom, s->maps, s->map_t, s->map_sum,
-                  s->inflow_t, s->inflow_v, s->gauge_cells, s->gauge};
+                  s->inflow_t, s->inflow_v, s->gauge_cells, s->gauge, s->max_risk, s->snap_vras,
+                  s->snap_vtot, s->snap_t, s->stack[0], s->stack[1], s->stack[2], s->stack[3],
+                  s->stack[4], s->stack[5], s->stack[6], s->stack[7], s->stack[8]};
     for (void *p : ps)
         if (p) hipFree(p);
     if (s->gwh) hp2d_destroy(s->gwh);
@@ -1311,9 +1396,13 @@ static hp2d_status sim_release_hold(hp2d_sim *s, SimState &st) {
     if (s->elem == 8) {
         e = launch_wq_finish<double>(sim_wq_args<double>(s), ctl.steps, s->stream);
         if (e == hipSuccess) e = launch_finalize<double>(sim_fin_args<double>(s), s->stream);
+        if (e == hipSuccess && s->snap_slots > 0)
+            e = launch_snapshot<double>(sim_snap_args<double>(s), s->stream);
     } else {
         e = launch_wq_finish<float>(sim_wq_args<float>(s), ctl.steps, s->stream);
         if (e == hipSuccess) e = launch_finalize<float>(sim_fin_args<float>(s), s->stream);
+        if (e == hipSuccess && s->snap_slots > 0)
+            e = launch_snapshot<float>(sim_snap_args<float>(s), s->stream);
     }
     STRY(s, e);
     return read_state(s, st);
@@ -1348,6 +1437,34 @@ static hp2d_status sim_gw_update(hp2d_sim *s, SimState &st) {
     return write_state(s, st);
 }
 
+// Move the records the device stack holds to the host (the Maps structure of
+// save_output_maps.m, kept in host memory as the reference keeps it).
+static hp2d_status sim_drain_maps(hp2d_sim *s, SimState &st) {
+    if (s->snap_slots == 0 || st.snap_n == 0) return HP2D_OK;
+    const int nr = st.snap_n;
+    std::vector<double> tt(nr);
+    STRY(s, hipMemcpy(tt.data(), s->snap_t, nr * sizeof(double), hipMemcpyDeviceToHost));
+    for (int f = 0; f < HP2D_MAP_FIELDS; ++f) {
+        if (!s->stack[f]) continue;
+        for (int r = 0; r < nr; ++r) {
+            std::vector<double> v(s->n);
+            if (s->elem == 8) {
+                STRY(s, hipMemcpy(v.data(), (char *)s->stack[f] + (size_t)r * s->n * 8, s->n * 8,
+                                  hipMemcpyDeviceToHost));
+            } else {
+                std::vector<float> w(s->n);
+                STRY(s, hipMemcpy(w.data(), (char *)s->stack[f] + (size_t)r * s->n * 4, s->n * 4,
+                                  hipMemcpyDeviceToHost));
+                for (size_t c = 0; c < s->n; ++c) v[c] = w[c];
+            }
+            s->records[f].push_back(std::move(v));
+        }
+    }
+    for (int r = 0; r < nr; ++r) s->record_t.push_back(tt[r]);
+    st.snap_n = 0;
+    return write_state(s, st);
+}
+
 // Advance by n_accept accepted steps (0: to the end of the run) in captured
 // graphs of `batch` steps, pausing for the groundwater calls and releasing
 // wash-off holds between graphs.
@@ -1358,7 +1475,8 @@ static hp2d_status sim_advance(hp2d_sim *s, int64_t n_accept, int batch, int32_t
     const int64_t target = n_accept > 0 ? st.accepted + n_accept : -1;
     const int every = s->gw ? s->cfg.gw_every_steps : 0;
     if (every > 0) batch = std::min(batch, every);
-    const bool use_graph = batch >= 8 || (batch >= 1 && (s->wq || s->gw));
+    if (s->snap_slots > 0) batch = std::min(batch, s->snap_slots);
+    const bool use_graph = batch >= 8 || (batch >= 1 && (s->wq || s->gw || s->snap_slots > 0));
     if (use_graph && (rc = ensure_graph(s, batch))) return rc;
     // every launch of a graph accepts, rejects or skips (paused / held) each of
     // its steps, and a release always completes the held step; a run longer
@@ -1388,6 +1506,7 @@ static hp2d_status sim_advance(hp2d_sim *s, int64_t n_accept, int batch, int32_t
         }
         if ((rc = read_state(s, st))) return rc;
         if (st.hold && (rc = sim_release_hold(s, st))) return rc;
+        if ((rc = sim_drain_maps(s, st))) return rc;
     }
     st.stop_accepted = -1;
     st.paused = 0;
@@ -1407,7 +1526,7 @@ hp2d_status hp2d_sim_run(hp2d_sim *s, int64_t max_steps, int32_t *done) {
         if (done) *done = 1;
         return HP2D_OK;
     }
-    if (s->wq || s->gw) {
+    if (s->wq || s->gw || s->snap_slots > 0) {
         const int batch = (int)std::min<int64_t>(64, max_steps > 0 ? max_steps : 64);
         if ((rc = sim_advance(s, max_steps, batch, done))) return rc;
         if ((rc = read_state(s, st))) return rc;
@@ -1457,11 +1576,12 @@ hp2d_status hp2d_sim_run_fixed(hp2d_sim *s, int64_t n_steps, int32_t graph_steps
         // the batch sim_advance uses (a groundwater call cuts every N steps), so
         // that the graph built here is the one it launches
         if (s->gw) graph_steps = std::min<int32_t>(graph_steps, s->cfg.gw_every_steps);
+        if (s->snap_slots > 0) graph_steps = std::min<int32_t>(graph_steps, s->snap_slots);
         hp2d_status rc = ensure_graph(s, graph_steps);
         if (rc) return rc;
     }
     STRY(s, hipEventRecord(e0, s->stream));
-    if (graph_steps > 0 && (s->wq || s->gw)) {
+    if (graph_steps > 0 && (s->wq || s->gw || s->snap_slots > 0)) {
         hp2d_status rc = sim_advance(s, n_steps, graph_steps, nullptr);
         if (rc) return rc;
     } else if (graph_steps > 0) {
@@ -1491,10 +1611,11 @@ hp2d_status hp2d_sim_run_fixed(hp2d_sim *s, int64_t n_steps, int32_t graph_steps
             else e = sim_after_li<float>(s);
             STRY(s, e);
             STRY(s, hipEventSynchronize(a1));
-            if (s->wq) {  // a held wash-off split completes before the next launch
-                SimState st;
+            if (s->wq || s->snap_slots > 0) {  // a held wash-off split completes before the
+                SimState st;                    // next launch; map records move to the host
                 hp2d_status rc = read_state(s, st);
                 if (!rc && st.hold) rc = sim_release_hold(s, st);
+                if (!rc) rc = sim_drain_maps(s, st);
                 if (rc) return rc;
             }
             float ms = 0;
@@ -1541,6 +1662,7 @@ hp2d_status hp2d_sim_report(hp2d_sim *s, hp2d_run_report *rep) {
     rep->outlet_runoff_volume_mm = st.runoff_mm;
     rep->cum_rain_m3 = st.cum_rain;
     rep->cum_inflow_m3 = st.cum_inflow;
+    rep->n_map_records = (int32_t)s->record_t.size();
     rep->cum_infiltration_m3 = st.cum_inf;
     rep->cum_outflow_m3 = st.cum_out;
     {
@@ -1700,6 +1822,41 @@ hp2d_status hp2d_sim_get_dt_history(hp2d_sim *s, double *dt_s, int32_t n_max, in
         dt_s[i] = ring[step % s->n_dt_max];
     }
     if (n_out) *n_out = (int32_t)take;
+    return HP2D_OK;
+}
+
+hp2d_status hp2d_sim_get_max_risk(hp2d_sim *s, double *max_risk) {
+    if (!s || !max_risk) return HP2D_E_ARG;
+    if (!s->max_risk) {
+        set_err(s, "flag_human_instability is off in this run");
+        return HP2D_E_STATE;
+    }
+    SimState st;
+    hp2d_status rc = read_state(s, st);
+    if (rc) return rc;
+    return fetch_real(s, max_risk, s->max_risk, 0.0);
+}
+
+hp2d_status hp2d_sim_get_map_records(hp2d_sim *s, int32_t field, int32_t first, int32_t count,
+                                     double *out, double *t_min, int32_t *n_records) {
+    if (!s || field < 0 || field >= HP2D_MAP_FIELDS || first < 0 || count < 0) return HP2D_E_ARG;
+    const int32_t total = (int32_t)s->record_t.size();
+    if (n_records) *n_records = total;
+    if (count == 0) return HP2D_OK;
+    if (first + count > total) {
+        set_err(s, "map records: the range passes the records taken");
+        return HP2D_E_ARG;
+    }
+    if (out) {
+        if (s->records[field].empty()) {
+            set_err(s, "map records: this field is not recorded in this run");
+            return HP2D_E_STATE;
+        }
+        for (int32_t r = 0; r < count; ++r)
+            std::memcpy(out + (size_t)r * s->n, s->records[field][first + r].data(), s->n * 8);
+    }
+    if (t_min)
+        for (int32_t r = 0; r < count; ++r) t_min[r] = s->record_t[first + r];
     return HP2D_OK;
 }
 
@@ -2057,9 +2214,10 @@ hp2d_status hp2d_ensemble_create(int32_t n_members, const hp2d_run_config *cfgs,
         return HP2D_E_ARG;
     }
     for (int i = 0; i < n_members; ++i) {
-        if (cfgs[i].flag_waterquality == 1 || cfgs[i].flag_groundwater == 1) {
+        if (cfgs[i].flag_waterquality == 1 || cfgs[i].flag_groundwater == 1 ||
+            cfgs[i].record_time_maps_min > 0) {
             g_last_error = "hp2d_ensemble_create: members run the surface step and Green-Ampt "
-                           "(no wash-off or groundwater module in the batched launch)";
+                           "(no wash-off, groundwater module or map stacks in the batched launch)";
             return HP2D_E_UNSUPPORTED;
         }
     }

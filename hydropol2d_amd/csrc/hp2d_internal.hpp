@@ -64,6 +64,11 @@ struct SimState {
     // inflow hydrographs of the next step: BC_States.delta_inflow_agg [mm]
     double inflow_agg[16];
     double cum_inflow;
+    // map stacks (save_output_maps.m:24-27): the last record state, whether this
+    // step took a record, the records waiting in the device stack, the records
+    // taken so far, and the dt the step used
+    int map_last, snap_event, snap_n, n_map_records;
+    double dt_used_min;
 };
 static_assert(sizeof(SimState) % 4 == 0, "SimState is copied as 32-bit words");
 

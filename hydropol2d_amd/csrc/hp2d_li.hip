@@ -223,7 +223,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     ms.n = 0;
     ms.dtot = ms.k = T(0);
     int fail_step = 0;
-    bool may_report = false;
+    bool may_report = false, may_snap = false;
     const T *d_src = a.d_src, *qx_src = a.qx_src, *qy_src = a.qy_src;
     T *d_dst = a.d_dst, *qx_dst = a.qx_dst, *qy_dst = a.qy_dst;
     const T *I_src = a.I_src, *Tc_src = a.T_src;
@@ -252,6 +252,10 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
         // storage sums needed (HydroPol2D_Main_While.m:787-803)
         const double prog_max = ((st->t + st->max_dt / 60.0) / a.routing_time) * 100.0;
         may_report = (prog_max >= st->next_report) || (prog_max >= 100.0);
+        // a map record can only follow this step if t + max_dt passes the next
+        // record time (the new dt never exceeds max_dt); only then are the
+        // velocities stored
+        may_snap = a.snap_vras && (st->t + st->max_dt / 60.0 > (double)st->map_last * a.rec_maps);
         const int src = (int)((kstep + 1) & 1), dst = (int)(kstep & 1);
         d_src = a.d_buf[src];
         d_dst = a.d_buf[dst];
@@ -492,14 +496,37 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
                                             : dmax(dmax(dmax(fabs(vl), fabs(vr)), fabs(vu)), fabs(vd));
                 const T rc = T(0.5) * (vr - vl), uc = T(0.5) * (vu - vd);
                 bool stagnant;
+                T vtot = T(0);  // velocities.total_velocity (Human_Instability_Module.m:9)
                 if (f64) {
                     const T tot = hypot(rc, uc);
                     vkey = dmax(vkey, tot);
                     stagnant = tot < T(0.01);
+                    vtot = tot;
                 } else {
                     const T tot2 = rc * rc + uc * uc;  // compared squared, rooted once per step
                     vkey = dmax(vkey, tot2);
                     stagnant = tot2 < T(1e-4);
+                    if (a.max_risk || may_snap) vtot = sqrt(tot2);
+                }
+                if (a.max_risk) {
+                    // Human_Instability_Module.m:8-25 and its running maximum
+                    const T h = mm2m(dn);
+                    const T F_person = a.hi_weight * a.hi_g;
+                    const T F_buoy = (a.hi_w1 * a.hi_w2) * h * a.hi_ro * a.hi_g;
+                    const T avail = a.hi_mu * (dmax(F_person - F_buoy, T(0)));
+                    const T hydro = dmax(T(1) / T(2) * (a.hi_Cd * a.hi_ro * a.hi_w1 * h * (vtot * vtot)), T(0));
+                    T risk = T(0);
+                    if (avail > T(0)) risk = hydro / avail;
+                    else if (h > T(0)) risk = T(1);
+                    risk = dmin(dmax(risk, T(0)), T(1));
+                    if (isnan(dn)) risk = T(NAN);
+                    const T prev = (kstep == 1) ? (isnan(dn) ? T(NAN) : T(0)) : a.max_risk[oid];
+                    const T nm = dmax(risk, prev);
+                    if (kstep == 1 || nm != prev) a.max_risk[oid] = nm;
+                }
+                if (may_snap) {
+                    a.snap_vras[oid] = dmax(dmax(dmax(fabs(vl), fabs(vr)), fabs(vu)), fabs(vd));
+                    a.snap_vtot[oid] = vtot;
                 }
                 if (!a.adaptive) rmax = dmax(rmax, raster);
                 if (!stagnant) {

@@ -43,6 +43,12 @@ template <typename T> struct LiArgs {
     int n_soil_classes;
     T I_min;
     int track_dmax;       // reduce the depth maxima every step (a late rollback can follow)
+    // Human_Instability_Module mode 1: max_risk map (or NULL) and the person's parameters
+    T *max_risk;
+    T hi_weight, hi_w1, hi_w2, hi_mu, hi_Cd, hi_ro, hi_g;
+    // map stacks: the event step's velocity_raster and total_velocity (or NULL)
+    T *snap_vras, *snap_vtot;
+    double rec_maps;
     T *dmax_map;
     T *tpeak_map;         // optional: model time [min] at which dmax was last raised
     // outlets (sorted linear indices)

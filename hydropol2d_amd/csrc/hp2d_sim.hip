@@ -203,8 +203,10 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         st->catch_index += 1;
         st->neg_events += 1;
     }
+    st->snap_event = 0;
     if (!failed) {
         const double dt_used_min = st->time_step_min;
+        st->dt_used_min = dt_used_min;
         double t_now = st->t;
         // ---------------- refreshing_timestep ----------------
         const long long pos_save = (long long)ceil((t_now * 60.0) / a.tsc);
@@ -277,6 +279,28 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
             }
         }
         st->last_record = actual;
+        // ---------------- map stacks (save_output_maps.m:24-27) ----------------
+        if (a.rec_maps > 0) {
+            // records j >= 0 at j * rec_maps <= routing_time; the state is the
+            // number of them before t_save
+            int jmax = (int)floor(routing_time / a.rec_maps + 1e-12);
+            while (jmax > 0 && (double)jmax * a.rec_maps > routing_time + 1e-12) --jmax;
+            int am = 0;
+            if (t_save > 0) {
+                am = (int)ceil(t_save / a.rec_maps);
+                while (am > 0 && (double)(am - 1) * a.rec_maps >= t_save) --am;
+                while (am <= jmax && (double)am * a.rec_maps < t_save) ++am;
+                if (am > jmax + 1) am = jmax + 1;
+            }
+            const int dm = am - st->map_last;
+            st->map_last = am;
+            if (dm > 0 && st->snap_n < a.snap_slots) {
+                st->snap_event = 1;
+                a.snap_t[st->snap_n] = t_now;
+                st->snap_n += 1;
+                st->n_map_records += 1;
+            }
+        }
         st->runoff_mm = qsum * st->time_step_min / 60.0 / a.drainage_area + st->runoff_mm;
         st->cum_rain += rain;
         st->cum_inflow += inflow_step;

@@ -23,6 +23,10 @@ template <typename T> struct FinArgs {
     // inflow hydrographs: shared end stamps, rates [n_times x n_gauges], valid cells per gauge
     const double *inflow_t, *inflow_v, *gauge_cells;
     int n_inflow_times, n_gauges;
+    // map stacks: record spacing [min] (0 = off), device slots, record times
+    double rec_maps;
+    int snap_slots;
+    double *snap_t;
     // outlets
     const T *out_q, *out_d;
     int n_outlet;

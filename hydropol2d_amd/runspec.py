@@ -142,6 +142,11 @@ class RunSpec:
     # replaces the lumped series and rain_pattern when set
     rain_maps_mmh: Optional[np.ndarray] = None
     rain_map_time_min: Optional[np.ndarray] = None
+    # Human_Instability_Module mode 1 (a person of weight [kg], body widths [m],
+    # friction coefficient and drag coefficient; water density, gravity)
+    human_instability: Optional[dict] = None
+    # map stacks every record_time_maps_min of model time (0: off)
+    record_time_maps_min: float = 0.0
 
     @property
     def shape(self):
@@ -230,6 +235,17 @@ class RunSpec:
             ins.wq_Bmin = wq.Bmin
             ins.wq_Bmax = wq.Bmax
             ins.wq_min_Bt = wq.min_Bt
+        hi = self.human_instability
+        if hi is not None:
+            cfg.flag_human_instability = 1
+            cfg.hi_weight_kg = hi.get("weight_person", 70.0)
+            cfg.hi_width1_m = hi.get("width1_person", 0.35)
+            cfg.hi_width2_m = hi.get("width2_person", 0.25)
+            cfg.hi_mu = hi.get("mu", 0.5)
+            cfg.hi_Cd = hi.get("Cd", 1.1)
+            cfg.hi_ro_water = hi.get("ro_water", 1000.0)
+            cfg.hi_gravity = hi.get("gravity", 9.81)
+        cfg.record_time_maps_min = self.record_time_maps_min
         fl = self.inflow
         if fl is not None:
             rate = np.asarray(fl.rate_mmh, dtype=np.float64)

@@ -124,6 +124,31 @@ class Simulation:
         self._check(self._L.hp2d_sim_get_time_to_peak(self._s, t.ctypes.data_as(_dp)))
         return t
 
+    def max_risk(self):
+        """Human_Instability.max_risk (Human_Instability_Module.m:22)."""
+        m = np.zeros((self.ny, self.nx), order="F")
+        self._check(self._L.hp2d_sim_get_max_risk(self._s, m.ctypes.data_as(_dp)))
+        return m
+
+    def map_records(self):
+        """The map stacks taken so far (save_output_maps.m:148-258): a dict of
+        field -> ny x nx x n_records arrays (the fields this run records) and
+        "t_min" -> the model time of each record."""
+        from .abi import MAP_FIELDS
+        n = C.c_int32()
+        self._check(self._L.hp2d_sim_get_map_records(self._s, 0, 0, 0, None, None, C.byref(n)))
+        nr = n.value
+        t = np.zeros(nr)
+        out = {"t_min": t}
+        for f, name in enumerate(MAP_FIELDS):
+            buf = np.zeros(nr * self.ny * self.nx)
+            st = self._L.hp2d_sim_get_map_records(self._s, f, 0, nr, buf.ctypes.data_as(_dp),
+                                                  t.ctypes.data_as(_dp), C.byref(n))
+            if st != 0:
+                continue  # a field this run does not record
+            out[name] = np.asfortranarray(buf.reshape(nr, self.nx, self.ny).transpose(2, 1, 0))
+        return out
+
     def dt_history(self, n_max: int = 4096):
         buf = np.zeros(n_max)
         got = C.c_int32()

@@ -310,7 +310,19 @@ typedef struct hp2d_run_config {
     int32_t n_inflow_gauges;           /* 1..HP2D_MAX_INFLOW_GAUGES */
     int32_t n_rain_maps;               /* > 0: rainfall as a series of rate maps instead of the
                                           lumped series (update_spatial_BC.m:469-518) */
+    int32_t flag_human_instability;    /* 1 = Human_Instability_Module.m:4-26 (risk, max_risk) */
+    double hi_weight_kg, hi_width1_m, hi_width2_m, hi_mu, hi_Cd, hi_ro_water, hi_gravity;
+    double record_time_maps_min;       /* > 0: map stacks at running_control.time_records
+                                          (save_output_maps.m:24-27, 148-258) */
 } hp2d_run_config;
+
+/* Fields of a map-stack record (save_output_maps.m:148-258: Maps.Hydro.d, velocity,
+ * hazard_dv, risk, I_t, GWdepth_save; Maps.WQ_States.Pol_Conc_Map, Pol_Mass_Map,
+ * Pol_Load_Map) */
+enum {
+    HP2D_MAP_DEPTH = 0, HP2D_MAP_VELOCITY, HP2D_MAP_HAZARD, HP2D_MAP_RISK, HP2D_MAP_INFILTRATION,
+    HP2D_MAP_GW_DEPTH, HP2D_MAP_POL_CONC, HP2D_MAP_POL_MASS, HP2D_MAP_POL_LOAD, HP2D_MAP_FIELDS
+};
 
 #define HP2D_MAX_INFLOW_GAUGES 16
 
@@ -402,6 +414,7 @@ typedef struct hp2d_run_report {
     double gw_error_m3;                /* errors(5) of the last update */
     int32_t gw_updates;
     double cum_inflow_m3;              /* every-step FP64 ledger of the inflow hydrographs */
+    int32_t n_map_records;             /* map-stack records taken so far */
 } hp2d_run_report;
 
 typedef struct hp2d_sim hp2d_sim;
@@ -446,6 +459,15 @@ hp2d_status hp2d_sim_get_dt_history(hp2d_sim *s, double *dt_s, int32_t n_max,
 
 /* Force the step with counter k == step_k to be rejected (kind 1), to test the
  * host rollback path of HydroPol2D_Main_While.m:926-993 without a real fault. */
+/* Human_Instability_Module.m:22 Human_Instability.max_risk (ny*nx) */
+hp2d_status hp2d_sim_get_max_risk(hp2d_sim *s, double *max_risk);
+/* The map-stack records taken so far (save_output_maps.m:148-258, Maps.Hydro and
+ * Maps.WQ_States), field one of HP2D_MAP_*: records first .. first + count - 1 into
+ * out (count * ny*nx, optional) with their model times t_min (optional);
+ * *n_records receives the number taken.  The records are held in host memory
+ * as the reference's Maps structure is. */
+hp2d_status hp2d_sim_get_map_records(hp2d_sim *s, int32_t field, int32_t first, int32_t count,
+                                     double *out, double *t_min, int32_t *n_records);
 hp2d_status hp2d_debug_inject_failure(hp2d_sim *s, int64_t step_k, int32_t kind);
 
 /* Project autotuner of the fused step's launch shape: times the candidate

@@ -321,6 +321,7 @@ double oracle_refresh_dt(int32_t ny, int32_t nx, const double *qout_right,
             if (velocity_raster) velocity_raster[c] = raster;
             double rc = 0.5 * (vr[c] - vl), uc = 0.5 * (vu[c] - vd);
             double tot = hypot(rc, uc);
+            if (p->total_velocity) p->total_velocity[c] = tot;
             int stagnant = tot < 0.01;
             vmax = fmax(vmax, tot);
             /* :227-238 */
@@ -465,6 +466,21 @@ void oracle_ga_cells(int64_t n, int32_t first_step, double dt_h, double I_min_mm
 }
 
 /* ------------------------------------------------------------------------ */
+/* Human_Instability_Module.m:4-26 (mode 1): the risk of one cell at depth h */
+/* [m] and total velocity v [m/s].                                           */
+/* ------------------------------------------------------------------------ */
+static double hi_risk(const hp2d_run_config *cfg, double h, double v) {
+    const double F_person = cfg->hi_weight_kg * cfg->hi_gravity;
+    const double F_buoy = (cfg->hi_width1_m * cfg->hi_width2_m) * h * cfg->hi_ro_water * cfg->hi_gravity;
+    const double avail = cfg->hi_mu * (fmax(F_person - F_buoy, 0.0));
+    const double hydro = fmax(1.0 / 2.0 * (cfg->hi_Cd * cfg->hi_ro_water * cfg->hi_width1_m * h * (v * v)), 0.0);
+    double r = 0.0;
+    if (avail > 0) r = hydro / avail;
+    else if (h > 0) r = 1.0;
+    return fmin(fmax(r, 0.0), 1.0);
+}
+
+/* ------------------------------------------------------------------------ */
 /* update_spatial_BC.m:362-398, 469-518: the inflow hydrographs and the      */
 /* rainfall maps of the next step over (t_prev, t).                          */
 /* ------------------------------------------------------------------------ */
@@ -536,6 +552,10 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     double *qr = zalloc(n), *qu = zalloc(n), *qof = zalloc(n), *dmax = zalloc(n);
     double *I_t = zalloc(n), *I_p = zalloc(n), *Tclk = zalloc(n), *vraster = zalloc(n);
     double *area_nan = zalloc(n), *ga_f = zalloc(n);
+    double *vtot = zalloc(n), *max_risk = zalloc(n), *risk = zalloc(n);
+    const int do_hi = cfg->flag_human_instability == 1;
+    const int do_maps = cfg->record_time_maps_min > 0;
+    int32_t last_record_maps = 1; /* recording_parameters.last_record_maps (preprocessing.m:2396) */
     uint8_t *is_nan = (uint8_t *)calloc((size_t)n, 1);
     int64_t *oidx = (int64_t *)calloc((size_t)n, sizeof(int64_t));
     int32_t n_out = 0;
@@ -647,7 +667,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     double routing_time = cfg->routing_time_min;
     oracle_dt_params dp = {R, cfg->alfa_min, cfg->alfa_max, 0, 0, cfg->time_step_increments_s,
                            cfg->depth_tolerance_mm, cfg->flag_timestep,
-                           cfg->flag_adaptive_timestepping, 0, 0.0, 0.0};
+                           cfg->flag_adaptive_timestepping, 0, 0.0, 0.0, NULL};
 
     /* Main_While.m:109-127 + preprocessing.m:2469,2615,2618 */
     int64_t k = 1;
@@ -823,6 +843,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                 dp.min_time_step = min_dt;
                 dp.flag_waterquality = do_wq;  /* refreshing_timestep.m:329-332 */
                 dp.tmin_wq = tmin_wq;
+                dp.total_velocity = vtot;
                 tcr = oracle_refresh_dt(ny, nx, qr, qu, Hf, &dp, &max_velocity, vraster);
                 if (out->dt_follow && n_dt < out->n_follow && out->dt_follow[n_dt] != tcr) {
                     const double inc = cfg->time_step_increments_s;
@@ -843,6 +864,13 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             }
             /* :711-715 first report check (consumes thresholds) */
             if (progress >= next_report || progress >= 100.0) next_report += cfg->report_every_percent;
+            /* :718 Human_Instability_Module (mode 1) on the new depths and velocities */
+            if (do_hi) {
+                for (int64_t c = 0; c < n; ++c) {
+                    risk[c] = is_nan[c] ? NAN : hi_risk(cfg, d_t[c] / 1000.0, vtot[c]);
+                    max_risk[c] = fmax(risk[c], k == 1 ? (is_nan[c] ? NAN : 0.0) : max_risk[c]);
+                }
+            }
             /* update_spatial_BC (:721) */
             {
                 double dt_step = fmax(t - t_previous, 0.0);
@@ -902,6 +930,48 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                 }
             }
             last_record = actual;
+            /* save_output_maps.m:24-27, 148-258: a map record when the record state advances */
+            if (do_maps) {
+                const double recm = cfg->record_time_maps_min;
+                int32_t actual_m = 0;
+                for (int32_t j = 0; (double)j * recm <= routing_time + 1e-12; ++j)
+                    if ((double)j * recm < t_save) actual_m = j + 1;
+                const int32_t delta_m = actual_m - last_record_maps;
+                last_record_maps = actual_m;
+                if (delta_m > 0 && out->rec && out->n_rec < out->n_rec_max) {
+                    double *R0 = out->rec + (int64_t)out->n_rec * HP2D_MAP_FIELDS * n;
+                    for (int64_t c = 0; c < n; ++c) {
+                        /* snisb_velocity_hazard_snapshot (save_output_maps.m:832-848) */
+                        const double dm = fmax(d_t[c] / 1000.0, 0.0);
+                        double v = vraster[c];
+                        if (!isfinite(v)) v = 0.0;
+                        double hz = dm * v;
+                        if (dm < 0.01) hz = 0.0;
+                        if (!isfinite(hz)) hz = 0.0;
+                        R0[HP2D_MAP_DEPTH * n + c] = d_t[c];
+                        R0[HP2D_MAP_VELOCITY * n + c] = is_nan[c] ? NAN : v;
+                        R0[HP2D_MAP_HAZARD * n + c] = is_nan[c] ? NAN : hz;
+                        R0[HP2D_MAP_RISK * n + c] = do_hi ? risk[c] : NAN;
+                        R0[HP2D_MAP_INFILTRATION * n + c] = do_ga ? I_t[c] : NAN;
+                        R0[HP2D_MAP_GW_DEPTH * n + c] = do_gw ? gw_h[c] - z[c] + gw_sd[c] : NAN;
+                        if (do_wq) {
+                            /* I_tot_end_cell of the step (Local_Inertial_Model_D4.m:513-514)
+                             * over the refreshed dt (save_output_maps.m:254-256) */
+                            const double itot = fabs(outflow[c] + outflow[n + c] + outflow[2 * n + c]) *
+                                                (dt_used_min * 60.0) / 1000.0 * 1.0 / 3600.0 * (R * R);
+                            R0[HP2D_MAP_POL_CONC * n + c] = Pc[c];
+                            R0[HP2D_MAP_POL_MASS * n + c] = B[c];
+                            R0[HP2D_MAP_POL_LOAD * n + c] = 1.0 / 1000.0 * Pc[c] * itot / (time_step * 60.0) / (R * R);
+                        } else {
+                            R0[HP2D_MAP_POL_CONC * n + c] = NAN;
+                            R0[HP2D_MAP_POL_MASS * n + c] = NAN;
+                            R0[HP2D_MAP_POL_LOAD * n + c] = NAN;
+                        }
+                    }
+                    if (out->rec_t) out->rec_t[out->n_rec] = t;
+                    out->n_rec++;
+                }
+            }
             /* :588-600 dmax */
             ORACLE_PAR
             for (int64_t c = 0; c < n; ++c) dmax[c] = (k == 1) ? d_t[c] : fmax(d_t[c], dmax[c]);
@@ -1038,8 +1108,10 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
         if (out->Tot_Washed) memcpy(out->Tot_Washed, TW, (size_t)n * sizeof(double));
     }
     if (do_gw && out->gw_h) memcpy(out->gw_h, gw_h, (size_t)n * sizeof(double));
+    if (do_hi && out->max_risk) memcpy(out->max_risk, max_risk, (size_t)n * sizeof(double));
     out->n_hydro = n_hydro;
     hp2d_run_report *rp = &out->rep;
+    rp->n_map_records = out->n_rec;
     rp->steps = accepted;
     rp->rejected_steps = rejected;
     rp->t_min = t;
@@ -1093,6 +1165,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     free(gw_kg); free(gw_Sy); free(gw_q); free(gw_pn); free(gw_pr); free(gw_pc); free(gw_cum);
     free(gw_dom);
     free(fs.dp_map);
+    free(vtot); free(max_risk); free(risk);
     return 0;
 }
 

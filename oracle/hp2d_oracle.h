@@ -51,6 +51,7 @@ typedef struct oracle_dt_params {
     int32_t flag_timestep, flag_adaptive_timestepping, flag_waterquality;
     double tmin_wq;
     double nt_raw;  /* out: the new time step before the snap to the increments */
+    double *total_velocity; /* optional out: velocities.total_velocity per cell [m/s] */
 } oracle_dt_params;
 
 /* returns time_calculation_routing [s]; writes max_velocity and optionally
@@ -101,6 +102,13 @@ typedef struct oracle_run_out {
      * (n_dt_mismatch still counts the steps where its own differed): the FP32
      * device path is then compared with it on the same time levels. */
     int32_t dt_force;
+    /* Human_Instability_Module max_risk (ny*nx, optional) and the map-stack
+     * records (save_output_maps.m:148-258): n_rec_max records of
+     * HP2D_MAP_FIELDS maps each, record r field f at rec + (r * F + f) * ny*nx
+     * (optional), with the model time of each record */
+    double *max_risk;
+    double *rec, *rec_t;
+    int32_t n_rec_max, n_rec;
 } oracle_run_out;
 
 /* Green-Ampt derived parameters of one soil class (HydroPol2D_preprocessing.m:

@@ -48,6 +48,7 @@ class DtParams(C.Structure):
         ("time_step_increments", C.c_double), ("depth_tolerance_mm", C.c_double),
         ("flag_timestep", C.c_int32), ("flag_adaptive_timestepping", C.c_int32),
         ("flag_waterquality", C.c_int32), ("tmin_wq", C.c_double), ("nt_raw", C.c_double),
+        ("total_velocity", _dp),
     ]
 
 
@@ -62,7 +63,8 @@ class RunOut(C.Structure):
         ("dt_follow", _dp), ("n_follow", C.c_int64), ("n_dt_ties", C.c_int64),
         ("n_dt_mismatch", C.c_int64), ("stop_after_rollback", C.c_int32),
         ("rollback_dt_s", C.c_double), ("rollback_max_dt_s", C.c_double),
-        ("dt_force", C.c_int32),
+        ("dt_force", C.c_int32), ("max_risk", _dp), ("rec", _dp), ("rec_t", _dp),
+        ("n_rec_max", C.c_int32), ("n_rec", C.c_int32),
     ]
 
 
@@ -266,7 +268,7 @@ def ga_step(params, first_step, dt_h, d, I_p, T, I_min_mm=1e-3):
 
 def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: int = 200000,
         n_report_max: int = 64, dt_follow=None, stop_after_rollback: bool = False,
-        dt_force: bool = False):
+        dt_force: bool = False, n_map_records: int = 0):
     """Full Main_While run on the oracle.  Returns a dict of final state + ledgers.
 
     dt_follow: the accepted-step dt sequence [s] of another implementation; the
@@ -309,6 +311,12 @@ def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: 
         follow = np.ascontiguousarray(dt_follow, dtype=np.float64)
         out.dt_follow = _p(follow)
         out.n_follow = follow.size
+    res["max_risk"] = np.zeros((ny, nx), order="F")
+    out.max_risk = _p(res["max_risk"])
+    if n_map_records > 0:
+        recbuf = np.zeros(n_map_records * len(abi.MAP_FIELDS) * ny * nx)
+        rect = np.zeros(n_map_records)
+        out.rec, out.rec_t, out.n_rec_max = _p(recbuf), _p(rect), n_map_records
     out.stop_after_rollback = int(stop_after_rollback)
     out.dt_force = int(dt_force)
     rc = lib().oracle_run(C.byref(cfg), C.byref(ins), C.byref(out))
@@ -316,6 +324,12 @@ def run(spec: RunSpec, max_steps: int = 0, inject_fail_step: int = 0, n_dt_max: 
         raise RuntimeError(f"oracle_run failed with status {rc}")
     rep = out.rep.as_dict()
     res["report"] = rep
+    if n_map_records > 0:
+        nr = out.n_rec
+        R = recbuf.reshape(n_map_records, len(abi.MAP_FIELDS), nx, ny)[:nr]
+        res["map_records"] = {name: np.asfortranarray(R[:, f].transpose(2, 1, 0))
+                              for f, name in enumerate(abi.MAP_FIELDS)}
+        res["map_record_t"] = rect[:nr]
     res["rollback_dt_s"] = out.rollback_dt_s
     res["rollback_max_dt_s"] = out.rollback_max_dt_s
     res["n_hydro"] = out.n_hydro
