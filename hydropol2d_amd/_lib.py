@@ -75,6 +75,10 @@ SIGNATURES = {
     "hp2d_sim_get_max_risk": (C.c_int32, [_vp, _dp]),
     "hp2d_sim_get_map_records": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp,
                                              C.POINTER(C.c_int32)]),
+    "hp2d_geotiff_info": (C.c_int32, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                      _dp]),
+    "hp2d_geotiff_read": (C.c_int32, [C.c_char_p, _dp]),
+    "hp2d_mat_write": (C.c_int32, [C.c_char_p, C.c_int32, _vp]),
     "hp2d_idw_interpolate": (C.c_int32, [C.c_int32, _dp, _dp, _dp, C.c_int32, _dp, C.c_int32,
                                          _dp, C.c_double, C.c_double, _dp]),
     "hp2d_nn_conv3x3_dgrad_noalloc": (C.c_int32, [C.c_int32] * 5 + [_vp] * 4),

@@ -52,7 +52,7 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(f"--- {src}\n{text}\n")
         raise RuntimeError("hipcc failed for: " + ", ".join(os.path.basename(s) for s, _ in failed))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lz"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

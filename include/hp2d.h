@@ -385,6 +385,35 @@ hp2d_status hp2d_idw_interpolate(int32_t n_obs, const double *x_obs, const doubl
                                  int32_t ny_grid, const double *y_grid, double p, double rad,
                                  double *out);
 
+/* ------------------------------------------------------------------------ */
+/* On-disk formats (host code)                                              */
+/* ------------------------------------------------------------------------ */
+
+/* GeoTIFF rasters as GRIDobj reads them (HydroPol2D_preprocessing.m:192-194):
+ * one band, strips or tiles, uncompressed / PackBits / LZW / Deflate,
+ * predictor 1-3, 8-64-bit integers or floats, either byte order.  info gives
+ * the size and geo6 = {x of the upper-left corner, y of it, dx, dy, nodata,
+ * EPSG code}; read writes Z (ny*nx column-major, row 1 = the first image row,
+ * GDAL_NODATA cells NaN). */
+hp2d_status hp2d_geotiff_info(const char *path, int32_t *ny, int32_t *nx, double *geo6);
+hp2d_status hp2d_geotiff_read(const char *path, double *Z);
+
+/* MATLAB Level 5 MAT-file writer for the temporary outputs of
+ * save_output_maps.m:575-580, 620-830 (save_map_hydro_N.mat, which holds the
+ * Maps struct, and current_time_series_outputs.mat, which holds TempTS):
+ * each variable is named by a dotted path ("Maps.Hydro.d") and the paths are
+ * written as nested 1 x 1 structs; data are column-major. */
+enum { HP2D_MAT_DOUBLE = 0, HP2D_MAT_SINGLE = 1, HP2D_MAT_INT32 = 2, HP2D_MAT_UINT8 = 3,
+       HP2D_MAT_CHAR = 4 };
+typedef struct hp2d_mat_var {
+    const char *name;
+    int32_t mclass;                    /* HP2D_MAT_* (CHAR: ASCII bytes) */
+    int32_t ndims;
+    const int64_t *dims;
+    const void *data;
+} hp2d_mat_var;
+hp2d_status hp2d_mat_write(const char *path, int32_t n_vars, const hp2d_mat_var *vars);
+
 /* Results of hp2d_sim_run, accumulated on the device in FP64. */
 typedef struct hp2d_run_report {
     int64_t steps;                     /* accepted steps (k - 1) */

@@ -74,3 +74,26 @@ def test_fp32_map_records_within_1mm():
           f"{np.nanmax(np.abs(mr - o['max_risk'])):.2e}", flush=True)
     assert err <= 1e-3
     assert np.nanmax(np.abs(mr - o["max_risk"])) <= 1e-3
+
+
+def test_map_stacks_and_time_series_written_as_mat(tmp_path):
+    # the device records written as save_map_hydro_N.mat and
+    # current_time_series_outputs.mat, read back with SciPy's MAT-file reader
+    from scipy.io import loadmat
+    from hydropol2d_amd import io as hio
+    from hydropol2d_amd.simulation import Simulation
+    spec = _plane()
+    with Simulation(spec) as s:
+        s.run()
+        rec = s.map_records()
+        paths = hio.save_map_stacks(str(tmp_path), rec)
+        ts = hio.write_time_series_outputs(str(tmp_path), s)
+        hy = s.hydrograph()
+    n = len(rec["t_min"])
+    assert len(paths) == (n + 11) // 12
+    H = loadmat(paths[0])["Maps"]["Hydro"][0, 0]
+    np.testing.assert_array_equal(H["d"][0, 0], rec["d"][:, :, :12].astype(np.float32))
+    np.testing.assert_array_equal(H["risk"][0, 0], rec["risk"][:, :, :12].astype(np.float32))
+    T = loadmat(ts)["TempTS"][0, 0]
+    q = T["outlet"][0, 0]["hydrograph_m3_s"][0, 0].ravel()
+    np.testing.assert_array_equal(q, hy["hydro_q"][:hy["n_hydro"]])
