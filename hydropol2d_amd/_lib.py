@@ -50,6 +50,17 @@ SIGNATURES = {
     "hp2d_build_up_wash_off": (C.c_int32, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                            C.c_double, C.c_double, _u8p, C.c_int32, _dp,
                                            C.c_double, C.c_double, C.c_double] + [_dp] * 7),
+    "hp2d_bates_inertial_8d": (C.c_int32, [_vp, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                           C.c_double, C.c_int32, C.c_double, _i32p, _i32p,
+                                           C.c_int32, _dp, C.c_int32, _dp, _dp, _dp, _dp]),
+    "hp2d_ca_routing_8d": (C.c_int32, [_vp, _dp, _dp, _dp, C.c_double, C.c_double, _dp,
+                                       C.c_double, _dp, C.c_int32, C.c_double, _i32p, _i32p,
+                                       C.c_int32, C.c_int32, C.c_double, _dp, _dp, _dp]),
+    "hp2d_inflow_gather": (C.c_int32, [_vp, _dp, C.c_int32, C.c_double, _dp, _dp]),
+    "hp2d_last_kernel_time": (C.c_int32, [_vp, _dp]),
+    "hp2d_build_up_wash_off_8d": (C.c_int32, [_vp, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                              _u8p, C.c_int32, _dp, C.c_double, C.c_double,
+                                              C.c_double] + [_dp] * 7),
     "hp2d_boussinesq_2d": (C.c_int32, [_vp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp,
                                        _dp, _dp, C.c_double, _dp, _u8p, _u8p, _dp, _dp, _dp, _dp,
                                        _dp, _dp, _dp, _i32p]),

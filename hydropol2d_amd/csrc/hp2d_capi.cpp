@@ -94,6 +94,8 @@ struct hp2d_handle {
     size_t ws_bytes = 0;
     void *ws2 = nullptr;         // second workspace (groundwater module around Boussinesq)
     size_t ws2_bytes = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};  // around the compute kernels of a timed drop-in
+    bool ev_valid = false;
 };
 
 static void set_err(hp2d_handle *h, const std::string &m) {
@@ -169,11 +171,26 @@ void hp2d_destroy(hp2d_handle *h) {
     if (h->out_n) hipFree(h->out_n);
     if (h->ws) hipFree(h->ws);
     if (h->ws2) hipFree(h->ws2);
+    for (auto &e : h->ev)
+        if (e) hipEventDestroy(e);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
 }
 
 const char *hp2d_last_error(const hp2d_handle *h) { return h ? h->err.c_str() : g_last_error.c_str(); }
+
+hp2d_status hp2d_last_kernel_time(hp2d_handle *h, double *ms) {
+    if (!h || !ms) return HP2D_E_ARG;
+    if (!h->ev_valid) {
+        set_err(h, "hp2d_last_kernel_time: no timed call on this handle yet");
+        return HP2D_E_STATE;
+    }
+    HTRY(h, hipEventSynchronize(h->ev[1]));
+    float t = 0.0f;
+    HTRY(h, hipEventElapsedTime(&t, h->ev[0], h->ev[1]));
+    *ms = (double)t;
+    return HP2D_OK;
+}
 
 }  // extern "C"
 
@@ -186,6 +203,17 @@ hp2d_status hp2d_handle_shape(hp2d_handle *h, int32_t *ny, int32_t *nx, int32_t 
 }
 
 hipStream_t hp2d_handle_stream(hp2d_handle *h) { return h->stream; }
+
+hipError_t hp2d_handle_mark(hp2d_handle *h, int which) {
+    if (!h->ev[which]) {
+        hipError_t e = hipEventCreate(&h->ev[which]);
+        if (e != hipSuccess) return e;
+    }
+    if (which == 0) h->ev_valid = false;
+    hipError_t e = hipEventRecord(h->ev[which], h->stream);
+    if (e == hipSuccess && which == 1) h->ev_valid = true;
+    return e;
+}
 
 hp2d_status hp2d_handle_fail(hp2d_handle *h, const std::string &msg) {
     set_err(h, msg);

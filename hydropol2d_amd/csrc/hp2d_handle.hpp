@@ -11,6 +11,9 @@
 hp2d_status hp2d_handle_shape(hp2d_handle *h, int32_t *ny, int32_t *nx, int32_t *precision);
 // the handle's HIP stream
 hipStream_t hp2d_handle_stream(hp2d_handle *h);
+// record the start (which = 0) or end (1) event of a timed drop-in's compute
+// kernels on the handle's stream (read back by hp2d_last_kernel_time)
+hipError_t hp2d_handle_mark(hp2d_handle *h, int which);
 // record an error on the handle and return HP2D_E_HIP
 hp2d_status hp2d_handle_fail(hp2d_handle *h, const std::string &msg);
 // device workspace of at least `bytes` owned by the handle (grown on demand,

@@ -14,6 +14,17 @@ constexpr int kHalo = 2;                   // radius of the step stencil (donor 
 constexpr int kRowsOut = kWave - 2 * kHalo; // rows finalised per wave slab
 constexpr int kXcds = 8;                    // accelerator complex dies of an MI355X
 constexpr int kMaxWavesPerBlock = 4;
+
+// XCD-aware workgroup order: the hardware deals workgroups round-robin to the
+// 8 XCDs, so workgroup w runs on XCD w % 8.  Give each XCD a contiguous run of
+// logical tiles so that the halo shared by neighbouring tiles is fetched into
+// one L2.
+__device__ __forceinline__ int xcd_logical_tile() {
+    const int n_wg = gridDim.x;
+    const int xcd = blockIdx.x % kXcds, k_in = blockIdx.x / kXcds;
+    const int per = n_wg / kXcds, rem = n_wg % kXcds;
+    return xcd * per + min(xcd, rem) + k_in;
+}
 constexpr int kMaxSoilClasses = 256;        // class index is one byte per cell
 
 // ----------------------------------------------------------------------------

@@ -193,17 +193,6 @@ __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d
     return v;
 }
 
-// XCD-aware tile order: the hardware deals workgroups round-robin to the 8
-// XCDs, so workgroup w runs on XCD w % 8.  Give each XCD a contiguous run of
-// logical tiles (row blocks fastest, then column chunks) so that the halo rows
-// shared by neighbouring row blocks are fetched into one L2.
-__device__ __forceinline__ int xcd_logical_tile() {
-    const int n_wg = gridDim.x;
-    const int xcd = blockIdx.x % kXcds, k_in = blockIdx.x / kXcds;
-    const int per = n_wg / kXcds, rem = n_wg % kXcds;
-    return xcd * per + min(xcd, rem) + k_in;
-}
-
 // One workgroup's tile of one grid: `logical` counts the grid's tiles, row
 // blocks fastest.
 template <typename T, int MODE, bool CRIT>

@@ -161,3 +161,104 @@ def CA_Routing(reservoir_x, reservoir_y, k1, h1, k2, k3, h2, k4, yds1, xds1, yds
                                *[p(o) for o in outs])
     _lib.check(status, handle=h)
     return tuple(outs)
+
+
+def _grid(a, ny, nx):
+    return np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (ny, nx)))
+
+
+def _pages9(a, ny, nx):
+    if _empty_or(a):
+        return np.zeros((ny, nx, 9), order="F")
+    return np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (ny, nx, 9)))
+
+
+def _outlets(row_outlet, col_outlet):
+    rows = np.ascontiguousarray(np.asarray(row_outlet, dtype=np.int32).ravel())
+    cols = np.ascontiguousarray(np.asarray(col_outlet, dtype=np.int32).ravel())
+    return rows, cols, int(min(rows.size, cols.size))
+
+
+def Bates_Inertial_8D(reservoir_x, reservoir_y, k1, h1, k2, k3, h2, k4, yds1, xds1, yds2, xds2,
+                      flag_reservoir, z, d_tot, d_p, roughness_cell, cell_area, time_step,
+                      Resolution, outlet_index, outlet_type, slope_outlet, row_outlet, col_outlet,
+                      d_tolerance, outflow, idx_nan, flag_critical):
+    """Bates_Inertial_8D.m:2 on the gfx950 kernel (flag_reservoir = 0).
+
+    Returns (qout_left, qout_right, qout_up, qout_down, outlet_flow, qout_ne,
+    qout_se, qout_sw, qout_nw, d_t, I_tot_end_cell, outflow, Hf): outflow and
+    Hf are (ny, nx, 9) page stacks; outflow (the argument) is the previous
+    call's page stack [mm/h] or empty/0 for none.  idx_nan is taken from NaN
+    elevations; outlet_index from the outlet rows/cols.
+    """
+    if flag_reservoir == 1:
+        raise _lib.HP2DError(abi.HP2D_E_UNSUPPORTED, "reservoir branch is outside this build")
+    z = np.asfortranarray(z, dtype=np.float64)
+    ny, nx = z.shape
+    rows, cols, n_out = _outlets(row_outlet, col_outlet)
+    q = np.zeros((ny, nx, 9), order="F")
+    Hf = np.zeros((ny, nx, 9), order="F")
+    d_t = np.zeros((ny, nx), order="F")
+    I_tot = np.zeros((ny, nx), order="F")
+    L = _lib.load()
+    h = _Handle.get(ny, nx, abi.HP2D_FP64)
+    p = (lambda a: a.ctypes.data_as(_dp))
+    status = L.hp2d_bates_inertial_8d(
+        h, p(z), p(_grid(d_tot, ny, nx)), p(_grid(d_p, ny, nx)), p(_grid(roughness_cell, ny, nx)),
+        float(cell_area), float(time_step), float(Resolution), int(outlet_type),
+        float(slope_outlet), rows.ctypes.data_as(_i32p), cols.ctypes.data_as(_i32p), n_out,
+        p(_pages9(outflow, ny, nx)), int(flag_critical), p(q), p(d_t), p(I_tot), p(Hf))
+    _lib.check(status, handle=h)
+    return tuple(q[:, :, k].copy(order="F") for k in range(9)) + (d_t, I_tot, q, Hf)
+
+
+def CA_Routing_8D(reservoir_x, reservoir_y, k1, h1, k2, k3, h2, k4, yds1, xds1, yds2, xds2,
+                  flag_reservoir, elevation_cell, d_tot, roughness_cell, cell_area, time_step,
+                  h_0_cell, Resolution, I_tot_end_cell, outlet_index, outlet_type, slope_outlet,
+                  row_outlet, col_outlet, idx_nan, flag_critical, d_tolerance):
+    """CA_Routing_8D.m:2 on the gfx950 kernel (flag_reservoir = 0).
+
+    Returns (qout_left, qout_right, qout_up, qout_down, outlet_flow, qout_ne,
+    qout_se, qout_sw, qout_nw, d_t, I_tot_end_cell).  idx_nan is taken from
+    NaN elevations; outlet_index from the outlet rows/cols.
+    """
+    if flag_reservoir == 1:
+        raise _lib.HP2DError(abi.HP2D_E_UNSUPPORTED, "reservoir branch is outside this build")
+    z = np.asfortranarray(elevation_cell, dtype=np.float64)
+    ny, nx = z.shape
+    rows, cols, n_out = _outlets(row_outlet, col_outlet)
+    h0 = None if _empty_or(h_0_cell) else _grid(h_0_cell, ny, nx)
+    q = np.zeros((ny, nx, 9), order="F")
+    d_t = np.zeros((ny, nx), order="F")
+    I_new = np.zeros((ny, nx), order="F")
+    L = _lib.load()
+    h = _Handle.get(ny, nx, abi.HP2D_FP64)
+    p = (lambda a: a.ctypes.data_as(_dp) if a is not None else _dp())
+    status = L.hp2d_ca_routing_8d(
+        h, p(z), p(_grid(d_tot, ny, nx)), p(_grid(roughness_cell, ny, nx)), float(cell_area),
+        float(time_step), p(h0), float(Resolution), p(_grid(I_tot_end_cell, ny, nx)),
+        int(outlet_type), float(slope_outlet), rows.ctypes.data_as(_i32p),
+        cols.ctypes.data_as(_i32p), n_out, int(flag_critical), float(d_tolerance), p(q), p(d_t),
+        p(I_new))
+    _lib.check(status, handle=h)
+    return tuple(q[:, :, k].copy(order="F") for k in range(9)) + (d_t, I_new)
+
+
+def outflows_to_inflows(outflow, time_step=0.0, d_t=None):
+    """The inflow gather of HydroPol2D_Main_While.m:631-660 on the GPU.
+
+    outflow: (ny, nx, 5) pages (left, right, up, down, outlet) or (ny, nx, 9)
+    with the diagonal pages; returns qin_t [mm/h] and, when d_t is given,
+    d_t + qin_t * time_step / 60 (the cellular-automata depth update).
+    """
+    q = np.asfortranarray(np.asarray(outflow, dtype=np.float64))
+    ny, nx, k = q.shape
+    qin = np.zeros((ny, nx), order="F")
+    dt_new = None if d_t is None else _grid(d_t, ny, nx).copy(order="F")
+    L = _lib.load()
+    h = _Handle.get(ny, nx, abi.HP2D_FP64)
+    status = L.hp2d_inflow_gather(h, q.ctypes.data_as(_dp), int(k), float(time_step),
+                                  qin.ctypes.data_as(_dp),
+                                  dt_new.ctypes.data_as(_dp) if dt_new is not None else _dp())
+    _lib.check(status, handle=h)
+    return qin, dt_new

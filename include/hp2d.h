@@ -9,6 +9,9 @@
  *   Local_Inertial_Model_D4    HydroPol2D_Functions/Local_Inertial_Model_D4.m:1-516
  *   CA_Routing                 HydroPol2D_Functions/CA_Routing.m:1-171
  *   build_up_wash_off          HydroPol2D_Functions/build_up_wash_off.m:1-183
+ *   Bates_Inertial_8D          HydroPol2D_Functions/Bates_Inertial_8D.m:1-274
+ *   CA_Routing_8D              HydroPol2D_Functions/CA_Routing_8D.m:1-188
+ *   build_up_wash_off_8D       HydroPol2D_Functions/build_up_wash_off_8D.m:1-143
  *   Boussinesq_2D_explicit     HydroPol2D_Functions/Boussinesq_2D_explicit.m:1-195
  *   refreshing_timestep        HydroPol2D_Functions/refreshing_timestep.m:17-368
  *   the Main_While step loop   HydroPol2D_Functions/HydroPol2D_Main_While.m:403-994
@@ -178,6 +181,87 @@ hp2d_status hp2d_build_up_wash_off(hp2d_handle *h, const double *C_3, const doub
                                    double *B_t_out, double *P_conc, double *Out_Conc,
                                    double *tmin_wq, double *tot_W_out, double *mass_lost,
                                    double *Tot_Washed_out);
+
+/* ------------------------------------------------------------------------ */
+/* Eight-direction (D8) routines (HydroPol2D_Main_While.m:444-460, 684-688,   */
+/* flags.flag_D8 = 1).  The nine flow pages are one ny*nx*9 array in the      */
+/* reference's page order: qout_left, qout_right, qout_up, qout_down,         */
+/* outlet_flow, qout_ne, qout_se, qout_sw, qout_nw [mm/h] (page p at offset   */
+/* p*ny*nx).  Host arrays, column-major doubles, computed in FP64 on a handle */
+/* of the grid's shape.                                                      */
+/* idx_nan is the set of cells with NaN elevation.                           */
+/* ------------------------------------------------------------------------ */
+
+/* [qout_left,...,qout_nw,d_t,I_tot_end_cell,outflow,Hf] = Bates_Inertial_8D(
+ *     ..., flag_reservoir = 0, z, d_tot, d_p, roughness_cell, cell_area,
+ *     time_step, Resolution, outlet_index, outlet_type, slope_outlet,
+ *     row_outlet, col_outlet, d_tolerance, outflow, idx_nan, flag_critical)
+ * replaces Bates_Inertial_8D.m:1-274 (called at HydroPol2D_Main_While.m:454-456).
+ * Inputs ny*nx: z [m], d_tot [mm] (the depth the outflow is taken from),
+ * d_p [mm] (the depth of the water surface and face depths), roughness_cell;
+ * outflow_prev: the nine pages of the previous call [mm/h].  Outputs (NULL
+ * skips): outflow, the nine new pages [mm/h] (the nine qout arrays and the
+ * `outflow` output are the same values); d_t [mm], I_tot_end_cell [m3]: ny*nx;
+ * Hf [m]: nine pages.  d_tolerance and outlet_index do not enter the result
+ * (Bates_Inertial_8D.m:29, 84-98 read the outlet rows/cols). */
+hp2d_status hp2d_bates_inertial_8d(hp2d_handle *h, const double *z, const double *d_tot,
+                                   const double *d_p, const double *roughness_cell,
+                                   double cell_area, double time_step_min, double resolution,
+                                   int32_t outlet_type, double slope_outlet,
+                                   const int32_t *row_outlet, const int32_t *col_outlet,
+                                   int32_t n_outlet, const double *outflow_prev,
+                                   int32_t flag_critical, double *outflow, double *d_t,
+                                   double *I_tot_end_cell, double *Hf);
+
+/* [qout_left,...,qout_nw,d_t,I_tot_end_cell] = CA_Routing_8D(..., flag_reservoir = 0,
+ *     elevation_cell, d_tot, roughness_cell, cell_area, time_step, h_0_cell,
+ *     Resolution, I_tot_end_cell, outlet_index, outlet_type, slope_outlet,
+ *     row_outlet, col_outlet, idx_nan, flag_critical, d_tolerance)
+ * replaces CA_Routing_8D.m:1-188 (called at HydroPol2D_Main_While.m:446-450).
+ * I_tot_end_cell_prev in [m3], the new I_tot_end_cell out (may alias);
+ * h_0 [mm] may be NULL (zeros); d_tolerance [mm].  Outputs: outflow (nine
+ * pages [mm/h]), d_t [mm] (outflow taken, inflow not yet added). */
+hp2d_status hp2d_ca_routing_8d(hp2d_handle *h, const double *z, const double *d_tot,
+                               const double *roughness, double cell_area, double time_step_min,
+                               const double *h_0, double resolution,
+                               const double *I_tot_end_cell_prev, int32_t outlet_type,
+                               double slope_outlet, const int32_t *row_outlet,
+                               const int32_t *col_outlet, int32_t n_outlet, int32_t flag_critical,
+                               double d_tolerance, double *outflow, double *d_t,
+                               double *I_tot_end_cell);
+
+/* The caller's gather of the routing outflows into inflows and the depth
+ * update of the cellular-automata branch, HydroPol2D_Main_While.m:631-660:
+ * qin_t from the four (n_pages = 5: left, right, up, down, outlet) or eight
+ * (n_pages = 9) neighbour pages of `outflow` [mm/h], summed in the order of
+ * :634-645; d_t [mm] (in/out, NULL skips) gets qin_t * time_step / 60.
+ * qin_t [mm/h] may be NULL. */
+hp2d_status hp2d_inflow_gather(hp2d_handle *h, const double *outflow, int32_t n_pages,
+                               double time_step_min, double *qin_t, double *d_t);
+
+/* Device time [ms] of the compute kernels of the last hp2d_bates_inertial_8d,
+ * hp2d_ca_routing_8d or hp2d_inflow_gather call on the handle, without its
+ * host<->device copies (HIP events on the handle's stream).  No counterpart in
+ * the reference: for callers that separate the kernel from the PCIe cost.
+ * HP2D_E_STATE before the first such call. */
+hp2d_status hp2d_last_kernel_time(hp2d_handle *h, double *ms);
+
+/* [B_t,P_conc,Out_Conc,tmin_wq,tot_W_out,mass_lost,Tot_Washed] =
+ *   build_up_wash_off_8D(C_3,C_4,qout_left_t,qout_right_t,qout_up_t,qout_down_t,
+ *     outlet_flow,qout_ne_t,qout_se_t,qout_sw_t,qout_nw_t,B_t,time_step,nx,ny,
+ *     cell_area,outlet_index,idx_nan_5,flag_wq_model,mass_lost,Tot_Washed,
+ *     Bmin,Bmax,min_Bt)
+ * replaces build_up_wash_off_8D.m:1-143 (called at HydroPol2D_Main_While.m:687).
+ * As hp2d_build_up_wash_off, with the nine flow pages in `outflow`; B_t_out
+ * may alias B_t and Tot_Washed_out may alias Tot_Washed. */
+hp2d_status hp2d_build_up_wash_off_8d(hp2d_handle *h, const double *C_3, const double *C_4,
+                                      const double *outflow, const double *B_t,
+                                      double time_step_min, double cell_area,
+                                      const uint8_t *nan_mask, int32_t flag_wq_model,
+                                      const double *Tot_Washed, double Bmin, double Bmax,
+                                      double min_Bt, double *B_t_out, double *P_conc,
+                                      double *Out_Conc, double *tmin_wq, double *tot_W_out,
+                                      double *mass_lost, double *Tot_Washed_out);
 
 /* [h_t,u_x,u_y,q_exf,q_river,error] = Boussinesq_2D_explicit(dt,dx,dy,h,z0,Sy,R,K,
  *     river_mask,K_river,h_river,z_river,Courant,h_soil,catchment_mask,

@@ -144,6 +144,31 @@ int32_t oracle_washoff(int32_t ny, int32_t nx, const double *C3, const double *C
                        double *Tot_Washed, double Bmin, double Bmax, double min_Bt,
                        double *P_conc, double *Out_Conc, double *tmin_wq, double *tot_W_out);
 
+/* D8 routines (hp2d_d8.c): 9-page arrays in the reference page order
+ * left, right, up, down, outlet, NE, SE, SW, NW; outflow pages in mm/h. */
+int32_t oracle_bates_8d(int32_t ny, int32_t nx, const double *z, const double *d_tot,
+                        const double *d_p, const double *rough, double cell_area,
+                        double time_step_min, double R, int32_t outlet_type,
+                        double slope_outlet, const int64_t *outlet_idx, int32_t n_outlet,
+                        const double *outflow_prev, int32_t flag_critical, double *outflow,
+                        double *d_t, double *I_tot_end_cell, double *Hf_out);
+int32_t oracle_ca_8d(int32_t ny, int32_t nx, const double *z, const double *d_tot,
+                     const double *rough, double cell_area, double time_step_min,
+                     const double *h_0, double R, const double *I_prev_in, int32_t outlet_type,
+                     double slope_outlet, const int64_t *outlet_idx, int32_t n_outlet,
+                     int32_t flag_critical, double d_tolerance, double *outflow, double *d_t,
+                     double *I_tot_end_cell);
+/* outflows become inflows (n_pages 5: four directions, 9: eight); d_t, when
+ * not NULL, gets qin * time_step / 60 added */
+void oracle_gather(int32_t ny, int32_t nx, const double *outflow, int32_t n_pages,
+                   double time_step_min, double *qin, double *d_t);
+/* returns as oracle_washoff */
+int32_t oracle_washoff_8d(int32_t ny, int32_t nx, const double *C3, const double *C4,
+                          const double *q9, double *B_t, double time_step_min, double cell_area,
+                          const uint8_t *nan_mask, int32_t model, double *mass_lost,
+                          double *Tot_Washed, double Bmin, double Bmax, double min_Bt,
+                          double *P_conc, double *Out_Conc, double *tmin_wq, double *tot_W_out);
+
 int32_t oracle_boussinesq_2d(int32_t ny, int32_t nx, double dt, double dx, double dy,
                              const double *h, const double *z0, const double *Sy, const double *R,
                              const double *K, double courant, const double *h_soil,

@@ -111,6 +111,20 @@ def _load(path):
                                      C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.oracle_washoff.restype = C.c_int32
         _u8 = C.POINTER(C.c_uint8)
+        L.oracle_bates_8d.argtypes = ([C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, C.c_double,
+                                       C.c_double, C.c_double, C.c_int32, C.c_double, _i64p,
+                                       C.c_int32, _dp, C.c_int32, _dp, _dp, _dp, _dp])
+        L.oracle_bates_8d.restype = C.c_int32
+        L.oracle_ca_8d.argtypes = ([C.c_int32, C.c_int32, _dp, _dp, _dp, C.c_double, C.c_double,
+                                    _dp, C.c_double, _dp, C.c_int32, C.c_double, _i64p,
+                                    C.c_int32, C.c_int32, C.c_double, _dp, _dp, _dp])
+        L.oracle_ca_8d.restype = C.c_int32
+        L.oracle_gather.argtypes = [C.c_int32, C.c_int32, _dp, C.c_int32, C.c_double, _dp, _dp]
+        L.oracle_gather.restype = None
+        L.oracle_washoff_8d.argtypes = ([C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, C.c_double,
+                                         C.c_double, _u8, C.c_int32, _dp, _dp, C.c_double,
+                                         C.c_double, C.c_double, _dp, _dp, _dp, _dp])
+        L.oracle_washoff_8d.restype = C.c_int32
         L.oracle_boussinesq_2d.argtypes = ([C.c_int32, C.c_int32, C.c_double, C.c_double,
                                             C.c_double] + [_dp] * 5 + [C.c_double, _dp, _u8, _u8,
                                             _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)])
@@ -367,6 +381,86 @@ def washoff(C3, C4, qout_left, qout_right, qout_up, qout_down, outlet_flow, B_t,
         raise ValueError("washoff: tmin_wq gives more than 1e7 sub-steps")
     return {"B_t": B, "P_conc": pc, "Out_Conc": oc.value, "tmin_wq": tm.value, "tot_W_out": tot,
             "mass_lost": ml.value, "Tot_Washed": tw}
+
+
+D8_PAGES = ("qout_left", "qout_right", "qout_up", "qout_down", "outlet_flow",
+            "qout_ne", "qout_se", "qout_sw", "qout_nw")
+
+
+def _pages(a, ny, nx, k=9):
+    """(ny, nx, k) page stack in Fortran order (zeros for None)."""
+    if a is None:
+        return np.zeros((ny, nx, k), order="F")
+    return np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (ny, nx, k)))
+
+
+def bates_8d(z, d_tot, d_p, roughness, cell_area, time_step_min, resolution, outlet_type,
+             slope_outlet, row_outlet, col_outlet, outflow_prev=None, flag_critical=0):
+    """Bates_Inertial_8D on the oracle: dict of outflow (ny, nx, 9) [mm/h], d_t,
+    I_tot_end_cell and Hf (ny, nx, 9)."""
+    z = _f(z)
+    ny, nx = z.shape
+    oidx = outlet_linear(row_outlet, col_outlet, ny)
+    rough = np.asfortranarray(np.broadcast_to(np.asarray(roughness, dtype=np.float64), (ny, nx)))
+    qp = _pages(outflow_prev, ny, nx)
+    out = {"outflow": np.zeros((ny, nx, 9), order="F"), "d_t": np.zeros((ny, nx), order="F"),
+           "I_tot_end_cell": np.zeros((ny, nx), order="F"), "Hf": np.zeros((ny, nx, 9), order="F")}
+    lib().oracle_bates_8d(ny, nx, _p(z), _p(_f(d_tot)), _p(_f(d_p)), _p(rough), cell_area,
+                          time_step_min, resolution, outlet_type, slope_outlet,
+                          oidx.ctypes.data_as(_i64p), oidx.size, _p(qp), flag_critical,
+                          _p(out["outflow"]), _p(out["d_t"]), _p(out["I_tot_end_cell"]),
+                          _p(out["Hf"]))
+    return out
+
+
+def ca_8d(z, d_tot, roughness, cell_area, time_step_min, h_0, resolution, I_tot_end_cell_prev,
+          outlet_type, slope_outlet, row_outlet, col_outlet, flag_critical=0, d_tolerance=1.0):
+    """CA_Routing_8D on the oracle: dict of outflow (ny, nx, 9) [mm/h], d_t and
+    I_tot_end_cell."""
+    z = _f(z)
+    ny, nx = z.shape
+    oidx = outlet_linear(row_outlet, col_outlet, ny)
+    sh = lambda a: np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (ny, nx)))
+    h0 = sh(h_0) if h_0 is not None else None
+    out = {"outflow": np.zeros((ny, nx, 9), order="F"), "d_t": np.zeros((ny, nx), order="F"),
+           "I_tot_end_cell": np.zeros((ny, nx), order="F")}
+    lib().oracle_ca_8d(ny, nx, _p(z), _p(_f(d_tot)), _p(sh(roughness)), cell_area, time_step_min,
+                       _p(h0), resolution, _p(sh(I_tot_end_cell_prev)), outlet_type, slope_outlet,
+                       oidx.ctypes.data_as(_i64p), oidx.size, flag_critical, d_tolerance,
+                       _p(out["outflow"]), _p(out["d_t"]), _p(out["I_tot_end_cell"]))
+    return out
+
+
+def gather(outflow, time_step_min=0.0, d_t=None):
+    """The caller's inflow gather (n_pages 5 or 9); returns (qin, d_t or None)."""
+    q = np.asfortranarray(np.asarray(outflow, dtype=np.float64))
+    ny, nx, k = q.shape
+    qin = np.zeros((ny, nx), order="F")
+    dt = None if d_t is None else _f(d_t).copy(order="F")
+    lib().oracle_gather(ny, nx, _p(q), k, time_step_min, _p(qin), _p(dt))
+    return qin, dt
+
+
+def washoff_8d(C3, C4, q9, B_t, time_step_min, cell_area, nan_mask=None, flag_wq_model=0,
+               mass_lost=0.0, Tot_Washed=None, Bmin=0.0, Bmax=0.0, min_Bt=0.0):
+    """build_up_wash_off_8D on the oracle (q9: (ny, nx, 9) pages [mm/h]); returns a
+    dict as washoff does."""
+    B = _f(B_t).copy(order="F")
+    ny, nx = B.shape
+    sh = lambda a: np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (ny, nx)))
+    q = _pages(q9, ny, nx)
+    tw = sh(0.0 if Tot_Washed is None else Tot_Washed).copy(order="F")
+    pc, tot = np.zeros((ny, nx), order="F"), np.zeros((ny, nx), order="F")
+    ml, oc, tm = C.c_double(mass_lost), C.c_double(), C.c_double()
+    nm = None if nan_mask is None else np.asfortranarray(np.asarray(nan_mask, dtype=np.uint8))
+    nmp = nm.ctypes.data_as(C.POINTER(C.c_uint8)) if nm is not None else C.POINTER(C.c_uint8)()
+    rc = lib().oracle_washoff_8d(ny, nx, _p(sh(C3)), _p(sh(C4)), _p(q), _p(B), time_step_min,
+                                 cell_area, nmp, flag_wq_model, C.byref(ml), _p(tw), Bmin, Bmax,
+                                 min_Bt, _p(pc), C.byref(oc), C.byref(tm), _p(tot))
+    if rc < 0:
+        raise ValueError("washoff_8d: tmin_wq gives more than 1e7 sub-steps")
+    return {"B_t": B, "P_conc": pc, "Out_Conc": oc.value, "tmin_wq": tm.value, "tot_W_out": tot,
+            "mass_lost": ml.value, "Tot_Washed": tw, "split": rc}
 
 
 def _mask(m, shape):

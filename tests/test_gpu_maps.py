@@ -94,6 +94,6 @@ def test_map_stacks_and_time_series_written_as_mat(tmp_path):
     H = loadmat(paths[0])["Maps"]["Hydro"][0, 0]
     np.testing.assert_array_equal(H["d"][0, 0], rec["d"][:, :, :12].astype(np.float32))
     np.testing.assert_array_equal(H["risk"][0, 0], rec["risk"][:, :, :12].astype(np.float32))
-    T = loadmat(ts)["TempTS"][0, 0]
-    q = T["outlet"][0, 0]["hydrograph_m3_s"][0, 0].ravel()
+    T = loadmat(ts)["TempTS"][0, 0]          # the struct's (1, 1) element
+    q = T["outlet"][0, 0]["hydrograph_m3_s"].ravel()
     np.testing.assert_array_equal(q, hy["hydro_q"][:hy["n_hydro"]])
