@@ -16,8 +16,8 @@ Prints one JSON line (value = cell-updates/s of the whole job), with the
 roofline of the dominant kernel, that of the whole step, and the FP64 CPU
 oracle on one thread and on all host cores.  Beside it: the batched-ensemble
 study (members/hour against the plain per-member loop, --ensemble-members)
-and the surrogate study (members/hour of the surrogate with its 5 % solver
-judge, --surrogate-train).
+the surrogate study (members/hour of the surrogate with its 5 % solver
+judge, --surrogate-train) and the D8 drop-in kernels (--d8-size, --d8-only).
 """
 from __future__ import annotations
 
@@ -62,6 +62,10 @@ def parse():
                    help="the surrogate study: train on N solved 256 x 256 members, judge 256 "
                         "more (0 skips it; about 1 min at N = 1536, 2.5 min at N = 4096)")
     p.add_argument("--surrogate-epochs", type=int, default=40)
+    p.add_argument("--d8-size", type=int, default=4096, metavar="N",
+                   help="the D8 drop-in study on an N x N grid (0 skips it)")
+    p.add_argument("--d8-only", action="store_true",
+                   help="run only the D8 drop-in study and print its JSON line")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run 5 device copies of MIB MiB (the known-size kernel that "
                         "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -212,6 +216,61 @@ def surrogate_study(n_train, epochs, n_eval=256, n_plain=8):
     }
 
 
+# algorithmic HBM bytes per cell of the D8 drop-in kernels (FP64): the inputs
+# read once, the outputs written once
+D8_BYTES = {
+    # z, d_tot, d_p, roughness, 9 previous pages + outlet byte | 9 pages, 9 Hf, d_t, I_tot
+    "bates8d_kernel": (13 * 8 + 1, 20 * 8),
+    # z, d_tot, roughness, h_0, I_prev + outlet byte | 9 pages, d_t, I_tot
+    "ca8d_kernel": (5 * 8 + 1, 11 * 8),
+    # 9 pages, d_t | qin, d_t
+    "gather_kernel": (10 * 8, 2 * 8),
+}
+
+
+def d8_study(n, calls=4):
+    """Kernel time of the D8 drop-ins on an n x n grid (HIP events around the
+    kernels on the handle's stream, host<->device copies excluded), as HBM
+    throughput of their algorithmic bytes."""
+    import ctypes as C
+
+    from hydropol2d_amd import _lib
+    from hydropol2d_amd import local_inertial as li
+    rng = np.random.default_rng(3)
+    z = np.asfortranarray(rng.random((n, n)) * 0.5 + np.linspace(0.0, 20.0, n)[None, :])
+    d = np.asfortranarray(rng.random((n, n)) * 200.0)
+    res = (None,) * 12
+    L = _lib.load()
+    h = li._Handle.get(n, n, 0)
+    t = C.c_double()
+    out = {}
+
+    def timed(fn):
+        ms = []
+        for _ in range(calls):
+            r = fn()
+            _lib.check(L.hp2d_last_kernel_time(h, C.byref(t)), handle=h)
+            ms.append(t.value)
+        return r, float(np.median(ms[1:]))
+
+    g, ms_b = timed(lambda: li.Bates_Inertial_8D(*res, 0, z, d, d, 0.03, 4.0, 0.002, 2.0, None, 2,
+                                                   0.0, [n], [n], 1.0, None, None, 1))
+    q = g[11]
+    del g
+    _, ms_c = timed(lambda: li.CA_Routing_8D(*res, 0, z, d, 0.03, 4.0, 0.002, None, 2.0,
+                                             np.zeros((n, n)), None, 2, 0.0, [n], [n], None, 1,
+                                             1.0))
+    _, ms_g = timed(lambda: li.outflows_to_inflows(q, 0.002, d))
+    for name, ms in (("bates8d_kernel", ms_b), ("ca8d_kernel", ms_c), ("gather_kernel", ms_g)):
+        rd, wr = D8_BYTES[name]
+        gbs = (rd + wr) * n * n / (ms / 1000.0) / 1e9
+        out[name] = {"avg_kernel_ms": ms, "bytes_per_cell": rd + wr, "achieved": gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "cell_updates_per_s": n * n / (ms / 1000.0)}
+        print(f"[bench] d8 {name}: {ms:.4f} ms, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    return {"grid": [n, n], "dtype": "f64", "calls": calls, "kernels": out}
+
+
 def step_bytes_per_cell(spec, elem, li_bytes):
     """Algorithmic bytes per cell of one whole step of the full-physics loop
     (FP32 state; the groundwater module keeps its head and ledgers in FP64),
@@ -278,6 +337,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "decomposition is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.d8_only:
+        print(json.dumps({"d8": d8_study(args.d8_size or 4096)}), flush=True)
+        return
     from hydropol2d_amd import synthetic
     from hydropol2d_amd.simulation import Simulation
 
@@ -364,6 +426,7 @@ def main():
            if args.ensemble_members > 0 else None)
     sur = (surrogate_study(args.surrogate_train, args.surrogate_epochs)
            if args.surrogate_train > 0 else None)
+    d8 = d8_study(args.d8_size) if args.d8_size > 0 else None
 
     line = {
         "metric": "cell-updates/s (% HBM roofline)",
@@ -401,6 +464,7 @@ def main():
         "cpu_baseline_1thread": cpu1,
         "ensemble": ens,
         "surrogate": sur,
+        "d8": d8,
     }
     print(json.dumps(line), flush=True)
 

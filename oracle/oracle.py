@@ -17,6 +17,9 @@ from hydropol2d_amd.runspec import RunSpec
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _SO = os.path.join(_HERE, "_build", "libhp2d_oracle.so")
 _SO_OMP = os.path.join(_HERE, "_build", "libhp2d_oracle_omp.so")
+# HP2D_ORACLE_SO names another build of the same sources (the sanitizer build
+# of the memory-safety test); it replaces the single-threaded library
+_SO_OVERRIDE = os.environ.get("HP2D_ORACLE_SO")
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
@@ -71,7 +74,7 @@ class RunOut(C.Structure):
 def lib():
     if _mode not in _libs:
         build()
-        _libs[_mode] = _load(_SO_OMP if _mode == "omp" else _SO)
+        _libs[_mode] = _load(_SO_OMP if _mode == "omp" else (_SO_OVERRIDE or _SO))
     return _libs[_mode]
 
 
