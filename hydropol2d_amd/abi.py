@@ -157,3 +157,22 @@ class RunReport(C.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+SUBGRID_FACE_FIELDS = ("zmin", "zmax", "havg", "nrep", "pwet", "navg", "ffit")
+SUBGRID_SIDES = ("north", "south", "west", "east")
+
+
+class SubgridFaces(C.Structure):
+    """hp2d_subgrid_faces: one set of velocity-point tables."""
+
+    _fields_ = [(k, _dp) for k in SUBGRID_FACE_FIELDS]
+
+
+class SubgridTablesC(C.Structure):
+    """hp2d_subgrid_tables: the SFINCS-style SubgridTables of a coarse grid."""
+
+    _fields_ = ([("nr_levels", C.c_int32), ("cell_area", C.c_double)]
+                + [(k, _dp) for k in ("z_zmin", "z_zmax", "z_volmax", "z_level", "z_volume",
+                                      "z_area")]
+                + [("u", SubgridFaces), ("v", SubgridFaces), ("side", SubgridFaces * 4)])

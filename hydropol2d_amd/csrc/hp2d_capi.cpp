@@ -96,6 +96,7 @@ struct hp2d_handle {
     size_t ws2_bytes = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};  // around the compute kernels of a timed drop-in
     bool ev_valid = false;
+    HandleSlot slot[kHandleSlots];          // per-module resident state (subgrid tables)
 };
 
 static void set_err(hp2d_handle *h, const std::string &m) {
@@ -173,6 +174,8 @@ void hp2d_destroy(hp2d_handle *h) {
     if (h->ws2) hipFree(h->ws2);
     for (auto &e : h->ev)
         if (e) hipEventDestroy(e);
+    for (auto &sl : h->slot)
+        if (sl.p && sl.del) sl.del(sl.p);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
 }
@@ -203,6 +206,10 @@ hp2d_status hp2d_handle_shape(hp2d_handle *h, int32_t *ny, int32_t *nx, int32_t 
 }
 
 hipStream_t hp2d_handle_stream(hp2d_handle *h) { return h->stream; }
+
+HandleSlot *hp2d_handle_slot(hp2d_handle *h, int which) {
+    return (which >= 0 && which < kHandleSlots) ? &h->slot[which] : nullptr;
+}
 
 hipError_t hp2d_handle_mark(hp2d_handle *h, int which) {
     if (!h->ev[which]) {

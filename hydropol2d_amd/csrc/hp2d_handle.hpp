@@ -7,6 +7,15 @@
 
 #include "hp2d.h"
 
+// per-module state a handle keeps until hp2d_destroy (which calls `del`)
+struct HandleSlot {
+    void *p = nullptr;
+    void (*del)(void *) = nullptr;
+};
+constexpr int kHandleSlots = 2;
+constexpr int kSlotSubgrid = 0;
+HandleSlot *hp2d_handle_slot(hp2d_handle *h, int which);
+
 // grid shape and precision of a handle
 hp2d_status hp2d_handle_shape(hp2d_handle *h, int32_t *ny, int32_t *nx, int32_t *precision);
 // the handle's HIP stream

@@ -12,6 +12,8 @@
  *   Bates_Inertial_8D          HydroPol2D_Functions/Bates_Inertial_8D.m:1-274
  *   CA_Routing_8D              HydroPol2D_Functions/CA_Routing_8D.m:1-188
  *   build_up_wash_off_8D       HydroPol2D_Functions/build_up_wash_off_8D.m:1-143
+ *   Local_Inertial_Model_D4_Subgrid  HydroPol2D_Functions/Local_Inertial_Model_D4_Subgrid.m:1-317
+ *   Subgrid_Properties_Lookup  HydroPol2D_Functions/Subgrid_Properties_Lookup.m:1-510
  *   Boussinesq_2D_explicit     HydroPol2D_Functions/Boussinesq_2D_explicit.m:1-195
  *   refreshing_timestep        HydroPol2D_Functions/refreshing_timestep.m:17-368
  *   the Main_While step loop   HydroPol2D_Functions/HydroPol2D_Main_While.m:403-994
@@ -262,6 +264,94 @@ hp2d_status hp2d_build_up_wash_off_8d(hp2d_handle *h, const double *C_3, const d
                                       double min_Bt, double *B_t_out, double *P_conc,
                                       double *Out_Conc, double *tmin_wq, double *tot_W_out,
                                       double *mass_lost, double *Tot_Washed_out);
+
+/* ------------------------------------------------------------------------ */
+/* Lookup-table subgrid local inertial (flags.flag_subgrid = 1,              */
+/* HydroPol2D_Main_While.m:564-582) and its SFINCS-style tables.             */
+/* ------------------------------------------------------------------------ */
+
+/* One set of velocity-point tables (Subgrid_Properties_Lookup.m:296-345):
+ * zmin, zmax, navg, ffit: one value per face; havg, nrep, pwet: nr_levels
+ * pages of one value per face (level k at offset k * n_faces). */
+typedef struct hp2d_subgrid_faces {
+    double *zmin, *zmax, *havg, *nrep, *pwet, *navg, *ffit;
+} hp2d_subgrid_faces;
+
+/* SubgridTables (Subgrid_Properties_Lookup.m:56-100) of a coarse ny x nx grid,
+ * column-major as MATLAB holds the struct's fields:
+ *   z_zmin, z_zmax, z_volmax          ny*nx
+ *   z_level, z_volume, z_area         ny*nx*nr_levels
+ *   u: faces between columns j, j+1   ny*max(nx-1,1)   [*nr_levels]
+ *   v: faces between rows i, i+1      max(ny-1,1)*nx   [*nr_levels]
+ *   side[0..3]: north, south, west, east boundary faces of each cell
+ *                                     ny*nx            [*nr_levels]
+ * cell_area = coarse_res^2.  The caller owns the arrays. */
+typedef struct hp2d_subgrid_tables {
+    int32_t nr_levels;
+    double cell_area;
+    double *z_zmin, *z_zmax, *z_volmax, *z_level, *z_volume, *z_area;
+    hp2d_subgrid_faces u, v, side[4];
+} hp2d_subgrid_tables;
+
+/* [SubgridTables, invert_el] = Subgrid_Properties_Lookup(DEM_raster,
+ *     Roughness_raster, Reference_raster, coarse_res, 'nr_levels', nr_levels)
+ * replaces Subgrid_Properties_Lookup.m:1-510 (called at
+ * HydroPol2D_preprocessing.m:3174, 3217).  dem: the fine DEM, fny x fnx,
+ * cell size `cellsize` [m]; roughness: Manning n on the fine grid
+ * (rough_is_fine = 1) or on the coarse grid (0).  The coarse grid is the
+ * handle's ny x nx at coarse_res [m]; coarse_res / cellsize must round to an
+ * even integer (HP2D_E_HIP with a message otherwise, where the reference
+ * errors).  Built on the GPU; the tables stay resident on the handle for
+ * hp2d_local_inertial_d4_subgrid, and are copied into `out` when it is not
+ * NULL (every array allocated by the caller with the sizes above;
+ * invert_el = out->z_zmin).  'dz', 'maxDepth', 'tol_rel' and 'verbose' do
+ * not change the tables and are not taken. */
+hp2d_status hp2d_subgrid_tables_build(hp2d_handle *h, const double *dem, int32_t fny,
+                                      int32_t fnx, double cellsize, const double *roughness,
+                                      int32_t rough_is_fine, double coarse_res,
+                                      int32_t nr_levels, hp2d_subgrid_tables *out);
+
+/* Upload SubgridTables (e.g. read back from a MAT-file) to the handle, where
+ * they stay resident for hp2d_local_inertial_d4_subgrid and
+ * hp2d_subgrid_apply_volume_change.  No counterpart in the reference (whose
+ * tables live in the MATLAB workspace): the device copy of the struct. */
+hp2d_status hp2d_subgrid_set_tables(hp2d_handle *h, const hp2d_subgrid_tables *tables);
+
+/* [qout_left,qout_right,qout_up,qout_down,outlet_flow,d_t,I_tot_end_cell,outflow,
+ *  Hf,Qc,Qf,Qci,Qfi,C_a,eta_t,V_t] = Local_Inertial_Model_D4_Subgrid(...,
+ *  flag_reservoir = 0, z, d_tot, ..., time_step, Resolution, outlet_index,
+ *  outlet_type, slope_outlet, row_outlet, col_outlet, ..., outflow, idx_nan, ...,
+ *  SubgridTables)
+ * replaces Local_Inertial_Model_D4_Subgrid.m:1-317 (called at
+ * HydroPol2D_Main_While.m:566-582) on the handle's resident tables.
+ * d_tot [mm]: depth above z_zmin; outflow_prev: NULL or n_prev_pages pages of
+ * ny*nx [mm/h] (pages 1-2: the x and y face fluxes the previous call
+ * returned); idx_nan: 0/1 bytes or NULL; slope_outlet: scalar.  Outputs
+ * (NULL skips): outflow [mm/h] 3 pages (x faces, y faces, outlet), d_t [mm],
+ * I_tot_end_cell [m3], Hf [m] 3 pages, C_a [m2], eta_t [m], V_t [m3];
+ * n_substeps: the accepted internal sub-steps.  The qout pages are the
+ * reference's views of `outflow` (qout_right = page 1, qout_up = page 2,
+ * qout_left / qout_down their negated shifts, outlet_flow = page 3), and
+ * Qc = Qf = Qci = Qfi = 0. */
+hp2d_status hp2d_local_inertial_d4_subgrid(hp2d_handle *h, const double *d_tot,
+                                           double time_step_min, double resolution,
+                                           int32_t outlet_type, double slope_outlet,
+                                           const int32_t *row_outlet, const int32_t *col_outlet,
+                                           int32_t n_outlet, const double *outflow_prev,
+                                           int32_t n_prev_pages, const uint8_t *idx_nan,
+                                           double *outflow, double *d_t, double *I_tot_end_cell,
+                                           double *Hf, double *C_a, double *eta_t, double *V_t,
+                                           int32_t *n_substeps);
+
+/* [drep_new,V_old,V_new,A_new,residual_m3] = hp2d_subgrid_apply_volume_change(
+ *     drep_old, delta_depth_mm, SubgridTables, cell_area)
+ * replaces hp2d_subgrid_apply_volume_change.m (SFINCS-exact branch) on the
+ * handle's resident tables: drep [m], delta_depth_mm [mm] per cell, cell_area
+ * [m2].  Outputs ny*nx (NULL skips all but drep_new). */
+hp2d_status hp2d_subgrid_apply_volume_change(hp2d_handle *h, const double *drep_old,
+                                             const double *delta_depth_mm, double cell_area,
+                                             double *drep_new, double *V_old, double *V_new,
+                                             double *A_new, double *residual_m3);
 
 /* [h_t,u_x,u_y,q_exf,q_river,error] = Boussinesq_2D_explicit(dt,dx,dy,h,z0,Sy,R,K,
  *     river_mask,K_river,h_river,z_river,Courant,h_soil,catchment_mask,

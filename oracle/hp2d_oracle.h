@@ -169,6 +169,27 @@ int32_t oracle_washoff_8d(int32_t ny, int32_t nx, const double *C3, const double
                           double *Tot_Washed, double Bmin, double Bmax, double min_Bt,
                           double *P_conc, double *Out_Conc, double *tmin_wq, double *tot_W_out);
 
+/* lookup-table subgrid (hp2d_subgrid.c) */
+int32_t oracle_subgrid_build(const double *dem, int32_t fny, int32_t fnx, double cellsize,
+                             const double *rough, int32_t rough_is_fine, double coarse_res,
+                             int32_t ny, int32_t nx, int32_t nl, hp2d_subgrid_tables *T);
+void oracle_subgrid_queries(const hp2d_subgrid_tables *T, int32_t ny, int32_t nx, const double *zs,
+                            double *V, double *zs_back, double *A);
+void oracle_subgrid_face_state(const hp2d_subgrid_faces *F, int64_t nf, int32_t nl,
+                               const double *zu, double *HG, double *nr, double *phi);
+void oracle_subgrid_volume(const hp2d_subgrid_tables *T, int32_t ny, int32_t nx,
+                           const double *drep_old, const double *delta_mm, double cell_area,
+                           double *drep_new, double *V_old, double *V_new, double *A_new,
+                           double *residual);
+int32_t oracle_li_subgrid(int32_t ny, int32_t nx, const hp2d_subgrid_tables *T,
+                          const double *d_tot, double time_step_min, double R,
+                          int32_t outlet_type, double slope_outlet, const int64_t *outlet_idx,
+                          const int32_t *outlet_side, int32_t n_outlet,
+                          const double *outflow_prev, int32_t n_prev_pages,
+                          const uint8_t *idx_nan, double *outflow, double *d_t,
+                          double *I_tot_end_cell, double *Hf_out, double *C_a_out,
+                          double *eta_out, double *V_out, int32_t *n_sub);
+
 int32_t oracle_boussinesq_2d(int32_t ny, int32_t nx, double dt, double dx, double dy,
                              const double *h, const double *z0, const double *Sy, const double *R,
                              const double *K, double courant, const double *h_soil,

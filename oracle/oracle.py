@@ -114,6 +114,23 @@ def _load(path):
                                      C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.oracle_washoff.restype = C.c_int32
         _u8 = C.POINTER(C.c_uint8)
+        _tp = C.POINTER(abi.SubgridTablesC)
+        L.oracle_subgrid_build.argtypes = [_dp, C.c_int32, C.c_int32, C.c_double, _dp, C.c_int32,
+                                           C.c_double, C.c_int32, C.c_int32, C.c_int32, _tp]
+        L.oracle_subgrid_build.restype = C.c_int32
+        L.oracle_subgrid_queries.argtypes = [_tp, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp]
+        L.oracle_subgrid_queries.restype = None
+        L.oracle_subgrid_face_state.argtypes = [C.POINTER(abi.SubgridFaces), C.c_int64, C.c_int32,
+                                                _dp, _dp, _dp, _dp]
+        L.oracle_subgrid_face_state.restype = None
+        L.oracle_subgrid_volume.argtypes = [_tp, C.c_int32, C.c_int32, _dp, _dp, C.c_double,
+                                            _dp, _dp, _dp, _dp, _dp]
+        L.oracle_subgrid_volume.restype = None
+        L.oracle_li_subgrid.argtypes = ([C.c_int32, C.c_int32, _tp, _dp, C.c_double, C.c_double,
+                                         C.c_int32, C.c_double, _i64p, C.POINTER(C.c_int32),
+                                         C.c_int32, _dp, C.c_int32, _u8] + [_dp] * 7
+                                        + [C.POINTER(C.c_int32)])
+        L.oracle_li_subgrid.restype = C.c_int32
         L.oracle_bates_8d.argtypes = ([C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, C.c_double,
                                        C.c_double, C.c_double, C.c_int32, C.c_double, _i64p,
                                        C.c_int32, _dp, C.c_int32, _dp, _dp, _dp, _dp])
@@ -464,6 +481,86 @@ def washoff_8d(C3, C4, q9, B_t, time_step_min, cell_area, nan_mask=None, flag_wq
         raise ValueError("washoff_8d: tmin_wq gives more than 1e7 sub-steps")
     return {"B_t": B, "P_conc": pc, "Out_Conc": oc.value, "tmin_wq": tm.value, "tot_W_out": tot,
             "mass_lost": ml.value, "Tot_Washed": tw, "split": rc}
+
+
+def subgrid_build(dem, cellsize, rough, coarse_res, ny, nx, nr_levels=20):
+    """Subgrid_Properties_Lookup on the oracle: a SubgridTables of the coarse
+    ny x nx grid (rough: fine- or coarse-shaped Manning n)."""
+    from hydropol2d_amd.subgrid import SubgridTables
+    dem = _f(dem)
+    rough = _f(rough)
+    fine = 1 if rough.shape == dem.shape else 0
+    T = SubgridTables(ny, nx, nr_levels, coarse_res ** 2)
+    t = T.c_struct()
+    rc = lib().oracle_subgrid_build(_p(dem), dem.shape[0], dem.shape[1], cellsize, _p(rough), fine,
+                                    coarse_res, ny, nx, nr_levels, C.byref(t))
+    if rc != 0:
+        raise ValueError(f"subgrid_build: error {rc}")
+    return T
+
+
+def subgrid_queries(T, zs):
+    """(V from zs, zs back from that V, wet area from zs) of every cell."""
+    zs = _f(zs)
+    V, zb, A = (np.zeros((T.ny, T.nx), order="F") for _ in range(3))
+    t = T.c_struct()
+    lib().oracle_subgrid_queries(C.byref(t), T.ny, T.nx, _p(zs), _p(V), _p(zb), _p(A))
+    return V, zb, A
+
+
+def subgrid_face_state(T, prefix, zu):
+    """(HG, nrep, phi) of a face set ('u', 'v', 'north', ...) at levels zu
+    (shaped as the set's faces)."""
+    zu = _f(zu)
+    t = T.c_struct()
+    F = getattr(t, prefix) if prefix in ("u", "v") else t.side[abi.SUBGRID_SIDES.index(prefix)]
+    HG, nr, phi = (np.zeros(zu.shape, order="F") for _ in range(3))
+    lib().oracle_subgrid_face_state(C.byref(F), zu.size, T.nr_levels, _p(zu), _p(HG), _p(nr),
+                                    _p(phi))
+    return HG, nr, phi
+
+
+def subgrid_volume(T, drep_old, delta_mm, cell_area):
+    sh = lambda a: np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (T.ny, T.nx)))
+    outs = [np.zeros((T.ny, T.nx), order="F") for _ in range(5)]
+    t = T.c_struct()
+    lib().oracle_subgrid_volume(C.byref(t), T.ny, T.nx, _p(sh(drep_old)), _p(sh(delta_mm)),
+                                cell_area, *[_p(o) for o in outs])
+    return tuple(outs)
+
+
+def li_subgrid(T, d_tot, time_step_min, resolution, outlet_type=2, slope_outlet=0.0,
+               row_outlet=(), col_outlet=(), outflow_prev=None, idx_nan=None):
+    """Local_Inertial_Model_D4_Subgrid on the oracle: dict of outflow (3 pages),
+    d_t, I_tot_end_cell, Hf (3 pages), C_a, eta_t, V_t, n_substeps."""
+    from hydropol2d_amd.subgrid import outlet_sides
+    ny, nx = T.ny, T.nx
+    d = np.asfortranarray(np.broadcast_to(np.asarray(d_tot, dtype=np.float64), (ny, nx)))
+    oidx = outlet_linear(row_outlet, col_outlet, ny)
+    sides = outlet_sides(row_outlet, col_outlet, ny, nx)
+    prev, npg = None, 0
+    if outflow_prev is not None:
+        prev = _f(outflow_prev)
+        if prev.ndim == 2:
+            prev = prev.reshape(ny, nx, 1, order="F")
+        npg = prev.shape[2]
+    m = None if idx_nan is None else np.asfortranarray(np.asarray(idx_nan, dtype=np.uint8))
+    out = {"outflow": np.zeros((ny, nx, 3), order="F"), "Hf": np.zeros((ny, nx, 3), order="F")}
+    for k in ("d_t", "I_tot_end_cell", "C_a", "eta_t", "V_t"):
+        out[k] = np.zeros((ny, nx), order="F")
+    ns = C.c_int32()
+    t = T.c_struct()
+    rc = lib().oracle_li_subgrid(ny, nx, C.byref(t), _p(d), time_step_min, resolution, outlet_type,
+                                 slope_outlet, oidx.ctypes.data_as(_i64p),
+                                 sides.ctypes.data_as(C.POINTER(C.c_int32)), oidx.size, _p(prev),
+                                 npg, m.ctypes.data_as(C.POINTER(C.c_uint8)) if m is not None
+                                 else C.POINTER(C.c_uint8)(), _p(out["outflow"]), _p(out["d_t"]),
+                                 _p(out["I_tot_end_cell"]), _p(out["Hf"]), _p(out["C_a"]),
+                                 _p(out["eta_t"]), _p(out["V_t"]), C.byref(ns))
+    if rc != 0:
+        raise ValueError("li_subgrid: sub-step loop did not end")
+    out["n_substeps"] = ns.value
+    return out
 
 
 def _mask(m, shape):
