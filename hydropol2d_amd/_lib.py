@@ -61,6 +61,10 @@ SIGNATURES = {
     "hp2d_subgrid_tables_build": (C.c_int32, [_vp, _dp, C.c_int32, C.c_int32, C.c_double, _dp,
                                               C.c_int32, C.c_double, C.c_int32, _vp]),
     "hp2d_subgrid_set_tables": (C.c_int32, [_vp, _vp]),
+    "hp2d_full_momentum_d4": (C.c_int32, [_vp, _dp, _dp, _dp, _dp, C.c_double, _dp, C.c_double,
+                                          C.c_double, _u8p, C.c_int32, C.c_double, _i32p, _i32p,
+                                          C.c_int32, C.c_double, _dp, C.c_int32, _u8p, C.c_int32,
+                                          _u8p, _u8p, _u8p, _u8p] + [_dp] * 5),
     "hp2d_local_inertial_d4_subgrid": (C.c_int32, [_vp, _dp, C.c_double, C.c_double, C.c_int32,
                                                    C.c_double, _i32p, _i32p, C.c_int32, _dp,
                                                    C.c_int32, _u8p] + [_dp] * 7 + [_i32p]),

@@ -262,3 +262,79 @@ def outflows_to_inflows(outflow, time_step=0.0, d_t=None):
                                   dt_new.ctypes.data_as(_dp) if dt_new is not None else _dp())
     _lib.check(status, handle=h)
     return qin, dt_new
+
+
+def _mask_u8(a, ny, nx):
+    if a is None or np.size(a) == 0:
+        return None
+    m = np.asarray(a)
+    if m.size == 1:
+        m = np.broadcast_to(m, (ny, nx))
+    return np.asfortranarray(m.reshape(ny, nx, order="F").astype(bool).astype(np.uint8))
+
+
+def Full_Momentum_Model_D4(flag_numerical_scheme, reservoir_x, reservoir_y, k1, h1, k2, k3, h2, k4,
+                           yds1, xds1, yds2, xds2, flag_reservoir, z, d_tot, d_p, roughness_cell,
+                           roughness_squared, cell_area, time_step, Resolution, outlet_index,
+                           outlet_type, slope_outlet, row_outlet, col_outlet, d_tolerance, outflow,
+                           idx_nan, flag_critical, flag_subgrid, nc, nf, River_Width, River_Depth,
+                           Qc_prev, Qf_prev, Qci_prev, Qfi_prev, C_a_prev, Subgrid_Properties,
+                           flag_overbanks, flag_inflow, SubgridTables, Outlet_Properties=None):
+    """Full_Momentum_Model_D4.m:1 on the gfx950 kernels (ordinary grid,
+    flag_reservoir = 0).
+
+    Returns (qout_left, qout_right, qout_up, qout_down, outlet_flow, d_t,
+    I_tot_end_cell, outflow, Hf, Qc, Qf, Qci, Qfi, C_a).  Outlet_Properties is
+    None or a dict / object with flag_HR_full_momentum and the face_right,
+    face_left, face_up, face_down masks.
+    """
+    if flag_reservoir == 1:
+        raise _lib.HP2DError(abi.HP2D_E_UNSUPPORTED, "reservoir branch is outside this build")
+    if flag_subgrid == 1 and flag_overbanks != 1 and SubgridTables is not None:
+        raise _lib.HP2DError(abi.HP2D_E_UNSUPPORTED,
+                             "the lookup-subgrid storage branch of the full-momentum model is "
+                             "outside this build")
+    z = np.asfortranarray(z, dtype=np.float64)
+    ny, nx = z.shape
+    g = (lambda a: None if a is None or np.size(a) == 0 else _grid(a, ny, nx))
+    ca = np.asarray(cell_area, dtype=np.float64)
+    ca_grid = None if ca.size == 1 else _grid(ca, ny, nx)
+    prev, n_prev = None, 0
+    if outflow is not None and np.size(outflow) > 0:
+        prev = np.asfortranarray(np.asarray(outflow, dtype=np.float64))
+        if prev.ndim == 3 and prev.shape[2] >= 2:   # MATLAB: a single page is a 2-D matrix
+            n_prev = prev.shape[2]
+        else:
+            prev = None
+
+    def prop(name):
+        if Outlet_Properties is None:
+            return None
+        if isinstance(Outlet_Properties, dict):
+            return Outlet_Properties.get(name)
+        return getattr(Outlet_Properties, name, None)
+    faces = [_mask_u8(prop(k), ny, nx) for k in ("face_right", "face_left", "face_up", "face_down")]
+    flag_hr = int(prop("flag_HR_full_momentum") or 0)
+    rows, cols, n_out = _outlets(row_outlet, col_outlet)
+    oi, nan = _mask_u8(outlet_index, ny, nx), _mask_u8(idx_nan, ny, nx)
+    out = np.zeros((ny, nx, 5), order="F")
+    Hf = np.zeros((ny, nx, 3), order="F")
+    d_t, I_tot, C_a = (np.zeros((ny, nx), order="F") for _ in range(3))
+    L = _lib.load()
+    h = _Handle.get(ny, nx, abi.HP2D_FP64)
+    p = (lambda a: a.ctypes.data_as(_dp) if a is not None else _dp())
+    u = (lambda a: a.ctypes.data_as(C.POINTER(C.c_uint8)) if a is not None
+         else C.POINTER(C.c_uint8)())
+    st = L.hp2d_full_momentum_d4(
+        h, p(z), p(_grid(d_tot, ny, nx)), p(_grid(roughness_cell, ny, nx)), p(g(roughness_squared)),
+        float(ca.ravel()[0]), p(ca_grid), float(time_step), float(Resolution), u(oi),
+        int(outlet_type), float(np.asarray(slope_outlet, dtype=np.float64).ravel()[0]),
+        rows.ctypes.data_as(_i32p), cols.ctypes.data_as(_i32p), n_out, float(d_tolerance),
+        p(prev), n_prev, u(nan), flag_hr, *[u(f) for f in faces], p(out), p(d_t), p(I_tot), p(Hf),
+        p(C_a))
+    _lib.check(st, handle=h)
+    ms = out[:, :, :2]
+    qout_left = np.asfortranarray(-np.concatenate([np.zeros((ny, 1)), ms[:, :-1, 0]], axis=1))
+    qout_down = np.asfortranarray(-np.concatenate([ms[1:, :, 1], np.zeros((1, nx))], axis=0))
+    return (qout_left, ms[:, :, 0].copy(order="F"), ms[:, :, 1].copy(order="F"), qout_down,
+            out[:, :, 2].copy(order="F"), d_t, I_tot, out, Hf, 0, 0, 0, 0, C_a)

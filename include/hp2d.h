@@ -13,6 +13,7 @@
  *   CA_Routing_8D              HydroPol2D_Functions/CA_Routing_8D.m:1-188
  *   build_up_wash_off_8D       HydroPol2D_Functions/build_up_wash_off_8D.m:1-143
  *   Local_Inertial_Model_D4_Subgrid  HydroPol2D_Functions/Local_Inertial_Model_D4_Subgrid.m:1-317
+ *   Full_Momentum_Model_D4     HydroPol2D_Functions/Full_Momentum_Model_D4.m:1-724
  *   Subgrid_Properties_Lookup  HydroPol2D_Functions/Subgrid_Properties_Lookup.m:1-510
  *   Boussinesq_2D_explicit     HydroPol2D_Functions/Boussinesq_2D_explicit.m:1-195
  *   refreshing_timestep        HydroPol2D_Functions/refreshing_timestep.m:17-368
@@ -264,6 +265,40 @@ hp2d_status hp2d_build_up_wash_off_8d(hp2d_handle *h, const double *C_3, const d
                                       double min_Bt, double *B_t_out, double *P_conc,
                                       double *Out_Conc, double *tmin_wq, double *tot_W_out,
                                       double *mass_lost, double *Tot_Washed_out);
+
+/* [qout_left,qout_right,qout_up,qout_down,outlet_flow,d_t,I_tot_end_cell,outflow,
+ *  Hf,Qc,Qf,Qci,Qfi,C_a] = Full_Momentum_Model_D4(..., flag_reservoir = 0, z, d_tot, d_p,
+ *  roughness_cell, roughness_squared, cell_area, time_step, Resolution, outlet_index,
+ *  outlet_type, slope_outlet, row_outlet, col_outlet, d_tolerance, outflow, idx_nan, ...,
+ *  flag_subgrid = 0, ..., Outlet_Properties)
+ * replaces Full_Momentum_Model_D4.m:1-724 on the ordinary grid (the lookup-subgrid
+ * storage branch and reservoirs are not taken).  Inputs ny*nx: z [m], d_tot [mm],
+ * roughness_cell; roughness_squared (NULL: roughness_cell^2); cell_area_grid (NULL:
+ * the scalar cell_area everywhere); outlet_index, idx_nan: 0/1 bytes or NULL;
+ * outflow_prev: NULL or n_prev_pages pages [mm/h, and m2/s in pages 4-5]: five or
+ * more pages carry the momentum memory (pages 4, 5), fewer the face fluxes of pages
+ * 1-2 (a MATLAB 2-D matrix is 0 pages); slope_outlet: scalar.  Outlet_Properties:
+ * flag_HR_full_momentum (hydrostatic reconstruction) and the face masks
+ * face_right, face_left, face_up, face_down (0/1 bytes or NULL); when a face mask
+ * marks an active cell the outlets are face-based, else the row/col cells.
+ * Outputs (NULL skips): outflow, 5 pages (x face, y face, outlet [mm/h]; hu, hv
+ * [m2/s]); d_t [mm]; I_tot_end_cell [m3]; Hf [m], 3 pages; C_a [m2].  The qout
+ * pages are the reference's views of outflow (qout_right = page 1, qout_up =
+ * page 2, qout_left / qout_down their negated shifts, outlet_flow = page 3), and
+ * Qc = Qf = Qci = Qfi = 0. */
+hp2d_status hp2d_full_momentum_d4(hp2d_handle *h, const double *z, const double *d_tot,
+                                  const double *roughness_cell, const double *roughness_squared,
+                                  double cell_area, const double *cell_area_grid,
+                                  double time_step_min, double resolution,
+                                  const uint8_t *outlet_index, int32_t outlet_type,
+                                  double slope_outlet, const int32_t *row_outlet,
+                                  const int32_t *col_outlet, int32_t n_outlet, double d_tolerance,
+                                  const double *outflow_prev, int32_t n_prev_pages,
+                                  const uint8_t *idx_nan, int32_t flag_HR_full_momentum,
+                                  const uint8_t *face_right, const uint8_t *face_left,
+                                  const uint8_t *face_up, const uint8_t *face_down,
+                                  double *outflow, double *d_t, double *I_tot_end_cell,
+                                  double *Hf, double *C_a);
 
 /* ------------------------------------------------------------------------ */
 /* Lookup-table subgrid local inertial (flags.flag_subgrid = 1,              */

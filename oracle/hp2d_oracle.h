@@ -190,6 +190,17 @@ int32_t oracle_li_subgrid(int32_t ny, int32_t nx, const hp2d_subgrid_tables *T,
                           double *I_tot_end_cell, double *Hf_out, double *C_a_out,
                           double *eta_out, double *V_out, int32_t *n_sub);
 
+/* Full_Momentum_Model_D4 (hp2d_fm.c): face_masks NULL or {right, left, up,
+ * down} (entries may be NULL); outflow 5 pages, Hf 3 pages */
+int32_t oracle_full_momentum(int32_t ny, int32_t nx, const double *z, const double *d_tot,
+                             const double *rough, const double *n2_in, const double *cell_area,
+                             double time_step_min, double R, const uint8_t *outlet_index,
+                             int32_t outlet_type, double slope_outlet, const int64_t *outlet_idx,
+                             int32_t n_outlet, double d_tolerance, const double *outflow_prev,
+                             int32_t n_prev_pages, const uint8_t *idx_nan, int32_t flag_HR,
+                             const uint8_t *const *face_masks, double *outflow, double *d_t,
+                             double *I_tot, double *Hf, double *C_a_out);
+
 int32_t oracle_boussinesq_2d(int32_t ny, int32_t nx, double dt, double dx, double dy,
                              const double *h, const double *z0, const double *Sy, const double *R,
                              const double *K, double courant, const double *h_soil,

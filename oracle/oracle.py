@@ -114,6 +114,11 @@ def _load(path):
                                      C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.oracle_washoff.restype = C.c_int32
         _u8 = C.POINTER(C.c_uint8)
+        L.oracle_full_momentum.argtypes = ([C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp,
+                                            C.c_double, C.c_double, _u8, C.c_int32, C.c_double,
+                                            _i64p, C.c_int32, C.c_double, _dp, C.c_int32, _u8,
+                                            C.c_int32, C.POINTER(_u8), _dp, _dp, _dp, _dp, _dp])
+        L.oracle_full_momentum.restype = C.c_int32
         _tp = C.POINTER(abi.SubgridTablesC)
         L.oracle_subgrid_build.argtypes = [_dp, C.c_int32, C.c_int32, C.c_double, _dp, C.c_int32,
                                            C.c_double, C.c_int32, C.c_int32, C.c_int32, _tp]
@@ -560,6 +565,56 @@ def li_subgrid(T, d_tot, time_step_min, resolution, outlet_type=2, slope_outlet=
     if rc != 0:
         raise ValueError("li_subgrid: sub-step loop did not end")
     out["n_substeps"] = ns.value
+    return out
+
+
+def unique_outlets(row_outlet, col_outlet, ny):
+    """Linear outlet indices without repeats, in first-seen order."""
+    idx = outlet_linear(row_outlet, col_outlet, ny)
+    _, first = np.unique(idx, return_index=True)
+    return np.ascontiguousarray(idx[np.sort(first)])
+
+
+def full_momentum(z, d_tot, roughness, cell_area, time_step_min, resolution, outlet_type=2,
+                  slope_outlet=0.0, row_outlet=(), col_outlet=(), outflow_prev=None,
+                  idx_nan=None, d_tolerance=1.0, roughness_squared=None, outlet_index=None,
+                  flag_HR=0, face_masks=None):
+    """Full_Momentum_Model_D4 on the oracle: dict of outflow (5 pages), d_t,
+    I_tot_end_cell, Hf (3 pages), C_a.  face_masks: None or (right, left, up,
+    down) cell masks (None entries allowed)."""
+    z = _f(z)
+    ny, nx = z.shape
+    sh = lambda a: np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (ny, nx)))
+    u8 = lambda a: np.asfortranarray(np.broadcast_to(np.asarray(a, dtype=np.uint8), (ny, nx)))
+    oidx = unique_outlets(row_outlet, col_outlet, ny)
+    prev, npg = None, 0
+    if outflow_prev is not None:
+        prev = _f(outflow_prev)
+        if prev.ndim == 3 and prev.shape[2] >= 2:
+            npg = prev.shape[2]
+    n2 = None if roughness_squared is None else sh(roughness_squared)
+    keep = []
+    up = C.POINTER(C.c_uint8)
+
+    def mp(a):
+        if a is None:
+            return up()
+        m = u8(a)
+        keep.append(m)
+        return m.ctypes.data_as(up)
+    fm = None
+    if face_masks is not None:
+        fm = (up * 4)(*[mp(m) for m in face_masks])
+    out = {"outflow": np.zeros((ny, nx, 5), order="F"), "Hf": np.zeros((ny, nx, 3), order="F")}
+    for k in ("d_t", "I_tot_end_cell", "C_a"):
+        out[k] = np.zeros((ny, nx), order="F")
+    lib().oracle_full_momentum(ny, nx, _p(z), _p(sh(d_tot)), _p(sh(roughness)), _p(n2),
+                               _p(sh(cell_area)), time_step_min, resolution, mp(outlet_index),
+                               outlet_type, slope_outlet, oidx.ctypes.data_as(_i64p), oidx.size,
+                               d_tolerance, _p(prev), npg, mp(idx_nan), int(flag_HR), fm,
+                               _p(out["outflow"]), _p(out["d_t"]), _p(out["I_tot_end_cell"]),
+                               _p(out["Hf"]), _p(out["C_a"]))
+    del keep
     return out
 
 
