@@ -225,13 +225,20 @@ D8_BYTES = {
     "ca8d_kernel": (5 * 8 + 1, 11 * 8),
     # 9 pages, d_t | qin, d_t
     "gather_kernel": (10 * 8, 2 * 8),
+    # z, d_tot, roughness, the two momentum pages | five outflow pages, d_t, I_tot, 3 Hf, C_a
+    "full momentum step (4 kernels)": (5 * 8, 11 * 8),
+    # d_tot, two previous flux pages | 3 outflow pages, d_t, I_tot, 3 Hf, C_a, eta, V
+    # (the table reads are counted apart: see subgrid_table_bytes)
+    "subgrid step (all kernels)": (3 * 8, 11 * 8),
 }
 
 
 def d8_study(n, calls=4):
-    """Kernel time of the D8 drop-ins on an n x n grid (HIP events around the
-    kernels on the handle's stream, host<->device copies excluded), as HBM
-    throughput of their algorithmic bytes."""
+    """Kernel time of the routing drop-ins beside the main step (the D8
+    routines, the gather, the full-momentum step on an n x n grid, the subgrid
+    step on n/2 x n/2 coarse cells): HIP events around the kernels on the
+    handle's stream, host<->device copies excluded, as HBM throughput of their
+    algorithmic bytes."""
     import ctypes as C
 
     from hydropol2d_amd import _lib
@@ -261,13 +268,51 @@ def d8_study(n, calls=4):
                                              np.zeros((n, n)), None, 2, 0.0, [n], [n], None, 1,
                                              1.0))
     _, ms_g = timed(lambda: li.outflows_to_inflows(q, 0.002, d))
-    for name, ms in (("bates8d_kernel", ms_b), ("ca8d_kernel", ms_c), ("gather_kernel", ms_g)):
+    del q
+    from hydropol2d_amd import subgrid as sgm
+    _, ms_f = timed(lambda: li.Full_Momentum_Model_D4(1, *res, 0, z, d, None, 0.03, None, 4.0,
+                                                      0.002, 2.0, None, 2, 0.0, [n], [n], 1.0,
+                                                      None, None, 0, 0, None, None, None, None,
+                                                      None, None, None, None, None, None, 0, 0,
+                                                      None, None))
+    # subgrid: an n/2 x n/2 coarse grid of 2 x 2 fine cells (10 levels)
+    nc = n // 2
+
+    class _R:
+        def __init__(self, Z, cs):
+            self.Z, self.cellsize = Z, cs
+    T, _ = sgm.Subgrid_Properties_Lookup(_R(z, 2.0), _R(np.full((n, n), 0.03), 2.0),
+                                         _R(np.zeros((nc, nc)), 4.0), 4.0, nr_levels=10)
+    hs = li._Handle.get(nc, nc, 0)
+    dc = np.asfortranarray(d[::2, ::2] * 4.0)
+    ms_s = []
+    prev = None
+    for _ in range(calls):
+        g = sgm.Local_Inertial_Model_D4_Subgrid(1, *res, 0, None, dc, None, None, None, 16.0,
+                                                0.002, 4.0, None, 2, 0.0, [nc], [nc], 1.0, prev,
+                                                None, 0, None, None, None, None, None, None, None,
+                                                None, None, None, 0, T)
+        prev = g[7]
+        _lib.check(L.hp2d_last_kernel_time(hs, C.byref(t)), handle=hs)
+        ms_s.append(t.value)
+    ms_s = float(np.median(ms_s[1:]))
+    del T
+    for name, ms in (("bates8d_kernel", ms_b), ("ca8d_kernel", ms_c), ("gather_kernel", ms_g),
+                     ("full momentum step (4 kernels)", ms_f)):
         rd, wr = D8_BYTES[name]
         gbs = (rd + wr) * n * n / (ms / 1000.0) / 1e9
         out[name] = {"avg_kernel_ms": ms, "bytes_per_cell": rd + wr, "achieved": gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "cell_updates_per_s": n * n / (ms / 1000.0)}
         print(f"[bench] d8 {name}: {ms:.4f} ms, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    rd, wr = D8_BYTES["subgrid step (all kernels)"]
+    out["subgrid step (all kernels)"] = {
+        "avg_kernel_ms": ms_s, "grid": [nc, nc], "nr_levels": 10, "bytes_per_cell": rd + wr,
+        "achieved": (rd + wr) * nc * nc / (ms_s / 1000.0) / 1e9, "unit": "GB/s",
+        "cell_updates_per_s": nc * nc / (ms_s / 1000.0),
+        "note": "HIP events around the whole step, which waits once per internal sub-step "
+                "for its negative-volume flag; table reads not in bytes_per_cell"}
+    print(f"[bench] subgrid step {nc}x{nc}: {ms_s:.4f} ms", file=sys.stderr, flush=True)
     return {"grid": [n, n], "dtype": "f64", "calls": calls, "kernels": out}
 
 
