@@ -143,7 +143,10 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     __syncthreads();
     if (t != 0) return;
     // the serial tail works on a private copy of the run state (registers:
-    // each dependent read-modify-write of a field costs no LDS round trip)
+    // each dependent read-modify-write of a field costs no LDS round trip).
+    // The rainfall-map and inflow fields, which are indexed by run-time
+    // values, stay in the LDS copy: a dynamically indexed array in the private
+    // copy would move the whole copy to scratch memory.
     SimState loc = sst;
     st = &loc;
     storage = uz = rain = infil = qsum = dsum = qraw = 0.0;
@@ -194,10 +197,10 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     const bool may_report = (prog_max >= st->next_report) || (prog_max >= 100.0);
     // rainfall volume of this step: the applied depth over the valid cells
     rain = failed ? 0.0
-                  : (a.n_maps > 0 ? map_depth_sum(a, st)
+                  : (a.n_maps > 0 ? map_depth_sum(a, &sst)
                                   : st->delta_p_agg * (a.has_pattern ? a.sum_pattern : a.n_valid)) *
                         (a.A / 1000.0);
-    const double inflow_step = failed ? 0.0 : inflow_depth_sum(a, st) * (a.A / 1000.0);
+    const double inflow_step = failed ? 0.0 : inflow_depth_sum(a, &sst) * (a.A / 1000.0);
     const double routing_time = a.routing_time;
     if (neg > 0) {  // Hydrological_Model.m:45-47
         st->catch_index += 1;
@@ -246,7 +249,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         int cur = st->cursor;
         st->delta_p_agg = rain_depth(ser, a.n_rain, st->t_previous, t_now, &cur);
         st->cursor = cur;
-        update_maps_inflow(a, st, st->t_previous, t_now);
+        update_maps_inflow(a, &sst, st->t_previous, t_now);
         // ---------------- EMC (:724-731), with the refreshed time_step ----------------
         if (a.flag_wq) {
             const double qo = qraw / 1000.0 / 3600.0 * 1000.0;
@@ -326,10 +329,10 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         if (do_report) {
             const double ts = st->time_step_min;
             st->outflow_volume = qsum / 1000.0 / 3600.0 * ts * 60.0 + st->outflow_volume;
-            const double Pv = a.n_maps > 0 ? map_depth_sum(a, st) / 1000.0 * a.A
+            const double Pv = a.n_maps > 0 ? map_depth_sum(a, &sst) / 1000.0 * a.A
                               : a.has_pattern ? st->delta_p_agg * a.sum_pattern / 1000.0 * a.A
                                               : a.n_valid * (st->delta_p_agg / 1000.0 * a.A);
-            const double inflow_part = inflow_depth_sum(a, st) / 1000.0 * a.A;
+            const double inflow_part = inflow_depth_sum(a, &sst) / 1000.0 * a.A;
             const double inflow_vol = inflow_part + Pv + 0.0;
             st->inflow_volume = inflow_vol + st->inflow_volume;
             const double Qin = inflow_vol - Pv;
@@ -371,10 +374,23 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         int cur = st->cursor;
         st->delta_p_agg = rain_depth(ser, a.n_rain, st->t_previous, t_now, &cur);
         st->cursor = cur;
-        update_maps_inflow(a, st, st->t_previous, t_now);
+        update_maps_inflow(a, &sst, st->t_previous, t_now);
     }
     st->done = !(st->t <= (routing_time + st->min_dt / 60.0)) ? 1 : 0;
     if (st->stop_accepted >= 0 && st->accepted >= st->stop_accepted) st->paused = 1;
+    loc.map_n = sst.map_n;
+    loc.map_cursor = sst.map_cursor;
+    loc.map_err = sst.map_err;
+    loc.map_dtot = sst.map_dtot;
+    loc.map_k = sst.map_k;
+    loc.inflow_cursor = sst.inflow_cursor;
+#pragma unroll
+    for (int j = 0; j < kMaxMapTerms; ++j) {
+        loc.map_idx[j] = sst.map_idx[j];
+        loc.map_w[j] = sst.map_w[j];
+    }
+#pragma unroll
+    for (int g = 0; g < kMaxInflowGauges; ++g) loc.inflow_agg[g] = sst.inflow_agg[g];
     *a.st = loc;
 }
 
