@@ -397,8 +397,7 @@ def main():
     sim = Simulation(spec)
     # warm-up (also lets water start moving: rainfall begins at t = 0)
     sim.run_fixed(args.warmup, graph_steps=min(args.graph_steps, max(args.warmup, 1)))
-    tune = {"tj": None, "waves_per_block": None} if args.no_tune else sim.autotune()
-    print(f"[bench] warm-up done, launch shape {tune}", file=sys.stderr, flush=True)
+    print("[bench] warm-up done", file=sys.stderr, flush=True)
     # untimed spin-up: the rain fills the domain and water starts to move; the
     # same graph size as the timed window, so its graph is built here
     ts = time.perf_counter()
@@ -412,8 +411,12 @@ def main():
               f"{time.perf_counter() - ts:.1f} s", file=sys.stderr, flush=True)
         if t_model >= args.spinup_min:
             break
-    # the graph of the timed window exists now: run one more graph's worth so
-    # that the timed interval holds launches only
+    # the launch shape is tuned on the wet domain of the timed window (the
+    # tuner drops the captured graph); the next graph's worth of steps builds
+    # the timed window's graph again, so that the timed interval holds
+    # launches only
+    tune = {"tj": None, "waves_per_block": None} if args.no_tune else sim.autotune()
+    print(f"[bench] launch shape {tune}", file=sys.stderr, flush=True)
     sim.run_fixed(args.graph_steps, graph_steps=args.graph_steps)
     t_start_min = sim.report()["t_min"]
     # timed region: K whole steps through captured graphs; run_fixed (like the

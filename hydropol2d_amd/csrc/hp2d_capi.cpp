@@ -1991,56 +1991,68 @@ extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t 
         set_err(s, "autotune needs a sim that has not finished");
         return HP2D_E_STATE;
     }
-    // neutralise the step: the kernel treats k == inject_k as a rejected step
-    // (writes only scratch buffers, no ledgers, no dmax) and no finalize runs.
-    const long long saved_inject = st.inject_k;
+    // The trials run the run's next step as it will run: the step kernel reads
+    // the current parity of the state and writes the other one, and the maps it
+    // raises (depth, time-to-peak and risk maxima) take the values the real
+    // next step gives them (a maximum is idempotent), so repeating it changes
+    // nothing the next step does not; the finalize kernel runs dry (its
+    // reduction only, into a sink), so no clock, ledger or dt moves.
     const long long saved_stop = st.stop_accepted;
     const int saved_paused = st.paused;
-    st.inject_k = st.k;
     st.stop_accepted = -1;
     st.paused = 0;
     if ((rc = write_state(s, st))) return rc;
     hipEvent_t e0, e1;
     STRY(s, hipEventCreate(&e0));
     STRY(s, hipEventCreate(&e1));
-    const int tjs[] = {16, 32, 64, 128};
-    const int wpbs[] = {1, 2, 4};
-    double best = 1e30;
-    int btj = s->tj, bwpb = s->wpb;
-    for (int tj : tjs) {
+    // the candidate shapes, each timed in three interleaved passes (so a
+    // slow moment of the device does not fall on one shape only), ranked by
+    // the median of its passes
+    std::vector<std::pair<int, int>> shapes;
+    for (int tj : {16, 32, 64, 128}) {
         if (tj > 16 && (s->nx + tj - 1) / tj < 2) continue;
-        for (int wpb : wpbs) {
-            s->tj = tj;
-            s->wpb = wpb;
-            // time the fused kernel plus the reduction of its partials
-            auto tune_step = [&]() -> hipError_t {
-                hipError_t e = step_launch(s, false);
-                if (e != hipSuccess) return e;
-                if (s->elem == 8) {
-                    FinArgs<double> f = sim_fin_args<double>(s);
-                    f.dry_run = 1;
-                    f.partials_out = s->tune_sink;
-                    return launch_finalize<double>(f, s->stream);
-                }
-                FinArgs<float> f = sim_fin_args<float>(s);
-                f.dry_run = 1;
-                f.partials_out = s->tune_sink;
-                return launch_finalize<float>(f, s->stream);
-            };
+        for (int wpb : {1, 2, 4}) shapes.push_back({tj, wpb});
+    }
+    // time the fused kernel plus the reduction of its partials
+    auto tune_step = [&]() -> hipError_t {
+        hipError_t e = step_launch(s, false);
+        if (e != hipSuccess) return e;
+        if (s->elem == 8) {
+            FinArgs<double> f = sim_fin_args<double>(s);
+            f.dry_run = 1;
+            f.partials_out = s->tune_sink;
+            return launch_finalize<double>(f, s->stream);
+        }
+        FinArgs<float> f = sim_fin_args<float>(s);
+        f.dry_run = 1;
+        f.partials_out = s->tune_sink;
+        return launch_finalize<float>(f, s->stream);
+    };
+    constexpr int kPasses = 3, kReps = 6;
+    std::vector<std::vector<double>> per(shapes.size());
+    for (int pass = 0; pass < kPasses; ++pass) {
+        for (size_t k = 0; k < shapes.size(); ++k) {
+            s->tj = shapes[k].first;
+            s->wpb = shapes[k].second;
             for (int i = 0; i < 2; ++i) STRY(s, tune_step());
             STRY(s, hipEventRecord(e0, s->stream));
-            const int reps = 5;
-            for (int i = 0; i < reps; ++i) STRY(s, tune_step());
+            for (int i = 0; i < kReps; ++i) STRY(s, tune_step());
             STRY(s, hipEventRecord(e1, s->stream));
             STRY(s, hipEventSynchronize(e1));
             float ms = 0;
             STRY(s, hipEventElapsedTime(&ms, e0, e1));
-            const double per = ms / reps;
-            if (per < best) {
-                best = per;
-                btj = tj;
-                bwpb = wpb;
-            }
+            per[k].push_back(ms / kReps);
+        }
+    }
+    double best = 1e30;
+    int btj = shapes.empty() ? s->tj : shapes[0].first, bwpb = shapes.empty() ? s->wpb : shapes[0].second;
+    for (size_t k = 0; k < shapes.size(); ++k) {
+        std::sort(per[k].begin(), per[k].end());
+        const double med = per[k][kPasses / 2];
+        if (med < best) {
+            best = med;
+            btj = shapes[k].first;
+            bwpb = shapes[k].second;
         }
     }
     hipEventDestroy(e0);
@@ -2051,7 +2063,6 @@ extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t 
         hipGraphExecDestroy(s->graph);
         s->graph = nullptr;
     }
-    st.inject_k = saved_inject;
     st.stop_accepted = saved_stop;
     st.paused = saved_paused;
     if ((rc = write_state(s, st))) return rc;

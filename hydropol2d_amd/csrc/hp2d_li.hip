@@ -135,6 +135,41 @@ __device__ __forceinline__ int find_outlet(const long long *idx, int n, long lon
     return -1;
 }
 
+// The step's optional inputs and outputs, as bits of a kernel's feature mask.
+enum : int {
+    kFeatGa = 1,        // Green-Ampt infiltration (I, T, soil class)
+    kFeatPattern = 2,   // a static rainfall pattern
+    kFeatMaps = 4,      // rainfall maps
+    kFeatGauge = 8,     // inflow hydrographs
+    kFeatRisk = 16,     // the human-instability risk map
+    kFeatSnap = 32,     // the map stacks' velocity snapshots
+    kFeatDmax = 64,     // the maximum-depth map
+    kFeatTpeak = 128,   // the time-to-peak map
+    kFeatRuntime = -1,  // all read from the arguments
+};
+template <int FEAT> __device__ __forceinline__ bool feat_on(int bit, bool runtime) {
+    if constexpr (FEAT < 0) return runtime;
+    else return (FEAT & bit) != 0;
+}
+
+// The run's optional inputs and outputs: fixed at compile time in the
+// specialised kernels (FEAT >= 0), read from the arguments otherwise.
+struct Feats {
+    bool ga, pat, maps, gauge, risk, snap, dmax, tpeak;
+};
+template <int FEAT, typename T> __device__ __forceinline__ Feats feats_of(const LiArgs<T> &a) {
+    Feats f;
+    f.ga = feat_on<FEAT>(kFeatGa, a.ga != nullptr);
+    f.pat = feat_on<FEAT>(kFeatPattern, a.rain_pattern != nullptr);
+    f.maps = feat_on<FEAT>(kFeatMaps, a.rain_maps != nullptr);
+    f.gauge = feat_on<FEAT>(kFeatGauge, a.inflow_gauge != nullptr);
+    f.risk = feat_on<FEAT>(kFeatRisk, a.max_risk != nullptr);
+    f.snap = feat_on<FEAT>(kFeatSnap, a.snap_vras != nullptr);
+    f.dmax = feat_on<FEAT>(kFeatDmax, a.dmax_map != nullptr);
+    f.tpeak = feat_on<FEAT>(kFeatTpeak, a.tpeak_map != nullptr);
+    return f;
+}
+
 // Per-column loads of one lane, fetched one column ahead of the compute.
 // z: the elevation (FP64), or dz of the cell's x face (FP32); zy: dz of its
 // y face (FP32 only)
@@ -153,10 +188,11 @@ template <typename T> struct MapStep {
 };
 
 template <typename T, int MODE>
-__device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d_src,
-                                                  const T *qx_src, const T *qy_src,
-                                                  const T *I_src, const T *Tc_src, bool in,
-                                                  unsigned id, const MapStep<T> &ms) {
+__device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const Feats &F,
+                                                  const T *d_src, const T *qx_src,
+                                                  const T *qy_src, const T *I_src,
+                                                  const T *Tc_src, bool in, unsigned id,
+                                                  const MapStep<T> &ms) {
     ColLoad<T> v;
     v.z = v.zy = v.d = v.n2 = v.qx = v.qy = v.I = v.Tc = T(0);
     v.pat = T(1);
@@ -173,17 +209,17 @@ __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d
         v.qx = qx_src[id];
         v.qy = qy_src[id];
         if (MODE == kModeSim) {
-            if (a.rain_maps) {
+            if (F.maps) {
                 // the step's rain depth from the maps (update_spatial_BC.m:1361-1408):
                 // (sum_i overlap_i * map_i) / (t - t_prev) * dt / 60
                 T acc = T(0);
                 for (int j = 0; j < ms.n; ++j) acc = acc + ms.w[j] * a.rain_maps[ms.off[j] + id];
                 v.pat = ms.n > 0 ? acc / ms.dtot * ms.k : T(0);
-            } else if (a.rain_pattern) {
+            } else if (F.pat) {
                 v.pat = a.rain_pattern[id];
             }
-            if (a.inflow_gauge) v.gauge = a.inflow_gauge[id];
-            if (a.ga) {
+            if (F.gauge) v.gauge = a.inflow_gauge[id];
+            if (F.ga) {
                 v.I = I_src[id];
                 v.Tc = Tc_src[id];
                 if (a.soil_class) v.soil = a.soil_class[id];
@@ -195,9 +231,10 @@ __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const T *d
 
 // One workgroup's tile of one grid: `logical` counts the grid's tiles, row
 // blocks fastest.
-template <typename T, int MODE, bool CRIT>
+template <typename T, int MODE, bool CRIT, int FEAT>
 __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logical) {
     constexpr bool f64 = sizeof(T) == 8;
+    const Feats F = feats_of<FEAT>(a);
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     const int n_bx = (a.n_slabs + a.waves_per_block - 1) / a.waves_per_block;
@@ -226,7 +263,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
         rain = T(st->delta_p_agg);
         t_end_min = T(st->t + st->time_step_min);
         fail_step = (st->inject_k == kstep);
-        if (a.rain_maps) {
+        if (F.maps) {
             const unsigned cells = (unsigned)ny * (unsigned)nx;
             ms.n = st->map_n;
             for (int j = 0; j < kMaxMapTerms; ++j) {
@@ -244,7 +281,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
         // a map record can only follow this step if t + max_dt passes the next
         // record time (the new dt never exceeds max_dt); only then are the
         // velocities stored
-        may_snap = a.snap_vras && (st->t + st->max_dt / 60.0 > (double)st->map_last * a.rec_maps);
+        may_snap = F.snap && (st->t + st->max_dt / 60.0 > (double)st->map_last * a.rec_maps);
         const int src = (int)((kstep + 1) & 1), dst = (int)(kstep & 1);
         d_src = a.d_buf[src];
         d_dst = a.d_buf[dst];
@@ -264,7 +301,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     // Green-Ampt class table (FP32 path) staged in LDS: the per-cell class
     // gather then costs an LDS read instead of a dependent global load
     __shared__ float4 ga_lds[2 * kMaxSoilClasses];
-    if (MODE == kModeSim && !f64 && a.ga) {
+    if (MODE == kModeSim && !f64 && F.ga) {
         for (int i = threadIdx.x; i < 2 * a.n_soil_classes; i += blockDim.x)
             ga_lds[i] = reinterpret_cast<const float4 *>(a.ga_f32)[i];
         __syncthreads();
@@ -292,7 +329,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     int neg = 0;
 
     // 32-bit element offsets (grids are limited to 2^32 cells at creation)
-    ColLoad<T> nxt = load_column<T, MODE>(a, d_src, qx_src, qy_src, I_src, Tc_src,
+    ColLoad<T> nxt = load_column<T, MODE>(a, F, d_src, qx_src, qy_src, I_src, Tc_src,
                                           active && row_in && c0 - 2 >= 0 && c0 - 2 < nx,
                                           (unsigned)r + (unsigned)(c0 - 2) * (unsigned)ny, ms);
     const int c_end = active ? c1 + 1 : c0 - 3;  // inactive waves skip the march
@@ -304,7 +341,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
         const ColLoad<T> cur = nxt;
         // issue the next column's loads before computing this one
         if (c + 1 <= c1 + 1)
-            nxt = load_column<T, MODE>(a, d_src, qx_src, qy_src, I_src, Tc_src,
+            nxt = load_column<T, MODE>(a, F, d_src, qx_src, qy_src, I_src, Tc_src,
                                        row_in && c + 1 >= 0 && c + 1 < nx, id + (unsigned)ny, ms);
         const T zc = cur.z, n2c = cur.n2, qxc = cur.qx, qyc = cur.qy;
         T dc = cur.d;
@@ -314,11 +351,11 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
         // ---- rainfall + Green-Ampt prologue: Hydrological_Model.m:15-55 ----
         if (MODE == kModeSim && cell_in) {
             const T dprev = dc;
-            T eff = a.rain_maps ? cur.pat : (a.rain_pattern ? cur.pat * rain : rain);
+            T eff = F.maps ? cur.pat : (F.pat ? cur.pat * rain : rain);
             if (nan_cell) eff = T(NAN);
             T dd = dc + eff;
             const bool own = row_own && c >= c0 && c < c1;
-            if (a.ga) {
+            if (F.ga) {
                 // legacy explicit Green-Ampt closure in the Infiltration_Module slot
                 // (HydroPol2D_Main_While_Ensemble.m:243-298)
                 T ksat, psi, dth, Lu_mm, k_out, Tr;
@@ -366,7 +403,7 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
                 }
             }
             // Hydrological_Model.m:30: the inflow hydrographs on their gauge cells
-            if (a.inflow_gauge && cur.gauge > 0) dd = dd + T(a.st->inflow_agg[cur.gauge - 1]);
+            if (F.gauge && cur.gauge > 0) dd = dd + T(a.st->inflow_agg[cur.gauge - 1]);
             if (dd < T(-1e-8)) neg = 1;
             if (dd < T(1e-6)) dd = T(0);  // Hydrological_Model.m:49
             if (dd < T(0)) dd = T(0);     // :52
@@ -495,9 +532,9 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
                     const T tot2 = rc * rc + uc * uc;  // compared squared, rooted once per step
                     vkey = dmax(vkey, tot2);
                     stagnant = tot2 < T(1e-4);
-                    if (a.max_risk || may_snap) vtot = sqrt(tot2);
+                    if (F.risk || may_snap) vtot = sqrt(tot2);
                 }
-                if (a.max_risk) {
+                if (F.risk) {
                     // Human_Instability_Module.m:8-25 and its running maximum
                     const T h = mm2m(dn);
                     const T F_person = a.hi_weight * a.hi_g;
@@ -534,14 +571,14 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
                 d_dst[oid] = dn;
                 qx_dst[oid] = ox;
                 qy_dst[oid] = oy;
-                if (a.dmax_map && !fail_step) {
+                if (F.dmax && !fail_step) {
                     const T prevmax = a.dmax_map[oid];
                     const T newmax = (kstep == 1) ? dn : dmax(dn, prevmax);
                     // stored only when it changes (a NaN old value always differs):
                     // a cell that is not rising costs no write
                     if (kstep == 1 || newmax != prevmax) a.dmax_map[oid] = newmax;
                     // time to peak: end time of the step that last raised the maximum
-                    if (a.tpeak_map && (kstep == 1 || dn > prevmax)) a.tpeak_map[oid] = t_end_min;
+                    if (F.tpeak && (kstep == 1 || dn > prevmax)) a.tpeak_map[oid] = t_end_min;
                 }
             }
         } else {
@@ -622,9 +659,9 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     }
 }
 
-template <typename T, int MODE, bool CRIT>
+template <typename T, int MODE, bool CRIT, int FEAT>
 __global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
-    li_step_tile<T, MODE, CRIT>(a, xcd_logical_tile());
+    li_step_tile<T, MODE, CRIT, FEAT>(a, xcd_logical_tile());
 }
 
 // Many grids in one launch (an ensemble): tile t of the launch belongs to grid
@@ -639,7 +676,36 @@ __global__ __launch_bounds__(256) void li_step_batch_kernel(const LiArgs<T> *__r
     // read in place: the fields are wave-uniform scalar loads from a read-only
     // table, kept in the scalar cache
     const LiArgs<T> &a = args[g];
-    li_step_tile<T, kModeSim, CRIT>(a, t - grid_first_tile[g]);
+    li_step_tile<T, kModeSim, CRIT, kFeatRuntime>(a, t - grid_first_tile[g]);
+}
+
+// The feature mask of a run's arguments.
+template <typename T> int li_feature_mask(const LiArgs<T> &a) {
+    return (a.ga ? kFeatGa : 0) | (a.rain_pattern ? kFeatPattern : 0) |
+           (a.rain_maps ? kFeatMaps : 0) | (a.inflow_gauge ? kFeatGauge : 0) |
+           (a.max_risk ? kFeatRisk : 0) | (a.snap_vras ? kFeatSnap : 0) |
+           (a.dmax_map ? kFeatDmax : 0) | (a.tpeak_map ? kFeatTpeak : 0);
+}
+
+// The simulation step: FP32 runs whose optional arrays are one of the common
+// sets (rain as a pattern or lumped, with or without Green-Ampt, with the
+// maximum-depth map) get a kernel specialised to that set, which keeps the
+// arrays it does not have out of its code and its scalar registers; any other
+// set runs the kernel that reads the set from the arguments.
+template <typename T, bool CRIT>
+void launch_li_sim(const LiArgs<T> &a, dim3 grid, dim3 block, hipStream_t s) {
+    if constexpr (sizeof(T) == 4) {
+        constexpr int kGP = kFeatGa | kFeatPattern | kFeatDmax, kG = kFeatGa | kFeatDmax;
+        constexpr int kP = kFeatPattern | kFeatDmax, kD = kFeatDmax;
+        switch (li_feature_mask(a)) {
+        case kGP: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kGP>), grid, block, 0, s, a); return;
+        case kG: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kG>), grid, block, 0, s, a); return;
+        case kP: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kP>), grid, block, 0, s, a); return;
+        case kD: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kD>), grid, block, 0, s, a); return;
+        default: break;
+        }
+    }
+    hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kFeatRuntime>), grid, block, 0, s, a);
 }
 
 template <typename T>
@@ -647,15 +713,14 @@ hipError_t launch_li_step(const LiArgs<T> &a, int mode, hipStream_t s) {
     dim3 block(kWave * a.waves_per_block);
     dim3 grid(((a.n_slabs + a.waves_per_block - 1) / a.waves_per_block) * ((a.nx + a.tj - 1) / a.tj));
     if (mode == kModeSim) {
-        if (a.flag_critical)
-            hipLaunchKernelGGL((li_step_kernel<T, kModeSim, true>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((li_step_kernel<T, kModeSim, false>), grid, block, 0, s, a);
+        if (a.flag_critical) launch_li_sim<T, true>(a, grid, block, s);
+        else launch_li_sim<T, false>(a, grid, block, s);
     } else {
+        // the drop-in mode has none of the simulation's optional arrays
         if (a.flag_critical)
-            hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn, true>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn, true, 0>), grid, block, 0, s, a);
         else
-            hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn, false>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn, false, 0>), grid, block, 0, s, a);
     }
     return hipGetLastError();
 }
