@@ -415,7 +415,7 @@ def main():
     # tuner drops the captured graph); the next graph's worth of steps builds
     # the timed window's graph again, so that the timed interval holds
     # launches only
-    tune = {"tj": None, "waves_per_block": None} if args.no_tune else sim.autotune()
+    tune = sim.launch_shape() if args.no_tune else sim.autotune()
     print(f"[bench] launch shape {tune}", file=sys.stderr, flush=True)
     sim.run_fixed(args.graph_steps, graph_steps=args.graph_steps)
     t_start_min = sim.report()["t_min"]
@@ -497,7 +497,8 @@ def main():
                                "wash-off passes, dt control; timed window after "
                                f"{t_start_min:.1f} min of model time",
                    "cells": cells, "precision": args.precision,
-                   "launch": {"cols_per_wave": tune["tj"], "waves_per_block": tune["waves_per_block"]},
+                   "launch": {"cols_per_wave": tune["tj"], "waves_per_block": tune["waves_per_block"],
+                              "rows_per_lane": tune["rows"]},
                    "bytes_per_cell_update": rd + wr},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),

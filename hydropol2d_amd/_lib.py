@@ -129,6 +129,8 @@ SIGNATURES = {
     "hp2d_debug_inject_failure": (C.c_int32, [_vp, C.c_int64, C.c_int32]),
     "hp2d_sim_kernel_time": (C.c_int32, [_vp, _dp, _dp, C.POINTER(C.c_int64)]),
     "hp2d_sim_autotune": (C.c_int32, [_vp, _i32p, _i32p, _dp]),
+    "hp2d_sim_launch_shape": (C.c_int32, [_vp, _i32p, _i32p, _i32p]),
+    "hp2d_sim_set_launch_shape": (C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32]),
 }
 
 

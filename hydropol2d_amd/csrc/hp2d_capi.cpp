@@ -89,7 +89,7 @@ struct hp2d_handle {
     long long *out_idx = nullptr;
     void *out_n = nullptr;
     int out_cap = 0;
-    int tj = 16, wpb = 4;
+    int tj = 16, wpb = 4, rows = 1;
     void *ws = nullptr;          // module workspace (wash-off, groundwater)
     size_t ws_bytes = 0;
     void *ws2 = nullptr;         // second workspace (groundwater module around Boussinesq)
@@ -151,6 +151,7 @@ hp2d_status hp2d_create(int32_t ny, int32_t nx, int32_t precision, hp2d_handle *
     h->precision = precision;
     h->elem = precision == HP2D_FP64 ? 8 : 4;
     h->tj = default_tj(nx);
+    h->rows = li_rows_ok(ny, 2) ? 2 : 1;
     const size_t n = (size_t)ny * nx;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     for (int i = 0; i < 19 && e == hipSuccess; ++i) e = hipMalloc(&h->dev[i], n * h->elem);
@@ -351,7 +352,8 @@ static hp2d_status li_dropin(hp2d_handle *h, int scheme, const T *z, const T *d_
     a.ny = ny;
     a.nx = nx;
     a.tj = h->tj;
-    a.n_slabs = (ny + kRowsOut - 1) / kRowsOut;
+    a.rows = h->rows;
+    a.n_slabs = li_n_slabs(ny, h->rows);
     a.waves_per_block = h->wpb;
     a.R = resolution;
     a.R2 = resolution * resolution;
@@ -512,7 +514,7 @@ struct hp2d_sim {
     double *tune_sink = nullptr;
     int n_rep_max = 256;
     double sum_pattern = 0, n_valid = 0, drainage_area = 0, I_min = 0;
-    int tj = 16, wpb = 4;
+    int tj = 16, wpb = 4, rows = 1;  // step kernel launch shape (rows per lane)
     // graph
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -563,7 +565,8 @@ template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
     a.ny = s->ny;
     a.nx = s->nx;
     a.tj = s->tj;
-    a.n_slabs = (s->ny + kRowsOut - 1) / kRowsOut;
+    a.rows = s->rows;
+    a.n_slabs = li_n_slabs(s->ny, s->rows);
     a.waves_per_block = s->wpb;
     a.R = T(c.resolution);
     a.R2 = T(c.resolution * c.resolution);
@@ -668,7 +671,7 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
     f.partials = s->partials;
     f.part_cap = s->part_cap;
     // one partial per workgroup of the current launch shape
-    f.n_partials = (((s->ny + kRowsOut - 1) / kRowsOut + s->wpb - 1) / s->wpb) *
+    f.n_partials = ((li_n_slabs(s->ny, s->rows) + s->wpb - 1) / s->wpb) *
                    ((s->nx + s->tj - 1) / s->tj);
     f.routing_time = c.routing_time_min;
     f.tsc = c.time_step_change_s;
@@ -828,6 +831,7 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     s->n = (size_t)cfg->ny * cfg->nx;
     s->elem = cfg->precision == HP2D_FP64 ? 8 : 4;
     s->tj = default_tj(cfg->nx);
+    s->rows = li_rows_ok(cfg->ny, 2) ? 2 : 1;
     const size_t n = s->n;
     const int ny = s->ny;
     const double R = cfg->resolution, A = R * R;
@@ -1959,7 +1963,10 @@ struct TuneKey {
     }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, std::pair<int, int>> g_tune_cache;
+struct Shape {
+    int tj, wpb, rows;
+};
+std::map<TuneKey, Shape> g_tune_cache;
 }  // namespace
 
 extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t *best_wpb,
@@ -1972,8 +1979,9 @@ extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t 
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
-            s->tj = it->second.first;
-            s->wpb = it->second.second;
+            s->tj = it->second.tj;
+            s->wpb = it->second.wpb;
+            s->rows = it->second.rows;
             if (s->graph) {
                 hipGraphExecDestroy(s->graph);
                 s->graph = nullptr;
@@ -2008,10 +2016,13 @@ extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t 
     // the candidate shapes, each timed in three interleaved passes (so a
     // slow moment of the device does not fall on one shape only), ranked by
     // the median of its passes
-    std::vector<std::pair<int, int>> shapes;
-    for (int tj : {16, 32, 64, 128}) {
-        if (tj > 16 && (s->nx + tj - 1) / tj < 2) continue;
-        for (int wpb : {1, 2, 4}) shapes.push_back({tj, wpb});
+    std::vector<Shape> shapes;
+    for (int rows : {1, 2}) {
+        if (!li_rows_ok(s->ny, rows)) continue;
+        for (int tj : {16, 32, 64, 128}) {
+            if (tj > 16 && (s->nx + tj - 1) / tj < 2) continue;
+            for (int wpb : {1, 2, 4}) shapes.push_back({tj, wpb, rows});
+        }
     }
     // time the fused kernel plus the reduction of its partials
     auto tune_step = [&]() -> hipError_t {
@@ -2032,8 +2043,9 @@ extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t 
     std::vector<std::vector<double>> per(shapes.size());
     for (int pass = 0; pass < kPasses; ++pass) {
         for (size_t k = 0; k < shapes.size(); ++k) {
-            s->tj = shapes[k].first;
-            s->wpb = shapes[k].second;
+            s->tj = shapes[k].tj;
+            s->wpb = shapes[k].wpb;
+            s->rows = shapes[k].rows;
             for (int i = 0; i < 2; ++i) STRY(s, tune_step());
             STRY(s, hipEventRecord(e0, s->stream));
             for (int i = 0; i < kReps; ++i) STRY(s, tune_step());
@@ -2045,20 +2057,20 @@ extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t 
         }
     }
     double best = 1e30;
-    int btj = shapes.empty() ? s->tj : shapes[0].first, bwpb = shapes.empty() ? s->wpb : shapes[0].second;
+    Shape bs = shapes.empty() ? Shape{s->tj, s->wpb, s->rows} : shapes[0];
     for (size_t k = 0; k < shapes.size(); ++k) {
         std::sort(per[k].begin(), per[k].end());
         const double med = per[k][kPasses / 2];
         if (med < best) {
             best = med;
-            btj = shapes[k].first;
-            bwpb = shapes[k].second;
+            bs = shapes[k];
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    s->tj = btj;
-    s->wpb = bwpb;
+    s->tj = bs.tj;
+    s->wpb = bs.wpb;
+    s->rows = bs.rows;
     if (s->graph) {
         hipGraphExecDestroy(s->graph);
         s->graph = nullptr;
@@ -2068,11 +2080,35 @@ extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t 
     if ((rc = write_state(s, st))) return rc;
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
-        g_tune_cache[key] = {btj, bwpb};
+        g_tune_cache[key] = bs;
     }
-    if (best_tj) *best_tj = btj;
-    if (best_wpb) *best_wpb = bwpb;
+    if (best_tj) *best_tj = bs.tj;
+    if (best_wpb) *best_wpb = bs.wpb;
     if (best_ms) *best_ms = best;
+    return HP2D_OK;
+}
+
+extern "C" hp2d_status hp2d_sim_launch_shape(hp2d_sim *s, int32_t *tj, int32_t *wpb, int32_t *rows) {
+    if (!s) return HP2D_E_ARG;
+    if (tj) *tj = s->tj;
+    if (wpb) *wpb = s->wpb;
+    if (rows) *rows = s->rows;
+    return HP2D_OK;
+}
+
+extern "C" hp2d_status hp2d_sim_set_launch_shape(hp2d_sim *s, int32_t tj, int32_t wpb, int32_t rows) {
+    if (!s) return HP2D_E_ARG;
+    if (tj < 16 || tj > 4096 || (wpb != 1 && wpb != 2 && wpb != 4) || !li_rows_ok(s->ny, rows)) {
+        set_err(s, "hp2d_sim_set_launch_shape: tj must be 16..4096, wpb 1, 2 or 4, rows 1 (or 2 for an even ny)");
+        return HP2D_E_ARG;
+    }
+    s->tj = tj;
+    s->wpb = wpb;
+    s->rows = rows;
+    if (s->graph) {  // captured launches carry the old shape
+        hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
     return HP2D_OK;
 }
 
@@ -2148,7 +2184,7 @@ __global__ void gather_done_kernel(SimState *const *states, int n, int *done) {
 }
 
 int member_tiles(const hp2d_sim *s) {
-    return (((s->ny + kRowsOut - 1) / kRowsOut + s->wpb - 1) / s->wpb) *
+    return ((li_n_slabs(s->ny, s->rows) + s->wpb - 1) / s->wpb) *
            ((s->nx + s->tj - 1) / s->tj);
 }
 
@@ -2287,6 +2323,7 @@ hp2d_status hp2d_ensemble_create(int32_t n_members, const hp2d_run_config *cfgs,
             return rc;
         }
         s->wpb = e->wpb;
+        s->rows = 1;  // the batched kernel holds one row per lane
         if (cols_per_wave) s->tj = cols_per_wave;
         e->m.push_back(s);
     }

@@ -11,7 +11,15 @@ namespace hp2d {
 
 constexpr int kWave = 64;                  // CDNA wavefront
 constexpr int kHalo = 2;                   // radius of the step stencil (donor limiter)
-constexpr int kRowsOut = kWave - 2 * kHalo; // rows finalised per wave slab
+constexpr int kRowsOut = kWave - 2 * kHalo; // rows finalised per wave slab (one row per lane)
+
+// The step kernel's slabs when each lane holds `rows` consecutive rows (1 or
+// 2): 64*rows rows, of which 64*rows - 4 are finalised.  Two rows per lane
+// need an even row count (a lane's rows are then both inside or both outside
+// the grid, and its loads are aligned 2-element vector loads).
+inline int li_rows_out(int rows) { return kWave * rows - 2 * kHalo; }
+inline int li_n_slabs(int ny, int rows) { return (ny + li_rows_out(rows) - 1) / li_rows_out(rows); }
+inline bool li_rows_ok(int ny, int rows) { return rows == 1 || (rows == 2 && ny % 2 == 0); }
 constexpr int kXcds = 8;                    // accelerator complex dies of an MI355X
 constexpr int kMaxWavesPerBlock = 4;
 

@@ -1,13 +1,14 @@
 // hp2d_li.hip - fused local-inertial D4 step for gfx950.
 //
-// One wavefront owns a slab of 64 grid rows (MATLAB index i, the contiguous
-// one) and marches along the columns j of its chunk.  Lanes 2..61 finalise
-// rows; lanes 0,1,62,63 are the radius-2 halo that the donor draining limiter
-// needs (a face flux depends on its donor cell's scale, which depends on all
-// four faces of that donor).  Every per-cell array is read once per column
-// with a 64-lane coalesced load issued one column ahead of the compute; the
-// 2-row halo is re-read from L2, and neighbours across lanes come from
-// wavefront-shift DPP moves (no LDS).  Per column the wave computes, with a
+// One wavefront owns a slab of 64*G grid rows (MATLAB index i, the contiguous
+// one; each lane holds G = 1 or 2 consecutive rows) and marches along the
+// columns j of its chunk.  The outer two rows on each side are the radius-2
+// halo that the donor draining limiter needs (a face flux depends on its donor
+// cell's scale, which depends on all four faces of that donor); the others are
+// finalised.  Every per-cell array is read once per column with a coalesced
+// load (8 bytes per lane in FP32 at G = 2) issued one column ahead of the
+// compute; the 2-row halo is re-read from L2, and neighbours across lanes come
+// from wavefront-shift DPP moves (no LDS).  Per column the wave computes, with a
 // lag of two columns:
 //   raw face fluxes  (Inertial_Solver.m:23-26 + caps, Local_Inertial_Model_D4.m:123-142,198,263-304)
 //   donor scales     (Local_Inertial_Model_D4.m:799-829)
@@ -23,19 +24,22 @@
 #include "hp2d_internal.hpp"
 #include "hp2d_li.hpp"
 
+#include <type_traits>
+
 namespace hp2d {
 
 // ---------------------------------------------------------------------------
 // cross-lane moves: lane i receives lane i-1 (up) / lane i+1 (down)
-// (DPP wave_shr:1 / wave_shl:1; the end lanes keep 0 and are halo lanes)
+// (DPP wave_shr:1 / wave_shl:1 with bound_ctrl: the end lanes receive 0 and
+// are halo lanes)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float lane_up(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
-                                                                 0x138, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v),
+                                                              0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float lane_down(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
-                                                                 0x130, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v),
+                                                              0x130, 0xf, 0xf, true));
 }
 __device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1); }
 __device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1); }
@@ -45,6 +49,9 @@ __device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// a*b + c: in the reference's order (two roundings) in FP64, fused in FP32
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return a * b + c; }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 // mm -> m
 __device__ __forceinline__ double mm2m(double x) { return x / 1000.0; }
 __device__ __forceinline__ float mm2m(float x) { return x * 1e-3f; }
@@ -92,8 +99,8 @@ __device__ __forceinline__ double bates(double q_p, double Hf, double S, double 
 __device__ __forceinline__ float bates(float q_p, float Hf, float S, float n2, float dt) {
     const float g = 9.81f;
     const float p73 = __builtin_amdgcn_exp2f((7.0f / 3.0f) * __builtin_amdgcn_logf(Hf));
-    const float den = 1.0f + g * dt * n2 * fabsf(q_p) * __builtin_amdgcn_rcpf(p73);
-    return (q_p - g * Hf * dt * S) * __builtin_amdgcn_rcpf(den);
+    const float den = fmadd(g * dt * n2 * fabsf(q_p), __builtin_amdgcn_rcpf(p73), 1.0f);
+    return fmadd(-(g * dt) * Hf, S, q_p) * __builtin_amdgcn_rcpf(den);
 }
 
 // face discharge [m3/s] -> stored mm/h of cell area (:309)
@@ -170,12 +177,79 @@ template <int FEAT, typename T> __device__ __forceinline__ Feats feats_of(const 
     return f;
 }
 
+// A lane holds G consecutive grid rows (G = 1 or 2; G = 2 needs an even row
+// count, so that the rows of a lane are all inside the grid or all outside and
+// their loads are aligned vector loads).  The neighbour above row k of a lane
+// is row k-1 of the same lane, or the last row of the lane above (a DPP move);
+// the neighbour below likewise.
+template <int G, typename T> __device__ __forceinline__ T up_of(const T (&v)[G], int k) {
+    return k > 0 ? v[k - 1] : lane_up(v[G - 1]);
+}
+template <int G, typename T> __device__ __forceinline__ T down_of(const T (&v)[G], int k) {
+    return k < G - 1 ? v[k + 1] : lane_down(v[0]);
+}
+
+// Address of p[id].  With OFF32 (every array of the launch under 4 GiB) it is
+// the array's base plus a 32-bit byte offset: the load then takes the base from
+// SGPRs and one shared VGPR offset, with no 64-bit address arithmetic.
+template <bool OFF32, typename E> __device__ __forceinline__ E *elem_ptr(E *p, unsigned id) {
+    if constexpr (OFF32) {
+        using B = typename std::conditional<std::is_const<E>::value, const char, char>::type;
+        return reinterpret_cast<E *>(reinterpret_cast<B *>(p) + id * (unsigned)sizeof(E));
+    } else {
+        return p + id;
+    }
+}
+
+// The G elements p[id .. id+G-1] (id a multiple of G): one vector access.
+template <int G, bool OFF32, typename T>
+__device__ __forceinline__ void ld_rows(const T *__restrict__ p, unsigned id, T (&o)[G]) {
+    const T *q = elem_ptr<OFF32>(p, id);
+    if constexpr (G == 2 && sizeof(T) == 4) {
+        const float2 v = *reinterpret_cast<const float2 *>(q);
+        o[0] = v.x;
+        o[1] = v.y;
+    } else if constexpr (G == 2 && sizeof(T) == 8) {
+        const double2 v = *reinterpret_cast<const double2 *>(q);
+        o[0] = v.x;
+        o[1] = v.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < G; ++k) o[k] = q[k];
+    }
+}
+template <int G, bool OFF32>
+__device__ __forceinline__ void ld_rows_u8(const uint8_t *__restrict__ p, unsigned id, int (&o)[G]) {
+    const uint8_t *q = elem_ptr<OFF32>(p, id);
+    if constexpr (G == 2) {
+        const unsigned v = *reinterpret_cast<const unsigned short *>(q);
+        o[0] = (int)(v & 0xffu);
+        o[1] = (int)(v >> 8);
+    } else {
+#pragma unroll
+        for (int k = 0; k < G; ++k) o[k] = q[k];
+    }
+}
+template <int G, bool OFF32, typename T>
+__device__ __forceinline__ void st_rows(T *__restrict__ p, unsigned id, const T (&v)[G]) {
+    T *q = elem_ptr<OFF32>(p, id);
+    if constexpr (G == 2 && sizeof(T) == 4) {
+        *reinterpret_cast<float2 *>(q) = make_float2(v[0], v[1]);
+    } else if constexpr (G == 2 && sizeof(T) == 8) {
+        *reinterpret_cast<double2 *>(q) = make_double2(v[0], v[1]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < G; ++k) q[k] = v[k];
+    }
+}
+
 // Per-column loads of one lane, fetched one column ahead of the compute.
 // z: the elevation (FP64), or dz of the cell's x face (FP32); zy: dz of its
 // y face (FP32 only)
-template <typename T> struct ColLoad {
-    T z, zy, d, n2, qx, qy, pat, I, Tc;
-    int soil, gauge;
+template <typename T, int G> struct ColLoad {
+    T z[G], zy[G], d[G], n2[G], qx[G], qy[G], pat[G], I[G], Tc[G];
+    int soil[G], gauge[G];
+    T dm[G];  // the depth maximum of the output column two behind (simulation mode)
 };
 
 // The rainfall maps' terms of this step (wave-uniform): the offsets of the
@@ -187,42 +261,57 @@ template <typename T> struct MapStep {
     T dtot, k;
 };
 
-template <typename T, int MODE>
-__device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const Feats &F,
-                                                  const T *d_src, const T *qx_src,
-                                                  const T *qy_src, const T *I_src,
-                                                  const T *Tc_src, bool in, unsigned id,
-                                                  const MapStep<T> &ms) {
-    ColLoad<T> v;
-    v.z = v.zy = v.d = v.n2 = v.qx = v.qy = v.I = v.Tc = T(0);
-    v.pat = T(1);
-    v.soil = v.gauge = 0;
+template <typename T, int MODE, int G, bool OFF32>
+__device__ __forceinline__ ColLoad<T, G> load_column(const LiArgs<T> &a, const Feats &F,
+                                                     const T *d_src, const T *qx_src,
+                                                     const T *qy_src, const T *I_src,
+                                                     const T *Tc_src, bool in, unsigned id,
+                                                     const MapStep<T> &ms, bool dm_in = false) {
+    ColLoad<T, G> v;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        v.z[k] = v.zy[k] = v.d[k] = v.n2[k] = v.qx[k] = v.qy[k] = v.I[k] = v.Tc[k] = T(0);
+        v.pat[k] = T(1);
+        v.soil[k] = v.gauge[k] = 0;
+        v.dm[k] = T(0);
+    }
+    // the depth maximum of column c - 1 when this loads column c + 1: the
+    // output column of the step that computes c + 1
+    if (MODE == kModeSim && dm_in) ld_rows<G, OFF32>(a.dmax_map, id - 2u * (unsigned)a.ny, v.dm);
     if (in) {
         if constexpr (sizeof(T) == 8) {
-            v.z = a.z[id];
+            ld_rows<G, OFF32>(a.z, id, v.z);
         } else {
-            v.z = a.dzx[id];
-            v.zy = a.dzy[id];
+            ld_rows<G, OFF32>(a.dzx, id, v.z);
+            ld_rows<G, OFF32>(a.dzy, id, v.zy);
         }
-        v.d = d_src[id];
-        v.n2 = a.n2s[id];
-        v.qx = qx_src[id];
-        v.qy = qy_src[id];
+        ld_rows<G, OFF32>(d_src, id, v.d);
+        ld_rows<G, OFF32>(a.n2s, id, v.n2);
+        ld_rows<G, OFF32>(qx_src, id, v.qx);
+        ld_rows<G, OFF32>(qy_src, id, v.qy);
         if (MODE == kModeSim) {
             if (F.maps) {
                 // the step's rain depth from the maps (update_spatial_BC.m:1361-1408):
                 // (sum_i overlap_i * map_i) / (t - t_prev) * dt / 60
-                T acc = T(0);
-                for (int j = 0; j < ms.n; ++j) acc = acc + ms.w[j] * a.rain_maps[ms.off[j] + id];
-                v.pat = ms.n > 0 ? acc / ms.dtot * ms.k : T(0);
+                T acc[G];
+#pragma unroll
+                for (int k = 0; k < G; ++k) acc[k] = T(0);
+                for (int j = 0; j < ms.n; ++j) {
+                    T m[G];
+                    ld_rows<G, false>(a.rain_maps + ms.off[j], id, m);
+#pragma unroll
+                    for (int k = 0; k < G; ++k) acc[k] = acc[k] + ms.w[j] * m[k];
+                }
+#pragma unroll
+                for (int k = 0; k < G; ++k) v.pat[k] = ms.n > 0 ? acc[k] / ms.dtot * ms.k : T(0);
             } else if (F.pat) {
-                v.pat = a.rain_pattern[id];
+                ld_rows<G, OFF32>(a.rain_pattern, id, v.pat);
             }
-            if (F.gauge) v.gauge = a.inflow_gauge[id];
+            if (F.gauge) ld_rows_u8<G, OFF32>(a.inflow_gauge, id, v.gauge);
             if (F.ga) {
-                v.I = I_src[id];
-                v.Tc = Tc_src[id];
-                if (a.soil_class) v.soil = a.soil_class[id];
+                ld_rows<G, OFF32>(I_src, id, v.I);
+                ld_rows<G, OFF32>(Tc_src, id, v.Tc);
+                if (a.soil_class) ld_rows_u8<G, OFF32>(a.soil_class, id, v.soil);
             }
         }
     }
@@ -230,10 +319,14 @@ __device__ __forceinline__ ColLoad<T> load_column(const LiArgs<T> &a, const Feat
 }
 
 // One workgroup's tile of one grid: `logical` counts the grid's tiles, row
-// blocks fastest.
-template <typename T, int MODE, bool CRIT, int FEAT>
+// blocks fastest.  A wave's slab is kWave*G rows, of which the outer kHalo on
+// each side are halo.  OFF32: every per-cell array is under 4 GiB, so element
+// addresses are 32-bit byte offsets from the array bases.
+template <typename T, int MODE, bool CRIT, int FEAT, int G, bool OFF32>
 __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logical) {
+    static_assert(kHalo % G == 0, "the rows of a lane are all halo or all owned");
     constexpr bool f64 = sizeof(T) == 8;
+    constexpr int kOut = kWave * G - 2 * kHalo;  // rows finalised per slab
     const Feats F = feats_of<FEAT>(a);
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
@@ -308,9 +401,11 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     }
 
     const T A = a.cell_area;
-    const int r = slab * kRowsOut - kHalo + lane;
-    const bool row_in = (r >= 0) && (r < ny);
-    const bool row_own = row_in && lane >= kHalo && lane < kWave - kHalo;
+    // rows r0 .. r0+G-1 of the lane; the row count is a multiple of G, so they
+    // are all inside the grid or all outside
+    const int r0 = slab * kOut - kHalo + lane * G;
+    const bool row_in = (r0 >= 0) && (r0 < ny);
+    const bool row_own = row_in && lane * G >= kHalo && lane * G < kWave * G - kHalo;
     const int c0 = by * a.tj;
     const int c1 = min(c0 + a.tj, nx);
     const T avail_den = dmax(dt_s, eps_of<T>());
@@ -319,306 +414,424 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     const T k_cont = T(1000) * frcp(a.C_a);           // FP32 path only
 
     // pipeline registers (suffix = column lag behind the load column c)
-    T h1 = 0, z1 = 0, n21 = 0, qxp1 = 0, d1 = 0, n2s1 = 0;   // column c-1
-    T rawQx2 = 0, rawQy1 = 0, rawQy2 = 0, Hfx2 = 0, Hfy1 = 0, Hfy2 = 0;
-    T d2 = 0, scale2 = 0, finQx3 = 0, ox3 = 0, n2s2 = 0;
+    T h1[G], z1[G], n21[G], qxp1[G], d1[G], n2s1[G];   // column c-1
+    T rawQx2[G], rawQy1[G], rawQy2[G], Hfx2[G], Hfy1[G], Hfy2[G];
+    T d2[G], scale2[G], finQx3[G], ox3[G], n2s2[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        h1[k] = z1[k] = n21[k] = qxp1[k] = d1[k] = n2s1[k] = T(0);
+        rawQx2[k] = rawQy1[k] = rawQy2[k] = Hfx2[k] = Hfy1[k] = Hfy2[k] = T(0);
+        d2[k] = scale2[k] = finQx3[k] = ox3[k] = n2s2[k] = T(0);
+    }
 
     // per-lane reductions
     T vkey = T(NAN), wkey = T(NAN), rmax = T(NAN), dtmax = T(NAN), dpmax = T(NAN), dnmax = T(NAN);
     double s_store = 0.0, s_uz = 0.0, s_inf = 0.0;
     int neg = 0;
 
-    // 32-bit element offsets (grids are limited to 2^32 cells at creation)
-    ColLoad<T> nxt = load_column<T, MODE>(a, F, d_src, qx_src, qy_src, I_src, Tc_src,
-                                          active && row_in && c0 - 2 >= 0 && c0 - 2 < nx,
-                                          (unsigned)r + (unsigned)(c0 - 2) * (unsigned)ny, ms);
-    const int c_end = active ? c1 + 1 : c0 - 3;  // inactive waves skip the march
-#pragma clang loop unroll_count(2)
-    for (int c = c0 - 2; c <= c_end; ++c) {
+    // 32-bit element offsets (grids are limited to 2^32 cells at creation).
+    // The loads run PF columns ahead of the compute (two at one row per lane:
+    // a wave then keeps two columns of loads in flight), rotating through
+    // PF + 1 buffers.
+    constexpr int PF = G == 1 ? 2 : 1;
+    ColLoad<T, G> buf0 = load_column<T, MODE, G, OFF32>(
+        a, F, d_src, qx_src, qy_src, I_src, Tc_src, active && row_in && c0 - 2 >= 0 && c0 - 2 < nx,
+        (unsigned)r0 + (unsigned)(c0 - 2) * (unsigned)ny, ms);
+    ColLoad<T, G> buf1, buf2;
+    if constexpr (PF == 2)
+        buf1 = load_column<T, MODE, G, OFF32>(
+            a, F, d_src, qx_src, qy_src, I_src, Tc_src, active && row_in && c0 - 1 >= 0 && c0 - 1 < nx,
+            (unsigned)r0 + (unsigned)(c0 - 1) * (unsigned)ny, ms);
+
+    // One column of the march: compute column c from `cur` while the loads of
+    // column c + PF go into `nxt`; output column o = c - 2.  STAGE 0 is the
+    // first column of the march (c0 - 2): only its depth after the rain and
+    // infiltration enters the pipeline; STAGE 1 (c0 - 1) adds the raw faces
+    // that the donor scale of c0 - 1 needs; STAGE 2 is the full column.
+    auto column = [&](const int c, auto stage, const ColLoad<T, G> &cur, ColLoad<T, G> &nxt) {
+        constexpr int STAGE = decltype(stage)::value;
         const bool col_in = (c >= 0) && (c < nx);
         const bool cell_in = row_in && col_in;
-        const unsigned id = (unsigned)r + (unsigned)c * (unsigned)ny;
-        const ColLoad<T> cur = nxt;
+        const unsigned id = (unsigned)r0 + (unsigned)c * (unsigned)ny;
         // issue the next column's loads before computing this one
-        if (c + 1 <= c1 + 1)
-            nxt = load_column<T, MODE>(a, F, d_src, qx_src, qy_src, I_src, Tc_src,
-                                       row_in && c + 1 >= 0 && c + 1 < nx, id + (unsigned)ny, ms);
-        const T zc = cur.z, n2c = cur.n2, qxc = cur.qx, qyc = cur.qy;
-        T dc = cur.d;
-        // cells outside the domain: NaN elevation (FP64) or NaN n^2 (FP32)
-        const bool nan_cell = f64 ? isnan(zc) : isnan(n2c);
-
+        if (c + PF <= c1 + 1)
+            nxt = load_column<T, MODE, G, OFF32>(a, F, d_src, qx_src, qy_src, I_src, Tc_src,
+                                                 row_in && c + PF >= 0 && c + PF < nx,
+                                                 id + (unsigned)(PF * ny), ms,
+                                                 F.dmax && !fail_step && row_own && c + PF - 2 >= c0 &&
+                                                     c + PF - 2 < c1);
+        T dc[G], hc[G];
         // ---- rainfall + Green-Ampt prologue: Hydrological_Model.m:15-55 ----
-        if (MODE == kModeSim && cell_in) {
-            const T dprev = dc;
-            T eff = F.maps ? cur.pat : (F.pat ? cur.pat * rain : rain);
-            if (nan_cell) eff = T(NAN);
-            T dd = dc + eff;
+        {
             const bool own = row_own && c >= c0 && c < c1;
-            if (F.ga) {
-                // legacy explicit Green-Ampt closure in the Infiltration_Module slot
-                // (HydroPol2D_Main_While_Ensemble.m:243-298)
-                T ksat, psi, dth, Lu_mm, k_out, Tr;
-                if (f64) {
-                    const GaClass g = a.ga[cur.soil];
-                    ksat = T(g.ksat); psi = T(g.psi); dth = T(g.dtheta);
-                    Lu_mm = T(g.Lu_mm); k_out = T(g.k_out); Tr = T(g.Tr);
-                } else {
-                    const float4 g0 = ga_lds[2 * cur.soil];
-                    const float4 g1 = ga_lds[2 * cur.soil + 1];
-                    ksat = g0.x; psi = g0.y; dth = g0.z; Lu_mm = g0.w; k_out = g1.x; Tr = g1.y;
-                }
-                const T Imin = a.I_min;
-                const T Ip = cur.I;
-                T Tn = cur.Tc;
-                const T i_a = f64 ? dd / dmax(dt_h, T(1e-12)) : dd * inv_dt_h;
-                T C = f64 ? ksat * (T(1) + ((dd + psi) * dth) / Ip)
-                          : ksat * (T(1) + ((dd + psi) * dth) * frcp(Ip));
-                T f, In;
-                if (kstep == 1) {
-                    f = dmin(C, i_a);
-                    In = dmax(Ip + f * dt_h, Imin);
-                    Tn = Tr;
-                } else {
-                    if (Ip > Lu_mm) C = ksat;
-                    const T idxC = (i_a <= C) ? T(1) : T(0);
-                    Tn = Tn - dt_h;
-                    if (i_a > C) Tn = Tr;
-                    f = dmin(C, i_a);
-                    In = dmax(Ip + f * dt_h - k_out * idxC * dt_h, Imin);
-                    if (Tn < T(0)) In = Imin;
-                }
-                if (nan_cell) {
-                    In = T(NAN);
-                    f = T(NAN);
-                }
-                dd = dd - f * dt_h;
-                if (own) {
-                    I_dst[id] = In;
-                    Tc_dst[id] = Tn;
-                    if (!fail_step) {
-                        if (!isnan(f)) s_inf += (double)(f * dt_h);
-                        if (may_report && !isnan(In)) s_uz += (double)In;
+            T In_k[G], Tn_k[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                dc[k] = cur.d[k];
+                // cells outside the domain: NaN elevation (FP64) or NaN n^2 (FP32)
+                const bool nan_cell = f64 ? isnan(cur.z[k]) : isnan(cur.n2[k]);
+                In_k[k] = Tn_k[k] = T(0);
+                if (MODE == kModeSim && cell_in) {
+                    const T dprev = dc[k];
+                    T eff = F.maps ? cur.pat[k] : (F.pat ? cur.pat[k] * rain : rain);
+                    if (nan_cell) eff = T(NAN);
+                    T dd = dc[k] + eff;
+                    if (F.ga) {
+                        // legacy explicit Green-Ampt closure in the Infiltration_Module slot
+                        // (HydroPol2D_Main_While_Ensemble.m:243-298)
+                        T ksat, psi, dth, Lu_mm, k_out, Tr;
+                        if (f64) {
+                            const GaClass g = a.ga[cur.soil[k]];
+                            ksat = T(g.ksat); psi = T(g.psi); dth = T(g.dtheta);
+                            Lu_mm = T(g.Lu_mm); k_out = T(g.k_out); Tr = T(g.Tr);
+                        } else {
+                            const float4 g0 = ga_lds[2 * cur.soil[k]];
+                            const float4 g1 = ga_lds[2 * cur.soil[k] + 1];
+                            ksat = g0.x; psi = g0.y; dth = g0.z; Lu_mm = g0.w; k_out = g1.x; Tr = g1.y;
+                        }
+                        const T Imin = a.I_min;
+                        const T Ip = cur.I[k];
+                        T Tn = cur.Tc[k];
+                        const T i_a = f64 ? dd / dmax(dt_h, T(1e-12)) : dd * inv_dt_h;
+                        T C = f64 ? ksat * (T(1) + ((dd + psi) * dth) / Ip)
+                                  : ksat * fmadd((dd + psi) * dth, frcp(Ip), T(1));
+                        T f, In;
+                        if (kstep == 1) {
+                            f = dmin(C, i_a);
+                            In = dmax(f64 ? Ip + f * dt_h : fmadd(f, dt_h, Ip), Imin);
+                            Tn = Tr;
+                        } else {
+                            if (Ip > Lu_mm) C = ksat;
+                            const T idxC = (i_a <= C) ? T(1) : T(0);
+                            Tn = Tn - dt_h;
+                            if (i_a > C) Tn = Tr;
+                            f = dmin(C, i_a);
+                            In = dmax(f64 ? Ip + f * dt_h - k_out * idxC * dt_h
+                                          : fmadd(f, dt_h, Ip) - k_out * idxC * dt_h,
+                                      Imin);
+                            if (Tn < T(0)) In = Imin;
+                        }
+                        if (nan_cell) {
+                            In = T(NAN);
+                            f = T(NAN);
+                        }
+                        dd = f64 ? dd - f * dt_h : fmadd(-f, dt_h, dd);
+                        In_k[k] = In;
+                        Tn_k[k] = Tn;
+                        if (own && !fail_step) {
+                            if (!isnan(f)) s_inf += (double)(f * dt_h);
+                            if (may_report && !isnan(In)) s_uz += (double)In;
+                        }
+                    }
+                    // Hydrological_Model.m:30: the inflow hydrographs on their gauge cells
+                    if (F.gauge && cur.gauge[k] > 0) dd = dd + T(a.st->inflow_agg[cur.gauge[k] - 1]);
+                    if (dd < T(-1e-8)) neg = 1;
+                    if (dd < T(1e-6)) dd = T(0);  // Hydrological_Model.m:49
+                    if (dd < T(0)) dd = T(0);     // :52
+                    dc[k] = dd;
+                    if (own && (fail_step || a.track_dmax)) {  // rollback celerity inputs (HydroPol2D_Main_While.m:932)
+                        dtmax = dmax(dtmax, dc[k]);
+                        dpmax = dmax(dpmax, dprev);
                     }
                 }
+                hc[k] = dmax(mm2m(dc[k]), T(0));  // depth_cell (:31)
             }
-            // Hydrological_Model.m:30: the inflow hydrographs on their gauge cells
-            if (F.gauge && cur.gauge > 0) dd = dd + T(a.st->inflow_agg[cur.gauge - 1]);
-            if (dd < T(-1e-8)) neg = 1;
-            if (dd < T(1e-6)) dd = T(0);  // Hydrological_Model.m:49
-            if (dd < T(0)) dd = T(0);     // :52
-            dc = dd;
-            if (own && (fail_step || a.track_dmax)) {  // rollback celerity inputs (HydroPol2D_Main_While.m:932)
-                dtmax = dmax(dtmax, dc);
-                dpmax = dmax(dpmax, dprev);
+            if (MODE == kModeSim && F.ga && cell_in && own) {
+                st_rows<G, OFF32>(I_dst, id, In_k);
+                st_rows<G, OFF32>(Tc_dst, id, Tn_k);
             }
         }
-        const T hc = dmax(mm2m(dc), T(0));  // depth_cell (:31)
 
         // ---- raw y-face flux at (r,c): face between rows r-1 and r ----
-        const T h_up = lane_up(hc);
-        const T z_up = f64 ? lane_up(zc) : cur.zy;  // FP32: dz = z(r-1) - z(r)
-        T rawQyc = 0, Hfyc = 0;
-        if (cell_in && r >= 1)
-            rawQyc = face_flux<T, CRIT>(qyc, zc, z_up, hc, h_up, fabs(n2c), dt_s, a, Hfyc);
-
-        // ---- raw x-face flux at (r,c-1): face between columns c-1 and c ----
-        T rawQx1 = 0, Hfx1 = 0;
-        // (FP32: z1 is dz = z(c) - z(c-1), stored with column c-1)
-        if (row_in && c - 1 >= 0 && c - 1 < nx - 1)
-            rawQx1 = face_flux<T, CRIT>(qxp1, z1, f64 ? zc : z1, h1, hc, fabs(n21), dt_s, a, Hfx1);
-
-        // ---- donor scale of cell (r,c-1): Local_Inertial_Model_D4.m:814-829 ----
-        const T qy1_dn = lane_down(rawQy1);
-        T outr = T(0) + dmax(rawQx1, T(0));
-        if (c - 1 >= 1) outr += dmax(-rawQx2, T(0));
-        outr += dmax(rawQy1, T(0));
-        if (r < ny - 1) outr += dmax(-qy1_dn, T(0));
-        const T avail = f64 ? dmax(d1 / T(1000), T(0)) * A / avail_den
-                            : dmax(d1 * T(1e-3), T(0)) * k_avail;
-        T s1 = T(1);
-        if (outr > avail && outr > T(0)) s1 = f64 ? avail / outr : avail * frcp(outr);
-        if (!isfinite(s1)) s1 = T(0);
-        s1 = dmax(dmin(s1, T(1)), T(0));
-
-        // ---- limited fluxes of output column o = c-2 (:831-857) ----
-        const int o = c - 2;
-        // donor-scaled with a select of the multiplier (a zero flux is never
-        // scaled, as the masked assignments of :834-851 leave it untouched)
-        T fqx = rawQx2 * ((rawQx2 > T(0)) ? scale2 : ((rawQx2 < T(0)) ? s1 : T(1)));
-        if (o == nx - 1) fqx = T(0);
-        if (!isfinite(fqx)) fqx = T(0);
-        const T s2_up = lane_up(scale2);
-        T fqy = rawQy2 * ((rawQy2 > T(0)) ? scale2 : ((rawQy2 < T(0)) ? s2_up : T(1)));
-        if (r == 0) fqy = T(0);
-        if (!isfinite(fqy)) fqy = T(0);
-        const T fqy_dn = lane_down(fqy);
-
-        // ---- continuity (:313-315, :390, :415) ----
-        const T west = (o >= 1) ? finQx3 : T(0);
-        const T south = (r < ny - 1) ? fqy_dn : T(0);
-        const T vol = dt_s * (west - fqx - fqy + south);
-        T dn = f64 ? d2 + T(1000) * vol / a.C_a : d2 + vol * k_cont;
-        if (dn < T(0)) dn = T(0);
-
-        const bool out_cell = row_own && o >= c0 && o < c1;
-        const unsigned oid = (unsigned)r + (unsigned)o * (unsigned)ny;
-        // ---- outlet (:418-498) ----
-        T hf3 = T(0), qo = T(0);
-        if (out_cell && signbit(n2s2) && a.n_outlet > 0) {
-            const int slot = find_outlet(a.out_idx, a.n_outlet, (long long)oid);
-            if (slot >= 0) {
-                const T h_out = dmax(dn, T(0)) / T(1000);
-                T sqrtS;
-                if (a.outlet_type == 1) {
-                    sqrtS = a.sqrtS_type1;
-                } else {
-                    const T h_safe = dmax(h_out, T(1e-12));
-                    sqrtS = sqrt(T(9.81) * fabs(n2s2) * powm13(h_safe));
-                    if (!isfinite(sqrtS)) sqrtS = T(0);
-                }
-                hf3 = h_out;
-                qo = (T(1) / a.out_n[slot]) * a.R * pow53(h_out) * sqrtS / A * T(1000) * T(3600);
-                const T qmax = dmax(dn - T(1000) * T(0), T(0)) / dt_h;
-                qo = dmin(qo, qmax);
-                if (!isfinite(qo)) qo = T(0);
-                dn = dn - qo * dt_h;
-                if (a.out_q) a.out_q[slot] = qo;
-                if (a.out_d) a.out_d[slot] = dn;
+        T rawQyc[G], Hfyc[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) rawQyc[k] = Hfyc[k] = T(0);
+        if constexpr (STAGE >= 1) {
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const T h_up = up_of<G>(hc, k);
+                const T z_up = f64 ? up_of<G>(cur.z, k) : cur.zy[k];  // FP32: dz = z(r-1) - z(r)
+                if (cell_in && r0 + k >= 1)
+                    rawQyc[k] = face_flux<T, CRIT>(cur.qy[k], cur.z[k], z_up, hc[k], h_up,
+                                                   fabs(cur.n2[k]), dt_s, a, Hfyc[k]);
             }
         }
-        const T ox = q_to_mmh(fqx, a);
-        const T oy = q_to_mmh(fqy, a);
 
-        if (MODE == kModeSim) {
-            // ---- refreshing_timestep.m:63-238 per-cell terms ----
-            const T tol = a.depth_tol_mm;
-            T vu;
-            if (f64) {
-                const T f2 = Hfy2 * T(1000);
-                const T s2d = (f2 < tol) ? T(1e12) : f2;
-                vu = (oy / T(1000) / T(3600)) * a.R / (s2d / T(1000));
-            } else {
-                const T hy = (Hfy2 * T(1000) < tol) ? T(1e9) : Hfy2;
-                vu = fqy * a.inv_R * frcp(hy);
+        // ---- raw x-face flux at (r,c-1): face between columns c-1 and c ----
+        T rawQx1[G], Hfx1[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) rawQx1[k] = Hfx1[k] = T(0);
+        if constexpr (STAGE >= 1) {
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                // (FP32: z1 is dz = z(c) - z(c-1), stored with column c-1)
+                if (row_in && c - 1 >= 0 && c - 1 < nx - 1)
+                    rawQx1[k] = face_flux<T, CRIT>(qxp1[k], z1[k], f64 ? cur.z[k] : z1[k], h1[k],
+                                                   hc[k], fabs(n21[k]), dt_s, a, Hfx1[k]);
             }
-            const T vu_dn = lane_down(vu);
-            if (out_cell && !fail_step) {
-                T vr, dts;
-                if (f64) {
-                    const T f1 = Hfx2 * T(1000), f2 = Hfy2 * T(1000), f3 = hf3 * T(1000);
-                    dts = dmax(dmax(f1 / T(1000), f2 / T(1000)), f3 / T(1000));
-                    const T s1d = (f1 < tol) ? T(1e12) : f1;
-                    vr = (ox / T(1000) / T(3600)) * a.R / (s1d / T(1000));
-                } else {
-                    dts = dmax(dmax(Hfx2, Hfy2), hf3);
-                    const T hx = (Hfx2 * T(1000) < tol) ? T(1e9) : Hfx2;
-                    vr = fqx * a.inv_R * frcp(hx);
-                }
-                const T vl = (o >= 1) ? -vr : -T(0);  // :73 unshifted quirk
-                const T vd = (r < ny - 1) ? -vu_dn : -T(0);
-                // velocity_raster only enters dt (non-adaptive) and the rollback celerity
-                const T raster = a.adaptive ? T(0)
-                                            : dmax(dmax(dmax(fabs(vl), fabs(vr)), fabs(vu)), fabs(vd));
-                const T rc = T(0.5) * (vr - vl), uc = T(0.5) * (vu - vd);
-                bool stagnant;
-                T vtot = T(0);  // velocities.total_velocity (Human_Instability_Module.m:9)
-                if (f64) {
-                    const T tot = hypot(rc, uc);
-                    vkey = dmax(vkey, tot);
-                    stagnant = tot < T(0.01);
-                    vtot = tot;
-                } else {
-                    const T tot2 = rc * rc + uc * uc;  // compared squared, rooted once per step
-                    vkey = dmax(vkey, tot2);
-                    stagnant = tot2 < T(1e-4);
-                    if (F.risk || may_snap) vtot = sqrt(tot2);
-                }
-                if (F.risk) {
-                    // Human_Instability_Module.m:8-25 and its running maximum
-                    const T h = mm2m(dn);
-                    const T F_person = a.hi_weight * a.hi_g;
-                    const T F_buoy = (a.hi_w1 * a.hi_w2) * h * a.hi_ro * a.hi_g;
-                    const T avail = a.hi_mu * (dmax(F_person - F_buoy, T(0)));
-                    const T hydro = dmax(T(1) / T(2) * (a.hi_Cd * a.hi_ro * a.hi_w1 * h * (vtot * vtot)), T(0));
-                    T risk = T(0);
-                    if (avail > T(0)) risk = hydro / avail;
-                    else if (h > T(0)) risk = T(1);
-                    risk = dmin(dmax(risk, T(0)), T(1));
-                    if (isnan(dn)) risk = T(NAN);
-                    const T prev = (kstep == 1) ? (isnan(dn) ? T(NAN) : T(0)) : a.max_risk[oid];
-                    const T nm = dmax(risk, prev);
-                    if (kstep == 1 || nm != prev) a.max_risk[oid] = nm;
-                }
-                if (may_snap) {
-                    a.snap_vras[oid] = dmax(dmax(dmax(fabs(vl), fabs(vr)), fabs(vu)), fabs(vd));
-                    a.snap_vtot[oid] = vtot;
-                }
-                if (!a.adaptive) rmax = dmax(rmax, raster);
-                if (!stagnant) {
-                    if (a.adaptive) {
-                        wkey = dmax(wkey, dts);  // min(alfa*dx/c) = alfa*dx/c(max depth)
-                    } else {
-                        T cw = sqrt(T(9.81) * dmax(dts, T(0)));
-                        cw = dmax(cw, T(1e-10));
-                        if (!isinf(cw)) wkey = dmin(wkey, a.alfa_min * a.R / (raster + cw));
+        }
+
+        // ---- donor scale of cell (r,c-1): Local_Inertial_Model_D4.m:814-829 ----
+        T s1[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) s1[k] = T(0);
+        if constexpr (STAGE >= 2) {
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const T qy1_dn = down_of<G>(rawQy1, k);
+                T outr = T(0) + dmax(rawQx1[k], T(0));
+                if (c - 1 >= 1) outr += dmax(-rawQx2[k], T(0));
+                outr += dmax(rawQy1[k], T(0));
+                if (r0 + k < ny - 1) outr += dmax(-qy1_dn, T(0));
+                const T avail = f64 ? dmax(d1[k] / T(1000), T(0)) * A / avail_den
+                                    : dmax(d1[k] * T(1e-3), T(0)) * k_avail;
+                T sc = T(1);
+                if (outr > avail && outr > T(0)) sc = f64 ? avail / outr : avail * frcp(outr);
+                if (!isfinite(sc)) sc = T(0);
+                s1[k] = dmax(dmin(sc, T(1)), T(0));
+            }
+        }
+
+        T fqx[G], fqy[G], ox[G], oy[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) fqx[k] = fqy[k] = ox[k] = oy[k] = T(0);
+        if constexpr (STAGE >= 2) {
+            // ---- limited fluxes of output column o = c-2 (:831-857) ----
+            const int o = c - 2;
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                // donor-scaled with a select of the multiplier (a zero flux is never
+                // scaled, as the masked assignments of :834-851 leave it untouched)
+                fqx[k] = rawQx2[k] * ((rawQx2[k] > T(0)) ? scale2[k] : ((rawQx2[k] < T(0)) ? s1[k] : T(1)));
+                if (o == nx - 1) fqx[k] = T(0);
+                if (!isfinite(fqx[k])) fqx[k] = T(0);
+                const T s2_up = up_of<G>(scale2, k);
+                fqy[k] = rawQy2[k] * ((rawQy2[k] > T(0)) ? scale2[k] : ((rawQy2[k] < T(0)) ? s2_up : T(1)));
+                if (r0 + k == 0) fqy[k] = T(0);
+                if (!isfinite(fqy[k])) fqy[k] = T(0);
+            }
+
+            // ---- continuity (:313-315, :390, :415) ----
+            T dn[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const T fqy_dn = down_of<G>(fqy, k);
+                const T west = (o >= 1) ? finQx3[k] : T(0);
+                const T south = (r0 + k < ny - 1) ? fqy_dn : T(0);
+                const T vol = dt_s * (west - fqx[k] - fqy[k] + south);
+                dn[k] = f64 ? d2[k] + T(1000) * vol / a.C_a : fmadd(vol, k_cont, d2[k]);
+                if (dn[k] < T(0)) dn[k] = T(0);
+            }
+
+            const bool out_cell = row_own && o >= c0 && o < c1;
+            const unsigned oid = (unsigned)r0 + (unsigned)o * (unsigned)ny;
+            // ---- outlet (:418-498) ----
+            T hf3[G], qo[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                hf3[k] = qo[k] = T(0);
+                if (out_cell && signbit(n2s2[k]) && a.n_outlet > 0) {
+                    const int slot = find_outlet(a.out_idx, a.n_outlet, (long long)(oid + k));
+                    if (slot >= 0) {
+                        const T h_out = dmax(dn[k], T(0)) / T(1000);
+                        T sqrtS;
+                        if (a.outlet_type == 1) {
+                            sqrtS = a.sqrtS_type1;
+                        } else {
+                            const T h_safe = dmax(h_out, T(1e-12));
+                            sqrtS = sqrt(T(9.81) * fabs(n2s2[k]) * powm13(h_safe));
+                            if (!isfinite(sqrtS)) sqrtS = T(0);
+                        }
+                        hf3[k] = h_out;
+                        T q = (T(1) / a.out_n[slot]) * a.R * pow53(h_out) * sqrtS / A * T(1000) * T(3600);
+                        const T qmax = dmax(dn[k] - T(1000) * T(0), T(0)) / dt_h;
+                        q = dmin(q, qmax);
+                        if (!isfinite(q)) q = T(0);
+                        dn[k] = dn[k] - q * dt_h;
+                        qo[k] = q;
+                        if (a.out_q) a.out_q[slot] = q;
+                        if (a.out_d) a.out_d[slot] = dn[k];
                     }
                 }
-                if (may_report && !isnan(dn)) s_store += (double)dn;
-                if (a.track_dmax) dnmax = dmax(dnmax, dn);
+                ox[k] = q_to_mmh(fqx[k], a);
+                oy[k] = q_to_mmh(fqy[k], a);
             }
-            if (out_cell) {
-                d_dst[oid] = dn;
-                qx_dst[oid] = ox;
-                qy_dst[oid] = oy;
-                if (F.dmax && !fail_step) {
-                    const T prevmax = a.dmax_map[oid];
-                    const T newmax = (kstep == 1) ? dn : dmax(dn, prevmax);
-                    // stored only when it changes (a NaN old value always differs):
-                    // a cell that is not rising costs no write
-                    if (kstep == 1 || newmax != prevmax) a.dmax_map[oid] = newmax;
-                    // time to peak: end time of the step that last raised the maximum
-                    if (F.tpeak && (kstep == 1 || dn > prevmax)) a.tpeak_map[oid] = t_end_min;
+
+            if (MODE == kModeSim) {
+                // ---- refreshing_timestep.m:63-238 per-cell terms ----
+                const T tol = a.depth_tol_mm;
+                T vu[G];
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    if (f64) {
+                        const T f2 = Hfy2[k] * T(1000);
+                        const T s2d = (f2 < tol) ? T(1e12) : f2;
+                        vu[k] = (oy[k] / T(1000) / T(3600)) * a.R / (s2d / T(1000));
+                    } else {
+                        const T hy = (Hfy2[k] * T(1000) < tol) ? T(1e9) : Hfy2[k];
+                        vu[k] = fqy[k] * a.inv_R * frcp(hy);
+                    }
                 }
-            }
-        } else {
-            const T oy_dn = lane_down(oy);
-            if (out_cell) {
-                d_dst[oid] = dn;
-                if (qx_dst) qx_dst[oid] = ox;
-                if (qy_dst) qy_dst[oid] = oy;
-                if (a.q3) a.q3[oid] = qo;
-                if (a.hfx) a.hfx[oid] = Hfx2;
-                if (a.hfy) a.hfy[oid] = Hfy2;
-                if (a.hf3) a.hf3[oid] = hf3;
-                if (a.outlet_flow) a.outlet_flow[oid] = qo;
-                if (a.qright) a.qright[oid] = ox;
-                if (a.qup) a.qup[oid] = oy;
-                if (a.qleft) a.qleft[oid] = -((o >= 1) ? ox3 : T(0));
-                if (a.qdown) a.qdown[oid] = -((r < ny - 1) ? oy_dn : T(0));
-                if (a.itot)
-                    a.itot[oid] = fabs(ox + oy + qo) * dt_s / T(1000) * T(1) / T(3600) * a.R2;
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    const T vu_dn = down_of<G>(vu, k);
+                    if (out_cell && !fail_step) {
+                        T vr, dts;
+                        if (f64) {
+                            const T f1 = Hfx2[k] * T(1000), f2 = Hfy2[k] * T(1000), f3 = hf3[k] * T(1000);
+                            dts = dmax(dmax(f1 / T(1000), f2 / T(1000)), f3 / T(1000));
+                            const T s1d = (f1 < tol) ? T(1e12) : f1;
+                            vr = (ox[k] / T(1000) / T(3600)) * a.R / (s1d / T(1000));
+                        } else {
+                            dts = dmax(dmax(Hfx2[k], Hfy2[k]), hf3[k]);
+                            const T hx = (Hfx2[k] * T(1000) < tol) ? T(1e9) : Hfx2[k];
+                            vr = fqx[k] * a.inv_R * frcp(hx);
+                        }
+                        const T vl = (o >= 1) ? -vr : -T(0);  // :73 unshifted quirk
+                        const T vd = (r0 + k < ny - 1) ? -vu_dn : -T(0);
+                        // velocity_raster only enters dt (non-adaptive) and the rollback celerity
+                        const T raster = a.adaptive ? T(0)
+                                                    : dmax(dmax(dmax(fabs(vl), fabs(vr)), fabs(vu[k])), fabs(vd));
+                        const T rc = T(0.5) * (vr - vl), uc = T(0.5) * (vu[k] - vd);
+                        bool stagnant;
+                        T vtot = T(0);  // velocities.total_velocity (Human_Instability_Module.m:9)
+                        if (f64) {
+                            const T tot = hypot(rc, uc);
+                            vkey = dmax(vkey, tot);
+                            stagnant = tot < T(0.01);
+                            vtot = tot;
+                        } else {
+                            const T tot2 = fmadd(rc, rc, uc * uc);  // compared squared, rooted once per step
+                            vkey = dmax(vkey, tot2);
+                            stagnant = tot2 < T(1e-4);
+                            if (F.risk || may_snap) vtot = sqrt(tot2);
+                        }
+                        if (F.risk) {
+                            // Human_Instability_Module.m:8-25 and its running maximum
+                            const T h = mm2m(dn[k]);
+                            const T F_person = a.hi_weight * a.hi_g;
+                            const T F_buoy = (a.hi_w1 * a.hi_w2) * h * a.hi_ro * a.hi_g;
+                            const T avail = a.hi_mu * (dmax(F_person - F_buoy, T(0)));
+                            const T hydro = dmax(T(1) / T(2) * (a.hi_Cd * a.hi_ro * a.hi_w1 * h * (vtot * vtot)), T(0));
+                            T risk = T(0);
+                            if (avail > T(0)) risk = hydro / avail;
+                            else if (h > T(0)) risk = T(1);
+                            risk = dmin(dmax(risk, T(0)), T(1));
+                            if (isnan(dn[k])) risk = T(NAN);
+                            const T prev = (kstep == 1) ? (isnan(dn[k]) ? T(NAN) : T(0)) : a.max_risk[oid + k];
+                            const T nm = dmax(risk, prev);
+                            if (kstep == 1 || nm != prev) a.max_risk[oid + k] = nm;
+                        }
+                        if (may_snap) {
+                            a.snap_vras[oid + k] = dmax(dmax(dmax(fabs(vl), fabs(vr)), fabs(vu[k])), fabs(vd));
+                            a.snap_vtot[oid + k] = vtot;
+                        }
+                        if (!a.adaptive) rmax = dmax(rmax, raster);
+                        if (!stagnant) {
+                            if (a.adaptive) {
+                                wkey = dmax(wkey, dts);  // min(alfa*dx/c) = alfa*dx/c(max depth)
+                            } else {
+                                T cw = sqrt(T(9.81) * dmax(dts, T(0)));
+                                cw = dmax(cw, T(1e-10));
+                                if (!isinf(cw)) wkey = dmin(wkey, a.alfa_min * a.R / (raster + cw));
+                            }
+                        }
+                        if (may_report && !isnan(dn[k])) s_store += (double)dn[k];
+                        if (a.track_dmax) dnmax = dmax(dnmax, dn[k]);
+                    }
+                }
+                if (out_cell) {
+                    st_rows<G, OFF32>(d_dst, oid, dn);
+                    st_rows<G, OFF32>(qx_dst, oid, ox);
+                    st_rows<G, OFF32>(qy_dst, oid, oy);
+                    if (F.dmax && !fail_step) {
+                        const T(&prevmax)[G] = cur.dm;  // fetched with the column two ahead
+#pragma unroll
+                        for (int k = 0; k < G; ++k) {
+                            const T newmax = (kstep == 1) ? dn[k] : dmax(dn[k], prevmax[k]);
+                            // stored only when it changes (a NaN old value always differs):
+                            // a cell that is not rising costs no write
+                            if (kstep == 1 || newmax != prevmax[k]) *elem_ptr<OFF32>(a.dmax_map, oid + k) = newmax;
+                            // time to peak: end time of the step that last raised the maximum
+                            if (F.tpeak && (kstep == 1 || dn[k] > prevmax[k])) a.tpeak_map[oid + k] = t_end_min;
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    const T oy_dn = down_of<G>(oy, k);
+                    if (out_cell) {
+                        const unsigned q = oid + (unsigned)k;
+                        d_dst[q] = dn[k];
+                        if (qx_dst) qx_dst[q] = ox[k];
+                        if (qy_dst) qy_dst[q] = oy[k];
+                        if (a.q3) a.q3[q] = qo[k];
+                        if (a.hfx) a.hfx[q] = Hfx2[k];
+                        if (a.hfy) a.hfy[q] = Hfy2[k];
+                        if (a.hf3) a.hf3[q] = hf3[k];
+                        if (a.outlet_flow) a.outlet_flow[q] = qo[k];
+                        if (a.qright) a.qright[q] = ox[k];
+                        if (a.qup) a.qup[q] = oy[k];
+                        if (a.qleft) a.qleft[q] = -((o >= 1) ? ox3[k] : T(0));
+                        if (a.qdown) a.qdown[q] = -((r0 + k < ny - 1) ? oy_dn : T(0));
+                        if (a.itot)
+                            a.itot[q] = fabs(ox[k] + oy[k] + qo[k]) * dt_s / T(1000) * T(1) / T(3600) * a.R2;
+                    }
+                }
             }
         }
 
         // ---- shift the pipeline ----
-        finQx3 = fqx;
-        ox3 = ox;
-        rawQx2 = rawQx1;
-        rawQy2 = rawQy1;
-        rawQy1 = rawQyc;
-        Hfx2 = Hfx1;
-        Hfy2 = Hfy1;
-        Hfy1 = Hfyc;
-        scale2 = s1;
-        d2 = d1;
-        d1 = dc;
-        h1 = hc;
-        z1 = zc;
-        n21 = n2c;
-        n2s2 = n2s1;
-        n2s1 = n2c;
-        qxp1 = qxc;
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            finQx3[k] = fqx[k];
+            ox3[k] = ox[k];
+            rawQx2[k] = rawQx1[k];
+            rawQy2[k] = rawQy1[k];
+            rawQy1[k] = rawQyc[k];
+            Hfx2[k] = Hfx1[k];
+            Hfy2[k] = Hfy1[k];
+            Hfy1[k] = Hfyc[k];
+            scale2[k] = s1[k];
+            d2[k] = d1[k];
+            d1[k] = dc[k];
+            h1[k] = hc[k];
+            z1[k] = cur.z[k];
+            n21[k] = cur.n2[k];
+            n2s2[k] = n2s1[k];
+            n2s1[k] = cur.n2[k];
+            qxp1[k] = cur.qx[k];
+        }
+    };
+
+    if (active) {
+        using S0 = std::integral_constant<int, 0>;
+        using S1 = std::integral_constant<int, 1>;
+        using S2 = std::integral_constant<int, 2>;
+        int c = c0;
+        if constexpr (PF == 2) {
+            // three columns per trip, so that the buffers and the pipeline
+            // registers rotate roles instead of being copied
+            column(c0 - 2, S0{}, buf0, buf2);
+            column(c0 - 1, S1{}, buf1, buf0);
+            for (; c + 2 <= c1 + 1; c += 3) {
+                column(c, S2{}, buf2, buf1);
+                column(c + 1, S2{}, buf0, buf2);
+                column(c + 2, S2{}, buf1, buf0);
+            }
+            if (c <= c1 + 1) column(c, S2{}, buf2, buf1);
+            if (c + 1 <= c1 + 1) column(c + 1, S2{}, buf0, buf2);
+        } else {
+            // at two rows per lane the registers of a rotation would lower
+            // the occupancy
+            column(c0 - 2, S0{}, buf0, buf1);
+            column(c0 - 1, S1{}, buf1, buf0);
+            for (; c <= c1 + 1; ++c) {
+                column(c, S2{}, buf0, buf1);
+                buf0 = buf1;
+            }
+        }
     }
 
     if (MODE == kModeSim) {
@@ -659,14 +872,19 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     }
 }
 
-template <typename T, int MODE, bool CRIT, int FEAT>
-__global__ __launch_bounds__(256) void li_step_kernel(LiArgs<T> a) {
-    li_step_tile<T, MODE, CRIT, FEAT>(a, xcd_logical_tile());
+// The simulation kernels specialised to a feature set (FEAT >= 0) address
+// the per-cell arrays with 32-bit byte offsets: their launcher takes them only
+// for arrays under 4 GiB.
+template <typename T, int MODE, bool CRIT, int FEAT, int G>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((G == 1 && sizeof(T) == 4 && FEAT >= 0) ? 5 : 1))) void li_step_kernel(LiArgs<T> a) {
+    constexpr bool kOff32 = MODE == kModeSim && FEAT >= 0;
+    li_step_tile<T, MODE, CRIT, FEAT, G, kOff32>(a, xcd_logical_tile());
 }
 
 // Many grids in one launch (an ensemble): tile t of the launch belongs to grid
 // tile_grid[t] and is that grid's tile t - grid_first_tile[grid].  Every grid
-// keeps its own arguments, state, partials and dt; the block size is shared.
+// keeps its own arguments, state, partials and dt; the block size is shared,
+// and a lane holds one row.
 template <typename T, bool CRIT>
 __global__ __launch_bounds__(256) void li_step_batch_kernel(const LiArgs<T> *__restrict__ args,
                                                             const int *__restrict__ tile_grid,
@@ -676,7 +894,7 @@ __global__ __launch_bounds__(256) void li_step_batch_kernel(const LiArgs<T> *__r
     // read in place: the fields are wave-uniform scalar loads from a read-only
     // table, kept in the scalar cache
     const LiArgs<T> &a = args[g];
-    li_step_tile<T, kModeSim, CRIT, kFeatRuntime>(a, t - grid_first_tile[g]);
+    li_step_tile<T, kModeSim, CRIT, kFeatRuntime, 1, false>(a, t - grid_first_tile[g]);
 }
 
 // The feature mask of a run's arguments.
@@ -692,36 +910,45 @@ template <typename T> int li_feature_mask(const LiArgs<T> &a) {
 // maximum-depth map) get a kernel specialised to that set, which keeps the
 // arrays it does not have out of its code and its scalar registers; any other
 // set runs the kernel that reads the set from the arguments.
-template <typename T, bool CRIT>
+template <typename T, bool CRIT, int G>
 void launch_li_sim(const LiArgs<T> &a, dim3 grid, dim3 block, hipStream_t s) {
+    // (the specialised kernels address with 32-bit byte offsets)
+    const bool off32 = (unsigned long long)a.ny * (unsigned long long)a.nx * sizeof(T) < (1ull << 32);
     if constexpr (sizeof(T) == 4) {
         constexpr int kGP = kFeatGa | kFeatPattern | kFeatDmax, kG = kFeatGa | kFeatDmax;
         constexpr int kP = kFeatPattern | kFeatDmax, kD = kFeatDmax;
-        switch (li_feature_mask(a)) {
-        case kGP: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kGP>), grid, block, 0, s, a); return;
-        case kG: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kG>), grid, block, 0, s, a); return;
-        case kP: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kP>), grid, block, 0, s, a); return;
-        case kD: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kD>), grid, block, 0, s, a); return;
+        switch (off32 ? li_feature_mask(a) : -1) {
+        case kGP: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kGP, G>), grid, block, 0, s, a); return;
+        case kG: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kG, G>), grid, block, 0, s, a); return;
+        case kP: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kP, G>), grid, block, 0, s, a); return;
+        case kD: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kD, G>), grid, block, 0, s, a); return;
         default: break;
         }
     }
-    hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kFeatRuntime>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kFeatRuntime, G>), grid, block, 0, s, a);
+}
+
+template <typename T, int G>
+void launch_li_rows(const LiArgs<T> &a, int mode, dim3 grid, dim3 block, hipStream_t s) {
+    if (mode == kModeSim) {
+        if (a.flag_critical) launch_li_sim<T, true, G>(a, grid, block, s);
+        else launch_li_sim<T, false, G>(a, grid, block, s);
+    } else {
+        // the drop-in mode has none of the simulation's optional arrays
+        if (a.flag_critical)
+            hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn, true, 0, G>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn, false, 0, G>), grid, block, 0, s, a);
+    }
 }
 
 template <typename T>
 hipError_t launch_li_step(const LiArgs<T> &a, int mode, hipStream_t s) {
+    if (!li_rows_ok(a.ny, a.rows) || a.n_slabs != li_n_slabs(a.ny, a.rows)) return hipErrorInvalidValue;
     dim3 block(kWave * a.waves_per_block);
     dim3 grid(((a.n_slabs + a.waves_per_block - 1) / a.waves_per_block) * ((a.nx + a.tj - 1) / a.tj));
-    if (mode == kModeSim) {
-        if (a.flag_critical) launch_li_sim<T, true>(a, grid, block, s);
-        else launch_li_sim<T, false>(a, grid, block, s);
-    } else {
-        // the drop-in mode has none of the simulation's optional arrays
-        if (a.flag_critical)
-            hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn, true, 0>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((li_step_kernel<T, kModeDropIn, false, 0>), grid, block, 0, s, a);
-    }
+    if (a.rows == 2) launch_li_rows<T, 2>(a, mode, grid, block, s);
+    else launch_li_rows<T, 1>(a, mode, grid, block, s);
     return hipGetLastError();
 }
 

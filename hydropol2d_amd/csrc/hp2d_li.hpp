@@ -10,7 +10,8 @@ enum { kModeDropIn = 0, kModeSim = 1 };
 template <typename T> struct LiArgs {
     int ny, nx;
     int tj;               // columns per wave task
-    int n_slabs;          // ceil(ny / kRowsOut)
+    int rows;             // rows per lane (1, or 2 for an even ny)
+    int n_slabs;          // li_n_slabs(ny, rows)
     int waves_per_block;  // 1..4
     // physics constants
     T R, R2, cell_area, C_a;

@@ -63,7 +63,18 @@ class Simulation:
         """Pick the fastest launch shape on this sim's data (does not advance it)."""
         tj, wpb, ms = C.c_int32(), C.c_int32(), C.c_double()
         self._check(self._L.hp2d_sim_autotune(self._s, C.byref(tj), C.byref(wpb), C.byref(ms)))
-        return {"tj": tj.value, "waves_per_block": wpb.value, "ms": ms.value}
+        return dict(self.launch_shape(), ms=ms.value)
+
+    def launch_shape(self) -> dict:
+        """The step kernel's launch shape: columns per wave task, waves per
+        block and grid rows per lane."""
+        tj, wpb, rows = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self._L.hp2d_sim_launch_shape(self._s, C.byref(tj), C.byref(wpb), C.byref(rows)))
+        return {"tj": tj.value, "waves_per_block": wpb.value, "rows": rows.value}
+
+    def set_launch_shape(self, tj: int, waves_per_block: int, rows: int):
+        """Set the step kernel's launch shape (results do not depend on it)."""
+        self._check(self._L.hp2d_sim_set_launch_shape(self._s, int(tj), int(waves_per_block), int(rows)))
 
     def kernel_time(self):
         li, tot, n = C.c_double(), C.c_double(), C.c_int64()

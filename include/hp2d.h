@@ -714,6 +714,13 @@ hp2d_status hp2d_debug_inject_failure(hp2d_sim *s, int64_t step_k, int32_t kind)
  * (device, grid shape, precision) for the process.  Outputs may be NULL. */
 hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t *best_wpb, double *best_ms);
 
+/* The fused step's launch shape (no reference counterpart): columns per wave
+ * task tj (16..4096), waves per block wpb (1, 2 or 4) and grid rows per lane
+ * (1, or 2 when ny is even).  Every shape gives bit-identical results; the
+ * setter refuses a shape outside these limits.  Outputs may be NULL. */
+hp2d_status hp2d_sim_launch_shape(hp2d_sim *s, int32_t *tj, int32_t *wpb, int32_t *rows);
+hp2d_status hp2d_sim_set_launch_shape(hp2d_sim *s, int32_t tj, int32_t wpb, int32_t rows);
+
 /* Average duration [ms] of the fused local-inertial kernel over the last
  * hp2d_sim_run_fixed call, measured with HIP events on the sim's stream. */
 hp2d_status hp2d_sim_kernel_time(hp2d_sim *s, double *li_ms, double *total_ms,

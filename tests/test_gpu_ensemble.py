@@ -20,9 +20,12 @@ def _specs(precision, sizes=(37, 64, 100, 128, 45, 200), minutes=(8.0, 5.0, 10.0
     return out
 
 
-def _solo(spec):
+def _solo(spec, shape):
+    """The member run alone with the member's launch shape (the shape sets the
+    order of the domain sums of the report)."""
     from hydropol2d_amd.simulation import Simulation
     with Simulation(spec) as s:
+        s.set_launch_shape(shape["tj"], shape["waves_per_block"], shape["rows"])
         s.run()
         return s.state(infiltration=True), s.report(), s.hydrograph(), s.dt_history()
 
@@ -38,7 +41,8 @@ def test_members_equal_solo_runs(precision, graph_steps):
         for i, sp in enumerate(specs):
             m = ens.member(i)
             st, rep, hy, dth = m.state(infiltration=True), m.report(), m.hydrograph(), m.dt_history()
-            st0, rep0, hy0, dth0 = _solo(sp)
+            assert m.launch_shape()["rows"] == 1
+            st0, rep0, hy0, dth0 = _solo(sp, m.launch_shape())
             for k in ("d_t", "outflow", "dmax", "I_t"):
                 np.testing.assert_array_equal(st[k], st0[k], err_msg=f"member {i} {k}")
             assert rep == rep0, i
