@@ -16,8 +16,10 @@ Prints one JSON line (value = cell-updates/s of the whole job), with the
 roofline of the dominant kernel, that of the whole step, and the FP64 CPU
 oracle on one thread and on all host cores.  Beside it: the batched-ensemble
 study (members/hour against the plain per-member loop, --ensemble-members)
-the surrogate study (members/hour of the surrogate with its 5 % solver
-judge, --surrogate-train) and the D8 drop-in kernels (--d8-size, --d8-only).
+the surrogate studies (members/hour of the surrogate with its 5 % solver
+judge, --surrogate-train; its skill against the training-mean map and its
+time-to-peak error on wetter members, --surrogate-wet-train) and the D8
+drop-in kernels (--d8-size, --d8-only).
 """
 from __future__ import annotations
 
@@ -59,9 +61,13 @@ def parse():
                    help="time the plain per-member solver on every (members/K)-th member only and "
                         "scale by cell-steps (0: every member)")
     p.add_argument("--surrogate-train", type=int, default=1536, metavar="N",
-                   help="the surrogate study: train on N solved 256 x 256 members, judge 256 "
-                        "more (0 skips it; about 1 min at N = 1536, 2.5 min at N = 4096)")
-    p.add_argument("--surrogate-epochs", type=int, default=40)
+                   help="the surrogate's members/hour: train on N solved 256 x 256 members of "
+                        "the dry family, judge 256 more (0 skips it; about 1 min at N = 1536)")
+    p.add_argument("--surrogate-wet-train", type=int, default=1024, metavar="N",
+                   help="the surrogate's skill: train on N members of the wet family (120 mm/h "
+                        "for 45 min), evaluate 256 more (0 skips it; about 1 min at N = 1024)")
+    p.add_argument("--surrogate-epochs", type=int, default=60,
+                   help="training epochs at batch 64")
     p.add_argument("--d8-size", type=int, default=4096, metavar="N",
                    help="the D8 drop-in study on an N x N grid (0 skips it)")
     p.add_argument("--d8-only", action="store_true",
@@ -150,37 +156,70 @@ def ensemble_throughput(n_members, max_log2=9, plain_sample=0):
                         "n/Ks/rain x e^U(ln .5, ln 1.5) around C1, 2 h each, FP32"}
 
 
-def surrogate_study(n_train, epochs, n_eval=256, n_plain=8):
-    """Members/hour of the surrogate with its 5 % solver judge against the
-    plain per-member solver and the batched ensemble, on 256 x 256 C4-style
-    members (2 h each, FP32).  The training data are the solver's own peak
-    depth and time-to-peak maps of n_train members; the n_eval held-out members
-    give the peak-depth RMSE against the solver and the judge's verdict."""
+def surrogate_study(n_train, epochs, n_eval=256, n_plain=8, wet=False, batch=64):
+    """The surrogate with its 5 % solver judge on 256 x 256 C4-style members
+    (2 h each, FP32).  The training data are the solver's own peak-depth and
+    time-to-peak maps of n_train members; the n_eval held-out members give the
+    peak-depth RMSE against the solver (and against the training-mean map, the
+    forecast without the network), the time-to-peak RMSE on their wet cells
+    (peak depth >= 1 cm) and the judge's verdict at 2 cm.  The dry family (50
+    mm/h for 30 min around the C1 values) sets the members/hour against the
+    plain per-member solver and the batched ensemble; the wet family (120 mm/h
+    for 45 min, RMS peak depth about 11 cm) measures the network's skill."""
     import torch
     from hydropol2d_amd import surrogate as S
     from hydropol2d_amd import synthetic
     from hydropol2d_amd.simulation import Simulation
-    def log(msg):
-        print(f"[bench] surrogate: {msg}", file=sys.stderr, flush=True)
+    tag = "wet" if wet else "dry"
 
-    specs = synthetic.ensemble_c4(n_train + n_eval, seed=5000, min_log2=8, max_log2=8,
-                                  routing_time_min=120.0, vary_soil_moisture=True, filled=True)
+    def log(msg):
+        print(f"[bench] surrogate ({tag}): {msg}", file=sys.stderr, flush=True)
+
+    rain = dict(rain_mmh=120.0, rain_duration_min=45.0) if wet else {}
+    specs = synthetic.ensemble_c4(n_train + n_eval, seed=6000 if wet else 5000, min_log2=8, max_log2=8,
+                                  routing_time_min=120.0, vary_soil_moisture=True, filled=True, **rain)
     pk, tp, t_data = S.solve_members(specs, log=log)
-    net = S.UNetSurrogate(256, 256, batch=32, seed=1)
+    net = S.UNetSurrogate(256, 256, batch=batch, seed=1)
     t0 = time.perf_counter()
     x = S.stack_inputs(net, specs[:n_train])
     y = S.stack_targets(net, pk[:n_train], tp[:n_train])
     log(f"input maps of {n_train} members in {time.perf_counter() - t0:.1f} s")
-    S.train(net, x, y, epochs=epochs, batch=32, lr=1e-3, lr_final=5e-5, log=log)
+    S.train(net, x, y, epochs=epochs, batch=batch, lr=1e-3, lr_final=5e-5, log=log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     del x, y
     res = S.judged_predictions(net, specs[n_train:], judge_fraction=0.05, tol_m=0.02)
-    ref = pk[n_train:]
+    ref, ref_tp = pk[n_train:], tp[n_train:]
     rmse = float(np.sqrt(np.mean((res["peak_m"] - ref) ** 2)))
     unjudged = np.setdiff1d(np.arange(n_eval), res["judged"])
-    rmse_net = float(np.sqrt(np.mean((res["peak_m"][unjudged] - ref[unjudged]) ** 2)))
+    rmse_net = float(np.sqrt(np.mean((res["net_peak_m"][unjudged] - ref[unjudged]) ** 2)))
+    rmse_mean = float(np.sqrt(np.mean((pk[:n_train].mean(axis=0)[None] - ref) ** 2)))
+    wetc = ref >= 0.01
+    ttp_net = float(np.sqrt(np.mean((res["net_tpeak_h"][wetc] - ref_tp[wetc]) ** 2)) * 60.0)
+    ttp_mean = float(np.sqrt(np.mean((np.broadcast_to(tp[:n_train].mean(axis=0)[None], ref_tp.shape)[wetc]
+                                      - ref_tp[wetc]) ** 2)) * 60.0)
     t_pred = res["t_inference_s"] + res["t_judge_s"] + res["t_fallback_s"]
+    out = {
+        "members_train": n_train, "members_eval": n_eval, "epochs": epochs, "batch": batch,
+        "rms_peak_depth_m": float(np.sqrt(np.mean(ref ** 2))),
+        "peak_depth_rmse_m": rmse, "surrogate_rmse_m_unjudged": rmse_net,
+        "rmse_mean_map_m": rmse_mean, "rmse_zero_map_m": float(np.sqrt(np.mean(ref ** 2))),
+        "skill_vs_mean_map": 1.0 - rmse_net / rmse_mean,
+        "time_to_peak_rmse_min_wet_cells": ttp_net, "time_to_peak_rmse_min_mean_map": ttp_mean,
+        "judge_rmse_m": res["judge_rmse_m"], "judge_members": int(len(res["judged"])),
+        "fallback": bool(res["fallback"]),
+        "s_inference": res["t_inference_s"], "s_judge": res["t_judge_s"],
+        "s_fallback": res["t_fallback_s"], "s_training_data": t_data, "s_training": t_train,
+    }
+    log(f"data {t_data:.1f} s, train {t_train:.1f} s, {n_eval} members in {t_pred:.3f} s; "
+        f"RMSE {rmse_net * 100:.2f} cm (mean map {rmse_mean * 100:.2f} cm), time to peak "
+        f"{ttp_net:.1f} min (mean map {ttp_mean:.1f} min), judge {res['judge_rmse_m'] * 100:.2f} cm"
+        f"{', fallback' if res['fallback'] else ''}")
+    if wet:
+        out["workload"] = ("256x256 C4-style members, dx=5 m, sink-filled fractal DEM on a 1 % tilt, "
+                           "n/Ks x e^U(ln .5, ln 1.5), rain 120 mm/h x e^U(ln .5, ln 1.5) for 45 min, "
+                           "soil moisture deficit U(0.15, 0.35), 2 h each, FP32")
+        return out
     t_plain = 0.0
     for sp in specs[n_train:n_train + n_plain]:
         with Simulation(sp) as sim:
@@ -191,18 +230,7 @@ def surrogate_study(n_train, epochs, n_eval=256, n_plain=8):
     per_batch = t_data / len(specs)
     study = 20000  # members of a probabilistic study (C5 scale)
     t_study = t_data + t_train + study / n_eval * t_pred
-    print(f"[bench] surrogate: data {t_data:.1f} s, train {t_train:.1f} s, {n_eval} members in "
-          f"{t_pred:.3f} s, judge RMSE {res['judge_rmse_m'] * 100:.2f} cm", file=sys.stderr,
-          flush=True)
-    return {
-        "members_train": n_train, "members_eval": n_eval, "epochs": epochs,
-        "peak_depth_rmse_m": rmse, "surrogate_rmse_m_unjudged": rmse_net,
-        "judge_rmse_m": res["judge_rmse_m"],
-        "judge_members": int(len(res["judged"])), "fallback": bool(res["fallback"]),
-        "rmse_mean_map_m": float(np.sqrt(np.mean((pk[:n_train].mean(axis=0)[None] - ref) ** 2))),
-        "rmse_zero_map_m": float(np.sqrt(np.mean(ref ** 2))),
-        "s_inference": res["t_inference_s"], "s_judge": res["t_judge_s"],
-        "s_training_data": t_data, "s_training": t_train,
+    out.update({
         "members_per_hour_surrogate": n_eval / t_pred * 3600.0,
         "members_per_hour_batched": 3600.0 / per_batch,
         "members_per_hour_plain": 3600.0 / per_plain,
@@ -213,7 +241,8 @@ def surrogate_study(n_train, epochs, n_eval=256, n_plain=8):
                     "n/Ks/rain x e^U(ln .5, ln 1.5), soil moisture deficit U(0.15, 0.35), 2 h "
                     "each, FP32; judge = 5 % of the members re-run by the solver, fallback above "
                     "2 cm",
-    }
+    })
+    return out
 
 
 # algorithmic HBM bytes per cell of the D8 drop-in kernels (FP64): the inputs
@@ -474,6 +503,8 @@ def main():
            if args.ensemble_members > 0 else None)
     sur = (surrogate_study(args.surrogate_train, args.surrogate_epochs)
            if args.surrogate_train > 0 else None)
+    sur_wet = (surrogate_study(args.surrogate_wet_train, args.surrogate_epochs, wet=True)
+               if args.surrogate_wet_train > 0 else None)
     d8 = d8_study(args.d8_size) if args.d8_size > 0 else None
 
     line = {
@@ -513,6 +544,7 @@ def main():
         "cpu_baseline_1thread": cpu1,
         "ensemble": ens,
         "surrogate": sur,
+        "surrogate_wet": sur_wet,
         "d8": d8,
     }
     print(json.dumps(line), flush=True)

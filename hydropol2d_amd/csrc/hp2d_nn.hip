@@ -16,6 +16,7 @@
 // which every lane reads its 8-element operand with one 16-byte LDS read.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -230,97 +231,189 @@ __global__ void flip_transpose_kernel(const uint16_t *w, int Cin, int Cout, uint
     }
 }
 
-// bias gradient: gb[c] += sum over pixels of g[p][c]
-__global__ void bias_grad_kernel(const uint16_t *g, long long P, int C, float *gb) {
-    const int c = threadIdx.x % C;
-    const int lanes_per_c = blockDim.x / C;
-    const int sub = threadIdx.x / C;
-    float s = 0.f;
-    for (long long p = blockIdx.x * (long long)lanes_per_c + sub; p < P;
-         p += (long long)gridDim.x * lanes_per_c)
-        s += bf2f(g[p * C + c]);
-    atomicAdd(&gb[c], s);
+// ---- 8-channel vectors: the element-wise passes move 16 bytes per thread ----
+// (every tensor here has a channel count that is a multiple of 32)
+__device__ __forceinline__ void unpack8(const uint4 u, float (&f)[8]) {
+    const unsigned w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        f[2 * k] = __uint_as_float(w[k] << 16);
+        f[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
+    unsigned w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = (unsigned)f2bf(f[2 * k]) | ((unsigned)f2bf(f[2 * k + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// bias gradient: gb[c] += sum over pixels of g[p][c].  A thread sums one
+// 8-channel vector of a pixel over a strided set of pixels; the block folds
+// the vectors of each channel group in LDS and adds one value per channel.
+__global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t *g, long long P, int C, float *gb) {
+    const int V = C / 8;                       // vectors per pixel (C <= 2048)
+    const int per = 256 / V;                   // pixels a block covers per pass
+    const int v = threadIdx.x % V, sub = threadIdx.x / V;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (sub < per) {
+        const uint4 *g4 = reinterpret_cast<const uint4 *>(g);
+        for (long long p = blockIdx.x * (long long)per + sub; p < P; p += (long long)gridDim.x * per) {
+            float f[8];
+            unpack8(g4[p * V + v], f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s[k] += f[k];
+        }
+    }
+    __shared__ float red[256 * 8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[threadIdx.x * 8 + k] = s[k];
+    __syncthreads();
+    for (int c = threadIdx.x; c < C; c += 256) {
+        const int vv = c / 8, k = c % 8;
+        float t = 0.f;
+        for (int q = 0; q < per; ++q) t += red[(q * V + vv) * 8 + k];
+        atomicAdd(&gb[c], t);
+    }
 }
 
 // g *= (y > 0): the ReLU backward on the stored activation
 __global__ void relu_mask_kernel(uint16_t *g, const uint16_t *y, long long n) {
-    for (long long i = gs_first(); i < n; i += gs_stride())
+    const long long n8 = n / 8;
+    uint4 *g4 = reinterpret_cast<uint4 *>(g);
+    const uint4 *y4 = reinterpret_cast<const uint4 *>(y);
+    for (long long i = gs_first(); i < n8; i += gs_stride()) {
+        float fy[8];
+        unpack8(y4[i], fy);
+        uint4 u = g4[i];
+        unsigned w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!(fy[2 * k] > 0.f)) w[k] &= 0xffff0000u;
+            if (!(fy[2 * k + 1] > 0.f)) w[k] &= 0x0000ffffu;
+        }
+        g4[i] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    for (long long i = n8 * 8 + gs_first(); i < n; i += gs_stride())
         if (!(bf2f(y[i]) > 0.f)) g[i] = 0;
 }
 
 // out = a + b (bf16, for gradients meeting at a skip connection)
 __global__ void add_kernel(const uint16_t *a, const uint16_t *b, uint16_t *out, long long n) {
-    for (long long i = gs_first(); i < n; i += gs_stride()) out[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+    const long long n8 = n / 8;
+    const uint4 *a4 = reinterpret_cast<const uint4 *>(a), *b4 = reinterpret_cast<const uint4 *>(b);
+    uint4 *o4 = reinterpret_cast<uint4 *>(out);
+    for (long long i = gs_first(); i < n8; i += gs_stride()) {
+        float fa[8], fb[8];
+        unpack8(a4[i], fa);
+        unpack8(b4[i], fb);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) fa[k] = fa[k] + fb[k];
+        o4[i] = pack8(fa);
+    }
+    for (long long i = n8 * 8 + gs_first(); i < n; i += gs_stride()) out[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
 }
 
-// 2x2 average pooling (H, W even) and its backward
+// Vector e of an N x H x W x (8V) tensor as (row q = n*H + h, column w, vector v)
+// with 32-bit arithmetic once the row is known.
+__device__ __forceinline__ void pix_of(long long e, int W, int V, long long &q, int &w, int &v) {
+    const long long WV = (long long)W * V;
+    q = e / WV;
+    const int r = (int)(e - q * WV);
+    w = r / V;
+    v = r - w * V;
+}
+
+// 2x2 average pooling (H, W even; output H/2 x W/2) and its backward
 __global__ void avgpool2_kernel(const uint16_t *x, int N, int H, int W, int C, uint16_t *y) {
-    const int Ho = H / 2, Wo = W / 2;
-    const long long total = (long long)N * Ho * Wo * C;
+    const int Ho = H / 2, Wo = W / 2, V = C / 8;
+    const long long total = (long long)N * Ho * Wo * V;
+    const uint4 *x4 = reinterpret_cast<const uint4 *>(x);
+    uint4 *y4 = reinterpret_cast<uint4 *>(y);
     for (long long e = gs_first(); e < total; e += gs_stride()) {
-        const int c = (int)(e % C);
-        long long r = e / C;
-        const int wo = (int)(r % Wo);
-        r /= Wo;
-        const int ho = (int)(r % Ho);
-        const int n = (int)(r / Ho);
-        auto at = [&](int h, int w) { return bf2f(x[(((long long)n * H + h) * W + w) * C + c]); };
-        const float s = at(2 * ho, 2 * wo) + at(2 * ho, 2 * wo + 1) + at(2 * ho + 1, 2 * wo) +
-                        at(2 * ho + 1, 2 * wo + 1);
-        y[e] = f2bf(0.25f * s);
+        long long q;
+        int wo, v;
+        pix_of(e, Wo, V, q, wo, v);  // q = n*Ho + ho
+        const long long r0 = 2 * q;  // = n*H + 2*ho
+        const long long i00 = (r0 * W + 2 * wo) * V + v, i10 = i00 + (long long)W * V;
+        float a[8], b[8], c[8], d[8];
+        unpack8(x4[i00], a);
+        unpack8(x4[i00 + V], b);
+        unpack8(x4[i10], c);
+        unpack8(x4[i10 + V], d);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = 0.25f * (a[k] + b[k] + c[k] + d[k]);
+        y4[e] = pack8(a);
     }
 }
 __global__ void avgpool2_bwd_kernel(const uint16_t *gy, int N, int H, int W, int C, uint16_t *gx) {
-    const long long total = (long long)N * H * W * C;
+    const int V = C / 8;
+    const long long total = (long long)N * H * W * V;
+    const uint4 *g4 = reinterpret_cast<const uint4 *>(gy);
+    uint4 *o4 = reinterpret_cast<uint4 *>(gx);
     for (long long e = gs_first(); e < total; e += gs_stride()) {
-        const int c = (int)(e % C);
-        long long r = e / C;
-        const int w = (int)(r % W);
-        r /= W;
-        const int h = (int)(r % H);
-        const int n = (int)(r / H);
-        gx[e] = f2bf(0.25f * bf2f(gy[(((long long)n * (H / 2) + h / 2) * (W / 2) + w / 2) * C + c]));
+        long long q;
+        int w, v;
+        pix_of(e, W, V, q, w, v);  // q = n*H + h
+        const long long qh = q / 2;  // = n*(H/2) + h/2 (H even)
+        float f[8];
+        unpack8(g4[(qh * (W / 2) + w / 2) * V + v], f);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) f[k] = 0.25f * f[k];
+        o4[e] = pack8(f);
     }
 }
 
 // nearest 2x upsampling (output H, W) and its backward (sum over the 2x2 block)
 __global__ void upsample2_kernel(const uint16_t *x, int N, int H, int W, int C, uint16_t *y) {
-    const long long total = (long long)N * H * W * C;
+    const int V = C / 8;
+    const long long total = (long long)N * H * W * V;
+    const uint4 *x4 = reinterpret_cast<const uint4 *>(x);
+    uint4 *y4 = reinterpret_cast<uint4 *>(y);
     for (long long e = gs_first(); e < total; e += gs_stride()) {
-        const int c = (int)(e % C);
-        long long r = e / C;
-        const int w = (int)(r % W);
-        r /= W;
-        const int h = (int)(r % H);
-        const int n = (int)(r / H);
-        y[e] = x[(((long long)n * (H / 2) + h / 2) * (W / 2) + w / 2) * C + c];
+        long long q;
+        int w, v;
+        pix_of(e, W, V, q, w, v);  // q = n*H + h
+        y4[e] = x4[((q / 2) * (W / 2) + w / 2) * V + v];
     }
 }
 __global__ void upsample2_bwd_kernel(const uint16_t *gy, int N, int H, int W, int C, uint16_t *gx) {
-    const int Ho = H / 2, Wo = W / 2;
-    const long long total = (long long)N * Ho * Wo * C;
+    const int Ho = H / 2, Wo = W / 2, V = C / 8;
+    const long long total = (long long)N * Ho * Wo * V;
+    const uint4 *g4 = reinterpret_cast<const uint4 *>(gy);
+    uint4 *o4 = reinterpret_cast<uint4 *>(gx);
     for (long long e = gs_first(); e < total; e += gs_stride()) {
-        const int c = (int)(e % C);
-        long long r = e / C;
-        const int wo = (int)(r % Wo);
-        r /= Wo;
-        const int ho = (int)(r % Ho);
-        const int n = (int)(r / Ho);
-        auto at = [&](int h, int w) { return bf2f(gy[(((long long)n * H + h) * W + w) * C + c]); };
-        gx[e] = f2bf(at(2 * ho, 2 * wo) + at(2 * ho, 2 * wo + 1) + at(2 * ho + 1, 2 * wo) +
-                     at(2 * ho + 1, 2 * wo + 1));
+        long long q;
+        int wo, v;
+        pix_of(e, Wo, V, q, wo, v);  // q = n*Ho + ho
+        const long long i00 = (2 * q * W + 2 * wo) * V + v, i10 = i00 + (long long)W * V;
+        float a[8], b[8], c[8], d[8];
+        unpack8(g4[i00], a);
+        unpack8(g4[i00 + V], b);
+        unpack8(g4[i10], c);
+        unpack8(g4[i10 + V], d);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = a[k] + b[k] + c[k] + d[k];
+        o4[e] = pack8(a);
     }
 }
 
-// 1x1 head: out[p][o] = b[o] + sum_c x[p][c] w[o][c] (FP32 out, O outputs)
+// 1x1 head: out[p][o] = b[o] + sum_c x[p][c] w[o][c] (FP32 out, O outputs);
+// channels read as 16-byte vectors
 __global__ void head_fwd_kernel(const uint16_t *x, long long P, int C, const float *w,
                                 const float *b, int O, float *out) {
+    const int V = C / 8;
+    const uint4 *x4 = reinterpret_cast<const uint4 *>(x);
     for (long long p = gs_first(); p < P; p += gs_stride()) {
-        for (int o = 0; o < O; ++o) {
-            float s = b[o];
-            for (int c = 0; c < C; ++c) s += bf2f(x[p * C + c]) * w[o * C + c];
-            out[p * O + o] = s;
+        float s[4] = {0.f, 0.f, 0.f, 0.f};  // O <= 4
+        for (int v = 0; v < V; ++v) {
+            float f[8];
+            unpack8(x4[p * V + v], f);
+            for (int o = 0; o < O; ++o)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) s[o] += f[k] * w[o * C + v * 8 + k];
         }
+        for (int o = 0; o < O; ++o) out[p * O + o] = b[o] + s[o];
     }
 }
 // its backward: gx[p][c] = sum_o gy[p][o] w[o][c] (bf16); gw, gb accumulated
@@ -375,9 +468,11 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const uint16_t *x, long l
     if (threadIdx.x < O) atomicAdd(&gb[threadIdx.x], red[O * C + threadIdx.x]);
 }
 
-// masked mean squared error over the O output maps; gy = d loss / d out
-__global__ void mse_kernel(const float *out, const float *target, const uint8_t *mask, long long P,
-                           int O, float inv_count, float *gy, float *loss) {
+// masked mean squared error over the O output maps; gy = d loss / d out.  The
+// loss is folded per block (one atomic add per block of a capped grid).
+__global__ __launch_bounds__(256) void mse_kernel(const float *out, const float *target,
+                                                  const uint8_t *mask, long long P, int O,
+                                                  float inv_count, float *gy, float *loss) {
     float l = 0.f;
     for (long long p = gs_first(); p < P; p += gs_stride()) {
         const bool on = !mask || mask[p];
@@ -388,7 +483,10 @@ __global__ void mse_kernel(const float *out, const float *target, const uint8_t 
         }
     }
     for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, l * inv_count);
+    __shared__ float red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (red[0] + red[1] + red[2] + red[3]) * inv_count);
 }
 
 // Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the bf16 copy
@@ -550,22 +648,22 @@ hp2d_status hp2d_nn_conv3x3_wgrad(int32_t N, int32_t H, int32_t W, int32_t Cin, 
         hipLaunchKernelGGL((conv3x3_wgrad_kernel<32, 32>), g, dim3(kBlock), 0, s, a);
     if (hipGetLastError() != hipSuccess) return HP2D_E_HIP;
     if (gb) {
-        const int per = 256 / Cout > 0 ? 256 / Cout : 1;
-        hipLaunchKernelGGL(bias_grad_kernel, dim3(1024), dim3(per * Cout), 0, s, gy, P, Cout, gb);
+        if (Cout / 8 > 256) return HP2D_E_UNSUPPORTED;
+        hipLaunchKernelGGL(bias_grad_kernel, dim3(1024), dim3(256), 0, s, gy, P, Cout, gb);
         if (hipGetLastError() != hipSuccess) return HP2D_E_HIP;
     }
     return HP2D_OK;
 }
 
 hp2d_status hp2d_nn_relu_mask(uint16_t *g, const uint16_t *y, int64_t n, void *stream) {
-    hipLaunchKernelGGL(relu_mask_kernel, grid1(n), dim3(256), 0, (hipStream_t)stream, g, y,
+    hipLaunchKernelGGL(relu_mask_kernel, grid1((n + 7) / 8), dim3(256), 0, (hipStream_t)stream, g, y,
                        (long long)n);
     return hipGetLastError() == hipSuccess ? HP2D_OK : HP2D_E_HIP;
 }
 
 hp2d_status hp2d_nn_add(const uint16_t *a, const uint16_t *b, uint16_t *out, int64_t n,
                         void *stream) {
-    hipLaunchKernelGGL(add_kernel, grid1(n), dim3(256), 0, (hipStream_t)stream, a, b, out,
+    hipLaunchKernelGGL(add_kernel, grid1((n + 7) / 8), dim3(256), 0, (hipStream_t)stream, a, b, out,
                        (long long)n);
     return hipGetLastError() == hipSuccess ? HP2D_OK : HP2D_E_HIP;
 }
@@ -573,7 +671,8 @@ hp2d_status hp2d_nn_add(const uint16_t *a, const uint16_t *b, uint16_t *out, int
 hp2d_status hp2d_nn_avgpool2(const uint16_t *x, int32_t N, int32_t H, int32_t W, int32_t C,
                              uint16_t *y, int32_t backward, void *stream) {
     if (H % 2 || W % 2) return HP2D_E_ARG;
-    const long long out_n = backward ? (long long)N * H * W * C : (long long)N * (H / 2) * (W / 2) * C;
+    if (C % 8) return HP2D_E_ARG;
+    const long long out_n = (backward ? (long long)N * H * W * C : (long long)N * (H / 2) * (W / 2) * C) / 8;
     if (backward)
         hipLaunchKernelGGL(avgpool2_bwd_kernel, grid1(out_n), dim3(256), 0, (hipStream_t)stream, x, N,
                            H, W, C, y);
@@ -586,7 +685,8 @@ hp2d_status hp2d_nn_avgpool2(const uint16_t *x, int32_t N, int32_t H, int32_t W,
 hp2d_status hp2d_nn_upsample2(const uint16_t *x, int32_t N, int32_t H, int32_t W, int32_t C,
                               uint16_t *y, int32_t backward, void *stream) {
     if (H % 2 || W % 2) return HP2D_E_ARG;
-    const long long out_n = backward ? (long long)N * (H / 2) * (W / 2) * C : (long long)N * H * W * C;
+    if (C % 8) return HP2D_E_ARG;
+    const long long out_n = (backward ? (long long)N * (H / 2) * (W / 2) * C : (long long)N * H * W * C) / 8;
     if (backward)
         hipLaunchKernelGGL(upsample2_bwd_kernel, grid1(out_n), dim3(256), 0, (hipStream_t)stream, x,
                            N, H, W, C, y);
@@ -600,6 +700,7 @@ hp2d_status hp2d_nn_head(const uint16_t *x, int64_t P, int32_t C, const float *w
                          int32_t O, float *out, const float *gy, uint16_t *gx, float *gw, float *gb,
                          void *stream) {
     if (!x || !w || (!gy && (!b || !out)) || (gy && (!gx || !gw || !gb))) return HP2D_E_ARG;
+    if (C % 8 || O > 4) return HP2D_E_UNSUPPORTED;
     if (!gy) {
         hipLaunchKernelGGL(head_fwd_kernel, grid1(P), dim3(256), 0, (hipStream_t)stream, x,
                            (long long)P, C, w, b, O, out);
@@ -614,7 +715,9 @@ hp2d_status hp2d_nn_head(const uint16_t *x, int64_t P, int32_t C, const float *w
 
 hp2d_status hp2d_nn_mse(const float *out, const float *target, const uint8_t *mask, int64_t P,
                         int32_t O, float inv_count, float *gy, float *loss, void *stream) {
-    hipLaunchKernelGGL(mse_kernel, grid1(P), dim3(256), 0, (hipStream_t)stream, out, target, mask,
+    const long long blocks = std::min<long long>((P + 255) / 256, 1024);
+    hipLaunchKernelGGL(mse_kernel, dim3((unsigned)std::max<long long>(blocks, 1)), dim3(256), 0,
+                       (hipStream_t)stream, out, target, mask,
                        (long long)P, O, inv_count, gy, loss);
     return hipGetLastError() == hipSuccess ? HP2D_OK : HP2D_E_HIP;
 }

@@ -307,11 +307,11 @@ def member_features(spec):
     return f
 
 
-# target units: peak depth in cm, time to peak in tenths of the hour, so that
+# target units: peak depth in cm, time to peak in units of 10 min, so that
 # the squared error the network is trained on is mostly the depth's (the judge
-# measures peak-depth RMSE)
+# measures peak-depth RMSE) while the time to peak still carries weight
 DEPTH_SCALE = 100.0
-TPEAK_SCALE = 0.1
+TPEAK_SCALE = 6.0
 
 
 def solve_members(specs, graph_steps=50, chunk=4096, log=None):
@@ -397,7 +397,8 @@ def judged_predictions(net, specs, judge_fraction=0.05, tol_m=0.02, seed=0, batc
     if the surrogate's peak-depth RMSE on that sample exceeds `tol_m` the
     error indicator trips and every member falls back to the solver.  Returns
     a dict with the per-member peak depth [m], the judge's RMSE, whether the
-    fallback fired and the wall times of each part."""
+    fallback fired, the network's own peak-depth [m] and time-to-peak [h] maps
+    and the wall times of each part."""
     import time
     torch = _torch()
     n = len(specs)
@@ -408,7 +409,9 @@ def judged_predictions(net, specs, judge_fraction=0.05, tol_m=0.02, seed=0, batc
     pred = net.predict(x)
     torch.cuda.synchronize()
     peak = (pred[..., 0] / DEPTH_SCALE).clamp(min=0.0).cpu().numpy()
+    tpeak = (pred[..., 1] / TPEAK_SCALE).clamp(min=0.0).cpu().numpy()
     t_inf = time.perf_counter() - t0
+    net_peak = peak.copy()
     k = max(1, int(round(judge_fraction * n)))
     judged = np.sort(rng.choice(n, size=k, replace=False))
     jp, _, t_judge = solve_members([specs[i] for i in judged])
@@ -420,6 +423,7 @@ def judged_predictions(net, specs, judge_fraction=0.05, tol_m=0.02, seed=0, batc
         peak = full
     else:
         peak[judged] = jp
-    return {"peak_m": peak, "judge_rmse_m": rmse, "fallback": fallback, "judged": judged,
+    return {"peak_m": peak, "net_peak_m": net_peak, "net_tpeak_h": tpeak,
+            "judge_rmse_m": rmse, "fallback": fallback, "judged": judged,
             "t_inference_s": t_inf, "t_judge_s": t_judge, "t_fallback_s": t_fb,
             "t_total_s": t_inf + t_judge + t_fb}

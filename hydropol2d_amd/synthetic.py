@@ -220,12 +220,14 @@ def dam_break_c3(ny=2048, nx=2048, dx=1.0, precision="fp32", routing_time_min=30
 
 
 def ensemble_c4(n_members=512, seed=1000, min_log2=7, max_log2=10, dx=5.0, precision="fp32",
-                routing_time_min=120.0, vary_soil_moisture=False, filled=False):
+                routing_time_min=120.0, vary_soil_moisture=False, filled=False, rain_mmh=50.0,
+                rain_duration_min=30.0):
     """Ensemble members around the C1 plane: grid side 2^U(min_log2, max_log2)
     (log-uniform), a seeded fractal DEM (Hurst 0.8, relief 5 % of the domain
     length) on a 1 % tilt towards the last column (critical-depth outlets
     there), and Manning n, Green-Ampt Ks and the rainfall intensity perturbed
-    by e^U(ln 0.5, ln 1.5) around n = 0.02, Ks = 5 mm/h, 50 mm/h for 30 min."""
+    by e^U(ln 0.5, ln 1.5) around n = 0.02, Ks = 5 mm/h and rain_mmh (50 mm/h)
+    for rain_duration_min (30 min)."""
     rng = np.random.default_rng(seed)
     specs = []
     for m in range(n_members):
@@ -244,8 +246,8 @@ def ensemble_c4(n_members=512, seed=1000, min_log2=7, max_log2=10, dx=5.0, preci
                        dtheta=np.array([dtheta]), theta_sat_minus_r=np.array([0.4]), I_min_mm=1e-3)
         specs.append(RunSpec(
             elevation=np.asfortranarray(z), roughness=np.full((ny, nx), 0.02 * f_n, order="F"),
-            resolution=dx, outlet_mask=outlet, rain_time_min=np.array([30.0, 1e9]),
-            rain_intensity_mmh=np.array([50.0 * f_rain, 0.0]), precision=precision,
+            resolution=dx, outlet_mask=outlet, rain_time_min=np.array([rain_duration_min, 1e9]),
+            rain_intensity_mmh=np.array([rain_mmh * f_rain, 0.0]), precision=precision,
             outlet_type=2, routing_time_min=routing_time_min, min_time_step_s=0.01,
             max_time_step_s=10.0, time_step_increments_s=0.001, time_step_change_s=1e-4,
             alfa_min=0.7, alfa_max=0.7, flag_timestep=1, flag_adaptive_timestepping=1,

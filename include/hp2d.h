@@ -798,21 +798,25 @@ hp2d_status hp2d_nn_conv3x3_dgrad_noalloc(int32_t N, int32_t H, int32_t W, int32
 hp2d_status hp2d_nn_conv3x3_wgrad(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
                                   const uint16_t *x, const uint16_t *gy, float *gw, float *gb,
                                   void *stream);
-/* g[i] = 0 where y[i] <= 0 (ReLU backward on the stored activation). */
+/* g[i] = 0 where y[i] <= 0 (ReLU backward on the stored activation); g and y
+ * 16-byte aligned (8 elements per thread). */
 hp2d_status hp2d_nn_relu_mask(uint16_t *g, const uint16_t *y, int64_t n, void *stream);
-/* out = a + b (bf16). */
+/* out = a + b (bf16); a, b, out 16-byte aligned. */
 hp2d_status hp2d_nn_add(const uint16_t *a, const uint16_t *b, uint16_t *out, int64_t n,
                         void *stream);
 /* 2x2 average pooling of x [N,H,W,C] into y [N,H/2,W/2,C]; with backward != 0,
- * x is the gradient of the pooled map and y the gradient of the input [N,H,W,C]. */
+ * x is the gradient of the pooled map and y the gradient of the input [N,H,W,C].
+ * C a multiple of 8, tensors 16-byte aligned. */
 hp2d_status hp2d_nn_avgpool2(const uint16_t *x, int32_t N, int32_t H, int32_t W, int32_t C,
                              uint16_t *y, int32_t backward, void *stream);
 /* nearest 2x upsampling of x [N,H/2,W/2,C] into y [N,H,W,C]; with backward != 0,
- * x is the gradient [N,H,W,C] and y receives [N,H/2,W/2,C]. */
+ * x is the gradient [N,H,W,C] and y receives [N,H/2,W/2,C].  C a multiple of 8,
+ * tensors 16-byte aligned. */
 hp2d_status hp2d_nn_upsample2(const uint16_t *x, int32_t N, int32_t H, int32_t W, int32_t C,
                               uint16_t *y, int32_t backward, void *stream);
 /* 1x1 head: out [P][O] = b + x [P][C] w^T (w [O][C]); with gy != NULL the
- * backward instead: gx (bf16), gw, gb accumulated. */
+ * backward instead: gx (bf16), gw, gb accumulated.  C a multiple of 8 and
+ * O <= 4 (the backward: C = 32, O = 2); x 16-byte aligned. */
 hp2d_status hp2d_nn_head(const uint16_t *x, int64_t P, int32_t C, const float *w, const float *b,
                          int32_t O, float *out, const float *gy, uint16_t *gx, float *gw, float *gb,
                          void *stream);

@@ -117,3 +117,40 @@ def test_c4_members_run():
             dm, tp = m.state()["dmax"], m.time_to_peak()
             assert np.isfinite(dm).all() and (dm >= 0).all()
             assert (tp >= 0).all() and (tp <= 10.0 + 1.0).all()
+
+
+def test_c4_full_batch_members_equal_solo_runs():
+    """The whole C4 ensemble: 512 members with sides log-uniform in 128..1024
+    (1.3e8 cells), perturbed n, Ks and rainfall, 2 h each, FP32, in one
+    batched run.  Every member ends with finite, non-negative depths and peak
+    depths, a time to peak inside the run, and its run clock at 2 h; eight
+    members across the size range, run alone with the member's launch shape,
+    equal their batched selves bit for bit."""
+    from hydropol2d_amd.simulation import Ensemble
+    specs = synthetic.ensemble_c4(512)
+    sides = np.array([sp.shape[0] for sp in specs])
+    print(f"C4 batch: 512 members, sides {sides.min()}..{sides.max()}, "
+          f"{int((sides.astype(np.int64) ** 2).sum())} cells", flush=True)
+    order = np.argsort(sides, kind="stable")
+    sample = [int(order[k]) for k in np.linspace(0, len(specs) - 1, 8).astype(int)]
+    with Ensemble(specs) as ens:
+        ens.record_time_to_peak()
+        assert ens.run() == len(specs)
+        got = {}
+        for i in range(len(specs)):
+            m = ens.member(i)
+            st, rep = m.state(infiltration=True), m.report()
+            assert np.isfinite(st["d_t"]).all() and (st["d_t"] >= 0).all(), i
+            assert np.isfinite(st["dmax"]).all() and (st["dmax"] >= st["d_t"]).all(), i
+            tp = m.time_to_peak()
+            # the last step may end up to one max_dt (10 s) past the 2 h
+            assert (tp >= 0).all() and (tp <= 120.0 + 10.0 / 60.0 + 1e-9).all(), i
+            assert rep["t_min"] == pytest.approx(120.0, abs=10.0 / 60.0), i
+            if i in sample:
+                got[i] = (st, rep, m.launch_shape())
+    for i in sample:
+        st, rep, shape = got[i]
+        st0, rep0, _, _ = _solo(specs[i], shape)
+        for k in ("d_t", "outflow", "dmax", "I_t"):
+            np.testing.assert_array_equal(st[k], st0[k], err_msg=f"member {i} {k}")
+        assert rep == rep0, i

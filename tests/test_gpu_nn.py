@@ -135,6 +135,32 @@ def test_network_forward_and_gradients_match_cpu_reference():
         assert rel(_np(net.convs[li]["gb"]), gb) < 5e-2, (li, rel(_np(net.convs[li]["gb"]), gb))
 
 
+def test_network_256_forward_and_gradients_match_cpu_reference():
+    """The surrogate network at its benchmark size (256 x 256 input stacks,
+    32 -> 256 channels over four levels): one forward and one backward pass of
+    one sample against the CPU reference network (FP64 convolutions with bf16
+    rounding where the GPU stores bf16)."""
+    torch = _torch()
+    print("network 256 x 256 forward + backward against the CPU reference", flush=True)
+    net, x, rng = _small_net(seed=3, H=256, W=256, B=1)
+    params = [(nn_cases.bf16(_np(p["w"])), _np(p["b"])) for p in net.convs]
+    hw, hb = _np(net.head["w"]), _np(net.head["b"])
+    out = _np(net.forward(_dev_bf16(x)))
+    ref_out, cache = nn_cases.forward(params, hw, hb, x)
+    assert np.linalg.norm(out - ref_out) / np.linalg.norm(ref_out) < 1e-2
+    g_out = rng.normal(size=out.shape).astype(np.float32) / out.size
+    net.zero_grad()
+    net.backward(_dev_f32(g_out), 1)
+    torch.cuda.synchronize()
+    grads, (hgw, hgb) = nn_cases.backward(params, hw, cache, g_out)
+    rel = lambda a, b: np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+    assert rel(_np(net.head["gw"]), hgw) < 3e-2
+    assert rel(_np(net.head["gb"]), hgb) < 3e-2
+    for li, (gw, gb) in enumerate(grads):
+        assert rel(_np(net.convs[li]["gw"]), gw) < 5e-2, (li, rel(_np(net.convs[li]["gw"]), gw))
+        assert rel(_np(net.convs[li]["gb"]), gb) < 5e-2, (li, rel(_np(net.convs[li]["gb"]), gb))
+
+
 def test_training_fits_a_smooth_map():
     # a learnable target (a smoothed, nonlinear function of the input maps):
     # 150 Adam steps cut the loss by 10x
