@@ -66,6 +66,7 @@ class RunConfig(C.Structure):
         ("hi_mu", C.c_double), ("hi_Cd", C.c_double), ("hi_ro_water", C.c_double),
         ("hi_gravity", C.c_double),
         ("record_time_maps_min", C.c_double),
+        ("gw_exact", C.c_int32),
     ]
 
 

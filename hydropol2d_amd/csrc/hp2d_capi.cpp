@@ -1459,6 +1459,7 @@ static hp2d_status sim_gw_update(hp2d_sim *s, SimState &st) {
     } else {  // the module reads and writes the FP32 state in place
         a.d32 = (float *)s->d[cur];
         a.I32 = (float *)s->I[cur];
+        a.fast_boussinesq = s->cfg.gw_exact ? 0 : 1;
     }
     s->gwp.time_step_min = st.gw_elapsed_min;
     s->gwp.t_min = st.t;

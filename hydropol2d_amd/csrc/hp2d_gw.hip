@@ -56,6 +56,7 @@ __global__ void gw_ctl_init(GwCtl *ctl, double T) {
     ctl->dtgw_bits = kInfBits;
     ctl->done = 0;
     ctl->nsub = 0;
+    ctl->last = 0;
 }
 
 __global__ void gw_ctl_begin(GwCtl *ctl) {
@@ -69,6 +70,11 @@ __global__ void gw_ctl_begin(GwCtl *ctl) {
     ctl->dt = fmin(dt_gw, ctl->dt);
     ctl->dt_used = ctl->dt;
     ctl->dtgw_bits = kInfBits;
+    // whether gw_ctl_end will find the loop over after this sub-step (the same
+    // trim and clock arithmetic): the next Courant limit is then not needed
+    const int kk = 2;
+    const double dt_next = ((double)kk == ctl->n_steps - 1.0) ? ctl->T - kk * ctl->dt : ctl->dt;
+    ctl->last = !(ctl->t + dt_next < ctl->T);
 }
 
 __global__ void gw_ctl_end(GwCtl *ctl) {
@@ -86,7 +92,7 @@ __global__ void gw_ctl_end(GwCtl *ctl) {
 __global__ __launch_bounds__(kGwBlock) void gw_flux_kernel(GwArgs a, const double *__restrict__ h,
                                                            double *Fx, double *Fy, int avg) {
     if (a.ctl->done) return;
-    const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
+    const long long c = xcd_logical_tile() * (long long)kGwBlock + threadIdx.x;
     if (c >= a.n) return;
     const int ny = a.ny;
     int i, j;
@@ -150,7 +156,7 @@ __global__ __launch_bounds__(kGwBlock) void gw_predict_kernel(GwArgs a, const do
                                                               const double *Fx, const double *Fy,
                                                               double *h_p) {
     if (a.ctl->done) return;
-    const long long c = blockIdx.x * (long long)kGwBlock + threadIdx.x;
+    const long long c = xcd_logical_tile() * (long long)kGwBlock + threadIdx.x;
     if (c >= a.n) return;
     int i, j;
     gw_rc(c, a.ny, i, j);
@@ -171,7 +177,9 @@ __global__ __launch_bounds__(kColThreads) void gw_update_kernel(GwArgs a, const 
                                                                 const double *h0, double *col) {
     if (a.ctl->done) return;
     const double dt = a.ctl->dt_used;
-    const int j = blockIdx.x;
+    // columns in XCD-contiguous runs: a column's neighbours, whose heads and
+    // fluxes it reads, sit in the same L2
+    const int j = xcd_logical_tile();
     const bool is_max[4] = {false, false, false, false};
     column_reduce<4>(a.ny, j, is_max, [&](long long c, double (&s)[4]) {
         const int i = (int)(c - (long long)j * a.ny);
@@ -192,6 +200,210 @@ __global__ __launch_bounds__(kColThreads) void gw_update_kernel(GwArgs a, const 
     }, col);
 }
 
+// ---- FP32 runs: one sub-step in two passes ----
+// The predictor and the corrector march along the columns: one wave owns 62
+// rows (lanes 1..62; lanes 0 and 63 are the one-row halo) and keeps the
+// column to its left in registers, so each array is read once and no flux
+// array is stored.  The heads, the saturated thicknesses and the head
+// differences across a face are taken in FP64; the face flux, its donor cap
+// and the divergence in FP32 with reciprocal multiplies (an FP32 run's
+// precision; a call moves the head by far less than the head's FP32
+// resolution, so the head and its update stay FP64).  Same fluxes, limiter,
+// divergence, bed, exfiltration and surface clamp as the four passes above,
+// which the FP64 runs keep in the reference's operation order.
+constexpr int kGwRowsOut = kWave - 2;  // rows finalised per wave
+
+__device__ __forceinline__ double gw_lane_up(double v) { return __shfl_up(v, 1); }
+__device__ __forceinline__ double gw_lane_down(double v) { return __shfl_down(v, 1); }
+__device__ __forceinline__ float gw_lane_up(float v) { return __shfl_up(v, 1); }
+__device__ __forceinline__ float gw_lane_down(float v) { return __shfl_down(v, 1); }
+
+// flux across the face from cell A (donor side: the left / upper cell) to B
+__device__ __forceinline__ float gw_face32(double hA, float HA, double hB, float HB, float K,
+                                           float Sy_A, float inv_d, float inv_dt) {
+    const float Hx = 0.5f * (HA + HB);
+    const float F = -K * Hx * ((float)(hB - hA) * inv_d);
+    const float cap = Sy_A * HA * inv_dt;
+    float Fl = copysignf(fminf(fabsf(F), cap), F);
+    if (F == 0.0f) Fl = 0.0f;
+    if (isnan(F)) Fl = F;  // signum(NaN) * ... of the reference
+    return Fl;
+}
+
+// one head's values in a column: h, H = max(h - z0, 0)
+struct GwCol {
+    double h;
+    float H;
+};
+
+// the head this sub-step starts from: the entry head, then the two masked-head
+// buffers in turn (a sub-step never writes the head it reads)
+__device__ __forceinline__ const double *gw_fast_head(const GwArgs &a, const double *h0,
+                                                      const double *hA, const double *hB) {
+    const int k = a.ctl->nsub;
+    return k == 0 ? h0 : ((k & 1) ? hA : hB);
+}
+
+// CORR = false: the predictor (:113-118), h_p from the fluxes of h_t.
+// CORR = true: the corrector (:119-148) from the averaged fluxes of h_t and
+// h_p; the masked head goes to the buffer the next sub-step reads (hA after an
+// even count, hB after an odd one), the unmasked head to h_out for the next
+// Courant limit, and each workgroup leaves its sums of the closing mass
+// balance (R*dt, q_exf*dt, Sy*max(h0 - z0, 0), Sy*max(h - z0, 0)) in parts.
+template <bool CORR>
+__global__ __launch_bounds__(256) void gw_march_kernel(GwArgs a, const double *h0, double *hA,
+                                                       double *hB, double *h_p, double *h_out,
+                                                       double *q_exf, double *parts, int tj) {
+    if (a.ctl->done) return;
+    const int ny = a.ny, nx = a.nx;
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int n_slabs = (ny + kGwRowsOut - 1) / kGwRowsOut;
+    const int n_bx = (n_slabs + 3) / 4;
+    const int logical = xcd_logical_tile();
+    const int bx = logical % n_bx, by = logical / n_bx;
+    const int slab = bx * 4 + wv;
+    const int r = slab * kGwRowsOut - 1 + lane;
+    const bool row_in = slab < n_slabs && r >= 0 && r < ny;
+    const bool own = row_in && lane >= 1 && lane <= kGwRowsOut;
+    const int c0 = by * tj, c1 = min(c0 + tj, nx);
+    const double dt = a.ctl->dt_used, inv_dt64 = 1.0 / dt;
+    const float inv_dt = (float)inv_dt64, inv_dx = (float)(1.0 / a.dx), inv_dy = (float)(1.0 / a.dy);
+    const double *h_t = gw_fast_head(a, h0, hA, hB);
+    double *h_mask = (a.ctl->nsub & 1) ? hB : hA;
+
+    // one column's loads of the lane's row (zero thickness outside the grid):
+    // the heads with their saturated thickness, the conductivity and specific
+    // yield, and (corrector) the recharge and the entry head of the balance
+    struct Col {
+        GwCol t, p;
+        double z0, R, h0;
+        float K, Sy;
+    };
+    auto load = [&](int j) {
+        Col v;
+        v.t = GwCol{0.0, 0.0f};
+        v.p = GwCol{0.0, 0.0f};
+        v.z0 = v.R = v.h0 = 0.0;
+        v.K = v.Sy = 0.0f;
+        if (row_in && j >= 0 && j < nx) {
+            const long long c = (long long)j * ny + r;
+            v.z0 = a.z0[c];
+            v.t.h = h_t[c];
+            v.t.H = (float)fmax(v.t.h - v.z0, 0.0);
+            if constexpr (CORR) {
+                v.p.h = h_p[c];
+                v.p.H = (float)fmax(v.p.h - v.z0, 0.0);
+                v.R = a.R[c];
+                v.h0 = h0[c];
+            }
+            v.K = (float)gw_K(a, c);
+            v.Sy = (float)gw_Sy(a, c);
+        }
+        return v;
+    };
+    // the flux of the x face between columns j and j + 1 (0 past the grid edge)
+    auto fx = [&](const GwCol &A, const GwCol &B, float KA, float KB, float SyA, int j) {
+        return (j >= 0 && j < nx - 1) ? gw_face32(A.h, A.H, B.h, B.H, 0.5f * (KA + KB), SyA, inv_dx, inv_dt)
+                                      : 0.0f;
+    };
+    // the flux of the y face between this row and the next one down (every
+    // lane takes part in the shuffles)
+    auto fy = [&](const GwCol &A, float KA, float SyA) {
+        const double hB = gw_lane_down(A.h);
+        const float HB = gw_lane_down(A.H), KB = gw_lane_down(KA);
+        const float F = gw_face32(A.h, A.H, hB, HB, 0.5f * (KA + KB), SyA, inv_dy, inv_dt);
+        return r < ny - 1 ? F : 0.0f;
+    };
+
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (slab < n_slabs) {
+        // loads run two columns ahead of the compute
+        Col prev = load(c0 - 1), cur = load(c0), nxt = load(c0 + 1);
+        // the x faces between c0 - 1 and c0
+        float fl1 = fx(prev.t, cur.t, prev.K, cur.K, prev.Sy, c0 - 1);
+        float fl2 = 0.0f;
+        if constexpr (CORR) fl2 = fx(prev.p, cur.p, prev.K, cur.K, prev.Sy, c0 - 1);
+        for (int j = c0; j < c1; ++j) {
+            const Col nxt2 = load(j + 2 < c1 + 1 ? j + 2 : nx);  // past the chunk: nothing
+            const float fr1 = fx(cur.t, nxt.t, cur.K, nxt.K, cur.Sy, j);
+            const float fd1 = fy(cur.t, cur.K, cur.Sy);
+            const float up1 = gw_lane_up(fd1);
+            const float fu1 = r > 0 ? up1 : 0.0f;
+            float fr = fr1, fl = fl1, fd = fd1, fu = fu1;
+            float fr2 = 0.0f;
+            if constexpr (CORR) {
+                fr2 = fx(cur.p, nxt.p, cur.K, nxt.K, cur.Sy, j);
+                const float fd2 = fy(cur.p, cur.K, cur.Sy);
+                const float up2 = gw_lane_up(fd2);
+                const float fu2 = r > 0 ? up2 : 0.0f;
+                fr = 0.5f * (fr1 + fr2);
+                fl = 0.5f * (fl1 + fl2);
+                fd = 0.5f * (fd1 + fd2);
+                fu = 0.5f * (fu1 + fu2);
+            }
+            if (own) {
+                const long long c = (long long)j * ny + r;
+                const float div = (fr - fl) * inv_dx + (fd - fu) * inv_dy;
+                const double Sy = gw_Sy(a, c);
+                if constexpr (!CORR) {
+                    double v = cur.t.h + dt / Sy * (-(double)div);
+                    if (a.dirichlet && a.dirichlet[c]) v = a.h_dir[c];
+                    h_p[c] = fmax(v, cur.z0);
+                } else {
+                    double v = cur.t.h + dt / Sy * (-(double)div + cur.R);
+                    if (a.dirichlet && a.dirichlet[c]) v = a.h_dir[c];
+                    v = fmax(v, cur.z0);
+                    const double hs = cur.z0 + (a.h_soil ? a.h_soil[c] : a.h_soil_u);
+                    const bool out = a.catchment && !a.catchment[c];
+                    double q = fmax(0.0, v - hs) * Sy * inv_dt64;
+                    if (out) q = NAN;
+                    q_exf[c] = q;
+                    const double hn = fmin(v, hs);
+                    h_out[c] = hn;
+                    h_mask[c] = out ? NAN : hn;
+                    const double v0 = cur.R * dt, v1 = q * dt;
+                    const double v2 = Sy * fmax(cur.h0 - cur.z0, 0.0);
+                    const double v3 = Sy * fmax((out ? NAN : hn) - cur.z0, 0.0);
+                    if (!isnan(v0)) s0 += v0;
+                    if (!isnan(v1)) s1 += v1;
+                    if (!isnan(v2)) s2 += v2;
+                    if (!isnan(v3)) s3 += v3;
+                }
+            }
+            // shift the march
+            cur = nxt;
+            nxt = nxt2;
+            fl1 = fr1;
+            if constexpr (CORR) fl2 = fr2;
+        }
+    }
+    if constexpr (CORR) {
+        // workgroup sums in a fixed order (the waves in turn)
+        __shared__ double sh[4][4];
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+        s3 = wave_sum(s3);
+        if (lane == 0) {
+            sh[wv][0] = s0;
+            sh[wv][1] = s1;
+            sh[wv][2] = s2;
+            sh[wv][3] = s3;
+        }
+        __syncthreads();
+        if (threadIdx.x < 4)
+            parts[4LL * logical + threadIdx.x] =
+                sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+    }
+}
+
+// workgroups of the marching passes (column chunks of kGwMarchCols)
+constexpr int kGwMarchCols = 32;
+inline int gw_march_blocks(int ny, int nx) {
+    const int n_slabs = (ny + kGwRowsOut - 1) / kGwRowsOut;
+    return ((n_slabs + 3) / 4) * ((nx + kGwMarchCols - 1) / kGwMarchCols);
+}
+
 // compute_boussinesq_velocities (+ mask) and the Courant candidates of
 // compute_stable_dt; h_masked (if given) receives h with NaN outside
 __global__ __launch_bounds__(kGwBlock) void gw_velocity_kernel(GwArgs a, const double *h,
@@ -199,6 +411,9 @@ __global__ __launch_bounds__(kGwBlock) void gw_velocity_kernel(GwArgs a, const d
                                                                double *h_masked, int mask_on,
                                                                int check_done) {
     if (check_done && a.ctl->done) return;
+    // the fast sub-step leaves the masked head itself: after the call's last
+    // sub-step nothing here is needed
+    if (check_done && !h_masked && !ux && !uy && a.ctl->last) return;
     double best = INFINITY;
     const int ny = a.ny;
     // grid-stride over the cells: one atomic per workgroup of a bounded grid
@@ -248,7 +463,8 @@ __global__ __launch_bounds__(kColThreads) void gw_colsum_kernel(GwArgs a, const 
 }
 
 hp2d_status gw_run(hp2d_handle *h, const GwArgs &a_in, double dt, const double *dh0,
-                   const GwScratch &w, double *error, int32_t *n_substeps) {
+                   const GwScratch &w, double *error, int32_t *n_substeps,
+                   const double **h_final) {
     const int ny = a_in.ny, nx = a_in.nx;
     const size_t n = (size_t)ny * nx, nb = n * sizeof(double);
     const int blocks = (int)((n + kGwBlock - 1) / kGwBlock);
@@ -275,14 +491,26 @@ hp2d_status gw_run(hp2d_handle *h, const GwArgs &a_in, double dt, const double *
             const double *ht = first ? dh0 : (const double *)w.ht;
             first = false;
             hipLaunchKernelGGL(gw_ctl_begin, dim3(1), dim3(1), 0, s, w.ctl);
-            hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, ht, w.Fx, w.Fy, 0);
-            hipLaunchKernelGGL(gw_predict_kernel, g, b, 0, s, a, ht, (const double *)w.Fx,
-                               (const double *)w.Fy, w.hp);
-            hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)w.hp, w.Fx, w.Fy, 1);
-            hipLaunchKernelGGL(gw_update_kernel, dim3(nx), dim3(kColThreads), 0, s, a, ht,
-                               (const double *)w.Fx, (const double *)w.Fy, w.tmp, w.q, dh0, w.col);
-            hipLaunchKernelGGL(gw_velocity_kernel, gv, b, 0, s, a, (const double *)w.tmp, w.ux, w.uy,
-                               w.ht, 1, 1);
+            if (a.fast) {
+                // masked heads alternate between w.ht and w.Fx (unused here);
+                // the workgroup sums of the balance go to w.Fy
+                const dim3 gm(gw_march_blocks(ny, nx));
+                hipLaunchKernelGGL(gw_march_kernel<false>, gm, dim3(256), 0, s, a, dh0, w.ht, w.Fx,
+                                   w.hp, w.tmp, w.q, w.Fy, kGwMarchCols);
+                hipLaunchKernelGGL(gw_march_kernel<true>, gm, dim3(256), 0, s, a, dh0, w.ht, w.Fx,
+                                   w.hp, w.tmp, w.q, w.Fy, kGwMarchCols);
+                hipLaunchKernelGGL(gw_velocity_kernel, gv, b, 0, s, a, (const double *)w.tmp, w.ux,
+                                   w.uy, (double *)nullptr, 1, 1);
+            } else {
+                hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, ht, w.Fx, w.Fy, 0);
+                hipLaunchKernelGGL(gw_predict_kernel, g, b, 0, s, a, ht, (const double *)w.Fx,
+                                   (const double *)w.Fy, w.hp);
+                hipLaunchKernelGGL(gw_flux_kernel, g, b, 0, s, a, (const double *)w.hp, w.Fx, w.Fy, 1);
+                hipLaunchKernelGGL(gw_update_kernel, dim3(nx), dim3(kColThreads), 0, s, a, ht,
+                                   (const double *)w.Fx, (const double *)w.Fy, w.tmp, w.q, dh0, w.col);
+                hipLaunchKernelGGL(gw_velocity_kernel, gv, b, 0, s, a, (const double *)w.tmp, w.ux,
+                                   w.uy, w.ht, 1, 1);
+            }
             hipLaunchKernelGGL(gw_ctl_end, dim3(1), dim3(1), 0, s, w.ctl);
         }
         e = hipGetLastError();
@@ -298,19 +526,26 @@ hp2d_status gw_run(hp2d_handle *h, const GwArgs &a_in, double dt, const double *
                            (const double *)w.ht, w.col);
         e = hipGetLastError();
     }
-    std::vector<double> col(4 * (size_t)nx);
+    // the closing sums: per column in column order, or (fast sub-steps) per
+    // workgroup of the marching corrector
+    const bool fast_sums = a.fast && hc.nsub > 0;
+    const size_t n_parts = fast_sums ? (size_t)gw_march_blocks(ny, nx) : (size_t)nx;
+    std::vector<double> col(4 * n_parts);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(col.data(), w.col, col.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(col.data(), fast_sums ? w.Fy : w.col, col.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hp2d_handle_fail(h, std::string("Boussinesq: ") + hipGetErrorString(e));
     double sums[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int j = 0; j < nx; ++j)
-        for (int k = 0; k < 4; ++k) sums[k] += col[4 * (size_t)j + k];
+    for (size_t j = 0; j < n_parts; ++j)
+        for (int k = 0; k < 4; ++k) sums[k] += col[4 * j + k];
     const double A = a.dx * a.dy;
     const double recharge_mass = sums[0] * A, exfil_mass = sums[1] * A;
     const double storage_prev = sums[2] * A, storage_curr = sums[3] * A;
     *error = recharge_mass - exfil_mass - (storage_curr - storage_prev);
     if (n_substeps) *n_substeps = hc.nsub;
+    // the final head: w.ht, or w.Fx after an even count of fast sub-steps
+    if (h_final) *h_final = (a.fast && hc.nsub > 0 && hc.nsub % 2 == 0) ? w.Fx : w.ht;
     return HP2D_OK;
 }
 
@@ -406,9 +641,11 @@ extern "C" hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, double dt, double dx, 
     a.catchment = dcm;
     a.dirichlet = ddm;
     a.ctl = w.ctl;
-    st = gw_run(h, a, dt, dh0, w, error, n_substeps);
+    a.fast = prec == HP2D_FP32;  // an FP32 handle: the FP32 runs' two-pass sub-steps
+    const double *h_fin = w.ht;
+    st = gw_run(h, a, dt, dh0, w, error, n_substeps, &h_fin);
     if (st != HP2D_OK) return st;
-    D2H(h_t, w.ht, nb);
+    D2H(h_t, h_fin, nb);
     D2H(q_exf, w.q, nb);
     if (u_x) D2H(u_x, w.ux, nb);
     if (u_y) D2H(u_y, w.uy, nb);

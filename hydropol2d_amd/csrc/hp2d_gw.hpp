@@ -19,6 +19,7 @@ struct GwCtl {
     double t, T, dt, dt_used, n_steps;
     unsigned long long dtgw_bits;  // min positive finite Courant candidate, as bits
     int done, nsub;
+    int last;  // the running sub-step is the call's last one
 };
 
 struct GwArgs {
@@ -31,6 +32,10 @@ struct GwArgs {
     // a uniform specific yield / conductivity [m/s] / soil depth [m] in place
     // of the Sy / K / h_soil maps when those are NULL
     double Sy_u = 0.0, K_u = 0.0, h_soil_u = 0.0;
+    // FP32 runs: the sub-step as two passes that recompute the face fluxes,
+    // with reciprocal multiplies (gw_run then returns the final head in
+    // w.ht or w.Fx, by the parity of the sub-step count)
+    int fast = 0;
 };
 
 // Sy and K of cell c: the map, or the uniform value
@@ -50,7 +55,8 @@ GwScratch gw_scratch(char *base, int ny, int nx);
 // R, K, h_soil, Dirichlet, masks), dh0 the entry head; the final head, q_exf
 // and velocities are left in w.ht, w.q, w.ux, w.uy; *error as the reference.
 hp2d_status gw_run(hp2d_handle *h, const GwArgs &a, double dt, const double *dh0,
-                   const GwScratch &w, double *error, int32_t *n_substeps);
+                   const GwScratch &w, double *error, int32_t *n_substeps,
+                   const double **h_final = nullptr);
 
 // Device fields of the groundwater module step (Groundwater_Module.m:37-404):
 // soil / aquifer maps, the workspace state it updates in place (head h [m],
@@ -78,6 +84,8 @@ struct GwmArgs {
     // z_bed and Kgw already hold elev - soil_depth and ksat_gw / 3.6e6 (the
     // recharge pass then leaves them alone)
     int statics_ready = 0;
+    // the Boussinesq sub-steps as two passes with reciprocal multiplies
+    int fast_boussinesq = 0;
 };
 
 // One module call on device arrays (the scalars of p, the state g updated);

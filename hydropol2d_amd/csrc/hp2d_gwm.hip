@@ -385,11 +385,11 @@ hp2d_status gwm_run(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *g, G
             ga.catchment = dom;
             ga.dirichlet = nullptr;
             ga.ctl = w.ctl;
+            ga.fast = a.fast_boussinesq;
             if (e == hipSuccess) {
-                hp2d_status st = gw_run(h, ga, dt_gw, a.h, w, &err, nullptr);
+                hp2d_status st = gw_run(h, ga, dt_gw, a.h, w, &err, nullptr, &h_new);
                 if (st != HP2D_OK) return st;
-                h_new = w.ht;  // taken over by the post pass
-                q_new = w.q;
+                q_new = w.q;  // the head and q_exf are taken over by the post pass
             }
         } else if (e == hipSuccess) {
             hipLaunchKernelGGL(gwm_local_kernel, G, B, 0, s, a, dt_gw);
@@ -511,6 +511,7 @@ extern "C" hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_par
     if (dom) H2D(dom, p->domain, n);
     if (e != hipSuccess)
         return hp2d_handle_fail(h, std::string("hp2d_groundwater_module: ") + hipGetErrorString(e));
+    a.fast_boussinesq = prec == HP2D_FP32;  // an FP32 handle: the FP32 runs' sub-steps
     double err = 0.0;
     int32_t did = 0;
     st = gwm_run(h, p, g, a, col, dom, &err, &did, nullptr);

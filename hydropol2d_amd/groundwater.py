@@ -21,12 +21,15 @@ _u8p = C.POINTER(C.c_uint8)
 
 def Boussinesq_2D_explicit(dt, dx, dy, h, z0, Sy, R, K, river_mask, K_river, h_river, z_river,
                            Courant, h_soil, catchment_mask, dirichlet_mask, h_dirichlet,
-                           perimeter):
-    """Boussinesq_2D_explicit.m:1 on the gfx950 kernels (FP64).
+                           perimeter, precision="fp64"):
+    """Boussinesq_2D_explicit.m:1 on the gfx950 kernels (FP64 arrays).
 
     Returns (h_t, u_x, u_y, q_exf, q_river, error).  The river arguments and
     the perimeter are accepted and ignored, as in the reference.  The number
     of sub-steps taken is available as `Boussinesq_2D_explicit.last_substeps`.
+    precision="fp32" takes the sub-steps as FP32 runs take them (two passes
+    that recompute the face fluxes, reciprocal multiplies); the default keeps
+    the reference's operation order.
     """
     h = np.asfortranarray(h, dtype=np.float64)
     ny, nx = h.shape
@@ -49,7 +52,7 @@ def Boussinesq_2D_explicit(dt, dx, dy, h, z0, Sy, R, K, river_mask, K_river, h_r
     outs = [np.zeros((ny, nx), order="F") for _ in range(5)]  # h_t u_x u_y q_exf q_river
     err, nsub = C.c_double(), C.c_int32()
     L = _lib.load()
-    hdl = _Handle.get(ny, nx, abi.HP2D_FP64)
+    hdl = _Handle.get(ny, nx, abi.HP2D_FP32 if precision == "fp32" else abi.HP2D_FP64)
     p = (lambda a: a.ctypes.data_as(_dp) if a is not None else _dp())
     pm = (lambda a: a.ctypes.data_as(_u8p) if a is not None else _u8p())
     status = L.hp2d_boussinesq_2d(hdl, float(dt), float(dx), float(dy), p(h), p(z0), p(Sy), p(R),

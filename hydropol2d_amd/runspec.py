@@ -82,6 +82,7 @@ class Groundwater:
     every_steps: int = 10
     courant: float = 0.25
     l_vg: float = 0.5
+    exact: bool = False  # FP32 runs: the FP64 runs' four-pass sub-steps
 
 
 @dataclass
@@ -200,6 +201,7 @@ class RunSpec:
         cfg.flag_groundwater = 1 if self.groundwater is not None else 0
         cfg.gw_every_steps = self.groundwater.every_steps if self.groundwater else 0
         cfg.gw_courant = self.groundwater.courant if self.groundwater else 0.25
+        cfg.gw_exact = int(bool(self.groundwater.exact)) if self.groundwater else 0
 
         ins = abi.RunInputs()
         ins.elevation = _ptr(f(self.elevation), C.c_double)

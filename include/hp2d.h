@@ -400,7 +400,9 @@ hp2d_status hp2d_subgrid_apply_volume_change(hp2d_handle *h, const double *drep_
  * Dirichlet mask.  The river arguments and the perimeter are not read by the reference
  * (q_river is zeros) and are not taken.  Outputs ny*nx: h_t [m] (NaN outside the
  * catchment), u_x, u_y [m/s] (may be NULL), q_exf [m/s], q_river (zeros, may be NULL);
- * error [m3] as the reference computes it; n_substeps (may be NULL). */
+ * error [m3] as the reference computes it; n_substeps (may be NULL).  On an FP32
+ * handle the sub-steps take the FP32 runs' two-pass form (face fluxes recomputed,
+ * reciprocal multiplies); FP64 keeps the reference's operation order. */
 hp2d_status hp2d_boussinesq_2d(hp2d_handle *h, double dt, double dx, double dy,
                                const double *h_in, const double *z0, const double *Sy,
                                const double *R, const double *K, double courant,
@@ -461,6 +463,9 @@ typedef struct hp2d_gw_state {
 /* Host arrays ny*nx updated in place: h_t [m], I_t [mm], d_t [mm],
  * cumulative_recharge [mm], max_gw_depth [m]; state updated; q_exf [m/s] out;
  * *error = errors(5) [m3]; *ran = 1 when the update ran. */
+/* On an FP32 handle the Boussinesq sub-steps take the FP32 runs' two-pass form
+ * (face fluxes recomputed, reciprocal multiplies); FP64 keeps the reference's
+ * operation order. */
 hp2d_status hp2d_groundwater_module(hp2d_handle *h, const hp2d_gw_params *p, hp2d_gw_state *state,
                                     double *h_t, double *I_t, double *d_t,
                                     double *cumulative_recharge, double *max_gw_depth,
@@ -523,6 +528,10 @@ typedef struct hp2d_run_config {
     double hi_weight_kg, hi_width1_m, hi_width2_m, hi_mu, hi_Cd, hi_ro_water, hi_gravity;
     double record_time_maps_min;       /* > 0: map stacks at running_control.time_records
                                           (save_output_maps.m:24-27, 148-258) */
+    int32_t gw_exact;                  /* FP32 runs: 1 = the Boussinesq sub-steps in the
+                                          reference's four passes and operation order (as FP64
+                                          runs take them); 0 = two passes that recompute the face
+                                          fluxes, with reciprocal multiplies */
 } hp2d_run_config;
 
 /* Fields of a map-stack record (save_output_maps.m:148-258: Maps.Hydro.d, velocity,

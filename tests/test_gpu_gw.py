@@ -137,3 +137,65 @@ def test_large_grid_closed_box():
     np.testing.assert_allclose(np.sum(Sy * np.maximum(got["h_t"] - z0, 0)), s0, rtol=1e-11)
     ref = oracle.boussinesq_2d(1800.0, 5.0, 5.0, h, z0, Sy, 0.0, K, 0.25, 100.0)
     _same(got, ref)
+
+
+# The FP32 runs' form of the sub-step: two passes that recompute the face
+# fluxes (FP32 face arithmetic and divergence on FP64 heads, thicknesses and
+# head differences, reciprocal multiplies), the masked heads in two
+# alternating buffers.  It must follow the oracle's FP64 head change to FP32
+# precision over many sub-steps, with the same sub-step count, on catchment
+# masks, Dirichlet cells, clamps and exfiltration, and with an even and an odd
+# sub-step count.
+RTOL32 = 1e-5
+
+
+def _fp32_form(T, dx, dy, h, z0, Sy, R, K, hs, cm=None, dm=None, hd=None):
+    from hydropol2d_amd.groundwater import Boussinesq_2D_explicit as B
+    ref = oracle.boussinesq_2d(T, dx, dy, h, z0, Sy, R, K, 0.25, hs, cm, dm, hd)
+    h_t, u_x, u_y, q_exf, _, err = B(T, dx, dy, h, z0, Sy, R, K, None, None, None, None, 0.25,
+                                     hs, cm, dm, hd, None, precision="fp32")
+    assert B.last_substeps == ref["n_sub"]
+    np.testing.assert_array_equal(np.isnan(h_t), np.isnan(ref["h_t"]))
+    dh_ref = ref["h_t"] - h
+    scale = np.nanmax(np.abs(dh_ref)) + 1e-300
+    assert np.nanmax(np.abs((h_t - h) - dh_ref)) <= RTOL32 * scale
+    assert np.nanmax(np.abs(q_exf - ref["q_exf"])) <= RTOL32 * (np.nanmax(np.abs(ref["q_exf"])) + 1e-300)
+    for got, r in ((u_x, ref["u_x"]), (u_y, ref["u_y"])):
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(r))
+        assert np.nanmax(np.abs(got - r)) <= RTOL32 * (np.nanmax(np.abs(r)) + 1e-300)
+    dS = np.nansum(np.abs(np.broadcast_to(Sy, h.shape) * dh_ref)) * dx * dy
+    assert abs(err - ref["error"]) <= RTOL32 * dS + 1e-9
+    return ref["n_sub"]
+
+
+@pytest.mark.parametrize("seed,ny,nx,T,masks,kmul", [
+    (0, 17, 23, 60.0, False, 1.0),
+    (1, 64, 40, 600.0, True, 100.0),
+    (3, 130, 257, 1800.0, True, 100.0),
+    (6, 40, 50, 1800.0, True, 1.0),
+])
+def test_fp32_form_follows_oracle(seed, ny, nx, T, masks, kmul):
+    z0, h, K, Sy, R, hs = _case(seed, ny, nx)
+    K = K * kmul
+    cm = dm = hd = None
+    if masks:
+        cm = np.ones((ny, nx), dtype=bool)
+        cm[: ny // 5, : nx // 4] = False
+        h[~cm] = np.nan
+        dm = np.zeros((ny, nx), dtype=bool)
+        dm[:, -1] = True
+        hd = z0 + 0.5
+    n_sub = _fp32_form(T, 5.0, 4.0, h, z0, Sy, R, K, hs, cm, dm, hd)
+    print(f"fp32-form Boussinesq seed {seed}: {n_sub} sub-steps", flush=True)
+
+
+def test_fp32_form_even_and_odd_substep_counts():
+    # the final head sits in one of two buffers by the parity of the count
+    z0, h, K, Sy = _sloped(7, 96, 160)
+    R = np.full(h.shape, 1e-6)
+    counts = set()
+    for T in (600.0, 900.0, 1200.0, 3600.0):
+        n_sub = _fp32_form(T, 5.0, 5.0, h, z0, Sy, R, K, 0.06)
+        assert n_sub >= 2
+        counts.add(n_sub % 2)
+    assert counts == {0, 1}
