@@ -50,6 +50,9 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=11,
                    help="steps of the CPU oracle sample (11 includes one groundwater call)")
     p.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
+    p.add_argument("--launch-shape", default="", metavar="TJ,WPB,ROWS",
+                   help="run the step kernel at this launch shape (columns per wave task, waves "
+                        "per block, rows per lane) instead of the tuned one")
     p.add_argument("--cpu-threads", type=int, default=0,
                    help="threads of the all-core CPU leg (0: OMP_NUM_THREADS, else every core)")
     p.add_argument("--ensemble-members", type=int, default=512,
@@ -70,6 +73,9 @@ def parse():
                    help="training epochs at batch 64")
     p.add_argument("--d8-size", type=int, default=4096, metavar="N",
                    help="the D8 drop-in study on an N x N grid (0 skips it)")
+    p.add_argument("--nn-only", action="store_true",
+                   help="run only the surrogate network's kernel study (training steps at batch "
+                        "32 and inference at batch 256 on 256 x 256 stacks) and print its JSON line")
     p.add_argument("--d8-only", action="store_true",
                    help="run only the D8 drop-in study and print its JSON line")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -245,6 +251,55 @@ def surrogate_study(n_train, epochs, n_eval=256, n_plain=8, wet=False, batch=64)
     return out
 
 
+def nn_study(train_batch=32, infer_batch=256, reps=10):
+    """The surrogate network's kernels on 256 x 256 input stacks (random
+    inputs and targets: the timing does not depend on them): training steps
+    (forward, MSE, backward, Adam) at batch train_batch and inference at
+    batch infer_batch, as FLOP rates of the 3x3 convolutions (forward, data
+    gradient and weight gradient of the training step each count one forward's
+    FLOPs) against the dense BF16 MFMA peak."""
+    import torch
+    from hydropol2d_amd import surrogate as S
+    H = W = 256
+    net = S.UNetSurrogate(H, W, batch=max(train_batch, infer_batch), seed=1)
+    rng = np.random.default_rng(0)
+    x32 = torch.from_numpy(rng.normal(size=(train_batch, H, W, 32)).astype(np.float32)).cuda()
+    x = net.to_bf16(x32)
+    y = torch.from_numpy(rng.normal(size=(train_batch, H, W, 2)).astype(np.float32)).cuda()
+    flop_fwd = 0
+    for lv, (ci, co) in enumerate(S.ENC):
+        flop_fwd += 2 * (H >> lv) * (W >> lv) * co * 9 * ci
+    for k, (ci, co) in enumerate(S.DEC):
+        flop_fwd += 2 * (H >> (2 - k)) * (W >> (2 - k)) * co * 9 * ci
+    for _ in range(2):
+        net.train_step(x, y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.train_step(x, y)
+    torch.cuda.synchronize()
+    t_train = (time.perf_counter() - t0) / reps
+    xb = x.repeat(infer_batch // train_batch, 1, 1, 1)
+    net.predict(xb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    net.predict(xb)
+    torch.cuda.synchronize()
+    t_inf = time.perf_counter() - t0
+    peak = 2.5e15
+    train_rate = 3 * flop_fwd * train_batch / t_train
+    inf_rate = flop_fwd * infer_batch / t_inf
+    print(f"[bench] nn: train step {t_train * 1e3:.2f} ms ({train_rate / 1e12:.1f} TFLOP/s), "
+          f"inference {t_inf * 1e3:.2f} ms ({inf_rate / 1e12:.1f} TFLOP/s)", file=sys.stderr,
+          flush=True)
+    return {"grid": [H, W], "conv_gflop_per_sample_forward": flop_fwd / 1e9,
+            "train_batch": train_batch, "ms_per_train_step": t_train * 1e3,
+            "train_tflops": train_rate / 1e12, "train_frac_of_bf16_peak": train_rate / peak,
+            "infer_batch": infer_batch, "ms_per_inference_batch": t_inf * 1e3,
+            "infer_tflops": inf_rate / 1e12, "infer_frac_of_bf16_peak": inf_rate / peak,
+            "peak_tflops": peak / 1e12, "dtype": "bf16 (fp32 accumulation)"}
+
+
 # algorithmic HBM bytes per cell of the D8 drop-in kernels (FP64): the inputs
 # read once, the outputs written once
 D8_BYTES = {
@@ -411,6 +466,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "decomposition is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.nn_only:
+        print(json.dumps({"nn": nn_study()}), flush=True)
+        return
     if args.d8_only:
         print(json.dumps({"d8": d8_study(args.d8_size or 4096)}), flush=True)
         return
@@ -444,7 +502,12 @@ def main():
     # tuner drops the captured graph); the next graph's worth of steps builds
     # the timed window's graph again, so that the timed interval holds
     # launches only
-    tune = sim.launch_shape() if args.no_tune else sim.autotune()
+    if args.launch_shape:
+        tj, wpb, rows = (int(v) for v in args.launch_shape.split(","))
+        sim.set_launch_shape(tj, wpb, rows)
+        tune = sim.launch_shape()
+    else:
+        tune = sim.launch_shape() if args.no_tune else sim.autotune()
     print(f"[bench] launch shape {tune}", file=sys.stderr, flush=True)
     sim.run_fixed(args.graph_steps, graph_steps=args.graph_steps)
     t_start_min = sim.report()["t_min"]
