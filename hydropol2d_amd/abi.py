@@ -67,6 +67,7 @@ class RunConfig(C.Structure):
         ("hi_gravity", C.c_double),
         ("record_time_maps_min", C.c_double),
         ("gw_exact", C.c_int32),
+        ("vg_Ltop", C.c_double), ("vg_dh_max", C.c_double), ("vg_Ksurf_weight", C.c_double),
     ]
 
 
@@ -118,6 +119,7 @@ class RunInputs(C.Structure):
         ("inflow_gauge", _u8p),
         ("rain_map_time_min", _dp),
         ("rain_maps_mmh", _dp),
+        ("impervious", _u8p),
     ]
 
 

@@ -483,6 +483,12 @@ struct hp2d_sim {
     uint8_t *soil = nullptr;
     GaClass *ga = nullptr;
     float *ga_f32 = nullptr;
+    // van Genuchten infiltration (flag_infiltration = 2): class tables and the
+    // depth to the water table per cell (sign bit = impervious)
+    VgClass *vg = nullptr;
+    float *vg_f32 = nullptr;
+    void *vg_zwt = nullptr;
+    double vg_l = 0.5;  // Mualem pore connectivity
     long long *out_idx = nullptr;
     void *out_n = nullptr, *out_q = nullptr, *out_d = nullptr;
     int n_outlet = 0;
@@ -599,7 +605,14 @@ template <typename T> static LiArgs<T> sim_li_args(hp2d_sim *s) {
     a.soil_class = s->soil;
     a.ga = s->ga;
     a.ga_f32 = s->ga_f32;
-    a.n_soil_classes = c.flag_infiltration == 1 ? c.n_soil_classes : 0;
+    a.vg = s->vg;
+    a.vg_f32 = s->vg_f32;
+    a.vg_zwt = (const T *)s->vg_zwt;
+    a.vg_Ltop = T(std::fmax(c.vg_Ltop, 1e-6));
+    a.vg_dhmax = T(c.vg_dh_max);
+    a.vg_wK = T(std::fmin(std::fmax(c.vg_Ksurf_weight, 0.0), 1.0));
+    a.vg_l = T(s->vg_l);
+    a.n_soil_classes = (c.flag_infiltration == 1 || c.flag_infiltration == 2) ? c.n_soil_classes : 0;
     a.routing_time = c.routing_time_min;
     a.I_min = T(s->I_min);
     a.dmax_map = (T *)s->dmax;
@@ -688,7 +701,7 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
     f.n_valid = s->n_valid;
     f.flag_timestep = c.flag_timestep;
     f.adaptive = c.flag_adaptive_timestepping;
-    f.ga = c.flag_infiltration == 1;
+    f.ga = c.flag_infiltration == 1 || c.flag_infiltration == 2;  // I_t is kept
     f.flag_wq = s->wq ? 1 : 0;
     f.flag_gw = s->gw ? 1 : 0;
     f.has_pattern = s->pattern != nullptr;
@@ -810,7 +823,11 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         g_last_error = "hp2d_sim_create: only local-inertial routing with the Bates scheme";
         return HP2D_E_UNSUPPORTED;
     }
-    if (cfg->flag_groundwater == 1 && cfg->flag_infiltration != 1) {
+    if (cfg->flag_infiltration < 0 || cfg->flag_infiltration > 2) {
+        g_last_error = "hp2d_sim_create: flag_infiltration must be 0, 1 (Green-Ampt) or 2 (van Genuchten)";
+        return HP2D_E_ARG;
+    }
+    if (cfg->flag_groundwater == 1 && cfg->flag_infiltration == 0) {
         g_last_error = "hp2d_sim_create: the groundwater module needs the infiltration state I_t";
         return HP2D_E_UNSUPPORTED;
     }
@@ -932,6 +949,40 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
             g.Lu_mm = Lu * 1000.0;
             gac.push_back(g);
         }
+    }
+    // van Genuchten classes and the depth to the water table
+    // (Infiltration_Module.m:194-198: zwt = Soil_Depth - (h_t - (elevation - Soil_Depth)),
+    // taken at max(zwt, 0); the head is gw_h0 until a groundwater update moves it)
+    const bool vg = cfg->flag_infiltration == 2;
+    const bool inf = ga || vg;
+    std::vector<VgClass> vgc;
+    std::vector<double> zw0;
+    if (vg) {
+        if (cfg->n_soil_classes < 1 || cfg->n_soil_classes > kMaxSoilClasses || !in->ksat_mmh ||
+            !in->gw_theta_sat || !in->gw_theta_r || !in->gw_n_vg || !in->gw_alpha_vg ||
+            !in->gw_h0 || !in->gw_soil_depth)
+            return fail(HP2D_E_ARG, "hp2d_sim_create: van Genuchten tables, gw_h0 or gw_soil_depth missing");
+        for (int k = 0; k < cfg->n_soil_classes; ++k) {
+            VgClass g;
+            g.ts = in->gw_theta_sat[k];
+            g.tr = in->gw_theta_r[k];
+            g.nv = in->gw_n_vg[k];
+            g.alpha = in->gw_alpha_vg[k];
+            g.ksat = in->ksat_mmh[k];
+            if (!(g.nv > 1.0) || !(g.alpha > 0.0) || !(g.ksat >= 0.0))
+                return fail(HP2D_E_ARG, "hp2d_sim_create: van Genuchten classes need n > 1, alpha > 0, ksat >= 0");
+            vgc.push_back(g);
+        }
+        s->vg_l = in->gw_l_vg;
+        zw0.resize(n);
+        for (size_t c = 0; c < n; ++c) {
+            const double sd = in->gw_soil_depth[c];
+            const double gw_depth = in->gw_h0[c] - (z[c] - sd);
+            const double zwt = std::fmax(sd - gw_depth, 0.0);
+            zw0[c] = (in->impervious && in->impervious[c]) ? std::copysign(zwt, -1.0) : zwt;
+        }
+    }
+    if (inf) {
         if (in->soil_class)
             for (size_t c = 0; c < n; ++c)
                 if (in->soil_class[c] >= cfg->n_soil_classes)
@@ -1013,7 +1064,7 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
             const size_t c = (size_t)i + (size_t)j * ny;
             const double v = A * d0[c] / 1000.0;
             if (!std::isnan(v)) cp += v;
-            if (ga) {
+            if (inf) {
                 const double u = A * I0[c] / 1000.0;
                 if (!std::isnan(u)) cu += u;
             }
@@ -1043,7 +1094,7 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         A_(&s->snap_vtot, nb);
         A_((void **)&s->snap_t, kSnapSlots * sizeof(double));
         const bool on[HP2D_MAP_FIELDS] = {true, true, true, cfg->flag_human_instability == 1,
-                                          cfg->flag_infiltration == 1, s->gw, s->wq, s->wq, s->wq};
+                                          inf, s->gw, s->wq, s->wq, s->wq};
         for (int f = 0; f < HP2D_MAP_FIELDS; ++f)
             if (on[f]) A_(&s->stack[f], nb * kSnapSlots);
     }
@@ -1060,14 +1111,17 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         A_(&s->d[p], nb);
         A_(&s->qx[p], nb);
         A_(&s->qy[p], nb);
-        if (ga) {
-            A_(&s->I[p], nb);
-            A_(&s->Tc[p], nb);
-        }
+        if (inf) A_(&s->I[p], nb);
+        if (ga) A_(&s->Tc[p], nb);
     }
     A_(&s->dmax, nb);
     if (in->rain_pattern) A_(&s->pattern, nb);
-    if (ga && in->soil_class) A_((void **)&s->soil, n);
+    if (inf && in->soil_class) A_((void **)&s->soil, n);
+    if (vg) {
+        A_((void **)&s->vg, vgc.size() * sizeof(VgClass));
+        A_((void **)&s->vg_f32, vgc.size() * 8 * sizeof(float));
+        A_(&s->vg_zwt, nb);
+    }
     if (ga) A_((void **)&s->ga, gac.size() * sizeof(GaClass));
     if (ga) A_((void **)&s->ga_f32, gac.size() * 8 * sizeof(float));
     s->n_outlet = (int)oidx.size();
@@ -1154,10 +1208,8 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         up_real(s->d[p], d0, 0.0);
         up_real(s->qx[p], zeros, 0.0);
         up_real(s->qy[p], zeros, 0.0);
-        if (ga) {
-            up_real(s->I[p], I0, 0.0);
-            up_real(s->Tc[p], zeros, 0.0);
-        }
+        if (inf) up_real(s->I[p], I0, 0.0);
+        if (ga) up_real(s->Tc[p], zeros, 0.0);
     }
     up_real(s->dmax, d0, 0.0);
     if (in->rain_pattern) up_real(s->pattern, std::vector<double>(in->rain_pattern, in->rain_pattern + n), 0.0);
@@ -1172,6 +1224,20 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
             std::memcpy(&gf[k * 8], v, sizeof(v));
         }
         up(s->ga_f32, gf.data(), gf.size() * sizeof(float));
+    }
+    if (vg) {
+        up(s->vg, vgc.data(), vgc.size() * sizeof(VgClass));
+        std::vector<float> vf(vgc.size() * 8, 0.0f);
+        for (size_t k = 0; k < vgc.size(); ++k) {
+            const VgClass &g = vgc[k];
+            const double m = 1.0 - 1.0 / g.nv;
+            const float v[8] = {(float)g.tr, (float)g.ts, (float)(1.0 / g.nv), (float)m,
+                                (float)(1.0 / m), (float)(-1.0 / m), (float)(1.0 / g.alpha),
+                                (float)(g.ksat / 1000.0 / 3600.0)};
+            std::memcpy(&vf[k * 8], v, sizeof(v));
+        }
+        up(s->vg_f32, vf.data(), vf.size() * sizeof(float));
+        up_real(s->vg_zwt, zw0, 0.0);
     }
     if (s->n_outlet > 0) {
         up(s->out_idx, oidx.data(), oidx.size() * sizeof(long long));
@@ -1353,6 +1419,7 @@ void hp2d_sim_destroy(hp2d_sim *s) {
     if (s->graph) hipGraphExecDestroy(s->graph);
     void *ps[] = {s->z, s->zy, s->n2s, s->pattern, s->dmax, s->d[0], s->d[1], s->qx[0], s->qx[1],
                   s->qy[0], s->qy[1], s->I[0], s->I[1], s->Tc[0], s->Tc[1], s->soil, s->ga, s->ga_f32,
+                  s->vg, s->vg_f32, s->vg_zwt,
                   s->out_idx, s->out_n, s->out_q, s->out_d, s->partials, s->st, s->rain_t,
                   s->rain_v, s->hq, s->ht, s->hd, s->dt_hist, s->vol_err, s->tune_sink,
                   s->tpeak, s->wqB[0], s->wqB[1], s->wqBs, s->wqBx[0], s->wqBx[1], s->wqOm,
@@ -1469,6 +1536,16 @@ static hp2d_status sim_gw_update(hp2d_sim *s, SimState &st) {
     if (rc != HP2D_OK) {
         set_err(s, std::string("groundwater module: ") + hp2d_last_error(s->gwh));
         return rc;
+    }
+    if (s->vg_zwt) {
+        // the head moved: the van Genuchten closure's water-table depth follows
+        const GwmArgs &g = s->gwa;
+        const hipError_t e = s->elem == 8
+            ? launch_vg_water_table<double>(g.h, g.elev, g.soil_depth, g.soil_depth_u,
+                                            (long long)s->n, (double *)s->vg_zwt, s->stream)
+            : launch_vg_water_table<float>(g.h, g.elev, g.soil_depth, g.soil_depth_u,
+                                           (long long)s->n, (float *)s->vg_zwt, s->stream);
+        STRY(s, e);
     }
     s->gw_error = err;
     s->gw_last = st.accepted;

@@ -95,6 +95,14 @@ struct GaClass {
     double ksat, psi, dtheta, Lu_mm, k_out, Tr;
 };
 
+// van Genuchten / Mualem soil class of the Infiltration_Module closure:
+// theta_sat, theta_r, n_vg, alpha_vg [1/m], ksat [mm/h].  The FP32 table holds
+// per class {theta_r, theta_sat, 1/n, m, 1/m, -1/m, 1/alpha, Ks [m/s]} with
+// m = 1 - 1/n.
+struct VgClass {
+    double ts, tr, nv, alpha, ksat;
+};
+
 // ----------------------------------------------------------------------------
 // Small math helpers with MATLAB semantics (max/min omit NaN -> fmax/fmin).
 // ----------------------------------------------------------------------------

@@ -152,6 +152,7 @@ enum : int {
     kFeatSnap = 32,     // the map stacks' velocity snapshots
     kFeatDmax = 64,     // the maximum-depth map
     kFeatTpeak = 128,   // the time-to-peak map
+    kFeatVg = 256,      // van Genuchten infiltration (I, soil class, water-table depth)
     kFeatRuntime = -1,  // all read from the arguments
 };
 template <int FEAT> __device__ __forceinline__ bool feat_on(int bit, bool runtime) {
@@ -162,7 +163,7 @@ template <int FEAT> __device__ __forceinline__ bool feat_on(int bit, bool runtim
 // The run's optional inputs and outputs: fixed at compile time in the
 // specialised kernels (FEAT >= 0), read from the arguments otherwise.
 struct Feats {
-    bool ga, pat, maps, gauge, risk, snap, dmax, tpeak;
+    bool ga, pat, maps, gauge, risk, snap, dmax, tpeak, vg;
 };
 template <int FEAT, typename T> __device__ __forceinline__ Feats feats_of(const LiArgs<T> &a) {
     Feats f;
@@ -174,6 +175,7 @@ template <int FEAT, typename T> __device__ __forceinline__ Feats feats_of(const 
     f.snap = feat_on<FEAT>(kFeatSnap, a.snap_vras != nullptr);
     f.dmax = feat_on<FEAT>(kFeatDmax, a.dmax_map != nullptr);
     f.tpeak = feat_on<FEAT>(kFeatTpeak, a.tpeak_map != nullptr);
+    f.vg = feat_on<FEAT>(kFeatVg, a.vg != nullptr);
     return f;
 }
 
@@ -308,14 +310,136 @@ __device__ __forceinline__ ColLoad<T, G> load_column(const LiArgs<T> &a, const F
                 ld_rows<G, OFF32>(a.rain_pattern, id, v.pat);
             }
             if (F.gauge) ld_rows_u8<G, OFF32>(a.inflow_gauge, id, v.gauge);
-            if (F.ga) {
+            if (F.ga || F.vg) {
                 ld_rows<G, OFF32>(I_src, id, v.I);
-                ld_rows<G, OFF32>(Tc_src, id, v.Tc);
+                // Green-Ampt: the recovery clock; van Genuchten: the water-table depth
+                ld_rows<G, OFF32>(F.ga ? Tc_src : a.vg_zwt, id, v.Tc);
                 if (a.soil_class) ld_rows_u8<G, OFF32>(a.soil_class, id, v.soil);
             }
         }
     }
     return v;
+}
+
+// x^y for x >= 0 in FP32 as exp2(y log2 x) on the hardware's log / exp
+// (0^y = 0 for y > 0, +Inf for y < 0)
+__device__ __forceinline__ float vg_pow(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
+// The van Genuchten / Mualem closure of Infiltration_Module.m:182-473
+// (flag_infiltration = 1, no subgrid, no layered soil) on one cell: dd is the
+// depth after the rain [mm] (the step's eff_depth, updated: :467-470), I the
+// unsaturated-zone storage [mm] (updated: :456-463), zw the depth to the water
+// table zwt = Soil_Depth - (h_t - (elevation - Soil_Depth)) [m] (:194-197) with
+// the sign bit set on impervious cells (idx_imp, :440).  Returns f [mm/h].
+// FP64 follows the reference's operations in order; FP32 takes the powers as
+// exp2 / log2 and the divisions as reciprocal multiplies.
+template <typename T>
+__device__ __forceinline__ T vg_infiltrate(const LiArgs<T> &a, const float4 *lds, int cls, T zw,
+                                           T dt_h, T inv_dt_h, T &dd, T &I) {
+    constexpr bool f64 = sizeof(T) == 8;
+    const bool imp = signbit(zw);
+    const T eff = dd;
+    T zwt = fabs(zw);
+    T ts, tr, f;
+    const T Ltop = a.vg_Ltop, Lmix = a.vg_Ltop;
+    if constexpr (f64) {
+        const VgClass g = a.vg[cls];
+        ts = g.ts;
+        tr = g.tr;
+        const T i_a = eff / fmax(dt_h, T(1e-12));                         // :182
+        const T h0 = fmax(eff, T(0)) / T(1000);                           // :185
+        zwt = fmax(zwt, T(0));                                            // :197-198
+        const T uz_max = zwt * (ts - tr);                                 // :206
+        const T S = fmax(I, T(0)) / T(1000);                              // :210
+        const T S_rem = fmax(uz_max - S, T(0));                           // :216
+        const bool no_uz = (zwt <= T(0)) || (uz_max <= T(0));            // :220
+        const T nv = g.nv, al = g.alpha;
+        const T m = T(1) - T(1) / nv;
+        const T Ks = g.ksat / T(1000) / T(3600);
+        const T zwt_safe = fmax(zwt, T(1e-6));
+        T th_b = tr + (S / zwt_safe);
+        th_b = fmin(fmax(th_b, tr), ts);
+        T Se_b = (th_b - tr) / fmax(ts - tr, T(1e-12));
+        Se_b = fmin(fmax(Se_b, T(1e-6)), T(1));
+        T h_b = -(T(1) / al) * pow(pow(Se_b, -T(1) / m) - T(1), T(1) / nv);
+        if (Se_b >= T(0.999999)) h_b = T(0);
+        const T w_avail = fmax(eff, T(0)) / T(1000);
+        const T w_top_def = Lmix * fmax(ts - th_b, T(0));
+        T w_wet = fmin(w_avail, S_rem);
+        w_wet = fmax(fmin(w_wet, w_top_def), T(0));
+        T th_t = th_b + w_wet / Lmix;
+        th_t = fmin(fmax(th_t, tr), ts);
+        T Se_t = (th_t - tr) / fmax(ts - tr, T(1e-12));
+        Se_t = fmin(fmax(Se_t, T(1e-6)), T(1));
+        T term = T(1) - pow(Se_t, T(1) / m);
+        term = fmin(fmax(term, T(0)), T(1));
+        T Kr = pow(Se_t, a.vg_l) * pow(T(1) - pow(term, m), T(2));
+        Kr = fmin(fmax(Kr, T(0)), T(1));
+        const T Ktop = Ks * Kr, Ksurf = Ks;
+        const bool wet = (i_a > T(0)) || (h0 > T(0));
+        T Ksoil = wet ? a.vg_wK * Ksurf + (T(1) - a.vg_wK) * Ktop : Ktop;
+        Ksoil = fmin(fmax(Ksoil, T(0)), Ksurf);
+        T dh = fmax(h0 - h_b, T(0));
+        dh = fmin(dh, a.vg_dhmax);
+        const T grad = (dh / Ltop) + T(1);
+        T C = Ksoil * grad;
+        C = C * T(1000) * T(3600);
+        const T f_store = (S_rem * T(1000)) / fmax(dt_h, T(1e-12));
+        f = fmin(fmin(i_a, C), f_store);
+        if (imp || no_uz || isnan(f)) f = T(0);
+        f = fmax(f, T(0));
+        const T inf_cell = f * dt_h;
+        T It = fmin(I + inf_cell, uz_max * T(1000));
+        I = fmax(It, T(0));
+        const T dn = dd - inf_cell * a.cell_area / a.C_a;
+        dd = dn < T(0) ? T(0) : dn;
+    } else {
+        // {tr, ts, 1/n, m, 1/m, -1/m, 1/alpha, Ks [m/s]}
+        const float4 g0 = lds[2 * cls], g1 = lds[2 * cls + 1];
+        tr = g0.x;
+        ts = g0.y;
+        const float inv_n = g0.z, m = g0.w, inv_m = g1.x, ninv_m = g1.y, inv_al = g1.z, Ks = g1.w;
+        const float i_a = eff * inv_dt_h;
+        const float h0 = fmaxf(eff, 0.0f) * 1e-3f;
+        zwt = fmaxf(zwt, 0.0f);
+        const float dts = ts - tr;
+        const float uz_max = zwt * dts;
+        const float S = fmaxf(I, 0.0f) * 1e-3f;
+        const float S_rem = fmaxf(uz_max - S, 0.0f);
+        const bool no_uz = (zwt <= 0.0f) || (uz_max <= 0.0f);
+        const float inv_dts = frcp(fmaxf(dts, 1e-12f));
+        float th_b = tr + S * frcp(fmaxf(zwt, 1e-6f));
+        th_b = fminf(fmaxf(th_b, tr), ts);
+        const float Se_b = fminf(fmaxf((th_b - tr) * inv_dts, 1e-6f), 1.0f);
+        float h_b = -inv_al * vg_pow(vg_pow(Se_b, ninv_m) - 1.0f, inv_n);
+        if (Se_b >= 0.999999f) h_b = 0.0f;
+        const float w_avail = h0;
+        const float w_top_def = Lmix * fmaxf(ts - th_b, 0.0f);
+        const float w_wet = fmaxf(fminf(fminf(w_avail, S_rem), w_top_def), 0.0f);
+        float th_t = th_b + w_wet * frcp(Lmix);
+        th_t = fminf(fmaxf(th_t, tr), ts);
+        const float Se_t = fminf(fmaxf((th_t - tr) * inv_dts, 1e-6f), 1.0f);
+        const float term = fminf(fmaxf(1.0f - vg_pow(Se_t, inv_m), 0.0f), 1.0f);
+        const float u = 1.0f - vg_pow(term, m);
+        const float Kr = fminf(fmaxf(vg_pow(Se_t, a.vg_l) * (u * u), 0.0f), 1.0f);
+        const float Ktop = Ks * Kr;
+        const bool wet = (i_a > 0.0f) || (h0 > 0.0f);
+        float Ksoil = wet ? fmadd(a.vg_wK, Ks - Ktop, Ktop) : Ktop;
+        Ksoil = fminf(fmaxf(Ksoil, 0.0f), Ks);
+        const float dh = fminf(fmaxf(h0 - h_b, 0.0f), a.vg_dhmax);
+        const float C = Ksoil * fmadd(dh, frcp(Ltop), 1.0f) * 3.6e6f;
+        const float f_store = S_rem * 1000.0f * inv_dt_h;
+        f = fminf(fminf(i_a, C), f_store);
+        if (imp || no_uz || isnan(f)) f = 0.0f;
+        f = fmaxf(f, 0.0f);
+        const float inf_cell = f * dt_h;
+        I = fmaxf(fminf(I + inf_cell, uz_max * 1000.0f), 0.0f);
+        const float dn = dd - inf_cell * a.cell_area / a.C_a;
+        dd = dn < 0.0f ? 0.0f : dn;
+    }
+    return f;
 }
 
 // One workgroup's tile of one grid: `logical` counts the grid's tiles, row
@@ -394,9 +518,9 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
     // Green-Ampt class table (FP32 path) staged in LDS: the per-cell class
     // gather then costs an LDS read instead of a dependent global load
     __shared__ float4 ga_lds[2 * kMaxSoilClasses];
-    if (MODE == kModeSim && !f64 && F.ga) {
-        for (int i = threadIdx.x; i < 2 * a.n_soil_classes; i += blockDim.x)
-            ga_lds[i] = reinterpret_cast<const float4 *>(a.ga_f32)[i];
+    if (MODE == kModeSim && !f64 && (F.ga || F.vg)) {
+        const float4 *tab = reinterpret_cast<const float4 *>(F.ga ? a.ga_f32 : a.vg_f32);
+        for (int i = threadIdx.x; i < 2 * a.n_soil_classes; i += blockDim.x) ga_lds[i] = tab[i];
         __syncthreads();
     }
 
@@ -523,6 +647,21 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
                             if (may_report && !isnan(In)) s_uz += (double)In;
                         }
                     }
+                    if (F.vg) {
+                        T f = T(0), In = cur.I[k];
+                        if (!nan_cell) {
+                            f = vg_infiltrate<T>(a, ga_lds, cur.soil[k], cur.Tc[k], dt_h, inv_dt_h,
+                                                 dd, In);
+                        } else {
+                            In = T(NAN);
+                            f = T(NAN);
+                        }
+                        In_k[k] = In;
+                        if (own && !fail_step) {
+                            if (!isnan(f)) s_inf += (double)(f * dt_h);
+                            if (may_report && !isnan(In)) s_uz += (double)In;
+                        }
+                    }
                     // Hydrological_Model.m:30: the inflow hydrographs on their gauge cells
                     if (F.gauge && cur.gauge[k] > 0) dd = dd + T(a.st->inflow_agg[cur.gauge[k] - 1]);
                     if (dd < T(-1e-8)) neg = 1;
@@ -536,9 +675,9 @@ __device__ __forceinline__ void li_step_tile(const LiArgs<T> &a, const int logic
                 }
                 hc[k] = dmax(mm2m(dc[k]), T(0));  // depth_cell (:31)
             }
-            if (MODE == kModeSim && F.ga && cell_in && own) {
+            if (MODE == kModeSim && (F.ga || F.vg) && cell_in && own) {
                 st_rows<G, OFF32>(I_dst, id, In_k);
-                st_rows<G, OFF32>(Tc_dst, id, Tn_k);
+                if (F.ga) st_rows<G, OFF32>(Tc_dst, id, Tn_k);
             }
         }
 
@@ -902,7 +1041,8 @@ template <typename T> int li_feature_mask(const LiArgs<T> &a) {
     return (a.ga ? kFeatGa : 0) | (a.rain_pattern ? kFeatPattern : 0) |
            (a.rain_maps ? kFeatMaps : 0) | (a.inflow_gauge ? kFeatGauge : 0) |
            (a.max_risk ? kFeatRisk : 0) | (a.snap_vras ? kFeatSnap : 0) |
-           (a.dmax_map ? kFeatDmax : 0) | (a.tpeak_map ? kFeatTpeak : 0);
+           (a.dmax_map ? kFeatDmax : 0) | (a.tpeak_map ? kFeatTpeak : 0) |
+           (a.vg ? kFeatVg : 0);
 }
 
 // The simulation step: FP32 runs whose optional arrays are one of the common
@@ -917,8 +1057,11 @@ void launch_li_sim(const LiArgs<T> &a, dim3 grid, dim3 block, hipStream_t s) {
     if constexpr (sizeof(T) == 4) {
         constexpr int kGP = kFeatGa | kFeatPattern | kFeatDmax, kG = kFeatGa | kFeatDmax;
         constexpr int kP = kFeatPattern | kFeatDmax, kD = kFeatDmax;
+        constexpr int kVP = kFeatVg | kFeatPattern | kFeatDmax, kV = kFeatVg | kFeatDmax;
         switch (off32 ? li_feature_mask(a) : -1) {
         case kGP: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kGP, G>), grid, block, 0, s, a); return;
+        case kVP: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kVP, G>), grid, block, 0, s, a); return;
+        case kV: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kV, G>), grid, block, 0, s, a); return;
         case kG: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kG, G>), grid, block, 0, s, a); return;
         case kP: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kP, G>), grid, block, 0, s, a); return;
         case kD: hipLaunchKernelGGL((li_step_kernel<T, kModeSim, CRIT, kD, G>), grid, block, 0, s, a); return;

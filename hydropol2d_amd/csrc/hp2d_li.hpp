@@ -41,6 +41,13 @@ template <typename T> struct LiArgs {
     const uint8_t *soil_class;
     const GaClass *ga;    // NULL = no infiltration
     const float *ga_f32;  // the same table as 8 floats per class (FP32 path)
+    // van Genuchten infiltration (Infiltration_Module) in place of Green-Ampt:
+    // class tables (NULL = off), the depth to the water table [m] per cell
+    // (sign bit set on impervious cells), Ltop, dh_max, Ksurf_weight, l_vg
+    const VgClass *vg;
+    const float *vg_f32;
+    const T *vg_zwt;
+    T vg_Ltop, vg_dhmax, vg_wK, vg_l;
     int n_soil_classes;
     T I_min;
     int track_dmax;       // reduce the depth maxima every step (a late rollback can follow)

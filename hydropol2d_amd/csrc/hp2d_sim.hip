@@ -467,4 +467,31 @@ template hipError_t launch_storage<double>(const double *, const double *, long 
                                            int, hipStream_t);
 template hipError_t launch_finalize<double>(const FinArgs<double> &, hipStream_t);
 
+// Infiltration_Module.m:194-198 on the current head, in the reference's order
+template <typename T>
+__global__ __launch_bounds__(256) void vg_water_table_kernel(const double *__restrict__ h,
+                                                             const double *__restrict__ elev,
+                                                             const double *__restrict__ sd_map,
+                                                             double sd_u, long long n, T *zw) {
+    for (long long c = blockIdx.x * 256ll + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+        const double sd = sd_map ? sd_map[c] : sd_u;
+        const double gw_depth = h[c] - (elev[c] - sd);
+        const double zwt = fmax(sd - gw_depth, 0.0);
+        zw[c] = copysign(T(zwt), zw[c]);
+    }
+}
+
+template <typename T>
+hipError_t launch_vg_water_table(const double *h, const double *elev, const double *soil_depth,
+                                 double sd_u, long long n, T *zw, hipStream_t s) {
+    const long long blocks = std::min<long long>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL((vg_water_table_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, h, elev,
+                       soil_depth, sd_u, n, zw);
+    return hipGetLastError();
+}
+template hipError_t launch_vg_water_table<float>(const double *, const double *, const double *,
+                                                 double, long long, float *, hipStream_t);
+template hipError_t launch_vg_water_table<double>(const double *, const double *, const double *,
+                                                  double, long long, double *, hipStream_t);
+
 }  // namespace hp2d

@@ -48,5 +48,11 @@ hipError_t launch_finalize_batch(const FinArgs<T> *args, const int *ids, int n_g
 template <typename T>
 hipError_t launch_storage(const T *d, const T *I, long long n, double *part, int blocks,
                           hipStream_t s);
+// The depth to the water table of the van Genuchten closure after a change of
+// the head h [m]: zw = max(sd - (h - (elev - sd)), 0), keeping the sign bit of
+// the old zw (the impervious flag); sd from soil_depth, or sd_u when NULL
+template <typename T>
+hipError_t launch_vg_water_table(const double *h, const double *elev, const double *soil_depth,
+                                 double sd_u, long long n, T *zw, hipStream_t s);
 
 }  // namespace hp2d

@@ -46,6 +46,37 @@ class GreenAmpt:
 
 
 @dataclass
+class VanGenuchten:
+    """The mainline Infiltration_Module (Infiltration_Module.m:182-473,
+    flag_infiltration = 1 there): van Genuchten / Mualem surface-interface
+    capacity with a Darcy driving head, limited by the supply and by the
+    unsaturated zone between the ground and the water table.
+
+    Per soil class: ksat_mmh [mm/h], theta_sat, theta_r, n_vg (> 1), alpha_vg
+    [1/m].  h0: water-table head [m] (ny x nx); soil_depth: soil above the
+    aquifer base [m] (grid or scalar).  With a `Groundwater` module in the run,
+    the head follows its updates and its soil parameters must be these.
+    impervious: idx_imp (ny x nx, 1 = no infiltration).  Ltop, dh_max,
+    Ksurf_weight, l_vg: Soil_Properties' scalars (the reference's defaults)."""
+
+    ksat_mmh: np.ndarray
+    theta_sat: np.ndarray
+    theta_r: np.ndarray
+    n_vg: np.ndarray
+    alpha_vg: np.ndarray
+    h0: np.ndarray
+    soil_depth: object
+    soil_class: Optional[np.ndarray] = None
+    impervious: Optional[np.ndarray] = None
+    I0_mm: Optional[np.ndarray] = None
+    I_min_mm: float = 0.0
+    l_vg: float = 0.5
+    Ltop: float = 0.05
+    dh_max: float = 1.0
+    Ksurf_weight: float = 0.5
+
+
+@dataclass
 class WaterQuality:
     """Class-indexed build-up / wash-off parameters (preprocessing.m:2159-2174)."""
 
@@ -135,7 +166,7 @@ class RunSpec:
     record_time_hydrographs_min: float = 1.0
     report_every_percent: float = 5.0
     factor_reduction: float = 2.0
-    infiltration: Optional[GreenAmpt] = None
+    infiltration: Optional[object] = None   # GreenAmpt or VanGenuchten
     water_quality: Optional[WaterQuality] = None
     groundwater: Optional[Groundwater] = None
     inflow: Optional[Inflow] = None
@@ -195,7 +226,8 @@ class RunSpec:
         cfg.record_time_hydrographs_min = self.record_time_hydrographs_min
         cfg.report_every_percent = self.report_every_percent
         cfg.factor_reduction = self.factor_reduction
-        cfg.flag_infiltration = 1 if self.infiltration is not None else 0
+        vgi = self.infiltration if isinstance(self.infiltration, VanGenuchten) else None
+        cfg.flag_infiltration = 0 if self.infiltration is None else (2 if vgi is not None else 1)
         cfg.flag_waterquality = 1 if self.water_quality is not None else 0
         cfg.flag_wq_model = self.water_quality.flag_wq_model if self.water_quality else 0
         cfg.flag_groundwater = 1 if self.groundwater is not None else 0
@@ -214,7 +246,7 @@ class RunSpec:
         ins.rain_intensity_mmh = _ptr(ri, C.c_double)
         ins.n_rain = int(min(rt.size, ri.size))
         ins.rain_pattern = _ptr(f(self.rain_pattern), C.c_double)
-        ga = self.infiltration
+        ga = self.infiltration if vgi is None else None
         if ga is not None:
             cfg.n_soil_classes = int(np.atleast_1d(ga.ksat_mmh).size)
             ins.soil_class = _ptr(u(ga.soil_class), C.c_uint8)
@@ -266,7 +298,37 @@ class RunSpec:
             cfg.n_rain_maps = int(maps.shape[2])
             ins.rain_maps_mmh = _ptr(f(maps), C.c_double)
             ins.rain_map_time_min = _ptr(f(np.atleast_1d(self.rain_map_time_min)), C.c_double)
+        if vgi is not None:
+            ny, nx = np.shape(self.elevation)
+            cfg.n_soil_classes = int(np.atleast_1d(vgi.ksat_mmh).size)
+            cfg.vg_Ltop, cfg.vg_dh_max = vgi.Ltop, vgi.dh_max
+            cfg.vg_Ksurf_weight = vgi.Ksurf_weight
+            ins.soil_class = _ptr(u(vgi.soil_class), C.c_uint8)
+            ins.ksat_mmh = _ptr(f(np.atleast_1d(vgi.ksat_mmh)), C.c_double)
+            ins.I0_mm = _ptr(f(vgi.I0_mm), C.c_double)
+            ins.I_min_mm = vgi.I_min_mm
+            ins.impervious = _ptr(u(vgi.impervious), C.c_uint8)
+            ins.gw_h0 = _ptr(f(vgi.h0), C.c_double)
+            sd = np.broadcast_to(np.asarray(vgi.soil_depth, dtype=np.float64), (ny, nx))
+            ins.gw_soil_depth = _ptr(f(sd), C.c_double)
+            ins.gw_theta_sat = _ptr(f(np.atleast_1d(vgi.theta_sat)), C.c_double)
+            ins.gw_theta_r = _ptr(f(np.atleast_1d(vgi.theta_r)), C.c_double)
+            ins.gw_n_vg = _ptr(f(np.atleast_1d(vgi.n_vg)), C.c_double)
+            ins.gw_alpha_vg = _ptr(f(np.atleast_1d(vgi.alpha_vg)), C.c_double)
+            ins.gw_l_vg = vgi.l_vg
         gw = self.groundwater
+        if gw is not None and vgi is not None:
+            # one set of soil parameters per run: the module's must be the closure's
+            same = (np.array_equal(np.asarray(gw.h0, float), np.asarray(vgi.h0, float))
+                    and np.array_equal(np.broadcast_to(np.asarray(gw.soil_depth, float), (ny, nx)),
+                                       np.broadcast_to(np.asarray(vgi.soil_depth, float), (ny, nx)))
+                    and all(np.array_equal(np.atleast_1d(getattr(gw, k)).astype(float),
+                                           np.atleast_1d(getattr(vgi, k)).astype(float))
+                            for k in ("theta_sat", "theta_r", "n_vg", "alpha_vg"))
+                    and gw.l_vg == vgi.l_vg)
+            if not same:
+                raise ValueError("Groundwater and VanGenuchten must carry the same head, soil depth "
+                                 "and van Genuchten parameters")
         if gw is not None:
             ny, nx = np.shape(self.elevation)
             ins.gw_h0 = _ptr(f(gw.h0), C.c_double)

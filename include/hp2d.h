@@ -510,11 +510,14 @@ typedef struct hp2d_run_config {
     double record_time_hydrographs_min;
     double report_every_percent;       /* forced to 5 by Main_While.m:106 */
     double factor_reduction;           /* rollback dt shrink, default 2 */
-    int32_t flag_infiltration;         /* 1 = Green-Ampt */
+    int32_t flag_infiltration;         /* 1 = Green-Ampt (legacy explicit closure);
+                                          2 = the van Genuchten Infiltration_Module
+                                          (Infiltration_Module.m:182-473, Hydrological_Model.m:24) */
     int32_t flag_waterquality;         /* 1 = build-up / wash-off */
     int32_t flag_wq_model;             /* 0 mass model, 1 rating model */
     int32_t flag_groundwater;          /* 1 = Groundwater_Module (Boussinesq baseflow) every
-                                          gw_every_steps accepted steps; needs flag_infiltration */
+                                          gw_every_steps accepted steps; needs flag_infiltration
+                                          1 or 2 */
     int32_t gw_every_steps;
     double gw_courant;                 /* flags.groundwater_courant */
     int32_t n_soil_classes;            /* GA class tables below */
@@ -532,6 +535,10 @@ typedef struct hp2d_run_config {
                                           reference's four passes and operation order (as FP64
                                           runs take them); 0 = two passes that recompute the face
                                           fluxes, with reciprocal multiplies */
+    /* van Genuchten infiltration (flag_infiltration = 2): Soil_Properties.Ltop [m],
+     * dh_max [m], Ksurf_weight [-] (the reference's defaults 0.05, 1.0, 0.5;
+     * Infiltration_Module.m:149-163) */
+    double vg_Ltop, vg_dh_max, vg_Ksurf_weight;
 } hp2d_run_config;
 
 /* Fields of a map-stack record (save_output_maps.m:148-258: Maps.Hydro.d, velocity,
@@ -590,6 +597,12 @@ typedef struct hp2d_run_inputs {
      * rasters; they replace rain_time_min / rain_intensity_mmh / rain_pattern */
     const double *rain_map_time_min;   /* [n_rain_maps] end stamps [min] */
     const double *rain_maps_mmh;       /* [ny*nx x n_rain_maps], NaN outside the domain */
+    /* van Genuchten infiltration (flag_infiltration = 2): the soil classes'
+     * gw_theta_sat, gw_theta_r, gw_n_vg (> 1), gw_alpha_vg and ksat_mmh, gw_l_vg,
+     * the head gw_h0 and gw_soil_depth above give the unsaturated zone; with the
+     * groundwater module the head follows its updates.  I0_mm / I_min_mm set
+     * the initial storage I_t [mm]. */
+    const uint8_t *impervious;         /* optional idx_imp (1 = impervious: no infiltration) */
 } hp2d_run_inputs;
 
 /* Rainfall_Interpolator.m:40-52 / idw_function.m:18-63: inverse-distance

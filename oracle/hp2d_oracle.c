@@ -544,7 +544,8 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     const int64_t n = (int64_t)ny * nx;
     const double R = cfg->resolution, A = R * R;
     if (cfg->routing != HP2D_ROUTING_INERTIAL) return HP2D_E_UNSUPPORTED;
-    if (cfg->flag_groundwater == 1 && cfg->flag_infiltration != 1) return HP2D_E_UNSUPPORTED;
+    if (cfg->flag_groundwater == 1 && cfg->flag_infiltration != 1 && cfg->flag_infiltration != 2)
+        return HP2D_E_UNSUPPORTED;
 
     double *z = zalloc(n), *nman = zalloc(n), *n2 = zalloc(n);
     double *d_t = zalloc(n), *d_p = zalloc(n), *d_tot = zalloc(n);
@@ -562,6 +563,9 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     int32_t n_classes = cfg->n_soil_classes > 0 ? cfg->n_soil_classes : 1;
     ga_tables *ga = (ga_tables *)calloc((size_t)n_classes, sizeof(ga_tables));
     const int do_ga = cfg->flag_infiltration == 1;
+    /* the van Genuchten Infiltration_Module in the same slot (flag_infiltration = 2) */
+    const int do_vg = cfg->flag_infiltration == 2;
+    const int do_inf = do_ga || do_vg;
     if (do_ga)
         for (int32_t k = 0; k < n_classes; ++k) ga[k] = ga_class(in, k);
 
@@ -574,7 +578,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
         if (is_nan[c]) d_t[c] = NAN;
         area_nan[c] = is_nan[c] ? NAN : A; /* mass_balance_check.m:12-13 */
         if (in->outlet_mask && in->outlet_mask[c]) oidx[n_out++] = c; /* find() order */
-        if (do_ga) {
+        if (do_inf) {
             I_t[c] = in->I0_mm ? in->I0_mm[c] : in->I_min_mm;
             if (is_nan[c]) I_t[c] = NAN;
         }
@@ -607,7 +611,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
      * every gw_every_steps accepted steps over the interval since its last call,
      * before the next step's hydrology; per-cell van Genuchten parameters from
      * the soil classes (initialize_groundwater_scheduler.m:38-66 state) */
-    const int do_gw = cfg->flag_groundwater == 1 && do_ga;
+    const int do_gw = cfg->flag_groundwater == 1 && do_inf;
     double *gw_h = NULL, *gw_sd = NULL, *gw_ts = NULL, *gw_tr = NULL, *gw_nv = NULL;
     double *gw_al = NULL, *gw_ks = NULL, *gw_kg = NULL, *gw_Sy = NULL, *gw_q = NULL;
     double *gw_pn = NULL, *gw_pr = NULL, *gw_pc = NULL, *gw_cum = NULL;
@@ -712,7 +716,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
         for (int64_t c = 0; c < n; ++c) tmp[c] = A * d_t[c] / 1000.0;
         double S_p = nansum2(ny, nx, tmp);
         double S_UZ = 0.0;
-        if (do_ga) {
+        if (do_inf) {
             for (int64_t c = 0; c < n; ++c) tmp[c] = A * I_t[c] / 1000.0;
             S_UZ = nansum2(ny, nx, tmp);
         }
@@ -783,6 +787,32 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                 }
                 d_t[c] = d_t[c] - f * dt_h;
                 ga_f[c] = f;
+            }
+            for (int64_t c = 0; c < n; ++c)
+                if (!isnan(ga_f[c])) inf_vol_step += ga_f[c] * dt_h / 1000.0 * A;
+        }
+        /* the van Genuchten closure (Infiltration_Module.m:182-473) on the head of
+         * the last groundwater update (gw_h0 without the module) */
+        if (do_vg) {
+            const double dt_h = time_step / 60.0;
+            const double Ltop = fmax(cfg->vg_Ltop, 1e-6);
+            const double wK = fmin(fmax(cfg->vg_Ksurf_weight, 0.0), 1.0);
+            ORACLE_PAR
+            for (int64_t c = 0; c < n; ++c) {
+                if (is_nan[c]) {
+                    I_t[c] = NAN;
+                    ga_f[c] = NAN;
+                    continue;
+                }
+                const int32_t cls = in->soil_class ? in->soil_class[c] : 0;
+                const double h = do_gw ? gw_h[c] : in->gw_h0[c];
+                const double sd = in->gw_soil_depth[c];
+                const double gw_depth = h - (z[c] - sd);
+                ga_f[c] = oracle_vg_cell(dt_h, A, A, sd - gw_depth, in->gw_theta_sat[cls],
+                                         in->gw_theta_r[cls], in->gw_n_vg[cls],
+                                         in->gw_alpha_vg[cls], in->ksat_mmh[cls],
+                                         in->impervious && in->impervious[c], in->gw_l_vg, Ltop,
+                                         cfg->vg_dh_max, wK, &d_t[c], &I_t[c]);
             }
             for (int64_t c = 0; c < n; ++c)
                 if (!isnan(ga_f[c])) inf_vol_step += ga_f[c] * dt_h / 1000.0 * A;
@@ -952,7 +982,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                         R0[HP2D_MAP_VELOCITY * n + c] = is_nan[c] ? NAN : v;
                         R0[HP2D_MAP_HAZARD * n + c] = is_nan[c] ? NAN : hz;
                         R0[HP2D_MAP_RISK * n + c] = do_hi ? risk[c] : NAN;
-                        R0[HP2D_MAP_INFILTRATION * n + c] = do_ga ? I_t[c] : NAN;
+                        R0[HP2D_MAP_INFILTRATION * n + c] = do_inf ? I_t[c] : NAN;
                         R0[HP2D_MAP_GW_DEPTH * n + c] = do_gw ? gw_h[c] - z[c] + gw_sd[c] : NAN;
                         if (do_wq) {
                             /* I_tot_end_cell of the step (Local_Inertial_Model_D4.m:513-514)
@@ -1024,7 +1054,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                 for (int64_t c = 0; c < n; ++c) tmp[c] = area_nan[c] * d_t[c] / 1000.0;
                 double S_p = nansum2(ny, nx, tmp);
                 double S_UZ = 0.0;
-                if (do_ga) {
+                if (do_inf) {
                     for (int64_t c = 0; c < n; ++c) tmp[c] = area_nan[c] * I_t[c] / 1000.0;
                     S_UZ = nansum2(ny, nx, tmp);
                 }
@@ -1100,7 +1130,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     /* outputs */
     if (out->d_t) memcpy(out->d_t, d_t, (size_t)n * sizeof(double));
     if (out->outflow) memcpy(out->outflow, outflow, (size_t)3 * n * sizeof(double));
-    if (out->I_t && do_ga) memcpy(out->I_t, I_t, (size_t)n * sizeof(double));
+    if (out->I_t && do_inf) memcpy(out->I_t, I_t, (size_t)n * sizeof(double));
     if (out->dmax) memcpy(out->dmax, dmax, (size_t)n * sizeof(double));
     if (do_wq) {
         if (out->B_t) memcpy(out->B_t, B, (size_t)n * sizeof(double));

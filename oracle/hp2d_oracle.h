@@ -220,6 +220,9 @@ void oracle_conv3x3_bwd(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Co
                         const float *x, const float *w, const float *gy, float *gx, float *gw,
                         float *gb);
 
+double oracle_vg_cell(double dt_h, double A, double C_a, double zwt_in, double ts, double tr,
+                      double nv, double a, double ksat_mmh, int imp, double l_vg, double Ltop,
+                      double dh_max, double wK, double *d, double *I);
 int32_t oracle_infiltration(int32_t ny, int32_t nx, double time_step_min, double resolution,
                             double C_a, const double *elevation, const double *h_t,
                             const double *soil_depth, const double *theta_sat,

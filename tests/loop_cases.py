@@ -50,3 +50,36 @@ def initial_mass(spec):
     A = spec.resolution ** 2
     b = wq.C1[cls] * (1.0 - np.exp(-wq.C2[cls] * wq.ADD_days)) * A / 1e4
     return float(np.sum(np.where(np.isnan(spec.elevation), 0.0, b)))
+
+
+def vg_plane(ny=32, nx=40, T=5.0, seed=1, wq=False, gw=False, every=5, d0=5.0, nan_corner=False,
+             precision="fp64", impervious=True, outlets=True, rain_mmh=50.0, I0=None, Ltop=0.05):
+    """The plane with the van Genuchten Infiltration_Module in place of Green-Ampt:
+    three soil classes (sand to clay), the water table 0.2-1.5 m below the ground
+    and at the surface on one row, a fifth of the cells impervious; with gw the
+    groundwater module carries the same head and soil parameters."""
+    from hydropol2d_amd.runspec import VanGenuchten
+    spec = plane(ny, nx, wq=wq, gw=False, T=T, seed=seed, d0=d0, nan_corner=nan_corner,
+                 precision=precision, outlets=outlets)
+    rng = np.random.default_rng(seed + 11)
+    z = np.nan_to_num(spec.elevation, nan=0.0)
+    wt = rng.uniform(0.2, 1.5, (ny, nx))
+    wt[ny // 2, :] = 0.0
+    soil = np.asfortranarray((rng.random((ny, nx)) * 3).astype(np.uint8))
+    imp = np.asfortranarray((rng.random((ny, nx)) < 0.2).astype(np.uint8)) if impervious else None
+    # end-stamped: the first block covers (0, 1000] min
+    spec.rain_time_min = np.array([1000.0, 2000.0])
+    spec.rain_intensity_mmh = np.array([rain_mmh, 0.0])
+    spec.infiltration = VanGenuchten(
+        ksat_mmh=np.array([60.0, 12.0, 2.0]), theta_sat=np.array([0.40, 0.44, 0.47]),
+        theta_r=np.array([0.05, 0.07, 0.09]), n_vg=np.array([2.6, 1.6, 1.25]),
+        alpha_vg=np.array([14.0, 4.0, 1.0]), h0=np.asfortranarray(z - wt), soil_depth=2.0,
+        soil_class=soil, impervious=imp,
+        I0_mm=None if I0 is None else np.full((ny, nx), I0, order="F"), Ltop=Ltop)
+    if gw:
+        v = spec.infiltration
+        from hydropol2d_amd.runspec import Groundwater
+        spec.groundwater = Groundwater(h0=v.h0, soil_depth=v.soil_depth, theta_sat=v.theta_sat,
+                                       theta_r=v.theta_r, n_vg=v.n_vg, alpha_vg=v.alpha_vg,
+                                       ksat_mmh=36.0, Sy=0.2, every_steps=every)
+    return spec
