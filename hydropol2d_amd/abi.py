@@ -120,6 +120,7 @@ class RunInputs(C.Structure):
         ("rain_map_time_min", _dp),
         ("rain_maps_mmh", _dp),
         ("impervious", _u8p),
+        ("ca_h0_mm", _dp),
     ]
 
 

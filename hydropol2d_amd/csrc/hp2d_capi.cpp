@@ -19,6 +19,7 @@
 #include "hp2d_li.hpp"
 #include "hp2d_maps.hpp"
 #include "hp2d_sim.hpp"
+#include "hp2d_simca.hpp"
 #include "hp2d_simwq.hpp"
 
 using namespace hp2d;
@@ -489,6 +490,12 @@ struct hp2d_sim {
     float *vg_f32 = nullptr;
     void *vg_zwt = nullptr;
     double vg_l = 0.5;  // Mualem pore connectivity
+    // CA routing (flags.flag_CA = 1): d_tot, the left / down / outlet outflows,
+    // I_tot_end_cell and the optional h_0; the right / up outflows use qx / qy
+    bool ca = false;
+    void *ca_dtot = nullptr, *ca_ql = nullptr, *ca_qd = nullptr, *ca_qo = nullptr;
+    void *ca_itot = nullptr, *ca_h0 = nullptr;
+    int ca_blocks = 0;
     long long *out_idx = nullptr;
     void *out_n = nullptr, *out_q = nullptr, *out_d = nullptr;
     int n_outlet = 0;
@@ -684,8 +691,10 @@ template <typename T> static FinArgs<T> sim_fin_args(hp2d_sim *s) {
     f.partials = s->partials;
     f.part_cap = s->part_cap;
     // one partial per workgroup of the current launch shape
-    f.n_partials = ((li_n_slabs(s->ny, s->rows) + s->wpb - 1) / s->wpb) *
-                   ((s->nx + s->tj - 1) / s->tj);
+    f.n_partials = s->ca ? s->ca_blocks
+                         : ((li_n_slabs(s->ny, s->rows) + s->wpb - 1) / s->wpb) *
+                               ((s->nx + s->tj - 1) / s->tj);
+    f.ca = s->ca ? 1 : 0;
     f.routing_time = c.routing_time_min;
     f.tsc = c.time_step_change_s;
     f.inc = c.time_step_increments_s;
@@ -789,9 +798,30 @@ template <typename T> static hipError_t sim_after_li(hp2d_sim *s) {
     return launch_snapshot<T>(sim_snap_args<T>(s), s->stream);
 }
 
+template <typename T> static CaSimArgs<T> sim_ca_args(hp2d_sim *s) {
+    CaSimArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.li = sim_li_args<T>(s);
+    a.dtot = (T *)s->ca_dtot;
+    a.ql = (T *)s->ca_ql;
+    a.qd = (T *)s->ca_qd;
+    a.qo = (T *)s->ca_qo;
+    a.itot = (T *)s->ca_itot;
+    a.h0 = (const T *)s->ca_h0;
+    a.blocks = s->ca_blocks;
+    a.depth_tol_mm = T(s->cfg.depth_tolerance_mm);
+    a.slope_outlet = T(s->cfg.slope_outlet);
+    return a;
+}
+
 template <typename T> static hipError_t sim_step_launch(hp2d_sim *s, bool with_finalize) {
-    LiArgs<T> a = sim_li_args<T>(s);
-    hipError_t e = launch_li_step<T>(a, kModeSim, s->stream);
+    hipError_t e;
+    if (s->ca) {
+        e = launch_ca_sim<T>(sim_ca_args<T>(s), s->stream);
+    } else {
+        LiArgs<T> a = sim_li_args<T>(s);
+        e = launch_li_step<T>(a, kModeSim, s->stream);
+    }
     if (e != hipSuccess || !with_finalize) return e;
     return sim_after_li<T>(s);
 }
@@ -819,8 +849,15 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         g_last_error = "hp2d_sim_create: grids are limited to 2^32 - 2^16 cells";
         return HP2D_E_ARG;
     }
-    if (cfg->routing != HP2D_ROUTING_INERTIAL || cfg->flag_numerical_scheme != 1) {
-        g_last_error = "hp2d_sim_create: only local-inertial routing with the Bates scheme";
+    if (cfg->routing == HP2D_ROUTING_CA) {
+        if (cfg->flag_waterquality == 1 || cfg->flag_human_instability == 1 ||
+            cfg->record_time_maps_min > 0) {
+            g_last_error = "hp2d_sim_create: CA routing in the loop runs without the wash-off, the "
+                           "human-instability module and the map stacks";
+            return HP2D_E_UNSUPPORTED;
+        }
+    } else if (cfg->routing != HP2D_ROUTING_INERTIAL || cfg->flag_numerical_scheme != 1) {
+        g_last_error = "hp2d_sim_create: local-inertial routing with the Bates scheme, or CA routing";
         return HP2D_E_UNSUPPORTED;
     }
     if (cfg->flag_infiltration < 0 || cfg->flag_infiltration > 2) {
@@ -847,6 +884,7 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     s->nx = cfg->nx;
     s->n = (size_t)cfg->ny * cfg->nx;
     s->elem = cfg->precision == HP2D_FP64 ? 8 : 4;
+    s->ca = cfg->routing == HP2D_ROUTING_CA;
     s->tj = default_tj(cfg->nx);
     s->rows = li_rows_ok(cfg->ny, 2) ? 2 : 1;
     const size_t n = s->n;
@@ -1135,6 +1173,16 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
     s->n_partials = n_slabs * ((s->nx + 16 - 1) / 16);  // capacity for the smallest tj
     s->part_cap = s->n_partials;
     A_((void **)&s->partials, (size_t)s->n_partials * kPartCount * sizeof(double));
+    if (s->ca) {
+        // one lane per cell, in at most as many workgroups as there are partials
+        s->ca_blocks = (int)std::min<long long>({(long long)((n + 255) / 256), (long long)s->n_partials, 8192LL});
+        A_(&s->ca_dtot, nb);
+        A_(&s->ca_ql, nb);
+        A_(&s->ca_qd, nb);
+        A_(&s->ca_qo, nb);
+        A_(&s->ca_itot, nb);
+        if (in->ca_h0_mm) A_(&s->ca_h0, nb);
+    }
     A_((void **)&s->st, sizeof(SimState));
     s->n_rain = in->n_rain;
     A_((void **)&s->rain_t, std::max(1, in->n_rain) * sizeof(double));
@@ -1212,6 +1260,14 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         if (ga) up_real(s->Tc[p], zeros, 0.0);
     }
     up_real(s->dmax, d0, 0.0);
+    if (s->ca) {
+        // CA_States.I_tot_end_cell starts at 0; the outflows of a step are all written
+        up_real(s->ca_itot, zeros, 0.0);
+        up_real(s->ca_ql, zeros, 0.0);
+        up_real(s->ca_qd, zeros, 0.0);
+        up_real(s->ca_qo, zeros, 0.0);
+        if (s->ca_h0) up_real(s->ca_h0, std::vector<double>(in->ca_h0_mm, in->ca_h0_mm + n), 0.0);
+    }
     if (in->rain_pattern) up_real(s->pattern, std::vector<double>(in->rain_pattern, in->rain_pattern + n), 0.0);
     if (s->soil) up(s->soil, in->soil_class, n);
     if (ga) {
@@ -1419,7 +1475,8 @@ void hp2d_sim_destroy(hp2d_sim *s) {
     if (s->graph) hipGraphExecDestroy(s->graph);
     void *ps[] = {s->z, s->zy, s->n2s, s->pattern, s->dmax, s->d[0], s->d[1], s->qx[0], s->qx[1],
                   s->qy[0], s->qy[1], s->I[0], s->I[1], s->Tc[0], s->Tc[1], s->soil, s->ga, s->ga_f32,
-                  s->vg, s->vg_f32, s->vg_zwt,
+                  s->vg, s->vg_f32, s->vg_zwt, s->ca_dtot, s->ca_ql, s->ca_qd, s->ca_qo,
+                  s->ca_itot, s->ca_h0,
                   s->out_idx, s->out_n, s->out_q, s->out_d, s->partials, s->st, s->rain_t,
                   s->rain_v, s->hq, s->ht, s->hd, s->dt_hist, s->vol_err, s->tune_sink,
                   s->tpeak, s->wqB[0], s->wqB[1], s->wqBs, s->wqBx[0], s->wqBx[1], s->wqOm,
@@ -1717,11 +1774,7 @@ hp2d_status hp2d_sim_run_fixed(hp2d_sim *s, int64_t n_steps, int32_t graph_steps
                 if (rc) return rc;
             }
             STRY(s, hipEventRecord(a0, s->stream));
-            LiArgs<double> ad;
-            hipError_t e;
-            if (s->elem == 8) e = launch_li_step<double>(sim_li_args<double>(s), kModeSim, s->stream);
-            else e = launch_li_step<float>(sim_li_args<float>(s), kModeSim, s->stream);
-            (void)ad;
+            hipError_t e = step_launch(s, false);  // the routing step (LI or the CA passes)
             STRY(s, e);
             STRY(s, hipEventRecord(a1, s->stream));
             if (s->elem == 8) e = sim_after_li<double>(s);
@@ -2050,6 +2103,10 @@ std::map<TuneKey, Shape> g_tune_cache;
 extern "C" hp2d_status hp2d_sim_autotune(hp2d_sim *s, int32_t *best_tj, int32_t *best_wpb,
                                          double *best_ms) {
     if (!s) return HP2D_E_ARG;
+    if (s->ca) {
+        set_err(s, "hp2d_sim_autotune: CA runs have one launch shape (one lane per cell)");
+        return HP2D_E_UNSUPPORTED;
+    }
     int dev = 0;
     STRY(s, hipGetDevice(&dev));
     const TuneKey key{dev, s->ny, s->nx, (int)s->elem};
@@ -2176,6 +2233,10 @@ extern "C" hp2d_status hp2d_sim_launch_shape(hp2d_sim *s, int32_t *tj, int32_t *
 
 extern "C" hp2d_status hp2d_sim_set_launch_shape(hp2d_sim *s, int32_t tj, int32_t wpb, int32_t rows) {
     if (!s) return HP2D_E_ARG;
+    if (s->ca) {
+        set_err(s, "hp2d_sim_set_launch_shape: CA runs have one launch shape (one lane per cell)");
+        return HP2D_E_UNSUPPORTED;
+    }
     if (tj < 16 || tj > 4096 || (wpb != 1 && wpb != 2 && wpb != 4) || !li_rows_ok(s->ny, rows)) {
         set_err(s, "hp2d_sim_set_launch_shape: tj must be 16..4096, wpb 1, 2 or 4, rows 1 (or 2 for an even ny)");
         return HP2D_E_ARG;
@@ -2375,7 +2436,7 @@ hp2d_status hp2d_ensemble_create(int32_t n_members, const hp2d_run_config *cfgs,
     }
     for (int i = 0; i < n_members; ++i) {
         if (cfgs[i].flag_waterquality == 1 || cfgs[i].flag_groundwater == 1 ||
-            cfgs[i].record_time_maps_min > 0) {
+            cfgs[i].record_time_maps_min > 0 || cfgs[i].routing != HP2D_ROUTING_INERTIAL) {
             g_last_error = "hp2d_ensemble_create: members run the surface step and Green-Ampt "
                            "(no wash-off, groundwater module or map stacks in the batched launch)";
             return HP2D_E_UNSUPPORTED;

@@ -51,6 +51,7 @@ enum PartField {
     kPartDprevMax,     // max d_p/1000                         [m]
     kPartNeg,          // 1 if a depth < -1e-8 mm was seen before clean-up
     kPartDnewMax,      // max new d_t/1000 (late rollback celerity)  [m]
+    kPartCaItot,       // CA routing: 1 if any I_tot_end_cell of the step is not 0
     kPartCount
 };
 
@@ -88,6 +89,9 @@ struct SimState {
     // taken so far, and the dt the step used
     int map_last, snap_event, snap_n, n_map_records;
     double dt_used_min;
+    // CA routing: 1 when the last routing left some I_tot_end_cell other than 0
+    // (CA_Routing.m:121 restarts the limiter from 0.9 A d when they sum to 0)
+    double ca_itot_any;
 };
 static_assert(sizeof(SimState) % 4 == 0, "SimState is copied as 32-bit words");
 

@@ -104,12 +104,12 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         }
         if (threadIdx.x < 2 && threadIdx.x < a.n_rain) rain_head[threadIdx.x] = a.rain_t[threadIdx.x];
     }
-    __shared__ double sh[kFinWaves][14];
+    __shared__ double sh[kFinWaves][15];
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
     const double *P = a.partials;
     const long long cap = a.part_cap;
     double storage = 0, uz = 0, rain = 0, infil = 0, vmax = NAN, dtw = NAN, raster = NAN;
-    double dtm = NAN, dpm = NAN, dnm = NAN, neg = 0, qsum = 0, dsum = 0, qraw = 0;
+    double dtm = NAN, dpm = NAN, dnm = NAN, neg = 0, qsum = 0, dsum = 0, qraw = 0, cai = 0;
 #pragma unroll 4
     for (int i = t; i < a.n_partials; i += kFinThreads) {
         storage += P[kPartStorage * cap + i];
@@ -122,6 +122,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         dpm = fmax(dpm, P[kPartDprevMax * cap + i]);
         neg = fmax(neg, P[kPartNeg * cap + i]);
         if (a.flag_wq) dnm = fmax(dnm, P[kPartDnewMax * cap + i]);
+        if (a.ca) cai = fmax(cai, P[kPartCaItot * cap + i]);
     }
     for (int i = t; i < a.n_outlet; i += kFinThreads) {
         qsum += (double)a.out_q[i] * a.A;
@@ -133,12 +134,12 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     vmax = wave_fmax(vmax); dtw = a.adaptive ? wave_fmax(dtw) : wave_fmin(dtw);
     raster = wave_fmax(raster);
     dtm = wave_fmax(dtm); dpm = wave_fmax(dpm); neg = wave_fmax(neg);
-    dnm = wave_fmax(dnm); qraw = wave_sum(qraw);
+    dnm = wave_fmax(dnm); qraw = wave_sum(qraw); cai = wave_fmax(cai);
     if (lane == 0) {
         sh[w][0] = storage; sh[w][1] = uz; sh[w][2] = rain; sh[w][3] = infil;
         sh[w][4] = vmax; sh[w][5] = dtw; sh[w][6] = raster; sh[w][7] = dtm;
         sh[w][8] = dpm; sh[w][9] = neg; sh[w][10] = qsum; sh[w][11] = dsum;
-        sh[w][12] = dnm; sh[w][13] = qraw;
+        sh[w][12] = dnm; sh[w][13] = qraw; sh[w][14] = cai;
     }
     __syncthreads();
     if (t != 0) return;
@@ -149,7 +150,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
     // copy would move the whole copy to scratch memory.
     SimState loc = sst;
     st = &loc;
-    storage = uz = rain = infil = qsum = dsum = qraw = 0.0;
+    storage = uz = rain = infil = qsum = dsum = qraw = cai = 0.0;
     vmax = dtw = raster = dtm = dpm = dnm = NAN;
     neg = 0.0;
     for (int v = 0; v < kFinWaves; ++v) {  // fixed order: deterministic
@@ -160,7 +161,10 @@ __device__ __forceinline__ void finalize_body(const FinArgs<T> &a) {
         dtm = fmax(dtm, sh[v][7]); dpm = fmax(dpm, sh[v][8]); neg = fmax(neg, sh[v][9]);
         qsum += sh[v][10]; dsum += sh[v][11];
         dnm = fmax(dnm, sh[v][12]); qraw += sh[v][13];
+        cai = fmax(cai, sh[v][14]);
     }
+    // the routing's I_tot_end_cell are kept even when the step is rejected
+    if (a.ca) st->ca_itot_any = cai;
     // volumes were reduced as sums of depths [mm]
     const double a1000 = a.A / 1000.0;
     storage *= a1000;

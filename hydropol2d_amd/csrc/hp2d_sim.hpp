@@ -14,6 +14,7 @@ template <typename T> struct FinArgs {
     double routing_time, tsc, inc, alfa_min, alfa_max, R, A, rec, report_pct;
     double factor_reduction, drainage_area, sum_pattern, n_valid;
     int flag_timestep, adaptive, ga, flag_wq, flag_gw, has_pattern, n_rec, vmax_squared;
+    int ca;  // CA routing: keep the I_tot_end_cell flag of the step's routing
     // lumped rainfall series
     const double *rain_t, *rain_v;
     int n_rain;

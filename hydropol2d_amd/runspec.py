@@ -179,6 +179,8 @@ class RunSpec:
     human_instability: Optional[dict] = None
     # map stacks every record_time_maps_min of model time (0: off)
     record_time_maps_min: float = 0.0
+    # routing = "ca": LULC_Properties.h_0 [mm] per cell (CA_Routing.m:110-113; None = 0)
+    ca_h0_mm: Optional[np.ndarray] = None
 
     @property
     def shape(self):
@@ -246,6 +248,7 @@ class RunSpec:
         ins.rain_intensity_mmh = _ptr(ri, C.c_double)
         ins.n_rain = int(min(rt.size, ri.size))
         ins.rain_pattern = _ptr(f(self.rain_pattern), C.c_double)
+        ins.ca_h0_mm = _ptr(f(self.ca_h0_mm), C.c_double)
         ga = self.infiltration if vgi is None else None
         if ga is not None:
             cfg.n_soil_classes = int(np.atleast_1d(ga.ksat_mmh).size)

@@ -494,7 +494,11 @@ hp2d_status hp2d_flow_accumulation(int32_t ny, int32_t nx, const double *z, doub
 typedef struct hp2d_run_config {
     int32_t ny, nx;
     int32_t precision;                 /* HP2D_FP64 / HP2D_FP32 (state + fluxes) */
-    int32_t routing;                   /* HP2D_ROUTING_INERTIAL / HP2D_ROUTING_CA */
+    int32_t routing;                   /* HP2D_ROUTING_INERTIAL, or HP2D_ROUTING_CA: CA_Routing
+                                          (HydroPol2D_Main_While.m:461-465) with the inflow gather and
+                                          mass balance of :631-660 and the CA time step of
+                                          refreshing_timestep.m:31-53; no wash-off, human instability
+                                          or map stacks, one launch shape, no ensembles */
     int32_t flag_numerical_scheme;     /* 1 = Bates */
     int32_t flag_critical;
     int32_t outlet_type;               /* 1 normal slope, 2 critical depth */
@@ -603,6 +607,8 @@ typedef struct hp2d_run_inputs {
      * groundwater module the head follows its updates.  I0_mm / I_min_mm set
      * the initial storage I_t [mm]. */
     const uint8_t *impervious;         /* optional idx_imp (1 = impervious: no infiltration) */
+    const double *ca_h0_mm;            /* CA routing: optional LULC_Properties.h_0 [mm] per cell
+                                          (CA_Routing.m:110-113; NULL = 0) */
 } hp2d_run_inputs;
 
 /* Rainfall_Interpolator.m:40-52 / idw_function.m:18-63: inverse-distance

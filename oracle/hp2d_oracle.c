@@ -358,6 +358,56 @@ double oracle_refresh_dt(int32_t ny, int32_t nx, const double *qout_right,
     return tcr;
 }
 
+/* refreshing_timestep.m:31-53, 93-108, 146-150, 180-353 with flags.flag_CA = 1
+ * (D4): face velocities are the outflow rates over the cell depth d_t, the
+ * cell components the plain differences, the wave depth d_t / 1000.
+ * q4: qout_left, qout_right, qout_up, qout_down (ny*nx each). */
+double oracle_refresh_dt_ca(int32_t ny, int32_t nx, const double *q4, const double *d_t,
+                            oracle_dt_params *p, double *max_velocity, double *velocity_raster) {
+    const int64_t n = (int64_t)ny * nx;
+    const double R = p->resolution;
+    double vmax = NAN, dtw = NAN;
+    for (int64_t c = 0; c < n; ++c) {
+        const double fd = d_t[c];
+        const double safe = (fd < p->depth_tolerance_mm) ? 1e12 : fd;
+        const double vl = (q4[c] / 1000.0 / 3600.0) * R / (safe / 1000.0);
+        const double vr = (q4[n + c] / 1000.0 / 3600.0) * R / (safe / 1000.0);
+        const double vu = (q4[2 * n + c] / 1000.0 / 3600.0) * R / (safe / 1000.0);
+        const double vd = (q4[3 * n + c] / 1000.0 / 3600.0) * R / (safe / 1000.0);
+        const double raster = fmax(fmax(fmax(fabs(vl), fabs(vr)), fabs(vu)), fabs(vd));
+        if (velocity_raster) velocity_raster[c] = raster;
+        const double rc = vr - vl, uc = vu - vd; /* :146-150, no 0.5 for CA */
+        const double tot = hypot(rc, uc);
+        if (p->total_velocity) p->total_velocity[c] = tot;
+        const int stagnant = tot < 0.01;
+        vmax = fmax(vmax, tot);
+        double cw = sqrt(9.81 * fmax(fd / 1000.0, 0.0)); /* depth_timestep = d_t / 1000 */
+        cw = fmax(cw, 1e-10);
+        if (isinf(cw)) cw = NAN;
+        if (stagnant) cw = NAN;
+        const double cand = (p->flag_adaptive_timestepping == 1) ? p->alfa_min * R / cw
+                                                                 : p->alfa_min * R / (raster + cw);
+        dtw = fmin(dtw, cand);
+    }
+    *max_velocity = vmax;
+    double dt_adv;
+    if (vmax > 0) {
+        dt_adv = 1.0 * R / fmax(vmax, 1e-10);
+        const double tsf = fmax(p->alfa_max - 2.0 * fmax(vmax - 5.0, 0.0), p->alfa_min);
+        dt_adv = dt_adv * tsf;
+        dt_adv = fmin(dt_adv, p->max_time_step);
+    } else {
+        dt_adv = p->max_time_step;
+    }
+    if (isnan(dtw)) dtw = p->max_time_step;
+    double nt = (p->flag_timestep == 1) ? fmin(dt_adv, dtw) : dtw;
+    if (p->flag_waterquality == 1) nt = fmin(1.0 * p->tmin_wq, nt);
+    if (isnan(nt) || isinf(nt)) nt = p->max_time_step;
+    p->nt_raw = nt;
+    double tcr = fmax(p->time_step_increments * floor(nt / p->time_step_increments), p->min_time_step);
+    return fmin(tcr, p->max_time_step);
+}
+
 /* ------------------------------------------------------------------------ */
 /* A18  localAverageRateOverStep + localAdvanceEndStampedCursor +            */
 /*      localEstimateConstantDt, update_spatial_BC.m:1095-1185,1240-1250.    */
@@ -543,7 +593,13 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     const int32_t ny = cfg->ny, nx = cfg->nx;
     const int64_t n = (int64_t)ny * nx;
     const double R = cfg->resolution, A = R * R;
-    if (cfg->routing != HP2D_ROUTING_INERTIAL) return HP2D_E_UNSUPPORTED;
+    /* CA routing (flags.flag_CA = 1) runs without the wash-off, the human
+     * instability module and the map stacks, as the device loop does */
+    const int do_ca = cfg->routing == HP2D_ROUTING_CA;
+    if (!do_ca && cfg->routing != HP2D_ROUTING_INERTIAL) return HP2D_E_UNSUPPORTED;
+    if (do_ca && (cfg->flag_waterquality == 1 || cfg->flag_human_instability == 1 ||
+                  cfg->record_time_maps_min > 0))
+        return HP2D_E_UNSUPPORTED;
     if (cfg->flag_groundwater == 1 && cfg->flag_infiltration != 1 && cfg->flag_infiltration != 2)
         return HP2D_E_UNSUPPORTED;
 
@@ -554,6 +610,11 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     double *I_t = zalloc(n), *I_p = zalloc(n), *Tclk = zalloc(n), *vraster = zalloc(n);
     double *area_nan = zalloc(n), *ga_f = zalloc(n);
     double *vtot = zalloc(n), *max_risk = zalloc(n), *risk = zalloc(n);
+    /* CA: the outflow pages (left, right, up, down, outlet), the gathered
+     * inflows and CA_States.I_tot_end_cell (kept through rejected steps, as the
+     * reference's catch block keeps it) */
+    double *ca_q = do_ca ? zalloc(5 * n) : NULL, *ca_qin = do_ca ? zalloc(n) : NULL;
+    double *ca_itot = do_ca ? zalloc(n) : NULL;
     const int do_hi = cfg->flag_human_instability == 1;
     const int do_maps = cfg->record_time_maps_min > 0;
     int32_t last_record_maps = 1; /* recording_parameters.last_record_maps (preprocessing.m:2396) */
@@ -841,10 +902,23 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
             if (d_t[c] < 0) d_t[c] = 0.0;
             d_tot[c] = d_t[c];
         }
-        /* ---------------- Local_Inertial_Model_D4 (:596-615) ---------------- */
-        oracle_li_d4(ny, nx, cfg->flag_numerical_scheme, z, d_tot, nman, n2, A, time_step, R,
-                     cfg->outlet_type, cfg->slope_outlet, oidx, n_out, outflow_prev,
-                     cfg->flag_critical, ql, qr, qu, qd, qof, d_t, NULL, outflow, Hf);
+        if (do_ca) {
+            /* ---------------- CA_Routing (:461-465), gather + mass balance (:631-660) ---------------- */
+            oracle_ca_routing(ny, nx, z, d_tot, nman, A, time_step, in->ca_h0_mm, R, ca_itot,
+                              cfg->outlet_type, cfg->slope_outlet, oidx, n_out, cfg->flag_critical,
+                              ca_q, ca_q + n, ca_q + 2 * n, ca_q + 3 * n, ca_q + 4 * n, d_t, ca_itot);
+            oracle_gather(ny, nx, ca_q, 5, time_step, ca_qin, d_t);
+            /* the outflow pages kept as the local-inertial ones are: right, up, outlet */
+            memcpy(outflow, ca_q + n, (size_t)n * sizeof(double));
+            memcpy(outflow + n, ca_q + 2 * n, (size_t)n * sizeof(double));
+            memcpy(outflow + 2 * n, ca_q + 4 * n, (size_t)n * sizeof(double));
+            memcpy(qof, ca_q + 4 * n, (size_t)n * sizeof(double));
+        } else {
+            /* ---------------- Local_Inertial_Model_D4 (:596-615) ---------------- */
+            oracle_li_d4(ny, nx, cfg->flag_numerical_scheme, z, d_tot, nman, n2, A, time_step, R,
+                         cfg->outlet_type, cfg->slope_outlet, oidx, n_out, outflow_prev,
+                         cfg->flag_critical, ql, qr, qu, qd, qof, d_t, NULL, outflow, Hf);
+        }
         if (inject > 0 && k == inject) {
             failed = 1;
             inject = 0;
@@ -874,7 +948,8 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
                 dp.flag_waterquality = do_wq;  /* refreshing_timestep.m:329-332 */
                 dp.tmin_wq = tmin_wq;
                 dp.total_velocity = vtot;
-                tcr = oracle_refresh_dt(ny, nx, qr, qu, Hf, &dp, &max_velocity, vraster);
+                tcr = do_ca ? oracle_refresh_dt_ca(ny, nx, ca_q, d_t, &dp, &max_velocity, vraster)
+                            : oracle_refresh_dt(ny, nx, qr, qu, Hf, &dp, &max_velocity, vraster);
                 if (out->dt_follow && n_dt < out->n_follow && out->dt_follow[n_dt] != tcr) {
                     const double inc = cfg->time_step_increments_s;
                     const double u = dp.nt_raw / inc;
@@ -1191,6 +1266,7 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     free(oidx);
     free(ga);
     free(B); free(C3m); free(C4m); free(Pc); free(TW); free(ql); free(qd);
+    free(ca_q); free(ca_qin); free(ca_itot);
     free(gw_h); free(gw_sd); free(gw_ts); free(gw_tr); free(gw_nv); free(gw_al); free(gw_ks);
     free(gw_kg); free(gw_Sy); free(gw_q); free(gw_pn); free(gw_pr); free(gw_pc); free(gw_cum);
     free(gw_dom);

@@ -61,6 +61,11 @@ double oracle_refresh_dt(int32_t ny, int32_t nx, const double *qout_right,
                          oracle_dt_params *p, double *max_velocity,
                          double *velocity_raster);
 
+/* the same with flags.flag_CA = 1: q4 = qout_left, right, up, down; d_t the
+ * depth after the gather */
+double oracle_refresh_dt_ca(int32_t ny, int32_t nx, const double *q4, const double *d_t,
+                            oracle_dt_params *p, double *max_velocity, double *velocity_raster);
+
 double oracle_rain_avg(const double *times, const double *values, int32_t n, double t0,
                        double t1, int32_t *cursor);
 

@@ -83,3 +83,27 @@ def vg_plane(ny=32, nx=40, T=5.0, seed=1, wq=False, gw=False, every=5, d0=5.0, n
                                        theta_r=v.theta_r, n_vg=v.n_vg, alpha_vg=v.alpha_vg,
                                        ksat_mmh=36.0, Sy=0.2, every_steps=every)
     return spec
+
+
+def ca_plane(ny=32, nx=40, T=5.0, seed=1, gw=False, every=3, d0=5.0, nan_corner=False,
+             precision="fp64", outlets=True, infiltration="ga", outlet_type=2, slope=0.0,
+             critical=0, h0=False):
+    """The plane routed by CA_Routing (flags.flag_CA = 1) instead of the
+    local-inertial step, with Green-Ampt ("ga"), the van Genuchten module
+    ("vg", thick wetting layer) or no infiltration (None)."""
+    if infiltration == "vg":
+        spec = vg_plane(ny, nx, T=T, seed=seed, gw=gw, every=every, d0=d0, nan_corner=nan_corner,
+                        precision=precision, outlets=outlets, Ltop=0.5)
+    else:
+        spec = plane(ny, nx, wq=False, gw=gw, T=T, every=every, seed=seed, d0=d0,
+                     nan_corner=nan_corner, precision=precision, outlets=outlets)
+        if infiltration is None:
+            spec.infiltration = None
+    spec.routing = "ca"
+    spec.outlet_type = outlet_type
+    spec.slope_outlet = slope
+    spec.flag_critical = critical
+    if h0:
+        rng = np.random.default_rng(seed + 3)
+        spec.ca_h0_mm = np.asfortranarray(rng.uniform(0.0, 3.0, (ny, nx)))
+    return spec
