@@ -2319,7 +2319,9 @@ namespace {
 
 __global__ void gather_done_kernel(SimState *const *states, int n, int *done) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) done[i] = (states[i]->done || states[i]->paused) ? 1 : 0;
+    // (a member paused at its groundwater call is not done: the host makes the
+    // call and releases it before the next chunk)
+    if (i < n) done[i] = states[i]->done ? 1 : 0;
 }
 
 int member_tiles(const hp2d_sim *s) {
@@ -2352,18 +2354,32 @@ hp2d_status build_tiles(hp2d_ensemble *e) {
     return HP2D_OK;
 }
 
+// the wash-off of each active member that has one, after the batched step
+// (HydroPol2D_Main_While.m:683-695): the member's own kernels and arguments
+template <typename T> hipError_t ensemble_wq(hp2d_ensemble *e) {
+    for (int id : e->active) {
+        hp2d_sim *m = e->m[id];
+        if (!m->wq) continue;
+        const hipError_t r = launch_wq_step<T>(sim_wq_args<T>(m), e->stream);
+        if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+}
+
 hipError_t ensemble_step(hp2d_ensemble *e) {
     hipError_t r;
     const int na = (int)e->active.size();
     if (e->elem == 4) {
         r = launch_li_step_batch<float>((const LiArgs<float> *)e->li_args, e->tile_grid,
                                         e->first_tile, e->n_tiles, e->wpb, e->critical, e->stream);
+        if (r == hipSuccess) r = ensemble_wq<float>(e);
         if (r == hipSuccess)
             r = launch_finalize_batch<float>((const FinArgs<float> *)e->fin_args, e->active_ids, na,
                                              e->stream);
     } else {
         r = launch_li_step_batch<double>((const LiArgs<double> *)e->li_args, e->tile_grid,
                                          e->first_tile, e->n_tiles, e->wpb, e->critical, e->stream);
+        if (r == hipSuccess) r = ensemble_wq<double>(e);
         if (r == hipSuccess)
             r = launch_finalize_batch<double>((const FinArgs<double> *)e->fin_args, e->active_ids,
                                               na, e->stream);
@@ -2435,10 +2451,9 @@ hp2d_status hp2d_ensemble_create(int32_t n_members, const hp2d_run_config *cfgs,
         return HP2D_E_ARG;
     }
     for (int i = 0; i < n_members; ++i) {
-        if (cfgs[i].flag_waterquality == 1 || cfgs[i].flag_groundwater == 1 ||
-            cfgs[i].record_time_maps_min > 0 || cfgs[i].routing != HP2D_ROUTING_INERTIAL) {
-            g_last_error = "hp2d_ensemble_create: members run the surface step and Green-Ampt "
-                           "(no wash-off, groundwater module or map stacks in the batched launch)";
+        if (cfgs[i].record_time_maps_min > 0 || cfgs[i].routing != HP2D_ROUTING_INERTIAL) {
+            g_last_error = "hp2d_ensemble_create: members run the local-inertial step (no CA "
+                           "routing or map stacks in the batched launch)";
             return HP2D_E_UNSUPPORTED;
         }
     }
@@ -2550,9 +2565,32 @@ hp2d_status hp2d_ensemble_run(hp2d_ensemble *e, int64_t max_steps, int32_t graph
             if (rc) return rc;
         }
     }
+    // members with the groundwater module pause at their next call (every N
+    // accepted steps of their own); a chunk is at most the shortest such cadence
+    int gw_min = 0;
+    for (hp2d_sim *m : e->m)
+        if (m->gw) gw_min = gw_min == 0 ? m->cfg.gw_every_steps : std::min(gw_min, m->cfg.gw_every_steps);
+    if (gw_min > 0 && graph_steps > gw_min) graph_steps = gw_min;
+    const int64_t eager_chunk = gw_min > 0 ? std::min(64, gw_min) : 64;
     while (steps < max_steps && !e->active.empty()) {
+        // groundwater calls that are due, then each such member's pause point
+        for (int id : e->active) {
+            hp2d_sim *m = e->m[id];
+            if (!m->gw) continue;
+            SimState st;
+            hp2d_status rc = read_state(m, st);
+            if (!rc && st.accepted - m->gw_last >= m->cfg.gw_every_steps) rc = sim_gw_update(m, st);
+            if (rc) {
+                set_err(e, "member " + std::to_string(id) + ": " + m->err);
+                return rc;
+            }
+            st.stop_accepted = m->gw_last + m->cfg.gw_every_steps;
+            st.paused = 0;
+            if ((rc = write_state(m, st))) return rc;
+            ETRY(e, hipStreamSynchronize(m->stream));
+        }
         const int64_t chunk = graph_steps > 0 ? std::min<int64_t>(graph_steps, max_steps - steps)
-                                              : std::min<int64_t>(64, max_steps - steps);
+                                              : std::min<int64_t>(eager_chunk, max_steps - steps);
         if (graph_steps > 0 && chunk == graph_steps) {
             hp2d_status rc = ensemble_graph(e, graph_steps);
             if (rc) return rc;
@@ -2561,6 +2599,24 @@ hp2d_status hp2d_ensemble_run(hp2d_ensemble *e, int64_t max_steps, int32_t graph
             for (int64_t k = 0; k < chunk; ++k) ETRY(e, ensemble_step(e));
         }
         steps += chunk;
+        // a wash-off split longer than the captured sub-steps held its member:
+        // the host completes the held step (as the single-run loop does)
+        bool any_wq = false;
+        for (int id : e->active) any_wq = any_wq || e->m[id]->wq;
+        if (any_wq) {
+            ETRY(e, hipStreamSynchronize(e->stream));
+            for (int id : e->active) {
+                hp2d_sim *m = e->m[id];
+                if (!m->wq) continue;
+                SimState st;
+                hp2d_status rc = read_state(m, st);
+                if (!rc && st.hold) rc = sim_release_hold(m, st);
+                if (rc) {
+                    set_err(e, "member " + std::to_string(id) + ": " + m->err);
+                    return rc;
+                }
+            }
+        }
         hipLaunchKernelGGL(gather_done_kernel, dim3(((int)e->m.size() + 255) / 256), dim3(256), 0,
                            e->stream, (SimState *const *)e->states, (int)e->m.size(), e->done);
         ETRY(e, hipGetLastError());
@@ -2583,6 +2639,15 @@ hp2d_status hp2d_ensemble_run(hp2d_ensemble *e, int64_t max_steps, int32_t graph
     float ms = 0;
     ETRY(e, hipEventElapsedTime(&ms, e0, e1));
     e->last_ms = ms;
+    for (hp2d_sim *m : e->m) {  // no pause point outlives the call
+        if (!m->gw) continue;
+        SimState st;
+        hp2d_status rc = read_state(m, st);
+        if (rc) return rc;
+        st.stop_accepted = -1;
+        st.paused = 0;
+        if ((rc = write_state(m, st))) return rc;
+    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (n_done) *n_done = (int32_t)(e->m.size() - e->active.size());

@@ -154,3 +154,39 @@ def test_c4_full_batch_members_equal_solo_runs():
         for k in ("d_t", "outflow", "dmax", "I_t"):
             np.testing.assert_array_equal(st[k], st0[k], err_msg=f"member {i} {k}")
         assert rep == rep0, i
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("graph_steps", [0, 20])
+def test_wash_off_and_groundwater_members_equal_solo_runs(precision, graph_steps):
+    """Members with the wash-off (Calibration_Water_Quality.m's batched
+    members), with the groundwater module (each called every N of its own
+    accepted steps), with both, and plain ones, in one batched run: each equals
+    the same member run alone, bit for bit, including the pollutant and head
+    fields and the ledgers."""
+    import loop_cases
+    from hydropol2d_amd.simulation import Ensemble, Simulation
+    specs = [loop_cases.plane(24, 30, C3=3e4, gw=False, precision=precision, T=3.0),
+             loop_cases.plane(32, 40, wq=False, every=3, precision=precision, T=4.0),
+             loop_cases.plane(40, 28, C3=3e5, min_Bt=5.0, every=5, precision=precision, seed=2,
+                              T=3.5),
+             loop_cases.plane(20, 20, wq=False, gw=False, precision=precision, seed=3, T=2.0)]
+    with Ensemble(specs) as ens:
+        assert ens.run(graph_steps=graph_steps) == len(specs)
+        for i, sp in enumerate(specs):
+            wq, gw = sp.water_quality is not None, sp.groundwater is not None
+            m = ens.member(i)
+            st = m.state(infiltration=True, water_quality=wq, groundwater=gw)
+            rep, shape = m.report(), m.launch_shape()
+            with Simulation(sp) as s:
+                s.set_launch_shape(shape["tj"], shape["waves_per_block"], shape["rows"])
+                s.run()
+                st0 = s.state(infiltration=True, water_quality=wq, groundwater=gw)
+                rep0 = s.report()
+            for k in st0:
+                np.testing.assert_array_equal(st[k], st0[k], err_msg=f"member {i} {k}")
+            assert rep == rep0, i
+            if gw:
+                assert rep["gw_updates"] > 0
+            if wq:
+                assert rep["wq_mass_kg"] > 0

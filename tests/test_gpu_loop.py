@@ -146,8 +146,12 @@ def test_fp32_loop_within_bounds_of_fp64_oracle():
     assert rep["wq_mass_kg"] == pytest.approx(o["report"]["wq_mass_kg"], rel=1e-3)
 
 
-def test_ensemble_refuses_water_quality_members():
+def test_ensemble_refuses_map_stack_members():
+    # members with wash-off and groundwater are batched (tests/test_gpu_ensemble.py);
+    # map stacks are not
     from hydropol2d_amd._lib import HP2DError
     from hydropol2d_amd.simulation import Ensemble
+    spec = loop_cases.plane(C3=3e4, gw=False)
+    spec.record_time_maps_min = 1.0
     with pytest.raises(HP2DError):
-        Ensemble([loop_cases.plane(C3=3e4, gw=False)])
+        Ensemble([spec])
