@@ -76,6 +76,9 @@ def parse():
     p.add_argument("--nn-only", action="store_true",
                    help="run only the surrogate network's kernel study (training steps at batch "
                         "32 and inference at batch 256 on 256 x 256 stacks) and print its JSON line")
+    p.add_argument("--variants-only", action="store_true",
+                   help="time the loop's other paths on the full-physics workload: the van Genuchten "
+                        "closure in place of Green-Ampt, and CA routing (without the wash-off)")
     p.add_argument("--d8-only", action="store_true",
                    help="run only the D8 drop-in study and print its JSON line")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -457,6 +460,49 @@ def algorithmic_bytes_per_cell(spec, elem):
     return reads, writes        # the maximum rises, so its write is not counted
 
 
+def variant_study(n=4096, steps=400, spin_steps=2000, graph_steps=50):
+    """The full-physics workload (FP32) with the van Genuchten Infiltration_Module
+    in place of Green-Ampt (the aquifer's soil tables, Ltop 5 cm), and with CA
+    routing in place of the local-inertial step (the loop runs CA without the
+    wash-off).  After a spin-up, `steps` steps are timed as graphs; the routing
+    kernels' own time is the average over 20 eager steps timed with HIP events."""
+    from hydropol2d_amd import synthetic
+    from hydropol2d_amd.runspec import VanGenuchten
+    from hydropol2d_amd.simulation import Simulation
+    import torch
+    out = {}
+    for name in ("vg", "ca"):
+        spec = synthetic.full_physics_c2(n, n, 2.0, precision="fp32", routing_time_min=1e6)
+        if name == "vg":
+            g, ga = spec.groundwater, spec.infiltration
+            spec.infiltration = VanGenuchten(
+                ksat_mmh=ga.ksat_mmh, theta_sat=g.theta_sat, theta_r=g.theta_r, n_vg=g.n_vg,
+                alpha_vg=g.alpha_vg, h0=g.h0, soil_depth=g.soil_depth, soil_class=ga.soil_class,
+                l_vg=g.l_vg)
+        else:
+            spec.routing = "ca"
+            spec.water_quality = None
+        with Simulation(spec) as sim:
+            sim.run_fixed(spin_steps, graph_steps=graph_steps)
+            torch.cuda.synchronize() if torch.cuda.is_available() else None
+            t0 = time.perf_counter()
+            sim.run_fixed(steps, graph_steps=graph_steps)
+            wall = time.perf_counter() - t0
+            rep = sim.report()
+            sim.run_fixed(20, graph_steps=0)
+            k_ms, _, launches = sim.kernel_time()
+        ms = wall / steps * 1e3
+        out[name] = {"ms_per_step": ms, "G_cell_updates_per_s": n * n / (ms * 1e-3) / 1e9,
+                     "routing_kernel_ms": k_ms, "routing_launches_timed": launches,
+                     "t_min": rep["t_min"],
+                     "workload": f"{n}x{n} full-physics FP32, " +
+                                 ("van Genuchten infiltration" if name == "vg" else
+                                  "CA routing, no wash-off")}
+        print(f"[bench] variant {name}: {ms:.3f} ms/step, routing kernels "
+              f"{out[name]['routing_kernel_ms']:.3f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -471,6 +517,9 @@ def main():
         return
     if args.d8_only:
         print(json.dumps({"d8": d8_study(args.d8_size or 4096)}), flush=True)
+        return
+    if args.variants_only:
+        print(json.dumps({"variants": variant_study(args.size, args.steps)}), flush=True)
         return
     from hydropol2d_amd import synthetic
     from hydropol2d_amd.simulation import Simulation
