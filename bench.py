@@ -484,6 +484,8 @@ def variant_study(n=4096, steps=400, spin_steps=2000, graph_steps=50):
             spec.water_quality = None
         with Simulation(spec) as sim:
             sim.run_fixed(spin_steps, graph_steps=graph_steps)
+            if name == "vg":  # the step kernel's launch shape, tuned as the default run's is
+                sim.autotune()
             torch.cuda.synchronize() if torch.cuda.is_available() else None
             t0 = time.perf_counter()
             sim.run_fixed(steps, graph_steps=graph_steps)
@@ -491,8 +493,10 @@ def variant_study(n=4096, steps=400, spin_steps=2000, graph_steps=50):
             rep = sim.report()
             sim.run_fixed(20, graph_steps=0)
             k_ms, _, launches = sim.kernel_time()
+            sim_shape = sim.launch_shape() if name == "vg" else "one lane per cell"
         ms = wall / steps * 1e3
         out[name] = {"ms_per_step": ms, "G_cell_updates_per_s": n * n / (ms * 1e-3) / 1e9,
+                     "launch": sim_shape,
                      "routing_kernel_ms": k_ms, "routing_launches_timed": launches,
                      "t_min": rep["t_min"],
                      "workload": f"{n}x{n} full-physics FP32, " +

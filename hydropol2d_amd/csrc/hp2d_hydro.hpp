@@ -165,9 +165,11 @@ __device__ __forceinline__ T vg_infiltrate(const LiArgs<T> &a, const float4 *lds
         float th_t = th_b + w_wet * frcp(Lmix);
         th_t = fminf(fmaxf(th_t, tr), ts);
         const float Se_t = fminf(fmaxf((th_t - tr) * inv_dts, 1e-6f), 1.0f);
-        const float term = fminf(fmaxf(1.0f - vg_pow(Se_t, inv_m), 0.0f), 1.0f);
+        // Se_top^(1/m) and Se_top^l share one log2
+        const float lse = __builtin_amdgcn_logf(Se_t);
+        const float term = fminf(fmaxf(1.0f - __builtin_amdgcn_exp2f(inv_m * lse), 0.0f), 1.0f);
         const float u = 1.0f - vg_pow(term, m);
-        const float Kr = fminf(fmaxf(vg_pow(Se_t, a.vg_l) * (u * u), 0.0f), 1.0f);
+        const float Kr = fminf(fmaxf(__builtin_amdgcn_exp2f(a.vg_l * lse) * (u * u), 0.0f), 1.0f);
         const float Ktop = Ks * Kr;
         const bool wet = (i_a > 0.0f) || (h0 > 0.0f);
         float Ksoil = wet ? fmadd(a.vg_wK, Ks - Ktop, Ktop) : Ktop;
