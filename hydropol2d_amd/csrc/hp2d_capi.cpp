@@ -850,10 +850,8 @@ hp2d_status hp2d_sim_create(const hp2d_run_config *cfg, const hp2d_run_inputs *i
         return HP2D_E_ARG;
     }
     if (cfg->routing == HP2D_ROUTING_CA) {
-        if (cfg->flag_waterquality == 1 || cfg->flag_human_instability == 1 ||
-            cfg->record_time_maps_min > 0) {
-            g_last_error = "hp2d_sim_create: CA routing in the loop runs without the wash-off, the "
-                           "human-instability module and the map stacks";
+        if (cfg->flag_waterquality == 1) {
+            g_last_error = "hp2d_sim_create: CA routing in the loop runs without the wash-off";
             return HP2D_E_UNSUPPORTED;
         }
     } else if (cfg->routing != HP2D_ROUTING_INERTIAL || cfg->flag_numerical_scheme != 1) {

@@ -327,6 +327,11 @@ __global__ __launch_bounds__(kCaThreads) void ca_gather_kernel(CaSimArgs<T> A) {
     const T R = a.R, tol = A.depth_tol_mm;
     const T t_end_min = T(sv.t + sv.dt_min);
     const bool track = a.dmax_map != nullptr && !sv.fail_step;
+    const bool risk_on = a.max_risk != nullptr && !sv.fail_step;
+    // a map record can only follow this step if t + max_dt passes the next record
+    // time (the new dt never exceeds max_dt): only then are the velocities stored
+    const bool may_snap = a.snap_vras != nullptr &&
+                          (a.st->t + a.st->max_dt / 60.0 > (double)a.st->map_last * a.rec_maps);
     T vkey = T(NAN), wkey = T(NAN), rmax = T(NAN);
     double s_store = 0.0;
     const int chunk = xcd_logical_tile();
@@ -371,6 +376,29 @@ __global__ __launch_bounds__(kCaThreads) void ca_gather_kernel(CaSimArgs<T> A) {
             vkey = fmax(vkey, vtot);
         }
         if (!a.adaptive) rmax = fmax(rmax, raster);
+        if (risk_on || may_snap) {
+            const T vt = f64 ? vtot : sqrt(vtot);  // velocities.total_velocity
+            if (risk_on) {
+                // Human_Instability_Module.m:8-25 and its running maximum
+                const T h = f64 ? dn / T(1000) : dn * T(1e-3);
+                const T F_person = a.hi_weight * a.hi_g;
+                const T F_buoy = (a.hi_w1 * a.hi_w2) * h * a.hi_ro * a.hi_g;
+                const T avail = a.hi_mu * (fmax(F_person - F_buoy, T(0)));
+                const T hydro = fmax(T(1) / T(2) * (a.hi_Cd * a.hi_ro * a.hi_w1 * h * (vt * vt)), T(0));
+                T risk = T(0);
+                if (avail > T(0)) risk = hydro / avail;
+                else if (h > T(0)) risk = T(1);
+                risk = fmin(fmax(risk, T(0)), T(1));
+                if (isnan(dn)) risk = T(NAN);
+                const T prev = (sv.k == 1) ? (isnan(dn) ? T(NAN) : T(0)) : a.max_risk[c];
+                const T nm = fmax(risk, prev);
+                if (sv.k == 1 || nm != prev) a.max_risk[c] = nm;
+            }
+            if (may_snap) {
+                a.snap_vras[c] = raster;
+                a.snap_vtot[c] = vt;
+            }
+        }
         // :227-238: the wave candidate over the non-stagnant cells (depth d / 1000;
         // a NaN depth counts as 0, as max(NaN, 0) does)
         const T dts = f64 ? dn / T(1000) : dn * T(1e-3);

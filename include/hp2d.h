@@ -497,8 +497,8 @@ typedef struct hp2d_run_config {
     int32_t routing;                   /* HP2D_ROUTING_INERTIAL, or HP2D_ROUTING_CA: CA_Routing
                                           (HydroPol2D_Main_While.m:461-465) with the inflow gather and
                                           mass balance of :631-660 and the CA time step of
-                                          refreshing_timestep.m:31-53; no wash-off, human instability
-                                          or map stacks, one launch shape, no ensembles */
+                                          refreshing_timestep.m:31-53; no wash-off, one launch shape,
+                                          no ensembles */
     int32_t flag_numerical_scheme;     /* 1 = Bates */
     int32_t flag_critical;
     int32_t outlet_type;               /* 1 normal slope, 2 critical depth */

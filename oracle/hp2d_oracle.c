@@ -593,13 +593,11 @@ int32_t oracle_run(const hp2d_run_config *cfg, const hp2d_run_inputs *in,
     const int32_t ny = cfg->ny, nx = cfg->nx;
     const int64_t n = (int64_t)ny * nx;
     const double R = cfg->resolution, A = R * R;
-    /* CA routing (flags.flag_CA = 1) runs without the wash-off, the human
-     * instability module and the map stacks, as the device loop does */
+    /* CA routing (flags.flag_CA = 1) runs without the wash-off, as the device
+     * loop does */
     const int do_ca = cfg->routing == HP2D_ROUTING_CA;
     if (!do_ca && cfg->routing != HP2D_ROUTING_INERTIAL) return HP2D_E_UNSUPPORTED;
-    if (do_ca && (cfg->flag_waterquality == 1 || cfg->flag_human_instability == 1 ||
-                  cfg->record_time_maps_min > 0))
-        return HP2D_E_UNSUPPORTED;
+    if (do_ca && cfg->flag_waterquality == 1) return HP2D_E_UNSUPPORTED;
     if (cfg->flag_groundwater == 1 && cfg->flag_infiltration != 1 && cfg->flag_infiltration != 2)
         return HP2D_E_UNSUPPORTED;
 

@@ -141,6 +141,32 @@ def test_ca_fp32_loop_within_bounds_of_fp64_oracle():
     assert rep["outflow_volume_m3"] == pytest.approx(o["report"]["outflow_volume_m3"], rel=2e-3)
 
 
+def test_ca_map_records_and_human_instability_match_oracle():
+    """CA runs with the map stacks (save_output_maps.m:24-27, 148-258) and the
+    Human_Instability_Module (HydroPol2D_Main_While.m:718) on the CA branch's
+    velocities.  (A 40 mm wet start makes this plane's CA run chaotic: the
+    oracle's own dt sequence changes at step 99 when the roughness moves by one
+    unit in the last place; the 5 mm start moves it by 1e-13 mm.)"""
+    from hydropol2d_amd.simulation import Simulation
+    spec = _spec("ca_ga", T=6.0, d0=5.0)
+    spec.human_instability = {"weight_person": 20.0}
+    spec.record_time_maps_min = 1.0
+    with Simulation(spec) as s:
+        s.run()
+        rec, mr, dts = s.map_records(), s.max_risk(), s.dt_history()
+    o = oracle.run(spec, dt_follow=dts, n_map_records=64)
+    assert o["n_dt_mismatch"] == 0
+    assert len(rec["t_min"]) == len(o["map_record_t"]) >= 5
+    np.testing.assert_allclose(rec["t_min"], o["map_record_t"], rtol=1e-12)
+    for name, arr in o["map_records"].items():
+        if np.all(np.isnan(arr)):
+            assert name not in rec, name
+            continue
+        _close(rec[name], arr, name)
+    _close(mr, o["max_risk"], "max_risk")
+    assert np.nanmax(mr) > 0 and np.nanmax(rec["velocity"]) > 0
+
+
 def test_ca_refuses_what_it_does_not_run():
     from hydropol2d_amd._lib import HP2DError
     from hydropol2d_amd.simulation import Ensemble, Simulation
