@@ -90,6 +90,9 @@ def _load(path):
         L.oracle_refresh_dt.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp,
                                         C.POINTER(DtParams), _dp, _dp]
         L.oracle_refresh_dt.restype = C.c_double
+        L.oracle_refresh_dt_ca.argtypes = [C.c_int32, C.c_int32, _dp, _dp,
+                                           C.POINTER(DtParams), _dp, _dp]
+        L.oracle_refresh_dt_ca.restype = C.c_double
         L.oracle_rain_avg.argtypes = [_dp, _dp, C.c_int32, C.c_double, C.c_double,
                                       C.POINTER(C.c_int32)]
         L.oracle_rain_avg.restype = C.c_double
@@ -271,6 +274,23 @@ def refresh_dt(qout_right, qout_up, Hf, resolution, alfa_min=0.7, alfa_max=0.7,
     raster = np.zeros((ny, nx), order="F")
     tcr = lib().oracle_refresh_dt(ny, nx, _p(qr), _p(qu), _p(Hf), C.byref(p), C.byref(vmax),
                                   _p(raster))
+    return tcr, vmax.value, raster
+
+
+def refresh_dt_ca(q4, d_t, resolution, alfa_min=0.7, alfa_max=0.7, max_time_step=10.0,
+                  min_time_step=0.01, time_step_increments=0.001, depth_tolerance_mm=1.0,
+                  flag_timestep=1, flag_adaptive_timestepping=1):
+    """refreshing_timestep with flags.flag_CA = 1; q4: ny x nx x 4 (left, right, up,
+    down) [mm/h], d_t [mm].  Returns (tcr [s], max_velocity, velocity_raster)."""
+    q = np.asfortranarray(np.asarray(q4, dtype=np.float64))
+    d = _f(d_t)
+    ny, nx = d.shape
+    p = DtParams(resolution, alfa_min, alfa_max, max_time_step, min_time_step,
+                 time_step_increments, depth_tolerance_mm, flag_timestep,
+                 flag_adaptive_timestepping, 0, np.inf)
+    vmax = C.c_double()
+    raster = np.zeros((ny, nx), order="F")
+    tcr = lib().oracle_refresh_dt_ca(ny, nx, _p(q), _p(d), C.byref(p), C.byref(vmax), _p(raster))
     return tcr, vmax.value, raster
 
 

@@ -98,3 +98,23 @@ def test_ca_refuses_wash_off():
     spec.water_quality = loop_cases.plane().water_quality
     with pytest.raises(RuntimeError):
         oracle.run(spec, max_steps=1)
+
+
+@pytest.mark.parametrize("adaptive", [1, 0])
+def test_ca_time_step_of_a_uniform_sheet(adaptive):
+    # every cell passes q = 360 mm/h to its right-hand neighbour at depth D = 20 mm
+    # on a 5 m grid: u = q / 3.6e6 * R / (D / 1000) = 0.025 m/s (the CA components
+    # are plain differences, no 1/2), c = sqrt(9.81 * 0.02) m/s; the new dt is the
+    # Courant candidate snapped down to 1 ms
+    ny, nx, R, q, D = 6, 7, 5.0, 360.0, 20.0
+    q4 = np.zeros((ny, nx, 4))
+    q4[:, :, 1] = q
+    tcr, vmax, raster = oracle.refresh_dt_ca(q4, np.full((ny, nx), D), R,
+                                             flag_adaptive_timestepping=adaptive, max_time_step=1e4)
+    u = q / 3.6e6 * R / (D / 1000.0)
+    c = np.sqrt(9.81 * D / 1000.0)
+    assert vmax == pytest.approx(u, rel=1e-14)
+    np.testing.assert_allclose(raster, u, rtol=1e-14)
+    dt_adv = R / u * 0.7
+    dt_wave = 0.7 * R / c if adaptive else 0.7 * R / (u + c)
+    assert tcr == pytest.approx(np.floor(min(dt_adv, dt_wave) / 0.001) * 0.001, abs=1e-9)
